@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the Stratton-Chu K/R path on one MI355X.
+
+Workload (default, --config sphere20k): perfectly conducting sphere of radius
+1, 100 x 200 theta-phi quads (20,000 cells, +-0.5% radial jitter, seed 1),
+ka = 10.  One step = the full K collocation matrix (40,000 x 40,000 complex
+FP64, 25.6 GB in HBM, K_RotRot_HS_Colloc per (point, cell, basis)) plus the
+scattered E and H at 100,000 points on a 200 x 500 theta-phi grid of radius 5
+(K_RotRot_Far + R_Rot_Colloc per (point, cell)).  Inputs are resident in HBM
+before timing; the currents come from one LU solve in the warm-up.
+
+Metric: cell-pair interactions per second (assembly pairs + field pairs) over
+the timed steps.  Also reported: the LU factorisation and solve times, the
+roofline of the dominant kernel (field tile kernel) and a CPU baseline of the
+reference compiled from its own sources (or the oracle when that library is
+absent), timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# nominal algorithmic FP64 flops per cell pair (n_start = 8, n_start_seg = 10,
+# flop weights: add/sub/mul 1, fma 2, sqrt 16, div 12, sincos 46)
+FLOPS_FIELD_PAIR = 12544      # fused E (K_RotRot_Far) + H (R_Rot_Colloc), far pair
+FLOPS_ASM_PAIR = 10256        # K_RotRot_HS_Colloc block, far quad pair
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector (dense) peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from bielcc_amd import mesh
+    if name == "sphere20k":
+        return dict(verts=mesh.sphere_quads(100, 200, radius=1.0, jitter=0.005, seed=1), k=10.0,
+                    pts=mesh.grid_points(200, 500, 5.0), desc="sphere r=1, 100x200 quads (20,000 cells), ka=10, "
+                                                                  "field at 200x500 points r=5")
+    if name == "sphere512":
+        return dict(verts=mesh.sphere_quads(16, 32, radius=1.0, jitter=0.005, seed=1), k=2.0,
+                    pts=mesh.sphere_points(1000, 3.0, seed=2), desc="sphere r=1, 16x32 quads (512 cells), ka=2, "
+                                                                    "field at 1,000 points r=3")
+    raise SystemExit(f"unknown config {name}")
+
+
+def cpu_baseline(verts, pts, k, budget_s=20.0):
+    """Time the reference (or the oracle) on a bounded sample of the same
+    workload: field pairs (K_RotRot_Far + R_Rot_Colloc) for a sample of
+    points over all cells, and assembly pairs (K_RotRot_HS_Colloc for both
+    basis vectors) for a sample of collocation points over all cells."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    kind = "reference"
+    try:
+        chk = O.Checker("ref")
+    except (FileNotFoundError, OSError):
+        chk, kind = O.Checker("oracle"), "port"
+    npp = O.num_param()
+    n = verts.shape[0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    rng = np.random.default_rng(0)
+    j = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    tabs = [chk.cellfill(v) for v in verts]
+    rkt = np.array([t["rkt"] for t in tabs])
+    tau = np.array([[t["tau1"], t["tau2"]] for t in tabs])
+    fsel = pts[rng.permutation(len(pts))]
+    asel = rkt[rng.permutation(n)]
+
+    def field_pts(m):
+        t0 = time.perf_counter()
+        chk.op_pairs(4, verts, j, fsel[:m], npp, k)
+        chk.op_pairs(6, verts, j, fsel[:m], npp, k)
+        return time.perf_counter() - t0
+
+    def asm_pts(m):
+        t0 = time.perf_counter()
+        for b in range(2):
+            chk.op_pairs(2, verts, tau[:, b, :], asel[:m], npp, k)
+        return time.perf_counter() - t0
+
+    # calibrate on a sample that keeps every thread busy, then scale to the budget
+    cal = 2 * threads
+    per_f = field_pts(cal) / cal
+    per_a = asm_pts(cal) / cal
+    m_f = int(max(cal, min(len(pts), 0.6 * budget_s / per_f)))
+    m_a = int(max(cal, min(n, 0.4 * budget_s / per_a)))
+    tf = field_pts(m_f)
+    ta = asm_pts(m_a)
+    pairs = (m_f + m_a) * n
+    return {"value": pairs / (tf + ta), "unit": "cell-pair interactions/s", "cores": threads, "kind": kind,
+            "sample": f"{m_f} field points x {n} cells (K_RotRot_Far + R_Rot_Colloc) and {m_a} collocation "
+                      f"points x {n} cells x 2 basis vectors (K_RotRot_HS_Colloc); OpenMP over points; "
+                      f"{tf + ta:.1f} s of CPU work",
+            "field_pairs_per_s": m_f * n / tf, "assembly_pairs_per_s": m_a * n / ta}
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A plain device copy of known size (torch), so that rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE can be scaled against it in the same run."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {nbytes} bytes copied")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="sphere20k")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-lu", action="store_true", help="skip the LU (currents from the right-hand side)")
+    ap.add_argument("--copy-calib", action="store_true", help="run a 1 GiB device copy first (counter scale)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the hot path is run on one MI355X; multi-GPU sharding is a design "
+              "note only (row blocks of the matrix, observation points of the field).", file=sys.stderr)
+        sys.exit(2)
+
+    import bielcc_amd as B
+    if args.copy_calib:
+        copy_calibration()
+    w = workload(args.config)
+    verts, k, pts = w["verts"], w["k"], w["pts"]
+    n, M = verts.shape[0], pts.shape[0]
+    n2 = 2 * n
+    log(f"[bench] {w['desc']}: N={n} cells, {n2} unknowns, M={M} points")
+    ctx = B.Context(0)
+    m = ctx.mesh(verts)
+    npp = B.num_param()
+    dZ = ctx.empty(n2 * n2 * 16)
+    dB = ctx.empty(n2 * 16)
+    dpiv = ctx.empty(n2 * 4)
+    dJ = ctx.empty(n * 48)
+    dP = ctx.to_device(pts)
+    dE = ctx.empty(M * 48)
+    dH = ctx.empty(M * 48)
+    kvec, e0 = (0.0, 0.0, k), (1.0, 0.0, 0.0)
+
+    lu_s = solve_s = None
+    # warm-up: assemble, LU, solve, currents, field
+    for wi in range(max(1, args.warmup)):
+        t0 = time.perf_counter()
+        m.assemble_into(dZ, n2, npp, k)
+        ctx.sync()
+        log(f"[bench] warmup {wi}: assembly {time.perf_counter() - t0:.3f} s")
+        if wi == 0:
+            m.rhs_planewave_into(dB, n2, kvec, e0)
+            if not args.no_lu:
+                t0 = time.perf_counter()
+                ctx.lu_factor(dZ, n2, n2, dpiv)
+                ctx.sync()
+                lu_s = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                ctx.lu_solve(dZ, n2, n2, dpiv, dB, 1, n2)
+                ctx.sync()
+                solve_s = time.perf_counter() - t0
+                log(f"[bench] LU factor {lu_s:.2f} s, solve {solve_s:.3f} s")
+            m.currents_into(dB, n2, 1, dJ)
+        t0 = time.perf_counter()
+        m.field_into(npp, k, dJ, dP, M, dE, dH)
+        ctx.sync()
+        log(f"[bench] warmup {wi}: field {time.perf_counter() - t0:.3f} s")
+
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    ctx.sync()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        m.assemble_into(dZ, n2, npp, k)
+        m.field_into(npp, k, dJ, dP, M, dE, dH)
+        ctx.sync()
+        log(f"[bench] step {s} done ({time.perf_counter() - t_start:.2f} s)")
+    ctx.sync()
+    elapsed = time.perf_counter() - t_start
+    fld_ms, fld_cnt = ctx.timing("field_far")
+    asm_ms, asm_cnt = ctx.timing("assemble_far")
+    near_ms, _ = ctx.timing("near")
+    ctx.timing_enable(False)
+
+    pairs_step = n * n + M * n
+    value = pairs_step * args.steps / elapsed
+    E = dE.to_numpy((M, 3), np.complex128)
+    assert np.all(np.isfinite(E)), "non-finite field"
+    # roofline of the dominant kernel (field tile kernel; every pair of this workload is far)
+    fld_avg_s = fld_ms / max(fld_cnt, 1) / 1e3
+    achieved = M * n * FLOPS_FIELD_PAIR / fld_avg_s / 1e12
+    asm_avg_s = asm_ms / max(asm_cnt, 1) / 1e3
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "kernel": "k_field_far (fused K_RotRot_Far + R_Rot_Colloc, FP64)",
+            "launch_ms": round(fld_avg_s * 1e3, 3),
+            "assembly_kernel_tflops": round(n * n * FLOPS_ASM_PAIR / asm_avg_s / 1e12, 3) if asm_cnt else None,
+            "assembly_launch_ms": round(asm_avg_s * 1e3, 3) if asm_cnt else None}
+    out = {"metric": "cell-pair interactions/s (K,R assembly+field)", "value": value,
+           "unit": "cell-pair interactions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered sphere mesh)",
+           "config": {"workload": w["desc"], "cells": n, "unknowns": n2, "points": M, "ka": k,
+                      "pairs_per_step": pairs_step, "assembly_pairs": n * n, "field_pairs": M * n},
+           "roofline": roof}
+    if lu_s is not None:
+        out["solve"] = {"lu_factor_s": round(lu_s, 3), "lu_solve_s": round(solve_s, 4),
+                        "lu_tflops": round(8.0 / 3.0 * n2 ** 3 / lu_s / 1e12, 2),
+                        "solves_per_s_after_factor": round(1.0 / solve_s, 2)}
+    out["timing_ms"] = {"field_far": fld_ms, "assemble_far": asm_ms, "near": near_ms}
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(verts, pts, k, args.cpu_budget)
+        log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} pairs/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

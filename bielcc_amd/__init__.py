@@ -1,0 +1,13 @@
+"""bielcc_amd - MI355X (gfx950) implementation of BIELcc's Stratton-Chu
+K / R / K0 path: cell-pair quadrature kernels in HIP behind a C ABI
+(include/bielcc_hip.h), with a Python host layer.
+
+The HIP library is loaded lazily by `Context()`; importing the package does
+not touch the GPU.
+"""
+from .api import (BielccError, Context, DeviceArray, Mesh, NumParam, KernelParam, IntegralParam, num_param,
+                  kernel_param, OPS, KERNELS, DEFAULT_NUM_PARAM)
+from . import mesh
+
+__all__ = ["BielccError", "Context", "DeviceArray", "Mesh", "NumParam", "KernelParam", "IntegralParam",
+           "num_param", "kernel_param", "OPS", "KERNELS", "DEFAULT_NUM_PARAM", "mesh"]

@@ -1,0 +1,293 @@
+// Collocation matrix, right-hand sides and currents.
+//
+// The reference documents K_RotRot_HS_Colloc (K.h:156-204) and R_Rot_Colloc
+// (R.h:65-111) as the operators for "assembling the matrix at collocation
+// points" but has no assembly loop.  This library defines it as
+//   Z[2i+a, 2j+b] = tau_i^a . Op(tau_j^b, rkt_i, cell_j)
+// (unknowns cell-major / basis-minor as SurfaceCurrents_byBasis reads them,
+// electrodynamics/currents.h:22-33), stored column-major, complex FP64.
+//
+// Far pairs: one tile kernel; 256 collocation points per workgroup (one per
+// lane) x a range of cells whose surface and edge node tables sit in LDS.
+// For K the per-pair work is S = sum w e^{ikr}/r (64 nodes) and, per edge e,
+// g_e = sum dl (ikr-1) e^{ikr} (x-y)/r^3 (10 nodes), shared by both basis
+// vectors b: K0_b = -sum_e (tau_j^b . nu_e) g_e.  Near pairs
+// (IntegralSurf_SimplePot_H_HS's analytic branch, integrals_analytic.h:151-153)
+// are skipped here and written by the exact per-pair kernel.
+#include "bielcc_internal.h"
+
+using namespace bielcc_dev;
+
+namespace bielcc_impl {
+
+constexpr int kAT = 256;
+constexpr int kASurf = 512;     // surface node slots in LDS
+constexpr int kASeg = 256;      // edge node slots in LDS
+constexpr int kACells = 8;
+constexpr int kARec = 24;       // center 3, diam, tau1 3, tau2 3, nu 12, pad 2
+
+template <bool CK, int WHICH>
+__global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_per_split) {
+    __shared__ double4 ssurf[kASurf];
+    __shared__ double4 sseg[kASeg];
+    __shared__ double scell[kACells][kARec];
+    const int tid = threadIdx.x;
+    const int li = blockIdx.x * kAT + tid;          // local point index
+    const bool active = li < a.ni;
+    const int i = a.i0 + li;
+    double x[3] = {0., 0., 0.}, ta[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    if (active) {
+        const double* t = a.ctab + (size_t)i * CT;
+        for (int d = 0; d < 3; d++) { x[d] = t[CT_RKT + d]; ta[0][d] = t[CT_TAU1 + d]; ta[1][d] = t[CT_TAU2 + d]; }
+    }
+    const int c_begin = blockIdx.y * cells_per_split;
+    const int c_end = min(a.N, c_begin + cells_per_split);
+    const int nn = a.nn, nv = a.nv, ns = a.ns, nseg = nv * ns;
+    int cb = kACells;
+    if (WHICH == 0) cb = min(cb, min(kASurf / nn, kASeg / nseg));
+    else cb = min(cb, kASurf / nn);
+    cb = max(cb, 1);
+    const double kr_ = a.k_re, ki_ = a.k_im;
+    const cplx k = mk(kr_, ki_);
+
+    for (int cs = c_begin; cs < c_end; cs += cb) {
+        const int ncs = min(cb, c_end - cs);
+        __syncthreads();
+        const double4* gs = reinterpret_cast<const double4*>(a.nodes) + (size_t)cs * nn;
+        for (int idx = tid; idx < ncs * nn; idx += kAT) ssurf[idx] = gs[idx];
+        if (WHICH == 0) {
+            const double4* gg = reinterpret_cast<const double4*>(a.segs) + (size_t)cs * nseg;
+            for (int idx = tid; idx < ncs * nseg; idx += kAT) sseg[idx] = gg[idx];
+        }
+        for (int idx = tid; idx < ncs * kARec; idx += kAT) {
+            int cc = idx / kARec, f = idx % kARec;
+            const double* t = a.ctab + (size_t)(cs + cc) * CT;
+            double v = 0.;
+            if (f < 3) v = t[CT_CENTER + f];
+            else if (f == 3) v = t[CT_DIAM];
+            else if (f < 7) v = t[CT_TAU1 + f - 4];
+            else if (f < 10) v = t[CT_TAU2 + f - 7];
+            else if (f < 22) v = t[CT_NU + f - 10];
+            scell[cc][f] = v;
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int cc = 0; cc < ncs; cc++) {
+            const int c = cs + cc;
+            const double* rec = scell[cc];
+            const double dcen = center_dist(x[0], x[1], x[2], rec);
+            cplx z[2][2];
+            if (WHICH == 0) {
+                if (near_rule(dcen, rec[3], 0, a.ac, 0.)) continue;   // near pair: exact kernel writes it
+                // S = int e^{ikr}/r  (F_SimplePot_H, smoothR = DBL_EPSILON: factor 1/r)
+                double Sr = 0., Si = 0.;
+                const double4* nd = ssurf + cc * nn;
+                for (int q = 0; q < nn; q++) {
+                    const double4 y = nd[q];
+                    const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
+                    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                    const double ri = rsqrt(r2);
+                    const double r = r2 * ri;
+                    double s, co;
+                    sincos(kr_ * r, &s, &co);
+                    double w = y.w * ri;
+                    if (CK) w *= exp(-ki_ * r);
+                    Sr += w * co;
+                    Si += w * s;
+                }
+                // K0 with F_GradSimplePot_H, smoothR = smoothRMin = DBL_EPSILON (K.h:189-200)
+                cplx k0b[2][3] = {{mk(0, 0), mk(0, 0), mk(0, 0)}, {mk(0, 0), mk(0, 0), mk(0, 0)}};
+                const double4* sg = sseg + cc * nseg;
+                for (int e = 0; e < nv; e++) {
+                    double gr0 = 0., gi0 = 0., gr1 = 0., gi1 = 0., gr2 = 0., gi2 = 0.;
+                    for (int q = 0; q < ns; q++) {
+                        const double4 y = sg[e * ns + q];
+                        const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
+                        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                        const double ri = rsqrt(r2);
+                        const double r = r2 * ri;
+                        double s, co;
+                        sincos(kr_ * r, &s, &co);
+                        double w = y.w * ri * ri * ri;
+                        if (CK) w *= exp(-ki_ * r);
+                        const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
+                        const double gr = (mr * co - mi * s) * w, gi = (mr * s + mi * co) * w;
+                        gr0 += gr * d0; gi0 += gi * d0;
+                        gr1 += gr * d1; gi1 += gi * d1;
+                        gr2 += gr * d2; gi2 += gi * d2;
+                    }
+                    const double* nu = rec + 10 + 3 * e;
+                    for (int b = 0; b < 2; b++) {
+                        const double* tb = rec + 4 + 3 * b;
+                        const double jn = -(tb[0] * nu[0] + tb[1] * nu[1] + tb[2] * nu[2]);
+                        k0b[b][0] += mk(gr0, gi0) * jn;
+                        k0b[b][1] += mk(gr1, gi1) * jn;
+                        k0b[b][2] += mk(gr2, gi2) * jn;
+                    }
+                }
+                const cplx Sk2 = (mk(Sr, Si) * k) * k;
+                for (int b = 0; b < 2; b++) {
+                    const double* tb = rec + 4 + 3 * b;
+                    cplx res[3];
+                    for (int g = 0; g < 3; g++) res[g] = (Sk2 * tb[g]) / (4. * kPi) + k0b[b][g] / (4. * kPi);
+                    for (int aa = 0; aa < 2; aa++)
+                        z[aa][b] = res[0] * ta[aa][0] + res[1] * ta[aa][1] + res[2] * ta[aa][2];
+                }
+            } else {
+                if (dcen < kMachineZero) {
+                    z[0][0] = z[0][1] = z[1][0] = z[1][1] = mk(0., 0.);
+                } else {
+                    double Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
+                    const double4* nd = ssurf + cc * nn;
+                    for (int q = 0; q < nn; q++) {
+                        const double4 y = nd[q];
+                        const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
+                        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                        const double ri = rsqrt(r2);
+                        const double r = r2 * ri;
+                        double s, co;
+                        sincos(kr_ * r, &s, &co);
+                        double w = y.w * ri * ri * ri;
+                        if (CK) w *= exp(-ki_ * r);
+                        const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
+                        const double gr = (mr * co - mi * s) * w, gi = (mr * s + mi * co) * w;
+                        Gr[0] += gr * d0; Gi[0] += gi * d0;
+                        Gr[1] += gr * d1; Gi[1] += gi * d1;
+                        Gr[2] += gr * d2; Gi[2] += gi * d2;
+                    }
+                    const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
+                    for (int b = 0; b < 2; b++) {
+                        const double* tb = rec + 4 + 3 * b;
+                        cplx res[3] = {(G[1] * tb[2] - G[2] * tb[1]) / (4. * kPi),
+                                       (G[2] * tb[0] - G[0] * tb[2]) / (4. * kPi),
+                                       (G[0] * tb[1] - G[1] * tb[0]) / (4. * kPi)};
+                        for (int aa = 0; aa < 2; aa++)
+                            z[aa][b] = res[0] * ta[aa][0] + res[1] * ta[aa][1] + res[2] * ta[aa][2];
+                    }
+                }
+            }
+            for (int b = 0; b < 2; b++) {
+                double2* col = reinterpret_cast<double2*>(a.Z) + (size_t)(2 * c + b) * a.ld + 2 * (size_t)li;
+                col[0] = make_double2(z[0][b].re, z[0][b].im);
+                col[1] = make_double2(z[1][b].re, z[1][b].im);
+            }
+        }
+    }
+}
+
+template <bool CK, int W>
+static void assemble_far_t(hipStream_t s, const AssembleArgs& a, int nsplit) {
+    int cps = (a.N + nsplit - 1) / nsplit;
+    dim3 g((a.ni + kAT - 1) / kAT, nsplit), b(kAT);
+    hipLaunchKernelGGL((k_assemble_far<CK, W>), g, b, 0, s, a, cps);
+}
+
+hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a) {
+    // enough workgroups to fill 256 CUs several times over
+    int bx = (a.ni + kAT - 1) / kAT;
+    int nsplit = (2048 + bx - 1) / bx;
+    nsplit = max(1, min(nsplit, (a.N + 63) / 64));
+    bool ck = a.k_im != 0.;
+    if (a.which == 0) { if (ck) assemble_far_t<true, 0>(s, a, nsplit); else assemble_far_t<false, 0>(s, a, nsplit); }
+    else { if (ck) assemble_far_t<true, 1>(s, a, nsplit); else assemble_far_t<false, 1>(s, a, nsplit); }
+    return hipGetLastError();
+}
+
+// Exact per-pair blocks: listed pairs (lq = point index relative to i0, lc =
+// cell) or, with lq == nullptr, every pair of points i0..i0+ni-1 x all cells.  Z_ab = tau_i^a . Op(tau_j^b, rkt_i, cell_j).
+template <int NV>
+__global__ __launch_bounds__(64) void k_assemble_generic(int which, long long npairs, const int* __restrict__ lq,
+                                                         const int* __restrict__ lc, int N, int i0,
+                                                         const double* __restrict__ verts,
+                                                         const double* __restrict__ ctab, NumP np, double k_re,
+                                                         double k_im, double* __restrict__ Z, long long ld) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    int i, c;
+    if (lq) { i = i0 + lq[p]; c = lc[p]; }
+    else { i = i0 + (int)(p / N); c = (int)(p % N); }
+    double cv[NV][3], x[3], ta[2][3];
+    for (int v = 0; v < NV; v++)
+        for (int d = 0; d < 3; d++) cv[v][d] = verts[((size_t)c * NV + v) * 3 + d];
+    const double* ti = ctab + (size_t)i * CT;
+    const double* tc = ctab + (size_t)c * CT;
+    for (int d = 0; d < 3; d++) { x[d] = ti[CT_RKT + d]; ta[0][d] = ti[CT_TAU1 + d]; ta[1][d] = ti[CT_TAU2 + d]; }
+    const cplx k = mk(k_re, k_im);
+    for (int b = 0; b < 2; b++) {
+        const double* tb = tc + CT_TAU1 + 3 * b;
+        cplx j[3] = {mk(tb[0], 0.), mk(tb[1], 0.), mk(tb[2], 0.)}, res[3];
+        op_eval<NV>(which == 0 ? OP_K_HS_COLLOC : OP_R_COLLOC, cv, x, j, np, k, res);
+        double2* col = reinterpret_cast<double2*>(Z) + (size_t)(2 * c + b) * ld + 2 * (size_t)(i - i0);
+        for (int aa = 0; aa < 2; aa++) {
+            cplx z = res[0] * ta[aa][0] + res[1] * ta[aa][1] + res[2] * ta[aa][2];
+            col[aa] = make_double2(z.re, z.im);
+        }
+    }
+}
+
+hipError_t launch_assemble_generic(hipStream_t s, int nv, int which, long long npairs, const int* lq,
+                                   const int* lc, int N, int i0, int ni, const double* verts, const double* ctab,
+                                   const NumP& np, double k_re, double k_im, double* Z, long long ld) {
+    (void)ni;
+    if (npairs <= 0) return hipSuccess;
+    dim3 g((unsigned)((npairs + 63) / 64)), b(64);
+    if (nv == 4) hipLaunchKernelGGL(k_assemble_generic<4>, g, b, 0, s, which, npairs, lq, lc, N, i0, verts, ctab, np, k_re, k_im, Z, ld);
+    else hipLaunchKernelGGL(k_assemble_generic<3>, g, b, 0, s, which, npairs, lq, lc, N, i0, verts, ctab, np, k_re, k_im, Z, ld);
+    return hipGetLastError();
+}
+
+// rhs[2i+a, r] = -tau_i^a . E_inc_r(rkt_i), E_inc_r = e0_r exp(i kvec_r . x)
+// (PlaneWaveFieldE incident_field.h:20-29; perfect conductor: the tangential
+// total field vanishes).  kvec_e0: [nrhs][6] device array (kvec, e0).
+__global__ void k_rhs_planewave(int N, const double* __restrict__ ctab, int nrhs,
+                                const double* __restrict__ kvec_e0, double* __restrict__ B, long long ldb) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)N * nrhs) return;
+    int c = (int)(idx % N), r = (int)(idx / N);
+    const double* t = ctab + (size_t)c * CT;
+    const double* ke = kvec_e0 + 6 * (size_t)r;
+    double ph = ke[0] * t[CT_RKT] + ke[1] * t[CT_RKT + 1] + ke[2] * t[CT_RKT + 2];
+    cplx deg = cexp(cplx{0., 1.} * ph);
+    cplx e[3] = {deg * ke[3], deg * ke[4], deg * ke[5]};
+    double2* out = reinterpret_cast<double2*>(B) + (size_t)r * ldb + 2 * (size_t)c;
+    for (int aa = 0; aa < 2; aa++) {
+        const double* ta = t + CT_TAU1 + 3 * aa;
+        cplx z = -(e[0] * ta[0] + e[1] * ta[1] + e[2] * ta[2]);
+        out[aa] = make_double2(z.re, z.im);
+    }
+}
+
+hipError_t launch_rhs_planewave(hipStream_t s, int N, const double* ctab, int nrhs, const double* kvec_e0,
+                                double* B, long long ldb) {
+    long long tot = (long long)N * nrhs;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rhs_planewave, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, ctab, nrhs,
+                       kvec_e0, B, ldb);
+    return hipGetLastError();
+}
+
+// SurfaceCurrents_byBasis (currents.h:22-33): j_c = b_{2c} tau_c^1 + b_{2c+1} tau_c^2
+__global__ void k_currents(int N, const double* __restrict__ ctab, int nrhs, const double* __restrict__ b,
+                           long long ldb, double* __restrict__ j) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)N * nrhs) return;
+    int c = (int)(idx % N), r = (int)(idx / N);
+    const double* t = ctab + (size_t)c * CT;
+    const double2* bb = reinterpret_cast<const double2*>(b) + (size_t)r * ldb + 2 * (size_t)c;
+    cplx b0 = mk(bb[0].x, bb[0].y), b1 = mk(bb[1].x, bb[1].y);
+    double* o = j + ((size_t)r * N + c) * 6;
+    for (int d = 0; d < 3; d++) {
+        cplx v = b0 * t[CT_TAU1 + d] + b1 * t[CT_TAU2 + d];
+        o[2 * d] = v.re; o[2 * d + 1] = v.im;
+    }
+}
+
+hipError_t launch_currents(hipStream_t s, int N, const double* ctab, int nrhs, const double* b, long long ldb,
+                           double* j) {
+    long long tot = (long long)N * nrhs;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_currents, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, ctab, nrhs, b, ldb, j);
+    return hipGetLastError();
+}
+
+}  // namespace bielcc_impl

@@ -1,0 +1,122 @@
+// Internal host-side declarations shared by the HIP translation units and the
+// C-ABI implementation.  Not installed; the public surface is include/bielcc_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+#include "bielcc_dev.h"
+
+namespace bielcc_impl {
+
+// Per-cell constant table, CT doubles per cell (row-major [n][CT]).
+enum : int {
+    CT_RKT = 0,      // collocation point (CellFill, mesh/cell.cpp:15-17 / :68-70)
+    CT_NORM = 3,     // CellFill normal (midline cross product for quads)
+    CT_TAU1 = 6,     // local basis tau^1
+    CT_TAU2 = 9,     // local basis tau^2
+    CT_AREA = 12,
+    CT_DIAM = 13,    // get_diam
+    CT_NFUNC = 14,   // norm_func (diagonal cross product) used by K0
+    CT_CENTER = 17,  // get_center_mass
+    CT_NU = 20,      // K0 edge normals nu_e, [4][3] (triangles: 3 used)
+    CT = 32
+};
+
+struct DevMesh {
+    int n = 0, nv = 0;
+    double grid_step = 0.;   // max cell diameter (SurfMesh::GetGridStep, mesh/surf_mesh.h:322-328)
+    double* verts = nullptr; // [n][nv][3]
+    double* ctab = nullptr;  // [n][CT]
+    double* nodes = nullptr; // [n][nn][4]: sub-cell centroid (3) + weight
+    double* segs = nullptr;  // [n][nv*ns][4]: edge node (3) + segment step length
+    int nodes_n = 0, segs_n = 0;
+};
+
+// ---- tables (tables.hip)
+hipError_t launch_cell_tables(hipStream_t s, int nv, int n, const double* verts, double* ctab);
+hipError_t launch_node_tables(hipStream_t s, int nv, int n, const double* verts, int n_start, double* nodes);
+hipError_t launch_seg_tables(hipStream_t s, int nv, int n, const double* verts, int n_seg, double* segs);
+
+// ---- near lists (tables.hip)
+// mode 0: near iff dist(q, center_c) < ac * diam_c
+// mode 1: near iff !(dist(q, center_c) > thresh)
+hipError_t launch_near_count(hipStream_t s, int nq, const double* q, int q_stride, int n, const double* ctab,
+                             int mode, double ac, double thresh, int* counts);
+hipError_t launch_scan(hipStream_t s, int n, const int* in, long long* out_excl, long long* total);
+hipError_t launch_near_fill(hipStream_t s, int nq, const double* q, int q_stride, int n, const double* ctab,
+                            int mode, double ac, double thresh, const long long* offs, int* list_q, int* list_c);
+
+// ---- field (field.hip)
+struct FieldArgs {
+    int nv, M, N, nn;
+    const double* pts;      // [M][3]
+    const double* ctab;
+    const double* nodes;
+    const double* jE;       // [N][3] complex
+    const double* jM;       // [N][3] complex or null
+    double k_re, k_im;
+    int mode;               // near rule (see launch_near_count)
+    double ac, thresh;
+    int nsplit;
+    double* part;           // [nsplit][NACC][M]
+};
+hipError_t launch_field_far(hipStream_t s, const FieldArgs& a);
+hipError_t launch_field_near(hipStream_t s, int nv, long long npairs, const int* lq, const int* lc,
+                             const double* pts, const double* verts, const double* jE, const double* jM,
+                             const bielcc_dev::NumP& np, double k_re, double k_im, int mode, double* out);
+hipError_t launch_field_finalize(hipStream_t s, int M, int nsplit, const double* part, int has_jm,
+                                 const long long* near_offs, const double* near_out, double mE_re, double mE_im,
+                                 double mH_re, double mH_im, int add_inc, const double* kvec_e0,
+                                 const double* pts, double* E, double* H);
+
+// ---- assembly (assemble.hip)
+struct AssembleArgs {
+    int nv, N, nn, ns;
+    int which;              // 0: K_RotRot_HS_Colloc, 1: R_Rot_Colloc
+    int i0, ni;             // collocation points i0 .. i0+ni-1 (rows 2*i0 .. 2*(i0+ni)-1)
+    const double* ctab;
+    const double* nodes;
+    const double* segs;
+    double k_re, k_im;
+    double ac;
+    double* Z;              // column-major, complex, leading dimension ld
+    long long ld;
+};
+hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a);
+hipError_t launch_assemble_generic(hipStream_t s, int nv, int which, long long npairs, const int* lq,
+                                   const int* lc, int N, int i0, int ni, const double* verts, const double* ctab,
+                                   const bielcc_dev::NumP& np, double k_re, double k_im, double* Z, long long ld);
+hipError_t launch_rhs_planewave(hipStream_t s, int N, const double* ctab, int nrhs, const double* kvec_e0,
+                                double* B, long long ldb);
+hipError_t launch_currents(hipStream_t s, int N, const double* ctab, int nrhs, const double* b, long long ldb,
+                           double* j);
+
+// ---- generic batches (generic.hip)
+hipError_t launch_op_batch(hipStream_t s, int op, int nv, int n, const double* cells, const double* x,
+                           const double* j, const bielcc_dev::NumP& np, double k_re, double k_im, double* res);
+hipError_t launch_kernel_batch(hipStream_t s, int f, int n, const double* x, const double* y,
+                               const bielcc_dev::KP& kp, double* res);
+hipError_t launch_integral_batch(hipStream_t s, int f, int nv, int n, const double* cells, const double* x,
+                                 const bielcc_dev::KP& kp, int idim, int n_start, int p_max, double eps,
+                                 double* res, double* r_out);
+hipError_t launch_k0_batch(hipStream_t s, int variant, int f, int nv, int n, const double* cells,
+                           const double* x, const double* j, const bielcc_dev::KP& kp, int n_start, int p_max,
+                           double eps, double analyticR, double* res, double* r_out);
+hipError_t launch_analytic_batch(hipStream_t s, int which, int nv, int n, const double* cells, const double* x,
+                                 int n_start, int p_max, double eps, double analyticR, double k_re, double k_im,
+                                 double* res);
+hipError_t launch_plane_wave(hipStream_t s, int n, const double* pts, const double* kvec_e0, double* out);
+
+// ---- LU (lu.hip)
+struct LuWork;
+LuWork* lu_work_create();
+void lu_work_destroy(LuWork*);
+// returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
+hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
+                     const char** msg);
+hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const int* piv, int nrhs,
+                    double* B, long long ldb, const char** msg);
+
+}  // namespace bielcc_impl

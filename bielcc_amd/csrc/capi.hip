@@ -1,0 +1,616 @@
+// Implementation of the C ABI declared in include/bielcc_hip.h.
+#include "bielcc_hip.h"
+#include "bielcc_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace bielcc_impl;
+using bielcc_dev::NumP;
+using bielcc_dev::KP;
+
+struct bielcc_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<int, DevMesh> meshes;
+    int next_id = 1;
+    LuWork* lu = nullptr;
+    void* scratch = nullptr;
+    size_t scratch_cap = 0;
+    void* lists = nullptr;
+    size_t lists_cap = 0;
+    bool timing = false;
+    std::map<std::string, std::pair<double, int>> times;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int fail(bielcc_ctx ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(bielcc_ctx ctx, hipError_t e, const char* where) {
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    return fail(ctx, e == hipErrorOutOfMemory ? BIELCC_ERR_OOM : BIELCC_ERR_HIP, m);
+}
+
+#define HIPCHK(expr)                                              \
+    do {                                                          \
+        hipError_t _e = (expr);                                   \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr);   \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// grow-only device buffer
+hipError_t ensure(void*& buf, size_t& cap, size_t need) {
+    if (cap >= need) return hipSuccess;
+    size_t n = std::max(need, cap * 3 / 2);
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&buf, n);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+NumP to_np(const bielcc_num_param* p) {
+    NumP n;
+    n.eps = p->eps; n.rs = p->rs; n.rs_seg = p->rs_seg; n.analytic_calc = p->analytic_calc;
+    n.n_start = p->n_start; n.n_start_seg = p->n_start_seg; n.p_max = p->p_max; n.p_max_seg = p->p_max_seg;
+    return n;
+}
+
+KP to_kp(const bielcc_kernel_param* p) {
+    KP k;
+    k.R = p->smoothR; k.Rmin = p->smoothRMin; k.k = bielcc_dev::mk(p->k_re, p->k_im);
+    for (int i = 0; i < 3; i++) {
+        k.vec[i] = p->vec_dbl[i];
+        k.n[i] = p->n[i];
+        k.cur[i] = bielcc_dev::mk(p->current[2 * i], p->current[2 * i + 1]);
+    }
+    return k;
+}
+
+int check_np(bielcc_ctx ctx, const bielcc_num_param* np) {
+    if (!np) return fail(ctx, BIELCC_ERR_ARG, "num_param is NULL");
+    if (np->n_start < 1 || np->n_start_seg < 1 || np->p_max < 1 || np->p_max_seg < 1)
+        return fail(ctx, BIELCC_ERR_ARG, "n_start, n_start_seg, p_max and p_max_seg must be >= 1");
+    if ((long long)np->n_start << (np->p_max - 1) > 4096 || (long long)np->n_start_seg << (np->p_max_seg - 1) > (1 << 20))
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED, "quadrature partition beyond 4096 (cell) / 2^20 (segment)");
+    return BIELCC_OK;
+}
+
+DevMesh* find_mesh(bielcc_ctx ctx, int id) {
+    auto it = ctx->meshes.find(id);
+    return it == ctx->meshes.end() ? nullptr : &it->second;
+}
+
+int ensure_tables(bielcc_ctx ctx, DevMesh& m, int n_start, int n_seg) {
+    if (m.nodes_n != n_start) {
+        if (m.nodes) (void)hipFree(m.nodes);
+        m.nodes = nullptr;
+        m.nodes_n = 0;
+        HIPCHK(hipMalloc(&m.nodes, (size_t)m.n * n_start * n_start * 4 * sizeof(double)));
+        HIPCHK(launch_node_tables(ctx->stream, m.nv, m.n, m.verts, n_start, m.nodes));
+        m.nodes_n = n_start;
+    }
+    if (n_seg > 0 && m.segs_n != n_seg) {
+        if (m.segs) (void)hipFree(m.segs);
+        m.segs = nullptr;
+        m.segs_n = 0;
+        HIPCHK(hipMalloc(&m.segs, (size_t)m.n * m.nv * n_seg * 4 * sizeof(double)));
+        HIPCHK(launch_seg_tables(ctx->stream, m.nv, m.n, m.verts, n_seg, m.segs));
+        m.segs_n = n_seg;
+    }
+    return BIELCC_OK;
+}
+
+struct Timer {
+    bielcc_ctx ctx;
+    const char* name;
+    explicit Timer(bielcc_ctx c, const char* n) : ctx(c), name(n) {
+        if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+    }
+    void stop() {
+        if (!ctx->timing) return;
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        (void)hipEventSynchronize(ctx->ev1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        auto& t = ctx->times[name];
+        t.first += ms;
+        t.second += 1;
+    }
+};
+
+// Near-pair list for nq query points: counts -> exclusive scan -> fill.
+// Layout in ctx->scratch region handed in: counts [nq], offs [nq+1], total [1].
+// The lists themselves (lq, lc) live in ctx->lists.
+int build_near_list(bielcc_ctx ctx, int nq, const double* q, int q_stride, const DevMesh& m, int mode, double ac,
+                    double thresh, int* counts, long long* offs, long long* total_d, long long* total_h,
+                    int** lq, int** lc) {
+    HIPCHK(launch_near_count(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, counts));
+    HIPCHK(launch_scan(ctx->stream, nq, counts, offs, total_d));
+    HIPCHK(hipMemcpyAsync(total_h, total_d, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const long long T = *total_h;
+    *lq = *lc = nullptr;
+    if (T > 0) {
+        HIPCHK(ensure(ctx->lists, ctx->lists_cap, align256((size_t)T * sizeof(int)) * 2));
+        *lq = reinterpret_cast<int*>(ctx->lists);
+        *lc = reinterpret_cast<int*>(reinterpret_cast<char*>(ctx->lists) + align256((size_t)T * sizeof(int)));
+        HIPCHK(launch_near_fill(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, offs, *lq, *lc));
+    }
+    return BIELCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bielcc_version(void) { return 100; }
+
+int bielcc_ctx_create(int device, bielcc_ctx* out) {
+    if (!out) return BIELCC_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BIELCC_ERR_HIP;
+    if (device < 0 || device >= ndev) return BIELCC_ERR_ARG;
+    bielcc_ctx ctx = new bielcc_ctx_s();
+    ctx->device = device;
+    DeviceGuard g(device);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return BIELCC_ERR_HIP;
+    }
+    ctx->lu = lu_work_create();
+    *out = ctx;
+    return BIELCC_OK;
+}
+
+int bielcc_ctx_destroy(bielcc_ctx ctx) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    {
+        DeviceGuard g(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto& kv : ctx->meshes) {
+            (void)hipFree(kv.second.verts); (void)hipFree(kv.second.ctab);
+            (void)hipFree(kv.second.nodes); (void)hipFree(kv.second.segs);
+        }
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        if (ctx->lists) (void)hipFree(ctx->lists);
+        lu_work_destroy(ctx->lu);
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return BIELCC_OK;
+}
+
+int bielcc_sync(bielcc_ctx ctx) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BIELCC_OK;
+}
+
+void* bielcc_stream(bielcc_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int bielcc_last_error(bielcc_ctx ctx, char* buf, int len) {
+    if (!ctx || !buf || len <= 0) return BIELCC_ERR_ARG;
+    std::snprintf(buf, (size_t)len, "%s", ctx->err.c_str());
+    return BIELCC_OK;
+}
+
+int bielcc_malloc(bielcc_ctx ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return BIELCC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    *dptr = nullptr;
+    HIPCHK(hipMalloc(dptr, bytes ? bytes : 1));
+    return BIELCC_OK;
+}
+
+int bielcc_free(bielcc_ctx ctx, void* dptr) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(dptr));
+    return BIELCC_OK;
+}
+
+int bielcc_memcpy(bielcc_ctx ctx, void* dst, const void* src, size_t bytes, int kind) {
+    if (!ctx || kind < 0 || kind > 2) return fail(ctx, BIELCC_ERR_ARG, "bad memcpy kind");
+    if (bytes == 0) return BIELCC_OK;
+    if (!dst || !src) return fail(ctx, BIELCC_ERR_ARG, "NULL memcpy pointer");
+    DeviceGuard g(ctx->device);
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BIELCC_OK;
+}
+
+int bielcc_mesh_create(bielcc_ctx ctx, int n_cells, int nv, const double* verts, int* mesh_id) {
+    if (!ctx || !verts || !mesh_id) return fail(ctx, BIELCC_ERR_ARG, "NULL argument");
+    if (n_cells <= 0 || (nv != 3 && nv != 4)) return fail(ctx, BIELCC_ERR_ARG, "n_cells must be > 0 and nv 3 or 4");
+    for (size_t i = 0; i < (size_t)n_cells * nv * 3; i++)
+        if (!std::isfinite(verts[i])) return fail(ctx, BIELCC_ERR_ARG, "non-finite vertex coordinate");
+    DeviceGuard g(ctx->device);
+    DevMesh m;
+    m.n = n_cells;
+    m.nv = nv;
+    size_t vb = (size_t)n_cells * nv * 3 * sizeof(double);
+    HIPCHK(hipMalloc(&m.verts, vb));
+    hipError_t e = hipMalloc(&m.ctab, (size_t)n_cells * CT * sizeof(double));
+    if (e != hipSuccess) { (void)hipFree(m.verts); return hip_fail(ctx, e, "hipMalloc(ctab)"); }
+    HIPCHK(hipMemcpyAsync(m.verts, verts, vb, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_cell_tables(ctx->stream, nv, n_cells, m.verts, m.ctab));
+    // grid step = max diameter (host reduction of the diameter column)
+    std::vector<double> tab((size_t)n_cells * CT);
+    HIPCHK(hipMemcpyAsync(tab.data(), m.ctab, tab.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double gs = 0.;
+    for (int c = 0; c < n_cells; c++) gs = std::max(gs, tab[(size_t)c * CT + CT_DIAM]);
+    m.grid_step = gs;
+    int id = ctx->next_id++;
+    ctx->meshes[id] = m;
+    *mesh_id = id;
+    return BIELCC_OK;
+}
+
+int bielcc_mesh_destroy(bielcc_ctx ctx, int mesh_id) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    DeviceGuard g(ctx->device);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(m->verts); (void)hipFree(m->ctab); (void)hipFree(m->nodes); (void)hipFree(m->segs);
+    ctx->meshes.erase(mesh_id);
+    return BIELCC_OK;
+}
+
+int bielcc_mesh_tables(bielcc_ctx ctx, int mesh_id, double* out) {
+    if (!ctx || !out) return fail(ctx, BIELCC_ERR_ARG, "NULL argument");
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    DeviceGuard g(ctx->device);
+    std::vector<double> tab((size_t)m->n * CT);
+    HIPCHK(hipMemcpyAsync(tab.data(), m->ctab, tab.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < m->n; c++) std::memcpy(out + 20 * (size_t)c, tab.data() + (size_t)c * CT, 20 * sizeof(double));
+    return BIELCC_OK;
+}
+
+int bielcc_mesh_grid_step(bielcc_ctx ctx, int mesh_id, double* grid_step) {
+    if (!ctx || !grid_step) return fail(ctx, BIELCC_ERR_ARG, "NULL argument");
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    *grid_step = m->grid_step;
+    return BIELCC_OK;
+}
+
+int bielcc_mesh_size(bielcc_ctx ctx, int mesh_id, int* n_cells, int* nv) {
+    if (!ctx || !n_cells || !nv) return fail(ctx, BIELCC_ERR_ARG, "NULL argument");
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    *n_cells = m->n;
+    *nv = m->nv;
+    return BIELCC_OK;
+}
+
+int bielcc_op_batch(bielcc_ctx ctx, int op, int nv, int n, const double* cells_d, const double* x_d,
+                    const double* j_d, const bielcc_num_param* np, double k_re, double k_im, double* res_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (op < 0 || op > 7 || (nv != 3 && nv != 4) || n < 0) return fail(ctx, BIELCC_ERR_ARG, "bad op, nv or n");
+    if (int rc = check_np(ctx, np)) return rc;
+    if (n > 0 && (!cells_d || !x_d || !j_d || !res_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    HIPCHK(launch_op_batch(ctx->stream, op, nv, n, cells_d, x_d, j_d, to_np(np), k_re, k_im, res_d));
+    return BIELCC_OK;
+}
+
+int bielcc_kernel_batch(bielcc_ctx ctx, int f, int n, const double* x_d, const double* y_d,
+                        const bielcc_kernel_param* kp, double* res_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (f < 0 || f > 11 || n < 0 || !kp) return fail(ctx, BIELCC_ERR_ARG, "bad kernel id, n or kp");
+    if (n > 0 && (!x_d || !y_d || !res_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    HIPCHK(launch_kernel_batch(ctx->stream, f, n, x_d, y_d, to_kp(kp), res_d));
+    return BIELCC_OK;
+}
+
+int bielcc_integral_batch(bielcc_ctx ctx, int f, int nv, int n, const double* cells_d, const double* x_d,
+                          const bielcc_kernel_param* kp, const bielcc_integral_param* ip, double* res_d,
+                          double* smoothR_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (f < 0 || f > 11 || n < 0 || !kp || !ip || (nv < 2 || nv > 4)) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if (ip->idim < 1 || ip->idim > 3 || ip->n_start < 1 || ip->p_max < 1)
+        return fail(ctx, BIELCC_ERR_ARG, "idim must be 1..3, n_start and p_max >= 1");
+    if (n > 0 && (!cells_d || !x_d || !res_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    HIPCHK(launch_integral_batch(ctx->stream, f, nv, n, cells_d, x_d, to_kp(kp), ip->idim, ip->n_start, ip->p_max,
+                                 ip->eps, res_d, smoothR_d));
+    return BIELCC_OK;
+}
+
+int bielcc_k0_batch(bielcc_ctx ctx, int variant, int f, int nv, int n, const double* cells_d, const double* x_d,
+                    const double* j_d, const bielcc_kernel_param* kp, const bielcc_integral_param* ip,
+                    double analyticR, double* res_d, double* smoothR_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (variant < 0 || variant > 2 || f < 0 || f > 11 || n < 0 || !kp || !ip || (nv != 3 && nv != 4))
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if (ip->idim != 3) return fail(ctx, BIELCC_ERR_UNSUPPORTED, "K0 is defined for idim = 3 only");
+    if (n > 0 && (!cells_d || !x_d || !res_d || (variant != 1 && !j_d))) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    HIPCHK(launch_k0_batch(ctx->stream, variant, f, nv, n, cells_d, x_d, variant == 1 ? x_d : j_d, to_kp(kp),
+                           ip->n_start, ip->p_max, ip->eps, analyticR, res_d, smoothR_d));
+    return BIELCC_OK;
+}
+
+int bielcc_analytic_batch(bielcc_ctx ctx, int which, int nv, int n, const double* cells_d, const double* x_d,
+                          const bielcc_integral_param* ip, double analyticR, double k_re, double k_im,
+                          double* res_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (which < 0 || which > 3 || n < 0 || ((which == 0 || which == 2) && nv != 3 && nv != 4))
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if ((which >= 2) && (!ip || ip->n_start < 1 || ip->p_max < 1)) return fail(ctx, BIELCC_ERR_ARG, "bad integral param");
+    if (n > 0 && (!cells_d || !x_d || !res_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    int ns = ip ? ip->n_start : 1, pm = ip ? ip->p_max : 1;
+    double eps = ip ? ip->eps : 0.;
+    HIPCHK(launch_analytic_batch(ctx->stream, which, nv, n, cells_d, x_d, ns, pm, eps, analyticR, k_re, k_im, res_d));
+    return BIELCC_OK;
+}
+
+int bielcc_plane_wave_e(bielcc_ctx ctx, int n, const double* pts_d, const double* kvec, const double* e0,
+                        double* out_d) {
+    if (!ctx || !kvec || !e0 || n < 0) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    double ke[6] = {kvec[0], kvec[1], kvec[2], e0[0], e0[1], e0[2]};
+    HIPCHK(launch_plane_wave(ctx->stream, n, pts_d, ke, out_d));
+    return BIELCC_OK;
+}
+
+int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re, double k_im,
+                    int row0, int nrows, double* Z_d, int64_t ld) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (int rc = check_np(ctx, np)) return rc;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    if (which != 0 && which != 1) return fail(ctx, BIELCC_ERR_ARG, "which must be 0 (K) or 1 (R)");
+    if (row0 < 0 || nrows < 0 || (row0 & 1) || (nrows & 1) || row0 + nrows > 2 * m->n)
+        return fail(ctx, BIELCC_ERR_ARG, "row range must be even and inside [0, 2N)");
+    if (nrows == 0) return BIELCC_OK;
+    if (!Z_d || ld < nrows) return fail(ctx, BIELCC_ERR_ARG, "Z is NULL or ld < nrows");
+    if (!std::isfinite(k_re) || !std::isfinite(k_im)) return fail(ctx, BIELCC_ERR_ARG, "non-finite wave number");
+    DeviceGuard g(ctx->device);
+    const NumP npp = to_np(np);
+    const int i0 = row0 / 2, ni = nrows / 2;
+    const int nn = np->n_start * np->n_start, nseg = m->nv * np->n_start_seg;
+    const bool fast = np->p_max == 1 && np->p_max_seg == 1 && nn <= 512 && nseg <= 256 && np->analytic_calc > 1.;
+    if (!fast) {
+        Timer t(ctx, "assemble_generic");
+        HIPCHK(launch_assemble_generic(ctx->stream, m->nv, which, (long long)ni * m->n, nullptr, nullptr, m->n, i0,
+                                       ni, m->verts, m->ctab, npp, k_re, k_im, Z_d, ld));
+        t.stop();
+        return BIELCC_OK;
+    }
+    if (int rc = ensure_tables(ctx, *m, np->n_start, which == 0 ? np->n_start_seg : 0)) return rc;
+    AssembleArgs a;
+    a.nv = m->nv; a.N = m->n; a.nn = nn; a.ns = np->n_start_seg; a.which = which;
+    a.i0 = i0; a.ni = ni; a.ctab = m->ctab; a.nodes = m->nodes; a.segs = m->segs;
+    a.k_re = k_re; a.k_im = k_im; a.ac = np->analytic_calc; a.Z = Z_d; a.ld = ld;
+    {
+        Timer t(ctx, "assemble_far");
+        HIPCHK(launch_assemble_far(ctx->stream, a));
+        t.stop();
+    }
+    if (which == 0) {
+        // near pairs (analytic branch of IntegralSurf_SimplePot_H_HS): exact per-pair kernel
+        size_t need = align256((size_t)ni * sizeof(int)) + align256((size_t)(ni + 1) * sizeof(long long)) + 256;
+        HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, need));
+        char* base = reinterpret_cast<char*>(ctx->scratch);
+        int* counts = reinterpret_cast<int*>(base);
+        long long* offs = reinterpret_cast<long long*>(base + align256((size_t)ni * sizeof(int)));
+        long long* tot_d = reinterpret_cast<long long*>(base + need - 256);
+        long long tot_h = 0;
+        int *lq = nullptr, *lc = nullptr;
+        const double* q = m->ctab + (size_t)i0 * CT + CT_RKT;
+        if (int rc = build_near_list(ctx, ni, q, CT, *m, 0, np->analytic_calc, 0., counts, offs, tot_d, &tot_h, &lq, &lc))
+            return rc;
+        if (tot_h > 0) {
+            // the list holds point indices local to i0; the kernel adds i0
+            Timer t(ctx, "near");
+            HIPCHK(launch_assemble_generic(ctx->stream, m->nv, 0, tot_h, lq, lc, m->n, i0, ni, m->verts, m->ctab,
+                                           npp, k_re, k_im, Z_d, ld));
+            t.stop();
+        }
+    }
+    return BIELCC_OK;
+}
+
+int bielcc_rhs_planewave(bielcc_ctx ctx, int mesh_id, int nrhs, const double* kvec, const double* e0, double* B_d,
+                         int64_t ldb) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    if (nrhs <= 0 || !kvec || !e0 || !B_d || ldb < 2 * m->n) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    std::vector<double> ke((size_t)nrhs * 6);
+    for (int r = 0; r < nrhs; r++)
+        for (int d = 0; d < 3; d++) { ke[6 * r + d] = kvec[3 * r + d]; ke[6 * r + 3 + d] = e0[3 * r + d]; }
+    HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, ke.size() * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ctx->scratch, ke.data(), ke.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_rhs_planewave(ctx->stream, m->n, m->ctab, nrhs, reinterpret_cast<double*>(ctx->scratch), B_d, ldb));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BIELCC_OK;
+}
+
+int bielcc_lu_factor(bielcc_ctx ctx, int n, double* A_d, int64_t lda, int* piv_d, int* info) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || !A_d || !piv_d || !info || lda < n) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    Timer t(ctx, "lu_factor");
+    hipError_t e = lu_factor(ctx->stream, ctx->lu, n, A_d, lda, piv_d, info, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    if (*info != 0) return fail(ctx, BIELCC_ERR_SINGULAR, "zero pivot at column " + std::to_string(*info));
+    return BIELCC_OK;
+}
+
+int bielcc_lu_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const int* piv_d, int nrhs,
+                    double* B_d, int64_t ldb) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || nrhs <= 0 || !A_d || !piv_d || !B_d || lda < n || ldb < n) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    Timer t(ctx, "lu_solve");
+    hipError_t e = lu_solve(ctx->stream, ctx->lu, n, A_d, lda, piv_d, nrhs, B_d, ldb, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    return BIELCC_OK;
+}
+
+int bielcc_currents_by_basis(bielcc_ctx ctx, int mesh_id, int nrhs, const double* b_d, int64_t ldb, double* j_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    if (nrhs <= 0 || !b_d || !j_d || ldb < 2 * m->n) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    HIPCHK(launch_currents(ctx->stream, m->n, m->ctab, nrhs, b_d, ldb, j_d));
+    return BIELCC_OK;
+}
+
+static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                      const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr, double mHi,
+                      const double* kvec, const double* e0, int near_mode, int M, const double* pts_d, double* E_d,
+                      double* H_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (int rc = check_np(ctx, np)) return rc;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    if (M < 0 || near_mode < 0 || near_mode > 1) return fail(ctx, BIELCC_ERR_ARG, "bad M or near_mode");
+    if (M == 0) return BIELCC_OK;
+    if (!jE_d || !pts_d || !E_d) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    if ((kvec == nullptr) != (e0 == nullptr)) return fail(ctx, BIELCC_ERR_ARG, "kvec and e0 go together");
+    if (!std::isfinite(k_re) || !std::isfinite(k_im)) return fail(ctx, BIELCC_ERR_ARG, "non-finite wave number");
+    const int nn = np->n_start * np->n_start;
+    if (np->p_max != 1 || nn > 512 || !(np->analytic_calc > 1.))
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED,
+                    "field driver needs p_max = 1, n_start <= 22 and analytic_calc > 1 (use bielcc_op_batch)");
+    DeviceGuard g(ctx->device);
+    if (int rc = ensure_tables(ctx, *m, np->n_start, 0)) return rc;
+    const int has_jm = jM_d != nullptr;
+    const int nacc = has_jm ? 24 : 12;
+    const double thresh = np->analytic_calc * m->grid_step;
+    // enough workgroups to fill the chip: split the cells when the points alone do not
+    const int bx = (M + 255) / 256;
+    int nsplit = std::max(1, std::min((2048 + bx - 1) / bx, (m->n + 63) / 64));
+    size_t part_b = align256((size_t)nsplit * nacc * M * sizeof(double));
+    size_t cnt_b = align256((size_t)M * sizeof(int));
+    size_t off_b = align256((size_t)(M + 1) * sizeof(long long));
+    size_t need = part_b + cnt_b + off_b + 256;
+    HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, need));
+    char* base = reinterpret_cast<char*>(ctx->scratch);
+    double* part = reinterpret_cast<double*>(base);
+    int* counts = reinterpret_cast<int*>(base + part_b);
+    long long* offs = reinterpret_cast<long long*>(base + part_b + cnt_b);
+    long long* tot_d = reinterpret_cast<long long*>(base + part_b + cnt_b + off_b);
+    FieldArgs a;
+    a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
+    a.jE = jE_d; a.jM = jM_d; a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
+    a.thresh = thresh; a.nsplit = nsplit; a.part = part;
+    {
+        Timer t(ctx, "field_far");
+        HIPCHK(launch_field_far(ctx->stream, a));
+        t.stop();
+    }
+    long long tot_h = 0;
+    int *lq = nullptr, *lc = nullptr;
+    if (int rc = build_near_list(ctx, M, pts_d, 3, *m, near_mode, np->analytic_calc, thresh, counts, offs, tot_d,
+                                 &tot_h, &lq, &lc))
+        return rc;
+    double* near_out = nullptr;
+    void* nbuf = nullptr;
+    if (tot_h > 0) {
+        HIPCHK(hipMalloc(&nbuf, (size_t)tot_h * nacc * sizeof(double)));
+        near_out = reinterpret_cast<double*>(nbuf);
+        Timer t(ctx, "near");
+        hipError_t e = launch_field_near(ctx->stream, m->nv, tot_h, lq, lc, pts_d, m->verts, jE_d, jM_d, to_np(np),
+                                         k_re, k_im, near_mode, near_out);
+        if (e != hipSuccess) { (void)hipFree(nbuf); return hip_fail(ctx, e, "launch_field_near"); }
+        t.stop();
+    }
+    double ke[6] = {0, 0, 0, 0, 0, 0};
+    if (kvec) for (int d = 0; d < 3; d++) { ke[d] = kvec[d]; ke[3 + d] = e0[d]; }
+    hipError_t e = launch_field_finalize(ctx->stream, M, nsplit, part, has_jm, offs, near_out, mEr, mEi, mHr, mHi,
+                                         kvec != nullptr, ke, pts_d, E_d, H_d);
+    if (nbuf) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(nbuf);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "launch_field_finalize");
+    return BIELCC_OK;
+}
+
+int bielcc_field(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                 const double* jE_d, const double* jM_d, double multE_re, double multE_im, double multH_re,
+                 double multH_im, const double* kvec, const double* e0, int near_mode, int M, const double* pts_d,
+                 double* E_d, double* H_d) {
+    return field_impl(ctx, mesh_id, np, k_re, k_im, jE_d, jM_d, multE_re, multE_im, multH_re, multH_im, kvec, e0,
+                      near_mode, M, pts_d, E_d, H_d);
+}
+
+int bielcc_get_field_e(bielcc_ctx ctx, int mesh_id, const double* jE_d, double mult_re, double mult_im,
+                       const double* kvec, const double* e0, const bielcc_num_param* np, double k_re, double k_im,
+                       int M, const double* pts_d, double* E_d) {
+    if (!kvec || !e0) return fail(ctx, BIELCC_ERR_ARG, "GetFieldE_Ideal needs the incident plane wave");
+    return field_impl(ctx, mesh_id, np, k_re, k_im, jE_d, nullptr, mult_re, mult_im, 1., 0., kvec, e0, 0, M, pts_d,
+                      E_d, nullptr);
+}
+
+int bielcc_timing_enable(bielcc_ctx ctx, int on) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    ctx->timing = on != 0;
+    return BIELCC_OK;
+}
+
+int bielcc_timing_reset(bielcc_ctx ctx) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    ctx->times.clear();
+    return BIELCC_OK;
+}
+
+int bielcc_timing(bielcc_ctx ctx, const char* name, double* total_ms, int* count) {
+    if (!ctx || !name || !total_ms || !count) return BIELCC_ERR_ARG;
+    auto it = ctx->times.find(name);
+    if (it == ctx->times.end()) { *total_ms = 0.; *count = 0; return BIELCC_OK; }
+    *total_ms = it->second.first;
+    *count = it->second.second;
+    return BIELCC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+/*
+ * bielcc_hip.h - C ABI of the MI355X (gfx950) implementation of BIELcc's
+ * Stratton-Chu K / R / K0 path.
+ *
+ * Conventions
+ *  - Every function returns an int status (BIELCC_OK = 0) and never throws;
+ *    bielcc_last_error() gives the message of the last failure on a context.
+ *  - Complex numbers are interleaved (re, im) doubles.  3-vectors are [3].
+ *    Cells are [n][nv][3] doubles, nv = 4 (CellQuad) or 3 (CellTriangle).
+ *  - Buffers named *_d are DEVICE pointers owned by the caller (hipMalloc,
+ *    bielcc_malloc or a framework allocator); everything else is host memory.
+ *    Meshes and scratch are owned by the context.
+ *  - Calls are ordered on the context's stream (bielcc_stream()); results are
+ *    complete after bielcc_sync().  Use one context per host thread.
+ *  - The wave number is complex (k_re, k_im): k_im = 0 is the reference's
+ *    KType = double, k_im != 0 its KType = std::complex<double> (lossy media).
+ *  - Outputs are overwritten, never accumulated (integral_universal.h:53-55).
+ */
+#ifndef BIELCC_HIP_H
+#define BIELCC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BIELCC_OK 0
+#define BIELCC_ERR_ARG 1          /* invalid argument */
+#define BIELCC_ERR_OOM 2          /* device allocation failed */
+#define BIELCC_ERR_HIP 3          /* HIP runtime or library error */
+#define BIELCC_ERR_UNSUPPORTED 4  /* valid input outside what is implemented */
+#define BIELCC_ERR_SINGULAR 5     /* zero pivot in the LU */
+#define BIELCC_ERR_NOTFOUND 6     /* unknown mesh id */
+
+typedef struct bielcc_ctx_s* bielcc_ctx;
+
+/* NumParam (electrodynamics/num_param.h:28-55): the fields the path reads. */
+typedef struct {
+    double eps, rs, rs_seg, analytic_calc;
+    int n_start, n_start_seg, p_max, p_max_seg;
+} bielcc_num_param;
+
+/* KernelParam (integrators/kernel_param.h:25-57). */
+typedef struct {
+    double smoothR, smoothRMin;
+    double k_re, k_im;
+    double vec_dbl[3];
+    double n[3];
+    double current[6];
+} bielcc_kernel_param;
+
+/* IntegralParam (integrators/integral_param.h:16-45). */
+typedef struct {
+    int idim, n_start, p_max;
+    double eps;
+} bielcc_integral_param;
+
+/* Operator ids (electrodynamics/K.h, R.h). */
+enum {
+    BIELCC_OP_K_NEAR_SMOOTH = 0, /* K_RotRot_Near_Smooth K.h:47-79 */
+    BIELCC_OP_K_HS = 1,          /* K_RotRot_HS          K.h:119-145 */
+    BIELCC_OP_K_HS_COLLOC = 2,   /* K_RotRot_HS_Colloc   K.h:179-204 */
+    BIELCC_OP_K_HS2_COLLOC = 3,  /* K_RotRot_HS2_Colloc  K.h:230-252 */
+    BIELCC_OP_K_FAR = 4,         /* K_RotRot_Far         K.h:288-301 */
+    BIELCC_OP_R_SMOOTH = 5,      /* R_Rot_Smooth         R.h:46-61 */
+    BIELCC_OP_R_COLLOC = 6,      /* R_Rot_Colloc         R.h:89-111 */
+    BIELCC_OP_R_NEAR = 7         /* R_Rot_Near           R.h:140-175 */
+};
+
+/* Integrand ids (electrodynamics/kernel_lib.h). */
+enum {
+    BIELCC_F_SIMPLEPOT_L = 0,       /* :62-69 */
+    BIELCC_F_GRADSIMPLEPOT_L = 1,   /* :87-98 */
+    BIELCC_F_DOUBLEPOT_L = 2,       /* :117-129 */
+    BIELCC_F_VECTORPOT_L = 3,       /* :149-177 */
+    BIELCC_F_SIMPLEPOT_H = 4,       /* :209-217 */
+    BIELCC_NUF_SIMPLEPOT_H = 5,     /* :235-245 */
+    BIELCC_F_SIMPLEPOT_HM1 = 6,     /* :262-278 */
+    BIELCC_F_GRADSIMPLEPOT_H = 7,   /* :295-313 */
+    BIELCC_F_GRADSIMPLEPOT_HM1 = 8, /* :332-356 */
+    BIELCC_F_DOUBLEPOT_H = 9,       /* :375-390 */
+    BIELCC_NF_DOUBLEPOT_H = 10,     /* :409-419 */
+    BIELCC_F_ROTROTSIMPLEPOT_H = 11 /* :453-492 */
+};
+
+/* ------------------------------------------------------------ context */
+int bielcc_version(void);
+/* New context on HIP device `device` with its own default-priority stream. */
+int bielcc_ctx_create(int device, bielcc_ctx* out);
+int bielcc_ctx_destroy(bielcc_ctx ctx);
+/* Blocks until all work queued on the context's stream has finished. */
+int bielcc_sync(bielcc_ctx ctx);
+/* The context's hipStream_t, as void*. */
+void* bielcc_stream(bielcc_ctx ctx);
+/* Copies the last error message (NUL-terminated, truncated to len). */
+int bielcc_last_error(bielcc_ctx ctx, char* buf, int len);
+
+/* ------------------------------------------------------------ memory */
+int bielcc_malloc(bielcc_ctx ctx, size_t bytes, void** dptr);
+int bielcc_free(bielcc_ctx ctx, void* dptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+int bielcc_memcpy(bielcc_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* ------------------------------------------------------------ mesh
+ * Replaces SurfMesh<CellQuad|CellTriangle> (mesh/surf_mesh.h:47-302) and the
+ * per-cell CellFill data (mesh/cell.cpp:8-100): cells are uploaded once and
+ * their tables (collocation point, normal, basis, area, diameter, K0 edge
+ * normals, quadrature nodes) are built on the device. */
+int bielcc_mesh_create(bielcc_ctx ctx, int n_cells, int verts_per_cell, const double* verts, int* mesh_id);
+int bielcc_mesh_destroy(bielcc_ctx ctx, int mesh_id);
+/* Host copy of the per-cell tables, [n][20]: rkt[3] norm[3] tau1[3] tau2[3]
+ * area diam norm_func[3] center[3]  (GetRkt/GetNorm/GetTau1/GetTau2/GetArea,
+ * get_diam, norm_func, get_center_mass). */
+int bielcc_mesh_tables(bielcc_ctx ctx, int mesh_id, double* out);
+/* SurfMesh::GetGridStep (mesh/surf_mesh.h:322-328): max cell diameter. */
+int bielcc_mesh_grid_step(bielcc_ctx ctx, int mesh_id, double* grid_step);
+int bielcc_mesh_size(bielcc_ctx ctx, int mesh_id, int* n_cells, int* verts_per_cell);
+
+/* ------------------------------------------------------------ single functions, batched
+ * Each item i is one call of the reference function on its own inputs. */
+
+/* Operators K_RotRot_* / R_Rot_* (K.h:47-301, R.h:46-175):
+ * res_d[i] = Op(j_d[i], x_d[i], cells_d[i], num_param, k).  j, res: [n][3] complex. */
+int bielcc_op_batch(bielcc_ctx ctx, int op, int nv, int n, const double* cells_d, const double* x_d,
+                    const double* j_d, const bielcc_num_param* np, double k_re, double k_im, double* res_d);
+
+/* Integrands F_* (kernel_lib.h:62-492) at (x_d[i], y_d[i]); res [n][3] complex. */
+int bielcc_kernel_batch(bielcc_ctx ctx, int f, int n, const double* x_d, const double* y_d,
+                        const bielcc_kernel_param* kp, double* res_d);
+
+/* IntegralUniversalPnt (integral_universal.h:38-220; nv = 4 or 3, cells_d
+ * [n][nv][3]) or IntegralUniversalSegPnt (integral_universal_seg.h:38-98;
+ * nv = 2, cells_d = segments [n][2][3]).  kp is the starting KernelParam;
+ * smoothR_d[i] (optional) receives its mutated smoothR. */
+int bielcc_integral_batch(bielcc_ctx ctx, int f, int nv, int n, const double* cells_d, const double* x_d,
+                          const bielcc_kernel_param* kp, const bielcc_integral_param* ip, double* res_d,
+                          double* smoothR_d);
+
+/* variant 0: K0 (K0.h:36-86), 1: K0_Common (:112-166), 2: K0_GradSimplePot_H_HS
+ * (:197-249, uses analyticR and kp->k only).  ip->idim must be 3. */
+int bielcc_k0_batch(bielcc_ctx ctx, int variant, int f, int nv, int n, const double* cells_d, const double* x_d,
+                    const double* j_d, const bielcc_kernel_param* kp, const bielcc_integral_param* ip,
+                    double analyticR, double* res_d, double* smoothR_d);
+
+/* which 0: integral1Divr (integrals_analytic.cpp:218-235, cells), 1:
+ * integralGrad1DivrSeg (:240-257, segments), 2: IntegralSurf_SimplePot_H_HS
+ * (integrals_analytic.h:136-164, cells), 3: IntegralSeg_GradSimplePot_H_HS
+ * (:193-221, segments).  res [n][3] complex. */
+int bielcc_analytic_batch(bielcc_ctx ctx, int which, int nv, int n, const double* cells_d, const double* x_d,
+                          const bielcc_integral_param* ip, double analyticR, double k_re, double k_im,
+                          double* res_d);
+
+/* PlaneWaveFieldE (incident_field.h:20-29): out [n][3] complex. */
+int bielcc_plane_wave_e(bielcc_ctx ctx, int n, const double* pts_d, const double* kvec, const double* e0,
+                        double* out_d);
+
+/* ------------------------------------------------------------ drivers */
+
+/* Collocation matrix (no reference loop; operators named at K.h:156-178, R.h:65-88):
+ *   Z[2i+a, 2j+b] = tau_i^a . Op(tau_j^b, rkt_i, cell_j)
+ * which 0: Op = K_RotRot_HS_Colloc, 1: Op = R_Rot_Colloc.  Rows row0 ..
+ * row0+nrows-1 (both even) of the 2N x 2N matrix go to Z_d rows 0..nrows-1,
+ * column-major complex with leading dimension ld (in complex elements). */
+int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re, double k_im,
+                    int row0, int nrows, double* Z_d, int64_t ld);
+
+/* rhs[2i+a, r] = -tau_i^a . E_inc_r(rkt_i), E_inc_r = e0_r exp(i kvec_r . x)
+ * (PlaneWaveFieldE).  kvec, e0: host [nrhs][3].  B_d column-major, ldb >= 2N. */
+int bielcc_rhs_planewave(bielcc_ctx ctx, int mesh_id, int nrhs, const double* kvec, const double* e0, double* B_d,
+                         int64_t ldb);
+
+/* LU with partial pivoting of the n x n complex column-major A_d (in place);
+ * piv_d[n] receives 0-based row interchanges.  Returns BIELCC_ERR_SINGULAR
+ * (and *info = first zero pivot, 1-based) for an exactly singular matrix. */
+int bielcc_lu_factor(bielcc_ctx ctx, int n, double* A_d, int64_t lda, int* piv_d, int* info);
+/* Solves A X = B with the factors of bielcc_lu_factor; B_d (n x nrhs, ldb) is overwritten by X. */
+int bielcc_lu_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const int* piv_d, int nrhs,
+                    double* B_d, int64_t ldb);
+
+/* SurfaceCurrents_byBasis (currents.h:22-33) for nrhs coefficient columns:
+ * j_d[r][c][3] complex = b[2c, r] tau_c^1 + b[2c+1, r] tau_c^2. */
+int bielcc_currents_by_basis(bielcc_ctx ctx, int mesh_id, int nrhs, const double* b_d, int64_t ldb, double* j_d);
+
+/* Stratton-Chu field at M points (pts_d [M][3]):
+ *   E = multE sum_j K[jE_j](x) - sum_j R[jM_j](x) (+ E_inc if kvec != NULL)
+ *   H = multH sum_j K[jM_j](x) + sum_j R[jE_j](x)
+ * (examples/field/main0_field_test.cpp:237-319).  near_mode 0: near iff
+ * dist(x, centroid_j) < analytic_calc * diam_j (get_field.h:58), K near =
+ * K_RotRot_Near_Smooth; near_mode 1: far iff dist > analytic_calc * grid_step
+ * (main0_field_test.cpp:242), K near = K_RotRot_HS.  Far: K_RotRot_Far and
+ * R_Rot_Colloc; near R: R_Rot_Smooth.  jE_d, jM_d: [N][3] complex (jM_d may
+ * be NULL); E_d, H_d: [M][3] complex (H_d may be NULL). */
+int bielcc_field(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                 const double* jE_d, const double* jM_d, double multE_re, double multE_im, double multH_re,
+                 double multH_im, const double* kvec, const double* e0, int near_mode, int M, const double* pts_d,
+                 double* E_d, double* H_d);
+
+/* GetFieldE_Ideal (get_field.h:35-78) with EInc_Func = PlaneWaveFieldE:
+ * E = mult sum_j (near ? K_RotRot_Near_Smooth : K_RotRot_Far)[jE_j](x) + E_inc(x). */
+int bielcc_get_field_e(bielcc_ctx ctx, int mesh_id, const double* jE_d, double mult_re, double mult_im,
+                       const double* kvec, const double* e0, const bielcc_num_param* np, double k_re, double k_im,
+                       int M, const double* pts_d, double* E_d);
+
+/* ------------------------------------------------------------ timing
+ * With timing on, the drivers bracket their dominant kernels with HIP events
+ * on the context's stream; bielcc_timing reads the accumulated milliseconds
+ * and launch count of one of: "field_far", "assemble_far", "lu_factor",
+ * "lu_solve", "near". */
+int bielcc_timing_enable(bielcc_ctx ctx, int on);
+int bielcc_timing_reset(bielcc_ctx ctx);
+int bielcc_timing(bielcc_ctx ctx, const char* name, double* total_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BIELCC_HIP_H */

@@ -79,7 +79,7 @@ def _cvec(z, n):
 
 
 def _to_c(a):
-    return a[0::2] + 1j * a[1::2]
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.complex128)
 
 
 class Checker:
@@ -284,7 +284,7 @@ class Checker:
         assert self.which == "oracle"
         n = Z.shape[0]
         A = np.asfortranarray(Z.astype(np.complex128))
-        B = np.asfortranarray(np.asarray(rhs, dtype=np.complex128).reshape(n, -1))
+        B = np.array(np.asarray(rhs, dtype=np.complex128).reshape(n, -1), order="F", copy=True)
         rc = self._f("lu_solve")(C.c_int(n), A.ctypes.data_as(_DP), C.c_longlong(n),
                                  B.ctypes.data_as(_DP), C.c_int(B.shape[1]), C.c_longlong(n))
         assert rc == 0, rc

@@ -1,0 +1,272 @@
+"""Record golden input/output vectors of the reference (BIELcc compiled from
+its own sources into oracle/_ref/ by `make -C oracle ref`).
+
+Run in the build container only (the reference is not on the GPU host):
+    python tests/golden/make_golden.py
+Writes tests/golden/*.npz.  Seeds are fixed; every file stores its inputs
+next to the reference's outputs.  The reference's default numerical
+parameters are Calculation_Constants (electrodynamics/constants.h:17-27).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import scipy.linalg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from bielcc_amd import mesh  # noqa: E402
+
+REF_DATA = "/root/reference/examples/data/surface_mesh"
+
+# cells of examples/ED_Operators/main0_K.cpp:101-112 and main0_integrals.cpp
+RUT1 = np.array([[0., 0., 0.], [2., 0., 0.], [3., 2., 0.], [0., 2., 0.]])
+RUT2 = np.array([[0., 2., 0.], [0., 0., 0.], [2., 0., 0.]])
+RUT3 = np.array([[3., 2., 0.], [2., 0., 0.], [0., 2., 0.]])
+
+# NumParam sets: defaults; main0_K.cpp:152-169 (x10 splits, p_max 6); main0_R.cpp:110-127
+NP_SETS = [
+    dict(eps=1e-6, rs=2.0, rs_seg=2.0, analytic_calc=3.0, n_start=8, n_start_seg=10, p_max=1, p_max_seg=1),
+    dict(eps=1e-6, rs=2.0, rs_seg=2.0, analytic_calc=3.0, n_start=8, n_start_seg=10, p_max=3, p_max_seg=3),
+    dict(eps=1e-5, rs=0.5, rs_seg=0.5, analytic_calc=3.0, n_start=20, n_start_seg=30, p_max=2, p_max_seg=2),
+    dict(eps=1e-6, rs=3.0, rs_seg=3.0, analytic_calc=3.0, n_start=160, n_start_seg=400, p_max=1, p_max_seg=1),
+    dict(eps=1e-6, rs=2.0, rs_seg=2.0, analytic_calc=3.0, n_start=5, n_start_seg=7, p_max=1, p_max_seg=1),
+]
+
+
+def case_cells():
+    q = mesh.sphere_quads(8, 16, jitter=0.005, seed=7)
+    t = mesh.triangulate(q)
+    quads = [RUT1, q[0], q[37], q[-3]]            # q[0] is a north-pole quad (v0 == v1)
+    tris = [RUT2, RUT3, t[3], t[50]]
+    return quads, tris
+
+
+def case_points(cell, rng):
+    c = cell.mean(axis=0)
+    d = np.max(np.linalg.norm(cell - c, axis=1))
+    if len(cell) == 4:
+        n = np.cross(cell[2] - cell[0], cell[3] - cell[1])
+    else:
+        n = np.cross(cell[1] - cell[0], cell[2] - cell[0])
+    n /= np.linalg.norm(n)
+    on = 0.3 * cell[0] + 0.3 * cell[1] + 0.4 * cell[2]                  # on the cell
+    pts = [c, on, c + 0.05 * d * n, c + 0.5 * d * n + 0.2 * (cell[1] - cell[0]),
+           c + 2.0 * d * rng.normal(size=3), c + 10.0 * d * n, np.array([2., 1.5, 0.]),
+           np.array([10., 10., 10.])]
+    return [np.asarray(p, dtype=np.float64) for p in pts]
+
+
+def gen_ops(ref):
+    rng = np.random.default_rng(11)
+    quads, tris = case_cells()
+    rec = dict(cells4=[], cells3=[], x4=[], x3=[], j4=[], j3=[], np4=[], np3=[], k4=[], k3=[], op4=[], op3=[],
+               res4=[], res3=[])
+    for nv, cells in ((4, quads), (3, tris)):
+        for cell in cells:
+            for x in case_points(cell, rng):
+                for ki, k in enumerate([1.0, 2.0, 2.0 + 0.3j]):
+                    j = (rng.normal(size=3) + 1j * rng.normal(size=3)) if ki else np.array([1., 1., 1.])
+                    for npi, npd in enumerate(NP_SETS):
+                        if npi == 3 and (ki != 0 or not np.allclose(x, [2., 1.5, 0.])):
+                            continue      # the expensive set only on the examples' own case
+                        npp = O.num_param(**npd)
+                        for op in range(8):
+                            r = ref.op(op, cell, x, j, npp, k)
+                            rec[f"cells{nv}"].append(cell); rec[f"x{nv}"].append(x); rec[f"j{nv}"].append(j)
+                            rec[f"np{nv}"].append(npi); rec[f"k{nv}"].append(k); rec[f"op{nv}"].append(op)
+                            rec[f"res{nv}"].append(r)
+    out = {k: np.asarray(v) for k, v in rec.items()}
+    out["np_sets"] = np.array([[d[k] for k in ("eps", "rs", "rs_seg", "analytic_calc", "n_start", "n_start_seg",
+                                                "p_max", "p_max_seg")] for d in NP_SETS])
+    np.savez_compressed(os.path.join(HERE, "ops.npz"), **out)
+    print("ops:", len(out["op4"]) + len(out["op3"]), "cases")
+
+
+def gen_integrators(ref):
+    rng = np.random.default_rng(12)
+    quads, tris = case_cells()
+    KP = [(1e-16, 1e-16, 1, 8), (0.5, 1e-6, 1, 8), (0.5, 1e-3, 3, 6), (2.0, 0.1, 4, 5)]  # R, Rmin, p_max, n
+    rec = {k: [] for k in ["kind", "f", "nv", "cell", "x", "j", "kp", "ip", "k", "res", "R_out"]}
+    for nv, cells in ((4, quads), (3, tris)):
+        for cell in cells[:3]:
+            for x in case_points(cell, rng)[:6]:
+                for k in (1.5, 2.0 - 0.4j):
+                    j = rng.normal(size=3) + 1j * rng.normal(size=3)
+                    for (R, Rmin, pm, n0) in KP:
+                        for f in range(12):
+                            def mk():
+                                return O.kernel_param(R, Rmin, k, vec_dbl=(0.3, -0.2, 0.5), n=(0.0, 0.6, 0.8),
+                                                      current=(1 + 1j, 0.5, -0.2j))
+                            for idim in (1, 3):
+                                ip = O.IntegralParam(idim, n0, pm, 1e-6)
+                                kp = mk(); r = ref.integral_pnt(f, cell, x, kp, ip)
+                                rec["kind"].append(0); rec["f"].append(f); rec["nv"].append(nv)
+                                rec["cell"].append(np.pad(cell, ((0, 4 - nv), (0, 0)))); rec["x"].append(x)
+                                rec["j"].append(j); rec["kp"].append([R, Rmin]); rec["ip"].append([idim, n0, pm, 1e-6])
+                                rec["k"].append(k); rec["res"].append(r); rec["R_out"].append(kp.smoothR)
+                            ip = O.IntegralParam(3, n0 + 2, pm, 1e-6)
+                            kp = mk(); r = ref.integral_seg(f, cell[0], cell[2], x, kp, ip)
+                            seg = np.zeros((4, 3)); seg[0], seg[1] = cell[0], cell[2]
+                            rec["kind"].append(1); rec["f"].append(f); rec["nv"].append(2); rec["cell"].append(seg)
+                            rec["x"].append(x); rec["j"].append(j); rec["kp"].append([R, Rmin])
+                            rec["ip"].append([3, n0 + 2, pm, 1e-6]); rec["k"].append(k); rec["res"].append(r)
+                            rec["R_out"].append(kp.smoothR)
+                            if f < 4:
+                                continue   # K0 with a real integrand needs a real current
+                            for var in (0, 1):
+                                kp = mk(); r = ref.k0(var, f, cell, x, j, kp, ip)
+                                rec["kind"].append(2 + var); rec["f"].append(f); rec["nv"].append(nv)
+                                rec["cell"].append(np.pad(cell, ((0, 4 - nv), (0, 0)))); rec["x"].append(x)
+                                rec["j"].append(j); rec["kp"].append([R, Rmin]); rec["ip"].append([3, n0 + 2, pm, 1e-6])
+                                rec["k"].append(k); rec["res"].append(r); rec["R_out"].append(kp.smoothR)
+    out = {k: np.asarray(v) for k, v in rec.items()}
+    np.savez_compressed(os.path.join(HERE, "integrators.npz"), **out)
+    print("integrators:", len(out["f"]), "cases")
+
+
+def gen_kernels_analytic(ref):
+    rng = np.random.default_rng(13)
+    rec = {k: [] for k in ["f", "x", "y", "kp", "k", "res"]}
+    pts = [rng.normal(size=3) for _ in range(12)] + [np.array([0.3, 0.1, 0.2])]
+    for f in range(12):
+        for a in range(len(pts)):
+            x = pts[a]
+            for dy in (1e-9, 1e-5, 0.01, 0.3, 2.0):
+                y = x + dy * rng.normal(size=3)
+                for (R, Rmin) in ((1e-16, 1e-16), (0.2, 1e-6), (5.0, 0.1)):
+                    for k in (1.0, 3.0 + 0.5j, 1e-5):
+                        kp = O.kernel_param(R, Rmin, k, vec_dbl=(0.3, -0.2, 0.5), n=(0.0, 0.6, 0.8),
+                                            current=(1 + 1j, 0.5, -0.2j))
+                        rec["f"].append(f); rec["x"].append(x); rec["y"].append(y); rec["kp"].append([R, Rmin])
+                        rec["k"].append(k); rec["res"].append(ref.kernel(f, x, y, kp))
+    out = {k: np.asarray(v) for k, v in rec.items()}
+    # analytic integrals
+    quads, tris = case_cells()
+    an = {k: [] for k in ["which", "nv", "cell", "x", "k", "res"]}
+    for nv, cells in ((4, quads), (3, tris)):
+        for cell in cells:
+            for x in case_points(cell, rng):
+                for k in (1.0, 2.5 + 0.2j):
+                    ip = O.IntegralParam(1, 8, 1, 1e-6)
+                    pad = np.pad(cell, ((0, 4 - nv), (0, 0)))
+                    an["which"].append(0); an["nv"].append(nv); an["cell"].append(pad); an["x"].append(x)
+                    an["k"].append(k); an["res"].append([ref.integral1divr(cell, x), 0, 0])
+                    an["which"].append(2); an["nv"].append(nv); an["cell"].append(pad); an["x"].append(x)
+                    an["k"].append(k); an["res"].append([ref.surf_hs(cell, x, 3.0, ip, k), 0, 0])
+                    seg = np.zeros((4, 3)); seg[0], seg[1] = cell[0], cell[1]
+                    g = ref.integral_grad1divr_seg(cell[0], cell[1], x)
+                    an["which"].append(1); an["nv"].append(2); an["cell"].append(seg); an["x"].append(x)
+                    an["k"].append(k); an["res"].append(g.astype(complex))
+                    ip3 = O.IntegralParam(3, 10, 2, 1e-6)
+                    an["which"].append(3); an["nv"].append(2); an["cell"].append(seg); an["x"].append(x)
+                    an["k"].append(k); an["res"].append(ref.seg_hs(cell[0], cell[1], x, 3.0, ip3, k))
+                    an["which"].append(4); an["nv"].append(nv); an["cell"].append(pad); an["x"].append(x)
+                    an["k"].append(k)
+                    an["res"].append(ref.k0_hs(cell, x, np.array([1 + 0.5j, -0.3, 0.2j]), ip3, 3.0, k))
+    for key, v in an.items():
+        out["an_" + key] = np.asarray(v)
+    # cell tables
+    ct = {k: [] for k in ["nv", "cell", "tab"]}
+    for nv, cells in ((4, quads), (3, tris)):
+        for cell in cells:
+            d = ref.cellfill(cell)
+            ct["nv"].append(nv); ct["cell"].append(np.pad(cell, ((0, 4 - nv), (0, 0))))
+            ct["tab"].append(np.concatenate([d["rkt"], d["norm"], d["tau1"], d["tau2"], [d["area"], d["diam"]],
+                                             d["norm_func"], d["center"]]))
+    for key, v in ct.items():
+        out["cf_" + key] = np.asarray(v)
+    np.savez_compressed(os.path.join(HERE, "kernels.npz"), **out)
+    print("kernels:", len(out["f"]), "analytic:", len(out["an_which"]))
+
+
+def ref_matrix(ref, verts, k, npd, op=2):
+    """Z[2i+a, 2j+b] = tau_i^a . Op(tau_j^b, rkt_i, cell_j) from the reference's operator."""
+    n = verts.shape[0]
+    tabs = [ref.cellfill(c) for c in verts]
+    rkt = np.array([t["rkt"] for t in tabs])
+    tau = np.array([[t["tau1"], t["tau2"]] for t in tabs])       # [n][2][3]
+    npp = O.num_param(**npd)
+    Z = np.empty((2 * n, 2 * n), dtype=complex)
+    for b in range(2):
+        res = ref.op_pairs(op, verts, tau[:, b, :], rkt, npp, k)   # [i][j][3]
+        for a in range(2):
+            t = tau[:, a, :]
+            z = (res[:, :, 0] * t[:, None, 0] + res[:, :, 1] * t[:, None, 1]) + res[:, :, 2] * t[:, None, 2]
+            Z[a::2, b::2] = z
+    return Z, rkt, tau
+
+
+def gen_config1(ref, nv=4):
+    """Config 1: PEC sphere r = 1, 16x32 theta-phi quads, +-0.5% jitter (seed 1),
+    ka = 2, E along x propagating +z, 1,000 points on r = 3 (seed 2)."""
+    k = 2.0
+    verts = mesh.sphere_quads(16, 32, radius=1.0, jitter=0.005, seed=1)
+    if nv == 3:
+        verts = mesh.triangulate(verts)
+    n = verts.shape[0]
+    npd = NP_SETS[0]
+    Z, rkt, tau = ref_matrix(ref, verts, k, npd, op=2)
+    ZR, _, _ = ref_matrix(ref, verts, k, npd, op=6)
+    kvec, e0 = np.array([0., 0., k]), np.array([1., 0., 0.])
+    einc = ref.plane_wave_e(rkt, kvec, e0)
+    rhs = np.empty(2 * n, dtype=complex)
+    for a in range(2):
+        t = tau[:, a, :]
+        rhs[a::2] = -((einc[:, 0] * t[:, 0] + einc[:, 1] * t[:, 1]) + einc[:, 2] * t[:, 2])
+    lu = scipy.linalg.lu_factor(Z)
+    b = scipy.linalg.lu_solve(lu, rhs)
+    jE = ref.ref_currents_by_basis(tau.reshape(n, 6), b)
+    pts = mesh.sphere_points(1000, 3.0, seed=2)
+    E = ref.get_field_e(verts, jE, 1.0, kvec, e0, k, O.num_param(**npd), pts)
+    Einc = ref.plane_wave_e(pts, kvec, e0)
+    H = ref.op_pairs(6, verts, jE, pts, O.num_param(**npd), k).sum(axis=1)
+    rng = np.random.default_rng(5)
+    w = rng.normal(size=2 * n) + 1j * rng.normal(size=2 * n)
+    sample = rng.choice(n * n, size=4000, replace=False)
+    si, sj = sample // n, sample % n
+    blocks = np.stack([Z[2 * si + a, 2 * sj + b] for a in range(2) for b in range(2)], axis=1)
+    rblocks = np.stack([ZR[2 * si + a, 2 * sj + b] for a in range(2) for b in range(2)], axis=1)
+    diag = np.stack([Z[2 * np.arange(n) + a, 2 * np.arange(n) + b] for a in range(2) for b in range(2)], axis=1)
+    name = "config1.npz" if nv == 4 else "config1_tri.npz"
+    np.savez_compressed(os.path.join(HERE, name), verts=verts, k=k, kvec=kvec, e0=e0, pts=pts,
+                        num_param=np.array([npd[kk] for kk in ("eps", "rs", "rs_seg", "analytic_calc", "n_start",
+                                                               "n_start_seg", "p_max", "p_max_seg")]),
+                        sample_i=si, sample_j=sj, blocks=blocks, rblocks=rblocks, diag=diag, row_w=w,
+                        row_sums=Z @ w, rrow_sums=ZR @ w, col_sums=w @ Z, rhs=rhs, b=b, jE=jE, E=E, Einc=Einc,
+                        H=H, Znorm=np.linalg.norm(Z), cond=np.linalg.cond(Z))
+    print(name, "n =", n, "cond(Z) =", np.linalg.cond(Z))
+
+
+def gen_mesh_files(ref):
+    out = {}
+    for fname, nv in (("sphere_quad.dat", 4), ("sphere_triangle.dat", 3), ("sphere_quad.vtk", 4)):
+        d = ref.ref_read_mesh(os.path.join(REF_DATA, fname), nv)
+        key = fname.replace(".", "_")
+        for kk, v in d.items():
+            out[f"{key}_{kk}"] = np.asarray(v)
+    np.savez_compressed(os.path.join(HERE, "mesh_files.npz"), **out)
+    print("mesh files:", [k for k in out if k.endswith("_verts")])
+
+
+if __name__ == "__main__":
+    ref = O.Checker("ref")
+    which = sys.argv[1:] or ["ops", "integrators", "kernels", "config1", "config1_tri", "mesh_files"]
+    if "ops" in which:
+        gen_ops(ref)
+    if "integrators" in which:
+        gen_integrators(ref)
+    if "kernels" in which:
+        gen_kernels_analytic(ref)
+    if "config1" in which:
+        gen_config1(ref, 4)
+    if "config1_tri" in which:
+        gen_config1(ref, 3)
+    if "mesh_files" in which:
+        gen_mesh_files(ref)

@@ -1,0 +1,143 @@
+"""Collocation matrix, RHS, LU solve and currents on the GPU against the
+reference's recorded outputs (config 1: 512-quad sphere and its 960-triangle
+split, ka = 2) and the oracle.  Bars: blocks 1e-12 (per-block metric),
+currents within the conditioning-limited bound stated per test."""
+import numpy as np
+import pytest
+
+import bielcc_amd as B
+import oracle as O
+from _util import golden, np_dict, rel_err, rowwise_rel, block_metric
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=["config1.npz", "config1_tri.npz"])
+def c1(request, ctx):
+    g = golden(request.param)
+    m = ctx.mesh(g["verts"])
+    npp = B.num_param(**np_dict(g["num_param"]))
+    Z = m.assemble(npp, float(g["k"]), which=0)
+    yield g, m, npp, Z
+    m.close()
+
+
+def test_matrix_vs_reference_samples(c1):
+    g, m, npp, Z = c1
+    n = g["verts"].shape[0]
+    si, sj = g["sample_i"], g["sample_j"]
+    blocks = np.stack([Z[2 * si + a, 2 * sj + b] for a in range(2) for b in range(2)], axis=1)
+    bm = np.sqrt((np.abs(blocks - g["blocks"]) ** 2).sum(1)) / np.sqrt((np.abs(g["blocks"]) ** 2).sum(1))
+    assert bm.max() <= 1e-12, bm.max()
+    diag = np.stack([Z[2 * np.arange(n) + a, 2 * np.arange(n) + b] for a in range(2) for b in range(2)], axis=1)
+    assert rel_err(diag, g["diag"]) <= 1e-12
+    assert np.linalg.norm(Z @ g["row_w"] - g["row_sums"]) <= 1e-12 * np.linalg.norm(g["row_sums"])
+    assert np.linalg.norm(g["row_w"] @ Z - g["col_sums"]) <= 1e-12 * np.linalg.norm(g["col_sums"])
+
+
+def test_matrix_vs_oracle_every_block(c1, orc):
+    g, m, npp, Z = c1
+    Zo = orc.assemble(g["verts"], O.num_param(**np_dict(g["num_param"])), float(g["k"]), which=0)
+    bm = block_metric(Z, Zo)
+    assert bm.max() <= 1e-12, bm.max()
+
+
+def test_r_matrix(c1, orc):
+    g, m, npp, _ = c1
+    ZR = m.assemble(npp, float(g["k"]), which=1)
+    si, sj = g["sample_i"], g["sample_j"]
+    rb = np.stack([ZR[2 * si + a, 2 * sj + b] for a in range(2) for b in range(2)], axis=1)
+    assert rel_err(rb, g["rblocks"]) <= 1e-12
+    assert np.linalg.norm(ZR @ g["row_w"] - g["rrow_sums"]) <= 1e-12 * np.linalg.norm(g["rrow_sums"])
+    # self blocks are exactly zero (R_Rot_Colloc at the centroid, R.h:94-97)
+    n = g["verts"].shape[0]
+    assert np.all(ZR[2 * np.arange(n), 2 * np.arange(n)] == 0)
+
+
+def test_partial_rows_and_complex_k(ctx, orc):
+    g = golden("config1.npz")
+    verts = g["verts"][:200]
+    m = ctx.mesh(verts)
+    k = 2.0 + 0.25j
+    npp = B.num_param()
+    Zfull = m.assemble(npp, k)
+    Zpart = m.assemble(npp, k, row0=40, nrows=60)
+    assert np.array_equal(Zpart, Zfull[40:100])
+    Zo = orc.assemble(verts, O.num_param(), k, row0=40, nrows=60)
+    assert block_metric(Zpart, Zo).max() <= 1e-12
+    m.close()
+
+
+def test_adaptive_parameters_generic_path(ctx, orc):
+    """p_max > 1 (the examples' settings, main0_K.cpp:40-73) runs the exact per-pair kernel."""
+    verts = B.mesh.sphere_quads(6, 12, jitter=0.005, seed=9)
+    m = ctx.mesh(verts)
+    npd = dict(n_start=6, n_start_seg=8, p_max=3, p_max_seg=2, eps=1e-5)
+    Z = m.assemble(B.num_param(**npd), 1.5)
+    Zo = orc.assemble(verts, O.num_param(**npd), 1.5)
+    assert block_metric(Z, Zo).max() <= 1e-12
+    m.close()
+
+
+def test_rhs_and_currents(c1):
+    g, m, npp, Z = c1
+    rhs = m.rhs_planewave(g["kvec"], g["e0"])[:, 0]
+    assert rel_err(rhs, g["rhs"]) <= 1e-15
+    j = m.currents(g["b"])[0]
+    assert rel_err(j, g["jE"]) <= 1e-15
+
+
+def test_lu_solve_currents(c1):
+    """Currents from the GPU LU of the GPU matrix vs LAPACK on the reference
+    matrix.  The reference has no solver (parity of the solve is against
+    LAPACK); the bound is cond(Z) * 1e-15 with cond recorded in the fixture."""
+    g, m, npp, Z = c1
+    b = m.solve(Z, g["rhs"])[:, 0]
+    cond = float(g["cond"])
+    err = np.linalg.norm(b - g["b"]) / np.linalg.norm(g["b"])
+    assert err <= max(1e-12, 2e-15 * cond), (err, cond)
+    res = np.linalg.norm(Z @ b - g["rhs"]) / np.linalg.norm(g["rhs"])
+    assert res <= 1e-13, res
+
+
+def test_lu_random_multirhs(ctx):
+    rng = np.random.default_rng(3)
+    for n in (1, 63, 64, 65, 700):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 4 * np.sqrt(n) * np.eye(n)
+        Bm = rng.normal(size=(n, 5)) + 1j * rng.normal(size=(n, 5))
+        X = ctx_solve(ctx, A, Bm)
+        Xr = np.linalg.solve(A, Bm)
+        assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-13, n
+
+
+def ctx_solve(ctx, A, Bm):
+    n = A.shape[0]
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dB = ctx.to_device(np.ascontiguousarray(Bm.T))
+    piv = ctx.empty(4 * n)
+    ctx.lu_factor(dA, n, n, piv)
+    ctx.lu_solve(dA, n, n, piv, dB, Bm.shape[1], n)
+    return dB.to_numpy((Bm.shape[1], n), np.complex128).T
+
+
+def test_lu_singular_reports(ctx):
+    n = 10
+    A = np.eye(n, dtype=complex)
+    A[:, 4] = 0
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    piv = ctx.empty(4 * n)
+    with pytest.raises(B.BielccError) as e:
+        ctx.lu_factor(dA, n, n, piv)
+    assert e.value.code == 5
+
+
+def test_bad_arguments(ctx):
+    m = ctx.mesh(golden("config1.npz")["verts"][:10])
+    with pytest.raises(B.BielccError) as e:
+        m.assemble(B.num_param(), 1.0, row0=1, nrows=2)
+    assert e.value.code == 1
+    with pytest.raises(B.BielccError):
+        m.assemble(B.num_param(n_start=0), 1.0)
+    with pytest.raises(ValueError):
+        ctx.mesh(np.zeros((3, 5, 3)))
+    m.close()

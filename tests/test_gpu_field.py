@@ -1,0 +1,81 @@
+"""Scattered field on the GPU against the reference's GetFieldE_Ideal output
+(config 1) and the oracle (near field, magnetic currents, the example's
+near rule, lossy media).  Bar: 1e-10 relative per point."""
+import numpy as np
+import pytest
+
+import bielcc_amd as B
+import oracle as O
+from _util import golden, np_dict, rowwise_rel
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.mark.parametrize("fname", ["config1.npz", "config1_tri.npz"])
+def test_get_field_e_vs_reference(ctx, fname):
+    g = golden(fname)
+    m = ctx.mesh(g["verts"])
+    npp = B.num_param(**np_dict(g["num_param"]))
+    E = m.get_field_e(g["jE"], 1.0, g["kvec"], g["e0"], npp, float(g["k"]), g["pts"])
+    assert rowwise_rel(E, g["E"]).max() <= TOL
+    Ef, Hf = m.field(npp, float(g["k"]), g["jE"], g["pts"], incident=(g["kvec"], g["e0"]))
+    assert rowwise_rel(Ef, g["E"]).max() <= TOL
+    assert rowwise_rel(Hf, g["H"]).max() <= TOL
+    m.close()
+
+
+def _near_points(rng, m_pts, radius):
+    p = rng.normal(size=(m_pts, 3))
+    return radius * p / np.linalg.norm(p, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("near_mode", [0, 1])
+def test_near_field_vs_oracle(ctx, orc, near_mode):
+    rng = np.random.default_rng(21)
+    verts = B.mesh.sphere_quads(10, 20, jitter=0.005, seed=5)
+    n = verts.shape[0]
+    jE = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    jM = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = np.concatenate([_near_points(rng, 24, 1.04), _near_points(rng, 8, 0.97), _near_points(rng, 8, 2.5)])
+    k = 3.0
+    npp = B.num_param()
+    m = ctx.mesh(verts)
+    E, H = m.field(npp, k, jE, pts, jM=jM, multE=0.3 + 1j, multH=-0.5j, near_mode=near_mode)
+    Eo, Ho = orc.field_eh(verts, jE, k, O.num_param(), pts, jM=jM, multE=0.3 + 1j, multH=-0.5j, near_mode=near_mode)
+    assert rowwise_rel(E, Eo).max() <= TOL
+    assert rowwise_rel(H, Ho).max() <= TOL
+    m.close()
+
+
+def test_triangles_complex_k_vs_oracle(ctx, orc):
+    rng = np.random.default_rng(22)
+    verts = B.mesh.triangulate(B.mesh.sphere_quads(8, 16, jitter=0.005, seed=6))
+    n = verts.shape[0]
+    jE = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = np.concatenate([_near_points(rng, 16, 1.05), _near_points(rng, 16, 3.0)])
+    k = 2.5 + 0.4j
+    m = ctx.mesh(verts)
+    E, H = m.field(B.num_param(), k, jE, pts, incident=((0., 0., 2.5), (1., 0., 0.)))
+    Eo, Ho = orc.field_eh(verts, jE, k, O.num_param(), pts, incident=((0., 0., 2.5), (1., 0., 0.)))
+    assert rowwise_rel(E, Eo).max() <= TOL
+    assert rowwise_rel(H, Ho).max() <= TOL
+    m.close()
+
+
+def test_field_linearity_in_currents(ctx):
+    """E and H are linear in (jE, jM): a size-independent property."""
+    rng = np.random.default_rng(23)
+    verts = B.mesh.sphere_quads(8, 16, seed=8)
+    n = verts.shape[0]
+    j1 = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    j2 = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = B.mesh.sphere_points(64, 2.0, seed=3)
+    m = ctx.mesh(verts)
+    npp = B.num_param()
+    E1, H1 = m.field(npp, 2.0, j1, pts)
+    E2, H2 = m.field(npp, 2.0, j2, pts)
+    E3, H3 = m.field(npp, 2.0, 2.0 * j1 - 0.5j * j2, pts)
+    assert rowwise_rel(E3, 2.0 * E1 - 0.5j * E2).max() <= 1e-12
+    assert rowwise_rel(H3, 2.0 * H1 - 0.5j * H2).max() <= 1e-12
+    m.close()

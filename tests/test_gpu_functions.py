@@ -1,0 +1,109 @@
+"""Single reference functions on the GPU (batched C-ABI entry points) against
+the reference's recorded outputs and the oracle.  FP64 bar: 1e-12 relative
+per result vector (non-finite results must match the reference's)."""
+import numpy as np
+import pytest
+
+import bielcc_amd as B
+from _util import golden, np_dict, rowwise_rel, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _groups(keys):
+    groups = {}
+    for i, key in enumerate(zip(*keys)):
+        groups.setdefault(tuple(k.item() if hasattr(k, "item") else k for k in key), []).append(i)
+    return groups
+
+
+def test_operators_vs_reference(ctx):
+    g = golden("ops.npz")
+    worst = 0.0
+    for nv in (4, 3):
+        for (op, npi, k), idx in _groups([g[f"op{nv}"], g[f"np{nv}"], g[f"k{nv}"]]).items():
+            idx = np.array(idx)
+            npp = B.num_param(**np_dict(g["np_sets"][npi]))
+            r = ctx.op_batch(op, g[f"cells{nv}"][idx], g[f"x{nv}"][idx], g[f"j{nv}"][idx], npp, k)
+            e = rowwise_rel(r, g[f"res{nv}"][idx])
+            worst = max(worst, float(e.max()) if e.size else 0.0)
+            assert np.all(e <= TOL), (nv, op, npi, k, float(e.max()))
+    print("operators worst rel err", worst)
+
+
+def test_kernels_vs_reference(ctx):
+    g = golden("kernels.npz")
+    for (f, R, Rmin, k), idx in _groups([g["f"], g["kp"][:, 0], g["kp"][:, 1], g["k"]]).items():
+        idx = np.array(idx)
+        kp = B.kernel_param(R, Rmin, k, vec_dbl=(0.3, -0.2, 0.5), n=(0.0, 0.6, 0.8), current=(1 + 1j, 0.5, -0.2j))
+        r = ctx.kernel_batch(f, g["x"][idx], g["y"][idx], kp)
+        e = rowwise_rel(r, g["res"][idx])
+        assert np.all(e <= TOL), (f, R, k, float(e.max()))
+
+
+def test_integrators_vs_reference(ctx):
+    g = golden("integrators.npz")
+    keys = [g["kind"], g["f"], g["nv"], g["kp"][:, 0], g["kp"][:, 1], g["ip"][:, 0], g["ip"][:, 1], g["ip"][:, 2],
+            g["k"]]
+    for (kind, f, nv, R, Rmin, idim, n0, pm, k), idx in _groups(keys).items():
+        idx = np.array(idx)
+        kp = B.kernel_param(R, Rmin, k, vec_dbl=(0.3, -0.2, 0.5), n=(0.0, 0.6, 0.8), current=(1 + 1j, 0.5, -0.2j))
+        ip = B.IntegralParam(int(idim), int(n0), int(pm), 1e-6)
+        if kind == 0:
+            r, Ro = ctx.integral_batch(f, g["cell"][idx][:, :nv], g["x"][idx], kp, ip)
+        elif kind == 1:
+            r, Ro = ctx.integral_batch(f, g["cell"][idx][:, :2], g["x"][idx], kp, ip)
+        else:
+            r, Ro = ctx.k0_batch(kind - 2, f, g["cell"][idx][:, :nv], g["x"][idx], g["j"][idx], kp, ip)
+        e = rowwise_rel(r, g["res"][idx])
+        assert np.all(e <= TOL), (kind, f, nv, R, pm, float(e.max()))
+        # the caller-visible smoothing radius after the call (KernelParam& side effect)
+        assert np.array_equal(Ro, g["R_out"][idx]), (kind, f)
+
+
+def test_analytic_vs_reference(ctx):
+    g = golden("kernels.npz")
+    for (w, nv, k), idx in _groups([g["an_which"], g["an_nv"], g["an_k"]]).items():
+        idx = np.array(idx)
+        cells, x = g["an_cell"][idx], g["an_x"][idx]
+        if w == 0:
+            r = ctx.analytic_batch(0, cells[:, :nv], x)
+        elif w == 1:
+            r = ctx.analytic_batch(1, cells[:, :2], x)
+        elif w == 2:
+            r = ctx.analytic_batch(2, cells[:, :nv], x, B.IntegralParam(1, 8, 1, 1e-6), 3.0, k)
+        elif w == 3:
+            r = ctx.analytic_batch(3, cells[:, :2], x, B.IntegralParam(3, 10, 2, 1e-6), 3.0, k)
+        else:
+            j = np.tile(np.array([1 + 0.5j, -0.3, 0.2j]), (len(idx), 1))
+            kp = B.kernel_param(0., 0., k)
+            r, _ = ctx.k0_batch(2, 0, cells[:, :nv], x, j, kp, B.IntegralParam(3, 10, 2, 1e-6), 3.0)
+        e = rowwise_rel(r, g["an_res"][idx])
+        assert np.all(e <= TOL), (w, nv, k, float(e.max()))
+
+
+def test_mesh_tables_vs_reference(ctx):
+    g = golden("kernels.npz")
+    for nv in (4, 3):
+        sel = np.where(g["cf_nv"] == nv)[0]
+        m = ctx.mesh(g["cf_cell"][sel][:, :nv])
+        t = m.tables()
+        tab = np.concatenate([t["rkt"], t["norm"], t["tau1"], t["tau2"], t["area"][:, None], t["diam"][:, None],
+                              t["norm_func"], t["center"]], axis=1)
+        assert rel_err(tab, g["cf_tab"][sel]) <= 1e-15
+        m.close()
+
+
+def test_mesh_tables_vs_oracle(ctx, orc):
+    verts = B.mesh.sphere_quads(12, 24, jitter=0.005, seed=4)
+    m = ctx.mesh(verts)
+    t = m.tables()
+    for c in range(0, verts.shape[0], 17):
+        d = orc.cellfill(verts[c])
+        for key in ("rkt", "norm", "tau1", "tau2", "norm_func", "center"):
+            assert np.allclose(t[key][c], d[key], rtol=0, atol=1e-15), key
+        assert abs(t["area"][c] - d["area"]) <= 1e-15 * d["area"]
+        assert t["diam"][c] == d["diam"]
+    g = m.grid_step()
+    assert g == max(orc.cellfill(v)["diam"] for v in verts)
