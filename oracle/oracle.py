@@ -95,10 +95,11 @@ class Checker:
             raise FileNotFoundError(f"{path} is not built (run `make -C oracle {'ref' if which == 'ref' else 'oracle'}`)")
         self.lib = C.CDLL(path)
         self._kt = [] if which == "oracle" else [C.c_int(1)]
-        self.lib[self.pre + "integral1divr"].restype = C.c_double
+        getattr(self.lib, self.pre + "integral1divr").restype = C.c_double
 
     def _f(self, name):
-        return self.lib[self.pre + name]
+        # attribute access caches the function object, so restype settings persist
+        return getattr(self.lib, self.pre + name)
 
     # -- geometry
     def cellfill(self, cell):

@@ -56,10 +56,41 @@ def test_integrators_vs_reference(ctx):
             r, Ro = ctx.integral_batch(f, g["cell"][idx][:, :2], g["x"][idx], kp, ip)
         else:
             r, Ro = ctx.k0_batch(kind - 2, f, g["cell"][idx][:, :nv], g["x"][idx], g["j"][idx], kp, ip)
-        e = rowwise_rel(r, g["res"][idx])
+        if kind >= 2:
+            # contour sums can cancel: measure against the sum of the edge terms' magnitudes
+            scale = np.array([_k0_scale(kind - 2, f, g["cell"][i][:nv], g["x"][i], g["j"][i], R, Rmin, k, ip)
+                              for i in idx])
+            d = np.linalg.norm(np.nan_to_num(r - g["res"][idx]), axis=1)
+            e = d / np.where(scale > 0, scale, 1.0)
+        else:
+            e = rowwise_rel(r, g["res"][idx])
         assert np.all(e <= TOL), (kind, f, nv, R, pm, float(e.max()))
         # the caller-visible smoothing radius after the call (KernelParam& side effect)
         assert np.array_equal(Ro, g["R_out"][idx]), (kind, f)
+
+
+def _k0_scale(variant, f, cell, x, j, R, Rmin, k, ip):
+    """sum over edges of |(j . nu_e)| |int_e F| (variant 0) or |int_e F| (variant 1),
+    with the smoothing radius carried from edge to edge as K0 does (oracle)."""
+    import oracle as O
+    kp = O.kernel_param(R, Rmin, k, vec_dbl=(0.3, -0.2, 0.5), n=(0.0, 0.6, 0.8), current=(1 + 1j, 0.5, -0.2j))
+    oip = O.IntegralParam(ip.idim, ip.n_start, ip.p_max, ip.eps)
+    orc = O.Checker("oracle")
+    nv = len(cell)
+    nrm = orc.cellfill(cell)["norm_func"]
+    tot = 0.0
+    for s in range(nv):
+        a, b = cell[s], cell[(s + 1) % nv]
+        nu = np.cross(b - a, nrm)
+        ln = np.linalg.norm(nu)
+        nu = nu / ln if ln > np.finfo(float).eps else nu
+        if variant == 1:
+            for i in range(3):
+                kp.vec_dbl[i] = nu[i]
+        ks = orc.integral_seg(f, a, b, x, kp, oip)
+        w = abs(np.dot(j, nu)) if variant == 0 else 1.0
+        tot += w * np.linalg.norm(np.nan_to_num(ks))
+    return tot
 
 
 def test_analytic_vs_reference(ctx):
@@ -80,7 +111,9 @@ def test_analytic_vs_reference(ctx):
             kp = B.kernel_param(0., 0., k)
             r, _ = ctx.k0_batch(2, 0, cells[:, :nv], x, j, kp, B.IntegralParam(3, 10, 2, 1e-6), 3.0)
         e = rowwise_rel(r, g["an_res"][idx])
-        assert np.all(e <= TOL), (w, nv, k, float(e.max()))
+        bad = np.where(e > TOL)[0]
+        assert bad.size == 0, [(w, nv, k, int(idx[b]), cells[b].tolist(), x[b].tolist(), r[b].tolist(),
+                                g["an_res"][idx[b]].tolist()) for b in bad[:3]]
 
 
 def test_mesh_tables_vs_reference(ctx):
