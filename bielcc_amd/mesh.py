@@ -20,7 +20,7 @@ import numpy as np
 
 __all__ = [
     "MT19937_64", "uniforms", "sphere_quads", "triangulate", "ellipsoid_quads",
-    "torus_quads", "cube_quads", "sphere_points", "grid_points", "write_dat", "read_dat",
+    "torus_quads", "cube_quads", "sphere_points", "grid_points", "write_dat", "read_dat", "read_vtk", "read_mesh",
 ]
 
 _MASK64 = (1 << 64) - 1
@@ -138,9 +138,9 @@ def torus_quads(n_u: int, n_v: int, R: float = 1.0, r: float = 0.3) -> np.ndarra
         for j in range(n_v):
             i1, j1 = (i + 1) % n_u, (j + 1) % n_v
             v[i, j, 0] = P(u[i], w[j])
-            v[i, j, 1] = P(u[i], w[j1])
+            v[i, j, 1] = P(u[i1], w[j])
             v[i, j, 2] = P(u[i1], w[j1])
-            v[i, j, 3] = P(u[i1], w[j])
+            v[i, j, 3] = P(u[i], w[j1])
     return v.reshape(-1, 4, 3)
 
 
@@ -205,6 +205,59 @@ def write_dat(path: str, verts: np.ndarray) -> None:
             for v in range(nv):
                 f.write("%.17g %.17g %.17g\n" % tuple(verts[c, v]))
         f.write("0\n")
+
+
+def read_vtk(path: str, nv: int) -> np.ndarray:
+    """Read an ASCII legacy VTK surface (POINTS + CELLS or POLYGONS) as the
+    reference's reader does (mesh/surf_mesh.h:367-555, surf_mesh.cpp:195-294):
+    cells with exactly nv vertices are kept in file order, others skipped."""
+    points, polys = None, []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = 0
+    while i < len(lines):
+        line = lines[i].strip()
+        i += 1
+        if not line or line.startswith("#"):
+            continue
+        tok = line.split()
+        kw = tok[0]
+        if kw == "POINTS":
+            npts = int(tok[1])
+            pts = []
+            while len(pts) < npts:
+                vals = lines[i].split()
+                i += 1
+                pts.append([float(v) for v in vals[:3]])
+            points = np.asarray(pts, dtype=np.float64)
+        elif kw in ("CELLS", "POLYGONS"):
+            ncell = int(tok[1])
+            polys = []
+            for _ in range(ncell):
+                vals = lines[i].split()
+                i += 1
+                n = int(vals[0])
+                polys.append([int(v) for v in vals[1:1 + n]])
+        elif kw == "CELL_TYPES":
+            i += int(tok[1])
+        elif kw in ("CELL_DATA", "POINT_DATA"):
+            break
+    if points is None or not polys:
+        raise ValueError(f"{path}: no points or cells")
+    cells = [points[p] for p in polys if len(p) == nv]
+    for p in polys:
+        if len(p) == nv and (min(p) < 0 or max(p) >= len(points)):
+            raise ValueError(f"{path}: point index out of range")
+    return np.ascontiguousarray(np.asarray(cells, dtype=np.float64).reshape(-1, nv, 3))
+
+
+def read_mesh(path: str, nv: int) -> np.ndarray:
+    """SurfMesh::ReadFromFile (mesh/surf_mesh.h:353-364): by file extension."""
+    if path.endswith(".dat"):
+        return read_dat(path, nv)
+    if path.endswith(".vtk"):
+        return read_vtk(path, nv)
+    raise ValueError(f"unsupported mesh file type: {path}")
 
 
 def read_dat(path: str, nv: int) -> np.ndarray:

@@ -1,0 +1,189 @@
+"""Pins of the oracle that do not come from the reference's recorded outputs:
+closed forms, finite-difference identities, the expectations the reference's
+examples state in their comments, and the Mie series for a perfectly
+conducting sphere (whole-chain physics check).  CPU only."""
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import oracle as O
+from bielcc_amd import mesh
+from _util import golden, np_dict
+
+RUT1 = np.array([[0., 0., 0.], [2., 0., 0.], [3., 2., 0.], [0., 2., 0.]])
+RUT2 = np.array([[0., 2., 0.], [0., 0., 0.], [2., 0., 0.]])
+RUT3 = np.array([[3., 2., 0.], [2., 0., 0.], [0., 2., 0.]])
+EPS = np.finfo(float).eps
+
+
+def _gauss_tri(tri, f, n=60):
+    """Brute-force integral over a flat triangle (collapsed Gauss-Legendre)."""
+    t, w = np.polynomial.legendre.leggauss(n)
+    t, w = 0.5 * (t + 1), 0.5 * w
+    tot = 0.0
+    a, b, c = tri
+    area2 = np.linalg.norm(np.cross(b - a, c - a))
+    for ui, wu in zip(t, w):
+        for vi, wv in zip(t, w):
+            u, v = ui, vi * (1 - ui)
+            y = a + u * (b - a) + v * (c - a)
+            tot += wu * wv * (1 - ui) * f(y)
+    return tot * area2
+
+
+def test_integral1divr_closed_form(orc):
+    """integral1Divr (integrals_analytic.cpp:37-235) = brute-force int 1/|x-y| dS."""
+    for tri in (RUT2, RUT3):
+        for x in ([0.7, 0.4, 0.3], [0.5, 0.5, -1.2], [3.0, -1.0, 0.8]):
+            x = np.array(x)
+            ref = _gauss_tri(tri, lambda y: 1.0 / np.linalg.norm(x - y))
+            assert abs(orc.integral1divr(tri, x) - ref) <= 1e-9 * abs(ref)
+    # a quad is the sum of its triangles (0,1,2) + (2,3,0) (integrals_analytic.cpp:218-231)
+    x = np.array([0.9, 1.1, 0.6])
+    s = orc.integral1divr(RUT1[[0, 1, 2]], x) + orc.integral1divr(RUT1[[2, 3, 0]], x)
+    assert orc.integral1divr(RUT1, x) == s
+
+
+def test_cell_rule_area_and_far_limit(orc):
+    """main0_integrals.cpp:24-27: the rut1 trapezoid has area 5.  Far away the
+    cell rule of 1/r (F_SimplePot_L, no smoothing) tends to area / distance and
+    agrees with the exact integral to O((h/d)^2)."""
+    d = orc.cellfill(RUT1)
+    assert d["area"] == 5.0
+    for cell in (RUT1, RUT2, RUT3):
+        x = cell.mean(axis=0) + np.array([0.0, 0.0, 1e5])
+        kp = O.kernel_param(EPS, EPS, 0.0)
+        num = orc.integral_pnt(0, cell, x, kp, O.IntegralParam(1, 8, 1, 1e-6))[0].real
+        area = 0.5 * np.linalg.norm(np.cross(cell[1] - cell[0], cell[2] - cell[0])) if len(cell) == 3 else 5.0
+        assert abs(num * 1e5 - area) <= 1e-9 * area
+        assert abs(num - orc.integral1divr(cell, x)) <= 1e-9 * num
+
+
+def test_segment_rule_matches_closed_form(orc):
+    """The segment midpoint rule of F_GradSimplePot_L = grad_x 1/|x-y| agrees
+    with integralGrad1DivrSeg's closed form (integrals_analytic.cpp:240-257)."""
+    A, B = np.array([0., 0., 0.]), np.array([1.5, 0.5, 0.2])
+    for x in ([0.3, 1.0, 0.4], [2.0, -1.0, 1.0]):
+        x = np.array(x)
+        kp = O.kernel_param(EPS, EPS, 0.0)
+        num = orc.integral_seg(1, A, B, x, kp, O.IntegralParam(3, 4000, 1, 1e-6)).real
+        ana = orc.integral_grad1divr_seg(A, B, x)
+        assert np.allclose(num, ana, rtol=1e-6, atol=0)
+
+
+def test_gradient_kernel_is_gradient(orc):
+    """F_GradSimplePot_H = grad_x F_SimplePot_H (kernel_lib.h:209-217 vs :295-313)."""
+    rng = np.random.default_rng(1)
+    for k in (1.0, 3.0 + 0.5j):
+        kp = O.kernel_param(EPS, EPS, k)
+        for _ in range(5):
+            x, y = rng.normal(size=3), rng.normal(size=3)
+            g = orc.kernel(7, x, y, kp)
+            h = 1e-6
+            fd = np.array([(orc.kernel(4, x + h * e, y, kp)[0] - orc.kernel(4, x - h * e, y, kp)[0]) / (2 * h)
+                           for e in np.eye(3)])
+            assert np.allclose(g, fd, rtol=1e-6, atol=1e-8)
+
+
+def test_rotrot_kernel_identity(orc):
+    """F_RotRotSimplePot_H = curl curl (F j) = grad (j . grad F) + k^2 F j (kernel_lib.h:453-492)."""
+    rng = np.random.default_rng(2)
+    k = 2.0
+    j = np.array([1.0 + 0.5j, -0.3, 0.2j])
+    kp = O.kernel_param(EPS, EPS, k, current=j)
+    for _ in range(4):
+        x, y = rng.normal(size=3), 3 * rng.normal(size=3)
+        rr = orc.kernel(11, x, y, kp)
+        h = 1e-4
+
+        def jdotgrad(p):
+            return np.dot(j, orc.kernel(7, p, y, kp))
+        gradjg = np.array([(jdotgrad(x + h * e) - jdotgrad(x - h * e)) / (2 * h) for e in np.eye(3)])
+        rhs = gradjg + k * k * orc.kernel(4, x, y, kp)[0] * j
+        assert np.allclose(rr, rhs, rtol=1e-6, atol=1e-8)
+
+
+def test_laplace_is_helmholtz_at_k0(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        x, y = rng.normal(size=3), rng.normal(size=3)
+        for R in (EPS, 0.5, 10.0):
+            kp = O.kernel_param(R, EPS, 0.0)
+            assert orc.kernel(0, x, y, kp)[0] == orc.kernel(4, x, y, kp)[0]
+            # (y - x)/r^3 (kernel_lib.h:87-98) = (x - y)(ikr - 1)/r^3 at k = 0 (:295-313)
+            assert np.array_equal(orc.kernel(1, x, y, kp).real, orc.kernel(7, x, y, kp).real)
+
+
+def test_k_operator_variants_agree_far_away(orc):
+    """main0_K.cpp:123-124: at x = (10,10,10) K_RotRot_Near_Smooth, K_RotRot_HS
+    and K_RotRot_Far agree (surface + contour form vs the rot-rot kernel)."""
+    x = np.array([10., 10., 10.])
+    j = np.array([1.0, 0.0, 0.0], dtype=complex)
+    npp = O.num_param(n_start=40, n_start_seg=80)
+    far = orc.op(4, RUT1, x, j, npp, 1.0)
+    for op in (0, 1, 2):
+        v = orc.op(op, RUT1, x, j, npp, 1.0)
+        assert np.linalg.norm(v - far) <= 2e-4 * np.linalg.norm(far), op
+
+
+def test_r_operator_collocation(orc):
+    """main0_R.cpp:161-184: R_Rot_Colloc is 0 at the centroid; off the centroid
+    it equals R_Rot_Smooth without smoothing."""
+    j = np.array([1.0, 1.0, 1.0], dtype=complex)
+    npp = O.num_param(rs=EPS, rs_seg=EPS, n_start=80, n_start_seg=100, p_max=4, p_max_seg=4)
+    c = orc.cellfill(RUT2)["rkt"]
+    assert np.all(orc.op(6, RUT2, c, j, npp, 1.0) == 0)
+    x = np.array([2. / 3., 2. / 3., 0.])
+    a, b = orc.op(6, RUT3, x, j, npp, 1.0), orc.op(5, RUT3, x, j, npp, 1.0)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+
+
+def _mie_pec_scattered(pts, k, a=1.0, nmax=30):
+    """Scattered E of a PEC sphere (radius a) for E_inc = x exp(ikz), time
+    convention exp(-i w t) (Bohren & Huffman 4.45-4.53 with m -> infinity)."""
+    x = k * a
+    n = np.arange(1, nmax + 1)
+    psi = x * sp.spherical_jn(n, x)
+    dpsi = sp.spherical_jn(n, x) + x * sp.spherical_jn(n, x, derivative=True)
+    h = sp.spherical_jn(n, x) + 1j * sp.spherical_yn(n, x)
+    dh = (sp.spherical_jn(n, x, derivative=True) + 1j * sp.spherical_yn(n, x, derivative=True))
+    xi, dxi = x * h, h + x * dh
+    an, bn = dpsi / dxi, psi / xi
+    En = 1j ** n * (2 * n + 1) / (n * (n + 1))
+    out = np.zeros((len(pts), 3), dtype=complex)
+    for p, r_ in enumerate(pts):
+        r = np.linalg.norm(r_)
+        th = np.arccos(r_[2] / r)
+        ph = np.arctan2(r_[1], r_[0])
+        mu = np.cos(th)
+        pi = np.zeros(nmax + 1); tau = np.zeros(nmax + 1)
+        pi[1] = 1.0
+        for m in range(2, nmax + 1):
+            pi[m] = (2 * m - 1) / (m - 1) * mu * pi[m - 1] - m / (m - 1) * pi[m - 2]
+        for m in range(1, nmax + 1):
+            tau[m] = m * mu * pi[m] - (m + 1) * pi[m - 1]
+        rho = k * r
+        z = sp.spherical_jn(n, rho) + 1j * sp.spherical_yn(n, rho)
+        dz = sp.spherical_jn(n, rho, derivative=True) + 1j * sp.spherical_yn(n, rho, derivative=True)
+        drz = (z + rho * dz) / rho
+        Er = np.sum(En * 1j * an * np.cos(ph) * n * (n + 1) * np.sin(th) * pi[1:] * z / rho)
+        Et = np.sum(En * (1j * an * np.cos(ph) * tau[1:] * drz - bn * np.cos(ph) * pi[1:] * z))
+        Ep = np.sum(En * (-1j * an * np.sin(ph) * pi[1:] * drz + bn * np.sin(ph) * tau[1:] * z))
+        rh = r_ / r
+        th_h = np.array([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
+        ph_h = np.array([-np.sin(ph), np.cos(ph), 0.0])
+        out[p] = Er * rh + Et * th_h + Ep * ph_h
+    return out
+
+
+def test_config1_scattered_field_vs_mie(orc):
+    """Whole chain (assembly, solve, currents, field) at config 1 against the Mie
+    series: discretisation-limited agreement, not a parity bar."""
+    g = golden("config1.npz")
+    npp = O.num_param(**np_dict(g["num_param"]))
+    pts = g["pts"][:200]
+    E = orc.get_field_e(g["verts"], g["jE"], 1.0, g["kvec"], g["e0"], float(g["k"]), npp, pts)
+    Es = E - orc.plane_wave_e(pts, g["kvec"], g["e0"])
+    Em = _mie_pec_scattered(pts, float(g["k"]))
+    err = np.linalg.norm(Es - Em) / np.linalg.norm(Em)
+    assert err < 0.1, err
