@@ -45,9 +45,24 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_example()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+EXAMPLE = os.path.join(HERE, "lib", "dropin_example")
+
+
+def build_cpp_example() -> str:
+    """The drop-in C++ layer's example program (host g++ against the C ABI)."""
+    src = os.path.join(HERE, "cpp", "examples", "dropin_example.cpp")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "cpp"),
+           src, "-o", EXAMPLE, "-L", os.path.dirname(LIB), "-lbielcc_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for the C++ example:\n{r.stderr}")
+    return EXAMPLE
 
 
 if __name__ == "__main__":

@@ -15,6 +15,7 @@
 // (IntegralSurf_SimplePot_H_HS's analytic branch, integrals_analytic.h:151-153)
 // are skipped here and written by the exact per-pair kernel.
 #include "bielcc_internal.h"
+#include "phase.h"
 
 using namespace bielcc_dev;
 
@@ -26,7 +27,7 @@ constexpr int kASeg = 256;      // edge node slots in LDS
 constexpr int kACells = 8;
 constexpr int kARec = 24;       // center 3, diam, tau1 3, tau2 3, nu 12, pad 2
 
-template <bool CK, int WHICH>
+template <bool CK, int WHICH, bool FP>
 __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_per_split) {
     __shared__ double4 ssurf[kASurf];
     __shared__ double4 sseg[kASeg];
@@ -76,6 +77,8 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
             const int c = cs + cc;
             const double* rec = scell[cc];
             const double dcen = center_dist(x[0], x[1], x[2], rec);
+            PhaseRef pr{0., 0., 0., 0};
+            if (FP) pr = phase_ref(kr_ * dcen);
             cplx z[2][2];
             if (WHICH == 0) {
                 if (near_rule(dcen, rec[3], 0, a.ac, 0.)) continue;   // near pair: exact kernel writes it
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                     const double ri = rsqrt(r2);
                     const double r = r2 * ri;
                     double s, co;
-                    sincos(kr_ * r, &s, &co);
+                    if (FP) sincos_near(kr_, r, pr, s, co); else sincos(kr_ * r, &s, &co);
                     double w = y.w * ri;
                     if (CK) w *= exp(-ki_ * r);
                     Sr += w * co;
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                         const double ri = rsqrt(r2);
                         const double r = r2 * ri;
                         double s, co;
-                        sincos(kr_ * r, &s, &co);
+                        if (FP) sincos_near(kr_, r, pr, s, co); else sincos(kr_ * r, &s, &co);
                         double w = y.w * ri * ri * ri;
                         if (CK) w *= exp(-ki_ * r);
                         const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
@@ -124,6 +127,11 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                         k0b[b][1] += mk(gr1, gi1) * jn;
                         k0b[b][2] += mk(gr2, gi2) * jn;
                     }
+                }
+                if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
+                    rot_q(Sr, Si, pr.q);
+                    for (int b = 0; b < 2; b++)
+                        for (int g = 0; g < 3; g++) rot_q(k0b[b][g].re, k0b[b][g].im, pr.q);
                 }
                 const cplx Sk2 = (mk(Sr, Si) * k) * k;
                 for (int b = 0; b < 2; b++) {
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                         const double ri = rsqrt(r2);
                         const double r = r2 * ri;
                         double s, co;
-                        sincos(kr_ * r, &s, &co);
+                        if (FP) sincos_near(kr_, r, pr, s, co); else sincos(kr_ * r, &s, &co);
                         double w = y.w * ri * ri * ri;
                         if (CK) w *= exp(-ki_ * r);
                         const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
@@ -155,6 +163,8 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                         Gr[1] += gr * d1; Gi[1] += gi * d1;
                         Gr[2] += gr * d2; Gi[2] += gi * d2;
                     }
+                    if (FP)
+                        for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
                     const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
                     for (int b = 0; b < 2; b++) {
                         const double* tb = rec + 4 + 3 * b;
@@ -179,7 +189,12 @@ template <bool CK, int W>
 static void assemble_far_t(hipStream_t s, const AssembleArgs& a, int nsplit) {
     int cps = (a.N + nsplit - 1) / nsplit;
     dim3 g((a.ni + kAT - 1) / kAT, nsplit), b(kAT);
-    hipLaunchKernelGGL((k_assemble_far<CK, W>), g, b, 0, s, a, cps);
+    if (a.fast_phase) hipLaunchKernelGGL((k_assemble_far<CK, W, true>), g, b, 0, s, a, cps);
+    else hipLaunchKernelGGL((k_assemble_far<CK, W, false>), g, b, 0, s, a, cps);
+}
+
+bool fast_phase_ok(double k_re, double grid_step) {
+    return 0.7853981633974483 + fabs(k_re) * grid_step <= kPhaseRange;
 }
 
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a) {

@@ -61,6 +61,7 @@ struct FieldArgs {
     double ac, thresh;
     int nsplit;
     double* part;           // [nsplit][NACC][M]
+    int fast_phase;         // node phases within kPhaseRange of the cell-centre quadrant (phase.h)
 };
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a);
 hipError_t launch_field_near(hipStream_t s, int nv, long long npairs, const int* lq, const int* lc,
@@ -83,7 +84,11 @@ struct AssembleArgs {
     double ac;
     double* Z;              // column-major, complex, leading dimension ld
     long long ld;
+    int fast_phase;         // see FieldArgs::fast_phase
 };
+
+// pi/4 + |k| * (max cell diameter) bounds every reduced node phase (phase.h)
+bool fast_phase_ok(double k_re, double grid_step);
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a);
 hipError_t launch_assemble_generic(hipStream_t s, int nv, int which, long long npairs, const int* lq,
                                    const int* lc, int N, int i0, int ni, const double* verts, const double* ctab,

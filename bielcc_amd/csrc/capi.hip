@@ -26,6 +26,7 @@ struct bielcc_ctx_s {
     void* lists = nullptr;
     size_t lists_cap = 0;
     bool timing = false;
+    bool no_fast_phase = false;   // BIELCC_GENERAL_SINCOS=1: tile kernels use the general sincos
     std::map<std::string, std::pair<double, int>> times;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -186,6 +187,8 @@ int bielcc_ctx_create(int device, bielcc_ctx* out) {
         return BIELCC_ERR_HIP;
     }
     ctx->lu = lu_work_create();
+    const char* gs = std::getenv("BIELCC_GENERAL_SINCOS");
+    ctx->no_fast_phase = gs && gs[0] == '1';
     *out = ctx;
     return BIELCC_OK;
 }
@@ -422,6 +425,7 @@ int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_par
     a.nv = m->nv; a.N = m->n; a.nn = nn; a.ns = np->n_start_seg; a.which = which;
     a.i0 = i0; a.ni = ni; a.ctab = m->ctab; a.nodes = m->nodes; a.segs = m->segs;
     a.k_re = k_re; a.k_im = k_im; a.ac = np->analytic_calc; a.Z = Z_d; a.ld = ld;
+    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
     {
         Timer t(ctx, "assemble_far");
         HIPCHK(launch_assemble_far(ctx->stream, a));
@@ -543,6 +547,7 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
     a.jE = jE_d; a.jM = jM_d; a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
     a.thresh = thresh; a.nsplit = nsplit; a.part = part;
+    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
     {
         Timer t(ctx, "field_far");
         HIPCHK(launch_field_far(ctx->stream, a));

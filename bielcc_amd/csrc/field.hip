@@ -15,6 +15,7 @@
 // so that K_far[j] = (A j + T j)/4pi and R_colloc[j] = (G x j)/4pi hold for the
 // electric and the magnetic current alike.
 #include "bielcc_internal.h"
+#include "phase.h"
 
 using namespace bielcc_dev;
 
@@ -30,7 +31,7 @@ constexpr int kCellRec = 16;      // staged per-cell record: center 3, diam, jE 
 //  6-11 sum G x j_E                  (H from R[j_E], unscaled)
 //  12-17 sum (A j_M + T j_M)        (H from K[j_M], unscaled)   [with j_M]
 //  18-23 sum G x j_M                 (E from R[j_M], unscaled)   [with j_M]
-template <bool CK, bool JM, bool WH>
+template <bool CK, bool JM, bool WH, bool FP>
 __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_split) {
     __shared__ double4 snode[kFNodes];
     __shared__ double scell[kFCells][kCellRec];
@@ -70,6 +71,8 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
             const double* rec = scell[cc];
             const double dcen = center_dist(x0, x1, x2, rec);
             if (near_rule(dcen, rec[3], a.mode, a.ac, a.thresh)) continue;
+            PhaseRef pr{0., 0., 0., 0};
+            if (FP) pr = phase_ref(kr_ * dcen);
             // A = sum w e f1 ; T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh ; G = sum w (ikr-1) e d / r^3
             double Ar = 0., Ai = 0.;
             double Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
@@ -83,7 +86,8 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
                 const double r = r2 * ri;
                 const double ri2 = ri * ri, ri3 = ri2 * ri;
                 double s, c;
-                sincos(kr_ * r, &s, &c);
+                if (FP) sincos_near(kr_, r, pr, s, c);     // e^{i(kr - n pi/2)}
+                else sincos(kr_ * r, &s, &c);
                 double w = y.w;
                 if (CK) w *= exp(-ki_ * r);
                 const double wc = w * c, ws = w * s;
@@ -116,6 +120,11 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
                     Gr[1] += gr * d1; Gi[1] += gi * d1;
                     Gr[2] += gr * d2; Gi[2] += gi * d2;
                 }
+            }
+            if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
+                rot_q(Ar, Ai, pr.q);
+                for (int t = 0; t < 6; t++) rot_q(Tr[t], Ti[t], pr.q);
+                for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
             }
             // apply to the cell's currents
             const cplx A = mk(Ar, Ai);
@@ -162,20 +171,24 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
     }
 }
 
-template <bool CK, bool JM, bool WH>
+template <bool CK, bool JM, bool WH, bool FP>
 static void field_far_t(hipStream_t s, const FieldArgs& a) {
     int cps = (a.N + a.nsplit - 1) / a.nsplit;
     dim3 g((a.M + kFT - 1) / kFT, a.nsplit), b(kFT);
-    hipLaunchKernelGGL((k_field_far<CK, JM, WH>), g, b, 0, s, a, cps);
+    hipLaunchKernelGGL((k_field_far<CK, JM, WH, FP>), g, b, 0, s, a, cps);
+}
+
+template <bool CK, bool JM>
+static void field_far_fp(hipStream_t s, const FieldArgs& a) {
+    if (a.fast_phase) field_far_t<CK, JM, true, true>(s, a);
+    else field_far_t<CK, JM, true, false>(s, a);
 }
 
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a) {
-    bool ck = a.k_im != 0.;
-    bool jm = a.jM != nullptr;
-    bool wh = true;
-    if (ck) { if (jm) field_far_t<true, true, true>(s, a); else field_far_t<true, false, true>(s, a); }
-    else { if (jm) field_far_t<false, true, true>(s, a); else field_far_t<false, false, true>(s, a); }
-    (void)wh;
+    const bool ck = a.k_im != 0.;
+    const bool jm = a.jM != nullptr;
+    if (ck) { if (jm) field_far_fp<true, true>(s, a); else field_far_fp<true, false>(s, a); }
+    else { if (jm) field_far_fp<false, true>(s, a); else field_far_fp<false, false>(s, a); }
     return hipGetLastError();
 }
 

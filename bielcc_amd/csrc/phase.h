@@ -1,0 +1,78 @@
+// e^{i k r} for the quadrature nodes of one cell, seen from one point.
+//
+// The tile kernels evaluate cos/sin of phi = k r at every node.  All nodes of
+// a cell lie within diam of its centroid, so phi stays within k*diam of the
+// centroid phase phi_c.  Per (point, cell) the kernels pick n = rint(phi_c*2/pi)
+// once; per node they reduce t = phi - n*pi/2 with a three-piece Cody-Waite
+// split of pi/2 (each piece has at most 33 significant bits, so n*piece is
+// exact), and evaluate sin t and cos t by their Taylor series (coefficients
+// (-1)^i/(2i+1)!, (-1)^i/(2i)!, rounded to double) on |t| <= 2.  The factor
+// e^{i n pi/2} = i^n is the same for every node of the cell; since all
+// accumulated sums are linear in e^{ikr}, it is applied once per cell (rot_q).
+// Accuracy on |t| <= 2: |error| <= 3.5e-16 for sin and cos (checked against
+// long double).  The host uses this path only when pi/4 + |k| * max diameter
+// <= kPhaseRange, else the kernels call the general sincos.
+#pragma once
+
+#include "bielcc_dev.h"
+
+namespace bielcc_impl {
+
+constexpr double kPio2_1 = 0x1.921fb544p+0;          // pi/2, first 33 bits
+constexpr double kPio2_2 = 0x1.0b4611a6p-34;         // next 33 bits
+constexpr double kPio2_3 = 0x1.3198a2e037073p-69;    // remainder
+constexpr double kTwoOverPi = 0.63661977236758138;
+constexpr double kPhaseRange = 1.95;
+
+struct PhaseRef {
+    double o1, o2, o3;
+    int q;
+};
+
+__device__ inline PhaseRef phase_ref(double phi_c) {
+    const double n = rint(phi_c * kTwoOverPi);
+    return PhaseRef{n * kPio2_1, n * kPio2_2, n * kPio2_3, static_cast<int>(n) & 3};
+}
+
+// sin and cos of (kr * r - n*pi/2)
+__device__ inline void sincos_near(double kr, double r, const PhaseRef& p, double& s, double& c) {
+    double t = fma(kr, r, -p.o1);
+    t -= p.o2;
+    t -= p.o3;
+    const double t2 = t * t;
+    double ps = -3.8681701706306841e-23;
+    ps = fma(ps, t2, 1.9572941063391263e-20);
+    ps = fma(ps, t2, -8.2206352466243295e-18);
+    ps = fma(ps, t2, 2.8114572543455206e-15);
+    ps = fma(ps, t2, -7.6471637318198164e-13);
+    ps = fma(ps, t2, 1.6059043836821613e-10);
+    ps = fma(ps, t2, -2.505210838544172e-08);
+    ps = fma(ps, t2, 2.7557319223985893e-06);
+    ps = fma(ps, t2, -0.00019841269841269841);
+    ps = fma(ps, t2, 0.0083333333333333332);
+    ps = fma(ps, t2, -0.16666666666666666);
+    ps = ps * t2;
+    s = fma(t, ps, t);
+    double pc = -8.8967913924505741e-22;
+    pc = fma(pc, t2, 4.1103176233121648e-19);
+    pc = fma(pc, t2, -1.5619206968586225e-16);
+    pc = fma(pc, t2, 4.7794773323873853e-14);
+    pc = fma(pc, t2, -1.1470745597729725e-11);
+    pc = fma(pc, t2, 2.08767569878681e-09);
+    pc = fma(pc, t2, -2.7557319223985888e-07);
+    pc = fma(pc, t2, 2.4801587301587302e-05);
+    pc = fma(pc, t2, -0.0013888888888888889);
+    pc = fma(pc, t2, 0.041666666666666664);
+    pc = fma(pc, t2, -0.5);
+    c = fma(pc, t2, 1.0);
+}
+
+// (re, im) * i^q
+__device__ inline void rot_q(double& re, double& im, int q) {
+    double a = re, b = im;
+    if (q == 1) { re = -b; im = a; }
+    else if (q == 2) { re = -a; im = -b; }
+    else if (q == 3) { re = b; im = -a; }
+}
+
+}  // namespace bielcc_impl

@@ -63,6 +63,33 @@ def test_triangles_complex_k_vs_oracle(ctx, orc):
     m.close()
 
 
+def test_reduced_phase_and_general_sincos_paths(ctx, orc, monkeypatch):
+    """The tile kernels' reduced-range e^{ikr} (one quadrant per cell) and the
+    general sincos give the oracle's values to the same bar."""
+    verts = B.mesh.sphere_quads(16, 32, jitter=0.005, seed=1)
+    n = verts.shape[0]
+    rng = np.random.default_rng(24)
+    jE = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = B.mesh.sphere_points(40, 3.0, seed=5)
+    k = 2.0
+    npp = B.num_param()
+    Eo, Ho = orc.field_eh(verts, jE, k, O.num_param(), pts)
+    Zo = orc.assemble(verts[:60], O.num_param(), k)
+    monkeypatch.setenv("BIELCC_GENERAL_SINCOS", "1")
+    cg = B.Context(0)
+    for c in (ctx, cg):
+        m = c.mesh(verts)
+        E, H = m.field(npp, k, jE, pts)
+        assert rowwise_rel(E, Eo).max() <= TOL
+        assert rowwise_rel(H, Ho).max() <= TOL
+        m2 = c.mesh(verts[:60])
+        Z = m2.assemble(npp, k)
+        from _util import block_metric
+        assert block_metric(Z, Zo).max() <= 1e-12
+        m.close(); m2.close()
+    cg.close()
+
+
 def test_field_linearity_in_currents(ctx):
     """E and H are linear in (jE, jM): a size-independent property."""
     rng = np.random.default_rng(23)
