@@ -26,8 +26,9 @@ constexpr int kPivBlocks = 256;
 
 struct LuWork {
     rocblas_handle blas = nullptr;
-    double* pval = nullptr;     // [kPivBlocks] partial maxima
-    int* pidx = nullptr;        // [kPivBlocks] partial argmax
+    double* pval = nullptr;     // [pcap] partial maxima
+    int* pidx = nullptr;        // [pcap] partial argmax
+    int pcap = 0;
     double* pinv = nullptr;     // reciprocal pivot (re, im) + flag
     int* info = nullptr;        // first zero pivot, 1-based (0 if none)
     int* perm = nullptr;        // composed permutation for the solve
@@ -44,11 +45,20 @@ void lu_work_destroy(LuWork* w) {
     delete w;
 }
 
-static hipError_t lu_work_init(LuWork* w, const char** msg) {
-    if (!w->pval) {
-        hipError_t e;
-        if ((e = hipMalloc(&w->pval, kPivBlocks * sizeof(double))) != hipSuccess) return e;
-        if ((e = hipMalloc(&w->pidx, kPivBlocks * sizeof(int))) != hipSuccess) return e;
+static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
+    hipError_t e;
+    const int need = std::max(kPivBlocks, (n + 255) / 256 + 1);
+    if (w->pcap < need) {
+        (void)hipFree(w->pval);
+        (void)hipFree(w->pidx);
+        w->pval = nullptr;
+        w->pidx = nullptr;
+        w->pcap = 0;
+        if ((e = hipMalloc(&w->pval, need * sizeof(double))) != hipSuccess) return e;
+        if ((e = hipMalloc(&w->pidx, need * sizeof(int))) != hipSuccess) return e;
+        w->pcap = need;
+    }
+    if (!w->pinv) {
         if ((e = hipMalloc(&w->pinv, 4 * sizeof(double))) != hipSuccess) return e;
         if ((e = hipMalloc(&w->info, sizeof(int))) != hipSuccess) return e;
     }
@@ -148,29 +158,55 @@ __global__ __launch_bounds__(256) void k_piv_final(int nparts, int j, int c0, in
 }
 
 // Scale column j below the diagonal and apply the rank-1 update to panel
-// columns j+1 .. c1-1 (one lane per row; row j staged in LDS).
+// columns j+1 .. c1-1 (one lane per row; row j staged in LDS).  When column
+// j+1 is in the panel, each workgroup also reduces the updated |re|+|im| of
+// its rows of column j+1 to a partial argmax, so the next column's pivot
+// search needs only the final reduction.
 __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, double2* __restrict__ A, long long lda,
-                                                      const double* __restrict__ pinv) {
+                                                      const double* __restrict__ pinv, double* __restrict__ pval,
+                                                      int* __restrict__ pidx) {
     __shared__ double2 urow[kNB];
+    __shared__ double sv[256];
+    __shared__ int si[256];
     const int nc = c1 - j - 1;
     for (int t = threadIdx.x; t < nc; t += 256) urow[t] = A[(size_t)(j + 1 + t) * lda + j];
     __syncthreads();
     const int i = j + 1 + blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    double2 l = A[(size_t)j * lda + i];
-    if (pinv[2] != 0.) {
-        double2 s = make_double2(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
-        l = s;
-        A[(size_t)j * lda + i] = l;
+    double best = -1.;
+    int bi = 0x7fffffff;
+    if (i < n) {
+        double2 l = A[(size_t)j * lda + i];
+        if (pinv[2] != 0.) {
+            double2 s = make_double2(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
+            l = s;
+            A[(size_t)j * lda + i] = l;
+        }
+        for (int t = 0; t < nc; t++) {
+            double2* p = A + (size_t)(j + 1 + t) * lda + i;
+            double2 u = urow[t];
+            double2 a = *p;
+            a.x -= l.x * u.x - l.y * u.y;
+            a.y -= l.x * u.y + l.y * u.x;
+            *p = a;
+            if (t == 0) { best = cabs1(a); bi = i; }
+        }
     }
-    for (int t = 0; t < nc; t++) {
-        double2* p = A + (size_t)(j + 1 + t) * lda + i;
-        double2 u = urow[t];
-        double2 a = *p;
-        a.x -= l.x * u.x - l.y * u.y;
-        a.y -= l.x * u.y + l.y * u.x;
-        *p = a;
+    if (nc <= 0) return;   // uniform: no next column inside the panel
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double v = sv[threadIdx.x + off];
+            int ix = si[threadIdx.x + off];
+            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && ix < si[threadIdx.x])) {
+                sv[threadIdx.x] = v;
+                si[threadIdx.x] = ix;
+            }
+        }
+        __syncthreads();
     }
+    if (threadIdx.x == 0) { pval[blockIdx.x] = sv[0]; pidx[blockIdx.x] = si[0]; }
 }
 
 // Apply the interchanges piv[k0 .. k0+nb) to columns [cbeg, cend).
@@ -191,49 +227,71 @@ __global__ __launch_bounds__(256) void k_laswp(int k0, int nb, int cbeg, int cen
 
 // X = T^{-1} X for the nb x nb triangle T at (t0, t0) of A (lower unit when
 // lower != 0, else upper non-unit), X = columns [cbeg, cend) of B rows t0..t0+nb.
-// One workgroup per 64 columns; T and the X tile live in LDS.
-__global__ __launch_bounds__(64) void k_trsm(int nb, int lower, const double2* __restrict__ A, long long lda,
-                                             long long t0, double2* __restrict__ B, long long ldb, long long brow,
-                                             int cbeg, int cend) {
+// One workgroup of 256 lanes per 64 columns; T and the X tile live in LDS.
+// Column sweep: lane (r = tid % 64, column group tid / 64) owns row r of
+// every fourth column; step q publishes x_q and every lane updates its rows.
+__global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const double2* __restrict__ A, long long lda,
+                                              long long t0, double2* __restrict__ B, long long ldb, long long brow,
+                                              int cbeg, int cend) {
     __shared__ double2 T[kNB][kNB + 1];
     __shared__ double2 X[kNB][kNB + 1];
+    __shared__ double2 dinv[kNB];
     const int tid = threadIdx.x;
-    for (int q = 0; q < nb; q++)
-        for (int r = tid; r < nb; r += 64) T[r][q] = A[(size_t)(t0 + q) * lda + t0 + r];
+    const int r = tid & 63, cg = tid >> 6;
+    for (int q = cg; q < nb; q += 4)
+        if (r < nb) T[r][q] = A[(size_t)(t0 + q) * lda + t0 + r];
     const int c0 = cbeg + blockIdx.x * 64;
     const int ncol = min(64, cend - c0);
-    for (int cc = 0; cc < ncol; cc++)
-        for (int r = tid; r < nb; r += 64) X[r][cc] = B[(size_t)(c0 + cc) * ldb + brow + r];
+    for (int cc = cg; cc < ncol; cc += 4)
+        if (r < nb) X[r][cc] = B[(size_t)(c0 + cc) * ldb + brow + r];
     __syncthreads();
-    if (tid < ncol) {
-        if (lower) {
-            for (int r = 0; r < nb; r++) {
-                double2 acc = X[r][tid];
-                for (int q = 0; q < r; q++) {
-                    double2 l = T[r][q], x = X[q][tid];
-                    acc.x -= l.x * x.x - l.y * x.y;
-                    acc.y -= l.x * x.y + l.y * x.x;
-                }
-                X[r][tid] = acc;
-            }
-        } else {
-            for (int r = nb - 1; r >= 0; r--) {
-                double2 acc = X[r][tid];
-                for (int q = r + 1; q < nb; q++) {
-                    double2 u = T[r][q], x = X[q][tid];
-                    acc.x -= u.x * x.x - u.y * x.y;
-                    acc.y -= u.x * x.y + u.y * x.x;
-                }
-                double2 d = T[r][r];
-                double den = d.x * d.x + d.y * d.y;
-                double ir = d.x / den, ii = -d.y / den;
-                X[r][tid] = make_double2(acc.x * ir - acc.y * ii, acc.x * ii + acc.y * ir);
-            }
-        }
+    if (!lower && tid < nb) {
+        double2 d = T[tid][tid];
+        double den = d.x * d.x + d.y * d.y;
+        dinv[tid] = make_double2(d.x / den, -d.y / den);
     }
     __syncthreads();
-    for (int cc = 0; cc < ncol; cc++)
-        for (int r = tid; r < nb; r += 64) B[(size_t)(c0 + cc) * ldb + brow + r] = X[r][cc];
+    if (lower) {
+        for (int q = 0; q < nb - 1; q++) {
+            if (r > q && r < nb) {
+                const double2 l = T[r][q];
+                for (int cc = cg; cc < ncol; cc += 4) {
+                    const double2 x = X[q][cc];
+                    double2 a = X[r][cc];
+                    a.x -= l.x * x.x - l.y * x.y;
+                    a.y -= l.x * x.y + l.y * x.x;
+                    X[r][cc] = a;
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        // X[q] is final once step q begins; rows < q are updated with x_q = X[q] / T[q][q]
+        for (int q = nb - 1; q > 0; q--) {
+            if (r < q) {
+                const double2 u = T[r][q], di = dinv[q];
+                for (int cc = cg; cc < ncol; cc += 4) {
+                    const double2 b = X[q][cc];
+                    const double2 x = make_double2(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
+                    double2 a = X[r][cc];
+                    a.x -= u.x * x.x - u.y * x.y;
+                    a.y -= u.x * x.y + u.y * x.x;
+                    X[r][cc] = a;
+                }
+            }
+            __syncthreads();
+        }
+        if (r < nb) {
+            const double2 di = dinv[r];
+            for (int cc = cg; cc < ncol; cc += 4) {
+                const double2 b = X[r][cc];
+                X[r][cc] = make_double2(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
+            }
+        }
+        __syncthreads();
+    }
+    for (int cc = cg; cc < ncol; cc += 4)
+        if (r < nb) B[(size_t)(c0 + cc) * ldb + brow + r] = X[r][cc];
 }
 
 static hipError_t zgemm_update(LuWork* w, hipStream_t s, long long m, long long n, long long k, const double2* Aa,
@@ -255,29 +313,34 @@ static hipError_t zgemm_update(LuWork* w, hipStream_t s, long long m, long long 
 
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda, int* piv, int* info,
                      const char** msg) {
-    hipError_t e = lu_work_init(w, msg);
+    hipError_t e = lu_work_init(w, msg, n);
     if (e != hipSuccess) return e;
     double2* A = reinterpret_cast<double2*>(Ad);
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
     for (int k0 = 0; k0 < n; k0 += kNB) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
+        int nparts = 0;
         for (int jj = 0; jj < nb; jj++) {
             const int j = k0 + jj;
             const int m = n - j;
-            const int nparts = min(kPivBlocks, (m + 255) / 256);
-            hipLaunchKernelGGL(k_piv_partial, dim3(nparts), dim3(256), 0, s, n, j, A, lda, w->pval, w->pidx);
+            if (jj == 0) {   // first panel column: fresh search (later ones come from k_panel_update)
+                nparts = min(kPivBlocks, (m + 255) / 256);
+                hipLaunchKernelGGL(k_piv_partial, dim3(nparts), dim3(256), 0, s, n, j, A, lda, w->pval, w->pidx);
+            }
             hipLaunchKernelGGL(k_piv_final, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval, w->pidx,
                                piv, w->pinv, w->info);
-            if (m > 1)
-                hipLaunchKernelGGL(k_panel_update, dim3((m - 1 + 255) / 256), dim3(256), 0, s, n, j, c1, A, lda,
-                                   w->pinv);
+            if (m > 1) {
+                nparts = (m - 1 + 255) / 256;
+                hipLaunchKernelGGL(k_panel_update, dim3(nparts), dim3(256), 0, s, n, j, c1, A, lda, w->pinv,
+                                   w->pval, w->pidx);
+            }
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (k0 > 0) hipLaunchKernelGGL(k_laswp, dim3((k0 + 255) / 256), dim3(256), 0, s, k0, nb, 0, k0, A, lda, piv);
         if (c1 < n) {
             hipLaunchKernelGGL(k_laswp, dim3((n - c1 + 255) / 256), dim3(256), 0, s, k0, nb, c1, n, A, lda, piv);
-            hipLaunchKernelGGL(k_trsm, dim3((n - c1 + 63) / 64), dim3(64), 0, s, nb, 1, A, lda, (long long)k0, A,
+            hipLaunchKernelGGL(k_trsm, dim3((n - c1 + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, A,
                                lda, (long long)k0, c1, n);
             e = zgemm_update(w, s, n - c1, n - c1, nb, A + (size_t)k0 * lda + c1, lda, A + (size_t)c1 * lda + k0,
                              lda, A + (size_t)c1 * lda + c1, lda, msg);
@@ -338,7 +401,7 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long
     // forward: L y = P b
     for (int k0 = 0; k0 < n; k0 += kNB) {
         const int nb = min(kNB, n - k0);
-        hipLaunchKernelGGL(k_trsm, dim3((nrhs + 63) / 64), dim3(64), 0, s, nb, 1, A, lda, (long long)k0, B, ldb,
+        hipLaunchKernelGGL(k_trsm, dim3((nrhs + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, B, ldb,
                            (long long)k0, 0, nrhs);
         if (k0 + nb < n) {
             e = zgemm_update(w, s, n - k0 - nb, nrhs, nb, A + (size_t)k0 * lda + k0 + nb, lda, B + k0, ldb,
@@ -350,7 +413,7 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long
     int last = ((n - 1) / kNB) * kNB;
     for (int k0 = last; k0 >= 0; k0 -= kNB) {
         const int nb = min(kNB, n - k0);
-        hipLaunchKernelGGL(k_trsm, dim3((nrhs + 63) / 64), dim3(64), 0, s, nb, 0, A, lda, (long long)k0, B, ldb,
+        hipLaunchKernelGGL(k_trsm, dim3((nrhs + 63) / 64), dim3(256), 0, s, nb, 0, A, lda, (long long)k0, B, ldb,
                            (long long)k0, 0, nrhs);
         if (k0 > 0) {
             e = zgemm_update(w, s, k0, nrhs, nb, A + (size_t)k0 * lda, lda, B + k0, ldb, B, ldb, msg);
