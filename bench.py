@@ -14,6 +14,12 @@ the timed steps.  Also reported: the LU factorisation and solve times, the
 roofline of the dominant kernel (field tile kernel) and a CPU baseline of the
 reference compiled from its own sources (or the oracle when that library is
 absent), timed on a bounded sample of the same workload.
+
+The mixed-precision form (FP32 LU on the matrix cores, FP64 iterative
+refinement to a 1e-10 residual, FP32 far-pair node sums in the field) is timed
+once on the same workload after the FP64 steps: its assembly, factorisation,
+refinement and field times, one cold solve end to end against the FP64 form,
+and its error against the FP64 results (currents and field).
 """
 from __future__ import annotations
 
@@ -103,6 +109,68 @@ def cpu_baseline(verts, pts, k, budget_s=20.0):
             "field_pairs_per_s": m_f * n / tf, "assembly_pairs_per_s": m_a * n / ta}
 
 
+def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve64_s, step64_s):
+    """One cold solve in the mixed-precision form on the workload: FP64 matrix,
+    FP32 LU on the matrix cores, FP64 refinement to a 1e-10 residual, currents,
+    field with FP32 far-pair sums.  Errors are against the FP64 form's currents
+    and field (which match the reference to 1e-12 / 1e-10).  FP32 far sums in
+    the matrix itself are left out: they shift the currents by cond(Z) x 1e-7
+    (4e-5 on this workload) and do not shorten the assembly, whose K pairs are
+    bounded by the FP64 edge contours."""
+    import bielcc_amd as B
+    n = m.n
+    n2 = 2 * n
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    dZ = ctx.empty(n2 * n2 * 16)
+    dF = ctx.empty(n2 * n2 * 8)
+    dpiv = ctx.empty(n2 * 4)
+    dB = ctx.empty(n2 * 16)
+    dJ = ctx.empty(n * 48)
+    dE, dH = ctx.empty(M * 48), ctx.empty(M * 48)
+    t_asm = timed(lambda: m.assemble_into(dZ, n2, npp, k))
+    t_lu = timed(lambda: ctx.lu32_factor(dZ, n2, n2, dF, n2, dpiv))
+    m.rhs_planewave_into(dB, n2, kvec, e0)
+    res = {}
+    t_ref = timed(lambda: res.update(zip(("iters", "resid"), ctx.refine_solve(dZ, n2, n2, dF, n2, dpiv, dB, 1, n2,
+                                                                               1e-10, 30))))
+    m.currents_into(dB, n2, 1, dJ)
+    t_fld = timed(lambda: m.field_batch_into(npp, k, 1, dJ, dP, M, dE, dH, precision=B.PREC_MIXED))
+    log(f"[bench] mixed: assembly {t_asm:.3f} s, LU32 {t_lu:.2f} s, refine {t_ref:.3f} s "
+        f"({res['iters']} corrections, residual {res['resid']:.1e}), field {t_fld:.3f} s")
+    j64, j32 = dJ64.to_numpy((n, 3), np.complex128), dJ.to_numpy((n, 3), np.complex128)
+    E64, E32 = dE64.to_numpy((M, 3), np.complex128), dE.to_numpy((M, 3), np.complex128)
+    H64, H32 = dH64.to_numpy((M, 3), np.complex128), dH.to_numpy((M, 3), np.complex128)
+    # the FP32 far sums alone: mixed field of the FP64 currents
+    m.field_batch_into(npp, k, 1, dJ64, dP, M, dE, dH, precision=B.PREC_MIXED)
+    ctx.sync()
+    Eq, Hq = dE.to_numpy((M, 3), np.complex128), dH.to_numpy((M, 3), np.complex128)
+    rowrel = lambda a, b: float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
+    glob = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    for d in (dZ, dF, dpiv, dB, dJ, dE, dH):
+        d.free()
+    cold64 = step64_s + lu64_s + solve64_s   # FP64 assembly + field (one timed step) + LU + solve
+    cold32 = t_asm + t_lu + t_ref + t_fld
+    return {"form": "FP64 matrix, FP32 LU (rocBLAS cgemm on MFMA) + FP64 refinement to 1e-10, FP32 far-pair "
+                    "field sums (near pairs, contours, cross-cell sums FP64)",
+            "assembly_s": round(t_asm, 4), "lu32_factor_s": round(t_lu, 3),
+            "lu32_tflops": round(8.0 / 3.0 * n2 ** 3 / t_lu / 1e12, 2),
+            "refine_s": round(t_ref, 4), "refine_corrections": res["iters"], "refine_residual": res["resid"],
+            "field_s": round(t_fld, 4), "field_pairs_per_s": M * n / t_fld,
+            "cold_solve_s": round(cold32, 3), "cold_solve_s_fp64": round(cold64, 3),
+            "cold_solves_per_s_ratio": round(cold64 / cold32, 3),
+            "err_currents_rel": glob(j32, j64),
+            "err_field_E_max_rel": rowrel(E32, E64), "err_field_H_max_rel": rowrel(H32, H64),
+            "err_field_E_rel": glob(E32, E64), "err_field_H_rel": glob(H32, H64),
+            "err_field_E_max_rel_fp64_currents": rowrel(Eq, E64), "err_field_H_max_rel_fp64_currents": rowrel(Hq, H64)}
+
+
 def copy_calibration(nbytes=1 << 30):
     """A plain device copy of known size (torch), so that rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE can be scaled against it in the same run."""
@@ -125,6 +193,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-lu", action="store_true", help="skip the LU (currents from the right-hand side)")
     ap.add_argument("--copy-calib", action="store_true", help="run a 1 GiB device copy first (counter scale)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path is run on one MI355X; multi-GPU sharding is a design "
@@ -218,6 +287,9 @@ def main():
                         "lu_tflops": round(8.0 / 3.0 * n2 ** 3 / lu_s / 1e12, 2),
                         "solves_per_s_after_factor": round(1.0 / solve_s, 2)}
     out["timing_ms"] = {"field_far": fld_ms, "assemble_far": asm_ms, "near": near_ms}
+    if lu_s is not None and not args.no_mixed:
+        out["precision_form"] = mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ, dE, dH, lu_s, solve_s,
+                                           elapsed / args.steps)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(verts, pts, k, args.cpu_budget)

@@ -6,8 +6,9 @@ The HIP library is loaded lazily by `Context()`; importing the package does
 not touch the GPU.
 """
 from .api import (BielccError, Context, DeviceArray, Mesh, NumParam, KernelParam, IntegralParam, num_param,
-                  kernel_param, OPS, KERNELS, DEFAULT_NUM_PARAM)
+                  kernel_param, OPS, KERNELS, DEFAULT_NUM_PARAM, PREC_FP64, PREC_MIXED)
 from . import mesh
 
 __all__ = ["BielccError", "Context", "DeviceArray", "Mesh", "NumParam", "KernelParam", "IntegralParam",
-           "num_param", "kernel_param", "OPS", "KERNELS", "DEFAULT_NUM_PARAM", "mesh"]
+           "num_param", "kernel_param", "OPS", "KERNELS", "DEFAULT_NUM_PARAM", "PREC_FP64", "PREC_MIXED",
+           "mesh"]

@@ -75,6 +75,17 @@ _SIGS = {
     "bielcc_get_field_e": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, _DP, _DP,
                                      C.POINTER(NumParam), C.c_double, C.c_double, C.c_int, C.c_void_p,
                                      C.c_void_p]),
+    "bielcc_field_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _DP,
+                                     C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bielcc_assemble_ex": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64]),
+    "bielcc_lu32_factor": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                     C.c_void_p, _IP]),
+    "bielcc_refine_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                      C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, _IP, _DP]),
+    "bielcc_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "bielcc_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, _IP]),
     "bielcc_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bielcc_timing_reset": (C.c_int, [C.c_void_p]),
     "bielcc_timing": (C.c_int, [C.c_void_p, C.c_char_p, _DP, _IP]),

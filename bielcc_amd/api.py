@@ -15,6 +15,10 @@ from ._lib import IntegralParam, KernelParam, NumParam
 __all__ = ["BielccError", "Context", "DeviceArray", "Mesh", "NumParam", "KernelParam", "IntegralParam",
            "num_param", "kernel_param", "OPS", "KERNELS", "DEFAULT_NUM_PARAM"]
 
+# far-pair quadrature precision (include/bielcc_hip.h BIELCC_PREC_*)
+PREC_FP64 = 0
+PREC_MIXED = 1
+
 OPS = {"K_RotRot_Near_Smooth": 0, "K_RotRot_HS": 1, "K_RotRot_HS_Colloc": 2, "K_RotRot_HS2_Colloc": 3,
        "K_RotRot_Far": 4, "R_Rot_Smooth": 5, "R_Rot_Colloc": 6, "R_Rot_Near": 7}
 KERNELS = {"F_SimplePot_L": 0, "F_GradSimplePot_L": 1, "F_DoublePot_L": 2, "F_VectorPot_L": 3,
@@ -238,6 +242,29 @@ class Context:
         self._check(self.lib.bielcc_lu_solve(self.h, int(n), _p(A), int(lda), _p(piv), int(nrhs), _p(B),
                                              int(ldb)))
 
+    def lu32_factor(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray) -> None:
+        """FP32 factors of the FP64 matrix A into F (complex64, n x n, ldf); A is kept."""
+        info = C.c_int(0)
+        self._check(self.lib.bielcc_lu32_factor(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv),
+                                                C.byref(info)))
+
+    def refine_solve(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
+                     B: DeviceArray, nrhs: int, ldb: int, tol=1e-10, max_iter=30):
+        """FP64 solution of A X = B by refinement on FP32 factors; returns (corrections, residual)."""
+        it, res = C.c_int(0), C.c_double(0.)
+        self._check(self.lib.bielcc_refine_solve(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv),
+                                                 int(nrhs), _p(B), int(ldb), float(tol), int(max_iter),
+                                                 C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    def set_tuning(self, key: str, value: int):
+        self._check(self.lib.bielcc_set_tuning(self.h, key.encode(), int(value)))
+
+    def get_tuning(self, key: str) -> int:
+        v = C.c_int(0)
+        self._check(self.lib.bielcc_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
+
     # -- timing
     def timing_enable(self, on=True):
         self._check(self.lib.bielcc_timing_enable(self.h, int(bool(on))))
@@ -283,11 +310,17 @@ class Mesh:
         return g.value
 
     # device-buffer drivers
-    def assemble_into(self, Z: DeviceArray, ld: int, np_: NumParam, k, which=0, row0=0, nrows=None):
+    def assemble_into(self, Z: DeviceArray, ld: int, np_: NumParam, k, which=0, row0=0, nrows=None,
+                      precision=PREC_FP64):
         nrows = 2 * self.n - row0 if nrows is None else nrows
         k = complex(k)
-        self.ctx._check(self.ctx.lib.bielcc_assemble(self.ctx.h, self.id, int(which), C.byref(np_), k.real,
-                                                     k.imag, int(row0), int(nrows), _p(Z), int(ld)))
+        if precision == PREC_FP64:
+            self.ctx._check(self.ctx.lib.bielcc_assemble(self.ctx.h, self.id, int(which), C.byref(np_), k.real,
+                                                         k.imag, int(row0), int(nrows), _p(Z), int(ld)))
+        else:
+            self.ctx._check(self.ctx.lib.bielcc_assemble_ex(self.ctx.h, self.id, int(which), C.byref(np_), k.real,
+                                                            k.imag, int(row0), int(nrows), int(precision), _p(Z),
+                                                            int(ld)))
 
     def rhs_planewave_into(self, B: DeviceArray, ldb: int, kvec, e0):
         kvec = np.ascontiguousarray(kvec, dtype=np.float64).reshape(-1, 3)
@@ -313,11 +346,24 @@ class Mesh:
                                                   _p(jM), multE.real, multE.imag, multH.real, multH.imag, kv, ev,
                                                   int(near_mode), int(M), _p(pts), _p(E), _p(H)))
 
+    def field_batch_into(self, np_: NumParam, k, nrhs: int, jE, pts, M, E, H=None, jM=None, multE=1.0,
+                         multH=1.0, inc=None, near_mode=0, precision=PREC_FP64):
+        """nrhs current sets (device [nrhs][N][3]) -> E, H (device [nrhs][M][3]); inc: host [nrhs][6] or None."""
+        k, multE, multH = complex(k), complex(multE), complex(multH)
+        incp = None
+        if inc is not None:
+            inc = np.ascontiguousarray(inc, dtype=np.float64).reshape(nrhs, 6)
+            incp = inc.ctypes.data_as(C.POINTER(C.c_double))
+        self.ctx._check(self.ctx.lib.bielcc_field_batch(self.ctx.h, self.id, C.byref(np_), k.real, k.imag,
+                                                        int(nrhs), _p(jE), _p(jM), multE.real, multE.imag,
+                                                        multH.real, multH.imag, incp, int(near_mode),
+                                                        int(precision), int(M), _p(pts), _p(E), _p(H)))
+
     # host convenience wrappers
-    def assemble(self, np_: NumParam, k, which=0, row0=0, nrows=None) -> np.ndarray:
+    def assemble(self, np_: NumParam, k, which=0, row0=0, nrows=None, precision=PREC_FP64) -> np.ndarray:
         nrows = 2 * self.n - row0 if nrows is None else nrows
         Z = self.ctx.empty(nrows * 2 * self.n * 16)
-        self.assemble_into(Z, nrows, np_, k, which, row0, nrows)
+        self.assemble_into(Z, nrows, np_, k, which, row0, nrows, precision)
         return Z.to_numpy((2 * self.n, nrows), np.complex128).T   # column-major -> (rows, cols)
 
     def rhs_planewave(self, kvec, e0) -> np.ndarray:
@@ -345,6 +391,26 @@ class Mesh:
         self.field_into(np_, k, dje, dp, M, dE, dH, djm, multE, multH, incident, near_mode)
         return dE.to_numpy((M, 3), np.complex128), dH.to_numpy((M, 3), np.complex128)
 
+    def field_batch(self, np_: NumParam, k, jE, pts, jM=None, multE=1.0, multH=1.0, incident=None, near_mode=0,
+                    precision=PREC_FP64):
+        """jE (and jM): [nrhs][N][3]; incident: None or (kvec, e0) with [3] or [nrhs][3] rows.
+        Returns E, H as [nrhs][M][3]."""
+        jE = np.asarray(jE, dtype=np.complex128).reshape(-1, self.n, 3)
+        nrhs = jE.shape[0]
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        M = pts.shape[0]
+        inc = None
+        if incident is not None:
+            kv = np.broadcast_to(np.asarray(incident[0], dtype=np.float64).reshape(-1, 3), (nrhs, 3))
+            ev = np.broadcast_to(np.asarray(incident[1], dtype=np.float64).reshape(-1, 3), (nrhs, 3))
+            inc = np.concatenate([kv, ev], axis=1)
+        dp, dje = self.ctx.to_device(pts), self.ctx.to_device(jE)
+        djm = self.ctx.to_device(np.asarray(jM, dtype=np.complex128).reshape(nrhs, self.n, 3)) \
+            if jM is not None else None
+        dE, dH = self.ctx.empty(nrhs * M * 48), self.ctx.empty(nrhs * M * 48)
+        self.field_batch_into(np_, k, nrhs, dje, dp, M, dE, dH, djm, multE, multH, inc, near_mode, precision)
+        return dE.to_numpy((nrhs, M, 3), np.complex128), dH.to_numpy((nrhs, M, 3), np.complex128)
+
     def get_field_e(self, jE, mult, kvec, e0, np_: NumParam, k, pts) -> np.ndarray:
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         M = pts.shape[0]
@@ -368,3 +434,15 @@ class Mesh:
         self.ctx.lu_factor(dA, n, n, piv)
         self.ctx.lu_solve(dA, n, n, piv, dB, B.shape[1], n)
         return dB.to_numpy((B.shape[1], n), np.complex128).T
+
+    def solve_mixed(self, Z_host, rhs_host, tol=1e-12, max_iter=30):
+        """FP32 LU + FP64 refinement on the device; returns (X, corrections, residual)."""
+        n = Z_host.shape[0]
+        B = np.asarray(rhs_host, dtype=np.complex128).reshape(n, -1, order="F")
+        dA = self.ctx.to_device(np.asfortranarray(Z_host).T.copy())
+        dB = self.ctx.to_device(np.asfortranarray(B).T.copy())
+        dF = self.ctx.empty(n * n * 8)
+        piv = self.ctx.empty(n * 4)
+        self.ctx.lu32_factor(dA, n, n, dF, n, piv)
+        it, res = self.ctx.refine_solve(dA, n, n, dF, n, piv, dB, B.shape[1], n, tol, max_iter)
+        return dB.to_numpy((B.shape[1], n), np.complex128).T, it, res

@@ -14,6 +14,12 @@
 // vectors b: K0_b = -sum_e (tau_j^b . nu_e) g_e.  Near pairs
 // (IntegralSurf_SimplePot_H_HS's analytic branch, integrals_analytic.h:151-153)
 // are skipped here and written by the exact per-pair kernel.
+//
+// Mixed precision (P32): the surface sums (S for K, G for R) run in FP32 in
+// cell-local coordinates as in the field kernel (node phase from
+// r - dcen = (|y_c|^2 - 2 x_c.y_c)/(r + dcen), centre phase and quadrant in
+// FP64); the edge contour of K0, whose four edge sums cancel to O(diam/r) of
+// each other, stays FP64.
 #include "bielcc_internal.h"
 #include "phase.h"
 
@@ -27,7 +33,57 @@ constexpr int kASeg = 256;      // edge node slots in LDS
 constexpr int kACells = 8;
 constexpr int kARec = 24;       // center 3, diam, tau1 3, tau2 3, nu 12, pad 2
 
+// FP32 surface sums of one (point, cell) pair in cell-local coordinates.
+// WHICH 0: S = sum w e^{ikr}/r; WHICH 1: G = sum w (ikr-1) e^{ikr} (x-y)/r^3.
+// Returned rotated by i^n and scaled by e^{-ki dcen} (FP64).
 template <bool CK, int WHICH, bool FP>
+__device__ inline void surf_sums32(const float4* nd, const float* ny, int nn, const double* x, const double* cen,
+                                   double dcen, double kr_, double ki_, double* out_re, double* out_im) {
+    const PhaseRef pr = phase_ref(kr_ * dcen);
+    const float t0 = (float)(fma(kr_, dcen, -pr.o1) - pr.o2 - pr.o3);
+    const float xf0 = (float)(x[0] - cen[0]), xf1 = (float)(x[1] - cen[1]), xf2 = (float)(x[2] - cen[2]);
+    const float m0 = -2.f * xf0, m1 = -2.f * xf1, m2 = -2.f * xf2;
+    const float dcf = (float)dcen, kf = (float)kr_, kif = (float)ki_;
+    float Sr = 0.f, Si = 0.f;
+    float Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
+    for (int q = 0; q < nn; q++) {
+        const float4 y = nd[q];
+        const float d0 = xf0 - y.x, d1 = xf1 - y.y, d2 = xf2 - y.z;
+        const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const float ri = rsqrtf(r2);
+        const float r = r2 * ri;
+        const float del = fmaf(m0, y.x, fmaf(m1, y.y, fmaf(m2, y.z, ny[q])));
+        const float dr = __fdividef(del, r + dcf);
+        const float t = fmaf(kf, dr, t0);
+        float s, c;
+        if (FP) sincos_near32(t, s, c);
+        else sincosf(t, &s, &c);
+        if (WHICH == 0) {
+            float w = y.w * ri;
+            if (CK) w *= __expf(-kif * dr);
+            Sr += w * c;
+            Si += w * s;
+        } else {
+            float w = y.w * ri * ri * ri;
+            if (CK) w *= __expf(-kif * dr);
+            const float mr = CK ? -(1.f + kif * r) : -1.f, mi = kf * r;
+            const float gr = (mr * c - mi * s) * w, gi = (mr * s + mi * c) * w;
+            Gr[0] += gr * d0; Gi[0] += gi * d0;
+            Gr[1] += gr * d1; Gi[1] += gi * d1;
+            Gr[2] += gr * d2; Gi[2] += gi * d2;
+        }
+    }
+    const double amp = CK ? exp(-ki_ * dcen) : 1.;
+    const int ncomp = WHICH == 0 ? 1 : 3;
+    for (int g = 0; g < ncomp; g++) {
+        double re = WHICH == 0 ? (double)Sr : (double)Gr[g], im = WHICH == 0 ? (double)Si : (double)Gi[g];
+        rot_q(re, im, pr.q);
+        out_re[g] = re * amp;
+        out_im[g] = im * amp;
+    }
+}
+
+template <bool CK, int WHICH, bool FP, bool P32>
 __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_per_split) {
     __shared__ double4 ssurf[kASurf];
     __shared__ double4 sseg[kASeg];
@@ -54,8 +110,16 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
     for (int cs = c_begin; cs < c_end; cs += cb) {
         const int ncs = min(cb, c_end - cs);
         __syncthreads();
-        const double4* gs = reinterpret_cast<const double4*>(a.nodes) + (size_t)cs * nn;
-        for (int idx = tid; idx < ncs * nn; idx += kAT) ssurf[idx] = gs[idx];
+        if (P32) {   // FP32 nodes (16 B) and |y_c|^2 (4 B) in the space of the FP64 table
+            float4* s32 = reinterpret_cast<float4*>(ssurf);
+            float* sy32 = reinterpret_cast<float*>(s32 + kASurf);
+            const float4* g4 = a.nodes32 + (size_t)cs * nn;
+            const float* gy = a.nodes32_yy + (size_t)cs * nn;
+            for (int idx = tid; idx < ncs * nn; idx += kAT) { s32[idx] = g4[idx]; sy32[idx] = gy[idx]; }
+        } else {
+            const double4* gs = reinterpret_cast<const double4*>(a.nodes) + (size_t)cs * nn;
+            for (int idx = tid; idx < ncs * nn; idx += kAT) ssurf[idx] = gs[idx];
+        }
         if (WHICH == 0) {
             const double4* gg = reinterpret_cast<const double4*>(a.segs) + (size_t)cs * nseg;
             for (int idx = tid; idx < ncs * nseg; idx += kAT) sseg[idx] = gg[idx];
@@ -85,7 +149,12 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                 // S = int e^{ikr}/r  (F_SimplePot_H, smoothR = DBL_EPSILON: factor 1/r)
                 double Sr = 0., Si = 0.;
                 const double4* nd = ssurf + cc * nn;
-                for (int q = 0; q < nn; q++) {
+                if (P32) {
+                    const float4* s32 = reinterpret_cast<const float4*>(ssurf);
+                    const float* sy32 = reinterpret_cast<const float*>(s32 + kASurf);
+                    surf_sums32<CK, 0, FP>(s32 + cc * nn, sy32 + cc * nn, nn, x, rec, dcen, kr_, ki_, &Sr, &Si);
+                }
+                for (int q = 0; q < (P32 ? 0 : nn); q++) {
                     const double4 y = nd[q];
                     const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
                     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -129,7 +198,7 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                     }
                 }
                 if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
-                    rot_q(Sr, Si, pr.q);
+                    if (!P32) rot_q(Sr, Si, pr.q);
                     for (int b = 0; b < 2; b++)
                         for (int g = 0; g < 3; g++) rot_q(k0b[b][g].re, k0b[b][g].im, pr.q);
                 }
@@ -146,8 +215,19 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                     z[0][0] = z[0][1] = z[1][0] = z[1][1] = mk(0., 0.);
                 } else {
                     double Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
-                    const double4* nd = ssurf + cc * nn;
-                    for (int q = 0; q < nn; q++) {
+                    // near pairs keep FP64 sums: for almost coplanar neighbours the
+                    // tau_i . (G x tau_j) projection cancels to O(diam/R) of |G|.
+                    // In mixed mode LDS holds the FP32 table, so they read the FP64
+                    // nodes from global memory (few pairs, L2-resident).
+                    const bool use32 = P32 && !near_rule(dcen, rec[3], 0, a.ac, 0.);
+                    const double4* nd = P32 ? reinterpret_cast<const double4*>(a.nodes) + (size_t)c * nn
+                                            : ssurf + cc * nn;
+                    if (use32) {
+                        const float4* s32 = reinterpret_cast<const float4*>(ssurf);
+                        const float* sy32 = reinterpret_cast<const float*>(s32 + kASurf);
+                        surf_sums32<CK, 1, FP>(s32 + cc * nn, sy32 + cc * nn, nn, x, rec, dcen, kr_, ki_, Gr, Gi);
+                    }
+                    for (int q = 0; q < (use32 ? 0 : nn); q++) {
                         const double4 y = nd[q];
                         const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
                         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -163,7 +243,7 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
                         Gr[1] += gr * d1; Gi[1] += gi * d1;
                         Gr[2] += gr * d2; Gi[2] += gi * d2;
                     }
-                    if (FP)
+                    if (FP && !use32)
                         for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
                     const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
                     for (int b = 0; b < 2; b++) {
@@ -185,12 +265,18 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
     }
 }
 
-template <bool CK, int W>
-static void assemble_far_t(hipStream_t s, const AssembleArgs& a, int nsplit) {
+template <bool CK, int W, bool P32>
+static void assemble_far_p(hipStream_t s, const AssembleArgs& a, int nsplit) {
     int cps = (a.N + nsplit - 1) / nsplit;
     dim3 g((a.ni + kAT - 1) / kAT, nsplit), b(kAT);
-    if (a.fast_phase) hipLaunchKernelGGL((k_assemble_far<CK, W, true>), g, b, 0, s, a, cps);
-    else hipLaunchKernelGGL((k_assemble_far<CK, W, false>), g, b, 0, s, a, cps);
+    if (a.fast_phase) hipLaunchKernelGGL((k_assemble_far<CK, W, true, P32>), g, b, 0, s, a, cps);
+    else hipLaunchKernelGGL((k_assemble_far<CK, W, false, P32>), g, b, 0, s, a, cps);
+}
+
+template <bool CK, int W>
+static void assemble_far_t(hipStream_t s, const AssembleArgs& a, int nsplit) {
+    if (a.prec) assemble_far_p<CK, W, true>(s, a, nsplit);
+    else assemble_far_p<CK, W, false>(s, a, nsplit);
 }
 
 bool fast_phase_ok(double k_re, double grid_step) {
@@ -198,9 +284,11 @@ bool fast_phase_ok(double k_re, double grid_step) {
 }
 
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a) {
+    if (a.prec && (!a.nodes32 || !a.nodes32_yy)) return hipErrorInvalidValue;
     // enough workgroups to fill 256 CUs several times over
     int bx = (a.ni + kAT - 1) / kAT;
-    int nsplit = (2048 + bx - 1) / bx;
+    const int wg = a.wg_target > 0 ? a.wg_target : 2048;
+    int nsplit = (wg + bx - 1) / bx;
     nsplit = max(1, min(nsplit, (a.N + 63) / 64));
     bool ck = a.k_im != 0.;
     if (a.which == 0) { if (ck) assemble_far_t<true, 0>(s, a, nsplit); else assemble_far_t<false, 0>(s, a, nsplit); }

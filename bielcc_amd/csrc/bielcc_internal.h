@@ -32,12 +32,18 @@ struct DevMesh {
     double* nodes = nullptr; // [n][nn][4]: sub-cell centroid (3) + weight
     double* segs = nullptr;  // [n][nv*ns][4]: edge node (3) + segment step length
     int nodes_n = 0, segs_n = 0;
+    // cell-local FP32 copies for the mixed-precision kernels (built on first use)
+    float4* nodes32 = nullptr;   // [n][nn]: node - centre (3), weight
+    float* nodes32_yy = nullptr; // [n][nn]: |node - centre|^2
+    int nodes32_n = 0;
 };
 
 // ---- tables (tables.hip)
 hipError_t launch_cell_tables(hipStream_t s, int nv, int n, const double* verts, double* ctab);
 hipError_t launch_node_tables(hipStream_t s, int nv, int n, const double* verts, int n_start, double* nodes);
 hipError_t launch_seg_tables(hipStream_t s, int nv, int n, const double* verts, int n_seg, double* segs);
+hipError_t launch_local32(hipStream_t s, int n, int cnt, const double* nodes, const double* ctab, float4* loc,
+                          float* yy);
 
 // ---- near lists (tables.hip)
 // mode 0: near iff dist(q, center_c) < ac * diam_c
@@ -54,23 +60,43 @@ struct FieldArgs {
     const double* pts;      // [M][3]
     const double* ctab;
     const double* nodes;
-    const double* jE;       // [N][3] complex
-    const double* jM;       // [N][3] complex or null
+    const float4* nodes32;  // cell-local FP32 nodes (prec = 1)
+    const float* nodes32_yy;
+    const double* jE;       // [nrhs][N][3] complex
+    const double* jM;       // [nrhs][N][3] complex or null
+    int nrhs;
+    int rchunk;             // current sets per pass: 1, 2 or 4 (1 with jM)
     double k_re, k_im;
     int mode;               // near rule (see launch_near_count)
     double ac, thresh;
     int nsplit;
-    double* part;           // [nsplit][NACC][M]
+    double* part;           // [nsplit][nrhs][NACC][M]
     int fast_phase;         // node phases within kPhaseRange of the cell-centre quadrant (phase.h)
+    int prec;               // 0: FP64 node sums; 1: FP32 cell-local node sums
 };
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a);
+// as_matrix = 0: per pair, the operators applied to jE / jM (one current set);
+// as_matrix = 1: per pair, the 3x3 operator matrices (any number of sets).
 hipError_t launch_field_near(hipStream_t s, int nv, long long npairs, const int* lq, const int* lc,
                              const double* pts, const double* verts, const double* jE, const double* jM,
-                             const bielcc_dev::NumP& np, double k_re, double k_im, int mode, double* out);
-hipError_t launch_field_finalize(hipStream_t s, int M, int nsplit, const double* part, int has_jm,
-                                 const long long* near_offs, const double* near_out, double mE_re, double mE_im,
-                                 double mH_re, double mH_im, int add_inc, const double* kvec_e0,
-                                 const double* pts, double* E, double* H);
+                             const bielcc_dev::NumP& np, double k_re, double k_im, int mode, int as_matrix,
+                             double* out);
+struct FinalizeArgs {
+    int M, N, nsplit, nrhs;
+    const double* part;
+    const double* jE;        // [nrhs][N][3] (read with near matrices)
+    const double* jM;        // null without magnetic currents
+    const long long* offs;   // near-pair offsets per point, or null
+    const int* lc;           // near-pair cell indices
+    const double* near_out;
+    int near_mat;            // near_out holds operator matrices
+    double mEr, mEi, mHr, mHi;
+    const double* inc;       // [nrhs][6] plane wave (kvec, e0) or null
+    const double* pts;
+    double* E;               // [nrhs][M][3] complex
+    double* H;               // same, or null
+};
+hipError_t launch_field_finalize(hipStream_t s, const FinalizeArgs& f);
 
 // ---- assembly (assemble.hip)
 struct AssembleArgs {
@@ -85,6 +111,10 @@ struct AssembleArgs {
     double* Z;              // column-major, complex, leading dimension ld
     long long ld;
     int fast_phase;         // see FieldArgs::fast_phase
+    int prec;               // 0: FP64; 1: FP32 surface sums (contours stay FP64)
+    const float4* nodes32;  // cell-local FP32 surface nodes (prec = 1)
+    const float* nodes32_yy;
+    int wg_target;          // workgroups the launch aims for (cells are split to reach it)
 };
 
 // pi/4 + |k| * (max cell diameter) bounds every reduced node phase (phase.h)
@@ -118,10 +148,20 @@ hipError_t launch_plane_wave(hipStream_t s, int n, const double* pts, const doub
 struct LuWork;
 LuWork* lu_work_create();
 void lu_work_destroy(LuWork*);
+// few-RHS solves: sync-free triangular kernels (on) or blocked TRSM + GEMM (off)
+void lu_set_trsv(LuWork*, bool on);
+bool lu_get_trsv(const LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);
 hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const int* piv, int nrhs,
                     double* B, long long ldb, const char** msg);
+// FP32 factors F of the FP64 matrix A (A untouched)
+hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* A, long long lda, float* F, long long ldf,
+                       int* piv, int* info, const char** msg);
+// FP64 iterative refinement on FP32 factors; status 1 = stagnated above tol
+hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const float* F,
+                        long long ldf, const int* piv, int nrhs, double* B, long long ldb, double tol, int max_iter,
+                        int* iters, double* resid, int* status, const char** msg);
 
 }  // namespace bielcc_impl

@@ -27,6 +27,9 @@ struct bielcc_ctx_s {
     size_t lists_cap = 0;
     bool timing = false;
     bool no_fast_phase = false;   // BIELCC_GENERAL_SINCOS=1: tile kernels use the general sincos
+    void* near_buf = nullptr;     // per-pair near results of the field driver
+    size_t near_cap = 0;
+    int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     std::map<std::string, std::pair<double, int>> times;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -127,6 +130,28 @@ int ensure_tables(bielcc_ctx ctx, DevMesh& m, int n_start, int n_seg) {
     return BIELCC_OK;
 }
 
+void free_mesh(DevMesh& m) {
+    (void)hipFree(m.verts); (void)hipFree(m.ctab); (void)hipFree(m.nodes); (void)hipFree(m.segs);
+    (void)hipFree(m.nodes32); (void)hipFree(m.nodes32_yy);
+}
+
+// cell-local FP32 surface nodes for the mixed-precision kernels
+int ensure_tables32(bielcc_ctx ctx, DevMesh& m, int n_start) {
+    if (int rc = ensure_tables(ctx, m, n_start, 0)) return rc;
+    if (m.nodes32_n == n_start) return BIELCC_OK;
+    (void)hipFree(m.nodes32);
+    (void)hipFree(m.nodes32_yy);
+    m.nodes32 = nullptr;
+    m.nodes32_yy = nullptr;
+    m.nodes32_n = 0;
+    const size_t cnt = (size_t)m.n * n_start * n_start;
+    HIPCHK(hipMalloc(&m.nodes32, cnt * sizeof(float4)));
+    HIPCHK(hipMalloc(&m.nodes32_yy, cnt * sizeof(float)));
+    HIPCHK(launch_local32(ctx->stream, m.n, n_start * n_start, m.nodes, m.ctab, m.nodes32, m.nodes32_yy));
+    m.nodes32_n = n_start;
+    return BIELCC_OK;
+}
+
 struct Timer {
     bielcc_ctx ctx;
     const char* name;
@@ -198,12 +223,10 @@ int bielcc_ctx_destroy(bielcc_ctx ctx) {
     {
         DeviceGuard g(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        for (auto& kv : ctx->meshes) {
-            (void)hipFree(kv.second.verts); (void)hipFree(kv.second.ctab);
-            (void)hipFree(kv.second.nodes); (void)hipFree(kv.second.segs);
-        }
+        for (auto& kv : ctx->meshes) free_mesh(kv.second);
         if (ctx->scratch) (void)hipFree(ctx->scratch);
         if (ctx->lists) (void)hipFree(ctx->lists);
+        if (ctx->near_buf) (void)hipFree(ctx->near_buf);
         lu_work_destroy(ctx->lu);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
@@ -289,7 +312,7 @@ int bielcc_mesh_destroy(bielcc_ctx ctx, int mesh_id) {
     if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
     DeviceGuard g(ctx->device);
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(m->verts); (void)hipFree(m->ctab); (void)hipFree(m->nodes); (void)hipFree(m->segs);
+    free_mesh(*m);
     ctx->meshes.erase(mesh_id);
     return BIELCC_OK;
 }
@@ -396,10 +419,12 @@ int bielcc_plane_wave_e(bielcc_ctx ctx, int n, const double* pts_d, const double
     return BIELCC_OK;
 }
 
-int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re, double k_im,
-                    int row0, int nrows, double* Z_d, int64_t ld) {
+static int assemble_impl(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re,
+                         double k_im, int row0, int nrows, int precision, double* Z_d, int64_t ld) {
     if (!ctx) return BIELCC_ERR_ARG;
     if (int rc = check_np(ctx, np)) return rc;
+    if (precision != BIELCC_PREC_FP64 && precision != BIELCC_PREC_MIXED)
+        return fail(ctx, BIELCC_ERR_ARG, "precision must be BIELCC_PREC_FP64 or BIELCC_PREC_MIXED");
     DevMesh* m = find_mesh(ctx, mesh_id);
     if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
     if (which != 0 && which != 1) return fail(ctx, BIELCC_ERR_ARG, "which must be 0 (K) or 1 (R)");
@@ -421,11 +446,17 @@ int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_par
         return BIELCC_OK;
     }
     if (int rc = ensure_tables(ctx, *m, np->n_start, which == 0 ? np->n_start_seg : 0)) return rc;
+    if (precision == BIELCC_PREC_MIXED)
+        if (int rc = ensure_tables32(ctx, *m, np->n_start)) return rc;
     AssembleArgs a;
     a.nv = m->nv; a.N = m->n; a.nn = nn; a.ns = np->n_start_seg; a.which = which;
     a.i0 = i0; a.ni = ni; a.ctab = m->ctab; a.nodes = m->nodes; a.segs = m->segs;
     a.k_re = k_re; a.k_im = k_im; a.ac = np->analytic_calc; a.Z = Z_d; a.ld = ld;
     a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.prec = precision;
+    a.nodes32 = m->nodes32;
+    a.nodes32_yy = m->nodes32_yy;
+    a.wg_target = ctx->assemble_wg;
     {
         Timer t(ctx, "assemble_far");
         HIPCHK(launch_assemble_far(ctx->stream, a));
@@ -453,6 +484,16 @@ int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_par
         }
     }
     return BIELCC_OK;
+}
+
+int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re, double k_im,
+                    int row0, int nrows, double* Z_d, int64_t ld) {
+    return assemble_impl(ctx, mesh_id, which, np, k_re, k_im, row0, nrows, BIELCC_PREC_FP64, Z_d, ld);
+}
+
+int bielcc_assemble_ex(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re,
+                       double k_im, int row0, int nrows, int precision, double* Z_d, int64_t ld) {
+    return assemble_impl(ctx, mesh_id, which, np, k_re, k_im, row0, nrows, precision, Z_d, ld);
 }
 
 int bielcc_rhs_planewave(bielcc_ctx ctx, int mesh_id, int nrhs, const double* kvec, const double* e0, double* B_d,
@@ -509,17 +550,19 @@ int bielcc_currents_by_basis(bielcc_ctx ctx, int mesh_id, int nrhs, const double
 }
 
 static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
-                      const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr, double mHi,
-                      const double* kvec, const double* e0, int near_mode, int M, const double* pts_d, double* E_d,
-                      double* H_d) {
+                      int nrhs, const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr,
+                      double mHi, const double* inc, int near_mode, int precision, int M, const double* pts_d,
+                      double* E_d, double* H_d) {
     if (!ctx) return BIELCC_ERR_ARG;
     if (int rc = check_np(ctx, np)) return rc;
     DevMesh* m = find_mesh(ctx, mesh_id);
     if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
     if (M < 0 || near_mode < 0 || near_mode > 1) return fail(ctx, BIELCC_ERR_ARG, "bad M or near_mode");
+    if (nrhs < 1) return fail(ctx, BIELCC_ERR_ARG, "nrhs must be >= 1");
+    if (precision != BIELCC_PREC_FP64 && precision != BIELCC_PREC_MIXED)
+        return fail(ctx, BIELCC_ERR_ARG, "precision must be BIELCC_PREC_FP64 or BIELCC_PREC_MIXED");
     if (M == 0) return BIELCC_OK;
     if (!jE_d || !pts_d || !E_d) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
-    if ((kvec == nullptr) != (e0 == nullptr)) return fail(ctx, BIELCC_ERR_ARG, "kvec and e0 go together");
     if (!std::isfinite(k_re) || !std::isfinite(k_im)) return fail(ctx, BIELCC_ERR_ARG, "non-finite wave number");
     const int nn = np->n_start * np->n_start;
     if (np->p_max != 1 || nn > 512 || !(np->analytic_calc > 1.))
@@ -527,27 +570,35 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
                     "field driver needs p_max = 1, n_start <= 22 and analytic_calc > 1 (use bielcc_op_batch)");
     DeviceGuard g(ctx->device);
     if (int rc = ensure_tables(ctx, *m, np->n_start, 0)) return rc;
+    if (precision == BIELCC_PREC_MIXED)
+        if (int rc = ensure_tables32(ctx, *m, np->n_start)) return rc;
     const int has_jm = jM_d != nullptr;
     const int nacc = has_jm ? 24 : 12;
     const double thresh = np->analytic_calc * m->grid_step;
     // enough workgroups to fill the chip: split the cells when the points alone do not
     const int bx = (M + 255) / 256;
-    int nsplit = std::max(1, std::min((2048 + bx - 1) / bx, (m->n + 63) / 64));
-    size_t part_b = align256((size_t)nsplit * nacc * M * sizeof(double));
+    int nsplit = std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64));
+    size_t part_b = align256((size_t)nsplit * nrhs * nacc * M * sizeof(double));
     size_t cnt_b = align256((size_t)M * sizeof(int));
     size_t off_b = align256((size_t)(M + 1) * sizeof(long long));
-    size_t need = part_b + cnt_b + off_b + 256;
+    size_t inc_b = align256((size_t)nrhs * 6 * sizeof(double));
+    size_t need = part_b + cnt_b + off_b + inc_b + 256;
     HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, need));
     char* base = reinterpret_cast<char*>(ctx->scratch);
     double* part = reinterpret_cast<double*>(base);
     int* counts = reinterpret_cast<int*>(base + part_b);
     long long* offs = reinterpret_cast<long long*>(base + part_b + cnt_b);
-    long long* tot_d = reinterpret_cast<long long*>(base + part_b + cnt_b + off_b);
+    double* inc_d = reinterpret_cast<double*>(base + part_b + cnt_b + off_b);
+    long long* tot_d = reinterpret_cast<long long*>(base + part_b + cnt_b + off_b + inc_b);
+    if (inc) HIPCHK(hipMemcpyAsync(inc_d, inc, (size_t)nrhs * 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     FieldArgs a;
     a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
-    a.jE = jE_d; a.jM = jM_d; a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
+    a.nodes32 = m->nodes32; a.nodes32_yy = m->nodes32_yy;
+    a.jE = jE_d; a.jM = jM_d; a.nrhs = nrhs; a.rchunk = ctx->field_rchunk;
+    a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
     a.thresh = thresh; a.nsplit = nsplit; a.part = part;
     a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.prec = precision;
     {
         Timer t(ctx, "field_far");
         HIPCHK(launch_field_far(ctx->stream, a));
@@ -558,26 +609,23 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     if (int rc = build_near_list(ctx, M, pts_d, 3, *m, near_mode, np->analytic_calc, thresh, counts, offs, tot_d,
                                  &tot_h, &lq, &lc))
         return rc;
+    // one current set: the operators applied to it per pair; several: per-pair 3x3 operator matrices
+    const int as_matrix = nrhs > 1;
     double* near_out = nullptr;
-    void* nbuf = nullptr;
     if (tot_h > 0) {
-        HIPCHK(hipMalloc(&nbuf, (size_t)tot_h * nacc * sizeof(double)));
-        near_out = reinterpret_cast<double*>(nbuf);
+        HIPCHK(ensure(ctx->near_buf, ctx->near_cap, (size_t)tot_h * (as_matrix ? 36 : nacc) * sizeof(double)));
+        near_out = reinterpret_cast<double*>(ctx->near_buf);
         Timer t(ctx, "near");
-        hipError_t e = launch_field_near(ctx->stream, m->nv, tot_h, lq, lc, pts_d, m->verts, jE_d, jM_d, to_np(np),
-                                         k_re, k_im, near_mode, near_out);
-        if (e != hipSuccess) { (void)hipFree(nbuf); return hip_fail(ctx, e, "launch_field_near"); }
+        HIPCHK(launch_field_near(ctx->stream, m->nv, tot_h, lq, lc, pts_d, m->verts, jE_d, jM_d, to_np(np), k_re,
+                                 k_im, near_mode, as_matrix, near_out));
         t.stop();
     }
-    double ke[6] = {0, 0, 0, 0, 0, 0};
-    if (kvec) for (int d = 0; d < 3; d++) { ke[d] = kvec[d]; ke[3 + d] = e0[d]; }
-    hipError_t e = launch_field_finalize(ctx->stream, M, nsplit, part, has_jm, offs, near_out, mEr, mEi, mHr, mHi,
-                                         kvec != nullptr, ke, pts_d, E_d, H_d);
-    if (nbuf) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipFree(nbuf);
-    }
-    if (e != hipSuccess) return hip_fail(ctx, e, "launch_field_finalize");
+    FinalizeArgs f;
+    f.M = M; f.N = m->n; f.nsplit = nsplit; f.nrhs = nrhs; f.part = part; f.jE = jE_d; f.jM = jM_d;
+    f.offs = offs; f.lc = lc; f.near_out = near_out; f.near_mat = as_matrix;
+    f.mEr = mEr; f.mEi = mEi; f.mHr = mHr; f.mHi = mHi;
+    f.inc = inc ? inc_d : nullptr; f.pts = pts_d; f.E = E_d; f.H = H_d;
+    HIPCHK(launch_field_finalize(ctx->stream, f));
     return BIELCC_OK;
 }
 
@@ -585,16 +633,99 @@ int bielcc_field(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double
                  const double* jE_d, const double* jM_d, double multE_re, double multE_im, double multH_re,
                  double multH_im, const double* kvec, const double* e0, int near_mode, int M, const double* pts_d,
                  double* E_d, double* H_d) {
-    return field_impl(ctx, mesh_id, np, k_re, k_im, jE_d, jM_d, multE_re, multE_im, multH_re, multH_im, kvec, e0,
-                      near_mode, M, pts_d, E_d, H_d);
+    if ((kvec == nullptr) != (e0 == nullptr)) return fail(ctx, BIELCC_ERR_ARG, "kvec and e0 go together");
+    double ke[6] = {0, 0, 0, 0, 0, 0};
+    if (kvec) for (int d = 0; d < 3; d++) { ke[d] = kvec[d]; ke[3 + d] = e0[d]; }
+    return field_impl(ctx, mesh_id, np, k_re, k_im, 1, jE_d, jM_d, multE_re, multE_im, multH_re, multH_im,
+                      kvec ? ke : nullptr, near_mode, BIELCC_PREC_FP64, M, pts_d, E_d, H_d);
 }
 
 int bielcc_get_field_e(bielcc_ctx ctx, int mesh_id, const double* jE_d, double mult_re, double mult_im,
                        const double* kvec, const double* e0, const bielcc_num_param* np, double k_re, double k_im,
                        int M, const double* pts_d, double* E_d) {
     if (!kvec || !e0) return fail(ctx, BIELCC_ERR_ARG, "GetFieldE_Ideal needs the incident plane wave");
-    return field_impl(ctx, mesh_id, np, k_re, k_im, jE_d, nullptr, mult_re, mult_im, 1., 0., kvec, e0, 0, M, pts_d,
-                      E_d, nullptr);
+    double ke[6] = {kvec[0], kvec[1], kvec[2], e0[0], e0[1], e0[2]};
+    return field_impl(ctx, mesh_id, np, k_re, k_im, 1, jE_d, nullptr, mult_re, mult_im, 1., 0., ke, 0,
+                      BIELCC_PREC_FP64, M, pts_d, E_d, nullptr);
+}
+
+int bielcc_field_batch(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                       int nrhs, const double* jE_d, const double* jM_d, double multE_re, double multE_im,
+                       double multH_re, double multH_im, const double* inc, int near_mode, int precision, int M,
+                       const double* pts_d, double* E_d, double* H_d) {
+    return field_impl(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, jM_d, multE_re, multE_im, multH_re, multH_im, inc,
+                      near_mode, precision, M, pts_d, E_d, H_d);
+}
+
+int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf, int* piv_d,
+                       int* info) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || !A_d || !F_d || !piv_d || !info || lda < n || ldf < n)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    Timer t(ctx, "lu32_factor");
+    hipError_t e = lu32_factor(ctx->stream, ctx->lu, n, A_d, lda, F_d, ldf, piv_d, info, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    if (*info != 0) return fail(ctx, BIELCC_ERR_SINGULAR, "zero FP32 pivot at column " + std::to_string(*info));
+    return BIELCC_OK;
+}
+
+int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
+                        const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
+                        int* iters, double* resid) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || nrhs <= 0 || !A_d || !F_d || !piv_d || !B_d || !iters || !resid || lda < n || ldf < n ||
+        ldb < n || !(tol > 0.) || max_iter < 0)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    int status = 0;
+    Timer t(ctx, "refine_solve");
+    hipError_t e = refine_solve(ctx->stream, ctx->lu, n, A_d, lda, F_d, ldf, piv_d, nrhs, B_d, ldb, tol, max_iter,
+                                iters, resid, &status, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    if (status != 0) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "refinement stopped at relative residual %.3e > tol %.3e after %d corrections",
+                      *resid, tol, *iters);
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED, buf);
+    }
+    return BIELCC_OK;
+}
+
+int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
+    if (!ctx || !key) return BIELCC_ERR_ARG;
+    const std::string k(key);
+    if (k == "field_wg" || k == "assemble_wg") {
+        if (value < 1 || value > (1 << 20)) return fail(ctx, BIELCC_ERR_ARG, "workgroup target out of range");
+        (k == "field_wg" ? ctx->field_wg : ctx->assemble_wg) = value;
+        return BIELCC_OK;
+    }
+    if (k == "field_rchunk") {
+        if (value != 1 && value != 2 && value != 4) return fail(ctx, BIELCC_ERR_ARG, "field_rchunk must be 1, 2 or 4");
+        ctx->field_rchunk = value;
+        return BIELCC_OK;
+    }
+    if (k == "trsv") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "trsv must be 0 or 1");
+        lu_set_trsv(ctx->lu, value == 1);
+        return BIELCC_OK;
+    }
+    return fail(ctx, BIELCC_ERR_ARG, "unknown tuning key " + k);
+}
+
+int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
+    if (!ctx || !key || !value) return BIELCC_ERR_ARG;
+    const std::string k(key);
+    if (k == "field_wg") *value = ctx->field_wg;
+    else if (k == "assemble_wg") *value = ctx->assemble_wg;
+    else if (k == "field_rchunk") *value = ctx->field_rchunk;
+    else if (k == "trsv") *value = lu_get_trsv(ctx->lu) ? 1 : 0;
+    else return fail(ctx, BIELCC_ERR_ARG, "unknown tuning key " + k);
+    return BIELCC_OK;
 }
 
 int bielcc_timing_enable(bielcc_ctx ctx, int on) {

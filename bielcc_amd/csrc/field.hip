@@ -1,4 +1,5 @@
-// Stratton-Chu field at M observation points over N cells.
+// Stratton-Chu field at M observation points over N cells, for one or more
+// sets of surface currents (right-hand sides).
 //
 // Far pairs (the near rule of electrodynamics/get_field.h:58 is false) go
 // through one fused tile kernel that evaluates, per quadrature node, the
@@ -13,7 +14,17 @@
 // as wave-wide broadcasts.  Per lane and far cell the kernel accumulates
 //   A = sum w e f1,  T_ab = sum w e f2 dh_a dh_b,  G = sum w (ikr-1) e d / r^3
 // so that K_far[j] = (A j + T j)/4pi and R_colloc[j] = (G x j)/4pi hold for the
-// electric and the magnetic current alike.
+// electric and the magnetic current alike.  A, T and G do not depend on the
+// currents, so one pass serves R current sets (R = 1, 2 or 4) for the price of
+// R small contractions per cell.
+//
+// Mixed precision (k_field_far32): the node sums run in FP32 in cell-local
+// coordinates.  Per (point, cell) the centre distance dcen, the near/far
+// decision and the centre phase k*dcen - n*pi/2 stay FP64; per node
+//   r - dcen = (|y_c|^2 - 2 x_c.y_c) / (r + dcen)      (x_c = x - c, y_c = y - c)
+// is formed from small cell-local quantities, so the node phase carries no
+// cancellation error, and the cell's sums are rotated by i^n, scaled and
+// contracted with the currents in FP64.
 #include "bielcc_internal.h"
 #include "phase.h"
 
@@ -22,17 +33,89 @@ using namespace bielcc_dev;
 namespace bielcc_impl {
 
 constexpr int kFT = 256;          // points per workgroup
-constexpr int kFNodes = 512;      // node slots staged in LDS (16 KB)
+constexpr int kFNodes = 512;      // FP64 node slots staged in LDS (16 KB)
+constexpr int kFNodes32 = 1024;   // FP32 node slots (16 KB + 4 KB)
 constexpr int kFCells = 16;       // max cells per LDS stage
-constexpr int kCellRec = 16;      // staged per-cell record: center 3, diam, jE 6, jM 6
+constexpr int kCellRec = 32;      // staged per-cell record: centre 3, diam, jE[R][6], jM[6]
+constexpr int kGroup32 = 8;       // FP32 node sums are flushed to FP64 every kGroup32 nodes
 
-// Accumulator layout of the partial sums, [NACC][M] per split:
+// Stage the per-cell records of cells [cs, cs+ncs): centre, diameter, the R
+// electric current sets r0..r0+R-1 and (R = 1 only) the magnetic current.
+template <bool JM, int R>
+__device__ inline void stage_cells(const FieldArgs& a, int cs, int ncs, int r0, double (*scell)[kCellRec]) {
+    constexpr int nf = 4 + 6 * R + (JM ? 6 : 0);
+    for (int idx = threadIdx.x; idx < ncs * nf; idx += kFT) {
+        const int cc = idx / nf, f = idx % nf;
+        const size_t c = (size_t)(cs + cc);
+        double v;
+        if (f < 3) v = a.ctab[c * CT + CT_CENTER + f];
+        else if (f == 3) v = a.ctab[c * CT + CT_DIAM];
+        else if (f < 4 + 6 * R) v = a.jE[((size_t)(r0 + (f - 4) / 6) * a.N + c) * 6 + (f - 4) % 6];
+        else v = a.jM[((size_t)r0 * a.N + c) * 6 + (f - 4 - 6 * R)];
+        scell[cc][f] = v;
+    }
+}
+
+// One far cell's sums applied to R current sets: K[j] += A j + T j and
+// R[j] += G x j; R_Rot_Colloc is exactly zero when x is the cell centroid
+// (R.h:94-97).  T = (xx, yy, zz, xy, xz, yz).
+template <bool JM, int R>
+__device__ inline void far_contract(const double* rec, const cplx& A, const cplx (&T)[6], const cplx (&G)[3],
+                                    bool self, cplx (&E)[R][3], cplx (&H)[R][3], cplx (&KH)[3], cplx (&RE)[3]) {
+    const cplx T00 = T[0], T11 = T[1], T22 = T[2], T01 = T[3], T02 = T[4], T12 = T[5];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const double* j = rec + 4 + 6 * r;
+        const cplx je[3] = {mk(j[0], j[1]), mk(j[2], j[3]), mk(j[4], j[5])};
+        E[r][0] += A * je[0] + T00 * je[0] + T01 * je[1] + T02 * je[2];
+        E[r][1] += A * je[1] + T01 * je[0] + T11 * je[1] + T12 * je[2];
+        E[r][2] += A * je[2] + T02 * je[0] + T12 * je[1] + T22 * je[2];
+        if (!self) {
+            cplx t[3];
+            ccross(G, je, t);
+            H[r][0] += t[0]; H[r][1] += t[1]; H[r][2] += t[2];
+        }
+    }
+    if (JM) {
+        const double* j = rec + 4 + 6 * R;
+        const cplx jm[3] = {mk(j[0], j[1]), mk(j[2], j[3]), mk(j[4], j[5])};
+        KH[0] += A * jm[0] + T00 * jm[0] + T01 * jm[1] + T02 * jm[2];
+        KH[1] += A * jm[1] + T01 * jm[0] + T11 * jm[1] + T12 * jm[2];
+        KH[2] += A * jm[2] + T02 * jm[0] + T12 * jm[1] + T22 * jm[2];
+        if (!self) {
+            cplx t[3];
+            ccross(G, jm, t);
+            RE[0] += t[0]; RE[1] += t[1]; RE[2] += t[2];
+        }
+    }
+}
+
+// Accumulator layout of the partial sums, [nsplit][nrhs][NACC][M]:
 //  0-5  sum (A j_E + T j_E)         (E from K[j_E], unscaled)
 //  6-11 sum G x j_E                  (H from R[j_E], unscaled)
 //  12-17 sum (A j_M + T j_M)        (H from K[j_M], unscaled)   [with j_M]
 //  18-23 sum G x j_M                 (E from R[j_M], unscaled)   [with j_M]
-template <bool CK, bool JM, bool WH, bool FP>
-__global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_split) {
+template <bool JM, int R>
+__device__ inline void write_partials(const FieldArgs& a, int r0, int pt, const cplx (&E)[R][3],
+                                      const cplx (&H)[R][3], const cplx (&KH)[3], const cplx (&RE)[3]) {
+    const int nacc = JM ? 24 : 12;
+    const size_t st = a.M;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        double* out = a.part + ((size_t)blockIdx.y * a.nrhs + r0 + r) * nacc * st + pt;
+        for (int g = 0; g < 3; g++) {
+            out[(2 * g) * st] = E[r][g].re; out[(2 * g + 1) * st] = E[r][g].im;
+            out[(6 + 2 * g) * st] = H[r][g].re; out[(6 + 2 * g + 1) * st] = H[r][g].im;
+            if (JM) {
+                out[(12 + 2 * g) * st] = KH[g].re; out[(12 + 2 * g + 1) * st] = KH[g].im;
+                out[(18 + 2 * g) * st] = RE[g].re; out[(18 + 2 * g + 1) * st] = RE[g].im;
+            }
+        }
+    }
+}
+
+template <bool CK, bool JM, bool FP, int R>
+__global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_split, int r0) {
     __shared__ double4 snode[kFNodes];
     __shared__ double scell[kFCells][kCellRec];
     const int tid = threadIdx.x;
@@ -47,7 +130,9 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
     const double kr_ = a.k_re, ki_ = a.k_im;
     const double k2r = kr_ * kr_ - ki_ * ki_, k2i = 2. * kr_ * ki_;
 
-    cplx E[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, H[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    cplx E[R][3], H[R][3];
+    for (int r = 0; r < R; r++)
+        for (int g = 0; g < 3; g++) { E[r][g] = mk(0, 0); H[r][g] = mk(0, 0); }
     cplx KH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, RE[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
 
     for (int cs = c_begin; cs < c_end; cs += cb) {
@@ -55,16 +140,7 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
         __syncthreads();
         const double4* gn = reinterpret_cast<const double4*>(a.nodes) + (size_t)cs * nn;
         for (int idx = tid; idx < ncs * nn; idx += kFT) snode[idx] = gn[idx];
-        for (int idx = tid; idx < ncs * kCellRec; idx += kFT) {
-            int cc = idx / kCellRec, f = idx % kCellRec;
-            const int c = cs + cc;
-            double v;
-            if (f < 3) v = a.ctab[(size_t)c * CT + CT_CENTER + f];
-            else if (f == 3) v = a.ctab[(size_t)c * CT + CT_DIAM];
-            else if (f < 10) v = a.jE[6 * (size_t)c + (f - 4)];
-            else v = JM ? a.jM[6 * (size_t)c + (f - 10)] : 0.;
-            scell[cc][f] = v;
-        }
+        stage_cells<JM, R>(a, cs, ncs, r0, scell);
         __syncthreads();
         if (!active) continue;
         for (int cc = 0; cc < ncs; cc++) {
@@ -111,80 +187,176 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
                 Tr[3] += e2r * p01; Ti[3] += e2i * p01;
                 Tr[4] += e2r * p02; Ti[4] += e2i * p02;
                 Tr[5] += e2r * p12; Ti[5] += e2i * p12;
-                if (WH || JM) {
-                    // (ikr - 1) e = (-(1 + ki r) + i kr r) (c + i s)
-                    const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
-                    const double wr3 = ri3;
-                    const double gr = (mr * wc - mi * ws) * wr3, gi = (mr * ws + mi * wc) * wr3;
-                    Gr[0] += gr * d0; Gi[0] += gi * d0;
-                    Gr[1] += gr * d1; Gi[1] += gi * d1;
-                    Gr[2] += gr * d2; Gi[2] += gi * d2;
-                }
+                // (ikr - 1) e = (-(1 + ki r) + i kr r) (c + i s)
+                const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
+                const double gr = (mr * wc - mi * ws) * ri3, gi = (mr * ws + mi * wc) * ri3;
+                Gr[0] += gr * d0; Gi[0] += gi * d0;
+                Gr[1] += gr * d1; Gi[1] += gi * d1;
+                Gr[2] += gr * d2; Gi[2] += gi * d2;
             }
             if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
                 rot_q(Ar, Ai, pr.q);
                 for (int t = 0; t < 6; t++) rot_q(Tr[t], Ti[t], pr.q);
                 for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
             }
-            // apply to the cell's currents
             const cplx A = mk(Ar, Ai);
-            const cplx T00 = mk(Tr[0], Ti[0]), T11 = mk(Tr[1], Ti[1]), T22 = mk(Tr[2], Ti[2]);
-            const cplx T01 = mk(Tr[3], Ti[3]), T02 = mk(Tr[4], Ti[4]), T12 = mk(Tr[5], Ti[5]);
-            const cplx je[3] = {mk(rec[4], rec[5]), mk(rec[6], rec[7]), mk(rec[8], rec[9])};
-            E[0] += A * je[0] + T00 * je[0] + T01 * je[1] + T02 * je[2];
-            E[1] += A * je[1] + T01 * je[0] + T11 * je[1] + T12 * je[2];
-            E[2] += A * je[2] + T02 * je[0] + T12 * je[1] + T22 * je[2];
+            const cplx T[6] = {mk(Tr[0], Ti[0]), mk(Tr[1], Ti[1]), mk(Tr[2], Ti[2]),
+                               mk(Tr[3], Ti[3]), mk(Tr[4], Ti[4]), mk(Tr[5], Ti[5])};
             const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
-            // R_Rot_Colloc is exactly zero when x is the cell centroid (R.h:94-97)
-            const bool self = dcen < kMachineZero;
-            if (WH && !self) {
-                cplx t[3];
-                ccross(G, je, t);
-                H[0] += t[0]; H[1] += t[1]; H[2] += t[2];
-            }
-            if (JM) {
-                const cplx jm[3] = {mk(rec[10], rec[11]), mk(rec[12], rec[13]), mk(rec[14], rec[15])};
-                if (WH) {
-                    KH[0] += A * jm[0] + T00 * jm[0] + T01 * jm[1] + T02 * jm[2];
-                    KH[1] += A * jm[1] + T01 * jm[0] + T11 * jm[1] + T12 * jm[2];
-                    KH[2] += A * jm[2] + T02 * jm[0] + T12 * jm[1] + T22 * jm[2];
-                }
-                if (!self) {
-                    cplx t[3];
-                    ccross(G, jm, t);
-                    RE[0] += t[0]; RE[1] += t[1]; RE[2] += t[2];
-                }
-            }
+            far_contract<JM, R>(rec, A, T, G, dcen < kMachineZero, E, H, KH, RE);
         }
     }
     if (!active) return;
-    const int nacc = JM ? 24 : 12;
-    double* out = a.part + (size_t)blockIdx.y * nacc * a.M + pt;
-    const size_t st = a.M;
-    for (int g = 0; g < 3; g++) {
-        out[(2 * g) * st] = E[g].re; out[(2 * g + 1) * st] = E[g].im;
-        out[(6 + 2 * g) * st] = H[g].re; out[(6 + 2 * g + 1) * st] = H[g].im;
-        if (JM) {
-            out[(12 + 2 * g) * st] = KH[g].re; out[(12 + 2 * g + 1) * st] = KH[g].im;
-            out[(18 + 2 * g) * st] = RE[g].re; out[(18 + 2 * g + 1) * st] = RE[g].im;
-        }
-    }
+    write_partials<JM, R>(a, r0, pt, E, H, KH, RE);
 }
 
-template <bool CK, bool JM, bool WH, bool FP>
-static void field_far_t(hipStream_t s, const FieldArgs& a) {
+template <bool CK, bool JM, bool FP, int R>
+__global__ __launch_bounds__(kFT) void k_field_far32(FieldArgs a, int cells_per_split, int r0) {
+    __shared__ float4 snode[kFNodes32];
+    __shared__ float syy[kFNodes32];
+    __shared__ double scell[kFCells][kCellRec];
+    const int tid = threadIdx.x;
+    const int pt = blockIdx.x * kFT + tid;
+    const bool active = pt < a.M;
+    double x0 = 0., x1 = 0., x2 = 0.;
+    if (active) { x0 = a.pts[3 * (size_t)pt]; x1 = a.pts[3 * (size_t)pt + 1]; x2 = a.pts[3 * (size_t)pt + 2]; }
+    const int c_begin = blockIdx.y * cells_per_split;
+    const int c_end = min(a.N, c_begin + cells_per_split);
+    const int nn = a.nn;
+    const int cb = max(1, min(kFCells, kFNodes32 / nn));
+    const double kr_ = a.k_re, ki_ = a.k_im;
+    const float kf = (float)kr_, kif = (float)ki_;
+    const float k2rf = (float)(kr_ * kr_ - ki_ * ki_), k2if = (float)(2. * kr_ * ki_);
+
+    cplx E[R][3], H[R][3];
+    for (int r = 0; r < R; r++)
+        for (int g = 0; g < 3; g++) { E[r][g] = mk(0, 0); H[r][g] = mk(0, 0); }
+    cplx KH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, RE[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+
+    for (int cs = c_begin; cs < c_end; cs += cb) {
+        const int ncs = min(cb, c_end - cs);
+        __syncthreads();
+        const float4* gn = a.nodes32 + (size_t)cs * nn;
+        const float* gy = a.nodes32_yy + (size_t)cs * nn;
+        for (int idx = tid; idx < ncs * nn; idx += kFT) { snode[idx] = gn[idx]; syy[idx] = gy[idx]; }
+        stage_cells<JM, R>(a, cs, ncs, r0, scell);
+        __syncthreads();
+        if (!active) continue;
+        for (int cc = 0; cc < ncs; cc++) {
+            const double* rec = scell[cc];
+            const double dcen = center_dist(x0, x1, x2, rec);
+            if (near_rule(dcen, rec[3], a.mode, a.ac, a.thresh)) continue;
+            const PhaseRef pr = phase_ref(kr_ * dcen);
+            const float t0 = (float)(fma(kr_, dcen, -pr.o1) - pr.o2 - pr.o3);
+            const float xf0 = (float)(x0 - rec[0]), xf1 = (float)(x1 - rec[1]), xf2 = (float)(x2 - rec[2]);
+            const float m0 = -2.f * xf0, m1 = -2.f * xf1, m2 = -2.f * xf2;
+            const float dcf = (float)dcen;
+            // FP32 sums over groups of kGroup32 nodes, flushed into FP64 (the
+            // recursive FP32 sum of 64 same-signed terms would lose ~sqrt(64) ulp)
+            double ArD = 0., AiD = 0., TrD[6] = {0, 0, 0, 0, 0, 0}, TiD[6] = {0, 0, 0, 0, 0, 0};
+            double GrD[3] = {0, 0, 0}, GiD[3] = {0, 0, 0};
+            const float4* nd = snode + cc * nn;
+            const float* ny = syy + cc * nn;
+            for (int q0 = 0; q0 < nn; q0 += kGroup32) {
+                float Ar = 0.f, Ai = 0.f;
+                float Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
+                float Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
+                const int q1 = min(nn, q0 + kGroup32);
+                for (int q = q0; q < q1; q++) {
+                    const float4 y = nd[q];
+                    const float d0 = xf0 - y.x, d1 = xf1 - y.y, d2 = xf2 - y.z;
+                    const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                    const float ri = rsqrtf(r2);
+                    const float r = r2 * ri;
+                    // r - dcen without cancellation: (|y_c|^2 - 2 x_c.y_c) / (r + dcen)
+                    const float del = fmaf(m0, y.x, fmaf(m1, y.y, fmaf(m2, y.z, ny[q])));
+                    const float dr = __fdividef(del, r + dcf);
+                    const float t = fmaf(kf, dr, t0);
+                    float s, c;
+                    if (FP) sincos_near32(t, s, c);
+                    else sincosf(t, &s, &c);
+                    float w = y.w;
+                    if (CK) w *= __expf(-kif * dr);   // e^{-ki dcen} is applied per cell in FP64
+                    const float ri2 = ri * ri, ri3 = ri2 * ri;
+                    const float wc = w * c, ws = w * s;
+                    float f1r, f1i, f2r, f2i;
+                    if (CK) {
+                        f1r = -ri3 - kif * ri2 + k2rf * ri; f1i = kf * ri2 + k2if * ri;
+                        f2r = 3.f * ri3 + 3.f * kif * ri2 - k2rf * ri; f2i = -3.f * kf * ri2 - k2if * ri;
+                    } else {
+                        f1r = k2rf * ri - ri3; f1i = kf * ri2;
+                        f2r = 3.f * ri3 - k2rf * ri; f2i = -3.f * f1i;
+                    }
+                    Ar += wc * f1r - ws * f1i;
+                    Ai += wc * f1i + ws * f1r;
+                    const float e2r = wc * f2r - ws * f2i, e2i = wc * f2i + ws * f2r;
+                    const float h0 = d0 * ri, h1 = d1 * ri, h2 = d2 * ri;
+                    const float p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p01 = h0 * h1, p02 = h0 * h2, p12 = h1 * h2;
+                    Tr[0] += e2r * p00; Ti[0] += e2i * p00;
+                    Tr[1] += e2r * p11; Ti[1] += e2i * p11;
+                    Tr[2] += e2r * p22; Ti[2] += e2i * p22;
+                    Tr[3] += e2r * p01; Ti[3] += e2i * p01;
+                    Tr[4] += e2r * p02; Ti[4] += e2i * p02;
+                    Tr[5] += e2r * p12; Ti[5] += e2i * p12;
+                    const float mr = CK ? -(1.f + kif * r) : -1.f, mi = kf * r;
+                    const float gr = (mr * wc - mi * ws) * ri3, gi = (mr * ws + mi * wc) * ri3;
+                    Gr[0] += gr * d0; Gi[0] += gi * d0;
+                    Gr[1] += gr * d1; Gi[1] += gi * d1;
+                    Gr[2] += gr * d2; Gi[2] += gi * d2;
+                }
+                ArD += Ar; AiD += Ai;
+                for (int t = 0; t < 6; t++) { TrD[t] += Tr[t]; TiD[t] += Ti[t]; }
+                for (int g = 0; g < 3; g++) { GrD[g] += Gr[g]; GiD[g] += Gi[g]; }
+            }
+            rot_q(ArD, AiD, pr.q);
+            for (int t = 0; t < 6; t++) rot_q(TrD[t], TiD[t], pr.q);
+            for (int g = 0; g < 3; g++) rot_q(GrD[g], GiD[g], pr.q);
+            if (CK) {
+                const double amp = exp(-ki_ * dcen);
+                ArD *= amp; AiD *= amp;
+                for (int t = 0; t < 6; t++) { TrD[t] *= amp; TiD[t] *= amp; }
+                for (int g = 0; g < 3; g++) { GrD[g] *= amp; GiD[g] *= amp; }
+            }
+            const cplx A = mk(ArD, AiD);
+            const cplx T[6] = {mk(TrD[0], TiD[0]), mk(TrD[1], TiD[1]), mk(TrD[2], TiD[2]),
+                               mk(TrD[3], TiD[3]), mk(TrD[4], TiD[4]), mk(TrD[5], TiD[5])};
+            const cplx G[3] = {mk(GrD[0], GiD[0]), mk(GrD[1], GiD[1]), mk(GrD[2], GiD[2])};
+            far_contract<JM, R>(rec, A, T, G, dcen < kMachineZero, E, H, KH, RE);
+        }
+    }
+    if (!active) return;
+    write_partials<JM, R>(a, r0, pt, E, H, KH, RE);
+}
+
+template <bool CK, bool JM, bool FP, int R>
+static void field_far_t(hipStream_t s, const FieldArgs& a, int r0) {
     int cps = (a.N + a.nsplit - 1) / a.nsplit;
     dim3 g((a.M + kFT - 1) / kFT, a.nsplit), b(kFT);
-    hipLaunchKernelGGL((k_field_far<CK, JM, WH, FP>), g, b, 0, s, a, cps);
+    if (a.prec) hipLaunchKernelGGL((k_field_far32<CK, JM, FP, R>), g, b, 0, s, a, cps, r0);
+    else hipLaunchKernelGGL((k_field_far<CK, JM, FP, R>), g, b, 0, s, a, cps, r0);
+}
+
+// current sets in passes of up to rchunk (4, 2 or 1; 1 with a magnetic current)
+template <bool CK, bool JM, bool FP>
+static void field_far_r(hipStream_t s, const FieldArgs& a) {
+    int r0 = 0;
+    if (!JM) {
+        if (a.rchunk >= 4)
+            for (; r0 + 4 <= a.nrhs; r0 += 4) field_far_t<CK, JM, FP, 4>(s, a, r0);
+        if (a.rchunk >= 2)
+            for (; r0 + 2 <= a.nrhs; r0 += 2) field_far_t<CK, JM, FP, 2>(s, a, r0);
+    }
+    for (; r0 < a.nrhs; r0++) field_far_t<CK, JM, FP, 1>(s, a, r0);
 }
 
 template <bool CK, bool JM>
 static void field_far_fp(hipStream_t s, const FieldArgs& a) {
-    if (a.fast_phase) field_far_t<CK, JM, true, true>(s, a);
-    else field_far_t<CK, JM, true, false>(s, a);
+    if (a.fast_phase) field_far_r<CK, JM, true>(s, a);
+    else field_far_r<CK, JM, false>(s, a);
 }
 
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a) {
+    if (a.prec && (!a.nodes32 || !a.nodes32_yy)) return hipErrorInvalidValue;
     const bool ck = a.k_im != 0.;
     const bool jm = a.jM != nullptr;
     if (ck) { if (jm) field_far_fp<true, true>(s, a); else field_far_fp<true, false>(s, a); }
@@ -229,34 +401,72 @@ __global__ __launch_bounds__(64) void k_field_near(long long np_, const int* __r
     }
 }
 
+// Several current sets: the near operators are linear in j, so each pair's
+// 3x3 matrices are formed once from the unit vectors (column b = Op(e_b)).
+// out[pair]: Kmat (9 complex, column-major) then Rmat (9 complex).
+template <int NV>
+__global__ __launch_bounds__(64) void k_field_near_mat(long long np_, const int* __restrict__ lq,
+                                                       const int* __restrict__ lc, const double* __restrict__ pts,
+                                                       const double* __restrict__ verts, NumP np, double k_re,
+                                                       double k_im, int mode, double* __restrict__ out) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np_) return;
+    const int q = lq[p], c = lc[p];
+    double cv[NV][3], x[3];
+    for (int v = 0; v < NV; v++)
+        for (int d = 0; d < 3; d++) cv[v][d] = verts[((size_t)c * NV + v) * 3 + d];
+    for (int d = 0; d < 3; d++) x[d] = pts[3 * (size_t)q + d];
+    const cplx k = mk(k_re, k_im);
+    const int opK = (mode == 0) ? OP_K_NEAR_SMOOTH : OP_K_HS;
+    double* o = out + (size_t)p * 36;
+    for (int b = 0; b < 3; b++) {
+        cplx j[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, r[3];
+        j[b] = mk(1., 0.);
+        op_eval<NV>(opK, cv, x, j, np, k, r);
+        for (int d = 0; d < 3; d++) { o[2 * (3 * b + d)] = r[d].re; o[2 * (3 * b + d) + 1] = r[d].im; }
+        op_eval<NV>(OP_R_SMOOTH, cv, x, j, np, k, r);
+        for (int d = 0; d < 3; d++) { o[18 + 2 * (3 * b + d)] = r[d].re; o[18 + 2 * (3 * b + d) + 1] = r[d].im; }
+    }
+}
+
 hipError_t launch_field_near(hipStream_t s, int nv, long long npairs, const int* lq, const int* lc,
                              const double* pts, const double* verts, const double* jE, const double* jM,
-                             const NumP& np, double k_re, double k_im, int mode, double* out) {
+                             const NumP& np, double k_re, double k_im, int mode, int as_matrix, double* out) {
     if (npairs <= 0) return hipSuccess;
     dim3 g((unsigned)((npairs + 63) / 64)), b(64);
-    if (nv == 4) hipLaunchKernelGGL(k_field_near<4>, g, b, 0, s, npairs, lq, lc, pts, verts, jE, jM, np, k_re, k_im, mode, out);
-    else hipLaunchKernelGGL(k_field_near<3>, g, b, 0, s, npairs, lq, lc, pts, verts, jE, jM, np, k_re, k_im, mode, out);
+    if (as_matrix) {
+        if (nv == 4) hipLaunchKernelGGL(k_field_near_mat<4>, g, b, 0, s, npairs, lq, lc, pts, verts, np, k_re, k_im, mode, out);
+        else hipLaunchKernelGGL(k_field_near_mat<3>, g, b, 0, s, npairs, lq, lc, pts, verts, np, k_re, k_im, mode, out);
+    } else {
+        if (nv == 4) hipLaunchKernelGGL(k_field_near<4>, g, b, 0, s, npairs, lq, lc, pts, verts, jE, jM, np, k_re, k_im, mode, out);
+        else hipLaunchKernelGGL(k_field_near<3>, g, b, 0, s, npairs, lq, lc, pts, verts, jE, jM, np, k_re, k_im, mode, out);
+    }
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- finalize
 // E = multE (sum K[jE]) - sum R[jM] (+ E_inc),  H = multH (sum K[jM]) + sum R[jE]
 // (main0_field_test.cpp:313-319; get_field.h:69-76 for the E-only driver).
-__global__ __launch_bounds__(256) void k_field_finalize(int M, int nsplit, const double* __restrict__ part,
-                                                        int has_jm, const long long* __restrict__ offs,
-                                                        const double* __restrict__ near_out, double mEr,
-                                                        double mEi, double mHr, double mHi, int add_inc,
-                                                        double kv0, double kv1, double kv2, double e00,
-                                                        double e01, double e02, const double* __restrict__ pts,
-                                                        double* __restrict__ E, double* __restrict__ H) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= M) return;
+// One lane per (point, current set); E, H are [nrhs][M][3] complex.
+__device__ inline cplx mat3_row(const double* m, int d, const cplx* j) {
+    // row d of a column-major 3x3 complex matrix times j
+    cplx s = mk(m[2 * d], m[2 * d + 1]) * j[0];
+    s += mk(m[2 * (3 + d)], m[2 * (3 + d) + 1]) * j[1];
+    s += mk(m[2 * (6 + d)], m[2 * (6 + d) + 1]) * j[2];
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_field_finalize(FinalizeArgs f) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (p >= f.M) return;
+    const int has_jm = f.jM != nullptr;
     const int nacc = has_jm ? 24 : 12;
     double acc[24];
     for (int a = 0; a < nacc; a++) acc[a] = 0.;
-    for (int sp = 0; sp < nsplit; sp++) {
-        const double* src = part + (size_t)sp * nacc * M + p;
-        for (int a = 0; a < nacc; a++) acc[a] += src[(size_t)a * M];
+    for (int sp = 0; sp < f.nsplit; sp++) {
+        const double* src = f.part + ((size_t)sp * f.nrhs + r) * nacc * f.M + p;
+        for (int a = 0; a < nacc; a++) acc[a] += src[(size_t)a * f.M];
     }
     // far sums are unscaled integrals: divide by 4pi as K.h:300 / R.h:109 do per cell
     for (int a = 0; a < nacc; a++) acc[a] = acc[a] / (4. * kPi);
@@ -269,9 +479,9 @@ __global__ __launch_bounds__(256) void k_field_finalize(int M, int nsplit, const
             sRE[g] = mk(acc[18 + 2 * g], acc[18 + 2 * g + 1]);
         }
     }
-    if (offs) {
-        for (long long q = offs[p]; q < offs[p + 1]; q++) {
-            const double* o = near_out + (size_t)q * nacc;
+    if (f.offs && !f.near_mat) {
+        for (long long q = f.offs[p]; q < f.offs[p + 1]; q++) {
+            const double* o = f.near_out + (size_t)q * nacc;
             for (int g = 0; g < 3; g++) {
                 sKE[g] += mk(o[2 * g], o[2 * g + 1]);
                 sRH[g] += mk(o[6 + 2 * g], o[6 + 2 * g + 1]);
@@ -281,32 +491,51 @@ __global__ __launch_bounds__(256) void k_field_finalize(int M, int nsplit, const
                 }
             }
         }
+    } else if (f.offs) {
+        for (long long q = f.offs[p]; q < f.offs[p + 1]; q++) {
+            const double* Km = f.near_out + (size_t)q * 36;
+            const double* Rm = Km + 18;
+            const size_t c = (size_t)f.lc[q];
+            const double* je = f.jE + ((size_t)r * f.N + c) * 6;
+            const cplx j[3] = {mk(je[0], je[1]), mk(je[2], je[3]), mk(je[4], je[5])};
+            for (int g = 0; g < 3; g++) {
+                sKE[g] += mat3_row(Km, g, j);
+                sRH[g] += mat3_row(Rm, g, j);
+            }
+            if (has_jm) {
+                const double* jm = f.jM + ((size_t)r * f.N + c) * 6;
+                const cplx m[3] = {mk(jm[0], jm[1]), mk(jm[2], jm[3]), mk(jm[4], jm[5])};
+                for (int g = 0; g < 3; g++) {
+                    sRE[g] += mat3_row(Rm, g, m);
+                    sKH[g] += mat3_row(Km, g, m);
+                }
+            }
+        }
     }
-    const cplx mE = mk(mEr, mEi), mH = mk(mHr, mHi);
+    const cplx mE = mk(f.mEr, f.mEi), mH = mk(f.mHr, f.mHi);
     cplx e[3], h[3];
     for (int g = 0; g < 3; g++) {
         e[g] = mE * sKE[g] - sRE[g];
         h[g] = mH * sKH[g] + sRH[g];
     }
-    if (add_inc) {   // PlaneWaveFieldE incident_field.h:20-29
-        const double* x = pts + 3 * (size_t)p;
-        double ph = kv0 * x[0] + kv1 * x[1] + kv2 * x[2];
+    if (f.inc) {   // PlaneWaveFieldE incident_field.h:20-29
+        const double* ke = f.inc + 6 * r;
+        const double* x = f.pts + 3 * (size_t)p;
+        double ph = ke[0] * x[0] + ke[1] * x[1] + ke[2] * x[2];
         cplx deg = cexp(cplx{0., 1.} * ph);
-        e[0] += deg * e00; e[1] += deg * e01; e[2] += deg * e02;
+        e[0] += deg * ke[3]; e[1] += deg * ke[4]; e[2] += deg * ke[5];
     }
+    double* Eo = f.E + ((size_t)r * f.M + p) * 6;
+    double* Ho = f.H ? f.H + ((size_t)r * f.M + p) * 6 : nullptr;
     for (int g = 0; g < 3; g++) {
-        E[6 * (size_t)p + 2 * g] = e[g].re; E[6 * (size_t)p + 2 * g + 1] = e[g].im;
-        if (H) { H[6 * (size_t)p + 2 * g] = h[g].re; H[6 * (size_t)p + 2 * g + 1] = h[g].im; }
+        Eo[2 * g] = e[g].re; Eo[2 * g + 1] = e[g].im;
+        if (Ho) { Ho[2 * g] = h[g].re; Ho[2 * g + 1] = h[g].im; }
     }
 }
 
-hipError_t launch_field_finalize(hipStream_t s, int M, int nsplit, const double* part, int has_jm,
-                                 const long long* near_offs, const double* near_out, double mE_re, double mE_im,
-                                 double mH_re, double mH_im, int add_inc, const double* kvec_e0,
-                                 const double* pts, double* E, double* H) {
-    hipLaunchKernelGGL(k_field_finalize, dim3((M + 255) / 256), dim3(256), 0, s, M, nsplit, part, has_jm, near_offs,
-                       near_out, mE_re, mE_im, mH_re, mH_im, add_inc, kvec_e0[0], kvec_e0[1], kvec_e0[2],
-                       kvec_e0[3], kvec_e0[4], kvec_e0[5], pts, E, H);
+hipError_t launch_field_finalize(hipStream_t s, const FinalizeArgs& f) {
+    if (f.M <= 0 || f.nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field_finalize, dim3((f.M + 255) / 256, f.nrhs), dim3(256), 0, s, f);
     return hipGetLastError();
 }
 

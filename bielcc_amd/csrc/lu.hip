@@ -1,20 +1,31 @@
-// Dense complex FP64 LU with partial pivoting and the multi-RHS solve.
+// Dense complex LU with partial pivoting, the multi-RHS solve, and the
+// mixed-precision solve (FP32 factors + FP64 iterative refinement).
 //
 // The reference links LAPACK/BLAS but never calls them (CMakeLists.txt:18-19,56),
 // so this is the library's own solver.  Right-looking blocked LU, panel width
-// kNB, column-major:
+// kNB, column-major, for complex FP64 (double2) and complex FP32 (float2):
 //   panel  : per column, an argmax of |re|+|im| over the remaining rows (the
 //            pivot rule of LAPACK's izamax, first index on ties), the row swap
 //            inside the panel, scaling by the reciprocal pivot and the rank-1
-//            update of the rest of the panel (own kernels);
+//            update of the rest of the panel (own kernels; the update also
+//            reduces the next column's pivot candidates);
 //   laswp  : the panel's row interchanges applied to the other columns;
 //   trsm   : U12 = L11^{-1} A12 with L11 and a 64-column tile of A12 in LDS;
-//   update : A22 -= L21 U12, a plain ZGEMM (rocBLAS, FP64 matrix cores).
+//   update : A22 -= L21 U12, a plain complex GEMM on the matrix cores
+//            (rocBLAS zgemm for FP64, cgemm with FP32 accumulation for FP32).
 // The solve applies the composed row permutation, then blocked forward and
-// back substitution (LDS triangular tiles + ZGEMM for the off-diagonal blocks).
+// back substitution (LDS triangular tiles + GEMM for the off-diagonal blocks).
+//
+// Mixed precision: F = fl32(A) is factored in FP32; X0 = F^{-1} B, then
+// X += F^{-1} (B - A X) with the residual formed from the FP64 matrix, until
+// the relative residual reaches the caller's tolerance.  Each correction
+// shrinks the error by about cond(A) * 2^-24, so well-conditioned systems
+// reach FP64 residuals in a few steps.
 #include "bielcc_internal.h"
 
 #include <rocblas/rocblas.h>
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 using namespace bielcc_dev;
@@ -24,24 +35,67 @@ namespace bielcc_impl {
 constexpr int kNB = 64;
 constexpr int kPivBlocks = 256;
 
+template <typename R> struct V2;
+template <> struct V2<double> { using T = double2; };
+template <> struct V2<float> { using T = float2; };
+template <typename R> using c2 = typename V2<R>::T;
+
+template <typename R>
+__host__ __device__ inline c2<R> mk2(R x, R y) {
+    c2<R> v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+
 struct LuWork {
     rocblas_handle blas = nullptr;
     double* pval = nullptr;     // [pcap] partial maxima
     int* pidx = nullptr;        // [pcap] partial argmax
     int pcap = 0;
-    double* pinv = nullptr;     // reciprocal pivot (re, im) + flag
+    double* pinv = nullptr;     // reciprocal pivot (re, im) + flag (FP32 runs store floats here)
     int* info = nullptr;        // first zero pivot, 1-based (0 if none)
-    int* perm = nullptr;        // composed permutation for the solve
+    int* perm = nullptr;        // composed permutation + solve staging
     size_t perm_cap = 0;
+    void* ref = nullptr;        // refinement work vectors
+    size_t ref_cap = 0;
+    // sync-free triangular solve: per-block ready flags (epoch-stamped, never
+    // reset), the ticket counter that orders the workgroups, a timeout flag
+    unsigned* flags = nullptr;
+    int nflags = 0;
+    unsigned long long* ticket = nullptr;
+    unsigned long long tbase = 0;   // tickets issued so far (host mirror of *ticket)
+    unsigned epoch = 0;
+    int* err = nullptr;
+    bool no_trsv = false;           // force the blocked TRSM + GEMM path (tests)
 };
 
 LuWork* lu_work_create() { return new LuWork(); }
+
+void lu_set_trsv(LuWork* w, bool on) { w->no_trsv = !on; }
+bool lu_get_trsv(const LuWork* w) { return !w->no_trsv; }
+
+// a sync-free triangular solve whose bounded wait expired (then cleared)
+static hipError_t trsv_check(hipStream_t s, LuWork* w, const char** msg) {
+    if (!w->err) return hipSuccess;
+    int h = 0;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(&h, w->err, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (h) {
+        (void)hipMemsetAsync(w->err, 0, sizeof(int), s);
+        *msg = "triangular solve: a block wait expired (result invalid)";
+        return hipErrorLaunchTimeOut;
+    }
+    return hipSuccess;
+}
 
 void lu_work_destroy(LuWork* w) {
     if (!w) return;
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
-    (void)hipFree(w->perm);
+    (void)hipFree(w->perm); (void)hipFree(w->ref);
+    (void)hipFree(w->flags); (void)hipFree(w->ticket); (void)hipFree(w->err);
     delete w;
 }
 
@@ -71,18 +125,20 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
     return hipSuccess;
 }
 
-__device__ inline double cabs1(double2 v) { return fabs(v.x) + fabs(v.y); }
+template <typename R>
+__device__ inline double cabs1(c2<R> v) { return fabs((double)v.x) + fabs((double)v.y); }
 
 // Phase 1: partial argmax of |re|+|im| over rows j..n-1 of column j.
-__global__ __launch_bounds__(256) void k_piv_partial(int n, int j, const double2* __restrict__ A, long long lda,
+template <typename R>
+__global__ __launch_bounds__(256) void k_piv_partial(int n, int j, const c2<R>* __restrict__ A, long long lda,
                                                      double* __restrict__ pval, int* __restrict__ pidx) {
     __shared__ double sv[256];
     __shared__ int si[256];
-    const double2* col = A + (size_t)j * lda;
+    const c2<R>* col = A + (size_t)j * lda;
     double best = -1.;
     int bi = 0x7fffffff;
     for (int i = j + blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        double v = cabs1(col[i]);
+        double v = cabs1<R>(col[i]);
         if (v > best) { best = v; bi = i; }
     }
     sv[threadIdx.x] = best;
@@ -104,10 +160,11 @@ __global__ __launch_bounds__(256) void k_piv_partial(int n, int j, const double2
 
 // Phase 2: final argmax, record the pivot, swap rows j and p inside the
 // panel columns [c0, c1), store the reciprocal pivot.
-__global__ __launch_bounds__(256) void k_piv_final(int nparts, int j, int c0, int c1, double2* __restrict__ A,
+template <typename R>
+__global__ __launch_bounds__(256) void k_piv_final(int nparts, int j, int c0, int c1, c2<R>* __restrict__ A,
                                                    long long lda, const double* __restrict__ pval,
                                                    const int* __restrict__ pidx, int* __restrict__ piv,
-                                                   double* __restrict__ pinv, int* __restrict__ info) {
+                                                   R* __restrict__ pinv, int* __restrict__ info) {
     __shared__ double sv[256];
     __shared__ int si[256];
     __shared__ int sp;
@@ -142,18 +199,18 @@ __global__ __launch_bounds__(256) void k_piv_final(int nparts, int j, int c0, in
     const int p = sp;
     if (p != j) {
         for (int c = c0 + threadIdx.x; c < c1; c += 256) {
-            double2* colp = A + (size_t)c * lda;
-            double2 t = colp[j];
+            c2<R>* colp = A + (size_t)c * lda;
+            c2<R> t = colp[j];
             colp[j] = colp[p];
             colp[p] = t;
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double2 a = A[(size_t)j * lda + j];
-        double den = a.x * a.x + a.y * a.y;
-        if (den > 0.) { pinv[0] = a.x / den; pinv[1] = -a.y / den; pinv[2] = 1.; }
-        else { pinv[0] = 0.; pinv[1] = 0.; pinv[2] = 0.; }
+        c2<R> a = A[(size_t)j * lda + j];
+        R den = a.x * a.x + a.y * a.y;
+        if (den > R(0)) { pinv[0] = a.x / den; pinv[1] = -a.y / den; pinv[2] = R(1); }
+        else { pinv[0] = R(0); pinv[1] = R(0); pinv[2] = R(0); }
     }
 }
 
@@ -162,10 +219,11 @@ __global__ __launch_bounds__(256) void k_piv_final(int nparts, int j, int c0, in
 // j+1 is in the panel, each workgroup also reduces the updated |re|+|im| of
 // its rows of column j+1 to a partial argmax, so the next column's pivot
 // search needs only the final reduction.
-__global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, double2* __restrict__ A, long long lda,
-                                                      const double* __restrict__ pinv, double* __restrict__ pval,
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, c2<R>* __restrict__ A, long long lda,
+                                                      const R* __restrict__ pinv, double* __restrict__ pval,
                                                       int* __restrict__ pidx) {
-    __shared__ double2 urow[kNB];
+    __shared__ c2<R> urow[kNB];
     __shared__ double sv[256];
     __shared__ int si[256];
     const int nc = c1 - j - 1;
@@ -175,20 +233,20 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, doub
     double best = -1.;
     int bi = 0x7fffffff;
     if (i < n) {
-        double2 l = A[(size_t)j * lda + i];
-        if (pinv[2] != 0.) {
-            double2 s = make_double2(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
+        c2<R> l = A[(size_t)j * lda + i];
+        if (pinv[2] != R(0)) {
+            c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
             l = s;
             A[(size_t)j * lda + i] = l;
         }
         for (int t = 0; t < nc; t++) {
-            double2* p = A + (size_t)(j + 1 + t) * lda + i;
-            double2 u = urow[t];
-            double2 a = *p;
+            c2<R>* p = A + (size_t)(j + 1 + t) * lda + i;
+            c2<R> u = urow[t];
+            c2<R> a = *p;
             a.x -= l.x * u.x - l.y * u.y;
             a.y -= l.x * u.y + l.y * u.x;
             *p = a;
-            if (t == 0) { best = cabs1(a); bi = i; }
+            if (t == 0) { best = cabs1<R>(a); bi = i; }
         }
     }
     if (nc <= 0) return;   // uniform: no next column inside the panel
@@ -210,15 +268,16 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, doub
 }
 
 // Apply the interchanges piv[k0 .. k0+nb) to columns [cbeg, cend).
-__global__ __launch_bounds__(256) void k_laswp(int k0, int nb, int cbeg, int cend, double2* __restrict__ A,
+template <typename R>
+__global__ __launch_bounds__(256) void k_laswp(int k0, int nb, int cbeg, int cend, c2<R>* __restrict__ A,
                                                long long lda, const int* __restrict__ piv) {
     const int c = cbeg + blockIdx.x * 256 + threadIdx.x;
     if (c >= cend) return;
-    double2* col = A + (size_t)c * lda;
+    c2<R>* col = A + (size_t)c * lda;
     for (int t = 0; t < nb; t++) {
         int r = k0 + t, p = piv[r];
         if (p != r) {
-            double2 v = col[r];
+            c2<R> v = col[r];
             col[r] = col[p];
             col[p] = v;
         }
@@ -230,12 +289,13 @@ __global__ __launch_bounds__(256) void k_laswp(int k0, int nb, int cbeg, int cen
 // One workgroup of 256 lanes per 64 columns; T and the X tile live in LDS.
 // Column sweep: lane (r = tid % 64, column group tid / 64) owns row r of
 // every fourth column; step q publishes x_q and every lane updates its rows.
-__global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const double2* __restrict__ A, long long lda,
-                                              long long t0, double2* __restrict__ B, long long ldb, long long brow,
+template <typename R>
+__global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const c2<R>* __restrict__ A, long long lda,
+                                              long long t0, c2<R>* __restrict__ B, long long ldb, long long brow,
                                               int cbeg, int cend) {
-    __shared__ double2 T[kNB][kNB + 1];
-    __shared__ double2 X[kNB][kNB + 1];
-    __shared__ double2 dinv[kNB];
+    __shared__ c2<R> T[kNB][kNB + 1];
+    __shared__ c2<R> X[kNB][kNB + 1];
+    __shared__ c2<R> dinv[kNB];
     const int tid = threadIdx.x;
     const int r = tid & 63, cg = tid >> 6;
     for (int q = cg; q < nb; q += 4)
@@ -246,18 +306,18 @@ __global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const double2* 
         if (r < nb) X[r][cc] = B[(size_t)(c0 + cc) * ldb + brow + r];
     __syncthreads();
     if (!lower && tid < nb) {
-        double2 d = T[tid][tid];
-        double den = d.x * d.x + d.y * d.y;
-        dinv[tid] = make_double2(d.x / den, -d.y / den);
+        c2<R> d = T[tid][tid];
+        R den = d.x * d.x + d.y * d.y;
+        dinv[tid] = mk2<R>(d.x / den, -d.y / den);
     }
     __syncthreads();
     if (lower) {
         for (int q = 0; q < nb - 1; q++) {
             if (r > q && r < nb) {
-                const double2 l = T[r][q];
+                const c2<R> l = T[r][q];
                 for (int cc = cg; cc < ncol; cc += 4) {
-                    const double2 x = X[q][cc];
-                    double2 a = X[r][cc];
+                    const c2<R> x = X[q][cc];
+                    c2<R> a = X[r][cc];
                     a.x -= l.x * x.x - l.y * x.y;
                     a.y -= l.x * x.y + l.y * x.x;
                     X[r][cc] = a;
@@ -269,11 +329,11 @@ __global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const double2* 
         // X[q] is final once step q begins; rows < q are updated with x_q = X[q] / T[q][q]
         for (int q = nb - 1; q > 0; q--) {
             if (r < q) {
-                const double2 u = T[r][q], di = dinv[q];
+                const c2<R> u = T[r][q], di = dinv[q];
                 for (int cc = cg; cc < ncol; cc += 4) {
-                    const double2 b = X[q][cc];
-                    const double2 x = make_double2(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
-                    double2 a = X[r][cc];
+                    const c2<R> b = X[q][cc];
+                    const c2<R> x = mk2<R>(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
+                    c2<R> a = X[r][cc];
                     a.x -= u.x * x.x - u.y * x.y;
                     a.y -= u.x * x.y + u.y * x.x;
                     X[r][cc] = a;
@@ -282,10 +342,10 @@ __global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const double2* 
             __syncthreads();
         }
         if (r < nb) {
-            const double2 di = dinv[r];
+            const c2<R> di = dinv[r];
             for (int cc = cg; cc < ncol; cc += 4) {
-                const double2 b = X[r][cc];
-                X[r][cc] = make_double2(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
+                const c2<R> b = X[r][cc];
+                X[r][cc] = mk2<R>(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
             }
         }
         __syncthreads();
@@ -294,9 +354,10 @@ __global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const double2* 
         if (r < nb) B[(size_t)(c0 + cc) * ldb + brow + r] = X[r][cc];
 }
 
-static hipError_t zgemm_update(LuWork* w, hipStream_t s, long long m, long long n, long long k, const double2* Aa,
-                               long long lda, const double2* Bb, long long ldb, double2* Cc, long long ldc,
-                               const char** msg) {
+// C -= A B (complex, column-major) on the matrix cores
+static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n, long long k, const double2* Aa,
+                              long long lda, const double2* Bb, long long ldb, double2* Cc, long long ldc,
+                              const char** msg) {
     if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
     rocblas_set_stream(w->blas, s);
     rocblas_double_complex alpha = {-1., 0.}, beta = {1., 0.};
@@ -311,11 +372,29 @@ static hipError_t zgemm_update(LuWork* w, hipStream_t s, long long m, long long 
     return hipGetLastError();
 }
 
-hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda, int* piv, int* info,
-                     const char** msg) {
+static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n, long long k, const float2* Aa,
+                              long long lda, const float2* Bb, long long ldb, float2* Cc, long long ldc,
+                              const char** msg) {
+    if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+    rocblas_set_stream(w->blas, s);
+    rocblas_float_complex alpha = {-1.f, 0.f}, beta = {1.f, 0.f};
+    rocblas_status st = rocblas_cgemm_64(w->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
+                                         reinterpret_cast<const rocblas_float_complex*>(Aa), lda,
+                                         reinterpret_cast<const rocblas_float_complex*>(Bb), ldb, &beta,
+                                         reinterpret_cast<rocblas_float_complex*>(Cc), ldc);
+    if (st != rocblas_status_success) {
+        *msg = "rocblas_cgemm_64 failed";
+        return hipErrorUnknown;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long long lda, int* piv, int* info,
+                              const char** msg) {
     hipError_t e = lu_work_init(w, msg, n);
     if (e != hipSuccess) return e;
-    double2* A = reinterpret_cast<double2*>(Ad);
+    R* pinv = reinterpret_cast<R*>(w->pinv);
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
     for (int k0 = 0; k0 < n; k0 += kNB) {
         const int nb = min(kNB, n - k0);
@@ -326,24 +405,25 @@ hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda,
             const int m = n - j;
             if (jj == 0) {   // first panel column: fresh search (later ones come from k_panel_update)
                 nparts = min(kPivBlocks, (m + 255) / 256);
-                hipLaunchKernelGGL(k_piv_partial, dim3(nparts), dim3(256), 0, s, n, j, A, lda, w->pval, w->pidx);
+                hipLaunchKernelGGL(k_piv_partial<R>, dim3(nparts), dim3(256), 0, s, n, j, A, lda, w->pval, w->pidx);
             }
-            hipLaunchKernelGGL(k_piv_final, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval, w->pidx,
-                               piv, w->pinv, w->info);
+            hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval,
+                               w->pidx, piv, pinv, w->info);
             if (m > 1) {
                 nparts = (m - 1 + 255) / 256;
-                hipLaunchKernelGGL(k_panel_update, dim3(nparts), dim3(256), 0, s, n, j, c1, A, lda, w->pinv,
+                hipLaunchKernelGGL(k_panel_update<R>, dim3(nparts), dim3(256), 0, s, n, j, c1, A, lda, pinv,
                                    w->pval, w->pidx);
             }
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (k0 > 0) hipLaunchKernelGGL(k_laswp, dim3((k0 + 255) / 256), dim3(256), 0, s, k0, nb, 0, k0, A, lda, piv);
+        if (k0 > 0)
+            hipLaunchKernelGGL(k_laswp<R>, dim3((k0 + 255) / 256), dim3(256), 0, s, k0, nb, 0, k0, A, lda, piv);
         if (c1 < n) {
-            hipLaunchKernelGGL(k_laswp, dim3((n - c1 + 255) / 256), dim3(256), 0, s, k0, nb, c1, n, A, lda, piv);
-            hipLaunchKernelGGL(k_trsm, dim3((n - c1 + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, A,
-                               lda, (long long)k0, c1, n);
-            e = zgemm_update(w, s, n - c1, n - c1, nb, A + (size_t)k0 * lda + c1, lda, A + (size_t)c1 * lda + k0,
-                             lda, A + (size_t)c1 * lda + c1, lda, msg);
+            hipLaunchKernelGGL(k_laswp<R>, dim3((n - c1 + 255) / 256), dim3(256), 0, s, k0, nb, c1, n, A, lda, piv);
+            hipLaunchKernelGGL(k_trsm<R>, dim3((n - c1 + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0,
+                               A, lda, (long long)k0, c1, n);
+            e = gemm_update(w, s, n - c1, n - c1, nb, A + (size_t)k0 * lda + c1, lda, A + (size_t)c1 * lda + k0,
+                            lda, A + (size_t)c1 * lda + c1, lda, msg);
             if (e != hipSuccess) return e;
         }
     }
@@ -351,30 +431,56 @@ hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda,
     return hipStreamSynchronize(s);
 }
 
-// B(i, :) <- B(perm(i), :) with perm the composition of the interchanges
-__global__ void k_gather_rows(int n, int nrhs, const int* __restrict__ perm, const double2* __restrict__ src,
-                              double2* __restrict__ B, long long ldb) {
+hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda, int* piv, int* info,
+                     const char** msg) {
+    return lu_factor_t<double>(s, w, n, reinterpret_cast<double2*>(Ad), lda, piv, info, msg);
+}
+
+// dst (FP32, ldd) = src (FP64, lds), n rows x ncol columns
+__global__ void k_to_f32(int n, int ncol, const double2* __restrict__ src, long long lds, float2* __restrict__ dst,
+                         long long ldd) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * ncol) return;
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    const double2 v = src[(size_t)c * lds + i];
+    dst[(size_t)c * ldd + i] = make_float2((float)v.x, (float)v.y);
+}
+
+hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, float* Fd, long long ldf,
+                       int* piv, int* info, const char** msg) {
+    const long long tot = (long long)n * n;
+    float2* F = reinterpret_cast<float2*>(Fd);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, n,
+                       reinterpret_cast<const double2*>(Ad), lda, F, ldf);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return lu_factor_t<float>(s, w, n, F, ldf, piv, info, msg);
+}
+
+// B(i, :) <- src(perm(i), :) with perm the composition of the interchanges
+template <typename R>
+__global__ void k_gather_rows(int n, int nrhs, const int* __restrict__ perm, const c2<R>* __restrict__ src,
+                              c2<R>* __restrict__ B, long long ldb) {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)n * nrhs) return;
     int i = (int)(idx % n), r = (int)(idx / n);
     B[(size_t)r * ldb + i] = src[(size_t)r * n + perm[i]];
 }
 
-__global__ void k_copy_cols(int n, int nrhs, const double2* __restrict__ B, long long ldb, double2* __restrict__ dst) {
+template <typename R>
+__global__ void k_copy_cols(int n, int nrhs, const c2<R>* __restrict__ B, long long ldb, c2<R>* __restrict__ dst) {
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)n * nrhs) return;
     int i = (int)(idx % n), r = (int)(idx / n);
     dst[(size_t)r * n + i] = B[(size_t)r * ldb + i];
 }
 
-hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const int* piv, int nrhs,
-                    double* Bd, long long ldb, const char** msg) {
-    hipError_t e = lu_work_init(w, msg);
-    if (e != hipSuccess) return e;
-    const double2* A = reinterpret_cast<const double2*>(Ad);
-    double2* B = reinterpret_cast<double2*>(Bd);
-    // compose the interchanges into a gather permutation (host, O(n))
+// composed gather permutation of the interchanges piv (host, O(n)); fills
+// w->perm (device) and returns the staging area behind it (n * nrhs * elem bytes)
+static hipError_t lu_perm(hipStream_t s, LuWork* w, int n, const int* piv, size_t stage_bytes, void** stage,
+                          const char** msg) {
     std::vector<int> hp(n), perm(n);
+    hipError_t e;
     if ((e = hipMemcpyAsync(hp.data(), piv, n * sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     for (int i = 0; i < n; i++) perm[i] = i;
@@ -384,7 +490,7 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long
         int t = perm[i]; perm[i] = perm[p]; perm[p] = t;
     }
     const size_t perm_bytes = ((size_t)n * sizeof(int) + 255) / 256 * 256;
-    const size_t need = perm_bytes + (size_t)n * nrhs * sizeof(double2);
+    const size_t need = perm_bytes + stage_bytes;
     if (w->perm_cap < need) {
         (void)hipFree(w->perm);
         w->perm = nullptr;
@@ -392,20 +498,264 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long
         if ((e = hipMalloc(&w->perm, need)) != hipSuccess) return e;
         w->perm_cap = need;
     }
-    int* dperm = w->perm;
-    double2* tmp = reinterpret_cast<double2*>(reinterpret_cast<char*>(w->perm) + perm_bytes);
-    if ((e = hipMemcpyAsync(dperm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    *stage = reinterpret_cast<char*>(w->perm) + perm_bytes;
+    return hipMemcpyAsync(w->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s);
+}
+
+// ---------------------------------------------------------------- sync-free TRSV
+// Few right-hand sides (nrhs <= kTrsvMaxR): one kernel per triangle instead of
+// a launch pair per block column.  Block row ib (64 rows) is owned by one
+// workgroup, which streams the blocks A[ib][jb] of its row in dependency order
+// as soon as x_jb is published, then solves its diagonal block in LDS and
+// publishes x_ib (stores, device-scope fence, release store of the block's
+// flag = epoch).  lower = 1: unit lower L, rows 0, 1, ...; lower = 0: upper U,
+// rows nblk-1, nblk-2, ....  Workgroups take block rows in the order of a
+// ticket counter, so a row only ever waits for rows whose workgroups have
+// already started: every wait ends.  Each wait is also bounded (kSpinLimit
+// polls); on expiry the workgroup raises *err and carries on, every other
+// wait then gives up too, so the kernel always drains and the host reports
+// the failure.
+constexpr int kTrsvMaxR = 8;
+constexpr int kSpinLimit = 1 << 20;   // polls per wait (about a second); a real wait lasts microseconds
+
+template <typename R>
+__device__ inline c2<R> load_relaxed(const c2<R>* p) {
+    // bypasses the CU cache: x blocks are written by other workgroups
+    const R* q = reinterpret_cast<const R*>(p);
+    return mk2<R>(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                  __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ inline double bcast_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float bcast_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename R, int NR>
+__global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const c2<R>* __restrict__ A,
+                                              long long lda, c2<R>* B, long long ldb, unsigned* flags,
+                                              unsigned long long* ticket, unsigned long long tbase,
+                                              unsigned epoch, int* err) {
+    __shared__ c2<R> T[kNB][kNB + 1];
+    __shared__ c2<R> X[kNB][NR];
+    __shared__ c2<R> red[4][kNB][NR];
+    __shared__ int s_blk, s_ready;
+    const int tid = threadIdx.x;
+    const int r = tid & 63, cg = tid >> 6;
+    const int nblk = (n + kNB - 1) / kNB;
+    if (tid == 0) s_blk = (int)(atomicAdd(ticket, 1ull) - tbase);
+    __syncthreads();
+    const int t = s_blk;
+    if (t < 0 || t >= nblk) {   // ticket counter out of step with the host: flag it, touch nothing
+        if (tid == 0) atomicExch(err, 2);
+        return;
+    }
+    const int ib = lower ? t : nblk - 1 - t;
+    const int r0 = ib * kNB, nb = min(kNB, n - r0);
+    c2<R> acc[NR];
+    for (int q = 0; q < NR; q++) acc[q] = mk2<R>(R(0), R(0));
+    // lane (r, cg) owns row r, columns cg, cg+4, ... of each 64 x 64 block;
+    // the block's values are loaded one step ahead of its x
+    c2<R> av[kNB / 4];
+    auto load_block = [&](int s) {
+        const int jb = lower ? s : nblk - 1 - s;
+        const int c0 = jb * kNB, ncb = min(kNB, n - c0);
+        const c2<R>* Ab = A + (size_t)c0 * lda + r0 + r;
+#pragma unroll
+        for (int k = 0; k < kNB / 4; k++) {
+            const int c = cg + 4 * k;
+            av[k] = (r < nb && c < ncb) ? Ab[(size_t)c * lda] : mk2<R>(R(0), R(0));
+        }
+    };
+    if (t > 0) load_block(0);
+    // the diagonal block does not depend on any x: stage it before the waits
+    for (int q = cg; q < nb; q += 4)
+        if (r < nb) T[r][q] = A[(size_t)(r0 + q) * lda + r0 + r];
+    int ready = 0;   // blocks known to be published (same value in every lane)
+    for (int s = 0; s < t; s++) {
+        const int jb = lower ? s : nblk - 1 - s;
+        if (s >= ready) {
+            if (tid == 0) {
+                // relaxed device-scope polls read the flag coherently without the
+                // cache invalidation an acquire load performs on every iteration;
+                // x_jb is then read with device-scope loads as well (load_relaxed),
+                // issued only after the flag value has been seen
+                int spins = 0;
+                while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                    if (++spins > kSpinLimit) { atomicExch(err, 1); break; }
+                    // once any wait has expired, nobody waits any more: the kernel drains
+                    if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                int rd = s + 1;   // extend the window without waiting
+                while (rd < t && rd < s + 32) {
+                    const int kb = lower ? rd : nblk - 1 - rd;
+                    if (__hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) break;
+                    rd++;
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                s_ready = rd;
+            }
+            __syncthreads();
+            ready = s_ready;
+        }
+        const int c0 = jb * kNB, ncb = min(kNB, n - c0);
+        for (int e = tid; e < ncb * nrhs; e += 256) {
+            const int c = e % ncb, q = e / ncb;
+            X[c][q] = load_relaxed<R>(B + (size_t)q * ldb + c0 + c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kNB / 4; k++) {
+            const int c = cg + 4 * k;
+            if (c < ncb) {
+                const c2<R> a = av[k];
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    if (q < nrhs) {
+                        const c2<R> x = X[c][q];
+                        acc[q].x -= a.x * x.x - a.y * x.y;
+                        acc[q].y -= a.x * x.y + a.y * x.x;
+                    }
+                }
+            }
+        }
+        if (s + 1 < t) load_block(s + 1);
+        __syncthreads();   // X is rewritten next step; s_ready too
+    }
+    // y = b + the four column groups' partial sums
+    for (int q = 0; q < NR; q++) red[cg][r][q] = acc[q];
+    __syncthreads();
+    if (tid < kNB) {
+        // wave 0: lane r holds row r of the tile; x_p is broadcast from lane p (no barriers)
+        c2<R> y[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) {
+            y[q] = mk2<R>(R(0), R(0));
+            if (q < nrhs && r < nb) {
+                y[q] = B[(size_t)q * ldb + r0 + r];
+                for (int g = 0; g < 4; g++) { y[q].x += red[g][r][q].x; y[q].y += red[g][r][q].y; }
+            }
+        }
+        if (lower) {
+            for (int p = 0; p < nb - 1; p++) {
+                const c2<R> l = T[r][p];
+                const bool upd = r > p && r < nb;
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    if (q < nrhs) {
+                        const c2<R> x = mk2<R>(bcast_lane(y[q].x, p), bcast_lane(y[q].y, p));
+                        if (upd) {
+                            y[q].x -= l.x * x.x - l.y * x.y;
+                            y[q].y -= l.x * x.y + l.y * x.x;
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int p = nb - 1; p >= 0; p--) {
+                const c2<R> d = T[p][p];
+                const R den = d.x * d.x + d.y * d.y;
+                const c2<R> di = mk2<R>(d.x / den, -d.y / den);
+                const c2<R> u = T[r][p];
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    if (q < nrhs) {
+                        const c2<R> b = mk2<R>(bcast_lane(y[q].x, p), bcast_lane(y[q].y, p));
+                        const c2<R> x = mk2<R>(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
+                        if (r == p) y[q] = x;
+                        else if (r < p) {
+                            y[q].x -= u.x * x.x - u.y * x.y;
+                            y[q].y -= u.x * x.y + u.y * x.x;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+            if (q < nrhs && r < nb) B[(size_t)q * ldb + r0 + r] = y[q];
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+static hipError_t trsv_init(LuWork* w, int n) {
+    hipError_t e;
+    const int nblk = (n + kNB - 1) / kNB;
+    if (!w->ticket) {
+        if ((e = hipMalloc(&w->ticket, sizeof(unsigned long long))) != hipSuccess) return e;
+        if ((e = hipMalloc(&w->err, sizeof(int))) != hipSuccess) return e;
+        if ((e = hipMemset(w->ticket, 0, sizeof(unsigned long long))) != hipSuccess) return e;
+        if ((e = hipMemset(w->err, 0, sizeof(int))) != hipSuccess) return e;
+        w->tbase = 0;
+    }
+    if (w->nflags < nblk) {
+        (void)hipFree(w->flags);
+        w->flags = nullptr;
+        w->nflags = 0;
+        if ((e = hipMalloc(&w->flags, nblk * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMemset(w->flags, 0, nblk * sizeof(unsigned))) != hipSuccess) return e;
+        w->nflags = nblk;
+        w->epoch = 0;
+    }
+    return hipSuccess;
+}
+
+template <typename R, int NR>
+static void trsv_launch(hipStream_t s, LuWork* w, int n, int nrhs, int lower, const c2<R>* A, long long lda,
+                        c2<R>* B, long long ldb) {
+    const int nblk = (n + kNB - 1) / kNB;
+    if (++w->epoch == 0) w->epoch = 1;   // flags start at 0: epoch 0 is never "ready"
+    hipLaunchKernelGGL((k_trsv<R, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, lower, A, lda, B, ldb, w->flags,
+                       w->ticket, w->tbase, w->epoch, w->err);
+    w->tbase += (unsigned long long)nblk;
+}
+
+template <typename R>
+static hipError_t trsv_solve(hipStream_t s, LuWork* w, int n, int nrhs, const c2<R>* A, long long lda, c2<R>* B,
+                             long long ldb) {
+    hipError_t e = trsv_init(w, n);
+    if (e != hipSuccess) return e;
+    for (int lower = 1; lower >= 0; lower--) {
+        if (nrhs <= 1) trsv_launch<R, 1>(s, w, n, nrhs, lower, A, lda, B, ldb);
+        else if (nrhs <= 2) trsv_launch<R, 2>(s, w, n, nrhs, lower, A, lda, B, ldb);
+        else if (nrhs <= 4) trsv_launch<R, 4>(s, w, n, nrhs, lower, A, lda, B, ldb);
+        else trsv_launch<R, 8>(s, w, n, nrhs, lower, A, lda, B, ldb);
+        if ((e = hipGetLastError()) != hipSuccess) {
+            // the kernel did not run: bring the ticket counter back in step
+            (void)hipMemset(w->ticket, 0, sizeof(unsigned long long));
+            w->tbase = 0;
+            return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// A X = B with the factors in A and the composed permutation in w->perm
+template <typename R>
+static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A, long long lda, int nrhs,
+                                c2<R>* B, long long ldb, c2<R>* tmp, const char** msg) {
+    hipError_t e;
     long long tot = (long long)n * nrhs;
-    hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, B, ldb, tmp);
-    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, dperm, tmp, B, ldb);
+    hipLaunchKernelGGL(k_copy_cols<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, B, ldb, tmp);
+    hipLaunchKernelGGL(k_gather_rows<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, w->perm,
+                       tmp, B, ldb);
+    if (nrhs <= kTrsvMaxR && !w->no_trsv) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
     // forward: L y = P b
     for (int k0 = 0; k0 < n; k0 += kNB) {
         const int nb = min(kNB, n - k0);
-        hipLaunchKernelGGL(k_trsm, dim3((nrhs + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, B, ldb,
+        hipLaunchKernelGGL(k_trsm<R>, dim3((nrhs + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, B, ldb,
                            (long long)k0, 0, nrhs);
         if (k0 + nb < n) {
-            e = zgemm_update(w, s, n - k0 - nb, nrhs, nb, A + (size_t)k0 * lda + k0 + nb, lda, B + k0, ldb,
-                             B + k0 + nb, ldb, msg);
+            e = gemm_update(w, s, n - k0 - nb, nrhs, nb, A + (size_t)k0 * lda + k0 + nb, lda, B + k0, ldb,
+                            B + k0 + nb, ldb, msg);
             if (e != hipSuccess) return e;
         }
     }
@@ -413,14 +763,139 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long
     int last = ((n - 1) / kNB) * kNB;
     for (int k0 = last; k0 >= 0; k0 -= kNB) {
         const int nb = min(kNB, n - k0);
-        hipLaunchKernelGGL(k_trsm, dim3((nrhs + 63) / 64), dim3(256), 0, s, nb, 0, A, lda, (long long)k0, B, ldb,
+        hipLaunchKernelGGL(k_trsm<R>, dim3((nrhs + 63) / 64), dim3(256), 0, s, nb, 0, A, lda, (long long)k0, B, ldb,
                            (long long)k0, 0, nrhs);
         if (k0 > 0) {
-            e = zgemm_update(w, s, k0, nrhs, nb, A + (size_t)k0 * lda, lda, B + k0, ldb, B, ldb, msg);
+            e = gemm_update(w, s, k0, nrhs, nb, A + (size_t)k0 * lda, lda, B + k0, ldb, B, ldb, msg);
             if (e != hipSuccess) return e;
         }
     }
     return hipGetLastError();
+}
+
+hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const int* piv, int nrhs,
+                    double* Bd, long long ldb, const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    void* stage = nullptr;
+    if ((e = lu_perm(s, w, n, piv, (size_t)n * nrhs * sizeof(double2), &stage, msg)) != hipSuccess) return e;
+    e = lu_solve_perm<double>(s, w, n, reinterpret_cast<const double2*>(Ad), lda, nrhs,
+                              reinterpret_cast<double2*>(Bd), ldb, reinterpret_cast<double2*>(stage), msg);
+    if (e != hipSuccess) return e;
+    return trsv_check(s, w, msg);
+}
+
+// ---------------------------------------------------------------- refinement
+// out[r] = ||X[:, r]||_2 (one workgroup per column, FP64 sum of squares)
+__global__ __launch_bounds__(256) void k_colnorm(int n, const double2* __restrict__ X, long long ldx,
+                                                 double* __restrict__ out) {
+    __shared__ double sh[256];
+    const double2* col = X + (size_t)blockIdx.x * ldx;
+    double acc = 0.;
+    for (int i = threadIdx.x; i < n; i += 256) acc += col[i].x * col[i].x + col[i].y * col[i].y;
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(sh[0]);
+}
+
+// X[:, r] += double(D[:, r])
+__global__ void k_add_f32(int n, int nrhs, const float2* __restrict__ D, double2* __restrict__ X, long long ldx) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int i = (int)(idx % n), r = (int)(idx / n);
+    const float2 d = D[(size_t)r * n + i];
+    double2* x = X + (size_t)r * ldx + i;
+    x->x += (double)d.x;
+    x->y += (double)d.y;
+}
+
+// X[:, r] = double(D[:, r])
+__global__ void k_from_f32(int n, int nrhs, const float2* __restrict__ D, double2* __restrict__ X, long long ldx) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int i = (int)(idx % n), r = (int)(idx / n);
+    const float2 d = D[(size_t)r * n + i];
+    X[(size_t)r * ldx + i] = make_double2((double)d.x, (double)d.y);
+}
+
+// dst (n x nrhs, ldd) = src (lds)
+__global__ void k_copy2(int n, int nrhs, const double2* __restrict__ src, long long lds, double2* __restrict__ dst,
+                        long long ldd) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int i = (int)(idx % n), r = (int)(idx / n);
+    dst[(size_t)r * ldd + i] = src[(size_t)r * lds + i];
+}
+
+hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const float* Fd,
+                        long long ldf, const int* piv, int nrhs, double* Bd, long long ldb, double tol, int max_iter,
+                        int* iters, double* resid, int* status, const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    *iters = 0;
+    *resid = 0.;
+    *status = 0;
+    const double2* A = reinterpret_cast<const double2*>(Ad);
+    const float2* F = reinterpret_cast<const float2*>(Fd);
+    double2* X = reinterpret_cast<double2*>(Bd);
+    const size_t nv = (size_t)n * nrhs;
+    // work: B0 (FP64), Rd (FP64), Rf (FP32), norms (2 nrhs)
+    const size_t b_b = (nv * sizeof(double2) + 255) / 256 * 256;
+    const size_t f_b = (nv * sizeof(float2) + 255) / 256 * 256;
+    const size_t need = 2 * b_b + f_b + 2 * nrhs * sizeof(double);
+    if (w->ref_cap < need) {
+        (void)hipFree(w->ref);
+        w->ref = nullptr;
+        w->ref_cap = 0;
+        if ((e = hipMalloc(&w->ref, need)) != hipSuccess) return e;
+        w->ref_cap = need;
+    }
+    char* base = reinterpret_cast<char*>(w->ref);
+    double2* B0 = reinterpret_cast<double2*>(base);
+    double2* Rd = reinterpret_cast<double2*>(base + b_b);
+    float2* Rf = reinterpret_cast<float2*>(base + 2 * b_b);
+    double* nrm = reinterpret_cast<double*>(base + 2 * b_b + f_b);
+    void* stage = nullptr;
+    if ((e = lu_perm(s, w, n, piv, nv * sizeof(float2), &stage, msg)) != hipSuccess) return e;
+    float2* tmp = reinterpret_cast<float2*>(stage);
+    const unsigned gv = (unsigned)((nv + 255) / 256);
+
+    std::vector<double> bn(nrhs), rn(nrhs);
+    hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, X, ldb, B0, (long long)n);
+    hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, B0, (long long)n, nrm);
+    if ((e = hipMemcpyAsync(bn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    // X0 = F^{-1} B
+    hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rf, (long long)n);
+    if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
+    double prev = HUGE_VAL;
+    for (int it = 0;; it++) {
+        // Rd = B0 - A X (FP64)
+        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
+        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, (long long)n, msg)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, (long long)n, nrm);
+        if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if ((e = trsv_check(s, w, msg)) != hipSuccess) return e;
+        double rel = 0.;
+        for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
+        *resid = rel;
+        if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }   // NaN / Inf: FP32 factors broke down
+        if (rel <= tol) return hipSuccess;
+        if (it >= max_iter) { *status = 1; return hipSuccess; }
+        if (!(rel < 0.5 * prev)) { *status = 1; return hipSuccess; }   // stagnation or divergence
+        prev = rel;
+        // X += F^{-1} Rd
+        hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, (long long)n, Rf, (long long)n);
+        if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
+        *iters = it + 1;
+    }
 }
 
 }  // namespace bielcc_impl

@@ -181,6 +181,29 @@ hipError_t launch_seg_tables(hipStream_t s, int nv, int n, const double* verts, 
     return hipGetLastError();
 }
 
+// Cell-local FP32 copy of a node table for the mixed-precision tile kernels:
+// loc[c][q] = (y - centre_c, weight), yy[c][q] = |y - centre_c|^2, with the
+// offsets and their squared length formed in FP64 and rounded once.
+__global__ void k_local32(int n, int cnt, const double* __restrict__ nodes, const double* __restrict__ ctab,
+                          float4* __restrict__ loc, float* __restrict__ yy) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * cnt) return;
+    const int c = (int)(idx / cnt);
+    const double* y = nodes + 4 * (size_t)idx;
+    const double* cen = ctab + (size_t)c * CT + CT_CENTER;
+    const double d0 = y[0] - cen[0], d1 = y[1] - cen[1], d2 = y[2] - cen[2];
+    loc[idx] = make_float4((float)d0, (float)d1, (float)d2, (float)y[3]);
+    yy[idx] = (float)(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+hipError_t launch_local32(hipStream_t s, int n, int cnt, const double* nodes, const double* ctab, float4* loc,
+                          float* yy) {
+    const long long tot = (long long)n * cnt;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_local32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, cnt, nodes, ctab, loc, yy);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- near lists
 constexpr int kNearTile = 256;
 

@@ -203,11 +203,59 @@ int bielcc_get_field_e(bielcc_ctx ctx, int mesh_id, const double* jE_d, double m
                        const double* kvec, const double* e0, const bielcc_num_param* np, double k_re, double k_im,
                        int M, const double* pts_d, double* E_d);
 
+/* ------------------------------------------------------------ batched and mixed-precision forms
+ * No reference counterpart: the reference evaluates one current set per
+ * GetFieldE_Ideal call (get_field.h:35-78) and has no solver.
+ *
+ * Precision of the far-pair surface quadrature.  Near and self pairs, the
+ * analytic terms, the edge contours (K0) and every sum across cells stay FP64
+ * in both forms. */
+#define BIELCC_PREC_FP64 0   /* reference-exact */
+#define BIELCC_PREC_MIXED 1  /* FP32 node sums in cell-local coordinates */
+
+/* bielcc_field for nrhs current sets at once.  jE_d, jM_d: [nrhs][N][3]
+ * complex (jM_d may be NULL); inc: host [nrhs][6] = (kvec, e0) per set, or
+ * NULL for no incident field; E_d, H_d: [nrhs][M][3] complex. */
+int bielcc_field_batch(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                       int nrhs, const double* jE_d, const double* jM_d, double multE_re, double multE_im,
+                       double multH_re, double multH_im, const double* inc, int near_mode, int precision, int M,
+                       const double* pts_d, double* E_d, double* H_d);
+
+/* bielcc_assemble with a choice of far-pair precision. */
+int bielcc_assemble_ex(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re,
+                       double k_im, int row0, int nrows, int precision, double* Z_d, int64_t ld);
+
+/* FP32 complex LU with partial pivoting on the matrix cores: F_d (n x n
+ * complex float, column-major, ldf) receives the factors of the FP64 matrix
+ * A_d rounded to FP32; A_d is not modified.  piv_d[n], info as bielcc_lu_factor. */
+int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf, int* piv_d,
+                       int* info);
+/* Solves A X = B in FP64 by iterative refinement on the FP32 factors:
+ * X0 = F^{-1} B, then X += F^{-1} (B - A X) with the residual in FP64, until
+ * max_r ||B_r - A X_r||_2 / ||B_r||_2 <= tol or max_iter corrections.  B_d is
+ * overwritten by X; *iters = corrections applied, *resid = final relative
+ * residual.  Returns BIELCC_ERR_UNSUPPORTED if the residual stops shrinking
+ * above tol (matrix too ill-conditioned for FP32 factors); X is then the best
+ * iterate. */
+int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
+                        const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
+                        int* iters, double* resid);
+
+/* Launch-shape tunables (the autotuner sets these; defaults in brackets):
+ * "field_wg" [2048] and "assemble_wg" [2048] = workgroups a tile launch aims
+ * for (cells are split until the grid reaches it); "field_rchunk" [4] =
+ * current sets per field pass (1, 2 or 4); "trsv" [1] = solves with at most
+ * 8 right-hand sides use the one-kernel-per-triangle solve (0: blocked
+ * TRSM + GEMM).  Returns BIELCC_ERR_ARG for an unknown key or a value out of
+ * range. */
+int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);
+int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);
+
 /* ------------------------------------------------------------ timing
  * With timing on, the drivers bracket their dominant kernels with HIP events
  * on the context's stream; bielcc_timing reads the accumulated milliseconds
  * and launch count of one of: "field_far", "assemble_far", "lu_factor",
- * "lu_solve", "near". */
+ * "lu_solve", "near", "assemble_generic", "lu32_factor", "refine_solve". */
 int bielcc_timing_enable(bielcc_ctx ctx, int on);
 int bielcc_timing_reset(bielcc_ctx ctx);
 int bielcc_timing(bielcc_ctx ctx, const char* name, double* total_ms, int* count);
