@@ -84,6 +84,9 @@ _SIGS = {
                                      C.c_void_p, _IP]),
     "bielcc_refine_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                       C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, _IP, _DP]),
+    "bielcc_far_pattern": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bielcc_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "bielcc_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, _IP]),
     "bielcc_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),

@@ -411,6 +411,31 @@ class Mesh:
         self.field_batch_into(np_, k, nrhs, dje, dp, M, dE, dH, djm, multE, multH, inc, near_mode, precision)
         return dE.to_numpy((nrhs, M, 3), np.complex128), dH.to_numpy((nrhs, M, 3), np.complex128)
 
+    def far_pattern_into(self, np_: NumParam, k, nrhs: int, jE, dirs, ndir, Einf, Hinf=None, jM=None,
+                         multE=1.0, multH=1.0, precision=PREC_FP64):
+        k, multE, multH = complex(k), complex(multE), complex(multH)
+        self.ctx._check(self.ctx.lib.bielcc_far_pattern(self.ctx.h, self.id, C.byref(np_), k.real, k.imag,
+                                                        int(nrhs), _p(jE), _p(jM), multE.real, multE.imag,
+                                                        multH.real, multH.imag, int(precision), int(ndir),
+                                                        _p(dirs), _p(Einf), _p(Hinf)))
+
+    def far_pattern(self, np_: NumParam, k, jE, dirs, jM=None, multE=1.0, multH=1.0, precision=PREC_FP64):
+        """Far-field pattern at unit directions dirs [ndir][3]; jE (jM) [N][3] or
+        [nrhs][N][3].  Returns (E_inf, H_inf) shaped like the currents' sets."""
+        jE = np.asarray(jE, dtype=np.complex128)
+        single = jE.ndim == 2
+        jE = jE.reshape(-1, self.n, 3)
+        nrhs = jE.shape[0]
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        nd = dirs.shape[0]
+        dd, dje = self.ctx.to_device(dirs), self.ctx.to_device(jE)
+        djm = self.ctx.to_device(np.asarray(jM, dtype=np.complex128).reshape(nrhs, self.n, 3)) \
+            if jM is not None else None
+        dE, dH = self.ctx.empty(nrhs * nd * 48), self.ctx.empty(nrhs * nd * 48)
+        self.far_pattern_into(np_, k, nrhs, dje, dd, nd, dE, dH, djm, multE, multH, precision)
+        E, H = dE.to_numpy((nrhs, nd, 3), np.complex128), dH.to_numpy((nrhs, nd, 3), np.complex128)
+        return (E[0], H[0]) if single else (E, H)
+
     def get_field_e(self, jE, mult, kvec, e0, np_: NumParam, k, pts) -> np.ndarray:
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         M = pts.shape[0]

@@ -98,6 +98,24 @@ struct FinalizeArgs {
 };
 hipError_t launch_field_finalize(hipStream_t s, const FinalizeArgs& f);
 
+// ---- far-field pattern (farfield.hip)
+struct PatternArgs {
+    int N, nn, ndir, nrhs, nsplit;
+    const double* dirs;     // [ndir][3] unit vectors
+    const double* ctab;
+    const double* nodes;
+    const float4* nodes32;  // cell-local FP32 nodes (prec = 1)
+    const double* jE;       // [nrhs][N][3] complex
+    const double* jM;       // same or null
+    double k_re, k_im;
+    int fast_phase;         // |k| * max diameter within the polynomial range
+    int prec;
+    double* part;           // [nsplit][nrhs][12][ndir]
+    double* Einf;           // [nrhs][ndir][3] complex
+    double* Hinf;           // same or null
+};
+hipError_t launch_far_pattern(hipStream_t s, const PatternArgs& a, double mEr, double mEi, double mHr, double mHi);
+
 // ---- assembly (assemble.hip)
 struct AssembleArgs {
     int nv, N, nn, ns;

@@ -657,6 +657,40 @@ int bielcc_field_batch(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, 
                       near_mode, precision, M, pts_d, E_d, H_d);
 }
 
+int bielcc_far_pattern(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                       int nrhs, const double* jE_d, const double* jM_d, double multE_re, double multE_im,
+                       double multH_re, double multH_im, int precision, int ndir, const double* dirs_d,
+                       double* Einf_d, double* Hinf_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (int rc = check_np(ctx, np)) return rc;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    if (!m) return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id");
+    if (nrhs < 1 || ndir < 0) return fail(ctx, BIELCC_ERR_ARG, "nrhs must be >= 1 and ndir >= 0");
+    if (precision != BIELCC_PREC_FP64 && precision != BIELCC_PREC_MIXED)
+        return fail(ctx, BIELCC_ERR_ARG, "precision must be BIELCC_PREC_FP64 or BIELCC_PREC_MIXED");
+    if (ndir == 0) return BIELCC_OK;
+    if (!jE_d || !dirs_d || !Einf_d) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    if (!std::isfinite(k_re) || !std::isfinite(k_im)) return fail(ctx, BIELCC_ERR_ARG, "non-finite wave number");
+    const int nn = np->n_start * np->n_start;
+    if (nn > 512) return fail(ctx, BIELCC_ERR_UNSUPPORTED, "far pattern needs n_start <= 22");
+    DeviceGuard g(ctx->device);
+    if (int rc = ensure_tables(ctx, *m, np->n_start, 0)) return rc;
+    if (precision == BIELCC_PREC_MIXED)
+        if (int rc = ensure_tables32(ctx, *m, np->n_start)) return rc;
+    const int bx = (ndir + 255) / 256;
+    const int nsplit = std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64));
+    HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, (size_t)nsplit * nrhs * 12 * ndir * sizeof(double)));
+    PatternArgs a;
+    a.N = m->n; a.nn = nn; a.ndir = ndir; a.nrhs = nrhs; a.nsplit = nsplit; a.dirs = dirs_d; a.ctab = m->ctab;
+    a.nodes = m->nodes; a.nodes32 = m->nodes32; a.jE = jE_d; a.jM = jM_d; a.k_re = k_re; a.k_im = k_im;
+    a.fast_phase = std::fabs(k_re) * m->grid_step <= 1.95 && !ctx->no_fast_phase;
+    a.prec = precision; a.part = reinterpret_cast<double*>(ctx->scratch); a.Einf = Einf_d; a.Hinf = Hinf_d;
+    Timer t(ctx, "far_pattern");
+    HIPCHK(launch_far_pattern(ctx->stream, a, multE_re, multE_im, multH_re, multH_im));
+    t.stop();
+    return BIELCC_OK;
+}
+
 int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf, int* piv_d,
                        int* info) {
     if (!ctx) return BIELCC_ERR_ARG;

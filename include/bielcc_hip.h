@@ -241,6 +241,20 @@ int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, c
                         const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
                         int* iters, double* resid);
 
+/* Far-field pattern F(d) = lim_{R->inf} R e^{-ikR} (E, H)(R d) of the field of
+ * bielcc_field (no reference counterpart: the limit of K_RotRot_Far and
+ * R_Rot_Colloc, kernel_lib.h:453-492 / :295-313, over the same cell nodes):
+ *   E_inf = multE k^2/(4pi) (I - d d) P[jE] - ik/(4pi) d x P[jM],
+ *   H_inf = multH k^2/(4pi) (I - d d) P[jM] + ik/(4pi) d x P[jE],
+ *   P[j] = sum_c (sum_q w_q e^{-ik d.y_q}) j_c.
+ * dirs_d [ndir][3] unit vectors; jE_d, jM_d [nrhs][N][3] (jM_d may be NULL);
+ * Einf_d, Hinf_d [nrhs][ndir][3] complex (Hinf_d may be NULL).  Uses
+ * np->n_start only. */
+int bielcc_far_pattern(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                       int nrhs, const double* jE_d, const double* jM_d, double multE_re, double multE_im,
+                       double multH_re, double multH_im, int precision, int ndir, const double* dirs_d,
+                       double* Einf_d, double* Hinf_d);
+
 /* Launch-shape tunables (the autotuner sets these; defaults in brackets):
  * "field_wg" [2048] and "assemble_wg" [2048] = workgroups a tile launch aims
  * for (cells are split until the grid reaches it); "field_rchunk" [4] =

@@ -908,6 +908,136 @@ int orc_field_eh(int nv, int ncell, const double* verts, const double* jE, const
     return 0;
 }
 
+/* Quadrature nodes and weights of one cell at partition n, in the order and
+ * with the weights of IntegralUniversalPnt's first pass
+ * (integral_universal.h:38-120 quads: sub-quad centroid, |m1 x m2|;
+ * :136-220 triangles: sub-triangle centroids, S / n^2). */
+static int cell_nodes(int nv, const double (*c)[3], int n, double* y, double* w) {
+    int cnt = 0;
+    if (nv == 4) {
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++) {
+                double p = (double)i / (double)n, q = (double)j / (double)n;
+                double p1 = ((double)i + 1.) / (double)n, q1 = ((double)j + 1.) / (double)n;
+                double a[3], b[3], a1[3], a2[3], a3[3], a4[3], m1[3], m2[3], rn[3];
+                for (int k = 0; k < 3; k++) {
+                    a[k] = q * c[1][k] + (1 - q) * c[0][k];
+                    b[k] = q * c[2][k] + (1 - q) * c[3][k];
+                    a1[k] = p * b[k] + (1 - p) * a[k];
+                    a4[k] = p1 * b[k] + (1 - p1) * a[k];
+                    a[k] = q1 * c[1][k] + (1 - q1) * c[0][k];
+                    b[k] = q1 * c[2][k] + (1 - q1) * c[3][k];
+                    a2[k] = p * b[k] + (1 - p) * a[k];
+                    a3[k] = p1 * b[k] + (1 - p1) * a[k];
+                }
+                for (int k = 0; k < 3; k++) {
+                    y[3 * cnt + k] = (a1[k] + a2[k] + a3[k] + a4[k]) / 4.0;
+                    m1[k] = ((a2[k] + a3[k]) - (a1[k] + a4[k])) / 2.0;
+                    m2[k] = ((a3[k] + a4[k]) - (a1[k] + a2[k])) / 2.0;
+                }
+                vprod(m1, m2, rn);
+                w[cnt++] = vlen(rn);
+            }
+        }
+        return cnt;
+    }
+    double pv[3], qv[3], rv[3];
+    const double wt = tr_square(c[0], c[1], c[2]) / n / n;
+    for (int k = 0; k < 3; k++) {
+        pv[k] = (c[1][k] - c[0][k]) / n;
+        qv[k] = (c[2][k] - c[0][k]) / n;
+        rv[k] = (c[2][k] - c[1][k]) / n;
+    }
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n - i; j++) {
+            double B[3], AM[3], CM[3];
+            for (int k = 0; k < 3; k++) {
+                B[k] = c[0][k] + (i + 1) * pv[k] + j * qv[k];
+                AM[k] = B[k] - pv[k];
+                CM[k] = AM[k] + qv[k];
+                y[3 * cnt + k] = (B[k] + AM[k] + CM[k]) / 3.0;
+            }
+            w[cnt++] = wt;
+            if (j < n - 1 - i) {
+                for (int k = 0; k < 3; k++) y[3 * cnt + k] = (B[k] + (B[k] + rv[k]) + (B[k] + qv[k])) / 3.0;
+                w[cnt++] = wt;
+            }
+        }
+    }
+    return cnt;
+}
+
+/* Far-field pattern of the Stratton-Chu field.  No reference counterpart
+ * (the reference evaluates the field at finite points only, get_field.h:35-78);
+ * this is the R -> infinity limit of its far operators with the same cell
+ * quadrature:  F(d) = lim R e^{-ikR} E(R d), d a unit vector,
+ *   K_RotRot_Far  (kernel_lib.h:453-492): f1 -> k^2/r, f2 -> -k^2/r, so
+ *                 K[j] -> k^2/(4 pi) int e^{-ik d.y} (j - d (d.j)),
+ *   R_Rot_Colloc  (kernel_lib.h:295-313): (ikr-1) e^{ikr} (x-y)/r^3 -> ik d e^{ikr}/r,
+ *                 so R[j] -> ik/(4 pi) d x int e^{-ik d.y} j,
+ * with e^{ik|x-y|}/|x-y| = e^{ikR}/R e^{-ik d.y} (1 + O(1/R)).  Then
+ *   E_inf = multE K_inf[jE] - R_inf[jM],  H_inf = multH K_inf[jM] + R_inf[jE]
+ * (main0_field_test.cpp:313-319).  Einf, Hinf: [ndir][3] complex. */
+int orc_far_pattern(int nv, int ncell, const double* verts, const double* jE, const double* jM,
+                    double multE_re, double multE_im, double multH_re, double multH_im,
+                    double kre, double kim, int n_start, int ndir, const double* dirs, double* Einf,
+                    double* Hinf) {
+    if ((nv != 3 && nv != 4) || n_start < 1) return -1;
+    const int nn = n_start * n_start;
+    double* y = (double*)malloc(sizeof(double) * 3 * (size_t)nn * ncell);
+    double* w = (double*)malloc(sizeof(double) * (size_t)nn * ncell);
+    if (!y || !w) { free(y); free(w); return -2; }
+    for (int c = 0; c < ncell; c++)
+        cell_nodes(nv, (const double (*)[3])(verts + (size_t)c * nv * 3), n_start, y + 3 * (size_t)c * nn,
+                   w + (size_t)c * nn);
+    const cplx k = C(kre, kim), mE = C(multE_re, multE_im), mH = C(multH_re, multH_im);
+    const cplx ik = C(-kim, kre), k2 = cmul(k, k);
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < ndir; i++) {
+        const double* d = dirs + 3 * (size_t)i;
+        cplx Pe[3] = {{0, 0}, {0, 0}, {0, 0}}, Pm[3] = {{0, 0}, {0, 0}, {0, 0}}, je[3], jm[3];
+        for (int c = 0; c < ncell; c++) {
+            cplx S = C(0., 0.);
+            for (int q = 0; q < nn; q++) {
+                const double* yq = y + 3 * ((size_t)c * nn + q);
+                const double dy = d[0] * yq[0] + d[1] * yq[1] + d[2] * yq[2];
+                /* e^{-ik d.y} = e^{ki d.y} (cos(kr d.y) - i sin(kr d.y)) */
+                const double a = exp(kim * dy) * w[(size_t)c * nn + q];
+                S = cadd(S, C(a * cos(kre * dy), -a * sin(kre * dy)));
+            }
+            load_j(jE + 6 * (size_t)c, je);
+            for (int g = 0; g < 3; g++) Pe[g] = cadd(Pe[g], cmul(S, je[g]));
+            if (jM) {
+                load_j(jM + 6 * (size_t)c, jm);
+                for (int g = 0; g < 3; g++) Pm[g] = cadd(Pm[g], cmul(S, jm[g]));
+            }
+        }
+        cplx e[3], h[3], te[3], tm[3], xe[3], xm[3];
+        const cplx de = cadd(cadd(cmuld(Pe[0], d[0]), cmuld(Pe[1], d[1])), cmuld(Pe[2], d[2]));
+        const cplx dm = cadd(cadd(cmuld(Pm[0], d[0]), cmuld(Pm[1], d[1])), cmuld(Pm[2], d[2]));
+        for (int g = 0; g < 3; g++) {
+            te[g] = csub(Pe[g], cmuld(de, d[g]));   /* (I - d d) Pe */
+            tm[g] = csub(Pm[g], cmuld(dm, d[g]));
+        }
+        /* d x P */
+        xe[0] = csub(cmuld(Pe[2], d[1]), cmuld(Pe[1], d[2]));
+        xe[1] = csub(cmuld(Pe[0], d[2]), cmuld(Pe[2], d[0]));
+        xe[2] = csub(cmuld(Pe[1], d[0]), cmuld(Pe[0], d[1]));
+        xm[0] = csub(cmuld(Pm[2], d[1]), cmuld(Pm[1], d[2]));
+        xm[1] = csub(cmuld(Pm[0], d[2]), cmuld(Pm[2], d[0]));
+        xm[2] = csub(cmuld(Pm[1], d[0]), cmuld(Pm[0], d[1]));
+        for (int g = 0; g < 3; g++) {
+            e[g] = cdivd(csub(cmul(mE, cmul(k2, te[g])), cmul(ik, xm[g])), FOUR_PI);
+            h[g] = cdivd(cadd(cmul(mH, cmul(k2, tm[g])), cmul(ik, xe[g])), FOUR_PI);
+        }
+        put3(e, 3, Einf + 6 * (size_t)i);
+        if (Hinf) put3(h, 3, Hinf + 6 * (size_t)i);
+    }
+    free(y);
+    free(w);
+    return 0;
+}
+
 /* Collocation matrix of this library (the reference names the operators but
  * has no assembly loop, electrodynamics/K.h:156-178, R.h:65-88):
  *   Z[2i+a, 2j+b] = tau_i^a . Op(tau_j^b, rkt_i, cell_j)

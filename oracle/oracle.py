@@ -251,6 +251,25 @@ class Checker:
         assert rc == 0
         return _to_c(E).reshape(-1, 3), _to_c(H).reshape(-1, 3)
 
+    def far_pattern(self, verts, jE, k, dirs, jM=None, multE=1.0, multH=1.0, n_start=8):
+        """Far-field pattern (E_inf, H_inf) [ndir][3] at unit directions dirs."""
+        assert self.which == "oracle"
+        verts, pv = _d(verts); dirs, pd = _d(dirs)
+        jj, pj = _d(_cvec(jE, 3 * verts.shape[0]))
+        if jM is not None:
+            jm, pm = _d(_cvec(jM, 3 * verts.shape[0]))
+        else:
+            pm = None
+        E = np.zeros(dirs.shape[0] * 6); H = np.zeros(dirs.shape[0] * 6)
+        multE, multH, k = complex(multE), complex(multH), complex(k)
+        rc = self._f("far_pattern")(C.c_int(verts.shape[1]), C.c_int(verts.shape[0]), pv, pj, pm,
+                                    C.c_double(multE.real), C.c_double(multE.imag), C.c_double(multH.real),
+                                    C.c_double(multH.imag), C.c_double(k.real), C.c_double(k.imag),
+                                    C.c_int(n_start), C.c_int(dirs.shape[0]), pd, E.ctypes.data_as(_DP),
+                                    H.ctypes.data_as(_DP))
+        assert rc == 0
+        return _to_c(E).reshape(-1, 3), _to_c(H).reshape(-1, 3)
+
     def assemble(self, verts, np_, k, which=0, row0=0, nrows=None):
         assert self.which == "oracle"
         verts, pv = _d(verts)

@@ -187,3 +187,100 @@ def test_config1_scattered_field_vs_mie(orc):
     Em = _mie_pec_scattered(pts, float(g["k"]))
     err = np.linalg.norm(Es - Em) / np.linalg.norm(Em)
     assert err < 0.1, err
+
+
+# ---------------------------------------------------------------- far-field pattern
+def _mie_pec_far(dirs, k, a=1.0, nmax=30):
+    """Far-field amplitude F = lim R e^{-ikR} E_s(R d) of the PEC-sphere Mie
+    series for E_inc = x exp(ikz): F_theta = i cos(phi) S2 / k,
+    F_phi = -i sin(phi) S1 / k (the large-rho limit of _mie_pec_scattered:
+    h_n(rho) -> (-i)^{n+1} e^{i rho}/rho)."""
+    x = k * a
+    n = np.arange(1, nmax + 1)
+    psi = x * sp.spherical_jn(n, x)
+    dpsi = sp.spherical_jn(n, x) + x * sp.spherical_jn(n, x, derivative=True)
+    h = sp.spherical_jn(n, x) + 1j * sp.spherical_yn(n, x)
+    dh = sp.spherical_jn(n, x, derivative=True) + 1j * sp.spherical_yn(n, x, derivative=True)
+    an, bn = dpsi / (h + x * dh), psi / (x * h)
+    w = (2 * n + 1) / (n * (n + 1))
+    out = np.zeros((len(dirs), 3), dtype=complex)
+    for p, d in enumerate(dirs):
+        th, ph = np.arccos(np.clip(d[2], -1, 1)), np.arctan2(d[1], d[0])
+        mu = np.cos(th)
+        pi = np.zeros(nmax + 1); tau = np.zeros(nmax + 1)
+        pi[1] = 1.0
+        for m in range(2, nmax + 1):
+            pi[m] = (2 * m - 1) / (m - 1) * mu * pi[m - 1] - m / (m - 1) * pi[m - 2]
+        for m in range(1, nmax + 1):
+            tau[m] = m * mu * pi[m] - (m + 1) * pi[m - 1]
+        S1 = np.sum(w * (an * pi[1:] + bn * tau[1:]))
+        S2 = np.sum(w * (an * tau[1:] + bn * pi[1:]))
+        th_h = np.array([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
+        ph_h = np.array([-np.sin(ph), np.cos(ph), 0.0])
+        out[p] = 1j * np.cos(ph) * S2 / k * th_h - 1j * np.sin(ph) * S1 / k * ph_h
+    return out
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_mie_far_limit_formula():
+    """The closed-form Mie far amplitude is the limit of the Mie field."""
+    rng = np.random.default_rng(41)
+    dirs = _unit(rng, 12)
+    k = 2.0
+    R = 2e5
+    Ef = _mie_pec_scattered(R * dirs, k) * R * np.exp(-1j * k * R)
+    F = _mie_pec_far(dirs, k)
+    assert np.linalg.norm(Ef - F) <= 1e-4 * np.linalg.norm(F)
+
+
+@pytest.mark.parametrize("k", [2.0, 2.0 + 1e-4j])
+def test_far_pattern_is_the_large_R_limit_of_the_field(orc, k):
+    """R e^{-ikR} E(R d) of the reference-parity field converges to the pattern
+    as 1/R; Richardson on R and 2R removes the 1/R term."""
+    rng = np.random.default_rng(42)
+    verts = mesh.sphere_quads(6, 12, jitter=0.005, seed=4)
+    n = verts.shape[0]
+    jE = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    jM = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    dirs = _unit(rng, 10)
+    npp = O.num_param()
+    mE, mH = 0.7 - 0.2j, 1.3j
+    Fe, Fh = orc.far_pattern(verts, jE, k, dirs, jM=jM, multE=mE, multH=mH)
+    lim = []
+    for R in (1e4, 2e4):
+        E, H = orc.field_eh(verts, jE, k, npp, R * dirs, jM=jM, multE=mE, multH=mH)
+        f = R * np.exp(-1j * k * R)
+        lim.append((E * f, H * f))
+    Er = 2 * lim[1][0] - lim[0][0]
+    Hr = 2 * lim[1][1] - lim[0][1]
+    assert np.linalg.norm(Er - Fe) <= 1e-6 * np.linalg.norm(Fe)
+    assert np.linalg.norm(Hr - Fh) <= 1e-6 * np.linalg.norm(Fh)
+    # a wrong sign in either the K or the R limit fails by O(1)
+    assert np.linalg.norm(lim[1][0] - Fe) <= 1e-3 * np.linalg.norm(Fe)
+
+
+def test_far_pattern_vs_mie_and_optical_theorem(orc):
+    """Reference-example currents (config 1, ka = 2): pattern against the Mie
+    series (discretisation-limited), and the optical theorem
+    sigma_ext = 4 pi / k Im(e0 . F(k_inc)) = int |F|^2 dOmega (lossless PEC)."""
+    g = golden("config1.npz")
+    k = float(g["k"])
+    nt, nph = 61, 120
+    th = (np.arange(nt) + 0.5) * np.pi / nt
+    ph = (np.arange(nph) + 0.5) * 2 * np.pi / nph
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    F, _ = orc.far_pattern(g["verts"], g["jE"], k, dirs)
+    Fm = _mie_pec_far(dirs, k)
+    assert np.linalg.norm(F - Fm) <= 0.1 * np.linalg.norm(Fm)
+    dA = (np.pi / nt) * (2 * np.pi / nph) * np.sin(T).reshape(-1)
+    sig_sca = np.sum(np.sum(np.abs(F) ** 2, axis=1) * dA)
+    Ff, _ = orc.far_pattern(g["verts"], g["jE"], k, np.array([[0., 0., 1.]]))
+    sig_ext = 4 * np.pi / k * np.imag(np.conj(g["e0"]) @ Ff[0])
+    assert abs(sig_sca - sig_ext) <= 0.05 * sig_ext, (sig_sca, sig_ext)
+    sig_mie = np.sum(np.sum(np.abs(Fm) ** 2, axis=1) * dA)
+    assert abs(sig_sca - sig_mie) <= 0.1 * sig_mie
