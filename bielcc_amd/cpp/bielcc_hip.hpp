@@ -658,6 +658,79 @@ void SurfaceCurrents_byBasis(const SurfMesh<CellType>& mesh, const std::complex<
     for (int i = 0; i < n; i++) for (int g = 0; g < 3; g++) j_vec[i][g] = {o[6 * i + 2 * g], o[6 * i + 2 * g + 1]};
 }
 
+// ------------------------------------------------------------------ writers
+// Host-side output in the reference's file formats (io/io_vtk.h, io/io_aereco.h):
+// same layouts and the stream's default number formatting, so files written
+// here and by the reference are byte-identical for the same values.
+struct IO_VTK {
+    // IO_VTK::WritePntsWithField_C (io/io_vtk.cpp:482-551): POLYDATA points with
+    // real part, imaginary part, per-component magnitude and phase.
+    void WritePntsWithField_C(const std::string& filename, const std::vector<std::array<double, 3>>& points,
+                              const std::vector<std::array<std::complex<double>, 3>>& field_vectors) {
+        if (points.size() != field_vectors.size())
+            throw std::runtime_error("Points and field vectors must have same size");
+        std::ofstream f(filename);
+        if (!f.is_open()) throw std::runtime_error("Cannot open file: " + filename);
+        f << "# vtk DataFile Version 3.0\nPoints with complex vector field\nASCII\nDATASET POLYDATA\n";
+        f << "POINTS " << points.size() << " double\n";
+        for (const auto& p : points) f << p[0] << " " << p[1] << " " << p[2] << "\n";
+        f << "POINT_DATA " << field_vectors.size() << "\n";
+        f << "VECTORS real_part double\n";
+        for (const auto& v : field_vectors) f << v[0].real() << " " << v[1].real() << " " << v[2].real() << "\n";
+        f << "VECTORS image_part double\n";
+        for (const auto& v : field_vectors) f << v[0].imag() << " " << v[1].imag() << " " << v[2].imag() << "\n";
+        f << "VECTORS magnitude double\n";
+        for (const auto& v : field_vectors) f << std::abs(v[0]) << " " << std::abs(v[1]) << " " << std::abs(v[2]) << "\n";
+        const char* comp[3] = {"x", "y", "z"};
+        for (int g = 0; g < 3; g++) {
+            f << "SCALARS phase_" << comp[g] << " double 1\nLOOKUP_TABLE default\n";
+            for (const auto& v : field_vectors) f << std::arg(v[g]) << "\n";
+        }
+    }
+};
+
+struct IO_AERECO {
+    // IO_AERECO::WriteVecs_C (io/io_aereco.cpp:98-128): header "2 n/2", one
+    // vector per line, real and imaginary parts in two files.
+    void WriteVecs_C(const std::string& filenameReal, const std::string& filenameImage,
+                     const std::vector<std::array<std::complex<double>, 3>>& vals) {
+        std::ofstream re(filenameReal), im(filenameImage);
+        if (!re.is_open() || !im.is_open())
+            throw std::runtime_error("IO_AERECO::WriteVecs_C error. Can't open files:" + filenameReal + ", " +
+                                     filenameImage);
+        const int n = (int)vals.size();
+        re << 2 << " " << n / 2 << std::endl;
+        im << 2 << " " << n / 2 << std::endl;
+        for (const auto& v : vals) {
+            re << v[0].real() << " " << v[1].real() << " " << v[2].real() << std::endl;
+            im << v[0].imag() << " " << v[1].imag() << " " << v[2].imag() << std::endl;
+        }
+    }
+    // IO_AERECO::WriteField_C (io/io_aereco.cpp:135-173): header "n1 n1" with
+    // n1 = size/2 and n1^2 rows (real, imaginary, |v|).  The reference reads
+    // past the end of the vector when n1^2 > size; that case is an error here.
+    void WriteField_C(const std::string& filenameReal, const std::string& filenameImage,
+                      const std::string& filenameAbs, const std::vector<std::array<std::complex<double>, 3>>& field_val) {
+        const int n1 = (int)field_val.size() / 2;
+        if ((size_t)n1 * (size_t)n1 > field_val.size())
+            throw std::runtime_error("IO_AERECO::WriteField_C: (size/2)^2 rows exceed the values given");
+        std::ofstream re(filenameReal), im(filenameImage), ab(filenameAbs);
+        if (!ab.is_open() || !im.is_open() || !re.is_open())
+            throw std::runtime_error("IO_AERECO::WriteField_C error. Can't open files:" + filenameReal + ", " +
+                                     filenameImage + ", " + filenameAbs);
+        re << n1 << " " << n1 << std::endl;
+        im << n1 << " " << n1 << std::endl;
+        ab << n1 << " " << n1 << std::endl;
+        for (int i = 0; i < n1 * n1; i++) {
+            const auto& v = field_val[i];
+            re << v[0].real() << " " << v[1].real() << " " << v[2].real() << std::endl;
+            im << v[0].imag() << " " << v[1].imag() << " " << v[2].imag() << std::endl;
+            ab << std::sqrt(std::abs(v[0]) * std::abs(v[0]) + std::abs(v[1]) * std::abs(v[1]) +
+                            std::abs(v[2]) * std::abs(v[2])) << std::endl;
+        }
+    }
+};
+
 // ------------------------------------------------------------------ whole-problem drivers (new)
 // Solve the collocation system for a plane wave (kvec, e0) and return the
 // basis coefficients b (2N) - assembly, LU and solve stay on the device.

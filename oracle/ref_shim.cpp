@@ -20,6 +20,8 @@
 #include "element_geom.h"
 #include "cell.h"
 #include "surf_mesh.h"
+#include "io_vtk.h"
+#include "io_aereco.h"
 #include "kernel_param.h"
 #include "integral_param.h"
 #include "integral_universal.h"
@@ -477,6 +479,54 @@ int ref_read_mesh(const char* path, int nv, int cap, int* ncell, double* verts, 
         return -2;
     }
     return -1;
+}
+
+
+// The reference's writers, for golden output files: points + complex vectors
+// (IO_VTK::WritePntsWithField_C, io/io_vtk.cpp:482-551), complex vectors as
+// two aereco files (IO_AERECO::WriteVecs_C, io/io_aereco.cpp:98-128) and the
+// complex field as three aereco files (IO_AERECO::WriteField_C, :135-173).
+static std::vector<std::array<cd, 3>> load_cvecs(int n, const double* v) {
+    std::vector<std::array<cd, 3>> out(n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) out[i][d] = cd(v[6 * i + 2 * d], v[6 * i + 2 * d + 1]);
+    return out;
+}
+
+int ref_write_vtk_field_c(const char* path, int n, const double* pts, const double* vals) {
+    std::vector<std::array<double, 3>> p(n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) p[i][d] = pts[3 * i + d];
+    try {
+        IO_VTK io;
+        io.WritePntsWithField_C(path, p, load_cvecs(n, vals));
+    } catch (...) {
+        return -2;
+    }
+    return 0;
+}
+
+int ref_write_aereco_vecs_c(const char* path_re, const char* path_im, int n, const double* vals) {
+    try {
+        IO_AERECO io;
+        io.WriteVecs_C(path_re, path_im, load_cvecs(n, vals));
+    } catch (...) {
+        return -2;
+    }
+    return 0;
+}
+
+int ref_write_aereco_field_c(const char* path_re, const char* path_im, const char* path_abs, int n,
+                             const double* vals) {
+    // the reference writes (n/2)^2 rows: only call it where that stays inside the n values
+    if ((long long)(n / 2) * (n / 2) > n) return -1;
+    try {
+        IO_AERECO io;
+        io.WriteField_C(path_re, path_im, path_abs, load_cvecs(n, vals));
+    } catch (...) {
+        return -2;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -255,9 +255,45 @@ def gen_mesh_files(ref):
     print("mesh files:", [k for k in out if k.endswith("_verts")])
 
 
+def writer_values(rng, n):
+    """points and complex 3-vectors with the formatting edge cases of ostream's
+    default (%g, 6 digits): exponents, exact zeros, negative zero, rounding."""
+    pts = rng.normal(size=(n, 3)) * np.array([1.0, 1e-7, 12345.678])
+    v = (rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))) * np.logspace(-12, 6, n)[:, None]
+    v[0] = [0.0, -0.0 + 1j * 0.0, 1.0 - 1j]
+    v[1] = [1234567.0 + 0.5j, -0.000123456789 - 1e-300j, 3.0 + 4.0j]
+    return pts, v
+
+
+def gen_writers(ref):
+    """Files written by the reference's own writers (IO_VTK::WritePntsWithField_C,
+    IO_AERECO::WriteVecs_C and ::WriteField_C) for byte-level comparison."""
+    import ctypes as C
+    rng = np.random.default_rng(61)
+    out = os.path.join(HERE, "writers")
+    os.makedirs(out, exist_ok=True)
+    pts, v = writer_values(rng, 9)
+    vals = np.ascontiguousarray(np.stack([v.real, v.imag], -1).reshape(-1))
+    P = np.ascontiguousarray(pts.reshape(-1))
+    dp = C.POINTER(C.c_double)
+    lib = ref.lib
+    assert lib.ref_write_vtk_field_c(os.path.join(out, "field_c.vtk").encode(), 9, P.ctypes.data_as(dp),
+                                     vals.ctypes.data_as(dp)) == 0
+    assert lib.ref_write_aereco_vecs_c(os.path.join(out, "vecs_real.gv").encode(),
+                                       os.path.join(out, "vecs_image.gv").encode(), 9,
+                                       vals.ctypes.data_as(dp)) == 0
+    # WriteField_C writes (n/2)^2 rows: n = 4 is the largest size it reads in bounds
+    assert lib.ref_write_aereco_field_c(os.path.join(out, "field_real.gv").encode(),
+                                        os.path.join(out, "field_image.gv").encode(),
+                                        os.path.join(out, "field_abs.gdr").encode(), 4,
+                                        vals.ctypes.data_as(dp)) == 0
+    np.savez(os.path.join(out, "inputs.npz"), pts=pts, vals=v)
+    print("writers:", sorted(os.listdir(out)))
+
+
 if __name__ == "__main__":
     ref = O.Checker("ref")
-    which = sys.argv[1:] or ["ops", "integrators", "kernels", "config1", "config1_tri", "mesh_files"]
+    which = sys.argv[1:] or ["ops", "integrators", "kernels", "config1", "config1_tri", "mesh_files", "writers"]
     if "ops" in which:
         gen_ops(ref)
     if "integrators" in which:
@@ -270,3 +306,5 @@ if __name__ == "__main__":
         gen_config1(ref, 3)
     if "mesh_files" in which:
         gen_mesh_files(ref)
+    if "writers" in which:
+        gen_writers(ref)
