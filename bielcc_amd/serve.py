@@ -1,0 +1,399 @@
+"""Serving loop for scattering queries on one GPU.
+
+A query is (mesh, k, incidence directions with polarisations, observation
+points); the answer is the scattered-plus-incident E and H at the points for
+every incidence (one current set per direction).  The reference has no
+solver or server; this is the request loop around the path:
+
+- continuous batching: worker threads, each with its own library context
+  (own HIP stream, scratch and BLAS handle), take queries from one FIFO queue
+  as soon as they are free, so queries that arrive while others run are
+  admitted as slots free up and their kernels overlap on the device;
+- resident state: meshes stay uploaded per worker; the factors of the
+  collocation matrix stay on the device per (mesh, k) in an LRU cache
+  (FP64 LU in the "fp64" form; the FP64 matrix plus its FP32 LU in the
+  "mixed" form, whose solves refine in FP64 to `refine_tol`);
+- admission by HBM budget: factors and every query's buffers (right-hand
+  sides, currents, points, fields and the field driver's scratch) are
+  reserved from one byte budget; a query waits while its bytes do not fit,
+  and idle factors are evicted least-recently-used first;
+- metrics: per-query latency (arrival to result), p50 / p99, solves/s.
+
+The library calls release the GIL (ctypes), so workers drive their streams
+concurrently.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import queue
+import threading
+import time
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import api
+
+
+@dataclasses.dataclass
+class Query:
+    """One scattering query.  dirs [nrhs][3] unit propagation directions,
+    pols [nrhs][3] polarisations (orthogonal to dirs), points [M][3]."""
+    mesh: str
+    k: float
+    dirs: np.ndarray
+    pols: np.ndarray
+    points: np.ndarray
+    qid: int = 0
+    arrival: float = 0.0
+
+    @property
+    def nrhs(self) -> int:
+        return int(np.asarray(self.dirs).reshape(-1, 3).shape[0])
+
+
+@dataclasses.dataclass
+class Result:
+    qid: int
+    E: np.ndarray          # [nrhs][M][3] complex, incident field included
+    H: np.ndarray          # [nrhs][M][3] complex, scattered only
+    arrival: float
+    start: float
+    end: float
+    factored: bool         # this query built the factors it used
+    refine_iters: int = 0
+    refine_resid: float = 0.0
+
+    @property
+    def latency(self) -> float:
+        return self.end - self.arrival
+
+
+class Budget:
+    """Device bytes reserved against a fixed budget, with eviction of idle
+    cache entries when a reservation does not fit."""
+
+    def __init__(self, total: int):
+        self.total = int(total)
+        self.used = 0
+        self.cv = threading.Condition()
+
+    def reserve(self, nbytes: int, evict: Callable[[int], int], timeout: float = 600.0) -> None:
+        """Block until nbytes fit; evict(need) frees idle cache bytes and returns
+        how many it freed.  Raises if nbytes exceeds the whole budget."""
+        nbytes = int(nbytes)
+        if nbytes > self.total:
+            raise MemoryError(f"request of {nbytes} B exceeds the HBM budget of {self.total} B")
+        deadline = time.monotonic() + timeout
+        with self.cv:
+            while self.used + nbytes > self.total:
+                freed = evict(self.used + nbytes - self.total)
+                if freed > 0:
+                    self.used -= freed
+                    continue
+                left = deadline - time.monotonic()
+                if left <= 0:
+                    raise TimeoutError("HBM budget: reservation did not fit in time")
+                self.cv.wait(timeout=min(left, 1.0))
+            self.used += nbytes
+
+    def release(self, nbytes: int) -> None:
+        with self.cv:
+            self.used -= int(nbytes)
+            self.cv.notify_all()
+
+
+class _Entry:
+    def __init__(self, key, nbytes):
+        self.key = key
+        self.nbytes = nbytes
+        self.ready = False
+        self.failed: Optional[BaseException] = None
+        self.refs = 0
+        self.bufs: dict = {}
+        self.owner = None        # worker context that allocated the buffers
+
+
+class FactorCache:
+    """Device-resident factors per (mesh, k) with reference counts and LRU
+    eviction of entries nobody uses.  Must be used under Budget.cv."""
+
+    def __init__(self, budget: Budget):
+        self.budget = budget
+        self.entries: "collections.OrderedDict[tuple, _Entry]" = collections.OrderedDict()
+        self.evictions = 0
+
+    def evict(self, need: int) -> int:
+        """Free idle entries (least recently used first) until `need` bytes are
+        freed or none is left; returns the bytes freed.  Caller holds budget.cv."""
+        freed = 0
+        for key in list(self.entries.keys()):
+            if freed >= need:
+                break
+            e = self.entries[key]
+            if e.refs == 0 and e.ready:
+                for b in e.bufs.values():
+                    b.free()
+                del self.entries[key]
+                freed += e.nbytes
+                self.evictions += 1
+        return freed
+
+    def pin(self, key, nbytes: int) -> tuple:
+        """Pin the entry of key (creating it, not yet built, if absent).
+        Returns (entry, builder): builder is True for the caller that created
+        it and must reserve its bytes and build it."""
+        with self.budget.cv:
+            e = self.entries.get(key)
+            if e is not None:
+                e.refs += 1
+                self.entries.move_to_end(key)
+                return e, False
+            e = _Entry(key, nbytes)
+            e.refs = 1
+            self.entries[key] = e
+            return e, True
+
+    def wait_ready(self, e: _Entry, timeout: float = 1200.0) -> None:
+        """Wait (holding no budget bytes) until another caller has built e.
+        The caller keeps its pin either way and releases it."""
+        cv = self.budget.cv
+        deadline = time.monotonic() + timeout
+        with cv:
+            while not e.ready and e.failed is None:
+                left = deadline - time.monotonic()
+                if left <= 0:
+                    raise TimeoutError(f"factors of {e.key} not ready in time")
+                cv.wait(timeout=min(left, 1.0))
+            if e.failed is not None:
+                raise RuntimeError(f"factorisation of {e.key} failed") from e.failed
+
+    def finish(self, e: _Entry, exc: Optional[BaseException] = None) -> None:
+        """Builder: mark e ready, or failed (its buffers freed, the entry dropped;
+        the builder returns the entry's bytes to the budget itself)."""
+        cv = self.budget.cv
+        with cv:
+            if exc is None:
+                e.ready = True
+            else:
+                e.failed = exc
+                for b in e.bufs.values():
+                    b.free()
+                e.bufs = {}
+                if self.entries.get(e.key) is e:
+                    del self.entries[e.key]
+            cv.notify_all()
+
+    def release(self, e: _Entry) -> None:
+        with self.budget.cv:
+            e.refs -= 1
+            self.budget.cv.notify_all()
+
+
+def _field_scratch_bytes(M: int, N: int, nrhs: int) -> int:
+    """Upper bound of the field driver's partial sums (bielcc_field_batch)."""
+    bx = (M + 255) // 256
+    nsplit = max(1, min((2048 + bx - 1) // bx, (N + 63) // 64))
+    return nsplit * nrhs * 12 * M * 8 + 64 * M
+
+
+class Server:
+    """Continuous-batching server over a pool of meshes.
+
+    meshes: name -> vertex array [n][3|4][3].  form: "fp64" (FP64 LU) or
+    "mixed" (FP32 LU + FP64 refinement, FP32 far-pair field sums).
+    """
+
+    def __init__(self, meshes: dict, form: str = "mixed", workers: int = 3, hbm_budget: float = 200e9,
+                 num_param=None, refine_tol: float = 1e-10, device: int = 0):
+        if form not in ("fp64", "mixed"):
+            raise ValueError("form must be 'fp64' or 'mixed'")
+        self.meshes = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in meshes.items()}
+        self.form = form
+        self.np = num_param if num_param is not None else api.num_param()
+        self.refine_tol = refine_tol
+        self.budget = Budget(int(hbm_budget))
+        self.cache = FactorCache(self.budget)
+        self.q: "queue.Queue[Optional[Query]]" = queue.Queue()
+        self.results: dict = {}
+        self.errors: dict = {}
+        self.lock = threading.Lock()
+        self.done_cv = threading.Condition(self.lock)
+        self.ctxs = [api.Context(device) for _ in range(workers)]
+        self.dev_meshes = [dict() for _ in range(workers)]
+        self.threads = [threading.Thread(target=self._run, args=(w,), daemon=True) for w in range(workers)]
+        for t in self.threads:
+            t.start()
+
+    # -- public
+    def submit(self, q: Query) -> None:
+        if q.arrival == 0.0:
+            q.arrival = time.perf_counter()
+        self.q.put(q)
+
+    def wait(self, qids, timeout: float = 1200.0) -> dict:
+        deadline = time.monotonic() + timeout
+        with self.done_cv:
+            while not all(i in self.results or i in self.errors for i in qids):
+                left = deadline - time.monotonic()
+                if left <= 0:
+                    raise TimeoutError("queries did not finish in time")
+                self.done_cv.wait(timeout=min(left, 1.0))
+        bad = {i: self.errors[i] for i in qids if i in self.errors}
+        if bad:
+            raise RuntimeError(f"queries failed: {bad}")
+        return {i: self.results[i] for i in qids}
+
+    def close(self) -> None:
+        for _ in self.threads:
+            self.q.put(None)
+        for t in self.threads:
+            t.join(timeout=600)
+        with self.budget.cv:
+            for e in list(self.cache.entries.values()):
+                for b in e.bufs.values():
+                    b.free()
+            self.cache.entries.clear()
+        for m in self.dev_meshes:
+            for dm in m.values():
+                dm.close()
+        for c in self.ctxs:
+            c.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- worker
+    def _mesh(self, w: int, name: str) -> api.Mesh:
+        m = self.dev_meshes[w].get(name)
+        if m is None:
+            m = self.ctxs[w].mesh(self.meshes[name])
+            self.dev_meshes[w][name] = m
+        return m
+
+    def _factor_bytes(self, n: int) -> int:
+        n2 = 2 * n
+        per = 16 + 8 if self.form == "mixed" else 16   # FP64 matrix (+ FP32 factors)
+        return per * n2 * n2 + 4 * n2
+
+    def _build(self, w: int, m: api.Mesh, k: float, e: _Entry) -> None:
+        ctx = self.ctxs[w]
+        n2 = 2 * m.n
+        e.owner = ctx
+        e.bufs["Z"] = ctx.empty(n2 * n2 * 16)
+        e.bufs["piv"] = ctx.empty(n2 * 4)
+        m.assemble_into(e.bufs["Z"], n2, self.np, k)
+        if self.form == "mixed":
+            e.bufs["F"] = ctx.empty(n2 * n2 * 8)
+            ctx.lu32_factor(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"])
+        else:
+            ctx.lu_factor(e.bufs["Z"], n2, n2, e.bufs["piv"])
+        ctx.sync()
+
+    def _run(self, w: int) -> None:
+        ctx = self.ctxs[w]
+        while True:
+            q = self.q.get()
+            if q is None:
+                return
+            try:
+                res = self._serve(w, ctx, q)
+                with self.done_cv:
+                    self.results[q.qid] = res
+                    self.done_cv.notify_all()
+            except BaseException as exc:   # report, keep serving
+                with self.done_cv:
+                    self.errors[q.qid] = repr(exc)
+                    self.done_cv.notify_all()
+
+    def _serve(self, w: int, ctx: api.Context, q: Query) -> Result:
+        start = time.perf_counter()
+        m = self._mesh(w, q.mesh)
+        n, n2 = m.n, 2 * m.n
+        dirs = np.asarray(q.dirs, dtype=np.float64).reshape(-1, 3)
+        pols = np.asarray(q.pols, dtype=np.float64).reshape(-1, 3)
+        pts = np.ascontiguousarray(q.points, dtype=np.float64).reshape(-1, 3)
+        nrhs, M = dirs.shape[0], pts.shape[0]
+        work = n2 * nrhs * 16 + n * nrhs * 48 + M * 24 + 2 * nrhs * M * 48 + _field_scratch_bytes(M, n, nrhs)
+        fbytes = self._factor_bytes(n)
+        e, builder = self.cache.pin((q.mesh, float(q.k)), fbytes)
+        reserved = 0
+        try:
+            if builder:
+                # factors and workspace in one reservation: a query waits holding nothing
+                try:
+                    self.budget.reserve(fbytes + work, self.cache.evict)
+                except BaseException as exc:
+                    self.cache.finish(e, exc)
+                    raise
+                reserved = work
+                try:
+                    self._build(w, m, q.k, e)
+                except BaseException as exc:
+                    self.cache.finish(e, exc)
+                    self.budget.release(fbytes)
+                    raise
+                self.cache.finish(e)
+            else:
+                self.cache.wait_ready(e)
+                self.budget.reserve(work, self.cache.evict)
+                reserved = work
+            kvec = q.k * dirs
+            dB = ctx.empty(n2 * nrhs * 16)
+            dJ = ctx.empty(n * nrhs * 48)
+            dP = ctx.to_device(pts)
+            dE, dH = ctx.empty(nrhs * M * 48), ctx.empty(nrhs * M * 48)
+            m.rhs_planewave_into(dB, n2, kvec, pols)
+            iters, resid = 0, 0.0
+            if self.form == "mixed":
+                iters, resid = ctx.refine_solve(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], dB, nrhs, n2,
+                                                self.refine_tol, 30)
+            else:
+                ctx.lu_solve(e.bufs["Z"], n2, n2, e.bufs["piv"], dB, nrhs, n2)
+            m.currents_into(dB, n2, nrhs, dJ)
+            inc = np.concatenate([kvec, pols], axis=1)
+            prec = api.PREC_MIXED if self.form == "mixed" else api.PREC_FP64
+            m.field_batch_into(self.np, q.k, nrhs, dJ, dP, M, dE, dH, inc=inc, precision=prec)
+            E = dE.to_numpy((nrhs, M, 3), np.complex128)
+            H = dH.to_numpy((nrhs, M, 3), np.complex128)
+            for d in (dB, dJ, dP, dE, dH):
+                d.free()
+        finally:
+            if e.ready or not builder:
+                self.cache.release(e)
+            if reserved:
+                self.budget.release(reserved)
+        return Result(q.qid, E, H, q.arrival, start, time.perf_counter(), builder, iters, resid)
+
+
+def percentiles(lat) -> dict:
+    lat = np.sort(np.asarray(lat, dtype=np.float64))
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}
+
+
+def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
+    """Submit queries at Poisson arrival times (rate qps), wait for all, and
+    return results with latency percentiles and throughput."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, size=len(queries))
+    t0 = time.perf_counter()
+    t = t0
+    for q, g in zip(queries, gaps):
+        t += g
+        now = time.perf_counter()
+        if t > now:
+            time.sleep(t - now)
+        q.arrival = time.perf_counter()
+        server.submit(q)
+    res = server.wait([q.qid for q in queries])
+    t1 = max(r.end for r in res.values())
+    solves = sum(q.nrhs for q in queries)
+    out = {"queries": len(queries), "solves": solves, "wall_s": t1 - t0, "solves_per_s": solves / (t1 - t0),
+           "factorisations": sum(r.factored for r in res.values()), "evictions": server.cache.evictions}
+    out.update(percentiles([r.latency for r in res.values()]))
+    return {"results": res, "stats": out}
