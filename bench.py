@@ -171,6 +171,107 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
             "err_field_E_max_rel_fp64_currents": rowrel(Eq, E64), "err_field_H_max_rel_fp64_currents": rowrel(Hq, H64)}
 
 
+def serving_pool(seed=5, n=16, lo=1000, hi=6000):
+    """Seeded pool of closed meshes: spheres, triaxial ellipsoids (semi-axes in
+    [0.5, 1.5]), tori (R = 1, r in [0.2, 0.4]) and cubes of side 2, lo..hi cells."""
+    from bielcc_amd import mesh
+    rng = np.random.default_rng(seed)
+    pool = {}
+    for i in range(n):
+        kind = ("sphere", "ellipsoid", "torus", "cube")[i % 4]
+        target = int(rng.uniform(lo, hi))
+        if kind == "sphere":
+            nt = max(4, int(round(np.sqrt(target / 2))))
+            v = mesh.sphere_quads(nt, 2 * nt, jitter=0.005, seed=100 + i)
+        elif kind == "ellipsoid":
+            nt = max(4, int(round(np.sqrt(target / 2))))
+            v = mesh.ellipsoid_quads(nt, 2 * nt, axes=tuple(rng.uniform(0.5, 1.5, 3)), seed=100 + i)
+        elif kind == "torus":
+            nv = max(4, int(round(np.sqrt(target / 3))))
+            v = mesh.torus_quads(3 * nv, nv, R=1.0, r=float(rng.uniform(0.2, 0.4)))
+        else:
+            v = mesh.cube_quads(max(2, int(round(np.sqrt(target / 6)))), side=2.0)
+        pool[f"{kind}{i}"] = v
+    return pool
+
+
+def serving_queries(pool, nq, seed=6, m_lo=1000, m_hi=20000):
+    """ka uniform in [1, 15] quantised to 8 levels, 1-16 incidences uniform on
+    the sphere with orthogonal polarisations, m_lo..m_hi points on a sphere of
+    radius uniform in [3, 5]."""
+    from bielcc_amd import mesh, serve
+    rng = np.random.default_rng(seed)
+    names = sorted(pool)
+    levels = 1.0 + 14.0 * (np.arange(8) + 0.5) / 8
+    qs = []
+    for i in range(nq):
+        nrhs = int(rng.integers(1, 17))
+        d = rng.normal(size=(nrhs, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        p = np.cross(d, rng.normal(size=(nrhs, 3)))
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        M = int(rng.integers(m_lo, m_hi + 1))
+        pts = mesh.sphere_points(M, float(rng.uniform(3.0, 5.0)), seed=int(rng.integers(1 << 30)))
+        qs.append(serve.Query(names[int(rng.integers(len(names)))], float(levels[int(rng.integers(8))]), d, p, pts,
+                              qid=i))
+    return qs
+
+
+def serving_leg(nq=96, workers=4):
+    """The same query stream served in the FP64 form and in the mixed form
+    (FP32 LU + FP64 refinement to 1e-13, FP32 far sums), HBM budget 200 GB:
+    once as a burst (every query queued at once: capacity in solves/s) and once
+    as a Poisson stream at 60% of the FP64 form's capacity (latency)."""
+    from bielcc_amd import serve
+    pool = serving_pool()
+    cells = [v.shape[0] for v in pool.values()]
+    out = {"workload": f"{len(pool)} meshes ({min(cells)}-{max(cells)} cells: spheres, ellipsoids, tori, cubes), "
+                       f"{nq} queries, ka in [1,15] on 8 levels, 1-16 incidences, 1,000-20,000 points at "
+                       f"r in [3,5]; {workers} workers; HBM budget 200 GB; mixed form refines to 1e-13"}
+    res = {}
+    rate = None
+    for mode in ("burst", "stream"):
+        out[mode] = {}
+        for form in ("fp64", "mixed"):
+            qs = serving_queries(pool, nq)
+            qps = 1e6 if mode == "burst" else rate
+            with serve.Server(pool, form=form, workers=workers, hbm_budget=200e9, refine_tol=1e-13) as s:
+                run = serve.poisson_run(s, qs, qps=qps, seed=7)
+            st = run["stats"]
+            log(f"[bench] serving {mode} {form}: {st['wall_s']:.1f} s, {st['solves_per_s']:.1f} solves/s, "
+                f"p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s, fallbacks {st['fp64_fallbacks']}")
+            out[mode][form] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in st.items()}
+            if mode == "burst":
+                res[form] = run["results"]
+        if mode == "burst":
+            rate = 0.6 * nq / out["burst"]["fp64"]["wall_s"]
+            out["stream_qps"] = round(rate, 3)
+    # error of the scattered field (the total field nearly vanishes at shadow points)
+    qs = {q.qid: q for q in serving_queries(pool, nq)}
+    err, glob, worst = 0.0, 0.0, None
+    for i, r64 in res["fp64"].items():
+        r32, q = res["mixed"][i], qs[i]
+        for r in range(r64.E.shape[0]):
+            einc = q.pols[r][None, :] * np.exp(1j * q.k * (q.points @ q.dirs[r]))[:, None]
+            es = r64.E[r] - einc
+            nes = np.linalg.norm(es, axis=1)
+            # per point, with a floor of 1e-3 of the set's largest scattered field (nulls)
+            e = np.linalg.norm(r32.E[r] - r64.E[r], axis=1) / np.maximum(nes, 1e-3 * nes.max())
+            glob = max(glob, float(np.linalg.norm(r32.E[r] - r64.E[r]) / np.linalg.norm(es)))
+            if e.max() > err:
+                err = float(e.max())
+                worst = {"query": i, "mesh": q.mesh, "k": q.k, "fallback": r32.fp64_fallback,
+                         "refine_resid": r32.refine_resid, "refine_iters": r32.refine_iters,
+                         "set_rel_err": float(np.linalg.norm(r32.E[r] - r64.E[r]) / np.linalg.norm(es))}
+    out["err_scattered_E_max_rel_mixed_vs_fp64"] = err
+    out["err_scattered_E_set_max_rel_mixed_vs_fp64"] = glob
+    out["worst"] = worst
+    out["capacity_solves_per_s_ratio"] = round(out["burst"]["mixed"]["solves_per_s"] /
+                                               out["burst"]["fp64"]["solves_per_s"], 3)
+    out["stream_p99_ratio"] = round(out["stream"]["fp64"]["p99_s"] / out["stream"]["mixed"]["p99_s"], 3)
+    return out
+
+
 def copy_calibration(nbytes=1 << 30):
     """A plain device copy of known size (torch), so that rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE can be scaled against it in the same run."""
@@ -194,6 +295,8 @@ def main():
     ap.add_argument("--no-lu", action="store_true", help="skip the LU (currents from the right-hand side)")
     ap.add_argument("--copy-calib", action="store_true", help="run a 1 GiB device copy first (counter scale)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-queries", type=int, default=96)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path is run on one MI355X; multi-GPU sharding is a design "
@@ -290,6 +393,10 @@ def main():
     if lu_s is not None and not args.no_mixed:
         out["precision_form"] = mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ, dE, dH, lu_s, solve_s,
                                            elapsed / args.steps)
+    if not args.no_serve:
+        for d in (dZ, dB, dpiv):
+            d.free()
+        out["serving"] = serving_leg(nq=args.serve_queries)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(verts, pts, k, args.cpu_budget)

@@ -156,6 +156,17 @@ class Context:
         return self.lib.bielcc_stream(self.h)
 
     # -- memory
+    def upload(self, dst: DeviceArray, a) -> None:
+        """Host array into the front of an existing device buffer (synchronous)."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes > dst.nbytes:
+            raise ValueError("array larger than the device buffer")
+        self._check(self.lib.bielcc_memcpy(self.h, C.c_void_p(dst.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, 0))
+
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
+        """Device-to-device copy of nbytes (synchronous)."""
+        self._check(self.lib.bielcc_memcpy(self.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), int(nbytes), 2))
+
     def to_device(self, a) -> DeviceArray:
         return DeviceArray.from_numpy(self, a)
 

@@ -12,11 +12,19 @@ solver or server; this is the request loop around the path:
 - resident state: meshes stay uploaded per worker; the factors of the
   collocation matrix stay on the device per (mesh, k) in an LRU cache
   (FP64 LU in the "fp64" form; the FP64 matrix plus its FP32 LU in the
-  "mixed" form, whose solves refine in FP64 to `refine_tol`);
-- admission by HBM budget: factors and every query's buffers (right-hand
-  sides, currents, points, fields and the field driver's scratch) are
-  reserved from one byte budget; a query waits while its bytes do not fit,
-  and idle factors are evicted least-recently-used first;
+  "mixed" form, whose solves refine in FP64 to `refine_tol`).  Systems too
+  ill-conditioned for FP32 factors (near an interior resonance of the closed
+  body, where refinement stalls or the FP32 LU meets a zero pivot) get FP64
+  factors once, beside the FP32 ones, and are solved with them from then on;
+  their fields are summed in FP64 too (the currents then carry a large
+  non-radiating part whose exterior field cancels);
+- admission by HBM budget: factors, every query's buffers (right-hand
+  sides, currents, points, fields and the field driver's scratch) and the
+  workers' grow-only buffer pools are reserved from one byte budget; a query
+  waits while its bytes do not fit, and idle factors are evicted
+  least-recently-used first.  Query buffers come from the worker's pool, so
+  serving a query allocates and frees nothing on the device once the pool
+  has grown (a device free would stall every other worker's stream);
 - metrics: per-query latency (arrival to result), p50 / p99, solves/s.
 
 The library calls release the GIL (ctypes), so workers drive their streams
@@ -64,6 +72,7 @@ class Result:
     factored: bool         # this query built the factors it used
     refine_iters: int = 0
     refine_resid: float = 0.0
+    fp64_fallback: bool = False   # mixed form: solved with FP64 factors
 
     @property
     def latency(self) -> float:
@@ -113,6 +122,7 @@ class _Entry:
         self.refs = 0
         self.bufs: dict = {}
         self.owner = None        # worker context that allocated the buffers
+        self.upgrade = threading.Lock()   # one FP64 fallback factorisation at a time
 
 
 class FactorCache:
@@ -222,6 +232,8 @@ class Server:
         self.done_cv = threading.Condition(self.lock)
         self.ctxs = [api.Context(device) for _ in range(workers)]
         self.dev_meshes = [dict() for _ in range(workers)]
+        self.pools = [dict() for _ in range(workers)]   # name -> DeviceArray, grow-only
+        self.scratch_hw = [0] * workers   # bytes charged for the context's own grow-only scratch
         self.threads = [threading.Thread(target=self._run, args=(w,), daemon=True) for w in range(workers)]
         for t in self.threads:
             t.start()
@@ -258,6 +270,9 @@ class Server:
         for m in self.dev_meshes:
             for dm in m.values():
                 dm.close()
+        for pool in self.pools:
+            for d in pool.values():
+                d.free()
         for c in self.ctxs:
             c.close()
 
@@ -275,6 +290,22 @@ class Server:
             self.dev_meshes[w][name] = m
         return m
 
+    def _buf(self, w: int, name: str, nbytes: int) -> api.DeviceArray:
+        """Worker w's pooled buffer `name` of at least nbytes; growth is charged
+        to the budget for good."""
+        pool = self.pools[w]
+        d = pool.get(name)
+        if d is not None and d.nbytes >= nbytes:
+            return d
+        grow = int(nbytes * 1.25) + 4096
+        self.budget.reserve(grow, self.cache.evict)
+        if d is not None:
+            d.free()
+            self.budget.release(d.nbytes)
+        d = self.ctxs[w].empty(grow)
+        pool[name] = d
+        return d
+
     def _factor_bytes(self, n: int) -> int:
         n2 = 2 * n
         per = 16 + 8 if self.form == "mixed" else 16   # FP64 matrix (+ FP32 factors)
@@ -289,10 +320,34 @@ class Server:
         m.assemble_into(e.bufs["Z"], n2, self.np, k)
         if self.form == "mixed":
             e.bufs["F"] = ctx.empty(n2 * n2 * 8)
-            ctx.lu32_factor(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"])
+            try:
+                ctx.lu32_factor(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"])
+            except api.BielccError as err:
+                if err.code != 5:          # only a zero FP32 pivot is handled here
+                    raise
+                ctx.sync()
+                self._upgrade(w, e)
         else:
             ctx.lu_factor(e.bufs["Z"], n2, n2, e.bufs["piv"])
         ctx.sync()
+
+    def _upgrade(self, w: int, e: _Entry) -> None:
+        """FP64 factors beside the FP32 ones (mixed form), for a system FP32
+        factors cannot refine.  Done once per entry."""
+        with e.upgrade:
+            if "LU64" in e.bufs:
+                return
+            ctx = self.ctxs[w]
+            n2 = int(round(np.sqrt(e.bufs["Z"].nbytes / 16)))
+            extra = 16 * n2 * n2 + 4 * n2
+            self.budget.reserve(extra, self.cache.evict)
+            lu, piv = ctx.empty(16 * n2 * n2), ctx.empty(4 * n2)
+            ctx.copy(lu, e.bufs["Z"], 16 * n2 * n2)
+            ctx.lu_factor(lu, n2, n2, piv)
+            ctx.sync()
+            with self.budget.cv:
+                e.bufs["LU64"], e.bufs["piv64"] = lu, piv
+                e.nbytes += extra
 
     def _run(self, w: int) -> None:
         ctx = self.ctxs[w]
@@ -318,19 +373,22 @@ class Server:
         pols = np.asarray(q.pols, dtype=np.float64).reshape(-1, 3)
         pts = np.ascontiguousarray(q.points, dtype=np.float64).reshape(-1, 3)
         nrhs, M = dirs.shape[0], pts.shape[0]
-        work = n2 * nrhs * 16 + n * nrhs * 48 + M * 24 + 2 * nrhs * M * 48 + _field_scratch_bytes(M, n, nrhs)
+        # the field driver's scratch is context-owned and grow-only: charge its high-water mark
+        grow = max(0, _field_scratch_bytes(M, n, nrhs) - self.scratch_hw[w])
+        if grow:
+            self.budget.reserve(grow, self.cache.evict)
+            self.scratch_hw[w] += grow
         fbytes = self._factor_bytes(n)
         e, builder = self.cache.pin((q.mesh, float(q.k)), fbytes)
-        reserved = 0
         try:
             if builder:
-                # factors and workspace in one reservation: a query waits holding nothing
+                # the factors' bytes are reserved before anything is built; waiting
+                # callers of the same key hold no bytes
                 try:
-                    self.budget.reserve(fbytes + work, self.cache.evict)
+                    self.budget.reserve(fbytes, self.cache.evict)
                 except BaseException as exc:
                     self.cache.finish(e, exc)
                     raise
-                reserved = work
                 try:
                     self._build(w, m, q.k, e)
                 except BaseException as exc:
@@ -340,34 +398,40 @@ class Server:
                 self.cache.finish(e)
             else:
                 self.cache.wait_ready(e)
-                self.budget.reserve(work, self.cache.evict)
-                reserved = work
             kvec = q.k * dirs
-            dB = ctx.empty(n2 * nrhs * 16)
-            dJ = ctx.empty(n * nrhs * 48)
-            dP = ctx.to_device(pts)
-            dE, dH = ctx.empty(nrhs * M * 48), ctx.empty(nrhs * M * 48)
+            dB = self._buf(w, "B", n2 * nrhs * 16)
+            dJ = self._buf(w, "J", n * nrhs * 48)
+            dP = self._buf(w, "P", M * 24)
+            dE, dH = self._buf(w, "E", nrhs * M * 48), self._buf(w, "H", nrhs * M * 48)
+            ctx.upload(dP, pts)
             m.rhs_planewave_into(dB, n2, kvec, pols)
-            iters, resid = 0, 0.0
-            if self.form == "mixed":
-                iters, resid = ctx.refine_solve(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], dB, nrhs, n2,
-                                                self.refine_tol, 30)
-            else:
+            iters, resid, fallback = 0, 0.0, False
+            if self.form == "mixed" and "LU64" not in e.bufs:
+                try:
+                    iters, resid = ctx.refine_solve(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], dB, nrhs,
+                                                    n2, self.refine_tol, 30)
+                except api.BielccError as err:
+                    if err.code != 4:      # refinement stalled: FP64 factors for this entry
+                        raise
+                    self._upgrade(w, e)
+                    m.rhs_planewave_into(dB, n2, kvec, pols)
+            if self.form == "mixed" and "LU64" in e.bufs:
+                ctx.lu_solve(e.bufs["LU64"], n2, n2, e.bufs["piv64"], dB, nrhs, n2)
+                fallback = True
+            elif self.form == "fp64":
                 ctx.lu_solve(e.bufs["Z"], n2, n2, e.bufs["piv"], dB, nrhs, n2)
             m.currents_into(dB, n2, nrhs, dJ)
             inc = np.concatenate([kvec, pols], axis=1)
-            prec = api.PREC_MIXED if self.form == "mixed" else api.PREC_FP64
+            # near a resonance the currents carry a large non-radiating part whose
+            # exterior field cancels: FP32 far sums would lose that cancellation
+            prec = api.PREC_MIXED if self.form == "mixed" and not fallback else api.PREC_FP64
             m.field_batch_into(self.np, q.k, nrhs, dJ, dP, M, dE, dH, inc=inc, precision=prec)
             E = dE.to_numpy((nrhs, M, 3), np.complex128)
             H = dH.to_numpy((nrhs, M, 3), np.complex128)
-            for d in (dB, dJ, dP, dE, dH):
-                d.free()
         finally:
             if e.ready or not builder:
                 self.cache.release(e)
-            if reserved:
-                self.budget.release(reserved)
-        return Result(q.qid, E, H, q.arrival, start, time.perf_counter(), builder, iters, resid)
+        return Result(q.qid, E, H, q.arrival, start, time.perf_counter(), builder, iters, resid, fallback)
 
 
 def percentiles(lat) -> dict:
@@ -394,6 +458,7 @@ def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
     t1 = max(r.end for r in res.values())
     solves = sum(q.nrhs for q in queries)
     out = {"queries": len(queries), "solves": solves, "wall_s": t1 - t0, "solves_per_s": solves / (t1 - t0),
-           "factorisations": sum(r.factored for r in res.values()), "evictions": server.cache.evictions}
+           "factorisations": sum(r.factored for r in res.values()), "evictions": server.cache.evictions,
+           "fp64_fallbacks": sum(r.fp64_fallback for r in res.values())}
     out.update(percentiles([r.latency for r in res.values()]))
     return {"results": res, "stats": out}

@@ -94,3 +94,19 @@ def test_fp64_query_against_the_oracle_chain(orc):
         Eo, Ho = orc.field_eh(verts, j, q.k, npp, q.points, incident=(kv, q.pols[r]))
         assert rowwise_rel(res.E[r], Eo).max() <= 1e-10
         assert rowwise_rel(res.H[r], Ho).max() <= 1e-10
+
+
+def test_mixed_form_falls_back_to_fp64_factors_when_refinement_stalls():
+    """An unreachable refinement tolerance stalls every refinement: the entry
+    gets FP64 factors once and its queries match the FP64 form."""
+    qs = _queries(seed=74, n=4)
+    r64 = _alone("fp64", qs)
+    out = {}
+    with serve.Server(_pool(), form="mixed", workers=1, hbm_budget=4e9, refine_tol=1e-30) as s:
+        for q in qs:
+            s.submit(_copy(q))
+            out[q.qid] = s.wait([q.qid])[q.qid]
+    for q in qs:
+        assert out[q.qid].fp64_fallback
+        for r in range(q.nrhs):
+            assert rowwise_rel(out[q.qid].E[r][:, :], r64[q.qid].E[r]).max() <= 1e-6
