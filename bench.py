@@ -39,6 +39,9 @@ sys.path.insert(0, ROOT)
 FLOPS_FIELD_PAIR = 12544      # fused E (K_RotRot_Far) + H (R_Rot_Colloc), far pair
 FLOPS_ASM_PAIR = 10256        # K_RotRot_HS_Colloc block, far quad pair
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector (dense) peak
+# HBM bytes moved per k_field_far launch on sphere20k: rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+# (scale factors from a 1 GiB copy / fill in the same run); the kernel is VALU-bound, the bytes are L2 re-reads
+FIELD_TRAFFIC_BYTES = {"sphere20k": 1.002e9}
 
 
 def log(*a):
@@ -373,7 +376,8 @@ def main():
     achieved = M * n * FLOPS_FIELD_PAIR / fld_avg_s / 1e12
     asm_avg_s = asm_ms / max(asm_cnt, 1) / 1e3
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": FIELD_TRAFFIC_BYTES.get(args.config),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "kernel": "k_field_far (fused K_RotRot_Far + R_Rot_Colloc, FP64)",
             "launch_ms": round(fld_avg_s * 1e3, 3),
             "assembly_kernel_tflops": round(n * n * FLOPS_ASM_PAIR / asm_avg_s / 1e12, 3) if asm_cnt else None,
