@@ -5,10 +5,10 @@ K / R / K0 path: cell-pair quadrature kernels in HIP behind a C ABI
 The HIP library is loaded lazily by `Context()`; importing the package does
 not touch the GPU.
 """
-from .api import (BielccError, Context, DeviceArray, Mesh, NumParam, KernelParam, IntegralParam, num_param,
+from .api import (BielccError, Context, DeviceArray, Mesh, Plan, NumParam, KernelParam, IntegralParam, num_param,
                   kernel_param, OPS, KERNELS, DEFAULT_NUM_PARAM, PREC_FP64, PREC_MIXED)
 from . import mesh
 
-__all__ = ["BielccError", "Context", "DeviceArray", "Mesh", "NumParam", "KernelParam", "IntegralParam",
+__all__ = ["BielccError", "Context", "DeviceArray", "Mesh", "Plan", "NumParam", "KernelParam", "IntegralParam",
            "num_param", "kernel_param", "OPS", "KERNELS", "DEFAULT_NUM_PARAM", "PREC_FP64", "PREC_MIXED",
            "mesh"]

@@ -34,6 +34,19 @@ class IntegralParam(C.Structure):
     _fields_ = [("idim", C.c_int), ("n_start", C.c_int), ("p_max", C.c_int), ("eps", C.c_double)]
 
 
+class AssembleItem(C.Structure):
+    """bielcc_assemble_item."""
+    _fields_ = [("mesh_id", C.c_int), ("k_re", C.c_double), ("k_im", C.c_double), ("Z_d", C.c_void_p),
+                ("ld", C.c_int64)]
+
+
+class FieldItem(C.Structure):
+    """bielcc_field_item."""
+    _fields_ = [("mesh_id", C.c_int), ("k_re", C.c_double), ("k_im", C.c_double), ("nrhs", C.c_int),
+                ("jE_d", C.c_void_p), ("inc_d", C.c_void_p), ("multE_re", C.c_double), ("multE_im", C.c_double),
+                ("M", C.c_int), ("pts_d", C.c_void_p), ("E_d", C.c_void_p), ("H_d", C.c_void_p)]
+
+
 # symbol -> (restype, argtypes)
 _SIGS = {
     "bielcc_version": (C.c_int, []),
@@ -87,6 +100,14 @@ _SIGS = {
     "bielcc_far_pattern": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bielcc_plan_assemble": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(AssembleItem), C.c_int,
+                                       C.POINTER(NumParam), C.c_int, C.POINTER(C.c_void_p)]),
+    "bielcc_plan_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(FieldItem), C.POINTER(NumParam), C.c_int,
+                                    C.c_int, C.POINTER(C.c_void_p)]),
+    "bielcc_plan_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "bielcc_plan_capture": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "bielcc_plan_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "bielcc_plan_info": (C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, _DP]),
     "bielcc_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "bielcc_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, _IP]),
     "bielcc_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),

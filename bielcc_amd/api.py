@@ -177,6 +177,32 @@ class Context:
     def mesh(self, verts) -> "Mesh":
         return Mesh(self, verts)
 
+    # -- plans (several meshes per launch)
+    def plan_assemble(self, items, np_: NumParam, which=0, precision=PREC_FP64) -> Plan:
+        """items: [(mesh, k, Z DeviceArray, ld)]."""
+        arr = (_lib.AssembleItem * len(items))()
+        for a, (m, k, Z, ld) in zip(arr, items):
+            k = complex(k)
+            a.mesh_id, a.k_re, a.k_im, a.Z_d, a.ld = m.id, k.real, k.imag, _p(Z), int(ld)
+        h = C.c_void_p()
+        self._check(self.lib.bielcc_plan_assemble(self.h, len(items), arr, int(which), C.byref(np_),
+                                                  int(precision), C.byref(h)))
+        return Plan(self, h, (arr, items))
+
+    def plan_field(self, items, np_: NumParam, near_mode=0, precision=PREC_FP64) -> Plan:
+        """items: [dict(mesh, k, nrhs, jE, M, pts, E, H=None, inc=None, multE=1)] with device buffers
+        (jE [nrhs][N][3], inc [nrhs][6] (kvec, e0), pts [M][3], E/H [nrhs][M][3])."""
+        arr = (_lib.FieldItem * len(items))()
+        for a, it in zip(arr, items):
+            k, mE = complex(it["k"]), complex(it.get("multE", 1.0))
+            a.mesh_id, a.k_re, a.k_im, a.nrhs = it["mesh"].id, k.real, k.imag, int(it["nrhs"])
+            a.jE_d, a.inc_d, a.multE_re, a.multE_im = _p(it["jE"]), _p(it.get("inc")), mE.real, mE.imag
+            a.M, a.pts_d, a.E_d, a.H_d = int(it["M"]), _p(it["pts"]), _p(it["E"]), _p(it.get("H"))
+        h = C.c_void_p()
+        self._check(self.lib.bielcc_plan_field(self.h, len(items), arr, C.byref(np_), int(near_mode),
+                                               int(precision), C.byref(h)))
+        return Plan(self, h, (arr, items))
+
     # -- single functions, batched (host arrays in, host arrays out)
     def op_batch(self, op, cells, x, j, np_: NumParam, k) -> np.ndarray:
         op = OPS[op] if isinstance(op, str) else int(op)
@@ -287,6 +313,40 @@ class Context:
         ms, cnt = C.c_double(0.), C.c_int(0)
         self._check(self.lib.bielcc_timing(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+class Plan:
+    """A bielcc_plan: the matrices or fields of several meshes, enqueued with
+    run() (no host synchronisation, no allocation), optionally replayed from a
+    HIP graph after capture().  The item buffers must outlive the plan; their
+    contents may change between runs."""
+
+    def __init__(self, ctx: "Context", handle, keep):
+        self.ctx = ctx
+        self.h = handle
+        self._keep = keep          # buffers and descriptors the plan points into
+
+    def run(self):
+        self.ctx._check(self.ctx.lib.bielcc_plan_run(self.ctx.h, self.h))
+
+    def capture(self):
+        self.ctx._check(self.ctx.lib.bielcc_plan_capture(self.ctx.h, self.h))
+
+    def info(self, key: str) -> float:
+        v = C.c_double(0.)
+        self.ctx._check(self.ctx.lib.bielcc_plan_info(self.ctx.h, self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h is not None and self.ctx.h is not None:
+            self.ctx.lib.bielcc_plan_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Mesh:

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libbielcc_hip.so")
-SOURCES = ["capi.hip", "tables.hip", "field.hip", "farfield.hip", "assemble.hip", "generic.hip", "lu.hip"]
+SOURCES = ["capi.hip", "plan.hip", "tables.hip", "field.hip", "farfield.hip", "assemble.hip", "generic.hip", "lu.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BIELCC_ARCH", "gfx950")
 # packed FP32 instructions are disabled for all device code

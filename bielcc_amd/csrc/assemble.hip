@@ -83,13 +83,13 @@ __device__ inline void surf_sums32(const float4* nd, const float* ny, int nn, co
     }
 }
 
+// One tile of far pairs: collocation points bx*256 .. +255 (one per lane) x
+// cells [c_begin, c_end), staged through the LDS arrays handed in.
 template <bool CK, int WHICH, bool FP, bool P32>
-__global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_per_split) {
-    __shared__ double4 ssurf[kASurf];
-    __shared__ double4 sseg[kASeg];
-    __shared__ double scell[kACells][kARec];
+__device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int c_begin, int c_end, double4* ssurf,
+                                              double4* sseg, double (*scell)[kARec]) {
     const int tid = threadIdx.x;
-    const int li = blockIdx.x * kAT + tid;          // local point index
+    const int li = bx * kAT + tid;          // local point index
     const bool active = li < a.ni;
     const int i = a.i0 + li;
     double x[3] = {0., 0., 0.}, ta[2][3] = {{0, 0, 0}, {0, 0, 0}};
@@ -97,8 +97,6 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
         const double* t = a.ctab + (size_t)i * CT;
         for (int d = 0; d < 3; d++) { x[d] = t[CT_RKT + d]; ta[0][d] = t[CT_TAU1 + d]; ta[1][d] = t[CT_TAU2 + d]; }
     }
-    const int c_begin = blockIdx.y * cells_per_split;
-    const int c_end = min(a.N, c_begin + cells_per_split);
     const int nn = a.nn, nv = a.nv, ns = a.ns, nseg = nv * ns;
     int cb = kACells;
     if (WHICH == 0) cb = min(cb, min(kASurf / nn, kASeg / nseg));
@@ -263,6 +261,55 @@ __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_
             }
         }
     }
+}
+
+template <bool CK, int WHICH, bool FP, bool P32>
+__global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_per_split) {
+    __shared__ double4 ssurf[kASurf];
+    __shared__ double4 sseg[kASeg];
+    __shared__ double scell[kACells][kARec];
+    const int c_begin = blockIdx.y * cells_per_split;
+    assemble_tile<CK, WHICH, FP, P32>(a, blockIdx.x, c_begin, min(a.N, c_begin + cells_per_split), ssurf, sseg,
+                                      scell);
+}
+
+// Persistent form over several meshes: workgroup b takes tiles b, b + gridDim.x,
+// ...; a tile is (item, point block, cell begin, cell end).  The reduced-phase
+// choice is per item.
+template <bool CK, int WHICH, bool P32>
+__global__ __launch_bounds__(kAT) void k_assemble_multi(const AssembleArgs* __restrict__ items,
+                                                        const int4* __restrict__ tiles, int ntiles) {
+    __shared__ double4 ssurf[kASurf];
+    __shared__ double4 sseg[kASeg];
+    __shared__ double scell[kACells][kARec];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const AssembleArgs a = items[tl.x];
+        if (a.fast_phase) assemble_tile<CK, WHICH, true, P32>(a, tl.y, tl.z, tl.w, ssurf, sseg, scell);
+        else assemble_tile<CK, WHICH, false, P32>(a, tl.y, tl.z, tl.w, ssurf, sseg, scell);
+    }
+}
+
+template <bool CK, int W, bool P32>
+static void assemble_multi_t(hipStream_t s, const AssembleArgs* items_d, const int4* tiles_d, int ntiles, int grid) {
+    hipLaunchKernelGGL((k_assemble_multi<CK, W, P32>), dim3(grid), dim3(kAT), 0, s, items_d, tiles_d, ntiles);
+}
+
+hipError_t launch_assemble_multi(hipStream_t s, const AssembleArgs* items_d, const int4* tiles_d, int ntiles,
+                                 int grid, int which, bool ck, bool p32) {
+    if (ntiles <= 0) return hipSuccess;
+    if (which == 0) {
+        if (ck) { if (p32) assemble_multi_t<true, 0, true>(s, items_d, tiles_d, ntiles, grid);
+                  else assemble_multi_t<true, 0, false>(s, items_d, tiles_d, ntiles, grid); }
+        else { if (p32) assemble_multi_t<false, 0, true>(s, items_d, tiles_d, ntiles, grid);
+               else assemble_multi_t<false, 0, false>(s, items_d, tiles_d, ntiles, grid); }
+    } else {
+        if (ck) { if (p32) assemble_multi_t<true, 1, true>(s, items_d, tiles_d, ntiles, grid);
+                  else assemble_multi_t<true, 1, false>(s, items_d, tiles_d, ntiles, grid); }
+        else { if (p32) assemble_multi_t<false, 1, true>(s, items_d, tiles_d, ntiles, grid);
+               else assemble_multi_t<false, 1, false>(s, items_d, tiles_d, ntiles, grid); }
+    }
+    return hipGetLastError();
 }
 
 template <bool CK, int W, bool P32>

@@ -98,6 +98,31 @@ struct FinalizeArgs {
 };
 hipError_t launch_field_finalize(hipStream_t s, const FinalizeArgs& f);
 
+// ---- operator-matrix field (field.hip): one segment = one mesh and a range of M points
+struct OpSeg {
+    int M, N, nn, nv;
+    const double* pts;       // [M][3]: the segment's first point
+    const double* ctab;
+    const double* nodes;
+    const float4* nodes32;   // prec = 1
+    const float* nodes32_yy;
+    const double* verts;     // [N][nv][3] (near pairs)
+    double k_re, k_im;
+    int fast_phase;
+    int mode;                // near rule (see launch_near_count)
+    double ac, thresh;
+    double* WK;              // [3N][3M] complex, column-major, ld 3M: K_RotRot_Far / K near, /4pi included
+    double* WR;              // same for R_Rot_Colloc / R_Rot_Smooth, or null
+};
+// Persistent tile kernel over tiles (segment, first point, cell begin, cell end).
+hipError_t launch_field_opmat(hipStream_t s, const OpSeg* segs_d, const int4* tiles_d, int ntiles, int grid, bool ck,
+                              bool p32);
+// Exact near pairs of one segment (lq = point within the segment) into WK / WR.
+hipError_t launch_opmat_near(hipStream_t s, long long npairs, const int* lq, const int* lc, const OpSeg& sg,
+                             const bielcc_dev::NumP& np, int mode);
+// E[r][p] = plane wave inc[r] = (kvec, e0) at pts[p], or 0 when inc is null.
+hipError_t launch_init_incident(hipStream_t s, int M, int nrhs, const double* pts, const double* inc, double* E);
+
 // ---- far-field pattern (farfield.hip)
 struct PatternArgs {
     int N, nn, ndir, nrhs, nsplit;
@@ -138,6 +163,11 @@ struct AssembleArgs {
 // pi/4 + |k| * (max cell diameter) bounds every reduced node phase (phase.h)
 bool fast_phase_ok(double k_re, double grid_step);
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a);
+// Persistent far-pair kernel over several meshes: items_d[tile.x] with point
+// block tile.y (256 collocation points) and cells [tile.z, tile.w).  All
+// items share which and prec; ck = some item has a complex wave number.
+hipError_t launch_assemble_multi(hipStream_t s, const AssembleArgs* items_d, const int4* tiles_d, int ntiles,
+                                 int grid, int which, bool ck, bool p32);
 hipError_t launch_assemble_generic(hipStream_t s, int nv, int which, long long npairs, const int* lq,
                                    const int* lc, int N, int i0, int ni, const double* verts, const double* ctab,
                                    const bielcc_dev::NumP& np, double k_re, double k_im, double* Z, long long ld);
@@ -181,5 +211,11 @@ hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* A, long lo
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const float* F,
                         long long ldf, const int* piv, int nrhs, double* B, long long ldb, double tol, int max_iter,
                         int* iters, double* resid, int* status, const char** msg);
+// C = alpha A B + beta C, complex FP64 column-major, on the BLAS handle of w
+// (matrix cores).  Creates the handle on first use: call once before a
+// stream capture.
+hipError_t blas_zgemm(hipStream_t s, LuWork* w, long long m, long long n, long long k, double2 alpha,
+                      const double* A, long long lda, const double* B, long long ldb, double2 beta, double* C,
+                      long long ldc, const char** msg);
 
 }  // namespace bielcc_impl

@@ -114,6 +114,152 @@ __device__ inline void write_partials(const FieldArgs& a, int r0, int pt, const 
     }
 }
 
+// Node sums of one far (point, cell) pair in FP64: A = sum w e f1,
+// T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh, G = sum w (ikr-1) e d / r^3
+// (e = e^{ikr}; unscaled, without 1/4pi).  nd: the cell's nn nodes.
+template <bool CK, bool FP>
+__device__ __forceinline__ void pair_sums64(double x0, double x1, double x2, const double4* nd, int nn, double dcen,
+                                            double kr_, double ki_, double k2r, double k2i, cplx& A, cplx (&T)[6],
+                                            cplx (&G)[3]) {
+    PhaseRef pr{0., 0., 0., 0};
+    if (FP) pr = phase_ref(kr_ * dcen);
+    // A = sum w e f1 ; T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh ; G = sum w (ikr-1) e d / r^3
+    double Ar = 0., Ai = 0.;
+    double Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
+    double Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
+    for (int q = 0; q < nn; q++) {
+        const double4 y = nd[q];
+        const double d0 = x0 - y.x, d1 = x1 - y.y, d2 = x2 - y.z;
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const double ri = rsqrt(r2);
+        const double r = r2 * ri;
+        const double ri2 = ri * ri, ri3 = ri2 * ri;
+        double s, c;
+        if (FP) sincos_near(kr_, r, pr, s, c);     // e^{i(kr - n pi/2)}
+        else sincos(kr_ * r, &s, &c);
+        double w = y.w;
+        if (CK) w *= exp(-ki_ * r);
+        const double wc = w * c, ws = w * s;
+        // f1 = -1/r^3 + ik/r^2 + k^2/r ; f2 = 3/r^3 - 3ik/r^2 - k^2/r  (ik = -ki + i kr)
+        double f1r, f1i, f2r, f2i;
+        if (CK) {
+            f1r = -ri3 - ki_ * ri2 + k2r * ri; f1i = kr_ * ri2 + k2i * ri;
+            f2r = 3. * ri3 + 3. * ki_ * ri2 - k2r * ri; f2i = -3. * kr_ * ri2 - k2i * ri;
+        } else {
+            f1r = k2r * ri - ri3; f1i = kr_ * ri2;
+            f2r = 3. * ri3 - k2r * ri; f2i = -3. * f1i;
+        }
+        Ar += wc * f1r - ws * f1i;
+        Ai += wc * f1i + ws * f1r;
+        const double e2r = wc * f2r - ws * f2i, e2i = wc * f2i + ws * f2r;
+        const double h0 = d0 * ri, h1 = d1 * ri, h2 = d2 * ri;
+        const double p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p01 = h0 * h1, p02 = h0 * h2, p12 = h1 * h2;
+        Tr[0] += e2r * p00; Ti[0] += e2i * p00;
+        Tr[1] += e2r * p11; Ti[1] += e2i * p11;
+        Tr[2] += e2r * p22; Ti[2] += e2i * p22;
+        Tr[3] += e2r * p01; Ti[3] += e2i * p01;
+        Tr[4] += e2r * p02; Ti[4] += e2i * p02;
+        Tr[5] += e2r * p12; Ti[5] += e2i * p12;
+        // (ikr - 1) e = (-(1 + ki r) + i kr r) (c + i s)
+        const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
+        const double gr = (mr * wc - mi * ws) * ri3, gi = (mr * ws + mi * wc) * ri3;
+        Gr[0] += gr * d0; Gi[0] += gi * d0;
+        Gr[1] += gr * d1; Gi[1] += gi * d1;
+        Gr[2] += gr * d2; Gi[2] += gi * d2;
+    }
+    if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
+        rot_q(Ar, Ai, pr.q);
+        for (int t = 0; t < 6; t++) rot_q(Tr[t], Ti[t], pr.q);
+        for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
+    }
+    A = mk(Ar, Ai);
+    for (int t = 0; t < 6; t++) T[t] = mk(Tr[t], Ti[t]);
+    for (int g = 0; g < 3; g++) G[g] = mk(Gr[g], Gi[g]);
+}
+
+// The same sums with FP32 node arithmetic in cell-local coordinates
+// (rec = cell centre; nd, ny = the cell's FP32 nodes and |y_c|^2), flushed
+// into FP64 every kGroup32 nodes; the cell phase and i^n stay FP64.
+template <bool CK, bool FP>
+__device__ __forceinline__ void pair_sums32(double x0, double x1, double x2, const double* rec, const float4* nd,
+                                            const float* ny, int nn, double dcen, double kr_, double ki_, cplx& A,
+                                            cplx (&T)[6], cplx (&G)[3]) {
+    const float kf = (float)kr_, kif = (float)ki_;
+    const float k2rf = (float)(kr_ * kr_ - ki_ * ki_), k2if = (float)(2. * kr_ * ki_);
+    const PhaseRef pr = phase_ref(kr_ * dcen);
+    const float t0 = (float)(fma(kr_, dcen, -pr.o1) - pr.o2 - pr.o3);
+    const float xf0 = (float)(x0 - rec[0]), xf1 = (float)(x1 - rec[1]), xf2 = (float)(x2 - rec[2]);
+    const float m0 = -2.f * xf0, m1 = -2.f * xf1, m2 = -2.f * xf2;
+    const float dcf = (float)dcen;
+    // FP32 sums over groups of kGroup32 nodes, flushed into FP64 (the
+    // recursive FP32 sum of 64 same-signed terms would lose ~sqrt(64) ulp)
+    double ArD = 0., AiD = 0., TrD[6] = {0, 0, 0, 0, 0, 0}, TiD[6] = {0, 0, 0, 0, 0, 0};
+    double GrD[3] = {0, 0, 0}, GiD[3] = {0, 0, 0};
+    for (int q0 = 0; q0 < nn; q0 += kGroup32) {
+        float Ar = 0.f, Ai = 0.f;
+        float Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
+        float Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
+        const int q1 = min(nn, q0 + kGroup32);
+        for (int q = q0; q < q1; q++) {
+            const float4 y = nd[q];
+            const float d0 = xf0 - y.x, d1 = xf1 - y.y, d2 = xf2 - y.z;
+            const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+            const float ri = rsqrtf(r2);
+            const float r = r2 * ri;
+            // r - dcen without cancellation: (|y_c|^2 - 2 x_c.y_c) / (r + dcen)
+            const float del = fmaf(m0, y.x, fmaf(m1, y.y, fmaf(m2, y.z, ny[q])));
+            const float dr = __fdividef(del, r + dcf);
+            const float t = fmaf(kf, dr, t0);
+            float s, c;
+            if (FP) sincos_near32(t, s, c);
+            else sincosf(t, &s, &c);
+            float w = y.w;
+            if (CK) w *= __expf(-kif * dr);   // e^{-ki dcen} is applied per cell in FP64
+            const float ri2 = ri * ri, ri3 = ri2 * ri;
+            const float wc = w * c, ws = w * s;
+            float f1r, f1i, f2r, f2i;
+            if (CK) {
+                f1r = -ri3 - kif * ri2 + k2rf * ri; f1i = kf * ri2 + k2if * ri;
+                f2r = 3.f * ri3 + 3.f * kif * ri2 - k2rf * ri; f2i = -3.f * kf * ri2 - k2if * ri;
+            } else {
+                f1r = k2rf * ri - ri3; f1i = kf * ri2;
+                f2r = 3.f * ri3 - k2rf * ri; f2i = -3.f * f1i;
+            }
+            Ar += wc * f1r - ws * f1i;
+            Ai += wc * f1i + ws * f1r;
+            const float e2r = wc * f2r - ws * f2i, e2i = wc * f2i + ws * f2r;
+            const float h0 = d0 * ri, h1 = d1 * ri, h2 = d2 * ri;
+            const float p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p01 = h0 * h1, p02 = h0 * h2, p12 = h1 * h2;
+            Tr[0] += e2r * p00; Ti[0] += e2i * p00;
+            Tr[1] += e2r * p11; Ti[1] += e2i * p11;
+            Tr[2] += e2r * p22; Ti[2] += e2i * p22;
+            Tr[3] += e2r * p01; Ti[3] += e2i * p01;
+            Tr[4] += e2r * p02; Ti[4] += e2i * p02;
+            Tr[5] += e2r * p12; Ti[5] += e2i * p12;
+            const float mr = CK ? -(1.f + kif * r) : -1.f, mi = kf * r;
+            const float gr = (mr * wc - mi * ws) * ri3, gi = (mr * ws + mi * wc) * ri3;
+            Gr[0] += gr * d0; Gi[0] += gi * d0;
+            Gr[1] += gr * d1; Gi[1] += gi * d1;
+            Gr[2] += gr * d2; Gi[2] += gi * d2;
+        }
+        ArD += Ar; AiD += Ai;
+        for (int t = 0; t < 6; t++) { TrD[t] += Tr[t]; TiD[t] += Ti[t]; }
+        for (int g = 0; g < 3; g++) { GrD[g] += Gr[g]; GiD[g] += Gi[g]; }
+    }
+    rot_q(ArD, AiD, pr.q);
+    for (int t = 0; t < 6; t++) rot_q(TrD[t], TiD[t], pr.q);
+    for (int g = 0; g < 3; g++) rot_q(GrD[g], GiD[g], pr.q);
+    if (CK) {
+        const double amp = exp(-ki_ * dcen);
+        ArD *= amp; AiD *= amp;
+        for (int t = 0; t < 6; t++) { TrD[t] *= amp; TiD[t] *= amp; }
+        for (int g = 0; g < 3; g++) { GrD[g] *= amp; GiD[g] *= amp; }
+    }
+    A = mk(ArD, AiD);
+    for (int t = 0; t < 6; t++) T[t] = mk(TrD[t], TiD[t]);
+    for (int g = 0; g < 3; g++) G[g] = mk(GrD[g], GiD[g]);
+}
+
 template <bool CK, bool JM, bool FP, int R>
 __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_split, int r0) {
     __shared__ double4 snode[kFNodes];
@@ -147,62 +293,8 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
             const double* rec = scell[cc];
             const double dcen = center_dist(x0, x1, x2, rec);
             if (near_rule(dcen, rec[3], a.mode, a.ac, a.thresh)) continue;
-            PhaseRef pr{0., 0., 0., 0};
-            if (FP) pr = phase_ref(kr_ * dcen);
-            // A = sum w e f1 ; T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh ; G = sum w (ikr-1) e d / r^3
-            double Ar = 0., Ai = 0.;
-            double Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
-            double Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
-            const double4* nd = snode + cc * nn;
-            for (int q = 0; q < nn; q++) {
-                const double4 y = nd[q];
-                const double d0 = x0 - y.x, d1 = x1 - y.y, d2 = x2 - y.z;
-                const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                const double ri = rsqrt(r2);
-                const double r = r2 * ri;
-                const double ri2 = ri * ri, ri3 = ri2 * ri;
-                double s, c;
-                if (FP) sincos_near(kr_, r, pr, s, c);     // e^{i(kr - n pi/2)}
-                else sincos(kr_ * r, &s, &c);
-                double w = y.w;
-                if (CK) w *= exp(-ki_ * r);
-                const double wc = w * c, ws = w * s;
-                // f1 = -1/r^3 + ik/r^2 + k^2/r ; f2 = 3/r^3 - 3ik/r^2 - k^2/r  (ik = -ki + i kr)
-                double f1r, f1i, f2r, f2i;
-                if (CK) {
-                    f1r = -ri3 - ki_ * ri2 + k2r * ri; f1i = kr_ * ri2 + k2i * ri;
-                    f2r = 3. * ri3 + 3. * ki_ * ri2 - k2r * ri; f2i = -3. * kr_ * ri2 - k2i * ri;
-                } else {
-                    f1r = k2r * ri - ri3; f1i = kr_ * ri2;
-                    f2r = 3. * ri3 - k2r * ri; f2i = -3. * f1i;
-                }
-                Ar += wc * f1r - ws * f1i;
-                Ai += wc * f1i + ws * f1r;
-                const double e2r = wc * f2r - ws * f2i, e2i = wc * f2i + ws * f2r;
-                const double h0 = d0 * ri, h1 = d1 * ri, h2 = d2 * ri;
-                const double p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p01 = h0 * h1, p02 = h0 * h2, p12 = h1 * h2;
-                Tr[0] += e2r * p00; Ti[0] += e2i * p00;
-                Tr[1] += e2r * p11; Ti[1] += e2i * p11;
-                Tr[2] += e2r * p22; Ti[2] += e2i * p22;
-                Tr[3] += e2r * p01; Ti[3] += e2i * p01;
-                Tr[4] += e2r * p02; Ti[4] += e2i * p02;
-                Tr[5] += e2r * p12; Ti[5] += e2i * p12;
-                // (ikr - 1) e = (-(1 + ki r) + i kr r) (c + i s)
-                const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
-                const double gr = (mr * wc - mi * ws) * ri3, gi = (mr * ws + mi * wc) * ri3;
-                Gr[0] += gr * d0; Gi[0] += gi * d0;
-                Gr[1] += gr * d1; Gi[1] += gi * d1;
-                Gr[2] += gr * d2; Gi[2] += gi * d2;
-            }
-            if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
-                rot_q(Ar, Ai, pr.q);
-                for (int t = 0; t < 6; t++) rot_q(Tr[t], Ti[t], pr.q);
-                for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
-            }
-            const cplx A = mk(Ar, Ai);
-            const cplx T[6] = {mk(Tr[0], Ti[0]), mk(Tr[1], Ti[1]), mk(Tr[2], Ti[2]),
-                               mk(Tr[3], Ti[3]), mk(Tr[4], Ti[4]), mk(Tr[5], Ti[5])};
-            const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
+            cplx A, T[6], G[3];
+            pair_sums64<CK, FP>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
             far_contract<JM, R>(rec, A, T, G, dcen < kMachineZero, E, H, KH, RE);
         }
     }
@@ -225,8 +317,6 @@ __global__ __launch_bounds__(kFT) void k_field_far32(FieldArgs a, int cells_per_
     const int nn = a.nn;
     const int cb = max(1, min(kFCells, kFNodes32 / nn));
     const double kr_ = a.k_re, ki_ = a.k_im;
-    const float kf = (float)kr_, kif = (float)ki_;
-    const float k2rf = (float)(kr_ * kr_ - ki_ * ki_), k2if = (float)(2. * kr_ * ki_);
 
     cplx E[R][3], H[R][3];
     for (int r = 0; r < R; r++)
@@ -246,81 +336,8 @@ __global__ __launch_bounds__(kFT) void k_field_far32(FieldArgs a, int cells_per_
             const double* rec = scell[cc];
             const double dcen = center_dist(x0, x1, x2, rec);
             if (near_rule(dcen, rec[3], a.mode, a.ac, a.thresh)) continue;
-            const PhaseRef pr = phase_ref(kr_ * dcen);
-            const float t0 = (float)(fma(kr_, dcen, -pr.o1) - pr.o2 - pr.o3);
-            const float xf0 = (float)(x0 - rec[0]), xf1 = (float)(x1 - rec[1]), xf2 = (float)(x2 - rec[2]);
-            const float m0 = -2.f * xf0, m1 = -2.f * xf1, m2 = -2.f * xf2;
-            const float dcf = (float)dcen;
-            // FP32 sums over groups of kGroup32 nodes, flushed into FP64 (the
-            // recursive FP32 sum of 64 same-signed terms would lose ~sqrt(64) ulp)
-            double ArD = 0., AiD = 0., TrD[6] = {0, 0, 0, 0, 0, 0}, TiD[6] = {0, 0, 0, 0, 0, 0};
-            double GrD[3] = {0, 0, 0}, GiD[3] = {0, 0, 0};
-            const float4* nd = snode + cc * nn;
-            const float* ny = syy + cc * nn;
-            for (int q0 = 0; q0 < nn; q0 += kGroup32) {
-                float Ar = 0.f, Ai = 0.f;
-                float Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
-                float Gr[3] = {0, 0, 0}, Gi[3] = {0, 0, 0};
-                const int q1 = min(nn, q0 + kGroup32);
-                for (int q = q0; q < q1; q++) {
-                    const float4 y = nd[q];
-                    const float d0 = xf0 - y.x, d1 = xf1 - y.y, d2 = xf2 - y.z;
-                    const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                    const float ri = rsqrtf(r2);
-                    const float r = r2 * ri;
-                    // r - dcen without cancellation: (|y_c|^2 - 2 x_c.y_c) / (r + dcen)
-                    const float del = fmaf(m0, y.x, fmaf(m1, y.y, fmaf(m2, y.z, ny[q])));
-                    const float dr = __fdividef(del, r + dcf);
-                    const float t = fmaf(kf, dr, t0);
-                    float s, c;
-                    if (FP) sincos_near32(t, s, c);
-                    else sincosf(t, &s, &c);
-                    float w = y.w;
-                    if (CK) w *= __expf(-kif * dr);   // e^{-ki dcen} is applied per cell in FP64
-                    const float ri2 = ri * ri, ri3 = ri2 * ri;
-                    const float wc = w * c, ws = w * s;
-                    float f1r, f1i, f2r, f2i;
-                    if (CK) {
-                        f1r = -ri3 - kif * ri2 + k2rf * ri; f1i = kf * ri2 + k2if * ri;
-                        f2r = 3.f * ri3 + 3.f * kif * ri2 - k2rf * ri; f2i = -3.f * kf * ri2 - k2if * ri;
-                    } else {
-                        f1r = k2rf * ri - ri3; f1i = kf * ri2;
-                        f2r = 3.f * ri3 - k2rf * ri; f2i = -3.f * f1i;
-                    }
-                    Ar += wc * f1r - ws * f1i;
-                    Ai += wc * f1i + ws * f1r;
-                    const float e2r = wc * f2r - ws * f2i, e2i = wc * f2i + ws * f2r;
-                    const float h0 = d0 * ri, h1 = d1 * ri, h2 = d2 * ri;
-                    const float p00 = h0 * h0, p11 = h1 * h1, p22 = h2 * h2, p01 = h0 * h1, p02 = h0 * h2, p12 = h1 * h2;
-                    Tr[0] += e2r * p00; Ti[0] += e2i * p00;
-                    Tr[1] += e2r * p11; Ti[1] += e2i * p11;
-                    Tr[2] += e2r * p22; Ti[2] += e2i * p22;
-                    Tr[3] += e2r * p01; Ti[3] += e2i * p01;
-                    Tr[4] += e2r * p02; Ti[4] += e2i * p02;
-                    Tr[5] += e2r * p12; Ti[5] += e2i * p12;
-                    const float mr = CK ? -(1.f + kif * r) : -1.f, mi = kf * r;
-                    const float gr = (mr * wc - mi * ws) * ri3, gi = (mr * ws + mi * wc) * ri3;
-                    Gr[0] += gr * d0; Gi[0] += gi * d0;
-                    Gr[1] += gr * d1; Gi[1] += gi * d1;
-                    Gr[2] += gr * d2; Gi[2] += gi * d2;
-                }
-                ArD += Ar; AiD += Ai;
-                for (int t = 0; t < 6; t++) { TrD[t] += Tr[t]; TiD[t] += Ti[t]; }
-                for (int g = 0; g < 3; g++) { GrD[g] += Gr[g]; GiD[g] += Gi[g]; }
-            }
-            rot_q(ArD, AiD, pr.q);
-            for (int t = 0; t < 6; t++) rot_q(TrD[t], TiD[t], pr.q);
-            for (int g = 0; g < 3; g++) rot_q(GrD[g], GiD[g], pr.q);
-            if (CK) {
-                const double amp = exp(-ki_ * dcen);
-                ArD *= amp; AiD *= amp;
-                for (int t = 0; t < 6; t++) { TrD[t] *= amp; TiD[t] *= amp; }
-                for (int g = 0; g < 3; g++) { GrD[g] *= amp; GiD[g] *= amp; }
-            }
-            const cplx A = mk(ArD, AiD);
-            const cplx T[6] = {mk(TrD[0], TiD[0]), mk(TrD[1], TiD[1]), mk(TrD[2], TiD[2]),
-                               mk(TrD[3], TiD[3]), mk(TrD[4], TiD[4]), mk(TrD[5], TiD[5])};
-            const cplx G[3] = {mk(GrD[0], GiD[0]), mk(GrD[1], GiD[1]), mk(GrD[2], GiD[2])};
+            cplx A, T[6], G[3];
+            pair_sums32<CK, FP>(x0, x1, x2, rec, snode + cc * nn, syy + cc * nn, nn, dcen, kr_, ki_, A, T, G);
             far_contract<JM, R>(rec, A, T, G, dcen < kMachineZero, E, H, KH, RE);
         }
     }
@@ -555,6 +572,187 @@ hipError_t launch_plane_wave(hipStream_t s, int n, const double* pts, const doub
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_plane_wave, dim3((n + 255) / 256), dim3(256), 0, s, n, pts, kvec_e0[0], kvec_e0[1],
                        kvec_e0[2], kvec_e0[3], kvec_e0[4], kvec_e0[5], out);
+    return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------- operator matrix
+// Field of many current sets: per segment (one mesh, a range of points) the
+// kernels write the operator matrices
+//   WK[(3p+g), (3c+h)] = K_RotRot_Far[e_h](x_p, cell_c)_g    (K.h:288-301)
+//   WR[(3p+g), (3c+h)] = R_Rot_Colloc[e_h](x_p, cell_c)_g    (R.h:89-111)
+// column-major with leading dimension 3M (complex), so that E = WK J and
+// H = WR J are complex GEMMs with J = the current sets as a [3N x nrhs]
+// matrix.  Far pairs come from the persistent tile kernel (node sums shared
+// with k_field_far: K = (A I + T)/4pi, R = [G]x/4pi); near pairs from the
+// exact per-pair kernel below.  Each entry is written by exactly one of them.
+namespace {
+constexpr double kInv4Pi = 0.079577471545947673;   // 1/(4 pi)
+
+__device__ inline void put(double* W, size_t idx, double re, double im) {
+    reinterpret_cast<double2*>(W)[idx] = make_double2(re, im);
+}
+}  // namespace
+
+// Persistent: workgroup b takes tiles b, b + gridDim.x, ...; a tile is
+// (segment, first point, cell range) = 256 points x up to plan cells.
+template <bool CK, bool P32>
+__global__ __launch_bounds__(kFT) void k_field_opmat(const OpSeg* __restrict__ segs, const int4* __restrict__ tiles,
+                                                     int ntiles) {
+    __shared__ double4 snode[P32 ? 1 : kFNodes];
+    __shared__ float4 snode32[P32 ? kFNodes32 : 1];
+    __shared__ float syy[P32 ? kFNodes32 : 1];
+    __shared__ double scell[kFCells][4];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const OpSeg sg = segs[tl.x];
+        const int pt = tl.y + tid;
+        const bool active = pt < sg.M;
+        double x0 = 0., x1 = 0., x2 = 0.;
+        if (active) { x0 = sg.pts[3 * (size_t)pt]; x1 = sg.pts[3 * (size_t)pt + 1]; x2 = sg.pts[3 * (size_t)pt + 2]; }
+        const int nn = sg.nn;
+        const int cb = max(1, min(kFCells, (P32 ? kFNodes32 : kFNodes) / nn));
+        const double kr_ = sg.k_re, ki_ = sg.k_im;
+        const double k2r = kr_ * kr_ - ki_ * ki_, k2i = 2. * kr_ * ki_;
+        const size_t ld = 3 * (size_t)sg.M;
+        for (int cs = tl.z; cs < tl.w; cs += cb) {
+            const int ncs = min(cb, tl.w - cs);
+            __syncthreads();
+            if (P32) {
+                const float4* gn = sg.nodes32 + (size_t)cs * nn;
+                const float* gy = sg.nodes32_yy + (size_t)cs * nn;
+                for (int idx = tid; idx < ncs * nn; idx += kFT) { snode32[idx] = gn[idx]; syy[idx] = gy[idx]; }
+            } else {
+                const double4* gn = reinterpret_cast<const double4*>(sg.nodes) + (size_t)cs * nn;
+                for (int idx = tid; idx < ncs * nn; idx += kFT) snode[idx] = gn[idx];
+            }
+            for (int idx = tid; idx < ncs * 4; idx += kFT) {
+                const int cc = idx / 4, f = idx % 4;
+                scell[cc][f] = sg.ctab[(size_t)(cs + cc) * CT + (f < 3 ? CT_CENTER + f : CT_DIAM)];
+            }
+            __syncthreads();
+            if (!active) continue;
+            for (int cc = 0; cc < ncs; cc++) {
+                const double* rec = scell[cc];
+                const double dcen = center_dist(x0, x1, x2, rec);
+                if (near_rule(dcen, rec[3], sg.mode, sg.ac, sg.thresh)) continue;
+                cplx A, T[6], G[3];
+                if (P32) {
+                    if (sg.fast_phase)
+                        pair_sums32<CK, true>(x0, x1, x2, rec, snode32 + cc * nn, syy + cc * nn, nn, dcen, kr_, ki_, A, T, G);
+                    else
+                        pair_sums32<CK, false>(x0, x1, x2, rec, snode32 + cc * nn, syy + cc * nn, nn, dcen, kr_, ki_, A, T, G);
+                } else {
+                    if (sg.fast_phase)
+                        pair_sums64<CK, true>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
+                    else
+                        pair_sums64<CK, false>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
+                }
+                const int c = cs + cc;
+                // K = (A I + T)/4pi, T = (xx, yy, zz, xy, xz, yz)
+                const int tix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+                for (int h = 0; h < 3; h++) {
+                    const size_t col = (size_t)(3 * c + h) * ld + 3 * (size_t)pt;
+                    for (int g = 0; g < 3; g++) {
+                        cplx v = T[tix[g][h]];
+                        if (g == h) v += A;
+                        put(sg.WK, col + g, v.re * kInv4Pi, v.im * kInv4Pi);
+                    }
+                }
+                if (sg.WR) {
+                    // R[j] = (G x j)/4pi; exactly zero when x is the centroid (R.h:94-97)
+                    const double z = dcen < kMachineZero ? 0. : kInv4Pi;
+                    const cplx Gs[3] = {G[0] * z, G[1] * z, G[2] * z};
+                    const cplx zero = mk(0., 0.);
+                    const cplx Rm[3][3] = {{zero, -Gs[2], Gs[1]}, {Gs[2], zero, -Gs[0]}, {-Gs[1], Gs[0], zero}};
+                    for (int h = 0; h < 3; h++) {
+                        const size_t col = (size_t)(3 * c + h) * ld + 3 * (size_t)pt;
+                        for (int g = 0; g < 3; g++) put(sg.WR, col + g, Rm[g][h].re, Rm[g][h].im);
+                    }
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_field_opmat(hipStream_t s, const OpSeg* segs_d, const int4* tiles_d, int ntiles, int grid, bool ck,
+                              bool p32) {
+    if (ntiles <= 0) return hipSuccess;
+    dim3 g(grid), b(kFT);
+    if (ck) {
+        if (p32) hipLaunchKernelGGL((k_field_opmat<true, true>), g, b, 0, s, segs_d, tiles_d, ntiles);
+        else hipLaunchKernelGGL((k_field_opmat<true, false>), g, b, 0, s, segs_d, tiles_d, ntiles);
+    } else {
+        if (p32) hipLaunchKernelGGL((k_field_opmat<false, true>), g, b, 0, s, segs_d, tiles_d, ntiles);
+        else hipLaunchKernelGGL((k_field_opmat<false, false>), g, b, 0, s, segs_d, tiles_d, ntiles);
+    }
+    return hipGetLastError();
+}
+
+// Near pairs of one segment, exact (as k_field_near_mat), written into WK/WR:
+// column h of a pair's block = Op(e_h).  lq = point index within the segment.
+template <int NV>
+__global__ __launch_bounds__(64) void k_opmat_near(long long np_, const int* __restrict__ lq,
+                                                   const int* __restrict__ lc, OpSeg sg, NumP np, int mode) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np_) return;
+    const int q = lq[p], c = lc[p];
+    double cv[NV][3], x[3];
+    for (int v = 0; v < NV; v++)
+        for (int d = 0; d < 3; d++) cv[v][d] = sg.verts[((size_t)c * NV + v) * 3 + d];
+    for (int d = 0; d < 3; d++) x[d] = sg.pts[3 * (size_t)q + d];
+    const cplx k = mk(sg.k_re, sg.k_im);
+    const int opK = (mode == 0) ? OP_K_NEAR_SMOOTH : OP_K_HS;
+    const size_t ld = 3 * (size_t)sg.M;
+    for (int h = 0; h < 3; h++) {
+        cplx j[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, r[3];
+        j[h] = mk(1., 0.);
+        const size_t col = (size_t)(3 * c + h) * ld + 3 * (size_t)q;
+        op_eval<NV>(opK, cv, x, j, np, k, r);
+        for (int g = 0; g < 3; g++) put(sg.WK, col + g, r[g].re, r[g].im);
+        if (sg.WR) {
+            op_eval<NV>(OP_R_SMOOTH, cv, x, j, np, k, r);
+            for (int g = 0; g < 3; g++) put(sg.WR, col + g, r[g].re, r[g].im);
+        }
+    }
+}
+
+hipError_t launch_opmat_near(hipStream_t s, long long npairs, const int* lq, const int* lc, const OpSeg& sg,
+                             const NumP& np, int mode) {
+    if (npairs <= 0) return hipSuccess;
+    dim3 g((unsigned)((npairs + 63) / 64)), b(64);
+    if (sg.nv == 4) hipLaunchKernelGGL(k_opmat_near<4>, g, b, 0, s, npairs, lq, lc, sg, np, mode);
+    else hipLaunchKernelGGL(k_opmat_near<3>, g, b, 0, s, npairs, lq, lc, sg, np, mode);
+    return hipGetLastError();
+}
+
+// E[r][p] = e0_r exp(i kvec_r . x_p) (PlaneWaveFieldE, incident_field.h:20-29)
+// from inc[r] = (kvec, e0), or 0 without inc; E is [nrhs][M][3] complex.
+__global__ __launch_bounds__(256) void k_init_incident(int M, const double* __restrict__ pts,
+                                                       const double* __restrict__ inc, double* __restrict__ E) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (p >= M) return;
+    double* o = E + ((size_t)r * M + p) * 6;
+    if (!inc) {
+        for (int a = 0; a < 6; a++) o[a] = 0.;
+        return;
+    }
+    const double* ke = inc + 6 * r;
+    const double* x = pts + 3 * (size_t)p;
+    const double ph = ke[0] * x[0] + ke[1] * x[1] + ke[2] * x[2];
+    const cplx deg = cexp(cplx{0., 1.} * ph);
+    for (int g = 0; g < 3; g++) {
+        const cplx e = deg * ke[3 + g];
+        o[2 * g] = e.re;
+        o[2 * g + 1] = e.im;
+    }
+}
+
+hipError_t launch_init_incident(hipStream_t s, int M, int nrhs, const double* pts, const double* inc, double* E) {
+    if (M <= 0 || nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_incident, dim3((M + 255) / 256, nrhs), dim3(256), 0, s, M, pts, inc, E);
     return hipGetLastError();
 }
 

@@ -446,6 +446,25 @@ __global__ void k_to_f32(int n, int ncol, const double2* __restrict__ src, long 
     dst[(size_t)c * ldd + i] = make_float2((float)v.x, (float)v.y);
 }
 
+hipError_t blas_zgemm(hipStream_t s, LuWork* w, long long m, long long n, long long k, double2 alpha,
+                      const double* A, long long lda, const double* B, long long ldb, double2 beta, double* C,
+                      long long ldc, const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    if (m <= 0 || n <= 0) return hipSuccess;
+    rocblas_set_stream(w->blas, s);
+    const rocblas_double_complex al = {alpha.x, alpha.y}, be = {beta.x, beta.y};
+    rocblas_status st = rocblas_zgemm_64(w->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &al,
+                                         reinterpret_cast<const rocblas_double_complex*>(A), lda,
+                                         reinterpret_cast<const rocblas_double_complex*>(B), ldb, &be,
+                                         reinterpret_cast<rocblas_double_complex*>(C), ldc);
+    if (st != rocblas_status_success) {
+        *msg = "rocblas_zgemm_64 failed";
+        return hipErrorUnknown;
+    }
+    return hipGetLastError();
+}
+
 hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, float* Fd, long long ldf,
                        int* piv, int* info, const char** msg) {
     const long long tot = (long long)n * n;

@@ -1,0 +1,445 @@
+// Plans: the collocation matrices or the fields of several meshes at once.
+//
+// A plan fixes its items (mesh, wave number, caller buffers) at creation and
+// precomputes everything that needs the host: the tile list of the persistent
+// kernel, the near-pair lists, the operator-matrix workspace and the BLAS
+// handle.  bielcc_plan_run then only enqueues kernels and GEMMs on the
+// context's stream -- no allocation, no host synchronisation, no memset -- so
+// bielcc_plan_capture can record it into a HIP graph that later runs replay.
+//
+// Field plans evaluate bielcc_field_batch (electric currents only) through the
+// operator matrices WK, WR of each (mesh, point range) segment: one persistent
+// launch writes the far-pair blocks of every segment of a workspace group, the
+// exact kernel writes the near-pair blocks, and E = multE WK J + E_inc,
+// H = WR J are complex GEMMs on the matrix cores over all current sets at once.
+#include "ctx_internal.h"
+
+using namespace bielcc_impl;
+using namespace bielcc_capi;
+
+namespace {
+
+struct NearSet {
+    long long n = 0;
+    int* lq = nullptr;   // point index within the segment (field) or item (assembly)
+    int* lc = nullptr;   // cell index
+};
+
+struct SegInfo {
+    int item, p0, M;     // points p0 .. p0+M-1 of the item
+    size_t ws_off;       // byte offset of WK in the workspace (WR follows)
+};
+
+struct Group {
+    int t0, nt;          // tiles
+    int s0, s1;          // segments
+    int grid;
+};
+
+}  // namespace
+
+struct bielcc_plan_s {
+    int kind = 0;        // 0 assembly, 1 field
+    int which = 0;
+    int precision = BIELCC_PREC_FP64;
+    int near_mode = 0;
+    bool ck = false;
+    NumP np{};
+    std::vector<void*> allocs;     // device memory owned by the plan
+    // assembly
+    std::vector<AssembleArgs> aitems;
+    std::vector<const DevMesh*> ameshes;
+    AssembleArgs* aitems_d = nullptr;
+    // field
+    std::vector<bielcc_field_item> fitems;
+    std::vector<int> fN;
+    std::vector<OpSeg> segs;
+    std::vector<SegInfo> seginfo;
+    OpSeg* segs_d = nullptr;
+    char* ws = nullptr;
+    size_t ws_bytes = 0;
+    // both
+    std::vector<NearSet> near;     // per segment (field) / per item (assembly)
+    std::vector<Group> groups;
+    int4* tiles_d = nullptr;
+    long long ntiles = 0, pairs = 0, near_pairs = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int runs = 0;
+};
+
+namespace {
+
+void plan_free(bielcc_plan p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (void* a : p->allocs) (void)hipFree(a);
+    delete p;
+}
+
+// plan-owned device allocation
+template <typename T>
+hipError_t palloc(bielcc_plan p, size_t bytes, T** out) {
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes ? bytes : 1);
+    if (e != hipSuccess) return e;
+    p->allocs.push_back(d);
+    *out = reinterpret_cast<T*>(d);
+    return hipSuccess;
+}
+
+// Near-pair list of nq query points (stride q_stride doubles) against mesh m,
+// in plan memory.  Host-synchronising: plan creation only.
+int plan_near(bielcc_ctx ctx, bielcc_plan p, int nq, const double* q, int q_stride, const DevMesh& m, int mode,
+              double ac, double thresh, NearSet* out) {
+    *out = NearSet{};
+    if (nq <= 0) return BIELCC_OK;
+    void* tmp = nullptr;
+    const size_t cnt_b = align256((size_t)nq * sizeof(int)), off_b = align256((size_t)(nq + 1) * sizeof(long long));
+    HIPCHK(hipMalloc(&tmp, cnt_b + off_b + 256));
+    int* counts = reinterpret_cast<int*>(tmp);
+    long long* offs = reinterpret_cast<long long*>(reinterpret_cast<char*>(tmp) + cnt_b);
+    long long* tot_d = reinterpret_cast<long long*>(reinterpret_cast<char*>(tmp) + cnt_b + off_b);
+    long long tot = 0;
+    hipError_t e = launch_near_count(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, counts);
+    if (e == hipSuccess) e = launch_scan(ctx->stream, nq, counts, offs, tot_d);
+    if (e == hipSuccess) e = hipMemcpyAsync(&tot, tot_d, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && tot > 0) {
+        e = palloc(p, (size_t)tot * sizeof(int), &out->lq);
+        if (e == hipSuccess) e = palloc(p, (size_t)tot * sizeof(int), &out->lc);
+        if (e == hipSuccess)
+            e = launch_near_fill(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, offs, out->lq, out->lc);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        out->n = tot;
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(ctx, e, "near-pair list");
+    return BIELCC_OK;
+}
+
+int upload_tiles(bielcc_ctx ctx, bielcc_plan p, const std::vector<int4>& tiles) {
+    p->ntiles = (long long)tiles.size();
+    HIPCHK(palloc(p, tiles.size() * sizeof(int4), &p->tiles_d));
+    HIPCHK(hipMemcpy(p->tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return BIELCC_OK;
+}
+
+int grid_for(bielcc_ctx ctx, int ntiles) {
+    return std::max(1, std::min(ntiles, ctx->n_cu * ctx->plan_wg_per_cu));
+}
+
+int run_assemble(bielcc_ctx ctx, bielcc_plan p) {
+    const Group& g = p->groups[0];
+    HIPCHK(launch_assemble_multi(ctx->stream, p->aitems_d, p->tiles_d, g.nt, g.grid, p->which, p->ck,
+                                 p->precision == BIELCC_PREC_MIXED));
+    for (size_t i = 0; i < p->aitems.size(); i++) {
+        const NearSet& ns = p->near[i];
+        if (ns.n == 0) continue;
+        const AssembleArgs& a = p->aitems[i];
+        const DevMesh& m = *p->ameshes[i];
+        HIPCHK(launch_assemble_generic(ctx->stream, m.nv, 0, ns.n, ns.lq, ns.lc, m.n, 0, m.n, m.verts, m.ctab, p->np,
+                                       a.k_re, a.k_im, a.Z, a.ld));
+    }
+    return BIELCC_OK;
+}
+
+int run_field(bielcc_ctx ctx, bielcc_plan p) {
+    for (const bielcc_field_item& it : p->fitems)
+        HIPCHK(launch_init_incident(ctx->stream, it.M, it.nrhs, it.pts_d, it.inc_d, it.E_d));
+    const bool p32 = p->precision == BIELCC_PREC_MIXED;
+    for (const Group& g : p->groups) {
+        HIPCHK(launch_field_opmat(ctx->stream, p->segs_d, p->tiles_d + g.t0, g.nt, g.grid, p->ck, p32));
+        for (int s = g.s0; s < g.s1; s++) {
+            const NearSet& ns = p->near[s];
+            if (ns.n > 0) HIPCHK(launch_opmat_near(ctx->stream, ns.n, ns.lq, ns.lc, p->segs[s], p->np, p->near_mode));
+        }
+        for (int s = g.s0; s < g.s1; s++) {
+            const SegInfo& si = p->seginfo[s];
+            const OpSeg& sg = p->segs[s];
+            const bielcc_field_item& it = p->fitems[si.item];
+            const long long m3 = 3LL * si.M, n3 = 3LL * sg.N, ldo = 3LL * it.M;
+            const char* msg = nullptr;
+            hipError_t e = blas_zgemm(ctx->stream, ctx->lu, m3, it.nrhs, n3, make_double2(it.multE_re, it.multE_im),
+                                      sg.WK, m3, it.jE_d, n3, make_double2(1., 0.), it.E_d + 6 * (size_t)si.p0, ldo,
+                                      &msg);
+            if (e == hipSuccess && it.H_d)
+                e = blas_zgemm(ctx->stream, ctx->lu, m3, it.nrhs, n3, make_double2(1., 0.), sg.WR, m3, it.jE_d, n3,
+                               make_double2(0., 0.), it.H_d + 6 * (size_t)si.p0, ldo, &msg);
+            if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+        }
+    }
+    return BIELCC_OK;
+}
+
+int run_impl(bielcc_ctx ctx, bielcc_plan p) {
+    return p->kind == 0 ? run_assemble(ctx, p) : run_field(ctx, p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bielcc_plan_assemble(bielcc_ctx ctx, int nitems, const bielcc_assemble_item* items, int which,
+                         const bielcc_num_param* np, int precision, bielcc_plan* out) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (!out || nitems < 1 || !items) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    *out = nullptr;
+    if (int rc = check_np(ctx, np)) return rc;
+    if (which != 0 && which != 1) return fail(ctx, BIELCC_ERR_ARG, "which must be 0 (K) or 1 (R)");
+    if (precision != BIELCC_PREC_FP64 && precision != BIELCC_PREC_MIXED)
+        return fail(ctx, BIELCC_ERR_ARG, "precision must be BIELCC_PREC_FP64 or BIELCC_PREC_MIXED");
+    const int nn = np->n_start * np->n_start;
+    if (np->p_max != 1 || np->p_max_seg != 1 || nn > 512 || !(np->analytic_calc > 1.))
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED,
+                    "plans need p_max = p_max_seg = 1, n_start <= 22 and analytic_calc > 1 (use bielcc_assemble)");
+    DeviceGuard dg(ctx->device);
+    bielcc_plan p = new bielcc_plan_s();
+    p->kind = 0;
+    p->which = which;
+    p->precision = precision;
+    p->np = to_np(np);
+    std::vector<int4> tiles;
+    const int cpt = ctx->plan_asm_cells;
+    for (int i = 0; i < nitems; i++) {
+        const bielcc_assemble_item& it = items[i];
+        DevMesh* m = find_mesh(ctx, it.mesh_id);
+        if (!m) { plan_free(p); return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id in item " + std::to_string(i)); }
+        if (!it.Z_d || it.ld < 2LL * m->n || !std::isfinite(it.k_re) || !std::isfinite(it.k_im)) {
+            plan_free(p);
+            return fail(ctx, BIELCC_ERR_ARG, "item " + std::to_string(i) + ": Z is NULL, ld < 2N or k not finite");
+        }
+        if (which == 0 && m->nv * np->n_start_seg > 256) {
+            plan_free(p);
+            return fail(ctx, BIELCC_ERR_UNSUPPORTED, "n_start_seg too large for the tile kernel");
+        }
+        int rc = ensure_tables(ctx, *m, np->n_start, which == 0 ? np->n_start_seg : 0);
+        if (!rc && precision == BIELCC_PREC_MIXED) rc = ensure_tables32(ctx, *m, np->n_start);
+        if (rc) { plan_free(p); return rc; }
+        AssembleArgs a;
+        a.nv = m->nv; a.N = m->n; a.nn = nn; a.ns = np->n_start_seg; a.which = which;
+        a.i0 = 0; a.ni = m->n; a.ctab = m->ctab; a.nodes = m->nodes; a.segs = m->segs;
+        a.k_re = it.k_re; a.k_im = it.k_im; a.ac = np->analytic_calc; a.Z = it.Z_d; a.ld = it.ld;
+        a.fast_phase = fast_phase_ok(it.k_re, m->grid_step) && !ctx->no_fast_phase;
+        a.prec = precision; a.nodes32 = m->nodes32; a.nodes32_yy = m->nodes32_yy; a.wg_target = 0;
+        p->aitems.push_back(a);
+        p->ameshes.push_back(m);
+        p->ck = p->ck || it.k_im != 0.;
+        for (int bx = 0; bx < (m->n + 255) / 256; bx++)
+            for (int c0 = 0; c0 < m->n; c0 += cpt) tiles.push_back(make_int4(i, bx, c0, std::min(m->n, c0 + cpt)));
+        p->pairs += (long long)m->n * m->n;
+        NearSet ns;
+        if (which == 0) {
+            if (int rc2 = plan_near(ctx, p, m->n, m->ctab + CT_RKT, CT, *m, 0, np->analytic_calc, 0., &ns)) {
+                plan_free(p);
+                return rc2;
+            }
+        }
+        p->near.push_back(ns);
+        p->near_pairs += ns.n;
+    }
+    hipError_t e = palloc(p, p->aitems.size() * sizeof(AssembleArgs), &p->aitems_d);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->aitems_d, p->aitems.data(), p->aitems.size() * sizeof(AssembleArgs), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { plan_free(p); return hip_fail(ctx, e, "plan upload"); }
+    if (int rc = upload_tiles(ctx, p, tiles)) { plan_free(p); return rc; }
+    p->groups.push_back(Group{0, (int)tiles.size(), 0, nitems, grid_for(ctx, (int)tiles.size())});
+    *out = p;
+    return BIELCC_OK;
+}
+
+int bielcc_plan_field(bielcc_ctx ctx, int nitems, const bielcc_field_item* items, const bielcc_num_param* np,
+                      int near_mode, int precision, bielcc_plan* out) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (!out || nitems < 1 || !items) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    *out = nullptr;
+    if (int rc = check_np(ctx, np)) return rc;
+    if (near_mode < 0 || near_mode > 1) return fail(ctx, BIELCC_ERR_ARG, "near_mode must be 0 or 1");
+    if (precision != BIELCC_PREC_FP64 && precision != BIELCC_PREC_MIXED)
+        return fail(ctx, BIELCC_ERR_ARG, "precision must be BIELCC_PREC_FP64 or BIELCC_PREC_MIXED");
+    const int nn = np->n_start * np->n_start;
+    if (np->p_max != 1 || nn > 512 || !(np->analytic_calc > 1.))
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED, "plans need p_max = 1, n_start <= 22 and analytic_calc > 1");
+    DeviceGuard dg(ctx->device);
+    bielcc_plan p = new bielcc_plan_s();
+    p->kind = 1;
+    p->precision = precision;
+    p->near_mode = near_mode;
+    p->np = to_np(np);
+    const size_t cap = (size_t)ctx->plan_ws_mb << 20;
+    // segments: items split into point ranges whose operator matrices fit the workspace
+    std::vector<const DevMesh*> meshes;
+    for (int i = 0; i < nitems; i++) {
+        const bielcc_field_item& it = items[i];
+        DevMesh* m = find_mesh(ctx, it.mesh_id);
+        if (!m) { plan_free(p); return fail(ctx, BIELCC_ERR_NOTFOUND, "unknown mesh id in item " + std::to_string(i)); }
+        if (it.nrhs < 1 || it.M < 0 || (it.M > 0 && (!it.jE_d || !it.pts_d || !it.E_d)) ||
+            !std::isfinite(it.k_re) || !std::isfinite(it.k_im)) {
+            plan_free(p);
+            return fail(ctx, BIELCC_ERR_ARG, "item " + std::to_string(i) + ": bad nrhs, M, buffer or wave number");
+        }
+        int rc = ensure_tables(ctx, *m, np->n_start, 0);
+        if (!rc && precision == BIELCC_PREC_MIXED) rc = ensure_tables32(ctx, *m, np->n_start);
+        if (rc) { plan_free(p); return rc; }
+        meshes.push_back(m);
+        p->fitems.push_back(it);
+        p->fN.push_back(m->n);
+        p->ck = p->ck || it.k_im != 0.;
+        const size_t per_pt = (it.H_d ? 2 : 1) * (size_t)9 * 16 * m->n;   // bytes of WK (+WR) per point
+        long long chunk = (long long)(cap / per_pt) / 256 * 256;
+        if (it.M > 0 && chunk < 256) {
+            plan_free(p);
+            return fail(ctx, BIELCC_ERR_UNSUPPORTED, "operator-matrix workspace (plan_ws_mb) below 256 points of item " +
+                                                         std::to_string(i));
+        }
+        for (int p0 = 0; p0 < it.M; p0 += (int)std::min<long long>(chunk, it.M)) {
+            const int Ms = (int)std::min<long long>(chunk, it.M - p0);
+            OpSeg sg;
+            sg.M = Ms; sg.N = m->n; sg.nn = nn; sg.nv = m->nv;
+            sg.pts = it.pts_d + 3 * (size_t)p0; sg.ctab = m->ctab; sg.nodes = m->nodes;
+            sg.nodes32 = m->nodes32; sg.nodes32_yy = m->nodes32_yy; sg.verts = m->verts;
+            sg.k_re = it.k_re; sg.k_im = it.k_im;
+            sg.fast_phase = fast_phase_ok(it.k_re, m->grid_step) && !ctx->no_fast_phase;
+            sg.mode = near_mode; sg.ac = np->analytic_calc; sg.thresh = np->analytic_calc * m->grid_step;
+            sg.WK = sg.WR = nullptr;
+            p->segs.push_back(sg);
+            p->seginfo.push_back(SegInfo{i, p0, Ms, 0});
+            p->pairs += (long long)Ms * m->n;
+        }
+    }
+    // workspace groups: consecutive segments whose matrices fit the cap together
+    std::vector<int4> tiles;
+    const int cpt = ctx->plan_field_cells;
+    size_t off = 0;
+    int s0 = 0;
+    auto close_group = [&](int s1) {
+        if (s1 <= s0) return;
+        const int t0 = (int)tiles.size();
+        for (int s = s0; s < s1; s++) {
+            const OpSeg& sg = p->segs[s];
+            for (int pb = 0; pb < sg.M; pb += 256)
+                for (int c0 = 0; c0 < sg.N; c0 += cpt) tiles.push_back(make_int4(s, pb, c0, std::min(sg.N, c0 + cpt)));
+        }
+        const int nt = (int)tiles.size() - t0;
+        p->groups.push_back(Group{t0, nt, s0, s1, grid_for(ctx, nt)});
+        p->ws_bytes = std::max(p->ws_bytes, off);
+        off = 0;
+        s0 = s1;
+    };
+    for (int s = 0; s < (int)p->segs.size(); s++) {
+        const SegInfo& si = p->seginfo[s];
+        const size_t wk = align256((size_t)9 * 16 * si.M * p->segs[s].N);
+        const size_t need = wk * (p->fitems[si.item].H_d ? 2 : 1);
+        if (off + need > cap) close_group(s);
+        p->seginfo[s].ws_off = off;
+        off += need;
+    }
+    close_group((int)p->segs.size());
+    hipError_t e = hipSuccess;
+    if (p->ws_bytes > 0) e = palloc(p, p->ws_bytes, &p->ws);
+    if (e != hipSuccess) { plan_free(p); return hip_fail(ctx, e, "operator-matrix workspace"); }
+    for (size_t s = 0; s < p->segs.size(); s++) {
+        const SegInfo& si = p->seginfo[s];
+        const size_t wk = align256((size_t)9 * 16 * si.M * p->segs[s].N);
+        p->segs[s].WK = reinterpret_cast<double*>(p->ws + si.ws_off);
+        p->segs[s].WR = p->fitems[si.item].H_d ? reinterpret_cast<double*>(p->ws + si.ws_off + wk) : nullptr;
+    }
+    // near-pair lists per segment
+    for (size_t s = 0; s < p->segs.size(); s++) {
+        const OpSeg& sg = p->segs[s];
+        NearSet ns;
+        if (int rc = plan_near(ctx, p, sg.M, sg.pts, 3, *meshes[p->seginfo[s].item], near_mode, sg.ac, sg.thresh, &ns)) {
+            plan_free(p);
+            return rc;
+        }
+        p->near.push_back(ns);
+        p->near_pairs += ns.n;
+    }
+    if (!p->segs.empty()) {
+        e = palloc(p, p->segs.size() * sizeof(OpSeg), &p->segs_d);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->segs_d, p->segs.data(), p->segs.size() * sizeof(OpSeg), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { plan_free(p); return hip_fail(ctx, e, "plan upload"); }
+        if (int rc = upload_tiles(ctx, p, tiles)) { plan_free(p); return rc; }
+    }
+    // the BLAS handle (and its workspace) exists before any capture
+    const char* msg = nullptr;
+    e = blas_zgemm(ctx->stream, ctx->lu, 0, 0, 0, make_double2(0., 0.), nullptr, 1, nullptr, 1, make_double2(0., 0.),
+                   nullptr, 1, &msg);
+    if (e != hipSuccess) { plan_free(p); return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e)); }
+    *out = p;
+    return BIELCC_OK;
+}
+
+int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan p) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (!p) return fail(ctx, BIELCC_ERR_ARG, "plan is NULL");
+    DeviceGuard dg(ctx->device);
+    Timer t(ctx, p->kind == 0 ? "plan_assemble" : "plan_field");
+    if (p->exec) {
+        HIPCHK(hipGraphLaunch(p->exec, ctx->stream));
+    } else if (int rc = run_impl(ctx, p)) {
+        return rc;
+    }
+    t.stop();
+    p->runs++;
+    return BIELCC_OK;
+}
+
+int bielcc_plan_capture(bielcc_ctx ctx, bielcc_plan p) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (!p) return fail(ctx, BIELCC_ERR_ARG, "plan is NULL");
+    DeviceGuard dg(ctx->device);
+    if (p->exec) return BIELCC_OK;
+    // one eager run first: libraries finish their lazy set-up outside the capture
+    if (int rc = run_impl(ctx, p)) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    int rc = run_impl(ctx, p);
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    p->graph = g;
+    p->exec = x;
+    return BIELCC_OK;
+}
+
+int bielcc_plan_destroy(bielcc_ctx ctx, bielcc_plan p) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (!p) return BIELCC_OK;
+    DeviceGuard dg(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    plan_free(p);
+    return BIELCC_OK;
+}
+
+int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan p, const char* key, double* value) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (!p || !key || !value) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    const std::string k(key);
+    if (k == "tiles") *value = (double)p->ntiles;
+    else if (k == "launches") *value = (double)p->groups.size();
+    else if (k == "grid") *value = p->groups.empty() ? 0. : (double)p->groups[0].grid;
+    else if (k == "pairs") *value = (double)p->pairs;
+    else if (k == "near_pairs") *value = (double)p->near_pairs;
+    else if (k == "segments") *value = (double)(p->kind == 1 ? p->segs.size() : p->aitems.size());
+    else if (k == "workspace_bytes") *value = (double)p->ws_bytes;
+    else if (k == "captured") *value = p->exec ? 1. : 0.;
+    else return fail(ctx, BIELCC_ERR_ARG, "unknown plan info key " + k);
+    return BIELCC_OK;
+}
+
+}  // extern "C"

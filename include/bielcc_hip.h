@@ -255,13 +255,71 @@ int bielcc_far_pattern(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, 
                        double multH_re, double multH_im, int precision, int ndir, const double* dirs_d,
                        double* Einf_d, double* Hinf_d);
 
+/* ------------------------------------------------------------ plans: several meshes per launch
+ * No reference counterpart (the reference evaluates one mesh per call and has
+ * no assembly loop).  A plan fixes its items -- mesh, wave number and the
+ * caller's device buffers -- at creation, when it builds the tile list of its
+ * persistent kernel, the near-pair lists and its workspace (plan-owned).
+ * bielcc_plan_run then enqueues kernels and GEMMs only: no allocation, no
+ * host synchronisation, no memset.  bielcc_plan_capture runs the plan once
+ * and records it into a HIP graph; later bielcc_plan_run calls replay it.
+ * Between runs the caller may change the CONTENTS of the item buffers
+ * (currents, incidences, points), not their addresses.  Tile shapes come from
+ * the "plan_*" tunables at creation. */
+typedef struct bielcc_plan_s* bielcc_plan;
+
+typedef struct {
+    int mesh_id;
+    double k_re, k_im;
+    double* Z_d;      /* 2N x 2N complex, column-major, leading dimension ld */
+    int64_t ld;
+} bielcc_assemble_item;
+
+/* bielcc_assemble_ex (all rows) for every item: the far pairs of all items in
+ * one persistent launch, then the exact near pairs per item (K only). */
+int bielcc_plan_assemble(bielcc_ctx ctx, int nitems, const bielcc_assemble_item* items, int which,
+                         const bielcc_num_param* np, int precision, bielcc_plan* out);
+
+typedef struct {
+    int mesh_id;
+    double k_re, k_im;
+    int nrhs;
+    const double* jE_d;      /* [nrhs][N][3] complex electric currents */
+    const double* inc_d;     /* DEVICE [nrhs][6] (kvec, e0) plane waves added to E, or NULL */
+    double multE_re, multE_im;
+    int M;
+    const double* pts_d;     /* [M][3] */
+    double* E_d;             /* [nrhs][M][3] complex */
+    double* H_d;             /* [nrhs][M][3] complex, or NULL */
+} bielcc_field_item;
+
+/* bielcc_field_batch with jM = NULL for every item:
+ *   E = multE sum_j K[jE_j] (+ E_inc),  H = sum_j R[jE_j].
+ * Evaluated through the operator matrices WK, WR ([3M x 3N] per item): one
+ * persistent launch writes the far-pair blocks of all items, the exact
+ * kernel the near-pair blocks, and E, H are complex GEMMs over all current
+ * sets.  Items whose matrices exceed the "plan_ws_mb" workspace are split
+ * into point ranges and groups of launches. */
+int bielcc_plan_field(bielcc_ctx ctx, int nitems, const bielcc_field_item* items, const bielcc_num_param* np,
+                      int near_mode, int precision, bielcc_plan* out);
+int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan plan);
+int bielcc_plan_capture(bielcc_ctx ctx, bielcc_plan plan);
+int bielcc_plan_destroy(bielcc_ctx ctx, bielcc_plan plan);
+/* keys: "tiles", "launches", "grid", "pairs", "near_pairs", "segments",
+ * "workspace_bytes", "captured" */
+int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* value);
+
 /* Launch-shape tunables (the autotuner sets these; defaults in brackets):
  * "field_wg" [2048] and "assemble_wg" [2048] = workgroups a tile launch aims
  * for (cells are split until the grid reaches it); "field_rchunk" [4] =
  * current sets per field pass (1, 2 or 4); "trsv" [1] = solves with at most
  * 8 right-hand sides use the one-kernel-per-triangle solve (0: blocked
- * TRSM + GEMM).  Returns BIELCC_ERR_ARG for an unknown key or a value out of
- * range. */
+ * TRSM + GEMM).  Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
+ * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
+ * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
+ * bielcc_get_tuning also reads
+ * "n_cu" (compute units).  Returns BIELCC_ERR_ARG for an unknown key or a
+ * value out of range. */
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);
 int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);
 
