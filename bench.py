@@ -275,6 +275,200 @@ def serving_leg(nq=96, workers=4):
     return out
 
 
+def batched_workload(seed=4, n_mesh=12, n_dir=64, n_pts=2000):
+    """12 seeded closed meshes: spheres, triaxial ellipsoids (semi-axes in
+    [0.5, 1.5]) and tori (R = 1, r in [0.2, 0.4]) of 500-8,000 quads, every
+    second one triangulated; ka uniform in [1, 12]; 64 incidence directions
+    uniform on the sphere with orthogonal polarisations; 2,000 field points on a
+    sphere of radius 3."""
+    from bielcc_amd import mesh
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_mesh):
+        kind = ("sphere", "ellipsoid", "torus")[i % 3]
+        target = int(rng.uniform(500, 8000))
+        if kind == "sphere":
+            nt = max(4, int(round(np.sqrt(target / 2))))
+            v = mesh.sphere_quads(nt, 2 * nt, jitter=0.005, seed=200 + i)
+        elif kind == "ellipsoid":
+            nt = max(4, int(round(np.sqrt(target / 2))))
+            v = mesh.ellipsoid_quads(nt, 2 * nt, axes=tuple(rng.uniform(0.5, 1.5, 3)), seed=200 + i)
+        else:
+            nv = max(4, int(round(np.sqrt(target / 3))))
+            v = mesh.torus_quads(3 * nv, nv, R=1.0, r=float(rng.uniform(0.2, 0.4)))
+        tri = i % 2 == 1
+        if tri:
+            v = mesh.triangulate(v)
+        k = float(rng.uniform(1.0, 12.0))
+        d = rng.normal(size=(n_dir, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pol = np.cross(d, rng.normal(size=(n_dir, 3)))
+        pol /= np.linalg.norm(pol, axis=1, keepdims=True)
+        pts = mesh.sphere_points(n_pts, 3.0, seed=300 + i)
+        out.append(dict(kind=kind, tri=tri, verts=v, k=k, dirs=d, pols=pol, pts=pts))
+    return out
+
+
+def batched_leg(tune_cache=None):
+    """Batched form on the 12-mesh workload, FP32 LU + FP64 refinement:
+    assembly of all matrices in one persistent launch, FP32 LU and refinement
+    (64 right-hand sides) per mesh, the field of all 768 current sets through
+    one captured field plan (operator matrices + ZGEMM), with tile shapes from
+    the autotuner.  The same batch in the FP64 form gives the reference for the
+    errors and the time ratio; the field is also timed through the per-mesh
+    driver (passes of 4 current sets) for comparison."""
+    import bielcc_amd as B
+    from bielcc_amd import tune as T
+    work = batched_workload()
+    ctx = B.Context(0)
+    npp = B.num_param()
+    nd, M = work[0]["dirs"].shape[0], work[0]["pts"].shape[0]
+    ms = [ctx.mesh(w["verts"]) for w in work]
+    bufs = []
+    for w, m in zip(work, ms):
+        n, n2 = m.n, 2 * m.n
+        inc = np.concatenate([w["k"] * w["dirs"], w["pols"]], axis=1)
+        bufs.append(dict(Z=ctx.empty(n2 * n2 * 16), F=ctx.empty(n2 * n2 * 8), piv=ctx.empty(n2 * 4),
+                         B=ctx.empty(n2 * nd * 16), J=ctx.empty(n * nd * 48), P=ctx.to_device(w["pts"]),
+                         inc=ctx.to_device(inc), E=ctx.empty(nd * M * 48), H=ctx.empty(nd * M * 48)))
+    cells = [m.n for m in ms]
+    sig = f"batched12|cells{sum(cells)}|nd{nd}|M{M}"
+    log(f"[bench] batched: {len(ms)} meshes, {min(cells)}-{max(cells)} cells ({sum(cells)} total), "
+        f"{nd} incidences, {M} points each")
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    # assembly: all matrices in one persistent launch, tile shape tuned
+    aitems = [(m, w["k"], b["Z"], 2 * m.n) for w, m, b in zip(work, ms, bufs)]
+    tune_a = T.tune(ctx, lambda: ctx.plan_assemble(aitems, npp), T.ASSEMBLE_SPACE, "plan_assemble",
+                    sig + "|asm", repeats=1, cache_path=tune_cache, log=log)
+    aplan = ctx.plan_assemble(aitems, npp)
+    t_asm = timed(aplan.run)
+    a_info = {k: aplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs")}
+    aplan.close()
+    # FP32 LU per mesh, refinement with 64 right-hand sides, currents
+    t_lu = timed(lambda: [ctx.lu32_factor(b["Z"], 2 * m.n, 2 * m.n, b["F"], 2 * m.n, b["piv"])
+                          for m, b in zip(ms, bufs)])
+    refine, fallbacks = [], []
+
+    def solve_all():
+        for i, (w, m, b) in enumerate(zip(work, ms, bufs)):
+            n2 = 2 * m.n
+            m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
+            try:
+                refine.append(ctx.refine_solve(b["Z"], n2, n2, b["F"], n2, b["piv"], b["B"], nd, n2, 1e-10, 30))
+            except B.BielccError as err:
+                if err.code != 4:      # refinement stalled: FP64 factors for this mesh (timed)
+                    raise
+                fallbacks.append(i)
+                lu = ctx.empty(n2 * n2 * 16)
+                ctx.copy(lu, b["Z"], n2 * n2 * 16)
+                ctx.lu_factor(lu, n2, n2, b["piv"])
+                m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
+                ctx.lu_solve(lu, n2, n2, b["piv"], b["B"], nd, n2)
+                ctx.sync()
+                lu.free()
+            m.currents_into(b["B"], n2, nd, b["J"])
+    t_solve = timed(solve_all)
+    # field of every current set: one field plan (mixed), tuned, captured, replayed
+    fitems = [dict(mesh=m, k=w["k"], nrhs=nd, jE=b["J"], inc=b["inc"], M=M, pts=b["P"], E=b["E"], H=b["H"])
+              for w, m, b in zip(work, ms, bufs)]
+    ctx.set_tuning("plan_ws_mb", 96 * 1024)   # every mesh's operator matrices in one workspace: one launch
+    tune_f = T.tune(ctx, lambda: ctx.plan_field(fitems, npp, precision=B.PREC_MIXED), T.FIELD_SPACE, "plan_field",
+                    sig + "|field32", repeats=2, cache_path=tune_cache, log=log)
+    fplan = ctx.plan_field(fitems, npp, precision=B.PREC_MIXED)
+    t_field_eager = timed(fplan.run)
+    fplan.capture()
+    reps = 3
+    t_field = timed(lambda: [fplan.run() for _ in range(reps)]) / reps
+    f_info = {k: fplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs", "workspace_bytes")}
+    fplan.close()
+    E32 = [b["E"].to_numpy((nd, M, 3), np.complex128) for b in bufs]
+    H32 = [b["H"].to_numpy((nd, M, 3), np.complex128) for b in bufs]
+    J32 = [b["J"].to_numpy((nd, m.n, 3), np.complex128) for m, b in zip(ms, bufs)]
+    # the same fields through the per-mesh driver (current sets in passes of 4)
+    t_field_driver = timed(lambda: [m.field_batch_into(npp, w["k"], nd, b["J"], b["P"], M, b["E"], b["H"],
+                                                       inc=np.concatenate([w["k"] * w["dirs"], w["pols"]], axis=1),
+                                                       precision=B.PREC_MIXED)
+                                    for w, m, b in zip(work, ms, bufs)])
+    # FP64 form of the same batch: FP64 LU on a copy of each matrix, FP64 solve, FP64 field plan
+    n2max = 2 * max(cells)
+    dLU, dpiv = ctx.empty(n2max * n2max * 16), ctx.empty(n2max * 4)
+    t_lu64 = t_solve64 = 0.0
+    for w, m, b in zip(work, ms, bufs):
+        n2 = 2 * m.n
+        ctx.copy(dLU, b["Z"], n2 * n2 * 16)
+        t_lu64 += timed(lambda: ctx.lu_factor(dLU, n2, n2, dpiv))
+
+        def s64():
+            m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
+            ctx.lu_solve(dLU, n2, n2, dpiv, b["B"], nd, n2)
+            m.currents_into(b["B"], n2, nd, b["J"])
+        t_solve64 += timed(s64)
+    dLU.free()
+    dpiv.free()
+    for b in bufs:
+        b["F"].free()
+    fplan64 = ctx.plan_field(fitems, npp, precision=B.PREC_FP64)
+    fplan64.run()
+    t_field64 = timed(fplan64.run)
+    fplan64.close()
+    ctx.set_tuning("plan_ws_mb", 24576)
+    e_cur, e_pt, e_set = 0.0, 0.0, 0.0
+    for i, (w, m, b) in enumerate(zip(work, ms, bufs)):
+        J64 = b["J"].to_numpy((nd, m.n, 3), np.complex128)
+        E64 = b["E"].to_numpy((nd, M, 3), np.complex128)
+        H64 = b["H"].to_numpy((nd, M, 3), np.complex128)
+        e_cur = max(e_cur, float(np.linalg.norm(J32[i] - J64) / np.linalg.norm(J64)))
+        for r in range(nd):
+            einc = w["pols"][r][None, :] * np.exp(1j * w["k"] * (w["pts"] @ w["dirs"][r]))[:, None]
+            for X32, X64, sub in ((E32[i][r], E64[r], einc), (H32[i][r], H64[r], 0.0)):
+                sc = X64 - sub
+                nsc = np.linalg.norm(sc, axis=1)
+                e_pt = max(e_pt, float((np.linalg.norm(X32 - X64, axis=1) / np.maximum(nsc, 1e-3 * nsc.max())).max()))
+                e_set = max(e_set, float(np.linalg.norm(X32 - X64) / np.linalg.norm(sc)))
+    for b in bufs:
+        for d in b.values():
+            if d.ptr:
+                d.free()
+    for m in ms:
+        m.close()
+    ctx.close()
+    pairs_asm, pairs_fld = a_info["pairs"], f_info["pairs"]
+    total32 = t_asm + t_lu + t_solve + t_field
+    total64 = t_asm + t_lu64 + t_solve64 + t_field64
+    solves = len(work) * nd
+    res = {"workload": f"{len(work)} meshes (spheres, ellipsoids, tori; half triangulated), {min(cells)}-{max(cells)} "
+                       f"cells, ka in [1,12], {nd} incidences and {M} points (r = 3) per mesh",
+           "meshes": [{"kind": w["kind"], "triangles": w["tri"], "cells": m_n, "ka": round(w["k"], 3)}
+                      for w, m_n in zip(work, cells)],
+           "tuned": {"assemble": tune_a["best"], "field": tune_f["best"], "cached": tune_a["cached"] and tune_f["cached"],
+                     "tuning_s": round(tune_a.get("tuning_s", 0) + tune_f.get("tuning_s", 0), 2)},
+           "assembly_plan": a_info, "field_plan": f_info,
+           "assembly_s": round(t_asm, 4), "lu32_s": round(t_lu, 3), "refine_solve_s": round(t_solve, 3),
+           "refine": {"max_corrections": max((r[0] for r in refine), default=0),
+                      "max_residual": max((r[1] for r in refine), default=0.0),
+                      "fp64_fallback_meshes": fallbacks},
+           "field_graph_replay_s": round(t_field, 4), "field_eager_s": round(t_field_eager, 4),
+           "field_per_mesh_driver_s": round(t_field_driver, 4),
+           "field_plan_speedup_vs_driver": round(t_field_driver / t_field, 2),
+           "assembly_pairs_per_s": pairs_asm / t_asm, "field_pair_sets_per_s": pairs_fld * nd / t_field,
+           "total_s": round(total32, 3), "solves": solves, "solves_per_s": round(solves / total32, 2),
+           "fp64_form": {"lu64_s": round(t_lu64, 3), "solve64_s": round(t_solve64, 3), "field64_s": round(t_field64, 4),
+                         "total_s": round(total64, 3), "solves_per_s": round(solves / total64, 2)},
+           "solves_per_s_ratio_vs_fp64": round(total64 / total32, 3),
+           "err_currents_rel_max": e_cur, "err_field_point_max_rel": e_pt, "err_field_set_max_rel": e_set}
+    log(f"[bench] batched: assembly {t_asm:.3f} s, LU32 {t_lu:.2f} s, refine+solve {t_solve:.2f} s, field "
+        f"{t_field * 1e3:.1f} ms (graph; eager {t_field_eager * 1e3:.1f} ms, per-mesh driver "
+        f"{t_field_driver * 1e3:.1f} ms); FP64 form {total64:.2f} s vs {total32:.2f} s; field err {e_pt:.2e}")
+    return res
+
+
 def copy_calibration(nbytes=1 << 30):
     """A plain device copy of known size (torch), so that rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE can be scaled against it in the same run."""
@@ -300,6 +494,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-queries", type=int, default=96)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 12-mesh batched form")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path is run on one MI355X; multi-GPU sharding is a design "
@@ -307,6 +504,9 @@ def main():
         sys.exit(2)
 
     import bielcc_amd as B
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args.tune_cache)}), flush=True)
+        return
     if args.copy_calib:
         copy_calibration()
     w = workload(args.config)
@@ -397,9 +597,12 @@ def main():
     if lu_s is not None and not args.no_mixed:
         out["precision_form"] = mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ, dE, dH, lu_s, solve_s,
                                            elapsed / args.steps)
-    if not args.no_serve:
+    if not (args.no_serve and args.no_batched):
         for d in (dZ, dB, dpiv):
             d.free()
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.tune_cache)
+    if not args.no_serve:
         out["serving"] = serving_leg(nq=args.serve_queries)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
