@@ -309,7 +309,39 @@ def batched_workload(seed=4, n_mesh=12, n_dir=64, n_pts=2000):
     return out
 
 
-def batched_leg(tune_cache=None):
+def lpt_assign(weights, n):
+    """Longest-processing-time assignment of jobs (by weight) to n workers."""
+    loads, out = [0.0] * n, [[] for _ in range(n)]
+    for i in sorted(range(len(weights)), key=lambda j: -weights[j]):
+        t = min(range(n), key=lambda u: loads[u])
+        out[t].append(i)
+        loads[t] += weights[i]
+    return out
+
+
+def run_workers(ctxs, assign, fn):
+    """fn(ctx, job) for every job of every worker, one host thread per worker
+    context (the library releases the GIL); re-raises the first failure."""
+    import threading
+    errs = []
+
+    def go(t):
+        try:
+            for i in assign[t]:
+                fn(ctxs[t], i)
+            ctxs[t].sync()
+        except BaseException as exc:   # noqa: BLE001 - re-raised below
+            errs.append(exc)
+    th = [threading.Thread(target=go, args=(t,)) for t in range(len(ctxs))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    if errs:
+        raise errs[0]
+
+
+def batched_leg(tune_cache=None, workers=4):
     """Batched form on the 12-mesh workload, FP32 LU + FP64 refinement:
     assembly of all matrices in one persistent launch, FP32 LU and refinement
     (64 right-hand sides) per mesh, the field of all 768 current sets through
@@ -351,30 +383,38 @@ def batched_leg(tune_cache=None):
     t_asm = timed(aplan.run)
     a_info = {k: aplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs")}
     aplan.close()
-    # FP32 LU per mesh, refinement with 64 right-hand sides, currents
-    t_lu = timed(lambda: [ctx.lu32_factor(b["Z"], 2 * m.n, 2 * m.n, b["F"], 2 * m.n, b["piv"])
-                          for m, b in zip(ms, bufs)])
-    refine, fallbacks = [], []
+    # FP32 LU per mesh and refinement with 64 right-hand sides: the independent
+    # systems run concurrently on worker contexts (own stream and BLAS handle each)
+    wctx = [B.Context(0) for _ in range(workers)]
+    assign = lpt_assign([float(m.n) ** 3 for m in ms], workers)
+    wmesh = {}
+    for t, idx in enumerate(assign):
+        for i in idx:
+            wmesh[i] = wctx[t].mesh(work[i]["verts"])
+    t_lu = timed(lambda: run_workers(wctx, assign, lambda c, i: c.lu32_factor(
+        bufs[i]["Z"], 2 * ms[i].n, 2 * ms[i].n, bufs[i]["F"], 2 * ms[i].n, bufs[i]["piv"])))
+    refine, fallbacks = {}, []
 
-    def solve_all():
-        for i, (w, m, b) in enumerate(zip(work, ms, bufs)):
-            n2 = 2 * m.n
+    def solve_one(c, i):
+        w, m, b = work[i], wmesh[i], bufs[i]
+        n2 = 2 * m.n
+        m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
+        try:
+            refine[i] = c.refine_solve(b["Z"], n2, n2, b["F"], n2, b["piv"], b["B"], nd, n2, 1e-10, 30)
+        except B.BielccError as err:
+            if err.code != 4:      # refinement stalled: FP64 factors for this mesh (timed)
+                raise
+            fallbacks.append(i)
+            lu = c.empty(n2 * n2 * 16)
+            c.copy(lu, b["Z"], n2 * n2 * 16)
+            c.lu_factor(lu, n2, n2, b["piv"])
             m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
-            try:
-                refine.append(ctx.refine_solve(b["Z"], n2, n2, b["F"], n2, b["piv"], b["B"], nd, n2, 1e-10, 30))
-            except B.BielccError as err:
-                if err.code != 4:      # refinement stalled: FP64 factors for this mesh (timed)
-                    raise
-                fallbacks.append(i)
-                lu = ctx.empty(n2 * n2 * 16)
-                ctx.copy(lu, b["Z"], n2 * n2 * 16)
-                ctx.lu_factor(lu, n2, n2, b["piv"])
-                m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
-                ctx.lu_solve(lu, n2, n2, b["piv"], b["B"], nd, n2)
-                ctx.sync()
-                lu.free()
-            m.currents_into(b["B"], n2, nd, b["J"])
-    t_solve = timed(solve_all)
+            c.lu_solve(lu, n2, n2, b["piv"], b["B"], nd, n2)
+            c.sync()
+            lu.free()
+        m.currents_into(b["B"], n2, nd, b["J"])
+    t_solve = timed(lambda: run_workers(wctx, assign, solve_one))
+    refine = list(refine.values())
     # field of every current set: one field plan (mixed), tuned, captured, replayed
     fitems = [dict(mesh=m, k=w["k"], nrhs=nd, jE=b["J"], inc=b["inc"], M=M, pts=b["P"], E=b["E"], H=b["H"])
               for w, m, b in zip(work, ms, bufs)]
@@ -396,31 +436,40 @@ def batched_leg(tune_cache=None):
                                                        inc=np.concatenate([w["k"] * w["dirs"], w["pols"]], axis=1),
                                                        precision=B.PREC_MIXED)
                                     for w, m, b in zip(work, ms, bufs)])
-    # FP64 form of the same batch: FP64 LU on a copy of each matrix, FP64 solve, FP64 field plan
-    n2max = 2 * max(cells)
-    dLU, dpiv = ctx.empty(n2max * n2max * 16), ctx.empty(n2max * 4)
-    t_lu64 = t_solve64 = 0.0
-    for w, m, b in zip(work, ms, bufs):
-        n2 = 2 * m.n
-        ctx.copy(dLU, b["Z"], n2 * n2 * 16)
-        t_lu64 += timed(lambda: ctx.lu_factor(dLU, n2, n2, dpiv))
-
-        def s64():
-            m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
-            ctx.lu_solve(dLU, n2, n2, dpiv, b["B"], nd, n2)
-            m.currents_into(b["B"], n2, nd, b["J"])
-        t_solve64 += timed(s64)
-    dLU.free()
-    dpiv.free()
+    # FP64 form of the same batch (same workers): FP64 LU of a copy of each matrix, FP64 solve, FP64 field plan
     for b in bufs:
         b["F"].free()
+    tmp = [(c.empty(max((4 * ms[i].n ** 2 * 16 for i in idx), default=16)),
+            c.empty(max((2 * ms[i].n * 4 for i in idx), default=4))) for c, idx in zip(wctx, assign)]
+
+    def solve64_one(c, i):
+        w, m, b = work[i], wmesh[i], bufs[i]
+        n2 = 2 * m.n
+        lu, piv = tmp[wctx.index(c)]
+        c.copy(lu, b["Z"], n2 * n2 * 16)
+        c.lu_factor(lu, n2, n2, piv)
+        m.rhs_planewave_into(b["B"], n2, w["k"] * w["dirs"], w["pols"])
+        c.lu_solve(lu, n2, n2, piv, b["B"], nd, n2)
+        m.currents_into(b["B"], n2, nd, b["J"])
+    t_lu64 = timed(lambda: run_workers(wctx, assign, solve64_one))
+    t_solve64 = 0.0
+    for lu, piv in tmp:
+        lu.free()
+        piv.free()
+    for i, m in wmesh.items():
+        m.close()
+    for c in wctx:
+        c.close()
+
     fplan64 = ctx.plan_field(fitems, npp, precision=B.PREC_FP64)
     fplan64.run()
     t_field64 = timed(fplan64.run)
     fplan64.close()
     ctx.set_tuning("plan_ws_mb", 24576)
     e_cur, e_pt, e_set = 0.0, 0.0, 0.0
+    per_mesh = []
     for i, (w, m, b) in enumerate(zip(work, ms, bufs)):
+        e_pt_i = 0.0
         J64 = b["J"].to_numpy((nd, m.n, 3), np.complex128)
         E64 = b["E"].to_numpy((nd, M, 3), np.complex128)
         H64 = b["H"].to_numpy((nd, M, 3), np.complex128)
@@ -430,8 +479,11 @@ def batched_leg(tune_cache=None):
             for X32, X64, sub in ((E32[i][r], E64[r], einc), (H32[i][r], H64[r], 0.0)):
                 sc = X64 - sub
                 nsc = np.linalg.norm(sc, axis=1)
-                e_pt = max(e_pt, float((np.linalg.norm(X32 - X64, axis=1) / np.maximum(nsc, 1e-3 * nsc.max())).max()))
+                e_pt_i = max(e_pt_i, float((np.linalg.norm(X32 - X64, axis=1) /
+                                            np.maximum(nsc, 1e-3 * nsc.max())).max()))
                 e_set = max(e_set, float(np.linalg.norm(X32 - X64) / np.linalg.norm(sc)))
+        e_pt = max(e_pt, e_pt_i)
+        per_mesh.append(float(f"{e_pt_i:.3e}"))
     for b in bufs:
         for d in b.values():
             if d.ptr:
@@ -459,10 +511,12 @@ def batched_leg(tune_cache=None):
            "field_plan_speedup_vs_driver": round(t_field_driver / t_field, 2),
            "assembly_pairs_per_s": pairs_asm / t_asm, "field_pair_sets_per_s": pairs_fld * nd / t_field,
            "total_s": round(total32, 3), "solves": solves, "solves_per_s": round(solves / total32, 2),
-           "fp64_form": {"lu64_s": round(t_lu64, 3), "solve64_s": round(t_solve64, 3), "field64_s": round(t_field64, 4),
+           "workers": workers,
+           "fp64_form": {"lu64_and_solve64_s": round(t_lu64, 3), "field64_s": round(t_field64, 4),
                          "total_s": round(total64, 3), "solves_per_s": round(solves / total64, 2)},
            "solves_per_s_ratio_vs_fp64": round(total64 / total32, 3),
-           "err_currents_rel_max": e_cur, "err_field_point_max_rel": e_pt, "err_field_set_max_rel": e_set}
+           "err_currents_rel_max": e_cur, "err_field_point_max_rel": e_pt, "err_field_set_max_rel": e_set,
+           "err_field_point_max_rel_per_mesh": per_mesh}
     log(f"[bench] batched: assembly {t_asm:.3f} s, LU32 {t_lu:.2f} s, refine+solve {t_solve:.2f} s, field "
         f"{t_field * 1e3:.1f} ms (graph; eager {t_field_eager * 1e3:.1f} ms, per-mesh driver "
         f"{t_field_driver * 1e3:.1f} ms); FP64 form {total64:.2f} s vs {total32:.2f} s; field err {e_pt:.2e}")

@@ -199,6 +199,9 @@ void lu_work_destroy(LuWork*);
 // few-RHS solves: sync-free triangular kernels (on) or blocked TRSM + GEMM (off)
 void lu_set_trsv(LuWork*, bool on);
 bool lu_get_trsv(const LuWork*);
+// many right-hand sides: rocBLAS trsm per triangle (on) or the blocked TRSM + GEMM loop (off)
+void lu_set_lib_trsm(LuWork*, bool on);
+bool lu_get_lib_trsm(const LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);

@@ -562,6 +562,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_trsv(ctx->lu, value == 1);
         return BIELCC_OK;
     }
+    if (k == "trsm_lib") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "trsm_lib must be 0 or 1");
+        lu_set_lib_trsm(ctx->lu, value == 1);
+        return BIELCC_OK;
+    }
     if (k == "plan_field_cells" || k == "plan_asm_cells") {
         if (value < 8 || value > 4096) return fail(ctx, BIELCC_ERR_ARG, "cells per tile must be in [8, 4096]");
         (k == "plan_field_cells" ? ctx->plan_field_cells : ctx->plan_asm_cells) = value;
@@ -592,6 +597,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "assemble_wg") *value = ctx->assemble_wg;
     else if (k == "field_rchunk") *value = ctx->field_rchunk;
     else if (k == "trsv") *value = lu_get_trsv(ctx->lu) ? 1 : 0;
+    else if (k == "trsm_lib") *value = lu_get_lib_trsm(ctx->lu) ? 1 : 0;
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;
     else if (k == "plan_asm_cells") *value = ctx->plan_asm_cells;
     else if (k == "plan_wg_per_cu") *value = ctx->plan_wg_per_cu;

@@ -68,12 +68,15 @@ struct LuWork {
     unsigned epoch = 0;
     int* err = nullptr;
     bool no_trsv = false;           // force the blocked TRSM + GEMM path (tests)
+    bool lib_trsm = true;           // many right-hand sides: rocBLAS trsm (else the blocked TRSM + GEMM loop)
 };
 
 LuWork* lu_work_create() { return new LuWork(); }
 
 void lu_set_trsv(LuWork* w, bool on) { w->no_trsv = !on; }
 bool lu_get_trsv(const LuWork* w) { return !w->no_trsv; }
+void lu_set_lib_trsm(LuWork* w, bool on) { w->lib_trsm = on; }
+bool lu_get_lib_trsm(const LuWork* w) { return w->lib_trsm; }
 
 // a sync-free triangular solve whose bounded wait expired (then cleared)
 static hipError_t trsv_check(hipStream_t s, LuWork* w, const char** msg) {
@@ -757,6 +760,32 @@ static hipError_t trsv_solve(hipStream_t s, LuWork* w, int n, int nrhs, const c2
     return hipSuccess;
 }
 
+// B = T^{-1} B for the whole triangle of the LU factors (unit lower L when
+// lower, else non-unit upper U), many right-hand sides, on rocBLAS trsm
+// (blocked inversion + GEMMs on the matrix cores).
+static hipError_t lib_trsm(LuWork* w, hipStream_t s, int lower, int n, int nrhs, const double2* A, long long lda,
+                           double2* B, long long ldb, const char** msg) {
+    rocblas_set_stream(w->blas, s);
+    const rocblas_double_complex one = {1., 0.};
+    rocblas_status st = rocblas_ztrsm(w->blas, rocblas_side_left, lower ? rocblas_fill_lower : rocblas_fill_upper,
+                                      rocblas_operation_none, lower ? rocblas_diagonal_unit : rocblas_diagonal_non_unit,
+                                      n, nrhs, &one, reinterpret_cast<const rocblas_double_complex*>(A), lda,
+                                      reinterpret_cast<rocblas_double_complex*>(B), ldb);
+    if (st != rocblas_status_success) { *msg = "rocblas_ztrsm failed"; return hipErrorUnknown; }
+    return hipSuccess;
+}
+static hipError_t lib_trsm(LuWork* w, hipStream_t s, int lower, int n, int nrhs, const float2* A, long long lda,
+                           float2* B, long long ldb, const char** msg) {
+    rocblas_set_stream(w->blas, s);
+    const rocblas_float_complex one = {1.f, 0.f};
+    rocblas_status st = rocblas_ctrsm(w->blas, rocblas_side_left, lower ? rocblas_fill_lower : rocblas_fill_upper,
+                                      rocblas_operation_none, lower ? rocblas_diagonal_unit : rocblas_diagonal_non_unit,
+                                      n, nrhs, &one, reinterpret_cast<const rocblas_float_complex*>(A), lda,
+                                      reinterpret_cast<rocblas_float_complex*>(B), ldb);
+    if (st != rocblas_status_success) { *msg = "rocblas_ctrsm failed"; return hipErrorUnknown; }
+    return hipSuccess;
+}
+
 // A X = B with the factors in A and the composed permutation in w->perm
 template <typename R>
 static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A, long long lda, int nrhs,
@@ -767,6 +796,11 @@ static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A,
     hipLaunchKernelGGL(k_gather_rows<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, w->perm,
                        tmp, B, ldb);
     if (nrhs <= kTrsvMaxR && !w->no_trsv) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
+    if (w->lib_trsm) {
+        if ((e = lib_trsm(w, s, 1, n, nrhs, A, lda, B, ldb, msg)) != hipSuccess) return e;
+        if ((e = lib_trsm(w, s, 0, n, nrhs, A, lda, B, ldb, msg)) != hipSuccess) return e;
+        return hipGetLastError();
+    }
     // forward: L y = P b
     for (int k0 = 0; k0 < n; k0 += kNB) {
         const int nb = min(kNB, n - k0);

@@ -313,8 +313,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * "field_wg" [2048] and "assemble_wg" [2048] = workgroups a tile launch aims
  * for (cells are split until the grid reaches it); "field_rchunk" [4] =
  * current sets per field pass (1, 2 or 4); "trsv" [1] = solves with at most
- * 8 right-hand sides use the one-kernel-per-triangle solve (0: blocked
- * TRSM + GEMM).  Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
+ * 8 right-hand sides use the one-kernel-per-triangle solve (0: as more);
+ * "trsm_lib" [1] = solves with more right-hand sides use rocBLAS trsm per
+ * triangle (0: the library's blocked TRSM + GEMM loop).  Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
  * bielcc_get_tuning also reads

@@ -110,23 +110,27 @@ def test_lu_random_multirhs(ctx):
         assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-13, n
 
 
-@pytest.mark.parametrize("nrhs", [1, 3, 8, 9])
+@pytest.mark.parametrize("nrhs", [1, 3, 8, 9, 64])
 def test_lu_solve_sync_free_and_blocked_agree(ctx, nrhs):
     """Up to 8 right-hand sides go through the one-kernel-per-triangle solve,
-    more through blocked TRSM + GEMM; both against numpy, ragged last block."""
+    more through rocBLAS trsm or the blocked TRSM + GEMM loop; every path
+    against numpy, ragged last block."""
     rng = np.random.default_rng(4)
     for n in (1, 64, 200, 1000):
         A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 4 * np.sqrt(n) * np.eye(n)
         Bm = rng.normal(size=(n, nrhs)) + 1j * rng.normal(size=(n, nrhs))
         Xr = np.linalg.solve(A, Bm)
         out = []
-        for on in (1, 0):
-            ctx.set_tuning("trsv", on)
+        for trsv, lib in ((1, 1), (0, 1), (0, 0)):
+            ctx.set_tuning("trsv", trsv)
+            ctx.set_tuning("trsm_lib", lib)
             X = ctx_solve(ctx, A, Bm)
-            assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-13, (n, on)
+            assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-13, (n, trsv, lib)
             out.append(X)
         ctx.set_tuning("trsv", 1)
-        assert np.linalg.norm(out[0] - out[1]) <= 1e-13 * np.linalg.norm(Xr)
+        ctx.set_tuning("trsm_lib", 1)
+        for X in out[1:]:
+            assert np.linalg.norm(out[0] - X) <= 1e-13 * np.linalg.norm(Xr)
 
 
 def ctx_solve(ctx, A, Bm):
