@@ -112,6 +112,118 @@ def cpu_baseline(verts, pts, k, budget_s=20.0):
             "field_pairs_per_s": m_f * n / tf, "assembly_pairs_per_s": m_a * n / ta}
 
 
+def far_directions(n_theta=181, n_phi=360):
+    """Unit directions on the 1-degree theta x phi grid (theta 0..180, phi 0..359)."""
+    th = np.deg2rad(np.arange(n_theta))[:, None]
+    ph = np.deg2rad(np.arange(n_phi))[None, :]
+    d = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th) * np.ones_like(ph)], axis=-1)
+    return np.ascontiguousarray(d.reshape(-1, 3))
+
+
+def _errs(a, b):
+    """(max over rows of |a-b| / max(|b|, 1e-3 max |b|), ||a-b|| / ||b||) for [n][3] complex rows."""
+    nb = np.linalg.norm(b, axis=1)
+    pt = float((np.linalg.norm(a - b, axis=1) / np.maximum(nb, 1e-3 * nb.max())).max())
+    return pt, float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def near_far_errors(ctx, m, npp, k, dJ64, dJ32, near_pts, dirs):
+    """Scattered E and H at near_pts and the far-field pattern at dirs, FP64
+    form (FP64 currents, FP64 sums) against the mixed form (refined currents,
+    FP32 far-pair node sums); errors and times of both."""
+    import bielcc_amd as B
+    out = {}
+    Mn, nd = near_pts.shape[0], dirs.shape[0]
+    dP, dD = ctx.to_device(near_pts), ctx.to_device(dirs)
+    dE, dH = ctx.empty(Mn * 48), ctx.empty(Mn * 48)
+    res = {}
+    for tag, dJ, prec in (("fp64", dJ64, B.PREC_FP64), ("mixed", dJ32, B.PREC_MIXED)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        m.field_batch_into(npp, k, 1, dJ, dP, Mn, dE, dH, precision=prec)
+        ctx.sync()
+        t1 = time.perf_counter()
+        res[tag] = (dE.to_numpy((Mn, 3), np.complex128), dH.to_numpy((Mn, 3), np.complex128))
+        out[f"near_field_{tag}_s"] = round(t1 - t0, 4)
+    eE, eH = _errs(res["mixed"][0], res["fp64"][0]), _errs(res["mixed"][1], res["fp64"][1])
+    out.update({"near_points": Mn, "err_near_E_point_max": eE[0], "err_near_E_set": eE[1],
+                "err_near_H_point_max": eH[0], "err_near_H_set": eH[1]})
+    dE.free(); dH.free(); dP.free()
+    dF, dG = ctx.empty(nd * 48), ctx.empty(nd * 48)
+    for tag, dJ, prec in (("fp64", dJ64, B.PREC_FP64), ("mixed", dJ32, B.PREC_MIXED)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        m.far_pattern_into(npp, k, 1, dJ, dD, nd, dF, dG, precision=prec)
+        ctx.sync()
+        t1 = time.perf_counter()
+        res[tag] = (dF.to_numpy((nd, 3), np.complex128), dG.to_numpy((nd, 3), np.complex128))
+        out[f"far_pattern_{tag}_s"] = round(t1 - t0, 4)
+    eE, eH = _errs(res["mixed"][0], res["fp64"][0]), _errs(res["mixed"][1], res["fp64"][1])
+    out.update({"far_directions": nd, "err_far_E_point_max": eE[0], "err_far_E_set": eE[1],
+                "err_far_H_point_max": eH[0], "err_far_H_set": eH[1]})
+    dF.free(); dG.free(); dD.free()
+    return out
+
+
+def cube_precision(k=10.0, n_face=60):
+    """Precision form on the cube of side 2 (6 x 60 x 60 quads): FP64 LU form
+    against FP32 LU + FP64 refinement to 1e-10, currents, near field at 10,000
+    points on r = 1.5 (partly inside the cube and close to its faces: exact near
+    pairs) and the far-field pattern on the 1-degree grid."""
+    import bielcc_amd as B
+    from bielcc_amd import mesh
+    verts = mesh.cube_quads(n_face, side=2.0)
+    ctx = B.Context(0)
+    m = ctx.mesh(verts)
+    npp = B.num_param()
+    n, n2 = m.n, 2 * m.n
+    kvec, e0 = (0.0, 0.0, k), (1.0, 0.0, 0.0)
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        return time.perf_counter() - t0
+    dZ, dF, dpiv = ctx.empty(n2 * n2 * 16), ctx.empty(n2 * n2 * 8), ctx.empty(n2 * 4)
+    dB, dJ32, dJ64 = ctx.empty(n2 * 16), ctx.empty(n * 48), ctx.empty(n * 48)
+    t_asm = timed(lambda: m.assemble_into(dZ, n2, npp, k))
+    t_lu32 = timed(lambda: ctx.lu32_factor(dZ, n2, n2, dF, n2, dpiv))
+    m.rhs_planewave_into(dB, n2, kvec, e0)
+    res, fallback = {}, False
+    try:
+        t_ref = timed(lambda: res.update(zip(("iters", "resid"), ctx.refine_solve(dZ, n2, n2, dF, n2, dpiv, dB, 1, n2,
+                                                                                   1e-10, 30))))
+    except B.BielccError as err:
+        if err.code != 4:
+            raise
+        fallback, t_ref = True, float("nan")
+    m.currents_into(dB, n2, 1, dJ32)
+    dF.free()
+    # FP64 form: LU in place of the matrix (assembled again after, if needed)
+    t_lu64 = timed(lambda: ctx.lu_factor(dZ, n2, n2, dpiv))
+    m.rhs_planewave_into(dB, n2, kvec, e0)
+    t_s64 = timed(lambda: ctx.lu_solve(dZ, n2, n2, dpiv, dB, 1, n2))
+    m.currents_into(dB, n2, 1, dJ64)
+    dZ.free()
+    j32, j64 = dJ32.to_numpy((n, 3), np.complex128), dJ64.to_numpy((n, 3), np.complex128)
+    out = {"workload": f"cube of side 2, 6 x {n_face} x {n_face} quads ({n} cells, {n2} unknowns), ka = {k}",
+           "assembly_s": round(t_asm, 4), "lu32_factor_s": round(t_lu32, 3), "refine_s": round(t_ref, 4),
+           "refine_corrections": res.get("iters"), "refine_residual": res.get("resid"),
+           "refine_stalled_fp64_needed": fallback,
+           "lu64_factor_s": round(t_lu64, 3), "lu64_solve_s": round(t_s64, 4),
+           "err_currents_rel": float(np.linalg.norm(j32 - j64) / np.linalg.norm(j64))}
+    out.update(near_far_errors(ctx, m, npp, k, dJ64, dJ32, mesh.sphere_points(10000, 1.5, seed=21), far_directions()))
+    log(f"[bench] cube precision: LU32 {t_lu32:.2f} s + refine {t_ref:.3f} s ({res.get('iters')} corrections) vs "
+        f"LU64 {t_lu64:.2f} s; currents {out['err_currents_rel']:.1e}, near E {out['err_near_E_set']:.1e}, "
+        f"far E {out['err_far_E_set']:.1e}")
+    for d in (dpiv, dB, dJ32, dJ64):
+        d.free()
+    m.close()
+    ctx.close()
+    return out
+
+
 def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve64_s, step64_s):
     """One cold solve in the mixed-precision form on the workload: FP64 matrix,
     FP32 LU on the matrix cores, FP64 refinement to a 1e-10 residual, currents,
@@ -154,6 +266,8 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
     m.field_batch_into(npp, k, 1, dJ64, dP, M, dE, dH, precision=B.PREC_MIXED)
     ctx.sync()
     Eq, Hq = dE.to_numpy((M, 3), np.complex128), dH.to_numpy((M, 3), np.complex128)
+    from bielcc_amd import mesh
+    nearfar = near_far_errors(ctx, m, npp, k, dJ64, dJ, mesh.sphere_points(10000, 1.5, seed=21), far_directions())
     rowrel = lambda a, b: float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
     glob = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     for d in (dZ, dF, dpiv, dB, dJ, dE, dH):
@@ -171,7 +285,8 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
             "err_currents_rel": glob(j32, j64),
             "err_field_E_max_rel": rowrel(E32, E64), "err_field_H_max_rel": rowrel(H32, H64),
             "err_field_E_rel": glob(E32, E64), "err_field_H_rel": glob(H32, H64),
-            "err_field_E_max_rel_fp64_currents": rowrel(Eq, E64), "err_field_H_max_rel_fp64_currents": rowrel(Hq, H64)}
+            "err_field_E_max_rel_fp64_currents": rowrel(Eq, E64), "err_field_H_max_rel_fp64_currents": rowrel(Hq, H64),
+            "near_r1.5_and_far_pattern": nearfar}
 
 
 def serving_pool(seed=5, n=16, lo=1000, hi=6000):
@@ -220,7 +335,7 @@ def serving_queries(pool, nq, seed=6, m_lo=1000, m_hi=20000):
     return qs
 
 
-def serving_leg(nq=96, workers=4):
+def serving_leg(nq=96, workers=4, refine_tol=1e-13):
     """The same query stream served in the FP64 form and in the mixed form
     (FP32 LU + FP64 refinement to 1e-13, FP32 far sums), HBM budget 200 GB:
     once as a burst (every query queued at once: capacity in solves/s) and once
@@ -230,7 +345,7 @@ def serving_leg(nq=96, workers=4):
     cells = [v.shape[0] for v in pool.values()]
     out = {"workload": f"{len(pool)} meshes ({min(cells)}-{max(cells)} cells: spheres, ellipsoids, tori, cubes), "
                        f"{nq} queries, ka in [1,15] on 8 levels, 1-16 incidences, 1,000-20,000 points at "
-                       f"r in [3,5]; {workers} workers; HBM budget 200 GB; mixed form refines to 1e-13"}
+                       f"r in [3,5]; {workers} workers; HBM budget 200 GB; mixed form refines to {refine_tol:g}"}
     res = {}
     rate = None
     for mode in ("burst", "stream"):
@@ -238,7 +353,7 @@ def serving_leg(nq=96, workers=4):
         for form in ("fp64", "mixed"):
             qs = serving_queries(pool, nq)
             qps = 1e6 if mode == "burst" else rate
-            with serve.Server(pool, form=form, workers=workers, hbm_budget=200e9, refine_tol=1e-13) as s:
+            with serve.Server(pool, form=form, workers=workers, hbm_budget=200e9, refine_tol=refine_tol) as s:
                 run = serve.poisson_run(s, qs, qps=qps, seed=7)
             st = run["stats"]
             log(f"[bench] serving {mode} {form}: {st['wall_s']:.1f} s, {st['solves_per_s']:.1f} solves/s, "
@@ -249,26 +364,34 @@ def serving_leg(nq=96, workers=4):
         if mode == "burst":
             rate = 0.6 * nq / out["burst"]["fp64"]["wall_s"]
             out["stream_qps"] = round(rate, 3)
-    # error of the scattered field (the total field nearly vanishes at shadow points)
+    # error of the scattered field (the total field nearly vanishes at shadow points), split
+    # by how the mixed server solved the query: FP32 factors + refinement, or FP64 factors
+    # for systems FP32 factors cannot refine (there both servers run FP64 arithmetic and
+    # differ by the system's own conditioning times rounding)
     qs = {q.qid: q for q in serving_queries(pool, nq)}
-    err, glob, worst = 0.0, 0.0, None
+    errs = {"refined": [0.0, 0.0, 0], "fp64_factors": [0.0, 0.0, 0]}
+    worst = None
     for i, r64 in res["fp64"].items():
         r32, q = res["mixed"][i], qs[i]
+        cat = errs["fp64_factors" if r32.fp64_fallback else "refined"]
+        cat[2] += 1
         for r in range(r64.E.shape[0]):
             einc = q.pols[r][None, :] * np.exp(1j * q.k * (q.points @ q.dirs[r]))[:, None]
             es = r64.E[r] - einc
             nes = np.linalg.norm(es, axis=1)
             # per point, with a floor of 1e-3 of the set's largest scattered field (nulls)
             e = np.linalg.norm(r32.E[r] - r64.E[r], axis=1) / np.maximum(nes, 1e-3 * nes.max())
-            glob = max(glob, float(np.linalg.norm(r32.E[r] - r64.E[r]) / np.linalg.norm(es)))
-            if e.max() > err:
-                err = float(e.max())
-                worst = {"query": i, "mesh": q.mesh, "k": q.k, "fallback": r32.fp64_fallback,
-                         "refine_resid": r32.refine_resid, "refine_iters": r32.refine_iters,
-                         "set_rel_err": float(np.linalg.norm(r32.E[r] - r64.E[r]) / np.linalg.norm(es))}
-    out["err_scattered_E_max_rel_mixed_vs_fp64"] = err
-    out["err_scattered_E_set_max_rel_mixed_vs_fp64"] = glob
-    out["worst"] = worst
+            g = float(np.linalg.norm(r32.E[r] - r64.E[r]) / np.linalg.norm(es))
+            cat[0], cat[1] = max(cat[0], float(e.max())), max(cat[1], g)
+            if not r32.fp64_fallback and (worst is None or g > worst["set_rel_err"]):
+                worst = {"query": i, "mesh": q.mesh, "k": q.k, "refine_resid": r32.refine_resid,
+                         "refine_iters": r32.refine_iters, "set_rel_err": g}
+    out["err_scattered_E_refined"] = {"point_max_rel": errs["refined"][0], "set_max_rel": errs["refined"][1],
+                                      "queries": errs["refined"][2]}
+    out["err_scattered_E_fp64_factors"] = {"point_max_rel": errs["fp64_factors"][0],
+                                           "set_max_rel": errs["fp64_factors"][1],
+                                           "queries": errs["fp64_factors"][2]}
+    out["worst_refined"] = worst
     out["capacity_solves_per_s_ratio"] = round(out["burst"]["mixed"]["solves_per_s"] /
                                                out["burst"]["fp64"]["solves_per_s"], 3)
     out["stream_p99_ratio"] = round(out["stream"]["fp64"]["p99_s"] / out["stream"]["mixed"]["p99_s"], 3)
@@ -550,6 +673,7 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=96)
     ap.add_argument("--no-batched", action="store_true", help="skip the 12-mesh batched form")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
+    ap.add_argument("--cube-only", action="store_true", help="run only the precision form on the cube")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -560,6 +684,9 @@ def main():
     import bielcc_amd as B
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args.tune_cache)}), flush=True)
+        return
+    if args.cube_only:
+        print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
         return
     if args.copy_calib:
         copy_calibration()
@@ -651,9 +778,11 @@ def main():
     if lu_s is not None and not args.no_mixed:
         out["precision_form"] = mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ, dE, dH, lu_s, solve_s,
                                            elapsed / args.steps)
-    if not (args.no_serve and args.no_batched):
+    if not (args.no_serve and args.no_batched and args.no_mixed):
         for d in (dZ, dB, dpiv):
             d.free()
+    if lu_s is not None and not args.no_mixed:
+        out["precision_form_cube"] = cube_precision()
     if not args.no_batched:
         out["batched"] = batched_leg(args.tune_cache)
     if not args.no_serve:

@@ -202,6 +202,10 @@ bool lu_get_trsv(const LuWork*);
 // many right-hand sides: rocBLAS trsm per triangle (on) or the blocked TRSM + GEMM loop (off)
 void lu_set_lib_trsm(LuWork*, bool on);
 bool lu_get_lib_trsm(const LuWork*);
+// polls per wait of the sync-free solve before it gives up and the solve is redone blocked
+// (default 2^20, about a second; a real wait lasts microseconds)
+void lu_set_spin_limit(LuWork*, int polls);
+int lu_get_spin_limit(const LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);

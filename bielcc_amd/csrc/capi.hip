@@ -41,6 +41,8 @@ int bielcc_ctx_destroy(bielcc_ctx ctx) {
         if (ctx->scratch) (void)hipFree(ctx->scratch);
         if (ctx->lists) (void)hipFree(ctx->lists);
         if (ctx->near_buf) (void)hipFree(ctx->near_buf);
+        if (ctx->opmat_ws) (void)hipFree(ctx->opmat_ws);
+        if (ctx->opmat_meta) (void)hipFree(ctx->opmat_meta);
         lu_work_destroy(ctx->lu);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
@@ -405,6 +407,9 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     double* inc_d = reinterpret_cast<double*>(base + part_b + cnt_b + off_b);
     long long* tot_d = reinterpret_cast<long long*>(base + part_b + cnt_b + off_b + inc_b);
     if (inc) HIPCHK(hipMemcpyAsync(inc_d, inc, (size_t)nrhs * 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (nrhs >= ctx->opmat_min_rhs && !jM_d)
+        return field_opmat(ctx, *m, np, k_re, k_im, nrhs, jE_d, mEr, mEi, inc ? inc_d : nullptr, near_mode, precision,
+                           M, pts_d, E_d, H_d, counts, offs, tot_d);
     FieldArgs a;
     a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
     a.nodes32 = m->nodes32; a.nodes32_yy = m->nodes32_yy;
@@ -562,6 +567,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_trsv(ctx->lu, value == 1);
         return BIELCC_OK;
     }
+    if (k == "trsv_spin") {
+        if (value < 1) return fail(ctx, BIELCC_ERR_ARG, "trsv_spin must be >= 1");
+        lu_set_spin_limit(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "trsm_lib") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "trsm_lib must be 0 or 1");
         lu_set_lib_trsm(ctx->lu, value == 1);
@@ -598,6 +608,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "field_rchunk") *value = ctx->field_rchunk;
     else if (k == "trsv") *value = lu_get_trsv(ctx->lu) ? 1 : 0;
     else if (k == "trsm_lib") *value = lu_get_lib_trsm(ctx->lu) ? 1 : 0;
+    else if (k == "trsv_spin") *value = lu_get_spin_limit(ctx->lu);
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;
     else if (k == "plan_asm_cells") *value = ctx->plan_asm_cells;
     else if (k == "plan_wg_per_cu") *value = ctx->plan_wg_per_cu;

@@ -33,6 +33,10 @@ struct bielcc_ctx_s {
     bool no_fast_phase = false;   // BIELCC_GENERAL_SINCOS=1: tile kernels use the general sincos
     void* near_buf = nullptr;     // per-pair near results of the field driver
     size_t near_cap = 0;
+    void* opmat_ws = nullptr;     // operator matrices of bielcc_field_batch's many-set path (grow-only)
+    size_t opmat_ws_cap = 0;
+    void* opmat_meta = nullptr;   // its segment descriptors and tiles (grow-only)
+    size_t opmat_meta_cap = 0;
     int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     // plan (multi-mesh) tunables: cells per field tile and per assembly tile,
     // persistent workgroups per CU, operator-matrix workspace cap (MiB), and the
@@ -203,5 +207,12 @@ inline int build_near_list(bielcc_ctx ctx, int nq, const double* q, int q_stride
     }
     return BIELCC_OK;
 }
+
+// bielcc_field_batch for many current sets (electric only) through the
+// operator matrices in context-owned grow-only buffers (plan.hip).  inc_d:
+// device [nrhs][6] or null.  counts/offs/tot_d: near-list scratch for M points.
+int field_opmat(bielcc_ctx ctx, DevMesh& m, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
+                const double* jE_d, double mEr, double mEi, const double* inc_d, int near_mode, int precision, int M,
+                const double* pts_d, double* E_d, double* H_d, int* counts, long long* offs, long long* tot_d);
 
 }  // namespace bielcc_capi

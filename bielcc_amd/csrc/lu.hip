@@ -69,6 +69,7 @@ struct LuWork {
     int* err = nullptr;
     bool no_trsv = false;           // force the blocked TRSM + GEMM path (tests)
     bool lib_trsm = true;           // many right-hand sides: rocBLAS trsm (else the blocked TRSM + GEMM loop)
+    int spin_limit = 1 << 20;       // polls per sync-free wait before it gives up (tests lower it)
 };
 
 LuWork* lu_work_create() { return new LuWork(); }
@@ -76,6 +77,8 @@ LuWork* lu_work_create() { return new LuWork(); }
 void lu_set_trsv(LuWork* w, bool on) { w->no_trsv = !on; }
 bool lu_get_trsv(const LuWork* w) { return !w->no_trsv; }
 void lu_set_lib_trsm(LuWork* w, bool on) { w->lib_trsm = on; }
+void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
+int lu_get_spin_limit(const LuWork* w) { return w->spin_limit; }
 bool lu_get_lib_trsm(const LuWork* w) { return w->lib_trsm; }
 
 // a sync-free triangular solve whose bounded wait expired (then cleared)
@@ -497,6 +500,15 @@ __global__ void k_copy_cols(int n, int nrhs, const c2<R>* __restrict__ B, long l
     dst[(size_t)r * n + i] = B[(size_t)r * ldb + i];
 }
 
+// B (ldb) = src (dense n x nrhs): puts the staged right-hand sides back
+template <typename R>
+__global__ void k_uncopy_cols(int n, int nrhs, const c2<R>* __restrict__ src, c2<R>* __restrict__ B, long long ldb) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int i = (int)(idx % n), q = (int)(idx / n);
+    B[(size_t)q * ldb + i] = src[(size_t)q * n + i];
+}
+
 // composed gather permutation of the interchanges piv (host, O(n)); fills
 // w->perm (device) and returns the staging area behind it (n * nrhs * elem bytes)
 static hipError_t lu_perm(hipStream_t s, LuWork* w, int n, const int* piv, size_t stage_bytes, void** stage,
@@ -533,12 +545,11 @@ static hipError_t lu_perm(hipStream_t s, LuWork* w, int n, const int* piv, size_
 // flag = epoch).  lower = 1: unit lower L, rows 0, 1, ...; lower = 0: upper U,
 // rows nblk-1, nblk-2, ....  Workgroups take block rows in the order of a
 // ticket counter, so a row only ever waits for rows whose workgroups have
-// already started: every wait ends.  Each wait is also bounded (kSpinLimit
+// already started: every wait ends.  Each wait is also bounded (spin_limit
 // polls); on expiry the workgroup raises *err and carries on, every other
-// wait then gives up too, so the kernel always drains and the host reports
-// the failure.
+// wait then gives up too, so the kernel always drains, and the host redoes
+// the solve on the blocked path.
 constexpr int kTrsvMaxR = 8;
-constexpr int kSpinLimit = 1 << 20;   // polls per wait (about a second); a real wait lasts microseconds
 
 template <typename R>
 __device__ inline c2<R> load_relaxed(const c2<R>* p) {
@@ -562,11 +573,15 @@ template <typename R, int NR>
 __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const c2<R>* __restrict__ A,
                                               long long lda, c2<R>* B, long long ldb, unsigned* flags,
                                               unsigned long long* ticket, unsigned long long tbase,
-                                              unsigned epoch, int* err) {
+                                              unsigned epoch, int* err, int spin_limit) {
     __shared__ c2<R> T[kNB][kNB + 1];
     __shared__ c2<R> X[kNB][NR];
     __shared__ c2<R> red[4][kNB][NR];
     __shared__ int s_blk, s_ready;
+    // The block chain is latency-critical and short: its waves take the highest issue priority,
+    // so that a long-running VALU-bound kernel on another stream (whose older waves would win the
+    // age-ordered arbitration) cannot starve a block a later one waits for.
+    __builtin_amdgcn_s_setprio(3);
     const int tid = threadIdx.x;
     const int r = tid & 63, cg = tid >> 6;
     const int nblk = (n + kNB - 1) / kNB;
@@ -609,7 +624,7 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
                 // issued only after the flag value has been seen
                 int spins = 0;
                 while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                    if (++spins > kSpinLimit) { atomicExch(err, 1); break; }
+                    if (++spins > spin_limit) { atomicExch(err, 1); break; }
                     // once any wait has expired, nobody waits any more: the kernel drains
                     if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                     __builtin_amdgcn_s_sleep(1);
@@ -736,7 +751,7 @@ static void trsv_launch(hipStream_t s, LuWork* w, int n, int nrhs, int lower, co
     const int nblk = (n + kNB - 1) / kNB;
     if (++w->epoch == 0) w->epoch = 1;   // flags start at 0: epoch 0 is never "ready"
     hipLaunchKernelGGL((k_trsv<R, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, lower, A, lda, B, ldb, w->flags,
-                       w->ticket, w->tbase, w->epoch, w->err);
+                       w->ticket, w->tbase, w->epoch, w->err, w->spin_limit);
     w->tbase += (unsigned long long)nblk;
 }
 
@@ -835,7 +850,21 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long
     e = lu_solve_perm<double>(s, w, n, reinterpret_cast<const double2*>(Ad), lda, nrhs,
                               reinterpret_cast<double2*>(Bd), ldb, reinterpret_cast<double2*>(stage), msg);
     if (e != hipSuccess) return e;
-    return trsv_check(s, w, msg);
+    e = trsv_check(s, w, msg);
+    if (e == hipErrorLaunchTimeOut) {
+        // a sync-free wait expired (the kernel drained, X is invalid): the right-hand sides are
+        // still in the staging copy; solve again on the blocked path
+        const long long tot = (long long)n * nrhs;
+        hipLaunchKernelGGL(k_uncopy_cols<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs,
+                           reinterpret_cast<const double2*>(stage), reinterpret_cast<double2*>(Bd), ldb);
+        const bool keep = w->no_trsv;
+        w->no_trsv = true;
+        *msg = nullptr;
+        e = lu_solve_perm<double>(s, w, n, reinterpret_cast<const double2*>(Ad), lda, nrhs,
+                                  reinterpret_cast<double2*>(Bd), ldb, reinterpret_cast<double2*>(stage), msg);
+        w->no_trsv = keep;
+    }
+    return e;
 }
 
 // ---------------------------------------------------------------- refinement
@@ -921,34 +950,53 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long 
     hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, X, ldb, B0, (long long)n);
     hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, B0, (long long)n, nrm);
     if ((e = hipMemcpyAsync(bn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-    // X0 = F^{-1} B
-    hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rf, (long long)n);
-    if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
-    double prev = HUGE_VAL;
-    for (int it = 0;; it++) {
-        // Rd = B0 - A X (FP64)
-        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
-        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, (long long)n, msg)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, (long long)n, nrm);
-        if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
-            return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-        if ((e = trsv_check(s, w, msg)) != hipSuccess) return e;
-        double rel = 0.;
-        for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
-        *resid = rel;
-        if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }   // NaN / Inf: FP32 factors broke down
-        if (rel <= tol) return hipSuccess;
-        if (it >= max_iter) { *status = 1; return hipSuccess; }
-        if (!(rel < 0.5 * prev)) { *status = 1; return hipSuccess; }   // stagnation or divergence
-        prev = rel;
-        // X += F^{-1} Rd
-        hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, (long long)n, Rf, (long long)n);
+    const bool keep_trsv = w->no_trsv;
+    auto attempt = [&]() -> hipError_t {
+        // X0 = F^{-1} B
+        hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rf, (long long)n);
         if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
-        *iters = it + 1;
+        hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
+        double prev = HUGE_VAL;
+        for (int it = 0;; it++) {
+            // Rd = B0 - A X (FP64)
+            hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
+            if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, (long long)n, msg)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, (long long)n, nrm);
+            if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+                return e;
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            if ((e = trsv_check(s, w, msg)) != hipSuccess) return e;
+            double rel = 0.;
+            for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
+            *resid = rel;
+            if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }   // NaN / Inf: FP32 factors broke down
+            if (rel <= tol) return hipSuccess;
+            if (it >= max_iter) { *status = 1; return hipSuccess; }
+            // stagnation or divergence: a correction must shrink the residual by 10%.  Near an
+            // interior resonance the contraction factor cond(A) u_fp32 approaches 1 and each
+            // correction gains little, but a correction (two FP32 triangular solves and one FP64
+            // GEMV-sized product) still costs far less than FP64 factors
+            if (!(rel < 0.9 * prev)) { *status = 1; return hipSuccess; }
+            prev = rel;
+            // X += F^{-1} Rd
+            hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, (long long)n, Rf, (long long)n);
+            if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
+            *iters = it + 1;
+        }
+    };
+    e = attempt();
+    if (e == hipErrorLaunchTimeOut) {
+        // a sync-free wait expired (see lu_solve): start again from B on the blocked path
+        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, X, ldb);
+        w->no_trsv = true;
+        *msg = nullptr;
+        *iters = 0;
+        *status = 0;
+        e = attempt();
+        w->no_trsv = keep_trsv;
     }
+    return e;
 }
 
 }  // namespace bielcc_impl

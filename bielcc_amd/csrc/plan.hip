@@ -179,6 +179,87 @@ int run_impl(bielcc_ctx ctx, bielcc_plan p) {
 
 }  // namespace
 
+namespace bielcc_capi {
+
+int field_opmat(bielcc_ctx ctx, DevMesh& m, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
+                const double* jE_d, double mEr, double mEi, const double* inc_d, int near_mode, int precision, int M,
+                const double* pts_d, double* E_d, double* H_d, int* counts, long long* offs, long long* tot_d) {
+    const int nn = np->n_start * np->n_start;
+    const int N = m.n;
+    // point chunks whose matrices fit a 4 GiB (at most plan_ws_mb) workspace
+    const size_t per_pt = (H_d ? 2 : 1) * (size_t)9 * 16 * N;
+    const size_t cap = std::min((size_t)ctx->plan_ws_mb << 20, (size_t)4 << 30);
+    const int chunk = (int)std::max<size_t>(256, cap / per_pt / 256 * 256);
+    const int Mc = std::min(M, chunk);
+    const size_t wk = align256((size_t)9 * 16 * Mc * N);
+    HIPCHK(ensure(ctx->opmat_ws, ctx->opmat_ws_cap, wk * (H_d ? 2 : 1)));
+    char* ws = reinterpret_cast<char*>(ctx->opmat_ws);
+    std::vector<OpSeg> segs;
+    std::vector<int4> tiles;
+    std::vector<int> t0;
+    const int cpt = ctx->plan_field_cells;
+    for (int p0 = 0; p0 < M; p0 += chunk) {
+        OpSeg sg;
+        sg.M = std::min(chunk, M - p0); sg.N = N; sg.nn = nn; sg.nv = m.nv;
+        sg.pts = pts_d + 3 * (size_t)p0; sg.ctab = m.ctab; sg.nodes = m.nodes;
+        sg.nodes32 = m.nodes32; sg.nodes32_yy = m.nodes32_yy; sg.verts = m.verts;
+        sg.k_re = k_re; sg.k_im = k_im;
+        sg.fast_phase = fast_phase_ok(k_re, m.grid_step) && !ctx->no_fast_phase;
+        sg.mode = near_mode; sg.ac = np->analytic_calc; sg.thresh = np->analytic_calc * m.grid_step;
+        // every chunk reuses the workspace: the stream orders chunk i's GEMMs before chunk i+1's tiles
+        sg.WK = reinterpret_cast<double*>(ws);
+        sg.WR = H_d ? reinterpret_cast<double*>(ws + align256((size_t)9 * 16 * sg.M * N)) : nullptr;
+        t0.push_back((int)tiles.size());
+        const int s = (int)segs.size();
+        for (int pb = 0; pb < sg.M; pb += 256)
+            for (int c0 = 0; c0 < N; c0 += cpt) tiles.push_back(make_int4(s, pb, c0, std::min(N, c0 + cpt)));
+        segs.push_back(sg);
+    }
+    t0.push_back((int)tiles.size());
+    const size_t seg_b = align256(segs.size() * sizeof(OpSeg));
+    HIPCHK(ensure(ctx->opmat_meta, ctx->opmat_meta_cap, seg_b + tiles.size() * sizeof(int4)));
+    OpSeg* segs_d = reinterpret_cast<OpSeg*>(ctx->opmat_meta);
+    int4* tiles_d = reinterpret_cast<int4*>(reinterpret_cast<char*>(ctx->opmat_meta) + seg_b);
+    HIPCHK(hipMemcpyAsync(segs_d, segs.data(), segs.size() * sizeof(OpSeg), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));   // the host vectors go out of scope
+    HIPCHK(launch_init_incident(ctx->stream, M, nrhs, pts_d, inc_d, E_d));
+    const NumP npp = to_np(np);
+    const int grid = grid_for(ctx, t0[1] - t0[0]);
+    for (size_t s = 0; s < segs.size(); s++) {
+        const OpSeg& sg = segs[s];
+        const int p0 = (int)(s * (size_t)chunk);
+        long long tot_h = 0;
+        int *lq = nullptr, *lc = nullptr;
+        // (synchronises: the previous chunk's GEMMs are done before its lists are overwritten)
+        if (int rc = build_near_list(ctx, sg.M, sg.pts, 3, m, near_mode, np->analytic_calc, sg.thresh, counts, offs,
+                                     tot_d, &tot_h, &lq, &lc))
+            return rc;
+        {
+            Timer t(ctx, "field_opmat");
+            HIPCHK(launch_field_opmat(ctx->stream, segs_d, tiles_d + t0[s], t0[s + 1] - t0[s], grid, k_im != 0.,
+                                      precision == BIELCC_PREC_MIXED));
+            t.stop();
+        }
+        if (tot_h > 0) {
+            Timer t(ctx, "near");
+            HIPCHK(launch_opmat_near(ctx->stream, tot_h, lq, lc, sg, npp, near_mode));
+            t.stop();
+        }
+        const long long m3 = 3LL * sg.M, n3 = 3LL * N, ldo = 3LL * M;
+        const char* msg = nullptr;
+        hipError_t e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, n3, make_double2(mEr, mEi), sg.WK, m3, jE_d, n3,
+                                  make_double2(1., 0.), E_d + 6 * (size_t)p0, ldo, &msg);
+        if (e == hipSuccess && H_d)
+            e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, n3, make_double2(1., 0.), sg.WR, m3, jE_d, n3,
+                           make_double2(0., 0.), H_d + 6 * (size_t)p0, ldo, &msg);
+        if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    }
+    return BIELCC_OK;
+}
+
+}  // namespace bielcc_capi
+
 extern "C" {
 
 int bielcc_plan_assemble(bielcc_ctx ctx, int nitems, const bielcc_assemble_item* items, int which,

@@ -228,6 +228,7 @@ class Server:
         self.q: "queue.Queue[Optional[Query]]" = queue.Queue()
         self.results: dict = {}
         self.errors: dict = {}
+        self.upgrades: list = []          # (mesh, k, reason) of entries given FP64 factors
         self.lock = threading.Lock()
         self.done_cv = threading.Condition(self.lock)
         self.ctxs = [api.Context(device) for _ in range(workers)]
@@ -326,17 +327,19 @@ class Server:
                 if err.code != 5:          # only a zero FP32 pivot is handled here
                     raise
                 ctx.sync()
-                self._upgrade(w, e)
+                self._upgrade(w, e, str(err))
         else:
             ctx.lu_factor(e.bufs["Z"], n2, n2, e.bufs["piv"])
         ctx.sync()
 
-    def _upgrade(self, w: int, e: _Entry) -> None:
+    def _upgrade(self, w: int, e: _Entry, reason: str = "") -> None:
         """FP64 factors beside the FP32 ones (mixed form), for a system FP32
         factors cannot refine.  Done once per entry."""
         with e.upgrade:
             if "LU64" in e.bufs:
                 return
+            with self.lock:
+                self.upgrades.append({"mesh": e.key[0], "k": e.key[1], "reason": reason})
             ctx = self.ctxs[w]
             n2 = int(round(np.sqrt(e.bufs["Z"].nbytes / 16)))
             extra = 16 * n2 * n2 + 4 * n2
@@ -413,7 +416,7 @@ class Server:
                 except api.BielccError as err:
                     if err.code != 4:      # refinement stalled: FP64 factors for this entry
                         raise
-                    self._upgrade(w, e)
+                    self._upgrade(w, e, str(err))
                     m.rhs_planewave_into(dB, n2, kvec, pols)
             if self.form == "mixed" and "LU64" in e.bufs:
                 ctx.lu_solve(e.bufs["LU64"], n2, n2, e.bufs["piv64"], dB, nrhs, n2)
@@ -459,6 +462,8 @@ def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
     solves = sum(q.nrhs for q in queries)
     out = {"queries": len(queries), "solves": solves, "wall_s": t1 - t0, "solves_per_s": solves / (t1 - t0),
            "factorisations": sum(r.factored for r in res.values()), "evictions": server.cache.evictions,
-           "fp64_fallbacks": sum(r.fp64_fallback for r in res.values())}
+           "fp64_fallbacks": sum(r.fp64_fallback for r in res.values()),
+           "fp64_upgrades": list(server.upgrades),
+           "refine_corrections_max": max((r.refine_iters for r in res.values()), default=0)}
     out.update(percentiles([r.latency for r in res.values()]))
     return {"results": res, "stats": out}

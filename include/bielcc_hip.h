@@ -234,9 +234,9 @@ int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, fl
  * X0 = F^{-1} B, then X += F^{-1} (B - A X) with the residual in FP64, until
  * max_r ||B_r - A X_r||_2 / ||B_r||_2 <= tol or max_iter corrections.  B_d is
  * overwritten by X; *iters = corrections applied, *resid = final relative
- * residual.  Returns BIELCC_ERR_UNSUPPORTED if the residual stops shrinking
- * above tol (matrix too ill-conditioned for FP32 factors); X is then the best
- * iterate. */
+ * residual.  Returns BIELCC_ERR_UNSUPPORTED if a correction shrinks the
+ * residual by less than 10% while it is above tol (matrix too ill-conditioned
+ * for FP32 factors); X is then the last iterate. */
 int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
                         const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
                         int* iters, double* resid);
@@ -315,7 +315,10 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * current sets per field pass (1, 2 or 4); "trsv" [1] = solves with at most
  * 8 right-hand sides use the one-kernel-per-triangle solve (0: as more);
  * "trsm_lib" [1] = solves with more right-hand sides use rocBLAS trsm per
- * triangle (0: the library's blocked TRSM + GEMM loop).  Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
+ * triangle (0: the library's blocked TRSM + GEMM loop); "trsv_spin"
+ * [1048576] = polls per wait of the one-kernel solve before it gives up (a
+ * solve whose wait gave up is redone on the blocked path; tests lower it).
+ * Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
  * bielcc_get_tuning also reads

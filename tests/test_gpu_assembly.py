@@ -164,3 +164,28 @@ def test_bad_arguments(ctx):
     with pytest.raises(ValueError):
         ctx.mesh(np.zeros((3, 5, 3)))
     m.close()
+
+
+def test_sync_free_solve_recovers_from_an_expired_wait(ctx):
+    """A wait of the one-kernel triangular solve that gives up (here forced from
+    the host with a one-poll limit, so every kernel still drains normally) is
+    redone on the blocked path: lu_solve and the FP32 refinement still return
+    the solution."""
+    rng = np.random.default_rng(6)
+    n = 1000
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    Bm = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    Xr = np.linalg.solve(A, Bm)
+    ctx.set_tuning("trsv_spin", 1)
+    try:
+        X = ctx_solve(ctx, A, Bm)
+        assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-13
+        dA = ctx.to_device(np.ascontiguousarray(A.T))
+        dB = ctx.to_device(np.ascontiguousarray(Bm.T))
+        dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+        ctx.lu32_factor(dA, n, n, dF, n, piv)
+        it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 3, n, 1e-13, 30)
+        X = dB.to_numpy((3, n), np.complex128).T
+        assert res <= 1e-13 and np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-12
+    finally:
+        ctx.set_tuning("trsv_spin", 1 << 20)

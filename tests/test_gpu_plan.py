@@ -220,3 +220,31 @@ def test_plan_argument_errors(ctx):
     with pytest.raises(B.BielccError):
         ctx.set_tuning("plan_wg_per_cu", 0)
     m.close()
+
+
+@pytest.mark.parametrize("precision,near_mode,with_h", [(B.PREC_FP64, 0, True), (B.PREC_FP64, 1, False),
+                                                        (B.PREC_MIXED, 0, True)])
+def test_field_batch_many_sets_through_operator_matrices(ctx, precision, near_mode, with_h):
+    """bielcc_field_batch with at least opmat_min_rhs current sets takes the
+    operator-matrix path; it equals the pass-per-4-sets path (FP64: 1e-12 per
+    point; mixed: 1e-6 against FP64), chunked or not."""
+    rng = np.random.default_rng(15)
+    npp = B.num_param()
+    m = ctx.mesh(B.mesh.triangulate(B.mesh.sphere_quads(7, 14, seed=6)))
+    nr, M = 11, 700
+    jE = rng.normal(size=(nr, m.n, 3)) + 1j * rng.normal(size=(nr, m.n, 3))
+    pts = np.concatenate([_points(rng, 200, 1.0, 1.2), _points(rng, M - 200, 2.0, 4.0)])
+    inc = (rng.normal(size=(nr, 3)), rng.normal(size=(nr, 3)))
+    ctx.set_tuning("opmat_min_rhs", 1000)
+    Er, Hr = m.field_batch(npp, 2.5, jE, pts, multE=0.4 + 0.9j, incident=inc, near_mode=near_mode)
+    tol = 1e-12 if precision == B.PREC_FP64 else 1e-6
+    for ws_mb in (24576, 16):          # 16 MiB: several point chunks through one workspace
+        ctx.set_tuning("opmat_min_rhs", 9)
+        ctx.set_tuning("plan_ws_mb", ws_mb)
+        E, H = m.field_batch(npp, 2.5, jE, pts, multE=0.4 + 0.9j, incident=inc, near_mode=near_mode,
+                             precision=precision)
+        ctx.set_tuning("plan_ws_mb", 24576)
+        assert rowwise_rel(E.reshape(-1, 3), Er.reshape(-1, 3)).max() <= tol, ws_mb
+        if with_h:
+            assert rowwise_rel(H.reshape(-1, 3), Hr.reshape(-1, 3)).max() <= tol, ws_mb
+    m.close()
