@@ -504,6 +504,12 @@ def batched_leg(tune_cache=None, workers=4):
                     sig + "|asm", repeats=1, cache_path=tune_cache, log=log)
     aplan = ctx.plan_assemble(aitems, npp)
     t_asm = timed(aplan.run)
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    aplan.run()
+    ctx.sync()
+    kt_asm = ctx.timing("plan_assemble_tiles")[0]
+    ctx.timing_enable(False)
     a_info = {k: aplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs")}
     aplan.close()
     # FP32 LU per mesh and refinement with 64 right-hand sides: the independent
@@ -546,6 +552,12 @@ def batched_leg(tune_cache=None, workers=4):
                     sig + "|field32", repeats=2, cache_path=tune_cache, log=log)
     fplan = ctx.plan_field(fitems, npp, precision=B.PREC_MIXED)
     t_field_eager = timed(fplan.run)
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    fplan.run()              # eager, with HIP events around the tile launch and the GEMMs
+    ctx.sync()
+    kt = {key: ctx.timing(key)[0] for key in ("plan_field_tiles", "plan_field_gemm")}
+    ctx.timing_enable(False)
     fplan.capture()
     reps = 3
     t_field = timed(lambda: [fplan.run() for _ in range(reps)]) / reps
@@ -554,11 +566,17 @@ def batched_leg(tune_cache=None, workers=4):
     E32 = [b["E"].to_numpy((nd, M, 3), np.complex128) for b in bufs]
     H32 = [b["H"].to_numpy((nd, M, 3), np.complex128) for b in bufs]
     J32 = [b["J"].to_numpy((nd, m.n, 3), np.complex128) for m, b in zip(ms, bufs)]
-    # the same fields through the per-mesh driver (current sets in passes of 4)
-    t_field_driver = timed(lambda: [m.field_batch_into(npp, w["k"], nd, b["J"], b["P"], M, b["E"], b["H"],
-                                                       inc=np.concatenate([w["k"] * w["dirs"], w["pols"]], axis=1),
-                                                       precision=B.PREC_MIXED)
-                                    for w, m, b in zip(work, ms, bufs)])
+    # the same fields through the per-mesh driver: one launch per mesh through operator
+    # matrices (its default at 64 sets), and current sets in passes of 4
+    def per_mesh():
+        for w, m, b in zip(work, ms, bufs):
+            m.field_batch_into(npp, w["k"], nd, b["J"], b["P"], M, b["E"], b["H"],
+                               inc=np.concatenate([w["k"] * w["dirs"], w["pols"]], axis=1), precision=B.PREC_MIXED)
+    t_field_opmat_mesh = timed(per_mesh)
+    min_rhs = ctx.get_tuning("opmat_min_rhs")
+    ctx.set_tuning("opmat_min_rhs", 1 << 20)
+    t_field_driver = timed(per_mesh)
+    ctx.set_tuning("opmat_min_rhs", min_rhs)
     # FP64 form of the same batch (same workers): FP64 LU of a copy of each matrix, FP64 solve, FP64 field plan
     for b in bufs:
         b["F"].free()
@@ -630,8 +648,13 @@ def batched_leg(tune_cache=None, workers=4):
                       "max_residual": max((r[1] for r in refine), default=0.0),
                       "fp64_fallback_meshes": fallbacks},
            "field_graph_replay_s": round(t_field, 4), "field_eager_s": round(t_field_eager, 4),
-           "field_per_mesh_driver_s": round(t_field_driver, 4),
-           "field_plan_speedup_vs_driver": round(t_field_driver / t_field, 2),
+           "kernel_ms": {"k_assemble_multi": round(kt_asm, 3), "k_field_opmat": round(kt["plan_field_tiles"], 3),
+                         "zgemm_all_meshes": round(kt["plan_field_gemm"], 3)},
+           "k_field_opmat_tflops": round(f_info["pairs"] * FLOPS_FIELD_PAIR / kt["plan_field_tiles"] / 1e9, 2),
+           "k_assemble_multi_pairs_per_s": a_info["pairs"] / (kt_asm / 1e3),
+           "field_per_mesh_operator_matrix_s": round(t_field_opmat_mesh, 4),
+           "field_per_mesh_passes_of_4_s": round(t_field_driver, 4),
+           "field_plan_speedup_vs_passes": round(t_field_driver / t_field, 2),
            "assembly_pairs_per_s": pairs_asm / t_asm, "field_pair_sets_per_s": pairs_fld * nd / t_field,
            "total_s": round(total32, 3), "solves": solves, "solves_per_s": round(solves / total32, 2),
            "workers": workers,
@@ -641,8 +664,9 @@ def batched_leg(tune_cache=None, workers=4):
            "err_currents_rel_max": e_cur, "err_field_point_max_rel": e_pt, "err_field_set_max_rel": e_set,
            "err_field_point_max_rel_per_mesh": per_mesh}
     log(f"[bench] batched: assembly {t_asm:.3f} s, LU32 {t_lu:.2f} s, refine+solve {t_solve:.2f} s, field "
-        f"{t_field * 1e3:.1f} ms (graph; eager {t_field_eager * 1e3:.1f} ms, per-mesh driver "
-        f"{t_field_driver * 1e3:.1f} ms); FP64 form {total64:.2f} s vs {total32:.2f} s; field err {e_pt:.2e}")
+        f"{t_field * 1e3:.1f} ms (graph; eager {t_field_eager * 1e3:.1f} ms; per mesh {t_field_opmat_mesh * 1e3:.1f} "
+        f"ms, passes of 4 {t_field_driver * 1e3:.1f} ms); FP64 form {total64:.2f} s vs {total32:.2f} s; "
+        f"field err {e_set:.2e} set-wise")
     return res
 
 
@@ -674,6 +698,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 12-mesh batched form")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
     ap.add_argument("--cube-only", action="store_true", help="run only the precision form on the cube")
+    ap.add_argument("--batched-workers", type=int, default=4, help="worker streams of the batched form's solves")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -683,7 +708,7 @@ def main():
 
     import bielcc_amd as B
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args.tune_cache)}), flush=True)
+        print(json.dumps({"batched": batched_leg(args.tune_cache, args.batched_workers)}), flush=True)
         return
     if args.cube_only:
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
@@ -784,7 +809,7 @@ def main():
     if lu_s is not None and not args.no_mixed:
         out["precision_form_cube"] = cube_precision()
     if not args.no_batched:
-        out["batched"] = batched_leg(args.tune_cache)
+        out["batched"] = batched_leg(args.tune_cache, args.batched_workers)
     if not args.no_serve:
         out["serving"] = serving_leg(nq=args.serve_queries)
     if not args.no_cpu:

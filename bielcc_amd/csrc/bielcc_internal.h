@@ -224,5 +224,15 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* A, long l
 hipError_t blas_zgemm(hipStream_t s, LuWork* w, long long m, long long n, long long k, double2 alpha,
                       const double* A, long long lda, const double* B, long long ldb, double2 beta, double* C,
                       long long ldc, const char** msg);
+// batch GEMMs C_b = alpha A_b B_b + beta C_b with A_b = A + b sa, B_b = B + b sb, C_b = C + b sc
+// (strides in complex elements)
+hipError_t blas_zgemm_strided(hipStream_t s, LuWork* w, long long m, long long n, long long k, double2 alpha,
+                              const double* A, long long lda, long long sa, const double* B, long long ldb,
+                              long long sb, double2 beta, double* C, long long ldc, long long sc, long long batch,
+                              const char** msg);
+// out[:, r] (+)= sum_b part[b][:, r] for an m x nrhs column-major block (ldo; part dense, m rows);
+// accumulate = 0 overwrites out
+hipError_t launch_sum_partials(hipStream_t s, long long m, int nrhs, int nparts, const double* part, double* out,
+                               long long ldo, int accumulate);
 
 }  // namespace bielcc_impl

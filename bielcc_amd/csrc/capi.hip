@@ -43,6 +43,7 @@ int bielcc_ctx_destroy(bielcc_ctx ctx) {
         if (ctx->near_buf) (void)hipFree(ctx->near_buf);
         if (ctx->opmat_ws) (void)hipFree(ctx->opmat_ws);
         if (ctx->opmat_meta) (void)hipFree(ctx->opmat_meta);
+        if (ctx->opmat_part) (void)hipFree(ctx->opmat_part);
         lu_work_destroy(ctx->lu);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);

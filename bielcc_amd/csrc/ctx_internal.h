@@ -37,6 +37,8 @@ struct bielcc_ctx_s {
     size_t opmat_ws_cap = 0;
     void* opmat_meta = nullptr;   // its segment descriptors and tiles (grow-only)
     size_t opmat_meta_cap = 0;
+    void* opmat_part = nullptr;   // its split-K GEMM partial products (grow-only)
+    size_t opmat_part_cap = 0;
     int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     // plan (multi-mesh) tunables: cells per field tile and per assembly tile,
     // persistent workgroups per CU, operator-matrix workspace cap (MiB), and the
@@ -207,6 +209,21 @@ inline int build_near_list(bielcc_ctx ctx, int nq, const double* q, int q_stride
     }
     return BIELCC_OK;
 }
+
+// Split-K parts for the operator-matrix GEMMs of a segment of Ms points and N
+// cells: enough that the (3Ms / 128-row tiles) x parts fill the device twice.
+inline int opmat_parts(bielcc_ctx ctx, int Ms, int N) {
+    const long long tiles_m = (3LL * Ms + 127) / 128;
+    long long s = (2LL * ctx->n_cu + tiles_m - 1) / tiles_m;
+    s = std::min<long long>(s, std::max(1, N / 32));   // at least 32 cells per part
+    return (int)std::max<long long>(1, std::min<long long>(s, 64));
+}
+
+// E = multE WK J + E, H = WR J for one segment (E, H at the segment's first
+// point, leading dimension ldo); with nparts > 1 the cells are split into
+// parts whose products go to part (nparts x 3Ms x nrhs complex) and are summed.
+int opmat_gemms(bielcc_ctx ctx, const bielcc_impl::OpSeg& sg, int nrhs, const double* jE, double2 mE, double* E,
+                double* H, long long ldo, double* part, int nparts);
 
 // bielcc_field_batch for many current sets (electric only) through the
 // operator matrices in context-owned grow-only buffers (plan.hip).  inc_d:

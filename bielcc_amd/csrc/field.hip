@@ -756,4 +756,30 @@ hipError_t launch_init_incident(hipStream_t s, int M, int nrhs, const double* pt
     return hipGetLastError();
 }
 
+// out[:, r] (+)= sum_b part[b][:, r]: the split-K partial products of the
+// operator-matrix GEMMs (each part m x nrhs, dense), in a fixed order.
+__global__ __launch_bounds__(256) void k_sum_partials(long long m, int nrhs, int nparts,
+                                                      const double2* __restrict__ part, double2* __restrict__ out,
+                                                      long long ldo, int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= m) return;
+    double2 acc = accumulate ? out[(size_t)r * ldo + i] : make_double2(0., 0.);
+    const size_t stride = (size_t)m * nrhs;
+    for (int b = 0; b < nparts; b++) {
+        const double2 v = part[b * stride + (size_t)r * m + i];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    out[(size_t)r * ldo + i] = acc;
+}
+
+hipError_t launch_sum_partials(hipStream_t s, long long m, int nrhs, int nparts, const double* part, double* out,
+                               long long ldo, int accumulate) {
+    if (m <= 0 || nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((m + 255) / 256), nrhs), dim3(256), 0, s, m, nrhs, nparts,
+                       reinterpret_cast<const double2*>(part), reinterpret_cast<double2*>(out), ldo, accumulate);
+    return hipGetLastError();
+}
+
 }  // namespace bielcc_impl

@@ -471,6 +471,27 @@ hipError_t blas_zgemm(hipStream_t s, LuWork* w, long long m, long long n, long l
     return hipGetLastError();
 }
 
+hipError_t blas_zgemm_strided(hipStream_t s, LuWork* w, long long m, long long n, long long k, double2 alpha,
+                              const double* A, long long lda, long long sa, const double* B, long long ldb,
+                              long long sb, double2 beta, double* C, long long ldc, long long sc, long long batch,
+                              const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    if (m <= 0 || n <= 0 || batch <= 0) return hipSuccess;
+    rocblas_set_stream(w->blas, s);
+    const rocblas_double_complex al = {alpha.x, alpha.y}, be = {beta.x, beta.y};
+    rocblas_status st = rocblas_zgemm_strided_batched_64(
+        w->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &al,
+        reinterpret_cast<const rocblas_double_complex*>(A), lda, sa,
+        reinterpret_cast<const rocblas_double_complex*>(B), ldb, sb, &be,
+        reinterpret_cast<rocblas_double_complex*>(C), ldc, sc, batch);
+    if (st != rocblas_status_success) {
+        *msg = "rocblas_zgemm_strided_batched_64 failed";
+        return hipErrorUnknown;
+    }
+    return hipGetLastError();
+}
+
 hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, float* Fd, long long ldf,
                        int* piv, int* info, const char** msg) {
     const long long tot = (long long)n * n;

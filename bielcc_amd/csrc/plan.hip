@@ -28,6 +28,7 @@ struct NearSet {
 struct SegInfo {
     int item, p0, M;     // points p0 .. p0+M-1 of the item
     size_t ws_off;       // byte offset of WK in the workspace (WR follows)
+    int parts;           // split-K parts of its GEMMs
 };
 
 struct Group {
@@ -58,6 +59,7 @@ struct bielcc_plan_s {
     OpSeg* segs_d = nullptr;
     char* ws = nullptr;
     size_t ws_bytes = 0;
+    double* part = nullptr;        // split-K partial products (the largest segment's)
     // both
     std::vector<NearSet> near;     // per segment (field) / per item (assembly)
     std::vector<Group> groups;
@@ -132,8 +134,10 @@ int grid_for(bielcc_ctx ctx, int ntiles) {
 
 int run_assemble(bielcc_ctx ctx, bielcc_plan p) {
     const Group& g = p->groups[0];
+    Timer t(ctx, "plan_assemble_tiles");
     HIPCHK(launch_assemble_multi(ctx->stream, p->aitems_d, p->tiles_d, g.nt, g.grid, p->which, p->ck,
                                  p->precision == BIELCC_PREC_MIXED));
+    t.stop();
     for (size_t i = 0; i < p->aitems.size(); i++) {
         const NearSet& ns = p->near[i];
         if (ns.n == 0) continue;
@@ -150,25 +154,25 @@ int run_field(bielcc_ctx ctx, bielcc_plan p) {
         HIPCHK(launch_init_incident(ctx->stream, it.M, it.nrhs, it.pts_d, it.inc_d, it.E_d));
     const bool p32 = p->precision == BIELCC_PREC_MIXED;
     for (const Group& g : p->groups) {
-        HIPCHK(launch_field_opmat(ctx->stream, p->segs_d, p->tiles_d + g.t0, g.nt, g.grid, p->ck, p32));
+        {
+            Timer t(ctx, "plan_field_tiles");
+            HIPCHK(launch_field_opmat(ctx->stream, p->segs_d, p->tiles_d + g.t0, g.nt, g.grid, p->ck, p32));
+            t.stop();
+        }
         for (int s = g.s0; s < g.s1; s++) {
             const NearSet& ns = p->near[s];
             if (ns.n > 0) HIPCHK(launch_opmat_near(ctx->stream, ns.n, ns.lq, ns.lc, p->segs[s], p->np, p->near_mode));
         }
+        Timer tg(ctx, "plan_field_gemm");
         for (int s = g.s0; s < g.s1; s++) {
             const SegInfo& si = p->seginfo[s];
-            const OpSeg& sg = p->segs[s];
             const bielcc_field_item& it = p->fitems[si.item];
-            const long long m3 = 3LL * si.M, n3 = 3LL * sg.N, ldo = 3LL * it.M;
-            const char* msg = nullptr;
-            hipError_t e = blas_zgemm(ctx->stream, ctx->lu, m3, it.nrhs, n3, make_double2(it.multE_re, it.multE_im),
-                                      sg.WK, m3, it.jE_d, n3, make_double2(1., 0.), it.E_d + 6 * (size_t)si.p0, ldo,
-                                      &msg);
-            if (e == hipSuccess && it.H_d)
-                e = blas_zgemm(ctx->stream, ctx->lu, m3, it.nrhs, n3, make_double2(1., 0.), sg.WR, m3, it.jE_d, n3,
-                               make_double2(0., 0.), it.H_d + 6 * (size_t)si.p0, ldo, &msg);
-            if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+            if (int rc = opmat_gemms(ctx, p->segs[s], it.nrhs, it.jE_d, make_double2(it.multE_re, it.multE_im),
+                                     it.E_d + 6 * (size_t)si.p0, it.H_d ? it.H_d + 6 * (size_t)si.p0 : nullptr,
+                                     3LL * it.M, p->part, si.parts))
+                return rc;
         }
+        tg.stop();
     }
     return BIELCC_OK;
 }
@@ -180,6 +184,39 @@ int run_impl(bielcc_ctx ctx, bielcc_plan p) {
 }  // namespace
 
 namespace bielcc_capi {
+
+int opmat_gemms(bielcc_ctx ctx, const OpSeg& sg, int nrhs, const double* jE, double2 mE, double* E, double* H,
+                long long ldo, double* part, int nparts) {
+    const long long m3 = 3LL * sg.M, n3 = 3LL * sg.N;
+    const char* msg = nullptr;
+    hipError_t e = hipSuccess;
+    if (nparts <= 1) {
+        e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, n3, mE, sg.WK, m3, jE, n3, make_double2(1., 0.), E, ldo, &msg);
+        if (e == hipSuccess && H)
+            e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, n3, make_double2(1., 0.), sg.WR, m3, jE, n3,
+                           make_double2(0., 0.), H, ldo, &msg);
+        if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+        return BIELCC_OK;
+    }
+    const int cpp = (sg.N + nparts - 1) / nparts;   // cells per part
+    const int full = sg.N / cpp, rest = sg.N - full * cpp;
+    const long long kc = 3LL * cpp, sc = m3 * nrhs;
+    for (int which = 0; which < (H ? 2 : 1); which++) {
+        const double* W = which == 0 ? sg.WK : sg.WR;
+        const double2 al = which == 0 ? mE : make_double2(1., 0.);
+        e = blas_zgemm_strided(ctx->stream, ctx->lu, m3, nrhs, kc, al, W, m3, kc * m3, jE, n3, kc,
+                               make_double2(0., 0.), part, m3, sc, full, &msg);
+        if (e == hipSuccess && rest > 0)
+            e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, 3LL * rest, al, W + 2 * (size_t)(full * kc * m3), m3,
+                           jE + 2 * (size_t)(full * kc), n3, make_double2(0., 0.), part + 2 * (size_t)(full * sc), m3,
+                           &msg);
+        if (e == hipSuccess)
+            e = launch_sum_partials(ctx->stream, m3, nrhs, full + (rest > 0), part, which == 0 ? E : H, ldo,
+                                    which == 0);
+        if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    }
+    return BIELCC_OK;
+}
 
 int field_opmat(bielcc_ctx ctx, DevMesh& m, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
                 const double* jE_d, double mEr, double mEi, const double* inc_d, int near_mode, int precision, int M,
@@ -246,14 +283,13 @@ int field_opmat(bielcc_ctx ctx, DevMesh& m, const bielcc_num_param* np, double k
             HIPCHK(launch_opmat_near(ctx->stream, tot_h, lq, lc, sg, npp, near_mode));
             t.stop();
         }
-        const long long m3 = 3LL * sg.M, n3 = 3LL * N, ldo = 3LL * M;
-        const char* msg = nullptr;
-        hipError_t e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, n3, make_double2(mEr, mEi), sg.WK, m3, jE_d, n3,
-                                  make_double2(1., 0.), E_d + 6 * (size_t)p0, ldo, &msg);
-        if (e == hipSuccess && H_d)
-            e = blas_zgemm(ctx->stream, ctx->lu, m3, nrhs, n3, make_double2(1., 0.), sg.WR, m3, jE_d, n3,
-                           make_double2(0., 0.), H_d + 6 * (size_t)p0, ldo, &msg);
-        if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+        const int parts = opmat_parts(ctx, sg.M, N);
+        if (parts > 1)
+            HIPCHK(ensure(ctx->opmat_part, ctx->opmat_part_cap, (size_t)parts * 3 * sg.M * nrhs * 16));
+        if (int rc = opmat_gemms(ctx, sg, nrhs, jE_d, make_double2(mEr, mEi), E_d + 6 * (size_t)p0,
+                                 H_d ? H_d + 6 * (size_t)p0 : nullptr, 3LL * M,
+                                 reinterpret_cast<double*>(ctx->opmat_part), parts))
+            return rc;
     }
     return BIELCC_OK;
 }
@@ -385,7 +421,7 @@ int bielcc_plan_field(bielcc_ctx ctx, int nitems, const bielcc_field_item* items
             sg.mode = near_mode; sg.ac = np->analytic_calc; sg.thresh = np->analytic_calc * m->grid_step;
             sg.WK = sg.WR = nullptr;
             p->segs.push_back(sg);
-            p->seginfo.push_back(SegInfo{i, p0, Ms, 0});
+            p->seginfo.push_back(SegInfo{i, p0, Ms, 0, opmat_parts(ctx, Ms, m->n)});
             p->pairs += (long long)Ms * m->n;
         }
     }
@@ -419,6 +455,11 @@ int bielcc_plan_field(bielcc_ctx ctx, int nitems, const bielcc_field_item* items
     close_group((int)p->segs.size());
     hipError_t e = hipSuccess;
     if (p->ws_bytes > 0) e = palloc(p, p->ws_bytes, &p->ws);
+    size_t part_bytes = 0;
+    for (const SegInfo& si : p->seginfo)
+        if (si.parts > 1)
+            part_bytes = std::max(part_bytes, (size_t)si.parts * 3 * si.M * p->fitems[si.item].nrhs * 16);
+    if (e == hipSuccess && part_bytes > 0) e = palloc(p, part_bytes, &p->part);
     if (e != hipSuccess) { plan_free(p); return hip_fail(ctx, e, "operator-matrix workspace"); }
     for (size_t s = 0; s < p->segs.size(); s++) {
         const SegInfo& si = p->seginfo[s];

@@ -331,7 +331,10 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);
  * With timing on, the drivers bracket their dominant kernels with HIP events
  * on the context's stream; bielcc_timing reads the accumulated milliseconds
  * and launch count of one of: "field_far", "assemble_far", "lu_factor",
- * "lu_solve", "near", "assemble_generic", "lu32_factor", "refine_solve". */
+ * "lu_solve", "near", "assemble_generic", "lu32_factor", "refine_solve",
+ * "field_opmat", "plan_assemble", "plan_field" (a whole plan run) and, for
+ * eager plan runs, "plan_assemble_tiles", "plan_field_tiles" (the persistent
+ * kernels) and "plan_field_gemm" (the GEMMs). */
 int bielcc_timing_enable(bielcc_ctx ctx, int on);
 int bielcc_timing_reset(bielcc_ctx ctx);
 int bielcc_timing(bielcc_ctx ctx, const char* name, double* total_ms, int* count);
