@@ -464,7 +464,7 @@ def run_workers(ctxs, assign, fn):
         raise errs[0]
 
 
-def batched_leg(tune_cache=None, workers=4):
+def batched_leg(tune_cache=None, workers=4, solve=True):
     """Batched form on the 12-mesh workload, FP32 LU + FP64 refinement:
     assembly of all matrices in one persistent launch, FP32 LU and refinement
     (64 right-hand sides) per mesh, the field of all 768 current sets through
@@ -512,6 +512,29 @@ def batched_leg(tune_cache=None, workers=4):
     ctx.timing_enable(False)
     a_info = {k: aplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs")}
     aplan.close()
+    if not solve:
+        # counter runs of the plan kernels: synthetic currents, no factorisation
+        rng = np.random.default_rng(1)
+        for m, b in zip(ms, bufs):
+            ctx.upload(b["J"], rng.normal(size=(nd, m.n, 3)) + 1j * rng.normal(size=(nd, m.n, 3)))
+        fitems = [dict(mesh=m, k=w["k"], nrhs=nd, jE=b["J"], inc=b["inc"], M=M, pts=b["P"], E=b["E"], H=b["H"])
+                  for w, m, b in zip(work, ms, bufs)]
+        ctx.set_tuning("plan_ws_mb", 96 * 1024)
+        fplan = ctx.plan_field(fitems, npp, precision=B.PREC_MIXED)
+        t_f = [timed(fplan.run) for _ in range(3)]
+        fplan.close()
+        fplan64 = ctx.plan_field(fitems, npp, precision=B.PREC_FP64)
+        t_f64 = [timed(fplan64.run) for _ in range(2)]
+        fplan64.close()
+        ctx.set_tuning("plan_ws_mb", 24576)
+        for b in bufs:
+            for d in b.values():
+                d.free()
+        for m in ms:
+            m.close()
+        ctx.close()
+        return {"assembly_s": round(t_asm, 4), "field_mixed_s": [round(t, 4) for t in t_f],
+                "field_fp64_s": [round(t, 4) for t in t_f64], "assembly_plan": a_info}
     # FP32 LU per mesh and refinement with 64 right-hand sides: the independent
     # systems run concurrently on worker contexts (own stream and BLAS handle each)
     wctx = [B.Context(0) for _ in range(workers)]
@@ -699,6 +722,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
     ap.add_argument("--cube-only", action="store_true", help="run only the precision form on the cube")
     ap.add_argument("--batched-workers", type=int, default=4, help="worker streams of the batched form's solves")
+    ap.add_argument("--batched-no-solve", action="store_true",
+                    help="batched form: plan kernels only, on synthetic currents (counter runs)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -707,14 +732,15 @@ def main():
         sys.exit(2)
 
     import bielcc_amd as B
+    if args.copy_calib:
+        copy_calibration()
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args.tune_cache, args.batched_workers)}), flush=True)
+        print(json.dumps({"batched": batched_leg(args.tune_cache, args.batched_workers,
+                                                 solve=not args.batched_no_solve)}), flush=True)
         return
     if args.cube_only:
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
         return
-    if args.copy_calib:
-        copy_calibration()
     w = workload(args.config)
     verts, k, pts = w["verts"], w["k"], w["pts"]
     n, M = verts.shape[0], pts.shape[0]
