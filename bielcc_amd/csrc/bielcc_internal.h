@@ -206,6 +206,10 @@ bool lu_get_lib_trsm(const LuWork*);
 // (default 2^20, about a second; a real wait lasts microseconds)
 void lu_set_spin_limit(LuWork*, int polls);
 int lu_get_spin_limit(const LuWork*);
+// LU panel: one launch per column (pivot of the next column fused into the update): 1 always,
+// 0 never (two launches), 2 for small orders only
+void lu_set_fused_panel(LuWork*, int mode);
+int lu_get_fused_panel(const LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);

@@ -573,6 +573,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_spin_limit(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "lu_fused_panel") {
+        if (value < 0 || value > 2) return fail(ctx, BIELCC_ERR_ARG, "lu_fused_panel must be 0, 1 or 2");
+        lu_set_fused_panel(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "trsm_lib") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "trsm_lib must be 0 or 1");
         lu_set_lib_trsm(ctx->lu, value == 1);
@@ -610,6 +615,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "trsv") *value = lu_get_trsv(ctx->lu) ? 1 : 0;
     else if (k == "trsm_lib") *value = lu_get_lib_trsm(ctx->lu) ? 1 : 0;
     else if (k == "trsv_spin") *value = lu_get_spin_limit(ctx->lu);
+    else if (k == "lu_fused_panel") *value = lu_get_fused_panel(ctx->lu);
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;
     else if (k == "plan_asm_cells") *value = ctx->plan_asm_cells;
     else if (k == "plan_wg_per_cu") *value = ctx->plan_wg_per_cu;

@@ -34,6 +34,10 @@ namespace bielcc_impl {
 
 constexpr int kNB = 64;
 constexpr int kPivBlocks = 256;
+constexpr int kPR = 32;   // panel rows per workgroup of the column kernels (x 8 column groups)
+// Up to this order the fused panel column (one launch instead of two) is faster: launches
+// dominate there; above it the fused step's serial last workgroup costs more than a launch.
+constexpr int kFusedMaxN = 6144;
 
 template <typename R> struct V2;
 template <> struct V2<double> { using T = double2; };
@@ -70,6 +74,8 @@ struct LuWork {
     bool no_trsv = false;           // force the blocked TRSM + GEMM path (tests)
     bool lib_trsm = true;           // many right-hand sides: rocBLAS trsm (else the blocked TRSM + GEMM loop)
     int spin_limit = 1 << 20;       // polls per sync-free wait before it gives up (tests lower it)
+    unsigned* step_count = nullptr; // blocks done in the current fused panel step (reset by its last block)
+    int fused_panel = 2;            // one launch per panel column: 1 always, 0 never, 2 for n <= kFusedMaxN
 };
 
 LuWork* lu_work_create() { return new LuWork(); }
@@ -78,6 +84,8 @@ void lu_set_trsv(LuWork* w, bool on) { w->no_trsv = !on; }
 bool lu_get_trsv(const LuWork* w) { return !w->no_trsv; }
 void lu_set_lib_trsm(LuWork* w, bool on) { w->lib_trsm = on; }
 void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
+void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
+int lu_get_fused_panel(const LuWork* w) { return w->fused_panel; }
 int lu_get_spin_limit(const LuWork* w) { return w->spin_limit; }
 bool lu_get_lib_trsm(const LuWork* w) { return w->lib_trsm; }
 
@@ -100,14 +108,14 @@ void lu_work_destroy(LuWork* w) {
     if (!w) return;
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
-    (void)hipFree(w->perm); (void)hipFree(w->ref);
+    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->step_count);
     (void)hipFree(w->flags); (void)hipFree(w->ticket); (void)hipFree(w->err);
     delete w;
 }
 
 static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
     hipError_t e;
-    const int need = std::max(kPivBlocks, (n + 255) / 256 + 1);
+    const int need = std::max(kPivBlocks, (n + kPR - 1) / kPR + 1);
     if (w->pcap < need) {
         (void)hipFree(w->pval);
         (void)hipFree(w->pidx);
@@ -121,6 +129,10 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
     if (!w->pinv) {
         if ((e = hipMalloc(&w->pinv, 4 * sizeof(double))) != hipSuccess) return e;
         if ((e = hipMalloc(&w->info, sizeof(int))) != hipSuccess) return e;
+    }
+    if (!w->step_count) {
+        if ((e = hipMalloc(&w->step_count, sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMemset(w->step_count, 0, sizeof(unsigned))) != hipSuccess) return e;
     }
     if (!w->blas) {
         if (rocblas_create_handle(&w->blas) != rocblas_status_success) {
@@ -235,7 +247,9 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, c2<R
     const int nc = c1 - j - 1;
     for (int t = threadIdx.x; t < nc; t += 256) urow[t] = A[(size_t)(j + 1 + t) * lda + j];
     __syncthreads();
-    const int i = j + 1 + blockIdx.x * 256 + threadIdx.x;
+    // lane (r = tid % kPR, column group cg = tid / kPR): row i, columns j+1+cg, j+1+cg+256/kPR, ...
+    const int r = threadIdx.x & (kPR - 1), cg = threadIdx.x / kPR;
+    const int i = j + 1 + blockIdx.x * kPR + r;
     double best = -1.;
     int bi = 0x7fffffff;
     if (i < n) {
@@ -243,9 +257,9 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, c2<R
         if (pinv[2] != R(0)) {
             c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
             l = s;
-            A[(size_t)j * lda + i] = l;
+            if (cg == 0) A[(size_t)j * lda + i] = l;
         }
-        for (int t = 0; t < nc; t++) {
+        for (int t = cg; t < nc; t += 256 / kPR) {
             c2<R>* p = A + (size_t)(j + 1 + t) * lda + i;
             c2<R> u = urow[t];
             c2<R> a = *p;
@@ -271,6 +285,117 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, c2<R
         __syncthreads();
     }
     if (threadIdx.x == 0) { pval[blockIdx.x] = sv[0]; pidx[blockIdx.x] = si[0]; }
+}
+
+// One panel column in one launch: k_panel_update for column j, then the last
+// workgroup to finish (device-scope counter, fenced) does k_piv_final's work
+// for column j + 1 -- the final argmax over the workgroups' partials, the row
+// interchange inside the panel, piv[j+1], the reciprocal pivot -- and resets
+// the counter.  No workgroup waits for another.  Requires j + 1 < c1.
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1, c2<R>* __restrict__ A,
+                                                    long long lda, R* __restrict__ pinv, double* __restrict__ pval,
+                                                    int* __restrict__ pidx, int* __restrict__ piv,
+                                                    int* __restrict__ info, unsigned* __restrict__ count) {
+    __shared__ c2<R> urow[kNB];
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    __shared__ int s_last, s_p;
+    const int nc = c1 - j - 1;
+    for (int t = threadIdx.x; t < nc; t += 256) urow[t] = A[(size_t)(j + 1 + t) * lda + j];
+    __syncthreads();
+    // lane (r = tid % kPR, column group cg = tid / kPR): row i, columns j+1+cg, j+1+cg+256/kPR, ...
+    const int r = threadIdx.x & (kPR - 1), cg = threadIdx.x / kPR;
+    const int i = j + 1 + blockIdx.x * kPR + r;
+    double best = -1.;
+    int bi = 0x7fffffff;
+    if (i < n) {
+        c2<R> l = A[(size_t)j * lda + i];
+        if (pinv[2] != R(0)) {
+            c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
+            l = s;
+            if (cg == 0) A[(size_t)j * lda + i] = l;
+        }
+        for (int t = cg; t < nc; t += 256 / kPR) {
+            c2<R>* p = A + (size_t)(j + 1 + t) * lda + i;
+            c2<R> u = urow[t];
+            c2<R> a = *p;
+            a.x -= l.x * u.x - l.y * u.y;
+            a.y -= l.x * u.y + l.y * u.x;
+            *p = a;
+            if (t == 0) { best = cabs1<R>(a); bi = i; }
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double v = sv[threadIdx.x + off];
+            int ix = si[threadIdx.x + off];
+            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && ix < si[threadIdx.x])) {
+                sv[threadIdx.x] = v;
+                si[threadIdx.x] = ix;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pval[blockIdx.x] = sv[0];
+        pidx[blockIdx.x] = si[0];
+        __threadfence();   // this workgroup's column and partial before its count
+        s_last = atomicAdd(count, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();       // every other workgroup's writes are visible from here on
+    // column j + 1: final argmax over the gridDim.x partials
+    const int jn = j + 1;
+    best = -1.;
+    bi = 0x7fffffff;
+    for (int t = threadIdx.x; t < (int)gridDim.x; t += 256) {
+        const double v = __hip_atomic_load(&pval[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int ix = __hip_atomic_load(&pidx[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v > best || (v == best && ix < bi)) { best = v; bi = ix; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double v = sv[threadIdx.x + off];
+            int ix = si[threadIdx.x + off];
+            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && ix < si[threadIdx.x])) {
+                sv[threadIdx.x] = v;
+                si[threadIdx.x] = ix;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int p = (si[0] == 0x7fffffff) ? jn : si[0];
+        piv[jn] = p;
+        s_p = p;
+        if (!(sv[0] > 0.) && *info == 0) *info = jn + 1;
+        *count = 0u;       // every workgroup has counted: ready for the next step
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != jn) {
+        for (int c = c0 + threadIdx.x; c < c1; c += 256) {
+            c2<R>* colp = A + (size_t)c * lda;
+            const c2<R> t = colp[jn];
+            colp[jn] = colp[p];
+            colp[p] = t;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const c2<R> a = A[(size_t)jn * lda + jn];
+        const R den = a.x * a.x + a.y * a.y;
+        if (den > R(0)) { pinv[0] = a.x / den; pinv[1] = -a.y / den; pinv[2] = R(1); }
+        else { pinv[0] = R(0); pinv[1] = R(0); pinv[2] = R(0); }
+    }
 }
 
 // Apply the interchanges piv[k0 .. k0+nb) to columns [cbeg, cend).
@@ -402,6 +527,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     if (e != hipSuccess) return e;
     R* pinv = reinterpret_cast<R*>(w->pinv);
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
+    const bool fused = w->fused_panel == 1 || (w->fused_panel == 2 && n <= kFusedMaxN);
     for (int k0 = 0; k0 < n; k0 += kNB) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
@@ -409,16 +535,23 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         for (int jj = 0; jj < nb; jj++) {
             const int j = k0 + jj;
             const int m = n - j;
-            if (jj == 0) {   // first panel column: fresh search (later ones come from k_panel_update)
+            if (jj == 0) {   // first panel column: fresh search (later ones come from the column before)
                 nparts = min(kPivBlocks, (m + 255) / 256);
                 hipLaunchKernelGGL(k_piv_partial<R>, dim3(nparts), dim3(256), 0, s, n, j, A, lda, w->pval, w->pidx);
+                hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval,
+                                   w->pidx, piv, pinv, w->info);
+            } else if (!fused) {
+                hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval,
+                                   w->pidx, piv, pinv, w->info);
             }
-            hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval,
-                               w->pidx, piv, pinv, w->info);
             if (m > 1) {
-                nparts = (m - 1 + 255) / 256;
-                hipLaunchKernelGGL(k_panel_update<R>, dim3(nparts), dim3(256), 0, s, n, j, c1, A, lda, pinv,
-                                   w->pval, w->pidx);
+                nparts = (m - 1 + kPR - 1) / kPR;
+                if (fused && j + 1 < c1)   // this column's update and the next column's pivot
+                    hipLaunchKernelGGL(k_panel_step<R>, dim3(nparts), dim3(256), 0, s, n, j, k0, c1, A, lda, pinv,
+                                       w->pval, w->pidx, piv, w->info, w->step_count);
+                else
+                    hipLaunchKernelGGL(k_panel_update<R>, dim3(nparts), dim3(256), 0, s, n, j, c1, A, lda, pinv,
+                                       w->pval, w->pidx);
             }
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -317,7 +317,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * "trsm_lib" [1] = solves with more right-hand sides use rocBLAS trsm per
  * triangle (0: the library's blocked TRSM + GEMM loop); "trsv_spin"
  * [1048576] = polls per wait of the one-kernel solve before it gives up (a
- * solve whose wait gave up is redone on the blocked path; tests lower it).
+ * solve whose wait gave up is redone on the blocked path; tests lower it);
+ * "lu_fused_panel" [2] = one launch per LU panel column (1), two (0), or
+ * one up to order 6,144 and two above (2).
  * Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.

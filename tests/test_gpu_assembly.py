@@ -189,3 +189,24 @@ def test_sync_free_solve_recovers_from_an_expired_wait(ctx):
         assert res <= 1e-13 and np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-12
     finally:
         ctx.set_tuning("trsv_spin", 1 << 20)
+
+
+@pytest.mark.parametrize("n", [65, 300, 1000])
+def test_fused_panel_matches_two_launch_panel(ctx, n):
+    """One launch per panel column (next column's pivot by the last workgroup)
+    gives bit for bit the factors and pivots of the two-launch panel, FP64 and FP32."""
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    out = {}
+    for fused in (1, 0):
+        ctx.set_tuning("lu_fused_panel", fused)
+        dA = ctx.to_device(np.ascontiguousarray(A.T))
+        piv = ctx.empty(4 * n)
+        ctx.lu_factor(dA, n, n, piv)
+        dF, piv32 = ctx.empty(n * n * 8), ctx.empty(4 * n)
+        ctx.lu32_factor(ctx.to_device(np.ascontiguousarray(A.T)), n, n, dF, n, piv32)
+        out[fused] = (dA.to_numpy((n, n), np.complex128), piv.to_numpy((n,), np.int32),
+                      dF.to_numpy((n, n), np.complex64), piv32.to_numpy((n,), np.int32))
+    ctx.set_tuning("lu_fused_panel", 2)
+    for a, b in zip(out[1], out[0]):
+        assert np.array_equal(a, b)
