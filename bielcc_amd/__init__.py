@@ -7,8 +7,8 @@ not touch the GPU.
 """
 from .api import (BielccError, Context, DeviceArray, Mesh, Plan, NumParam, KernelParam, IntegralParam, num_param,
                   kernel_param, OPS, KERNELS, DEFAULT_NUM_PARAM, PREC_FP64, PREC_MIXED)
-from . import mesh, tune
+from . import mesh, rwg, tune
 
 __all__ = ["BielccError", "Context", "DeviceArray", "Mesh", "Plan", "NumParam", "KernelParam", "IntegralParam",
            "num_param", "kernel_param", "OPS", "KERNELS", "DEFAULT_NUM_PARAM", "PREC_FP64", "PREC_MIXED",
-           "mesh", "tune"]
+           "mesh", "rwg", "tune"]

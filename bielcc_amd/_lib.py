@@ -108,6 +108,13 @@ _SIGS = {
     "bielcc_plan_capture": (C.c_int, [C.c_void_p, C.c_void_p]),
     "bielcc_plan_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "bielcc_plan_info": (C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, _DP]),
+    "bielcc_aver_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                    C.c_void_p]),
+    "bielcc_rwg_create": (C.c_int, [C.c_int, C.c_int, _DP, C.POINTER(C.c_void_p)]),
+    "bielcc_rwg_destroy": (C.c_int, [C.c_void_p]),
+    "bielcc_rwg_sizes": (C.c_int, [C.c_void_p, _IP, _IP, _IP]),
+    "bielcc_rwg_get": (C.c_int, [C.c_void_p, _DP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _IP, _IP]),
+    "bielcc_rwg_write_vtk": (C.c_int, [C.c_void_p, _DP, C.c_char_p]),
     "bielcc_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "bielcc_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, _IP]),
     "bielcc_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),

@@ -260,6 +260,16 @@ class Context:
                                                    float(analyticR), k.real, k.imag, _p(dr)))
         return dr.to_numpy((n, 3), np.complex128)
 
+    def aver_batch(self, which, x, y, eps) -> np.ndarray:
+        """Averaging functions (kernel_lib.h:560-634): which 0 Ker_AverDiv, 1
+        Ker_AverDiv_NearEdge, 2 Ker_BasicAverFunc (column 0).  Returns [n][3]."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        dx, dy, dr = self.to_device(x), self.to_device(y), self.empty(max(n, 1) * 24)
+        self._check(self.lib.bielcc_aver_batch(self.h, int(which), n, _p(dx), _p(dy), float(eps), _p(dr)))
+        return dr.to_numpy((n, 3))
+
     def plane_wave_e(self, pts, kvec, e0) -> np.ndarray:
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
         n = pts.shape[0]

@@ -179,6 +179,9 @@ hipError_t launch_currents(hipStream_t s, int N, const double* ctab, int nrhs, c
 // ---- generic batches (generic.hip)
 hipError_t launch_op_batch(hipStream_t s, int op, int nv, int n, const double* cells, const double* x,
                            const double* j, const bielcc_dev::NumP& np, double k_re, double k_im, double* res);
+// averaging functions: which 0 Ker_AverDiv, 1 Ker_AverDiv_NearEdge, 2 Ker_BasicAverFunc
+hipError_t launch_aver_batch(hipStream_t s, int which, int n, const double* x, const double* y, double eps,
+                             double* res);
 hipError_t launch_kernel_batch(hipStream_t s, int f, int n, const double* x, const double* y,
                                const bielcc_dev::KP& kp, double* res);
 hipError_t launch_integral_batch(hipStream_t s, int f, int nv, int n, const double* cells, const double* x,

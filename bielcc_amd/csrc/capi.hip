@@ -184,6 +184,17 @@ int bielcc_kernel_batch(bielcc_ctx ctx, int f, int n, const double* x_d, const d
     return BIELCC_OK;
 }
 
+int bielcc_aver_batch(bielcc_ctx ctx, int which, int n, const double* x_d, const double* y_d, double eps,
+                      double* res_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (which < 0 || which > 2 || n < 0) return fail(ctx, BIELCC_ERR_ARG, "which must be 0, 1 or 2 and n >= 0");
+    if (!(eps > 0.) || !std::isfinite(eps)) return fail(ctx, BIELCC_ERR_ARG, "eps must be positive and finite");
+    if (n > 0 && (!x_d || !y_d || !res_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    HIPCHK(launch_aver_batch(ctx->stream, which, n, x_d, y_d, eps, res_d));
+    return BIELCC_OK;
+}
+
 int bielcc_integral_batch(bielcc_ctx ctx, int f, int nv, int n, const double* cells_d, const double* x_d,
                           const bielcc_kernel_param* kp, const bielcc_integral_param* ip, double* res_d,
                           double* smoothR_d) {

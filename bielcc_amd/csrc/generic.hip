@@ -76,6 +76,42 @@ hipError_t launch_kernel_batch(hipStream_t s, int f, int n, const double* x, con
     return hipGetLastError();
 }
 
+// Averaging functions (electrodynamics/kernel_lib.h:560-634), rounded as the
+// reference writes them (no contraction): which 0 Ker_AverDiv, 1
+// Ker_AverDiv_NearEdge (eps = eps_edge), 2 Ker_BasicAverFunc (in res[i][0]).
+#pragma clang fp contract(off)
+__global__ void k_aver_batch(int which, int n, const double* __restrict__ x, const double* __restrict__ y, double eps,
+                             double* __restrict__ res) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* a = x + 3 * i;
+    const double* b = y + 3 * i;
+    double* o = res + 3 * i;
+    const double d = sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+    if (which == 2) {
+        double v = 0.;
+        if (!(d > eps)) {
+            const double deg = d * d / eps / eps;
+            v = exp(deg) / kPi / eps / eps;
+        }
+        o[0] = v; o[1] = 0.; o[2] = 0.;
+        return;
+    }
+    const double reps = d / eps;
+    const double deg = which == 0
+        ? (-6.0 + 2. * reps * reps) * exp(-reps * reps) / kPi / (eps * eps) / (eps * eps)
+        : (-20.0 + 8. * reps * reps) * exp(-reps * reps) / kPi / eps / eps / eps / eps;
+    for (int g = 0; g < 3; g++) o[g] = (a[g] - b[g]) * deg;
+}
+#pragma clang fp contract(fast)
+
+hipError_t launch_aver_batch(hipStream_t s, int which, int n, const double* x, const double* y, double eps,
+                             double* res) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_aver_batch, dim3((n + 255) / 256), dim3(256), 0, s, which, n, x, y, eps, res);
+    return hipGetLastError();
+}
+
 // nv = 4 / 3: IntegralUniversalPnt over cells [n][nv][3]; nv = 2: IntegralUniversalSegPnt
 // over segments [n][2][3].  r_out (optional) receives the mutated smoothR.
 template <int NV>

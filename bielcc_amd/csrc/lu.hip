@@ -252,8 +252,12 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, c2<R
     const int i = j + 1 + blockIdx.x * kPR + r;
     double best = -1.;
     int bi = 0x7fffffff;
+    // every column group reads the unscaled entry of column j before group 0
+    // writes the scaled one back (the groups are in different waves)
+    c2<R> l = mk2<R>(R(0), R(0));
+    if (i < n) l = A[(size_t)j * lda + i];
+    __syncthreads();
     if (i < n) {
-        c2<R> l = A[(size_t)j * lda + i];
         if (pinv[2] != R(0)) {
             c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
             l = s;
@@ -309,8 +313,12 @@ __global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1
     const int i = j + 1 + blockIdx.x * kPR + r;
     double best = -1.;
     int bi = 0x7fffffff;
+    // every column group reads the unscaled entry of column j before group 0
+    // writes the scaled one back (the groups are in different waves)
+    c2<R> l = mk2<R>(R(0), R(0));
+    if (i < n) l = A[(size_t)j * lda + i];
+    __syncthreads();
     if (i < n) {
-        c2<R> l = A[(size_t)j * lda + i];
         if (pinv[2] != R(0)) {
             c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
             l = s;

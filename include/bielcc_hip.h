@@ -130,6 +130,15 @@ int bielcc_op_batch(bielcc_ctx ctx, int op, int nv, int n, const double* cells_d
 int bielcc_kernel_batch(bielcc_ctx ctx, int f, int n, const double* x_d, const double* y_d,
                         const bielcc_kernel_param* kp, double* res_d);
 
+/* Averaging functions (electrodynamics/kernel_lib.h:560-634) at (x_d[i], y_d[i]),
+ * res [n][3] real: which 0 Ker_AverDiv (:573-583), 1 Ker_AverDiv_NearEdge
+ * (:599-609), both with eps_edge = eps (the reference's templates read
+ * KernelParam::eps_edge, a member KernelParam does not have, so they cannot be
+ * instantiated there); 2 Ker_BasicAverFunc (:626-634) in res[i][0], the other
+ * two entries 0. */
+int bielcc_aver_batch(bielcc_ctx ctx, int which, int n, const double* x_d, const double* y_d, double eps,
+                      double* res_d);
+
 /* IntegralUniversalPnt (integral_universal.h:38-220; nv = 4 or 3, cells_d
  * [n][nv][3]) or IntegralUniversalSegPnt (integral_universal_seg.h:38-98;
  * nv = 2, cells_d = segments [n][2][3]).  kp is the starting KernelParam;
@@ -155,6 +164,31 @@ int bielcc_analytic_batch(bielcc_ctx ctx, int which, int nv, int n, const double
 /* PlaneWaveFieldE (incident_field.h:20-29): out [n][3] complex. */
 int bielcc_plane_wave_e(bielcc_ctx ctx, int n, const double* pts_d, const double* kvec, const double* e0,
                         double* out_d);
+
+/* ------------------------------------------------------------ RWG topology (host side, no context)
+ * RwgSurfMesh (mesh/rwg_surf_mesh.h:300-526) of a cell list [n][nv][3]: unique
+ * points (1e-5 tolerance, geometry/tolerance_constants.h:12), unique edges,
+ * edge-cell incidence, cell sides with their direction (+1 / -1; a side whose
+ * points coincide is a pseudo-edge: edge -5, direction 0), edge mid points and
+ * orts, and the cells beside each side (-5: none).  Same orders as the
+ * reference, entry for entry. */
+typedef struct bielcc_rwg_s* bielcc_rwg;
+int bielcc_rwg_create(int n_cells, int verts_per_cell, const double* verts, bielcc_rwg* out);
+int bielcc_rwg_destroy(bielcc_rwg r);
+/* n_neighbours = total entries of the neighbour lists */
+int bielcc_rwg_sizes(bielcc_rwg r, int* n_points, int* n_edges, int* n_neighbours);
+/* Host arrays, any may be NULL: points [np][3], cell_points [n][nv],
+ * point_num_cells [np], point_num_edges [np], edges [ne][2] (lower, higher
+ * point), edge_num_cells [ne], cell_edges [n][nv], cell_edge_dir [n][nv],
+ * edge_mid [ne][3], edge_ort [ne][3], neighbour_offsets [n*nv+1] and
+ * neighbours [n_neighbours] (the list of side (c, v) is neighbours[off[c*nv+v]
+ * .. off[c*nv+v+1])). */
+int bielcc_rwg_get(bielcc_rwg r, double* points, int* cell_points, int* point_num_cells, int* point_num_edges,
+                   int* edges, int* edge_num_cells, int* cell_edges, int* cell_edge_dir, double* edge_mid,
+                   double* edge_ort, int* neighbour_offsets, int* neighbours);
+/* RwgSurfMesh::ExportRwgfMeshToVTK (rwg_surf_mesh.h:541-660); cell_normals [n][3]
+ * are the cells' CellFill normals (bielcc_mesh_tables). */
+int bielcc_rwg_write_vtk(bielcc_rwg r, const double* cell_normals, const char* path);
 
 /* ------------------------------------------------------------ drivers */
 

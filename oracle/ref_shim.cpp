@@ -35,6 +35,7 @@
 #include "currents.h"
 #include "incident_field.h"
 #include "num_param.h"
+#include "rwg_surf_mesh.h"
 
 using namespace bielcc;
 using cd = std::complex<double>;
@@ -528,5 +529,80 @@ int ref_write_aereco_field_c(const char* path_re, const char* path_im, const cha
     }
     return 0;
 }
+
+}  // extern "C"
+
+// RwgSurfMesh (mesh/rwg_surf_mesh.h:300-526) of a mesh file: the topology
+// arrays, flattened, and the reference's own VTK export of it.  counts[0..2] =
+// points, edges, neighbour entries; every array is caller-sized (cap entries).
+template <typename CellT>
+static int rwg_dump(const char* path, int cap, int* ncell, int* counts, double* points, int* cell_points,
+                    int* point_num_cells, int* point_num_edges, int* edges, int* edge_num_cells, int* cell_edges,
+                    int* cell_dir, double* edge_mid, double* edge_ort, int* nghb_off, int* nghb, const char* vtk) {
+    SurfMesh<CellT> base(path);
+    RwgSurfMesh<CellT> m(base);
+    const int n = m.GetNFrm(), np_ = m.GetNPoints(), ne = m.GetNEdges();
+    constexpr int nv = std::is_same<CellT, CellQuad>::value ? 4 : 3;
+    *ncell = n;
+    counts[0] = np_;
+    counts[1] = ne;
+    if (n * nv > cap || np_ > cap || ne > cap) return 1;
+    for (int i = 0; i < np_; i++) {
+        for (int d = 0; d < 3; d++) points[3 * i + d] = m.GetPoint(i)[d];
+        point_num_cells[i] = m.GetPointNumCells(i);
+        point_num_edges[i] = (int)m.GetPointEdges(i).size();
+    }
+    for (int c = 0; c < n; c++)
+        for (int v = 0; v < nv; v++) {
+            cell_points[c * nv + v] = m.GetCellPoints(c)[v];
+            cell_edges[c * nv + v] = m.GetCellEdge(c, v);
+            cell_dir[c * nv + v] = m.GetCellEdgeDirIndx(c, v);
+        }
+    for (int e = 0; e < ne; e++) {
+        edges[2 * e] = (int)m.GetEdge(e)[0];
+        edges[2 * e + 1] = (int)m.GetEdge(e)[1];
+        edge_num_cells[e] = m.GetEdgeNumCells(e);
+        for (int d = 0; d < 3; d++) {
+            edge_mid[3 * e + d] = m.GetEdgeMid(e)[d];
+            edge_ort[3 * e + d] = m.GetEdgeOrt(e)[d];
+        }
+    }
+    int k = 0;
+    nghb_off[0] = 0;
+    for (int c = 0; c < n; c++)
+        for (int v = 0; v < nv; v++) {
+            for (int x : m.GetCellNghbsByEdge(c, v)) {
+                if (k >= 4 * cap) return 1;
+                nghb[k++] = x;
+            }
+            nghb_off[c * nv + v + 1] = k;
+        }
+    counts[2] = k;
+    if (vtk && vtk[0]) m.ExportRwgfMeshToVTK(vtk);
+    return 0;
+}
+
+extern "C" {
+
+int ref_rwg(const char* path, int nv, int cap, int* ncell, int* counts, double* points, int* cell_points,
+            int* point_num_cells, int* point_num_edges, int* edges, int* edge_num_cells, int* cell_edges,
+            int* cell_dir, double* edge_mid, double* edge_ort, int* nghb_off, int* nghb, const char* vtk) {
+    try {
+        if (nv == 4)
+            return rwg_dump<CellQuad>(path, cap, ncell, counts, points, cell_points, point_num_cells, point_num_edges,
+                                      edges, edge_num_cells, cell_edges, cell_dir, edge_mid, edge_ort, nghb_off, nghb,
+                                      vtk);
+        if (nv == 3)
+            return rwg_dump<CellTriangle>(path, cap, ncell, counts, points, cell_points, point_num_cells,
+                                          point_num_edges, edges, edge_num_cells, cell_edges, cell_dir, edge_mid,
+                                          edge_ort, nghb_off, nghb, vtk);
+    } catch (...) {
+        return -2;
+    }
+    return -1;
+}
+
+// Ker_BasicAverFunc (electrodynamics/kernel_lib.h:626-634)
+double ref_basic_aver(const double* x, const double* y, double eps) { return Ker_BasicAverFunc(x, y, eps); }
 
 }  // extern "C"

@@ -9,6 +9,7 @@ parameters are Calculation_Constants (electrodynamics/constants.h:17-27).
 """
 from __future__ import annotations
 
+import gzip
 import os
 import sys
 
@@ -291,9 +292,93 @@ def gen_writers(ref):
     print("writers:", sorted(os.listdir(out)))
 
 
+RWG_MESHES = (("sphere_quad", 4), ("sphere_triangle", 3), ("pole_sphere", 4), ("open_face", 4),
+              ("torus_tri", 3))
+
+
+def rwg_inputs(tmp):
+    """Mesh files of the RwgSurfMesh cases: the reference's two example meshes,
+    plus written ones: a sphere with degenerate pole quads (pseudo-edges), one
+    open cube face (boundary edges) and a triangulated torus."""
+    paths = {"sphere_quad": os.path.join(REF_DATA, "sphere_quad.dat"),
+             "sphere_triangle": os.path.join(REF_DATA, "sphere_triangle.dat")}
+    extra = {"pole_sphere": mesh.sphere_quads(5, 10, jitter=0.005, seed=4),
+             "open_face": mesh.cube_quads(4)[:16],
+             "torus_tri": mesh.triangulate(mesh.torus_quads(12, 6, R=1.0, r=0.3))}
+    for name, v in extra.items():
+        paths[name] = os.path.join(tmp, name + ".dat")
+        mesh.write_dat(paths[name], v)
+    return paths
+
+
+def gen_rwg(ref):
+    """RwgSurfMesh topology arrays and the reference's VTK export of each case."""
+    import ctypes as C
+    import tempfile
+    ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    out, wdir = {}, os.path.join(HERE, "writers")
+    with tempfile.TemporaryDirectory() as tmp:
+        paths = rwg_inputs(tmp)
+        for name, nv in RWG_MESHES:
+            cap = 200000
+            ncell, counts = C.c_int(0), (C.c_int * 3)()
+            arrs = dict(points=np.zeros(3 * cap), cell_points=np.zeros(cap, np.int32),
+                        point_num_cells=np.zeros(cap, np.int32), point_num_edges=np.zeros(cap, np.int32),
+                        edges=np.zeros(2 * cap, np.int32), edge_num_cells=np.zeros(cap, np.int32),
+                        cell_edges=np.zeros(cap, np.int32), cell_dir=np.zeros(cap, np.int32),
+                        edge_mid=np.zeros(3 * cap), edge_ort=np.zeros(3 * cap),
+                        nghb_off=np.zeros(cap + 1, np.int32), nghb=np.zeros(4 * cap, np.int32))
+            ptr = lambda a: a.ctypes.data_as(dp if a.dtype == np.float64 else ip)
+            vtk = os.path.join(wdir, f"rwg_{name}.vtk")
+            rc = ref.lib.ref_rwg(paths[name].encode(), nv, cap, C.byref(ncell), counts,
+                                 *[ptr(arrs[k]) for k in ("points", "cell_points", "point_num_cells",
+                                                          "point_num_edges", "edges", "edge_num_cells",
+                                                          "cell_edges", "cell_dir", "edge_mid", "edge_ort",
+                                                          "nghb_off", "nghb")], vtk.encode())
+            assert rc == 0, (name, rc)
+            with open(vtk, "rb") as fi, gzip.GzipFile(vtk + ".gz", "wb", mtime=0) as fo:
+                fo.write(fi.read())
+            os.remove(vtk)
+            n, npt, ne, ng = ncell.value, counts[0], counts[1], counts[2]
+            d = dict(points=arrs["points"][:3 * npt].reshape(npt, 3),
+                     cell_points=arrs["cell_points"][:n * nv].reshape(n, nv),
+                     point_num_cells=arrs["point_num_cells"][:npt], point_num_edges=arrs["point_num_edges"][:npt],
+                     edges=arrs["edges"][:2 * ne].reshape(ne, 2), edge_num_cells=arrs["edge_num_cells"][:ne],
+                     cell_edges=arrs["cell_edges"][:n * nv].reshape(n, nv),
+                     cell_dir=arrs["cell_dir"][:n * nv].reshape(n, nv),
+                     edge_mid=arrs["edge_mid"][:3 * ne].reshape(ne, 3), edge_ort=arrs["edge_ort"][:3 * ne].reshape(ne, 3),
+                     nghb_off=arrs["nghb_off"][:n * nv + 1], nghb=arrs["nghb"][:ng])
+            rd = ref.ref_read_mesh(paths[name], nv)
+            d["verts"], d["norm"] = np.asarray(rd["verts"]), np.asarray(rd["norm"])
+            for k, v in d.items():
+                out[f"{name}_{k}"] = v
+            print("rwg", name, "cells", n, "points", npt, "edges", ne, "neighbour entries", ng)
+    np.savez_compressed(os.path.join(HERE, "rwg.npz"), **out)
+
+
+def gen_aver(ref):
+    """Ker_BasicAverFunc (kernel_lib.h:626-634) at seeded pairs inside and outside eps."""
+    import ctypes as C
+    rng = np.random.default_rng(71)
+    dp = C.POINTER(C.c_double)
+    ref.lib.ref_basic_aver.restype = C.c_double
+    ref.lib.ref_basic_aver.argtypes = [dp, dp, C.c_double]
+    x = rng.normal(size=(400, 3))
+    eps = rng.uniform(0.05, 2.0, 400)
+    y = x + rng.normal(size=(400, 3)) * eps[:, None] * rng.uniform(0.1, 1.0, (400, 1))
+    y[0] = x[0]                                             # r = 0
+    y[1] = x[1] + np.array([eps[1], 0.0, 0.0])              # r = eps exactly
+    res = np.array([ref.lib.ref_basic_aver(np.ascontiguousarray(x[i]).ctypes.data_as(dp),
+                                           np.ascontiguousarray(y[i]).ctypes.data_as(dp), float(eps[i]))
+                    for i in range(400)])
+    np.savez_compressed(os.path.join(HERE, "aver.npz"), x=x, y=y, eps=eps, basic=res)
+    print("aver: inside eps", int((res > 0).sum()), "of", len(res))
+
+
 if __name__ == "__main__":
     ref = O.Checker("ref")
-    which = sys.argv[1:] or ["ops", "integrators", "kernels", "config1", "config1_tri", "mesh_files", "writers"]
+    which = sys.argv[1:] or ["ops", "integrators", "kernels", "config1", "config1_tri", "mesh_files", "writers",
+                             "rwg", "aver"]
     if "ops" in which:
         gen_ops(ref)
     if "integrators" in which:
@@ -308,3 +393,7 @@ if __name__ == "__main__":
         gen_mesh_files(ref)
     if "writers" in which:
         gen_writers(ref)
+    if "rwg" in which:
+        gen_rwg(ref)
+    if "aver" in which:
+        gen_aver(ref)

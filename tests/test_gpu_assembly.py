@@ -210,3 +210,22 @@ def test_fused_panel_matches_two_launch_panel(ctx, n):
     ctx.set_tuning("lu_fused_panel", 2)
     for a, b in zip(out[1], out[0]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [2500, 6500])
+def test_lu_residual_at_panel_sizes(ctx, n):
+    """FP64 and FP32 LU at orders where the panel kernels run hundreds of
+    workgroups (2,500: fused panel steps; 6,500: two launches per column):
+    the FP64 solve's residual, and FP32 factors refined to 1e-12."""
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2 * np.sqrt(n) * np.eye(n)
+    b = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    X = ctx_solve(ctx, A, b)
+    assert np.linalg.norm(A @ X - b) / np.linalg.norm(b) <= 1e-13
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dB = ctx.to_device(np.ascontiguousarray(b.T))
+    dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.lu32_factor(dA, n, n, dF, n, piv)
+    it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 2, n, 1e-12, 10)
+    X = dB.to_numpy((2, n), np.complex128).T
+    assert it <= 4 and np.linalg.norm(A @ X - b) / np.linalg.norm(b) <= 1e-12
