@@ -667,6 +667,7 @@ def batched_leg(tune_cache=None, workers=4, solve=True):
         piv.free()
     for i, m in wmesh.items():
         m.close()
+    panel_aborts = sum(max(c.get_tuning("lu_panel_aborts"), 0) for c in wctx)
     for c in wctx:
         c.close()
 
@@ -725,7 +726,7 @@ def batched_leg(tune_cache=None, workers=4, solve=True):
            "field_plan_speedup_vs_passes": round(t_field_driver / t_field, 2),
            "assembly_pairs_per_s": pairs_asm / t_asm, "field_pair_sets_per_s": pairs_fld * nd / t_field,
            "total_s": round(total32, 3), "solves": solves, "solves_per_s": round(solves / total32, 2),
-           "workers": workers,
+           "workers": workers, "lu_panel_aborts": panel_aborts,
            "fp64_form": {"lu64_and_solve64_s": round(t_lu64, 3), "field64_s": round(t_field64, 4),
                          "total_s": round(total64, 3), "solves_per_s": round(solves / total64, 2)},
            "solves_per_s_ratio_vs_fp64": round(total64 / total32, 3),
@@ -770,6 +771,9 @@ def main():
     ap.add_argument("--batched-no-solve", action="store_true",
                     help="batched form: plan kernels only, on synthetic currents (counter runs)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--lu-panel", type=int, default=1, choices=(0, 1),
+                    help="LU panels as one whole-panel launch (1) or per-column launches (0), every context")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path is run on one MI355X; multi-GPU sharding is a design "
@@ -777,6 +781,7 @@ def main():
         sys.exit(2)
 
     import bielcc_amd as B
+    B.Context.default_tuning["lu_panel_lds"] = args.lu_panel
     if args.copy_calib:
         copy_calibration()
     if args.batched_only:
@@ -785,6 +790,9 @@ def main():
         return
     if args.cube_only:
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps({"serving": serving_leg(nq=args.serve_queries)}), flush=True)
         return
     w = workload(args.config)
     verts, k, pts = w["verts"], w["k"], w["pts"]

@@ -116,7 +116,12 @@ def _vec3(v):
 
 
 class Context:
-    """A bielcc_ctx: one HIP device, one stream."""
+    """A bielcc_ctx: one HIP device, one stream.
+
+    ``Context.default_tuning`` (key -> value) is applied to every context
+    created afterwards in this process, e.g. by a server's workers."""
+
+    default_tuning: dict = {}
 
     def __init__(self, device: int = 0):
         self.lib = _lib.load()
@@ -125,6 +130,8 @@ class Context:
         if rc != 0:
             raise BielccError(rc, f"bielcc_ctx_create(device={device}) failed (no HIP device?)")
         self.h = h
+        for k, v in Context.default_tuning.items():
+            self.set_tuning(k, v)
 
     def close(self):
         if self.h is not None:

@@ -213,6 +213,15 @@ int lu_get_spin_limit(const LuWork*);
 // 0 never (two launches), 2 for small orders only
 void lu_set_fused_panel(LuWork*, int mode);
 int lu_get_fused_panel(const LuWork*);
+// LU look-ahead: next panel factored on a side stream during the trailing update
+void lu_set_look_ahead(LuWork*, int mode);
+int lu_get_look_ahead(const LuWork*);
+// whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
+void lu_set_panel_lds(LuWork*, bool on);
+bool lu_get_panel_lds(const LuWork*);
+void lu_set_panel_wait_us(LuWork*, int us);
+int lu_get_panel_wait_us(const LuWork*);
+int lu_panel_aborts(LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);

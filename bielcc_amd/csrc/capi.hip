@@ -589,6 +589,21 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_fused_panel(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "lu_panel_lds") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_lds must be 0 or 1");
+        lu_set_panel_lds(ctx->lu, value == 1);
+        return BIELCC_OK;
+    }
+    if (k == "lu_panel_wait_us") {
+        if (value < 0) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_wait_us must be >= 0");
+        lu_set_panel_wait_us(ctx->lu, value);
+        return BIELCC_OK;
+    }
+    if (k == "lu_look_ahead") {
+        if (value < 0 || value > 2) return fail(ctx, BIELCC_ERR_ARG, "lu_look_ahead must be 0, 1 or 2");
+        lu_set_look_ahead(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "trsm_lib") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "trsm_lib must be 0 or 1");
         lu_set_lib_trsm(ctx->lu, value == 1);
@@ -627,6 +642,10 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "trsm_lib") *value = lu_get_lib_trsm(ctx->lu) ? 1 : 0;
     else if (k == "trsv_spin") *value = lu_get_spin_limit(ctx->lu);
     else if (k == "lu_fused_panel") *value = lu_get_fused_panel(ctx->lu);
+    else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
+    else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu) ? 1 : 0;
+    else if (k == "lu_panel_wait_us") *value = lu_get_panel_wait_us(ctx->lu);
+    else if (k == "lu_panel_aborts") *value = lu_panel_aborts(ctx->lu);
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;
     else if (k == "plan_asm_cells") *value = ctx->plan_asm_cells;
     else if (k == "plan_wg_per_cu") *value = ctx->plan_wg_per_cu;

@@ -25,6 +25,7 @@
 
 #include <rocblas/rocblas.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -38,6 +39,11 @@ constexpr int kPR = 32;   // panel rows per workgroup of the column kernels (x 8
 // Up to this order the fused panel column (one launch instead of two) is faster: launches
 // dominate there; above it the fused step's serial last workgroup costs more than a launch.
 constexpr int kFusedMaxN = 6144;
+// Look-ahead pays only where the trailing GEMM is long and the first panels are too tall for
+// the whole-panel kernel: FP64 above this order (n = 28,000: -3%; n <= 12,000 and FP32: slower).
+constexpr int kLookAheadMinN = 20000;
+// candidate buffers of the whole-panel kernel: cval, cidx [2][256], crow [2][256][64], rowj [2][64] (FP64 sizes)
+constexpr size_t kPanelBufBytes = 2 * 256 * (8 + 4) + 2 * 256 * 64 * 16 + 2 * 64 * 16;
 
 template <typename R> struct V2;
 template <> struct V2<double> { using T = double2; };
@@ -76,6 +82,16 @@ struct LuWork {
     int spin_limit = 1 << 20;       // polls per sync-free wait before it gives up (tests lower it)
     unsigned* step_count = nullptr; // blocks done in the current fused panel step (reset by its last block)
     int fused_panel = 2;            // one launch per panel column: 1 always, 0 never, 2 for n <= kFusedMaxN
+    int look_ahead = 2;             // next panel on the side stream during the trailing update: 1 always, 0 never,
+                                    // 2 for FP64 above kLookAheadMinN
+    // whole-panel kernel (k_panel_lds): barrier words, published candidates, settings
+    unsigned* psync = nullptr;      // [3]: barrier word, workgroups gone, aborted panels
+    void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
+    bool panel_lds = true;
+    int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
+    int dev = 0, n_cu = 0, clk_khz = 0;
+    hipStream_t side = nullptr;     // side stream (default priority) and its two events
+    hipEvent_t ev_upd = nullptr, ev_panel = nullptr;
 };
 
 LuWork* lu_work_create() { return new LuWork(); }
@@ -85,9 +101,24 @@ bool lu_get_trsv(const LuWork* w) { return !w->no_trsv; }
 void lu_set_lib_trsm(LuWork* w, bool on) { w->lib_trsm = on; }
 void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
 void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
+void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
+void lu_set_panel_lds(LuWork* w, bool on) { w->panel_lds = on; }
+bool lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
+void lu_set_panel_wait_us(LuWork* w, int us) { w->panel_wait_us = us; }
+int lu_get_panel_wait_us(const LuWork* w) { return w->panel_wait_us; }
+int lu_get_look_ahead(const LuWork* w) { return w->look_ahead; }
 int lu_get_fused_panel(const LuWork* w) { return w->fused_panel; }
 int lu_get_spin_limit(const LuWork* w) { return w->spin_limit; }
 bool lu_get_lib_trsm(const LuWork* w) { return w->lib_trsm; }
+
+// whole panels that aborted to the one-workgroup path so far (-1 before the first LU)
+int lu_panel_aborts(LuWork* w) {
+    if (!w->psync) return -1;
+    unsigned h[4] = {0, 0, 0, 0};
+    if (hipDeviceSynchronize() != hipSuccess) return -1;
+    if (hipMemcpy(h, w->psync, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return (int)h[2];
+}
 
 // a sync-free triangular solve whose bounded wait expired (then cleared)
 static hipError_t trsv_check(hipStream_t s, LuWork* w, const char** msg) {
@@ -109,6 +140,10 @@ void lu_work_destroy(LuWork* w) {
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
     (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->step_count);
+    (void)hipFree(w->psync); (void)hipFree(w->pbuf);
+    if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
+    if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
+    if (w->side) (void)hipStreamDestroy(w->side);
     (void)hipFree(w->flags); (void)hipFree(w->ticket); (void)hipFree(w->err);
     delete w;
 }
@@ -134,6 +169,21 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
         if ((e = hipMalloc(&w->step_count, sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipMemset(w->step_count, 0, sizeof(unsigned))) != hipSuccess) return e;
     }
+    if (!w->psync) {
+        if ((e = hipMalloc(&w->psync, 4 * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMemset(w->psync, 0, 4 * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMalloc(&w->pbuf, kPanelBufBytes)) != hipSuccess) return e;
+        if ((e = hipGetDevice(&w->dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&w->n_cu, hipDeviceAttributeMultiprocessorCount, w->dev)) != hipSuccess)
+            return e;
+        if ((e = hipDeviceGetAttribute(&w->clk_khz, hipDeviceAttributeWallClockRate, w->dev)) != hipSuccess)
+            return e;
+    }
+    if (!w->side) {
+        if ((e = hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking)) != hipSuccess) return e;
+        if ((e = hipEventCreateWithFlags(&w->ev_upd, hipEventDisableTiming)) != hipSuccess) return e;
+        if ((e = hipEventCreateWithFlags(&w->ev_panel, hipEventDisableTiming)) != hipSuccess) return e;
+    }
     if (!w->blas) {
         if (rocblas_create_handle(&w->blas) != rocblas_status_success) {
             *msg = "rocblas_create_handle failed";
@@ -145,6 +195,16 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
 
 template <typename R>
 __device__ inline double cabs1(c2<R> v) { return fabs((double)v.x) + fabs((double)v.y); }
+
+// Complex product and squared modulus with the rounding written out (explicit
+// FMAs: the compiler may otherwise contract either product of a sum), so every
+// panel kernel produces the same bits.
+template <typename R>
+__device__ inline c2<R> cmul_fx(c2<R> a, c2<R> b) {
+    return mk2<R>(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+template <typename R>
+__device__ inline R norm2_fx(c2<R> a) { return fma(a.x, a.x, a.y * a.y); }
 
 // Phase 1: partial argmax of |re|+|im| over rows j..n-1 of column j.
 template <typename R>
@@ -226,7 +286,7 @@ __global__ __launch_bounds__(256) void k_piv_final(int nparts, int j, int c0, in
     __syncthreads();
     if (threadIdx.x == 0) {
         c2<R> a = A[(size_t)j * lda + j];
-        R den = a.x * a.x + a.y * a.y;
+        R den = norm2_fx<R>(a);
         if (den > R(0)) { pinv[0] = a.x / den; pinv[1] = -a.y / den; pinv[2] = R(1); }
         else { pinv[0] = R(0); pinv[1] = R(0); pinv[2] = R(0); }
     }
@@ -259,16 +319,15 @@ __global__ __launch_bounds__(256) void k_panel_update(int n, int j, int c1, c2<R
     __syncthreads();
     if (i < n) {
         if (pinv[2] != R(0)) {
-            c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
-            l = s;
+            l = cmul_fx<R>(l, mk2<R>(pinv[0], pinv[1]));
             if (cg == 0) A[(size_t)j * lda + i] = l;
         }
         for (int t = cg; t < nc; t += 256 / kPR) {
             c2<R>* p = A + (size_t)(j + 1 + t) * lda + i;
-            c2<R> u = urow[t];
+            const c2<R> lu = cmul_fx<R>(l, urow[t]);
             c2<R> a = *p;
-            a.x -= l.x * u.x - l.y * u.y;
-            a.y -= l.x * u.y + l.y * u.x;
+            a.x -= lu.x;
+            a.y -= lu.y;
             *p = a;
             if (t == 0) { best = cabs1<R>(a); bi = i; }
         }
@@ -320,16 +379,15 @@ __global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1
     __syncthreads();
     if (i < n) {
         if (pinv[2] != R(0)) {
-            c2<R> s = mk2<R>(l.x * pinv[0] - l.y * pinv[1], l.x * pinv[1] + l.y * pinv[0]);
-            l = s;
+            l = cmul_fx<R>(l, mk2<R>(pinv[0], pinv[1]));
             if (cg == 0) A[(size_t)j * lda + i] = l;
         }
         for (int t = cg; t < nc; t += 256 / kPR) {
             c2<R>* p = A + (size_t)(j + 1 + t) * lda + i;
-            c2<R> u = urow[t];
+            const c2<R> lu = cmul_fx<R>(l, urow[t]);
             c2<R> a = *p;
-            a.x -= l.x * u.x - l.y * u.y;
-            a.y -= l.x * u.y + l.y * u.x;
+            a.x -= lu.x;
+            a.y -= lu.y;
             *p = a;
             if (t == 0) { best = cabs1<R>(a); bi = i; }
         }
@@ -400,9 +458,289 @@ __global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1
     __syncthreads();
     if (threadIdx.x == 0) {
         const c2<R> a = A[(size_t)jn * lda + jn];
-        const R den = a.x * a.x + a.y * a.y;
+        const R den = norm2_fx<R>(a);
         if (den > R(0)) { pinv[0] = a.x / den; pinv[1] = -a.y / den; pinv[2] = R(1); }
         else { pinv[0] = R(0); pinv[1] = R(0); pinv[2] = R(0); }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Whole panel in one launch.  The panel's m = n - k0 rows are split over G
+// workgroups of RPW rows; each keeps its rows of the panel columns in LDS for
+// the whole panel.  Per column: every workgroup publishes its pivot candidate
+// (|re|+|im|, row, the row's panel entries) and the owner of the diagonal row
+// publishes that row; one grid barrier; every workgroup then picks the same
+// winner (largest value, first row on ties), applies the interchange to the
+// rows it holds, scales and updates them.  Global A is written only after the
+// last barrier.
+//
+// The barrier wait is bounded by the wall clock.  A workgroup whose wait
+// expires sets the abort bit of the barrier word by compare-and-swap, and only
+// while that barrier is incomplete, so all workgroups reach the same verdict;
+// an aborted panel has written nothing to A, and the last workgroup to leave
+// factors it alone from global memory.  That workgroup also resets the words.
+constexpr unsigned kAbortBit = 0x80000000u;
+constexpr int kPanelMaxWG = 256;
+
+template <typename R> struct PanelRows;
+template <> struct PanelRows<double> { static constexpr int v = 64; };    // 64 KB of LDS
+template <> struct PanelRows<float> { static constexpr int v = 128; };    // 64 KB of LDS
+
+template <typename R>
+__device__ inline c2<R> ld_agent2(const c2<R>* p) {
+    const R* q = reinterpret_cast<const R*>(p);
+    return mk2<R>(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                  __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+template <typename T>
+__device__ inline void st_agent(T* p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename R>
+__device__ inline void st_agent2(c2<R>* p, c2<R> v) {
+    R* q = reinterpret_cast<R*>(p);
+    st_agent<R>(q, v.x);
+    st_agent<R>(q + 1, v.y);
+}
+
+// One lane per workgroup.  word = arrivals (cumulative over the panel's
+// barriers) | abort bit; barrier b completes at G * (b + 1) arrivals.
+// No cache-wide fences: everything one workgroup passes to another goes
+// through device-scope atomic stores and loads, and every wave waits for its
+// stores to complete before the workgroup barrier that precedes the arrival.
+// (An agent fence would write back and invalidate the whole L2 of the XCD at
+// every column, in every workgroup, and slow whatever else runs there.)
+__device__ inline bool grid_barrier(unsigned* word, unsigned target, long long wait_ticks) {
+    const unsigned old = atomicAdd(word, 1u);
+    if (old & kAbortBit) return false;
+    if ((old & ~kAbortBit) + 1u >= target) return true;
+    const long long t0 = wall_clock64();
+    unsigned v;
+    for (;;) {
+        v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v & kAbortBit) return false;
+        if ((v & ~kAbortBit) >= target) return true;
+        if (wall_clock64() - t0 > wait_ticks) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    for (;;) {   // expired: abort, unless the barrier completed meanwhile
+        if (v & kAbortBit) return false;
+        if ((v & ~kAbortBit) >= target) return true;
+        const unsigned prev = atomicCAS(word, v, v | kAbortBit);
+        if (prev == v) return false;
+        v = prev;   // the word moved (an arrival or another abort): decide again
+    }
+}
+
+__device__ inline void argmax_merge(double& v, int& i, double ov, int oi) {
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+
+// The panel by one workgroup from global memory: the classic column loop
+// (argmax, interchange inside the panel, reciprocal pivot, scale, rank-1
+// update), same arithmetic as k_piv_final + k_panel_update.
+template <typename R>
+__device__ void panel_one_wg(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda, int* __restrict__ piv,
+                             int* __restrict__ info) {
+    __shared__ double fv[256];
+    __shared__ int fi[256];
+    __shared__ c2<R> fu[kNB];
+    const int tid = threadIdx.x, c1 = k0 + nb;
+    for (int j = k0; j < c1; j++) {
+        double best = -1.;
+        int bi = 0x7fffffff;
+        for (int i = j + tid; i < n; i += 256) argmax_merge(best, bi, cabs1<R>(A[(size_t)j * lda + i]), i);
+        fv[tid] = best;
+        fi[tid] = bi;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (tid < off) {
+                double v = fv[tid];
+                int ix = fi[tid];
+                argmax_merge(v, ix, fv[tid + off], fi[tid + off]);
+                fv[tid] = v;
+                fi[tid] = ix;
+            }
+            __syncthreads();
+        }
+        const int p = (fi[0] == 0x7fffffff) ? j : fi[0];
+        if (tid == 0) {
+            piv[j] = p;
+            if (!(fv[0] > 0.) && *info == 0) *info = j + 1;
+        }
+        if (p != j)
+            for (int c = k0 + tid; c < c1; c += 256) {
+                c2<R>* col = A + (size_t)c * lda;
+                const c2<R> t = col[j];
+                col[j] = col[p];
+                col[p] = t;
+            }
+        __syncthreads();
+        const c2<R> a = A[(size_t)j * lda + j];
+        const R den = norm2_fx<R>(a);
+        const bool sc = den > R(0);
+        const R q0 = sc ? a.x / den : R(0), q1 = sc ? -a.y / den : R(0);
+        const int nc = c1 - j - 1;
+        for (int t = tid; t < nc; t += 256) fu[t] = A[(size_t)(j + 1 + t) * lda + j];
+        __syncthreads();
+        for (int i = j + 1 + tid; i < n; i += 256) {
+            c2<R> l = A[(size_t)j * lda + i];
+            if (sc) {
+                l = cmul_fx<R>(l, mk2<R>(q0, q1));
+                A[(size_t)j * lda + i] = l;
+            }
+            for (int t = 0; t < nc; t++) {
+                c2<R>* pa = A + (size_t)(j + 1 + t) * lda + i;
+                const c2<R> lu = cmul_fx<R>(l, fu[t]);
+                c2<R> x = *pa;
+                x.x -= lu.x;
+                x.y -= lu.y;
+                *pa = x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// sync[0] barrier word, sync[1] workgroups gone, sync[2] aborted panels (count).
+// cval/cidx [2][kPanelMaxWG], crow [2][kPanelMaxWG][kNB], rowj [2][kNB]:
+// double-buffered by column parity (a workgroup can be at most one barrier ahead).
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
+                                                   int* __restrict__ piv, int* __restrict__ info,
+                                                   unsigned* __restrict__ sync, double* __restrict__ cval,
+                                                   int* __restrict__ cidx, c2<R>* __restrict__ crow,
+                                                   c2<R>* __restrict__ rowj, long long wait_ticks) {
+    constexpr int RPW = PanelRows<R>::v;
+    constexpr int RPL = RPW / 64;   // rows per lane
+    constexpr int NONE = 0x7fffffff;
+    __shared__ c2<R> sa[kNB][RPW];  // [column][row]
+    __shared__ c2<R> urow[kNB];
+    __shared__ double s_v4[4];
+    __shared__ int s_i4[4];
+    __shared__ double s_cv;
+    __shared__ int s_cr, s_ok, s_last;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int m = n - k0, r0 = g * RPW, nr = min(RPW, m - r0);
+    for (int e = tid; e < nb * RPW; e += 256) {
+        const int c = e / RPW, r = e % RPW;
+        if (r < nr) sa[c][r] = A[(size_t)(k0 + c) * lda + k0 + r0 + r];
+    }
+    __syncthreads();
+    // this workgroup's candidate for column c over its rows at or below the diagonal (wave c % 4)
+    auto candidate = [&](int c) {
+        double best = -1.;
+        int bi = NONE;
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+            const int r = lane + 64 * q;
+            if (r < nr && r0 + r >= c) {
+                const double v = cabs1<R>(sa[c][r]);
+                if (v > best) { best = v; bi = r; }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) argmax_merge(best, bi, __shfl_xor(best, off, 64), __shfl_xor(bi, off, 64));
+        if (lane == 0) { s_cv = best; s_cr = bi; }
+    };
+    if (w == 0) candidate(0);
+    __syncthreads();
+    bool ok = true;
+    for (int jj = 0; jj < nb; jj++) {
+        const int b = jj & 1, j = k0 + jj;
+        // publish
+        const int cr = s_cr;
+        const int pr = (cr != NONE) ? cr : (g == 0 ? jj : 0);
+        if (tid == 0) {
+            st_agent<double>(&cval[b * kPanelMaxWG + g], s_cv);
+            st_agent<int>(&cidx[b * kPanelMaxWG + g], (cr != NONE) ? k0 + r0 + cr : NONE);
+        }
+        if (tid < nb) {
+            st_agent2<R>(&crow[((size_t)b * kPanelMaxWG + g) * kNB + tid], sa[tid][pr]);
+            if (g == 0) st_agent2<R>(&rowj[b * kNB + tid], sa[tid][jj]);
+        }
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+        __syncthreads();
+        if (tid == 0) s_ok = grid_barrier(&sync[0], (unsigned)G * (unsigned)(jj + 1), wait_ticks);
+        __syncthreads();
+        if (!s_ok) { ok = false; break; }
+        // the winner, identically in every workgroup
+        double best = -1.;
+        int bi = NONE;
+        if (tid < G) {
+            best = __hip_atomic_load(&cval[b * kPanelMaxWG + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bi = __hip_atomic_load(&cidx[b * kPanelMaxWG + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (int off = 32; off > 0; off >>= 1) argmax_merge(best, bi, __shfl_xor(best, off, 64), __shfl_xor(bi, off, 64));
+        if (lane == 0) { s_v4[w] = best; s_i4[w] = bi; }
+        __syncthreads();
+        best = s_v4[0];
+        bi = s_i4[0];
+        for (int q = 1; q < 4; q++) argmax_merge(best, bi, s_v4[q], s_i4[q]);
+        const int p = (bi == NONE) ? j : bi;
+        const int gw = (p - k0) / RPW;
+        if (g == 0 && tid == 0) {
+            piv[j] = p;
+            if (!(best > 0.) && *info == 0) *info = j + 1;
+        }
+        // interchange rows j and p: row j lives in workgroup 0, row p in workgroup gw
+        if (tid < nb) {
+            const c2<R> u = ld_agent2<R>(&crow[((size_t)b * kPanelMaxWG + gw) * kNB + tid]);
+            urow[tid] = u;
+            if (p != j) {
+                if (g == gw) sa[tid][p - k0 - r0] = (g == 0) ? sa[tid][jj] : ld_agent2<R>(&rowj[b * kNB + tid]);
+                if (g == 0) sa[tid][jj] = u;
+            }
+        }
+        __syncthreads();
+        const c2<R> a = urow[jj];
+        const R den = norm2_fx<R>(a);
+        const bool sc = den > R(0);
+        const R q0 = sc ? a.x / den : R(0), q1 = sc ? -a.y / den : R(0);
+        // scale column jj below the diagonal (its wave), then the rank-1 update of the later columns
+        if (w == (jj & 3) && sc) {
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+                const int r = lane + 64 * q;
+                if (r < nr && r0 + r > jj) sa[jj][r] = cmul_fx<R>(sa[jj][r], mk2<R>(q0, q1));
+            }
+        }
+        __syncthreads();
+        const int cfirst = jj + 1 + ((w - (jj + 1)) & 3);   // first column > jj of this wave (c % 4 == w)
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+            const int r = lane + 64 * q;
+            if (r < nr && r0 + r > jj) {
+                const c2<R> l = sa[jj][r];
+                for (int c = cfirst; c < nb; c += 4) {
+                    const c2<R> lu = cmul_fx<R>(l, urow[c]);
+                    c2<R> x = sa[c][r];
+                    x.x -= lu.x;
+                    x.y -= lu.y;
+                    sa[c][r] = x;
+                }
+            }
+        }
+        if (jj + 1 < nb && w == ((jj + 1) & 3)) candidate(jj + 1);
+        __syncthreads();
+    }
+    if (ok)
+        for (int e = tid; e < nb * RPW; e += 256) {
+            const int c = e / RPW, r = e % RPW;
+            if (r < nr) A[(size_t)(k0 + c) * lda + k0 + r0 + r] = sa[c][r];
+        }
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&sync[1], 1u) == (unsigned)G - 1u;
+    __syncthreads();
+    if (!s_last) return;
+    // every workgroup is gone: the verdict is final
+    if (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kAbortBit) {
+        panel_one_wg<R>(n, k0, nb, A, lda, piv, info);
+        if (tid == 0) atomicAdd(&sync[2], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sync[0] = 0u;
+        sync[1] = 0u;
+        __threadfence();
     }
 }
 
@@ -528,6 +866,39 @@ static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n
     return hipGetLastError();
 }
 
+// One LU per device at a time may use the whole-panel kernel (its grid barrier needs all
+// its workgroups resident at once), and only while no other LU runs in the process: its
+// workgroups hold 70 KB of LDS on up to every compute unit for a whole panel, which halves
+// the occupancy of the GEMMs of LUs on other streams (measured: four concurrent FP64 LUs 6%
+// slower with it).  The per-column path's kernels never wait on other workgroups.
+static std::atomic<int> g_panel_busy[64];
+static std::atomic<int> g_lu_running[64];
+
+struct LuRunning {   // counts this LU among those running on the device
+    std::atomic<int>* c = nullptr;
+    explicit LuRunning(int dev) {
+        if (dev >= 0 && dev < 64) (c = &g_lu_running[dev])->fetch_add(1);
+    }
+    bool alone() const { return c && c->load() == 1; }
+    ~LuRunning() {
+        if (c) c->fetch_sub(1);
+    }
+};
+
+struct PanelAdmit {
+    std::atomic<int>* flag = nullptr;
+    bool take(int dev) {
+        if (dev < 0 || dev >= 64) return false;
+        int expect = 0;
+        if (!g_panel_busy[dev].compare_exchange_strong(expect, 1)) return false;
+        flag = &g_panel_busy[dev];
+        return true;
+    }
+    ~PanelAdmit() {
+        if (flag) flag->store(0);
+    }
+};
+
 template <typename R>
 static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long long lda, int* piv, int* info,
                               const char** msg) {
@@ -536,44 +907,95 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     R* pinv = reinterpret_cast<R*>(w->pinv);
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
     const bool fused = w->fused_panel == 1 || (w->fused_panel == 2 && n <= kFusedMaxN);
-    for (int k0 = 0; k0 < n; k0 += kNB) {
+    // whole-panel kernel for panels of at most kPanelMaxWG workgroups (decided per panel:
+    // once another LU starts, the rest of this one's panels take the per-column path)
+    constexpr int RPW = PanelRows<R>::v;
+    LuRunning running(w->dev);
+    PanelAdmit admit;
+    const bool lds = w->panel_lds && w->n_cu > 0 && admit.take(w->dev);
+    const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
+    char* pb = static_cast<char*>(w->pbuf);
+    double* cval = reinterpret_cast<double*>(pb);
+    int* cidx = reinterpret_cast<int*>(pb + 2 * 256 * 8);
+    c2<R>* crow = reinterpret_cast<c2<R>*>(pb + 2 * 256 * 12);
+    c2<R>* rowj = crow + 2 * kPanelMaxWG * kNB;
+    // panel [k0, k0 + nb) on stream st: pivot, interchange inside the panel, scale,
+    // rank-1 update of the panel's remaining columns
+    auto factor_panel = [&](hipStream_t st, int k0) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
+        const int G = (n - k0 + RPW - 1) / RPW;
+        if (lds && G <= kPanelMaxWG && running.alone()) {
+            hipLaunchKernelGGL(k_panel_lds<R>, dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info, w->psync,
+                               cval, cidx, crow, rowj, wait_ticks);
+            return;
+        }
         int nparts = 0;
         for (int jj = 0; jj < nb; jj++) {
             const int j = k0 + jj;
             const int m = n - j;
             if (jj == 0) {   // first panel column: fresh search (later ones come from the column before)
                 nparts = min(kPivBlocks, (m + 255) / 256);
-                hipLaunchKernelGGL(k_piv_partial<R>, dim3(nparts), dim3(256), 0, s, n, j, A, lda, w->pval, w->pidx);
-                hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval,
+                hipLaunchKernelGGL(k_piv_partial<R>, dim3(nparts), dim3(256), 0, st, n, j, A, lda, w->pval, w->pidx);
+                hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, st, nparts, j, k0, c1, A, lda, w->pval,
                                    w->pidx, piv, pinv, w->info);
             } else if (!fused) {
-                hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, s, nparts, j, k0, c1, A, lda, w->pval,
+                hipLaunchKernelGGL(k_piv_final<R>, dim3(1), dim3(256), 0, st, nparts, j, k0, c1, A, lda, w->pval,
                                    w->pidx, piv, pinv, w->info);
             }
             if (m > 1) {
                 nparts = (m - 1 + kPR - 1) / kPR;
                 if (fused && j + 1 < c1)   // this column's update and the next column's pivot
-                    hipLaunchKernelGGL(k_panel_step<R>, dim3(nparts), dim3(256), 0, s, n, j, k0, c1, A, lda, pinv,
+                    hipLaunchKernelGGL(k_panel_step<R>, dim3(nparts), dim3(256), 0, st, n, j, k0, c1, A, lda, pinv,
                                        w->pval, w->pidx, piv, w->info, w->step_count);
                 else
-                    hipLaunchKernelGGL(k_panel_update<R>, dim3(nparts), dim3(256), 0, s, n, j, c1, A, lda, pinv,
+                    hipLaunchKernelGGL(k_panel_update<R>, dim3(nparts), dim3(256), 0, st, n, j, c1, A, lda, pinv,
                                        w->pval, w->pidx);
             }
         }
+    };
+    // Look-ahead: after panel k0's interchanges and U12, the trailing update is split.
+    // The next panel's columns [c1, c2) are updated first; that panel is then factored on
+    // the side stream while the rest of the update, columns [c2, n), runs on s.  The two
+    // touch disjoint columns (the panel's interchanges stay inside its own columns until
+    // the next step's row swaps), and only the panel kernels use the pivot scratch.
+    const bool ahead = w->side != nullptr &&
+                       (w->look_ahead == 1 || (w->look_ahead == 2 && sizeof(R) == 8 && n > kLookAheadMinN));
+    factor_panel(s, 0);
+    for (int k0 = 0; k0 < n; k0 += kNB) {
+        const int nb = min(kNB, n - k0);
+        const int c1 = k0 + nb;
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (k0 > 0)
             hipLaunchKernelGGL(k_laswp<R>, dim3((k0 + 255) / 256), dim3(256), 0, s, k0, nb, 0, k0, A, lda, piv);
-        if (c1 < n) {
-            hipLaunchKernelGGL(k_laswp<R>, dim3((n - c1 + 255) / 256), dim3(256), 0, s, k0, nb, c1, n, A, lda, piv);
-            hipLaunchKernelGGL(k_trsm<R>, dim3((n - c1 + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0,
-                               A, lda, (long long)k0, c1, n);
-            e = gemm_update(w, s, n - c1, n - c1, nb, A + (size_t)k0 * lda + c1, lda, A + (size_t)c1 * lda + k0,
-                            lda, A + (size_t)c1 * lda + c1, lda, msg);
+        if (c1 >= n) break;
+        hipLaunchKernelGGL(k_laswp<R>, dim3((n - c1 + 255) / 256), dim3(256), 0, s, k0, nb, c1, n, A, lda, piv);
+        hipLaunchKernelGGL(k_trsm<R>, dim3((n - c1 + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, A, lda,
+                           (long long)k0, c1, n);
+        const c2<R>* L21 = A + (size_t)k0 * lda + c1;
+        const int c2n = min(n, c1 + kNB);
+        if (!ahead) {
+            e = gemm_update(w, s, n - c1, n - c1, nb, L21, lda, A + (size_t)c1 * lda + k0, lda,
+                            A + (size_t)c1 * lda + c1, lda, msg);
+            if (e != hipSuccess) return e;
+            factor_panel(s, c1);
+            continue;
+        }
+        e = gemm_update(w, s, n - c1, c2n - c1, nb, L21, lda, A + (size_t)c1 * lda + k0, lda,
+                        A + (size_t)c1 * lda + c1, lda, msg);
+        if (e != hipSuccess) return e;
+        if ((e = hipEventRecord(w->ev_upd, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(w->side, w->ev_upd, 0)) != hipSuccess) return e;
+        factor_panel(w->side, c1);
+        if ((e = hipEventRecord(w->ev_panel, w->side)) != hipSuccess) return e;
+        if (c2n < n) {
+            e = gemm_update(w, s, n - c1, n - c2n, nb, L21, lda, A + (size_t)c2n * lda + k0, lda,
+                            A + (size_t)c2n * lda + c1, lda, msg);
             if (e != hipSuccess) return e;
         }
+        if ((e = hipStreamWaitEvent(s, w->ev_panel, 0)) != hipSuccess) return e;
     }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(info, w->info, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     return hipStreamSynchronize(s);
 }

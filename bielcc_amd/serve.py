@@ -464,6 +464,7 @@ def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
            "factorisations": sum(r.factored for r in res.values()), "evictions": server.cache.evictions,
            "fp64_fallbacks": sum(r.fp64_fallback for r in res.values()),
            "fp64_upgrades": list(server.upgrades),
-           "refine_corrections_max": max((r.refine_iters for r in res.values()), default=0)}
+           "refine_corrections_max": max((r.refine_iters for r in res.values()), default=0),
+           "lu_panel_aborts": sum(max(c.get_tuning("lu_panel_aborts"), 0) for c in server.ctxs)}
     out.update(percentiles([r.latency for r in res.values()]))
     return {"results": res, "stats": out}

@@ -353,12 +353,22 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * [1048576] = polls per wait of the one-kernel solve before it gives up (a
  * solve whose wait gave up is redone on the blocked path; tests lower it);
  * "lu_fused_panel" [2] = one launch per LU panel column (1), two (0), or
- * one up to order 6,144 and two above (2).
+ * one up to order 6,144 and two above (2); "lu_look_ahead" [2] = the next
+ * LU panel is factored on a side stream during the trailing update: always
+ * (1), never (0), or for FP64 above order 20,000 (2); "lu_panel_lds" [1] =
+ * LU panels of at most 16,384 rows (FP64) or 32,768 (FP32) run as one launch
+ * that holds the panel in LDS with one grid barrier per column (0: the
+ * per-column launches); one LU per device at a time uses it, LUs running
+ * meanwhile take the per-column path; "lu_panel_wait_us"
+ * [250000] = bound of each of its barrier waits: an expired wait aborts the
+ * launch before it writes the matrix and one workgroup then factors the
+ * panel (slow, correct; tests set 0 to force it).
  * Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
  * bielcc_get_tuning also reads
- * "n_cu" (compute units).  Returns BIELCC_ERR_ARG for an unknown key or a
+ * "n_cu" (compute units) and "lu_panel_aborts" (whole-panel launches that
+ * aborted to one workgroup, for the context, since its first LU).  Returns BIELCC_ERR_ARG for an unknown key or a
  * value out of range. */
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);
 int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);
