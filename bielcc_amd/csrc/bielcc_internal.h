@@ -218,6 +218,9 @@ void lu_set_look_ahead(LuWork*, int mode);
 int lu_get_look_ahead(const LuWork*);
 // whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
 void lu_set_panel_lds(LuWork*, bool on);
+// rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
+void lu_set_blas_atomics(LuWork*, bool on);
+bool lu_get_blas_atomics(const LuWork*);
 bool lu_get_panel_lds(const LuWork*);
 void lu_set_panel_wait_us(LuWork*, int us);
 int lu_get_panel_wait_us(const LuWork*);

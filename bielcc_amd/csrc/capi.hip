@@ -589,6 +589,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_fused_panel(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "blas_atomics") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "blas_atomics must be 0 or 1");
+        lu_set_blas_atomics(ctx->lu, value == 1);
+        return BIELCC_OK;
+    }
     if (k == "lu_panel_lds") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_lds must be 0 or 1");
         lu_set_panel_lds(ctx->lu, value == 1);
@@ -644,6 +649,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_fused_panel") *value = lu_get_fused_panel(ctx->lu);
     else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu) ? 1 : 0;
+    else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_panel_wait_us") *value = lu_get_panel_wait_us(ctx->lu);
     else if (k == "lu_panel_aborts") *value = lu_panel_aborts(ctx->lu);
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;

@@ -90,6 +90,7 @@ struct LuWork {
     bool panel_lds = true;
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
     int dev = 0, n_cu = 0, clk_khz = 0;
+    bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
     hipStream_t side = nullptr;     // side stream (default priority) and its two events
     hipEvent_t ev_upd = nullptr, ev_panel = nullptr;
 };
@@ -103,6 +104,8 @@ void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
 void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
 void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, bool on) { w->panel_lds = on; }
+void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
+bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
 bool lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
 void lu_set_panel_wait_us(LuWork* w, int us) { w->panel_wait_us = us; }
 int lu_get_panel_wait_us(const LuWork* w) { return w->panel_wait_us; }
@@ -189,6 +192,13 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
             *msg = "rocblas_create_handle failed";
             return hipErrorUnknown;
         }
+    }
+    // reproducible GEMMs: no kernels that sum partial results with atomics (rocBLAS
+    // allows them by default, and their order varies from run to run)
+    if (rocblas_set_atomics_mode(w->blas, w->blas_atomics ? rocblas_atomics_allowed : rocblas_atomics_not_allowed) !=
+        rocblas_status_success) {
+        *msg = "rocblas_set_atomics_mode failed";
+        return hipErrorUnknown;
     }
     return hipSuccess;
 }

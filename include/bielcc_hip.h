@@ -362,7 +362,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * meanwhile take the per-column path; "lu_panel_wait_us"
  * [250000] = bound of each of its barrier waits: an expired wait aborts the
  * launch before it writes the matrix and one workgroup then factors the
- * panel (slow, correct; tests set 0 to force it).
+ * panel (slow, correct; tests set 0 to force it); "blas_atomics" [0] =
+ * rocBLAS may use kernels that sum partial results with atomics (1); by
+ * default it may not, so factors and solves are the same bits on every run.
  * Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.

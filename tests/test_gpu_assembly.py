@@ -335,3 +335,16 @@ def test_whole_panel_concurrent_lus():
         assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("n", [1000, 3000])
+def test_lu_reproducible(ctx, n):
+    """The same matrix factored three times gives the same bits (FP64 and FP32
+    factors, pivots): no GEMM with atomic partial sums, no order that depends
+    on timing in the panel, the laswp or the trailing update."""
+    rng = np.random.default_rng(14)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    runs = [_factors(ctx, A) for _ in range(3)]
+    for r in runs[1:]:
+        for x, y in zip(runs[0], r):
+            assert np.array_equal(x, y)
