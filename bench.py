@@ -408,6 +408,7 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10):
     qs = {q.qid: q for q in serving_queries(pool, nq)}
     errs = {"refined": [0.0, 0.0, 0], "fp64_factors": [0.0, 0.0, 0]}
     worst = None
+    fp64_factor_queries = []
     for i, r64 in res["fp64"].items():
         r32, q = res["mixed"][i], qs[i]
         cat = errs["fp64_factors" if r32.fp64_fallback else "refined"]
@@ -420,6 +421,9 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10):
             e = np.linalg.norm(r32.E[r] - r64.E[r], axis=1) / np.maximum(nes, 1e-3 * nes.max())
             g = float(np.linalg.norm(r32.E[r] - r64.E[r]) / np.linalg.norm(es))
             cat[0], cat[1] = max(cat[0], float(e.max())), max(cat[1], g)
+            if r32.fp64_fallback and r == 0:
+                fp64_factor_queries.append({"query": i, "mesh": q.mesh, "k": q.k, "nrhs": int(r64.E.shape[0]),
+                                            "built_here": bool(r32.factored), "set_rel_err_first_set": g})
             if not r32.fp64_fallback and (worst is None or g > worst["set_rel_err"]):
                 worst = {"query": i, "mesh": q.mesh, "k": q.k, "refine_resid": r32.refine_resid,
                          "refine_iters": r32.refine_iters, "set_rel_err": g}
@@ -429,6 +433,7 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10):
                                            "set_max_rel": errs["fp64_factors"][1],
                                            "queries": errs["fp64_factors"][2]}
     out["worst_refined"] = worst
+    out["fp64_factor_queries"] = fp64_factor_queries
     out["capacity_solves_per_s_ratio"] = round(out["burst"]["mixed"]["solves_per_s"] /
                                                out["burst"]["fp64"]["solves_per_s"], 3)
     # the plain FP64 GPU form on the same queries: each query on its own, in arrival
@@ -772,6 +777,7 @@ def main():
                     help="batched form: plan kernels only, on synthetic currents (counter runs)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the served forms")
     ap.add_argument("--lu-panel", type=int, default=1, choices=(0, 1),
                     help="LU panels as one whole-panel launch (1) or per-column launches (0), every context")
     args = ap.parse_args()
@@ -792,7 +798,7 @@ def main():
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
         return
     if args.serving_only:
-        print(json.dumps({"serving": serving_leg(nq=args.serve_queries)}), flush=True)
+        print(json.dumps({"serving": serving_leg(nq=args.serve_queries, workers=args.serve_workers)}), flush=True)
         return
     w = workload(args.config)
     verts, k, pts = w["verts"], w["k"], w["pts"]

@@ -402,6 +402,10 @@ __global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1
             if (t == 0) { best = cabs1<R>(a); bi = i; }
         }
     }
+    // Every wave's stores to A must have completed before lane 0's device fence below
+    // writes this XCD's L2 back: a workgroup barrier does not wait for them, and the
+    // last workgroup (on any XCD) reads rows j+1 and p of these columns.
+    __builtin_amdgcn_s_waitcnt(0);
     sv[threadIdx.x] = best;
     si[threadIdx.x] = bi;
     __syncthreads();

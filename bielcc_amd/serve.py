@@ -409,7 +409,10 @@ class Server:
             ctx.upload(dP, pts)
             m.rhs_planewave_into(dB, n2, kvec, pols)
             iters, resid, fallback = 0, 0.0, False
-            if self.form == "mixed" and "LU64" not in e.bufs:
+            # decided once: another worker may add FP64 factors to the entry while this
+            # one refines, and a refined solution must not be solved a second time
+            fallback = self.form == "mixed" and "LU64" in e.bufs
+            if self.form == "mixed" and not fallback:
                 try:
                     iters, resid = ctx.refine_solve(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], dB, nrhs,
                                                     n2, self.refine_tol, 30)
@@ -418,9 +421,9 @@ class Server:
                         raise
                     self._upgrade(w, e, str(err))
                     m.rhs_planewave_into(dB, n2, kvec, pols)
-            if self.form == "mixed" and "LU64" in e.bufs:
+                    fallback = True
+            if fallback:
                 ctx.lu_solve(e.bufs["LU64"], n2, n2, e.bufs["piv64"], dB, nrhs, n2)
-                fallback = True
             elif self.form == "fp64":
                 ctx.lu_solve(e.bufs["Z"], n2, n2, e.bufs["piv"], dB, nrhs, n2)
             m.currents_into(dB, n2, nrhs, dJ)
