@@ -343,6 +343,7 @@ def plain_fp64_run(pool, queries):
     npp = B.num_param()
     ctx.set_tuning("opmat_min_rhs", 1 << 20)
     meshes = {}
+    phases = {}
     t0 = time.perf_counter()
     lat = []
     for q in queries:
@@ -351,14 +352,24 @@ def plain_fp64_run(pool, queries):
         n2, nr, M = 2 * m.n, q.nrhs, q.points.shape[0]
         dZ, dpiv, dB = ctx.empty(n2 * n2 * 16), ctx.empty(n2 * 4), ctx.empty(n2 * nr * 16)
         dJ, dP, dE, dH = ctx.empty(m.n * nr * 48), ctx.to_device(q.points), ctx.empty(nr * M * 48), ctx.empty(nr * M * 48)
+        t1 = time.perf_counter()
         m.assemble_into(dZ, n2, npp, q.k)
+        ctx.sync()
+        t2 = time.perf_counter()
         ctx.lu_factor(dZ, n2, n2, dpiv)
+        t3 = time.perf_counter()
         kvec = q.k * np.asarray(q.dirs)
         m.rhs_planewave_into(dB, n2, kvec, q.pols)
         ctx.lu_solve(dZ, n2, n2, dpiv, dB, nr, n2)
         m.currents_into(dB, n2, nr, dJ)
+        ctx.sync()
+        t4 = time.perf_counter()
         m.field_batch_into(npp, q.k, nr, dJ, dP, M, dE, dH, inc=np.concatenate([kvec, q.pols], axis=1))
         dE.to_numpy((nr, M, 3), np.complex128)
+        t5 = time.perf_counter()
+        for k, v in (("setup", t1 - ts), ("assemble", t2 - t1), ("lu", t3 - t2), ("solve", t4 - t3),
+                     ("field_and_copy", t5 - t4)):
+            phases[k] = phases.get(k, 0.0) + v
         for d in (dZ, dpiv, dB, dJ, dP, dE, dH):
             d.free()
         lat.append(time.perf_counter() - ts)
@@ -369,7 +380,8 @@ def plain_fp64_run(pool, queries):
     solves = sum(q.nrhs for q in queries)
     log(f"[bench] plain FP64, one query at a time: {wall:.1f} s, {solves / wall:.1f} solves/s")
     return {"queries": len(queries), "solves": solves, "wall_s": round(wall, 3),
-            "solves_per_s": round(solves / wall, 3), "mean_query_s": round(float(np.mean(lat)), 4)}
+            "solves_per_s": round(solves / wall, 3), "mean_query_s": round(float(np.mean(lat)), 4),
+            "seconds": {k: round(v, 3) for k, v in phases.items()}}
 
 
 def serving_leg(nq=96, workers=4, refine_tol=1e-10):

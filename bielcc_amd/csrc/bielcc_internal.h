@@ -220,6 +220,8 @@ int lu_get_look_ahead(const LuWork*);
 void lu_set_panel_lds(LuWork*, bool on);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);
+// sync-free triangular solves redone on the blocked path (which 0: a wait expired, 1: ticket fault)
+int lu_trsv_retries(const LuWork*, int which);
 bool lu_get_blas_atomics(const LuWork*);
 bool lu_get_panel_lds(const LuWork*);
 void lu_set_panel_wait_us(LuWork*, int us);

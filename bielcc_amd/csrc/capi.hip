@@ -650,6 +650,8 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu) ? 1 : 0;
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
+    else if (k == "trsv_expired") *value = lu_trsv_retries(ctx->lu, 0);
+    else if (k == "trsv_ticket_faults") *value = lu_trsv_retries(ctx->lu, 1);
     else if (k == "lu_panel_wait_us") *value = lu_get_panel_wait_us(ctx->lu);
     else if (k == "lu_panel_aborts") *value = lu_panel_aborts(ctx->lu);
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;

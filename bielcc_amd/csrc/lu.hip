@@ -91,6 +91,8 @@ struct LuWork {
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
     int dev = 0, n_cu = 0, clk_khz = 0;
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
+    int trsv_expired = 0;           // sync-free solves redone on the blocked path: a wait expired
+    int trsv_ticket_faults = 0;     // ... the ticket counter was out of step with the host
     hipStream_t side = nullptr;     // side stream (default priority) and its two events
     hipEvent_t ev_upd = nullptr, ev_panel = nullptr;
 };
@@ -105,6 +107,7 @@ void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
 void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, bool on) { w->panel_lds = on; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
+int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
 bool lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
 void lu_set_panel_wait_us(LuWork* w, int us) { w->panel_wait_us = us; }
@@ -132,6 +135,8 @@ static hipError_t trsv_check(hipStream_t s, LuWork* w, const char** msg) {
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     if (h) {
         (void)hipMemsetAsync(w->err, 0, sizeof(int), s);
+        if (h == 2) w->trsv_ticket_faults++;
+        else w->trsv_expired++;
         *msg = "triangular solve: a block wait expired (result invalid)";
         return hipErrorLaunchTimeOut;
     }
@@ -168,13 +173,18 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
         if ((e = hipMalloc(&w->pinv, 4 * sizeof(double))) != hipSuccess) return e;
         if ((e = hipMalloc(&w->info, sizeof(int))) != hipSuccess) return e;
     }
+    // Counters the kernels expect at zero.  hipMemset runs on the null stream, which is not
+    // ordered with the context's non-blocking streams: wait for it before any kernel can
+    // use them (a late zeroing would land between kernels that count on these words).
     if (!w->step_count) {
         if ((e = hipMalloc(&w->step_count, sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipMemset(w->step_count, 0, sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
     }
     if (!w->psync) {
         if ((e = hipMalloc(&w->psync, 4 * sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipMemset(w->psync, 0, 4 * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
         if ((e = hipMalloc(&w->pbuf, kPanelBufBytes)) != hipSuccess) return e;
         if ((e = hipGetDevice(&w->dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&w->n_cu, hipDeviceAttributeMultiprocessorCount, w->dev)) != hipSuccess)
@@ -1321,24 +1331,29 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
     if (tid == 0) __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-static hipError_t trsv_init(LuWork* w, int n) {
+// The ticket counter, the timeout flag and the ready flags are zeroed on the solve's own
+// stream, ahead of the first kernel that uses them.  A zeroing on the null stream is not
+// ordered with the context's non-blocking stream: landing after a launch had taken
+// tickets, it would leave the counter out of step for every later solve on the context.
+// Epochs keep counting across a reallocation of the flags.
+static hipError_t trsv_init(hipStream_t s, LuWork* w, int n) {
     hipError_t e;
     const int nblk = (n + kNB - 1) / kNB;
     if (!w->ticket) {
         if ((e = hipMalloc(&w->ticket, sizeof(unsigned long long))) != hipSuccess) return e;
         if ((e = hipMalloc(&w->err, sizeof(int))) != hipSuccess) return e;
-        if ((e = hipMemset(w->ticket, 0, sizeof(unsigned long long))) != hipSuccess) return e;
-        if ((e = hipMemset(w->err, 0, sizeof(int))) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(w->ticket, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(w->err, 0, sizeof(int), s)) != hipSuccess) return e;
         w->tbase = 0;
     }
     if (w->nflags < nblk) {
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;   // no kernel still reads the old flags
         (void)hipFree(w->flags);
         w->flags = nullptr;
         w->nflags = 0;
         if ((e = hipMalloc(&w->flags, nblk * sizeof(unsigned))) != hipSuccess) return e;
-        if ((e = hipMemset(w->flags, 0, nblk * sizeof(unsigned))) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(w->flags, 0, nblk * sizeof(unsigned), s)) != hipSuccess) return e;
         w->nflags = nblk;
-        w->epoch = 0;
     }
     return hipSuccess;
 }
@@ -1356,7 +1371,7 @@ static void trsv_launch(hipStream_t s, LuWork* w, int n, int nrhs, int lower, co
 template <typename R>
 static hipError_t trsv_solve(hipStream_t s, LuWork* w, int n, int nrhs, const c2<R>* A, long long lda, c2<R>* B,
                              long long ldb) {
-    hipError_t e = trsv_init(w, n);
+    hipError_t e = trsv_init(s, w, n);
     if (e != hipSuccess) return e;
     for (int lower = 1; lower >= 0; lower--) {
         if (nrhs <= 1) trsv_launch<R, 1>(s, w, n, nrhs, lower, A, lda, B, ldb);
@@ -1365,7 +1380,7 @@ static hipError_t trsv_solve(hipStream_t s, LuWork* w, int n, int nrhs, const c2
         else trsv_launch<R, 8>(s, w, n, nrhs, lower, A, lda, B, ldb);
         if ((e = hipGetLastError()) != hipSuccess) {
             // the kernel did not run: bring the ticket counter back in step
-            (void)hipMemset(w->ticket, 0, sizeof(unsigned long long));
+            (void)hipMemsetAsync(w->ticket, 0, sizeof(unsigned long long), s);
             w->tbase = 0;
             return e;
         }

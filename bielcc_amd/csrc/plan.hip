@@ -124,7 +124,10 @@ int plan_near(bielcc_ctx ctx, bielcc_plan p, int nq, const double* q, int q_stri
 int upload_tiles(bielcc_ctx ctx, bielcc_plan p, const std::vector<int4>& tiles) {
     p->ntiles = (long long)tiles.size();
     HIPCHK(palloc(p, tiles.size() * sizeof(int4), &p->tiles_d));
-    HIPCHK(hipMemcpy(p->tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    // on the context's stream (a null-stream copy is not ordered with it), complete before
+    // the host vector goes away
+    HIPCHK(hipMemcpyAsync(p->tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return BIELCC_OK;
 }
 
@@ -358,7 +361,8 @@ int bielcc_plan_assemble(bielcc_ctx ctx, int nitems, const bielcc_assemble_item*
     }
     hipError_t e = palloc(p, p->aitems.size() * sizeof(AssembleArgs), &p->aitems_d);
     if (e == hipSuccess)
-        e = hipMemcpy(p->aitems_d, p->aitems.data(), p->aitems.size() * sizeof(AssembleArgs), hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(p->aitems_d, p->aitems.data(), p->aitems.size() * sizeof(AssembleArgs),
+                           hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) { plan_free(p); return hip_fail(ctx, e, "plan upload"); }
     if (int rc = upload_tiles(ctx, p, tiles)) { plan_free(p); return rc; }
     p->groups.push_back(Group{0, (int)tiles.size(), 0, nitems, grid_for(ctx, (int)tiles.size())});
@@ -481,7 +485,8 @@ int bielcc_plan_field(bielcc_ctx ctx, int nitems, const bielcc_field_item* items
     if (!p->segs.empty()) {
         e = palloc(p, p->segs.size() * sizeof(OpSeg), &p->segs_d);
         if (e == hipSuccess)
-            e = hipMemcpy(p->segs_d, p->segs.data(), p->segs.size() * sizeof(OpSeg), hipMemcpyHostToDevice);
+            e = hipMemcpyAsync(p->segs_d, p->segs.data(), p->segs.size() * sizeof(OpSeg), hipMemcpyHostToDevice,
+                               ctx->stream);
         if (e != hipSuccess) { plan_free(p); return hip_fail(ctx, e, "plan upload"); }
         if (int rc = upload_tiles(ctx, p, tiles)) { plan_free(p); return rc; }
     }

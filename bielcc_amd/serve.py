@@ -73,6 +73,7 @@ class Result:
     refine_iters: int = 0
     refine_resid: float = 0.0
     fp64_fallback: bool = False   # mixed form: solved with FP64 factors
+    phases: dict = dataclasses.field(default_factory=dict)   # host seconds: build, wait, solve, field, copy
 
     @property
     def latency(self) -> float:
@@ -382,6 +383,7 @@ class Server:
             self.budget.reserve(grow, self.cache.evict)
             self.scratch_hw[w] += grow
         fbytes = self._factor_bytes(n)
+        ph: dict = {}
         e, builder = self.cache.pin((q.mesh, float(q.k)), fbytes)
         try:
             if builder:
@@ -392,6 +394,7 @@ class Server:
                 except BaseException as exc:
                     self.cache.finish(e, exc)
                     raise
+                t0 = time.perf_counter()
                 try:
                     self._build(w, m, q.k, e)
                 except BaseException as exc:
@@ -399,13 +402,17 @@ class Server:
                     self.budget.release(fbytes)
                     raise
                 self.cache.finish(e)
+                ph["build"] = time.perf_counter() - t0
             else:
+                t0 = time.perf_counter()
                 self.cache.wait_ready(e)
+                ph["wait"] = time.perf_counter() - t0
             kvec = q.k * dirs
             dB = self._buf(w, "B", n2 * nrhs * 16)
             dJ = self._buf(w, "J", n * nrhs * 48)
             dP = self._buf(w, "P", M * 24)
             dE, dH = self._buf(w, "E", nrhs * M * 48), self._buf(w, "H", nrhs * M * 48)
+            t0 = time.perf_counter()
             ctx.upload(dP, pts)
             m.rhs_planewave_into(dB, n2, kvec, pols)
             iters, resid, fallback = 0, 0.0, False
@@ -427,17 +434,24 @@ class Server:
             elif self.form == "fp64":
                 ctx.lu_solve(e.bufs["Z"], n2, n2, e.bufs["piv"], dB, nrhs, n2)
             m.currents_into(dB, n2, nrhs, dJ)
+            ctx.sync()
+            ph["solve"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
             inc = np.concatenate([kvec, pols], axis=1)
             # near a resonance the currents carry a large non-radiating part whose
             # exterior field cancels: FP32 far sums would lose that cancellation
             prec = api.PREC_MIXED if self.form == "mixed" and not fallback else api.PREC_FP64
             m.field_batch_into(self.np, q.k, nrhs, dJ, dP, M, dE, dH, inc=inc, precision=prec)
+            ctx.sync()
+            ph["field"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
             E = dE.to_numpy((nrhs, M, 3), np.complex128)
             H = dH.to_numpy((nrhs, M, 3), np.complex128)
+            ph["copy"] = time.perf_counter() - t0
         finally:
             if e.ready or not builder:
                 self.cache.release(e)
-        return Result(q.qid, E, H, q.arrival, start, time.perf_counter(), builder, iters, resid, fallback)
+        return Result(q.qid, E, H, q.arrival, start, time.perf_counter(), builder, iters, resid, fallback, ph)
 
 
 def percentiles(lat) -> dict:
@@ -468,6 +482,10 @@ def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
            "fp64_fallbacks": sum(r.fp64_fallback for r in res.values()),
            "fp64_upgrades": list(server.upgrades),
            "refine_corrections_max": max((r.refine_iters for r in res.values()), default=0),
-           "lu_panel_aborts": sum(max(c.get_tuning("lu_panel_aborts"), 0) for c in server.ctxs)}
+           "lu_panel_aborts": sum(max(c.get_tuning("lu_panel_aborts"), 0) for c in server.ctxs),
+           "trsv_expired": sum(c.get_tuning("trsv_expired") for c in server.ctxs),
+           "trsv_ticket_faults": sum(c.get_tuning("trsv_ticket_faults") for c in server.ctxs),
+           "worker_seconds": {k: round(sum(r.phases.get(k, 0.0) for r in res.values()), 3)
+                              for k in ("build", "wait", "solve", "field", "copy")}}
     out.update(percentiles([r.latency for r in res.values()]))
     return {"results": res, "stats": out}

@@ -370,7 +370,10 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
  * bielcc_get_tuning also reads
  * "n_cu" (compute units) and "lu_panel_aborts" (whole-panel launches that
- * aborted to one workgroup, for the context, since its first LU).  Returns BIELCC_ERR_ARG for an unknown key or a
+ * aborted to one workgroup, for the context, since its first LU),
+ * "trsv_expired" and "trsv_ticket_faults" (one-kernel triangular solves
+ * redone on the blocked path because a wait expired, or because the ticket
+ * counter was out of step).  Returns BIELCC_ERR_ARG for an unknown key or a
  * value out of range. */
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);
 int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);
