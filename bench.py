@@ -289,6 +289,41 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
             "near_r1.5_and_far_pattern": nearfar}
 
 
+def lu_only(n, reps=3):
+    """FP64 and FP32 LU of a seeded random complex n x n matrix (entries N(0,1) + i N(0,1)),
+    each factored reps times from the same copy; the fastest of each is reported."""
+    import bielcc_amd as B
+    ctx = B.Context(0)
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    dA0 = ctx.to_device(np.ascontiguousarray(A.T))
+    del A
+    dA, piv, dF = ctx.empty(n * n * 16), ctx.empty(4 * n), ctx.empty(n * n * 8)
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        return time.perf_counter() - t0
+    t64, t32 = [], []
+    for _ in range(reps):
+        ctx.copy(dA, dA0, n * n * 16)
+        ctx.sync()
+        t64.append(timed(lambda: ctx.lu_factor(dA, n, n, piv)))
+        t32.append(timed(lambda: ctx.lu32_factor(dA0, n, n, dF, n, piv)))
+    flop = 8.0 / 3.0 * n ** 3
+    out = {"n": n, "lu64_s": round(min(t64), 4), "lu64_tflops": round(flop / min(t64) / 1e12, 2),
+           "lu32_s": round(min(t32), 4), "lu32_tflops": round(flop / min(t32) / 1e12, 2),
+           "panel_aborts": ctx.get_tuning("lu_panel_aborts")}
+    log(f"[bench] LU n={n}: FP64 {out['lu64_s']} s ({out['lu64_tflops']} TFLOP/s), FP32 {out['lu32_s']} s "
+        f"({out['lu32_tflops']} TFLOP/s)")
+    for d in (dA0, dA, piv, dF):
+        d.free()
+    ctx.close()
+    return out
+
+
 def serving_pool(seed=5, n=16, lo=1000, hi=6000):
     """Seeded pool of closed meshes: spheres, triaxial ellipsoids (semi-axes in
     [0.5, 1.5]), tori (R = 1, r in [0.2, 0.4]) and cubes of side 2, lo..hi cells."""
@@ -789,7 +824,11 @@ def main():
                     help="batched form: plan kernels only, on synthetic currents (counter runs)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--lu-only", type=int, default=0, metavar="N",
+                    help="time only the FP64 and FP32 LU of a seeded random complex N x N matrix")
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the served forms")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="library tunable for every context (repeatable), e.g. lu_look_ahead=0")
     ap.add_argument("--lu-panel", type=int, default=1, choices=(0, 1),
                     help="LU panels as one whole-panel launch (1) or per-column launches (0), every context")
     args = ap.parse_args()
@@ -800,6 +839,9 @@ def main():
 
     import bielcc_amd as B
     B.Context.default_tuning["lu_panel_lds"] = args.lu_panel
+    for kv in args.tune:
+        key, _, val = kv.partition("=")
+        B.Context.default_tuning[key] = int(val)
     if args.copy_calib:
         copy_calibration()
     if args.batched_only:
@@ -808,6 +850,9 @@ def main():
         return
     if args.cube_only:
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
+        return
+    if args.lu_only:
+        print(json.dumps({"lu": lu_only(args.lu_only)}), flush=True)
         return
     if args.serving_only:
         print(json.dumps({"serving": serving_leg(nq=args.serve_queries, workers=args.serve_workers)}), flush=True)

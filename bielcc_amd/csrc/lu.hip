@@ -42,8 +42,8 @@ constexpr int kFusedMaxN = 6144;
 // Look-ahead pays only where the trailing GEMM is long and the first panels are too tall for
 // the whole-panel kernel: FP64 above this order (n = 28,000: -3%; n <= 12,000 and FP32: slower).
 constexpr int kLookAheadMinN = 20000;
-// candidate buffers of the whole-panel kernel: cval, cidx [2][256], crow [2][256][64], rowj [2][64] (FP64 sizes)
-constexpr size_t kPanelBufBytes = 2 * 256 * (8 + 4) + 2 * 256 * 64 * 16 + 2 * 64 * 16;
+// candidate buffers of the whole-panel kernel: cval, cidx [2][512], crow [2][512][64], rowj [2][64] (FP64 sizes)
+constexpr size_t kPanelBufBytes = 2 * 512 * (8 + 4) + 2 * 512 * 64 * 16 + 2 * 64 * 16;
 
 template <typename R> struct V2;
 template <> struct V2<double> { using T = double2; };
@@ -86,6 +86,7 @@ struct LuWork {
                                     // 2 for FP64 above kLookAheadMinN
     // whole-panel kernel (k_panel_lds): barrier words, published candidates, settings
     unsigned* psync = nullptr;      // [3]: barrier word, workgroups gone, aborted panels
+    int2* smap = nullptr;           // the current panel's interchanges as a gather (2 kNB entries + count)
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
     bool panel_lds = true;
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
@@ -148,7 +149,7 @@ void lu_work_destroy(LuWork* w) {
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
     (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->step_count);
-    (void)hipFree(w->psync); (void)hipFree(w->pbuf);
+    (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap);
     if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
     if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
     if (w->side) (void)hipStreamDestroy(w->side);
@@ -186,6 +187,7 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
         if ((e = hipMemset(w->psync, 0, 4 * sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
         if ((e = hipMalloc(&w->pbuf, kPanelBufBytes)) != hipSuccess) return e;
+        if ((e = hipMalloc(&w->smap, (2 * kNB + 1) * sizeof(int2))) != hipSuccess) return e;
         if ((e = hipGetDevice(&w->dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&w->n_cu, hipDeviceAttributeMultiprocessorCount, w->dev)) != hipSuccess)
             return e;
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1
 // an aborted panel has written nothing to A, and the last workgroup to leave
 // factors it alone from global memory.  That workgroup also resets the words.
 constexpr unsigned kAbortBit = 0x80000000u;
-constexpr int kPanelMaxWG = 256;
+constexpr int kPanelMaxWG = 512;   // two per compute unit
 
 template <typename R> struct PanelRows;
 template <> struct PanelRows<double> { static constexpr int v = 64; };    // 64 KB of LDS
@@ -689,10 +691,10 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
         // the winner, identically in every workgroup
         double best = -1.;
         int bi = NONE;
-        if (tid < G) {
-            best = __hip_atomic_load(&cval[b * kPanelMaxWG + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bi = __hip_atomic_load(&cidx[b * kPanelMaxWG + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (int q = tid; q < G; q += 256)
+            argmax_merge(best, bi,
+                         __hip_atomic_load(&cval[b * kPanelMaxWG + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         __hip_atomic_load(&cidx[b * kPanelMaxWG + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         for (int off = 32; off > 0; off >>= 1) argmax_merge(best, bi, __shfl_xor(best, off, 64), __shfl_xor(bi, off, 64));
         if (lane == 0) { s_v4[w] = best; s_i4[w] = bi; }
         __syncthreads();
@@ -768,21 +770,71 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
     }
 }
 
-// Apply the interchanges piv[k0 .. k0+nb) to columns [cbeg, cend).
-template <typename R>
-__global__ __launch_bounds__(256) void k_laswp(int k0, int nb, int cbeg, int cend, c2<R>* __restrict__ A,
-                                               long long lda, const int* __restrict__ piv) {
-    const int c = cbeg + blockIdx.x * 256 + threadIdx.x;
-    if (c >= cend) return;
-    c2<R>* col = A + (size_t)c * lda;
-    for (int t = 0; t < nb; t++) {
-        int r = k0 + t, p = piv[r];
-        if (p != r) {
-            c2<R> v = col[r];
-            col[r] = col[p];
-            col[p] = v;
+// The interchanges of one panel, as a gather.  The nb swaps (row k0 + t with row
+// piv[k0 + t], in order) permute the rows R = {k0 .. k0+nb-1} together with the
+// distinct pivot rows below the panel.  k_swap_map works out, for every row d of R,
+// the row whose old contents end at d: map[q] = (d, source), the panel's own rows
+// first (q < nb, d = k0 + q), then the outside pivot rows in order of appearance;
+// *count = |R| <= 2 nb.  One wave.
+__global__ __launch_bounds__(64) void k_swap_map(int k0, int nb, const int* __restrict__ piv,
+                                                 int2* __restrict__ map, int* __restrict__ count) {
+    __shared__ int orow[kNB];         // outside rows, by first appearance
+    __shared__ int slot[kNB];         // slot of step t's partner row
+    __shared__ int content[2 * kNB];  // original row now held by each slot
+    const int t = threadIdx.x;
+    int p = 0;
+    bool first = false;
+    if (t < nb) {
+        p = piv[k0 + t];
+        if (p >= k0 + nb) {
+            first = true;
+            for (int u = 0; u < t; u++)
+                if (piv[k0 + u] == p) { first = false; break; }
         }
     }
+    const unsigned long long bal = __ballot(first);
+    const int nout = __popcll(bal);
+    if (first) orow[__popcll(bal & ((1ull << t) - 1ull))] = p;
+    __syncthreads();
+    if (t < nb) {
+        int sl = p - k0;
+        if (p >= k0 + nb)
+            for (int u = 0; u < nout; u++)
+                if (orow[u] == p) { sl = nb + u; break; }
+        slot[t] = sl;
+    }
+    for (int q = t; q < nb + nout; q += 64) content[q] = q < nb ? k0 + q : orow[q - nb];
+    __syncthreads();
+    if (t == 0)
+        for (int u = 0; u < nb; u++) {
+            const int b = slot[u], x = content[u];
+            content[u] = content[b];
+            content[b] = x;
+        }
+    __syncthreads();
+    for (int q = t; q < nb + nout; q += 64) map[q] = make_int2(q < nb ? k0 + q : orow[q - nb], content[q]);
+    if (t == 0) *count = nb + nout;
+}
+
+// Apply a panel's interchanges (k_swap_map) to columns [cbeg, cend): one wave per
+// column reads the rows of R it needs, then writes them to their new places (the
+// panel's 64 rows are contiguous, so half the accesses coalesce; the old
+// one-lane-per-column sweep touched a different column with every lane).  A wave's
+// loads and stores of one address stay in program order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_row_gather(int cbeg, int cend, c2<R>* __restrict__ A, long long lda,
+                                                    const int2* __restrict__ map, const int* __restrict__ count) {
+    const int c = cbeg + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= cend) return;   // whole waves
+    const int lane = threadIdx.x & 63, cnt = *count;
+    c2<R>* col = A + (size_t)c * lda;
+    int2 m0 = make_int2(0, 0), m1 = make_int2(0, 0);
+    c2<R> v0 = mk2<R>(R(0), R(0)), v1 = v0;
+    const bool h0 = lane < cnt, h1 = lane + 64 < cnt;
+    if (h0) { m0 = map[lane]; v0 = col[m0.y]; }
+    if (h1) { m1 = map[lane + 64]; v1 = col[m1.y]; }
+    if (h0) col[m0.x] = v0;
+    if (h1) col[m1.x] = v1;
 }
 
 // X = T^{-1} X for the nb x nb triangle T at (t0, t0) of A (lower unit when
@@ -940,18 +992,20 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
     char* pb = static_cast<char*>(w->pbuf);
     double* cval = reinterpret_cast<double*>(pb);
-    int* cidx = reinterpret_cast<int*>(pb + 2 * 256 * 8);
-    c2<R>* crow = reinterpret_cast<c2<R>*>(pb + 2 * 256 * 12);
+    int* cidx = reinterpret_cast<int*>(pb + 2 * kPanelMaxWG * 8);
+    c2<R>* crow = reinterpret_cast<c2<R>*>(pb + 2 * kPanelMaxWG * 12);
     c2<R>* rowj = crow + 2 * kPanelMaxWG * kNB;
+    int* smap_count = reinterpret_cast<int*>(w->smap + 2 * kNB);
     // panel [k0, k0 + nb) on stream st: pivot, interchange inside the panel, scale,
     // rank-1 update of the panel's remaining columns
     auto factor_panel = [&](hipStream_t st, int k0) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
         const int G = (n - k0 + RPW - 1) / RPW;
-        if (lds && G <= kPanelMaxWG && running.alone()) {
+        if (lds && G <= kPanelMaxWG && G <= 2 * w->n_cu && running.alone()) {
             hipLaunchKernelGGL(k_panel_lds<R>, dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info, w->psync,
                                cval, cidx, crow, rowj, wait_ticks);
+            hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, k0, nb, piv, w->smap, smap_count);
             return;
         }
         int nparts = 0;
@@ -977,6 +1031,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                                        w->pval, w->pidx);
             }
         }
+        hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, k0, nb, piv, w->smap, smap_count);
     };
     // Look-ahead: after panel k0's interchanges and U12, the trailing update is split.
     // The next panel's columns [c1, c2) are updated first; that panel is then factored on
@@ -991,9 +1046,11 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         const int c1 = k0 + nb;
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (k0 > 0)
-            hipLaunchKernelGGL(k_laswp<R>, dim3((k0 + 255) / 256), dim3(256), 0, s, k0, nb, 0, k0, A, lda, piv);
+            hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + 3) / 4), dim3(256), 0, s, 0, k0, A, lda, w->smap,
+                               smap_count);
         if (c1 >= n) break;
-        hipLaunchKernelGGL(k_laswp<R>, dim3((n - c1 + 255) / 256), dim3(256), 0, s, k0, nb, c1, n, A, lda, piv);
+        hipLaunchKernelGGL(k_row_gather<R>, dim3((n - c1 + 3) / 4), dim3(256), 0, s, c1, n, A, lda, w->smap,
+                           smap_count);
         hipLaunchKernelGGL(k_trsm<R>, dim3((n - c1 + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, A, lda,
                            (long long)k0, c1, n);
         const c2<R>* L21 = A + (size_t)k0 * lda + c1;

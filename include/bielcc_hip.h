@@ -356,7 +356,7 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * one up to order 6,144 and two above (2); "lu_look_ahead" [2] = the next
  * LU panel is factored on a side stream during the trailing update: always
  * (1), never (0), or for FP64 above order 20,000 (2); "lu_panel_lds" [1] =
- * LU panels of at most 16,384 rows (FP64) or 32,768 (FP32) run as one launch
+ * LU panels of at most 32,768 rows (FP64) or 65,536 (FP32) run as one launch
  * that holds the panel in LDS with one grid barrier per column (0: the
  * per-column launches); one LU per device at a time uses it, LUs running
  * meanwhile take the per-column path; "lu_panel_wait_us"

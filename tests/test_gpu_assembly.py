@@ -259,7 +259,7 @@ def test_look_ahead_matches_in_order_lu(ctx, n):
     assert np.abs(out[1][2] - out[0][2]).max() <= 1e-6 * n * scale
 
 
-def _factors(ctx, A, **tuning):
+def _factors(ctx, A, fp32=True, **tuning):
     n = A.shape[0]
     old = {k: ctx.get_tuning(k) for k in tuning}
     try:
@@ -268,10 +268,13 @@ def _factors(ctx, A, **tuning):
         dA = ctx.to_device(np.ascontiguousarray(A.T))
         piv = ctx.empty(4 * n)
         ctx.lu_factor(dA, n, n, piv)
+        out = (dA.to_numpy((n, n), np.complex128), piv.to_numpy((n,), np.int32))
+        dA.free()
+        if not fp32:
+            return out
         dF, piv32 = ctx.empty(n * n * 8), ctx.empty(4 * n)
         ctx.lu32_factor(ctx.to_device(np.ascontiguousarray(A.T)), n, n, dF, n, piv32)
-        return (dA.to_numpy((n, n), np.complex128), piv.to_numpy((n,), np.int32),
-                dF.to_numpy((n, n), np.complex64), piv32.to_numpy((n,), np.int32))
+        return out + (dF.to_numpy((n, n), np.complex64), piv32.to_numpy((n,), np.int32))
     finally:
         for k, v in old.items():
             ctx.set_tuning(k, v)
@@ -380,3 +383,15 @@ def test_concurrent_per_column_lus_match_alone():
         finally:
             for c in ctxs:
                 c.close()
+
+
+def test_whole_panel_kernel_two_workgroups_per_cu(ctx):
+    """Order 17,000 in FP64: the first panels need 266 workgroups of 64 rows, more
+    than one per compute unit; factors and pivots equal the per-column path's bit
+    for bit (the interchanges of both paths go through the same row gather)."""
+    n = 17000
+    rng = np.random.default_rng(16)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = _factors(ctx, A, fp32=False, lu_panel_lds=1, lu_look_ahead=0)
+    b = _factors(ctx, A, fp32=False, lu_panel_lds=0, lu_look_ahead=0)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
