@@ -220,6 +220,9 @@ int lu_get_look_ahead(const LuWork*);
 void lu_set_panel_lds(LuWork*, bool on);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);
+// columns per outer block of the two-level LU (64: one level, with the optional look-ahead)
+void lu_set_outer_nb(LuWork*, int nb);
+int lu_get_outer_nb(const LuWork*);
 // sync-free triangular solves redone on the blocked path (which 0: a wait expired, 1: ticket fault)
 int lu_trsv_retries(const LuWork*, int which);
 bool lu_get_blas_atomics(const LuWork*);

@@ -353,9 +353,13 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * [1048576] = polls per wait of the one-kernel solve before it gives up (a
  * solve whose wait gave up is redone on the blocked path; tests lower it);
  * "lu_fused_panel" [2] = one launch per LU panel column (1), two (0), or
- * one up to order 6,144 and two above (2); "lu_look_ahead" [2] = the next
- * LU panel is factored on a side stream during the trailing update: always
- * (1), never (0), or for FP64 above order 20,000 (2); "lu_panel_lds" [1] =
+ * one up to order 6,144 and two above (2); "lu_outer_nb" [0] = columns
+ * per outer block of the two-level LU (64, 128, 256 or 512): the panels'
+ * updates stay inside the block and the rest of the matrix gets one update of
+ * that rank (64: one level; 0: one level below order 16,000, 512 from there);
+ * "lu_look_ahead" [2] = with a one-level LU, the
+ * next LU panel is factored on a side stream during the trailing update:
+ * always (1), never (0), or for FP64 above order 20,000 (2); "lu_panel_lds" [1] =
  * LU panels of at most 32,768 rows (FP64) or 65,536 (FP32) run as one launch
  * that holds the panel in LDS with one grid barrier per column (0: the
  * per-column launches); one LU per device at a time uses it, LUs running

@@ -240,6 +240,7 @@ def test_look_ahead_matches_in_order_lu(ctx, n):
     rng = np.random.default_rng(9)
     A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     out = {}
+    ctx.set_tuning("lu_outer_nb", 64)   # the look-ahead belongs to the one-level loop
     try:
         for la in (1, 0):
             ctx.set_tuning("lu_look_ahead", la)
@@ -252,6 +253,7 @@ def test_look_ahead_matches_in_order_lu(ctx, n):
                        dF.to_numpy((n, n), np.complex64), piv32.to_numpy((n,), np.int32))
     finally:
         ctx.set_tuning("lu_look_ahead", 2)
+        ctx.set_tuning("lu_outer_nb", 0)
     assert np.array_equal(out[1][1], out[0][1]) and np.array_equal(out[1][3], out[0][3])
     scale = np.abs(out[0][0]).max()
     assert np.abs(out[1][0] - out[0][0]).max() <= 1e-14 * n * scale
@@ -395,3 +397,25 @@ def test_whole_panel_kernel_two_workgroups_per_cu(ctx):
     a = _factors(ctx, A, fp32=False, lu_panel_lds=1, lu_look_ahead=0)
     b = _factors(ctx, A, fp32=False, lu_panel_lds=0, lu_look_ahead=0)
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+
+
+@pytest.mark.parametrize("n", [700, 2500])
+def test_two_level_lu_matches_one_level(ctx, n):
+    """Outer blocks of 128, 256 and 512 columns (panel updates kept inside the block,
+    one update of that rank for the rest) pick the pivots of the one-level LU and
+    give its factors to rounding; the FP64 factors reproduce P A to a backward
+    error of n * 1e-15 * max|A| (LAPACK-style bound)."""
+    rng = np.random.default_rng(17)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    ref = _factors(ctx, A, lu_outer_nb=64)
+    for nbo in (128, 256, 512):
+        out = _factors(ctx, A, lu_outer_nb=nbo)
+        assert np.array_equal(out[1], ref[1]) and np.array_equal(out[3], ref[3])
+        assert np.abs(out[0] - ref[0]).max() <= 1e-14 * n * np.abs(ref[0]).max()
+        assert np.abs(out[2] - ref[2]).max() <= 1e-6 * n * np.abs(ref[2]).max()
+    F, piv = out[0].T, out[1]
+    L, U = np.tril(F, -1) + np.eye(n), np.triu(F)
+    PA = A.copy()
+    for j in range(n):
+        PA[[j, piv[j]]] = PA[[piv[j], j]]
+    assert np.abs(PA - L @ U).max() <= n * 1e-15 * np.abs(A).max()
