@@ -222,6 +222,8 @@ void lu_set_panel_lds(LuWork*, bool on);
 void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)
 void lu_set_outer_nb(LuWork*, int nb);
+void lu_set_trsm_lib_panel(LuWork*, bool on);
+bool lu_get_trsm_lib_panel(const LuWork*);
 int lu_get_outer_nb(const LuWork*);
 // sync-free triangular solves redone on the blocked path (which 0: a wait expired, 1: ticket fault)
 int lu_trsv_retries(const LuWork*, int which);

@@ -402,20 +402,26 @@ def test_whole_panel_kernel_two_workgroups_per_cu(ctx):
 @pytest.mark.parametrize("n", [700, 2500])
 def test_two_level_lu_matches_one_level(ctx, n):
     """Outer blocks of 128, 256 and 512 columns (panel updates kept inside the block,
-    one update of that rank for the rest) pick the pivots of the one-level LU and
-    give its factors to rounding; the FP64 factors reproduce P A to a backward
-    error of n * 1e-15 * max|A| (LAPACK-style bound)."""
+    one update of that rank for the rest) pick the FP64 pivots of the one-level LU
+    and give its FP64 factors to rounding.  Both precisions' factors reproduce P A
+    to a backward error of n * u * max|A| (LAPACK-style bound; u = 1e-15 FP64,
+    1e-6 FP32).  FP32 pivots may differ: the blockings round the triangles'
+    columns differently (library TRSM), and near-ties then resolve differently."""
     rng = np.random.default_rng(17)
     A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+
+    def backward(F, piv):
+        F = F.T.astype(np.complex128)
+        L, U = np.tril(F, -1) + np.eye(n), np.triu(F)
+        PA = A.copy()
+        for j in range(n):
+            PA[[j, piv[j]]] = PA[[piv[j], j]]
+        return np.abs(PA - L @ U).max() / np.abs(A).max()
+
     ref = _factors(ctx, A, lu_outer_nb=64)
     for nbo in (128, 256, 512):
         out = _factors(ctx, A, lu_outer_nb=nbo)
-        assert np.array_equal(out[1], ref[1]) and np.array_equal(out[3], ref[3])
+        assert np.array_equal(out[1], ref[1])
         assert np.abs(out[0] - ref[0]).max() <= 1e-14 * n * np.abs(ref[0]).max()
-        assert np.abs(out[2] - ref[2]).max() <= 1e-6 * n * np.abs(ref[2]).max()
-    F, piv = out[0].T, out[1]
-    L, U = np.tril(F, -1) + np.eye(n), np.triu(F)
-    PA = A.copy()
-    for j in range(n):
-        PA[[j, piv[j]]] = PA[[piv[j], j]]
-    assert np.abs(PA - L @ U).max() <= n * 1e-15 * np.abs(A).max()
+        assert backward(out[0], out[1]) <= n * 1e-15
+        assert backward(out[2], out[3]) <= n * 1e-6
