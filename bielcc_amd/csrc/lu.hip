@@ -787,19 +787,19 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
 // *count = |R| <= 2 nb.  One wave.
 __global__ __launch_bounds__(64) void k_swap_map(int k0, int nb, const int* __restrict__ piv,
                                                  int2* __restrict__ map, int* __restrict__ count) {
+    __shared__ int sp[kNB];           // the panel's pivot rows
     __shared__ int orow[kNB];         // outside rows, by first appearance
     __shared__ int slot[kNB];         // slot of step t's partner row
     __shared__ int content[2 * kNB];  // original row now held by each slot
     const int t = threadIdx.x;
     int p = 0;
     bool first = false;
-    if (t < nb) {
-        p = piv[k0 + t];
-        if (p >= k0 + nb) {
-            first = true;
-            for (int u = 0; u < t; u++)
-                if (piv[k0 + u] == p) { first = false; break; }
-        }
+    if (t < nb) sp[t] = p = piv[k0 + t];
+    __syncthreads();
+    if (t < nb && p >= k0 + nb) {
+        first = true;
+        for (int u = 0; u < t; u++)
+            if (sp[u] == p) { first = false; break; }
     }
     const unsigned long long bal = __ballot(first);
     const int nout = __popcll(bal);
