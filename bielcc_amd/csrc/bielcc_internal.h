@@ -217,7 +217,7 @@ int lu_get_fused_panel(const LuWork*);
 void lu_set_look_ahead(LuWork*, int mode);
 int lu_get_look_ahead(const LuWork*);
 // whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
-void lu_set_panel_lds(LuWork*, bool on);
+void lu_set_panel_lds(LuWork*, int mode);   // 1 alone only, 0 never, 2 always
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)
@@ -228,7 +228,7 @@ int lu_get_outer_nb(const LuWork*);
 // sync-free triangular solves redone on the blocked path (which 0: a wait expired, 1: ticket fault)
 int lu_trsv_retries(const LuWork*, int which);
 bool lu_get_blas_atomics(const LuWork*);
-bool lu_get_panel_lds(const LuWork*);
+int lu_get_panel_lds(const LuWork*);
 void lu_set_panel_wait_us(LuWork*, int us);
 int lu_get_panel_wait_us(const LuWork*);
 int lu_panel_aborts(LuWork*);

@@ -90,7 +90,7 @@ struct LuWork {
     unsigned* psync = nullptr;      // [3]: barrier word, workgroups gone, aborted panels
     int2* smap = nullptr;           // the current panel's interchanges as a gather (2 kNB entries + count)
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
-    bool panel_lds = true;
+    int panel_lds = 1;              // whole-panel kernel: 1 alone only, 0 never, 2 always (tests)
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
     int dev = 0, n_cu = 0, clk_khz = 0;
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
@@ -111,7 +111,7 @@ void lu_set_lib_trsm(LuWork* w, bool on) { w->lib_trsm = on; }
 void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
 void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
 void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
-void lu_set_panel_lds(LuWork* w, bool on) { w->panel_lds = on; }
+void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
 void lu_set_outer_nb(LuWork* w, int nb) { w->outer_nb = nb; }
 void lu_set_trsm_lib_panel(LuWork* w, bool on) { w->lu_trsm_lib = on; }
@@ -119,7 +119,7 @@ bool lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
 int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
-bool lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
+int lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
 void lu_set_panel_wait_us(LuWork* w, int us) { w->panel_wait_us = us; }
 int lu_get_panel_wait_us(const LuWork* w) { return w->panel_wait_us; }
 int lu_get_look_ahead(const LuWork* w) { return w->look_ahead; }
@@ -196,7 +196,7 @@ static hipError_t lu_work_init(LuWork* w, const char** msg, int n = 0) {
         if ((e = hipMemset(w->psync, 0, 4 * sizeof(unsigned))) != hipSuccess) return e;
         if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
         if ((e = hipMalloc(&w->pbuf, kPanelBufBytes)) != hipSuccess) return e;
-        if ((e = hipMalloc(&w->smap, (2 * kNB + 1) * sizeof(int2))) != hipSuccess) return e;
+        if ((e = hipMalloc(&w->smap, (2 * kNB + 1) * sizeof(int2))) != hipSuccess) return e;   // + count, bad
         if ((e = hipGetDevice(&w->dev)) != hipSuccess) return e;
         if ((e = hipDeviceGetAttribute(&w->n_cu, hipDeviceAttributeMultiprocessorCount, w->dev)) != hipSuccess)
             return e;
@@ -784,9 +784,11 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
 // distinct pivot rows below the panel.  k_swap_map works out, for every row d of R,
 // the row whose old contents end at d: map[q] = (d, source), the panel's own rows
 // first (q < nb, d = k0 + q), then the outside pivot rows in order of appearance;
-// *count = |R| <= 2 nb.  One wave.
-__global__ __launch_bounds__(64) void k_swap_map(int k0, int nb, const int* __restrict__ piv,
-                                                 int2* __restrict__ map, int* __restrict__ count) {
+// *count = |R| <= 2 nb.  A pivot row outside [k0 + t, n) is not used: it raises
+// *bad, counts as no interchange, and the LU reports the error.  One wave.
+__global__ __launch_bounds__(64) void k_swap_map(int n, int k0, int nb, const int* __restrict__ piv,
+                                                 int2* __restrict__ map, int* __restrict__ count,
+                                                 int* __restrict__ bad) {
     __shared__ int sp[kNB];           // the panel's pivot rows
     __shared__ int orow[kNB];         // outside rows, by first appearance
     __shared__ int slot[kNB];         // slot of step t's partner row
@@ -794,7 +796,14 @@ __global__ __launch_bounds__(64) void k_swap_map(int k0, int nb, const int* __re
     const int t = threadIdx.x;
     int p = 0;
     bool first = false;
-    if (t < nb) sp[t] = p = piv[k0 + t];
+    if (t < nb) {
+        p = piv[k0 + t];
+        if (p < k0 + t || p >= n) {
+            atomicExch(bad, 1);
+            p = k0 + t;
+        }
+        sp[t] = p;
+    }
     __syncthreads();
     if (t < nb && p >= k0 + nb) {
         first = true;
@@ -1003,6 +1012,23 @@ struct PanelAdmit {
     }
 };
 
+// the factorisation's status: info (first zero pivot) and the interchange check
+static hipError_t lu_finish(hipStream_t s, LuWork* w, int* info, const char** msg) {
+    hipError_t e;
+    int bad = 0;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(info, w->info, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&bad, reinterpret_cast<int*>(w->smap + 2 * kNB) + 1, sizeof(int), hipMemcpyDeviceToHost,
+                            s)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (bad) {
+        *msg = "LU: a pivot row index was out of range (interchange skipped; factors invalid)";
+        return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
 template <typename R>
 static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long long lda, int* piv, int* info,
                               const char** msg) {
@@ -1010,13 +1036,15 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     if (e != hipSuccess) return e;
     R* pinv = reinterpret_cast<R*>(w->pinv);
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(reinterpret_cast<int*>(w->smap + 2 * kNB) + 1, 0, sizeof(int), s)) != hipSuccess) return e;
     const bool fused = w->fused_panel == 1 || (w->fused_panel == 2 && n <= kFusedMaxN);
     // whole-panel kernel for panels of at most kPanelMaxWG workgroups (decided per panel:
     // once another LU starts, the rest of this one's panels take the per-column path)
     constexpr int RPW = PanelRows<R>::v;
     LuRunning running(w->dev);
     PanelAdmit admit;
-    const bool lds = w->panel_lds && w->n_cu > 0 && admit.take(w->dev);
+    const bool force = w->panel_lds == 2;
+    const bool lds = w->panel_lds != 0 && w->n_cu > 0 && (force || admit.take(w->dev));
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
     char* pb = static_cast<char*>(w->pbuf);
     double* cval = reinterpret_cast<double*>(pb);
@@ -1030,10 +1058,11 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
         const int G = (n - k0 + RPW - 1) / RPW;
-        if (lds && G <= kPanelMaxWG && G <= 2 * w->n_cu && running.alone()) {
+        if (lds && G <= kPanelMaxWG && G <= 2 * w->n_cu && (force || running.alone())) {
             hipLaunchKernelGGL(k_panel_lds<R>, dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info, w->psync,
                                cval, cidx, crow, rowj, wait_ticks);
-            hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, k0, nb, piv, w->smap, smap_count);
+            hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, w->smap, smap_count,
+                               smap_count + 1);
             return;
         }
         int nparts = 0;
@@ -1059,7 +1088,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                                        w->pval, w->pidx);
             }
         }
-        hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, k0, nb, piv, w->smap, smap_count);
+        hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, w->smap, smap_count, smap_count + 1);
     };
     // Look-ahead: after panel k0's interchanges and U12, the trailing update is split.
     // The next panel's columns [c1, c2) are updated first; that panel is then factored on
@@ -1112,9 +1141,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                             lda, A + (size_t)KE * lda + KE, lda, msg);
             if (e != hipSuccess) return e;
         }
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(info, w->info, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        return hipStreamSynchronize(s);
+        return lu_finish(s, w, info, msg);
     }
     const bool ahead = w->side != nullptr &&
                        (w->look_ahead == 1 || (w->look_ahead == 2 && sizeof(R) == 8 && n > kLookAheadMinN));
@@ -1153,9 +1180,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         }
         if ((e = hipStreamWaitEvent(s, w->ev_panel, 0)) != hipSuccess) return e;
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(info, w->info, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-    return hipStreamSynchronize(s);
+    return lu_finish(s, w, info, msg);
 }
 
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda, int* piv, int* info,

@@ -364,7 +364,8 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * LU panels of at most 32,768 rows (FP64) or 65,536 (FP32) run as one launch
  * that holds the panel in LDS with one grid barrier per column (0: the
  * per-column launches); one LU per device at a time uses it, LUs running
- * meanwhile take the per-column path; "lu_panel_wait_us"
+ * meanwhile take the per-column path (2: every LU uses it, tests only);
+ * "lu_panel_wait_us"
  * [250000] = bound of each of its barrier waits: an expired wait aborts the
  * launch before it writes the matrix and one workgroup then factors the
  * panel (slow, correct; tests set 0 to force it); "blas_atomics" [0] =
