@@ -217,7 +217,7 @@ int lu_get_fused_panel(const LuWork*);
 void lu_set_look_ahead(LuWork*, int mode);
 int lu_get_look_ahead(const LuWork*);
 // whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
-void lu_set_panel_lds(LuWork*, int mode);   // 1 alone only, 0 never, 2 always
+void lu_set_panel_lds(LuWork*, int mode);   // 1 within the workgroup budget, 0 never, 2 always
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)

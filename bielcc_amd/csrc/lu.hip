@@ -90,7 +90,7 @@ struct LuWork {
     unsigned* psync = nullptr;      // [3]: barrier word, workgroups gone, aborted panels
     int2* smap = nullptr;           // the current panel's interchanges as a gather (2 kNB entries + count)
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
-    int panel_lds = 1;              // whole-panel kernel: 1 alone only, 0 never, 2 always (tests)
+    int panel_lds = 1;              // whole-panel kernel: 1 within the workgroup budget, 0 never, 2 always
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
     int dev = 0, n_cu = 0, clk_khz = 0;
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
@@ -979,12 +979,17 @@ static hipError_t panel_trsm(LuWork* w, hipStream_t s, int nb, c2<R>* A, long lo
     return hipSuccess;
 }
 
-// One LU per device at a time may use the whole-panel kernel (its grid barrier needs all
-// its workgroups resident at once), and only while no other LU runs in the process: its
-// workgroups hold 70 KB of LDS on up to every compute unit for a whole panel, which halves
-// the occupancy of the GEMMs of LUs on other streams (measured: four concurrent FP64 LUs 6%
-// slower with it).  The per-column path's kernels never wait on other workgroups.
-static std::atomic<int> g_panel_busy[64];
+// Admission to the whole-panel kernel.  Its grid barrier needs every workgroup of a
+// launch resident at once: two per compute unit fit.  An LU reserves the workgroups of
+// its tallest eligible panel (its first) against a process-wide budget of two per
+// compute unit, and uses the kernel only if the reservation fits; otherwise it takes the
+// per-column path, whose kernels never wait on other workgroups.  A panel that needs
+// more than one workgroup per compute unit runs as one launch only while its LU is the
+// only one running: its workgroups then hold LDS on nearly every compute unit, which
+// other LUs' GEMMs need.  Measured on four worker streams: served queries 10-13% faster
+// than with one such LU at a time; four concurrent FP64 LUs of order up to 28,000 (up to
+// 443 workgroups) 66% slower with neither rule, 34% slower with the budget alone.
+static std::atomic<int> g_panel_wgs[64];
 static std::atomic<int> g_lu_running[64];
 
 struct LuRunning {   // counts this LU among those running on the device
@@ -999,16 +1004,22 @@ struct LuRunning {   // counts this LU among those running on the device
 };
 
 struct PanelAdmit {
-    std::atomic<int>* flag = nullptr;
-    bool take(int dev) {
+    std::atomic<int>* ctr = nullptr;
+    int amount = 0;
+    bool take(int dev, int a, int cap) {
         if (dev < 0 || dev >= 64) return false;
-        int expect = 0;
-        if (!g_panel_busy[dev].compare_exchange_strong(expect, 1)) return false;
-        flag = &g_panel_busy[dev];
-        return true;
+        std::atomic<int>& c = g_panel_wgs[dev];
+        int cur = c.load();
+        while (cur + a <= cap)
+            if (c.compare_exchange_weak(cur, cur + a)) {
+                ctr = &c;
+                amount = a;
+                return true;
+            }
+        return false;
     }
     ~PanelAdmit() {
-        if (flag) flag->store(0);
+        if (ctr) ctr->fetch_sub(amount);
     }
 };
 
@@ -1038,13 +1049,14 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(reinterpret_cast<int*>(w->smap + 2 * kNB) + 1, 0, sizeof(int), s)) != hipSuccess) return e;
     const bool fused = w->fused_panel == 1 || (w->fused_panel == 2 && n <= kFusedMaxN);
-    // whole-panel kernel for panels of at most kPanelMaxWG workgroups (decided per panel:
-    // once another LU starts, the rest of this one's panels take the per-column path)
+    // whole-panel kernel for panels of at most kPanelMaxWG workgroups, two per compute unit
     constexpr int RPW = PanelRows<R>::v;
+    const int gmax = std::min(kPanelMaxWG, 2 * w->n_cu);
     LuRunning running(w->dev);
     PanelAdmit admit;
     const bool force = w->panel_lds == 2;
-    const bool lds = w->panel_lds != 0 && w->n_cu > 0 && (force || admit.take(w->dev));
+    const bool lds = w->panel_lds != 0 && w->n_cu > 0 &&
+                     (force || admit.take(w->dev, std::min((n + RPW - 1) / RPW, gmax), 2 * w->n_cu));
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
     char* pb = static_cast<char*>(w->pbuf);
     double* cval = reinterpret_cast<double*>(pb);
@@ -1058,7 +1070,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
         const int G = (n - k0 + RPW - 1) / RPW;
-        if (lds && G <= kPanelMaxWG && G <= 2 * w->n_cu && (force || running.alone())) {
+        if (lds && G <= gmax && (force || G <= w->n_cu || running.alone())) {
             hipLaunchKernelGGL(k_panel_lds<R>, dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info, w->psync,
                                cval, cidx, crow, rowj, wait_ticks);
             hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, w->smap, smap_count,

@@ -363,8 +363,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * always (1), never (0), or for FP64 above order 20,000 (2); "lu_panel_lds" [1] =
  * LU panels of at most 32,768 rows (FP64) or 65,536 (FP32) run as one launch
  * that holds the panel in LDS with one grid barrier per column (0: the
- * per-column launches); one LU per device at a time uses it, LUs running
- * meanwhile take the per-column path (2: every LU uses it, tests only);
+ * per-column launches); an LU uses it if its tallest panel's workgroups fit,
+ * beside those of the LUs running at the same time, in two per compute unit,
+ * else it takes the per-column path (2: every LU uses it, tests only);
  * "lu_panel_wait_us"
  * [250000] = bound of each of its barrier waits: an expired wait aborts the
  * launch before it writes the matrix and one workgroup then factors the
