@@ -222,6 +222,10 @@ void lu_set_panel_lds(LuWork*, int mode);   // 1 within the workgroup budget, 0 
 void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)
 void lu_set_outer_nb(LuWork*, int nb);
+// one-kernel triangular solves: FP64 up to max64 right-hand sides; FP32 beyond 8 in chunks (-1: keep)
+void lu_set_trsv_limits(LuWork*, int max64, int chunks32);
+int lu_get_trsv_max64(const LuWork*);
+bool lu_get_trsv_chunks32(const LuWork*);
 void lu_set_trsm_lib_panel(LuWork*, bool on);
 bool lu_get_trsm_lib_panel(const LuWork*);
 int lu_get_outer_nb(const LuWork*);

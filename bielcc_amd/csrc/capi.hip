@@ -589,6 +589,16 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_fused_panel(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "trsv_max64") {
+        if (value < 0 || value > 8) return fail(ctx, BIELCC_ERR_ARG, "trsv_max64 must be 0..8");
+        lu_set_trsv_limits(ctx->lu, value, -1);
+        return BIELCC_OK;
+    }
+    if (k == "trsv_chunks32") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "trsv_chunks32 must be 0 or 1");
+        lu_set_trsv_limits(ctx->lu, -1, value);
+        return BIELCC_OK;
+    }
     if (k == "lu_trsm_lib") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "lu_trsm_lib must be 0 or 1");
         lu_set_trsm_lib_panel(ctx->lu, value == 1);
@@ -663,6 +673,8 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu) ? 1 : 0;
+    else if (k == "trsv_max64") *value = lu_get_trsv_max64(ctx->lu);
+    else if (k == "trsv_chunks32") *value = lu_get_trsv_chunks32(ctx->lu) ? 1 : 0;
     else if (k == "trsv_expired") *value = lu_trsv_retries(ctx->lu, 0);
     else if (k == "trsv_ticket_faults") *value = lu_trsv_retries(ctx->lu, 1);
     else if (k == "lu_panel_wait_us") *value = lu_get_panel_wait_us(ctx->lu);

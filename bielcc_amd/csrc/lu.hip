@@ -95,6 +95,9 @@ struct LuWork {
     int dev = 0, n_cu = 0, clk_khz = 0;
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
     int outer_nb = 0;               // columns per outer block (two-level LU); kNB: one level; 0: by order
+    int trsv_max64 = 1;             // FP64 solves with at most this many right-hand sides use the one-kernel solve
+    bool trsv_chunks32 = false;     // FP32 solves with more than kTrsvMaxR right-hand sides: one-kernel solves
+                                    // in chunks (else rocBLAS trsm)
     bool lu_trsm_lib = true;        // the factorisation's U12 triangles on rocBLAS trsm (else k_trsm): FP64 LU
                                     // 3-9% faster at orders 3,000-40,000, FP32 within 1%
     int trsv_expired = 0;           // sync-free solves redone on the blocked path: a wait expired
@@ -114,6 +117,12 @@ void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
 void lu_set_outer_nb(LuWork* w, int nb) { w->outer_nb = nb; }
+void lu_set_trsv_limits(LuWork* w, int max64, int chunks32) {
+    if (max64 >= 0) w->trsv_max64 = max64;
+    if (chunks32 >= 0) w->trsv_chunks32 = chunks32 != 0;
+}
+int lu_get_trsv_max64(const LuWork* w) { return w->trsv_max64; }
+bool lu_get_trsv_chunks32(const LuWork* w) { return w->trsv_chunks32; }
 void lu_set_trsm_lib_panel(LuWork* w, bool on) { w->lu_trsm_lib = on; }
 bool lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
 int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
@@ -1594,7 +1603,20 @@ static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A,
     hipLaunchKernelGGL(k_copy_cols<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, B, ldb, tmp);
     hipLaunchKernelGGL(k_gather_rows<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nrhs, w->perm,
                        tmp, B, ldb);
-    if (nrhs <= kTrsvMaxR && !w->no_trsv) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
+    // The one-kernel solve substitutes block by block; rocBLAS trsm multiplies by inverted
+    // diagonal blocks.  FP64: the one-kernel solve only for one right-hand side (n = 10,816:
+    // 4.1 vs 30 ms; from two on trsm is 1.5-4x faster).  FP32 (refinement): the one-kernel
+    // solve up to kTrsvMaxR right-hand sides, trsm above unless trsv_chunks32 asks for
+    // chunks of the one-kernel solve.
+    const int tmax = sizeof(R) == 8 ? w->trsv_max64 : kTrsvMaxR;
+    if (!w->no_trsv && nrhs <= tmax) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
+    if (!w->no_trsv && sizeof(R) == 4 && w->trsv_chunks32) {
+        for (int c0 = 0; c0 < nrhs; c0 += kTrsvMaxR) {
+            hipError_t ec = trsv_solve<R>(s, w, n, std::min(kTrsvMaxR, nrhs - c0), A, lda, B + (size_t)c0 * ldb, ldb);
+            if (ec != hipSuccess) return ec;
+        }
+        return hipSuccess;
+    }
     if (w->lib_trsm) {
         if ((e = lib_trsm(w, s, 1, n, nrhs, A, lda, B, ldb, msg)) != hipSuccess) return e;
         if ((e = lib_trsm(w, s, 0, n, nrhs, A, lda, B, ldb, msg)) != hipSuccess) return e;

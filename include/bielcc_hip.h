@@ -346,10 +346,12 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
 /* Launch-shape tunables (the autotuner sets these; defaults in brackets):
  * "field_wg" [2048] and "assemble_wg" [2048] = workgroups a tile launch aims
  * for (cells are split until the grid reaches it); "field_rchunk" [4] =
- * current sets per field pass (1, 2 or 4); "trsv" [1] = solves with at most
- * 8 right-hand sides use the one-kernel-per-triangle solve (0: as more);
- * "trsm_lib" [1] = solves with more right-hand sides use rocBLAS trsm per
- * triangle (0: the library's blocked TRSM + GEMM loop); "trsv_spin"
+ * current sets per field pass (1, 2 or 4); "trsv" [1] = small solves use the
+ * one-kernel-per-triangle solve (0: as larger ones): FP64 solves with at most
+ * "trsv_max64" [1] right-hand sides, FP32 solves with at most 8, and, with
+ * "trsv_chunks32" [0] = 1, FP32 solves with more, in chunks of 8;
+ * "trsm_lib" [1] = the other solves use rocBLAS trsm per triangle (0: the
+ * library's blocked TRSM + GEMM loop); "trsv_spin"
  * [1048576] = polls per wait of the one-kernel solve before it gives up (a
  * solve whose wait gave up is redone on the blocked path; tests lower it);
  * "lu_fused_panel" [2] = one launch per LU panel column (1), two (0), or
