@@ -1329,8 +1329,8 @@ static hipError_t lu_perm(hipStream_t s, LuWork* w, int n, const int* piv, size_
 // a launch pair per block column.  Block row ib (64 rows) is owned by one
 // workgroup, which streams the blocks A[ib][jb] of its row in dependency order
 // as soon as x_jb is published, then solves its diagonal block in LDS and
-// publishes x_ib (stores, device-scope fence, release store of the block's
-// flag = epoch).  lower = 1: unit lower L, rows 0, 1, ...; lower = 0: upper U,
+// publishes x_ib (device-scope stores, completed before the block's flag =
+// epoch is stored).  lower = 1: unit lower L, rows 0, 1, ...; lower = 0: upper U,
 // rows nblk-1, nblk-2, ....  Workgroups take block rows in the order of a
 // ticket counter, so a row only ever waits for rows whose workgroups have
 // already started: every wait ends.  Each wait is also bounded (spin_limit
@@ -1504,11 +1504,13 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
         }
 #pragma unroll
         for (int q = 0; q < NR; q++)
-            if (q < nrhs && r < nb) B[(size_t)q * ldb + r0 + r] = y[q];
+            if (q < nrhs && r < nb) st_agent2<R>(&B[(size_t)q * ldb + r0 + r], y[q]);
     }
-    __threadfence();
+    // x_ib goes out with device-scope stores that each wave waits for, so the flag needs
+    // no release fence (a device fence writes back and invalidates this XCD's whole L2)
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The ticket counter, the timeout flag and the ready flags are zeroed on the solve's own
