@@ -1456,13 +1456,14 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
     // y = b + the four column groups' partial sums
     for (int q = 0; q < NR; q++) red[cg][r][q] = acc[q];
     __syncthreads();
-    if (tid < kNB) {
-        // wave 0: lane r holds row r of the tile; x_p is broadcast from lane p (no barriers)
+    {
+        // every wave: lane r holds row r of the tile; wave cg solves the right-hand sides
+        // q with q % 4 == cg (a wave-uniform choice); x_p is broadcast from lane p
         c2<R> y[NR];
 #pragma unroll
         for (int q = 0; q < NR; q++) {
             y[q] = mk2<R>(R(0), R(0));
-            if (q < nrhs && r < nb) {
+            if ((q & 3) == cg && q < nrhs && r < nb) {
                 y[q] = B[(size_t)q * ldb + r0 + r];
                 for (int g = 0; g < 4; g++) { y[q].x += red[g][r][q].x; y[q].y += red[g][r][q].y; }
             }
@@ -1473,7 +1474,7 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
                 const bool upd = r > p && r < nb;
 #pragma unroll
                 for (int q = 0; q < NR; q++) {
-                    if (q < nrhs) {
+                    if ((q & 3) == cg && q < nrhs) {
                         const c2<R> x = mk2<R>(bcast_lane(y[q].x, p), bcast_lane(y[q].y, p));
                         if (upd) {
                             y[q].x -= l.x * x.x - l.y * x.y;
@@ -1490,7 +1491,7 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
                 const c2<R> u = T[r][p];
 #pragma unroll
                 for (int q = 0; q < NR; q++) {
-                    if (q < nrhs) {
+                    if ((q & 3) == cg && q < nrhs) {
                         const c2<R> b = mk2<R>(bcast_lane(y[q].x, p), bcast_lane(y[q].y, p));
                         const c2<R> x = mk2<R>(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
                         if (r == p) y[q] = x;
@@ -1504,7 +1505,7 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
         }
 #pragma unroll
         for (int q = 0; q < NR; q++)
-            if (q < nrhs && r < nb) st_agent2<R>(&B[(size_t)q * ldb + r0 + r], y[q]);
+            if ((q & 3) == cg && q < nrhs && r < nb) st_agent2<R>(&B[(size_t)q * ldb + r0 + r], y[q]);
     }
     // x_ib goes out with device-scope stores that each wave waits for, so the flag needs
     // no release fence (a device fence writes back and invalidates this XCD's whole L2)
