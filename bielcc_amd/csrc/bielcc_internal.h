@@ -225,6 +225,9 @@ void lu_set_outer_nb(LuWork*, int nb);
 // one-kernel triangular solves: FP64 up to max64 right-hand sides; FP32 beyond 8 in chunks (-1: keep)
 void lu_set_trsv_limits(LuWork*, int max64, int chunks32);
 int lu_get_trsv_max64(const LuWork*);
+// refinement: also stop only once cond_est x residual <= ferr (0: residual only)
+void lu_set_refine_ferr(LuWork*, double ferr);
+double lu_get_refine_ferr(const LuWork*);
 bool lu_get_trsv_chunks32(const LuWork*);
 void lu_set_trsm_lib_panel(LuWork*, bool on);
 bool lu_get_trsm_lib_panel(const LuWork*);

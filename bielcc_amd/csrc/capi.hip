@@ -589,6 +589,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_fused_panel(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "refine_ferr_exp") {
+        if (value < 0 || value > 15) return fail(ctx, BIELCC_ERR_ARG, "refine_ferr_exp must be 0..15");
+        lu_set_refine_ferr(ctx->lu, value == 0 ? 0. : std::pow(10., -value));
+        return BIELCC_OK;
+    }
     if (k == "trsv_max64") {
         if (value < 0 || value > 8) return fail(ctx, BIELCC_ERR_ARG, "trsv_max64 must be 0..8");
         lu_set_trsv_limits(ctx->lu, value, -1);
@@ -674,6 +679,10 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu) ? 1 : 0;
     else if (k == "trsv_max64") *value = lu_get_trsv_max64(ctx->lu);
+    else if (k == "refine_ferr_exp") {
+        const double f = lu_get_refine_ferr(ctx->lu);
+        *value = f > 0. ? (int)std::lround(-std::log10(f)) : 0;
+    }
     else if (k == "trsv_chunks32") *value = lu_get_trsv_chunks32(ctx->lu) ? 1 : 0;
     else if (k == "trsv_expired") *value = lu_trsv_retries(ctx->lu, 0);
     else if (k == "trsv_ticket_faults") *value = lu_trsv_retries(ctx->lu, 1);

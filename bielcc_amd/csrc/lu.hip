@@ -42,6 +42,8 @@ constexpr int kFusedMaxN = 6144;
 // Look-ahead pays only where the trailing GEMM is long and the first panels are too tall for
 // the whole-panel kernel: FP64 above this order (n = 28,000: -3%; n <= 12,000 and FP32: slower).
 constexpr int kLookAheadMinN = 20000;
+// relative residual of an FP64 solve at roundoff: refinement needs no error estimate below it
+constexpr double kResidFloor = 1e-14;
 // From this order on the LU is blocked in two levels (outer blocks of 512 columns).
 constexpr int kTwoLevelMinN = 16000;
 // candidate buffers of the whole-panel kernel: cval, cidx [2][512], crow [2][512][64], rowj [2][64] (FP64 sizes)
@@ -96,6 +98,7 @@ struct LuWork {
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
     int outer_nb = 0;               // columns per outer block (two-level LU); kNB: one level; 0: by order
     int trsv_max64 = 1;             // FP64 solves with at most this many right-hand sides use the one-kernel solve
+    double refine_ferr = 0.;        // refinement also stops on an error estimate below this (0: residual only)
     bool trsv_chunks32 = false;     // FP32 solves with more than kTrsvMaxR right-hand sides: one-kernel solves
                                     // in chunks (else rocBLAS trsm)
     bool lu_trsm_lib = true;        // the factorisation's U12 triangles on rocBLAS trsm (else k_trsm): FP64 LU
@@ -122,6 +125,8 @@ void lu_set_trsv_limits(LuWork* w, int max64, int chunks32) {
     if (chunks32 >= 0) w->trsv_chunks32 = chunks32 != 0;
 }
 int lu_get_trsv_max64(const LuWork* w) { return w->trsv_max64; }
+void lu_set_refine_ferr(LuWork* w, double ferr) { w->refine_ferr = ferr; }
+double lu_get_refine_ferr(const LuWork* w) { return w->refine_ferr; }
 bool lu_get_trsv_chunks32(const LuWork* w) { return w->trsv_chunks32; }
 void lu_set_trsm_lib_panel(LuWork* w, bool on) { w->lu_trsm_lib = on; }
 bool lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
@@ -1765,7 +1770,7 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long 
         hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rf, (long long)n);
         if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
-        double prev = HUGE_VAL;
+        double prev = HUGE_VAL, rho_min = HUGE_VAL;
         for (int it = 0;; it++) {
             // Rd = B0 - A X (FP64)
             hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
@@ -1779,7 +1784,19 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long 
             for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
             *resid = rel;
             if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }   // NaN / Inf: FP32 factors broke down
-            if (rel <= tol) return hipSuccess;
+            if (it > 0) rho_min = std::min(rho_min, rel / prev);
+            if (rel <= tol) {
+                // Optional forward-error stop: a correction shrinks the residual by about
+                // cond(A) u32, so rho / u32 estimates cond(A) (rho: the smallest ratio of a
+                // residual to the one before; near the FP64 floor the ratio tends to 1 and
+                // says nothing), and cond(A) x residual the error left in X.  Refine on
+                // until that estimate is below ferr as well, or the residual reaches the
+                // FP64 floor; a system that cannot get there stalls or diverges below and
+                // goes to FP64 factors.
+                if (w->refine_ferr <= 0. || it == 0 || rel <= kResidFloor) return hipSuccess;
+                const double cond_est = std::max(1., rho_min / 5.96e-8);
+                if (cond_est * rel <= w->refine_ferr) return hipSuccess;
+            }
             if (it >= max_iter) { *status = 1; return hipSuccess; }
             // stagnation or divergence: a correction must shrink the residual by 10%.  Near an
             // interior resonance the contraction factor cond(A) u_fp32 approaches 1 and each

@@ -351,7 +351,10 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * "trsv_max64" [1] right-hand sides, FP32 solves with at most 8, and, with
  * "trsv_chunks32" [0] = 1, FP32 solves with more, in chunks of 8;
  * "trsm_lib" [1] = the other solves use rocBLAS trsm per triangle (0: the
- * library's blocked TRSM + GEMM loop); "trsv_spin"
+ * library's blocked TRSM + GEMM loop); "refine_ferr_exp" [0] = e > 0: the
+ * refinement also requires an estimated forward error (cond estimated from
+ * the residual's contraction, times the residual) below 10^-e before it
+ * stops; "trsv_spin"
  * [1048576] = polls per wait of the one-kernel solve before it gives up (a
  * solve whose wait gave up is redone on the blocked path; tests lower it);
  * "lu_fused_panel" [2] = one launch per LU panel column (1), two (0), or

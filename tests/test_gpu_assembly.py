@@ -421,3 +421,31 @@ def test_two_level_lu_matches_one_level(ctx, n):
         assert np.abs(out[0] - ref[0]).max() <= 1e-14 * n * np.abs(ref[0]).max()
         assert backward(out[0], out[1]) <= n * 1e-15
         assert backward(out[2], out[3]) <= n * 1e-6
+
+
+def test_refine_forward_error_stop(ctx):
+    """refine_ferr_exp = e: the refinement goes on past the residual target until the
+    estimated forward error (cond estimated from the residual's contraction, times
+    the residual) is below 10^-e; the solution then matches the FP64 solve at least
+    as closely, with at least as many corrections."""
+    n = 600
+    rng = np.random.default_rng(18)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2 * np.sqrt(n) * np.eye(n)
+    b = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    Xr = np.linalg.solve(A, b)
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.lu32_factor(dA, n, n, dF, n, piv)
+    out = {}
+    try:
+        for e in (0, 11):
+            ctx.set_tuning("refine_ferr_exp", e)
+            dB = ctx.to_device(np.ascontiguousarray(b.T))
+            it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 3, n, 1e-8, 30)
+            X = dB.to_numpy((3, n), np.complex128).T
+            out[e] = (it, res, np.linalg.norm(X - Xr) / np.linalg.norm(Xr))
+    finally:
+        ctx.set_tuning("refine_ferr_exp", 0)
+    assert ctx.get_tuning("refine_ferr_exp") == 0
+    assert out[11][0] >= out[0][0] and out[11][1] <= out[0][1]
+    assert out[11][2] <= max(out[0][2], 1e-13) and out[11][2] <= 1e-11
