@@ -531,9 +531,8 @@ __global__ __launch_bounds__(256) void k_panel_step(int n, int j, int c0, int c1
 constexpr unsigned kAbortBit = 0x80000000u;
 constexpr int kPanelMaxWG = 512;   // two per compute unit
 
-template <typename R> struct PanelRows;
-template <> struct PanelRows<double> { static constexpr int v = 64; };    // 64 KB of LDS
-template <> struct PanelRows<float> { static constexpr int v = 128; };    // 64 KB of LDS
+// Rows per workgroup of the whole-panel kernel: 64 (FP64: 64 KB of LDS; FP32: 32 KB), or
+// 128 for FP32 panels too tall for 512 workgroups of 64 rows (64 KB).
 
 template <typename R>
 __device__ inline c2<R> ld_agent2(const c2<R>* p) {
@@ -652,13 +651,12 @@ __device__ void panel_one_wg(int n, int k0, int nb, c2<R>* __restrict__ A, long 
 // sync[0] barrier word, sync[1] workgroups gone, sync[2] aborted panels (count).
 // cval/cidx [2][kPanelMaxWG], crow [2][kPanelMaxWG][kNB], rowj [2][kNB]:
 // double-buffered by column parity (a workgroup can be at most one barrier ahead).
-template <typename R>
+template <typename R, int RPW>
 __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
                                                    int* __restrict__ piv, int* __restrict__ info,
                                                    unsigned* __restrict__ sync, double* __restrict__ cval,
                                                    int* __restrict__ cidx, c2<R>* __restrict__ crow,
                                                    c2<R>* __restrict__ rowj, long long wait_ticks) {
-    constexpr int RPW = PanelRows<R>::v;
     constexpr int RPL = RPW / 64;   // rows per lane
     constexpr int NONE = 0x7fffffff;
     __shared__ c2<R> sa[kNB][RPW];  // [column][row]
@@ -1063,14 +1061,16 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     if ((e = hipMemsetAsync(w->info, 0, sizeof(int), s)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(reinterpret_cast<int*>(w->smap + 2 * kNB) + 1, 0, sizeof(int), s)) != hipSuccess) return e;
     const bool fused = w->fused_panel == 1 || (w->fused_panel == 2 && n <= kFusedMaxN);
-    // whole-panel kernel for panels of at most kPanelMaxWG workgroups, two per compute unit
-    constexpr int RPW = PanelRows<R>::v;
+    // whole-panel kernel for panels of at most kPanelMaxWG workgroups, two per compute unit;
+    // 64 rows per workgroup, or 128 for FP32 panels that need more than gmax of 64
     const int gmax = std::min(kPanelMaxWG, 2 * w->n_cu);
+    auto rows_per_wg = [&](int m) { return (sizeof(R) == 4 && (m + 63) / 64 > gmax) ? 128 : 64; };
+    auto wgs_for = [&](int m) { const int r = rows_per_wg(m); return (m + r - 1) / r; };
     LuRunning running(w->dev);
     PanelAdmit admit;
     const bool force = w->panel_lds == 2;
     const bool lds = w->panel_lds != 0 && w->n_cu > 0 &&
-                     (force || admit.take(w->dev, std::min((n + RPW - 1) / RPW, gmax), 2 * w->n_cu));
+                     (force || admit.take(w->dev, std::min(wgs_for(n), gmax), 2 * w->n_cu));
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
     char* pb = static_cast<char*>(w->pbuf);
     double* cval = reinterpret_cast<double*>(pb);
@@ -1083,10 +1083,14 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     auto factor_panel = [&](hipStream_t st, int k0) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
-        const int G = (n - k0 + RPW - 1) / RPW;
+        const int G = wgs_for(n - k0);
         if (lds && G <= gmax && (force || G <= w->n_cu || running.alone())) {
-            hipLaunchKernelGGL(k_panel_lds<R>, dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info, w->psync,
-                               cval, cidx, crow, rowj, wait_ticks);
+            if (rows_per_wg(n - k0) == 64)
+                hipLaunchKernelGGL((k_panel_lds<R, 64>), dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info,
+                                   w->psync, cval, cidx, crow, rowj, wait_ticks);
+            else
+                hipLaunchKernelGGL((k_panel_lds<R, 128>), dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv,
+                                   w->info, w->psync, cval, cidx, crow, rowj, wait_ticks);
             hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, w->smap, smap_count,
                                smap_count + 1);
             return;
