@@ -846,16 +846,17 @@ __global__ __launch_bounds__(64) void k_swap_map(int n, int k0, int nb, const in
     if (t == 0) *count = nb + nout;
 }
 
-// Apply a panel's interchanges (k_swap_map) to columns [cbeg, cend): one wave per
-// column reads the rows of R it needs, then writes them to their new places (the
-// panel's 64 rows are contiguous, so half the accesses coalesce; the old
-// one-lane-per-column sweep touched a different column with every lane).  A wave's
-// loads and stores of one address stay in program order.
+// Apply a panel's interchanges (k_swap_map) to the columns outside the panel,
+// [0, k0) and [c1, n): one wave per column reads the rows of R it needs, then
+// writes them to their new places (the panel's 64 rows are contiguous, so half the
+// accesses coalesce; the old one-lane-per-column sweep touched a different column
+// with every lane).  A wave's loads and stores of one address stay in program order.
 template <typename R>
-__global__ __launch_bounds__(256) void k_row_gather(int cbeg, int cend, c2<R>* __restrict__ A, long long lda,
+__global__ __launch_bounds__(256) void k_row_gather(int k0, int c1, int n, c2<R>* __restrict__ A, long long lda,
                                                     const int2* __restrict__ map, const int* __restrict__ count) {
-    const int c = cbeg + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= cend) return;   // whole waves
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= k0 + (n - c1)) return;   // whole waves
+    const int c = j < k0 ? j : c1 + (j - k0);
     const int lane = threadIdx.x & 63, cnt = *count;
     c2<R>* col = A + (size_t)c * lda;
     int2 m0 = make_int2(0, 0), m1 = make_int2(0, 0);
@@ -1143,12 +1144,9 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                 const int c1 = k0 + nb;
                 factor_panel(s, k0);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
-                if (k0 > 0)
-                    hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + 3) / 4), dim3(256), 0, s, 0, k0, A, lda, w->smap,
-                                       smap_count);
-                if (c1 < n)
-                    hipLaunchKernelGGL(k_row_gather<R>, dim3((n - c1 + 3) / 4), dim3(256), 0, s, c1, n, A, lda,
-                                       w->smap, smap_count);
+                if (k0 + (n - c1) > 0)
+                    hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + (n - c1) + 3) / 4), dim3(256), 0, s, k0, c1, n, A,
+                                       lda, w->smap, smap_count);
                 if (c1 < KE) {   // the block's remaining columns
                     if ((e = panel_trsm<R>(w, s, nb, A, lda, k0, c1, KE, msg)) != hipSuccess) return e;
                     e = gemm_update(w, s, n - c1, KE - c1, nb, A + (size_t)k0 * lda + c1, lda,
@@ -1180,12 +1178,10 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (k0 > 0)
-            hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + 3) / 4), dim3(256), 0, s, 0, k0, A, lda, w->smap,
-                               smap_count);
+        if (k0 + (n - c1) > 0)
+            hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + (n - c1) + 3) / 4), dim3(256), 0, s, k0, c1, n, A, lda,
+                               w->smap, smap_count);
         if (c1 >= n) break;
-        hipLaunchKernelGGL(k_row_gather<R>, dim3((n - c1 + 3) / 4), dim3(256), 0, s, c1, n, A, lda, w->smap,
-                           smap_count);
         if ((e = panel_trsm<R>(w, s, nb, A, lda, k0, c1, n, msg)) != hipSuccess) return e;
         const c2<R>* L21 = A + (size_t)k0 * lda + c1;
         const int c2n = min(n, c1 + kNB);
