@@ -223,8 +223,9 @@ void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)
 void lu_set_outer_nb(LuWork*, int nb);
 // one-kernel triangular solves: FP64 up to max64 right-hand sides; FP32 beyond 8 in chunks (-1: keep)
-void lu_set_trsv_limits(LuWork*, int max64, int chunks32);
+void lu_set_trsv_limits(LuWork*, int max64, int chunks32, int max32 = -1);
 int lu_get_trsv_max64(const LuWork*);
+int lu_get_trsv_max32(const LuWork*);
 // refinement: also stop only once cond_est x residual <= ferr (0: residual only)
 void lu_set_refine_ferr(LuWork*, double ferr);
 double lu_get_refine_ferr(const LuWork*);

@@ -553,9 +553,15 @@ int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, c
     if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
     t.stop();
     if (status != 0) {
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "refinement stopped at relative residual %.3e > tol %.3e after %d corrections",
-                      *resid, tol, *iters);
+        char buf[200];
+        if (*resid <= tol)   // the residual met tol; the forward-error estimate did not reach refine_ferr
+            std::snprintf(buf, sizeof buf,
+                          "refinement stalled at relative residual %.3e (<= tol %.3e) before the estimated forward "
+                          "error reached %.1e, after %d corrections",
+                          *resid, tol, lu_get_refine_ferr(ctx->lu), *iters);
+        else
+            std::snprintf(buf, sizeof buf, "refinement stopped at relative residual %.3e > tol %.3e after %d corrections",
+                          *resid, tol, *iters);
         return fail(ctx, BIELCC_ERR_UNSUPPORTED, buf);
     }
     return BIELCC_OK;
@@ -597,6 +603,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
     if (k == "trsv_max64") {
         if (value < 0 || value > 8) return fail(ctx, BIELCC_ERR_ARG, "trsv_max64 must be 0..8");
         lu_set_trsv_limits(ctx->lu, value, -1);
+        return BIELCC_OK;
+    }
+    if (k == "trsv_max32") {
+        if (value < 0 || value > 16) return fail(ctx, BIELCC_ERR_ARG, "trsv_max32 must be 0..16");
+        lu_set_trsv_limits(ctx->lu, -1, -1, value);
         return BIELCC_OK;
     }
     if (k == "trsv_chunks32") {
@@ -679,6 +690,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu) ? 1 : 0;
     else if (k == "trsv_max64") *value = lu_get_trsv_max64(ctx->lu);
+    else if (k == "trsv_max32") *value = lu_get_trsv_max32(ctx->lu);
     else if (k == "refine_ferr_exp") {
         const double f = lu_get_refine_ferr(ctx->lu);
         *value = f > 0. ? (int)std::lround(-std::log10(f)) : 0;

@@ -98,8 +98,9 @@ struct LuWork {
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
     int outer_nb = 0;               // columns per outer block (two-level LU); kNB: one level; 0: by order
     int trsv_max64 = 1;             // FP64 solves with at most this many right-hand sides use the one-kernel solve
+    int trsv_max32 = 16;            // FP32 solves likewise
     double refine_ferr = 0.;        // refinement also stops on an error estimate below this (0: residual only)
-    bool trsv_chunks32 = false;     // FP32 solves with more than kTrsvMaxR right-hand sides: one-kernel solves
+    bool trsv_chunks32 = false;     // FP32 solves with more than 16 right-hand sides: one-kernel solves
                                     // in chunks (else rocBLAS trsm)
     bool lu_trsm_lib = true;        // the factorisation's U12 triangles on rocBLAS trsm (else k_trsm): FP64 LU
                                     // 3-9% faster at orders 3,000-40,000, FP32 within 1%
@@ -120,11 +121,13 @@ void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
 void lu_set_outer_nb(LuWork* w, int nb) { w->outer_nb = nb; }
-void lu_set_trsv_limits(LuWork* w, int max64, int chunks32) {
+void lu_set_trsv_limits(LuWork* w, int max64, int chunks32, int max32) {
     if (max64 >= 0) w->trsv_max64 = max64;
+    if (max32 >= 0) w->trsv_max32 = max32;
     if (chunks32 >= 0) w->trsv_chunks32 = chunks32 != 0;
 }
 int lu_get_trsv_max64(const LuWork* w) { return w->trsv_max64; }
+int lu_get_trsv_max32(const LuWork* w) { return w->trsv_max32; }
 void lu_set_refine_ferr(LuWork* w, double ferr) { w->refine_ferr = ferr; }
 double lu_get_refine_ferr(const LuWork* w) { return w->refine_ferr; }
 bool lu_get_trsv_chunks32(const LuWork* w) { return w->trsv_chunks32; }
@@ -1330,7 +1333,7 @@ static hipError_t lu_perm(hipStream_t s, LuWork* w, int n, const int* piv, size_
 }
 
 // ---------------------------------------------------------------- sync-free TRSV
-// Few right-hand sides (nrhs <= kTrsvMaxR): one kernel per triangle instead of
+// Few right-hand sides (see kTrsvMaxR below): one kernel per triangle instead of
 // a launch pair per block column.  Block row ib (64 rows) is owned by one
 // workgroup, which streams the blocks A[ib][jb] of its row in dependency order
 // as soon as x_jb is published, then solves its diagonal block in LDS and
@@ -1341,8 +1344,10 @@ static hipError_t lu_perm(hipStream_t s, LuWork* w, int n, const int* piv, size_
 // already started: every wait ends.  Each wait is also bounded (spin_limit
 // polls); on expiry the workgroup raises *err and carries on, every other
 // wait then gives up too, so the kernel always drains, and the host redoes
-// the solve on the blocked path.
+// the solve on the blocked path.  FP64: up to kTrsvMaxR right-hand sides (the
+// 16-wide FP64 tile would need 146 KB of LDS); FP32: up to kTrsvMaxR32 (74 KB).
 constexpr int kTrsvMaxR = 8;
+constexpr int kTrsvMaxR32 = 16;
 
 template <typename R>
 __device__ inline c2<R> load_relaxed(const c2<R>* p) {
@@ -1565,7 +1570,8 @@ static hipError_t trsv_solve(hipStream_t s, LuWork* w, int n, int nrhs, const c2
         if (nrhs <= 1) trsv_launch<R, 1>(s, w, n, nrhs, lower, A, lda, B, ldb);
         else if (nrhs <= 2) trsv_launch<R, 2>(s, w, n, nrhs, lower, A, lda, B, ldb);
         else if (nrhs <= 4) trsv_launch<R, 4>(s, w, n, nrhs, lower, A, lda, B, ldb);
-        else trsv_launch<R, 8>(s, w, n, nrhs, lower, A, lda, B, ldb);
+        else if (nrhs <= 8) trsv_launch<R, 8>(s, w, n, nrhs, lower, A, lda, B, ldb);
+        else if constexpr (sizeof(R) == 4) trsv_launch<R, kTrsvMaxR32>(s, w, n, nrhs, lower, A, lda, B, ldb);
         if ((e = hipGetLastError()) != hipSuccess) {
             // the kernel did not run: bring the ticket counter back in step
             (void)hipMemsetAsync(w->ticket, 0, sizeof(unsigned long long), s);
@@ -1614,13 +1620,14 @@ static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A,
     // The one-kernel solve substitutes block by block; rocBLAS trsm multiplies by inverted
     // diagonal blocks.  FP64: the one-kernel solve only for one right-hand side (n = 10,816:
     // 4.1 vs 30 ms; from two on trsm is 1.5-4x faster).  FP32 (refinement): the one-kernel
-    // solve up to kTrsvMaxR right-hand sides, trsm above unless trsv_chunks32 asks for
-    // chunks of the one-kernel solve.
-    const int tmax = sizeof(R) == 8 ? w->trsv_max64 : kTrsvMaxR;
+    // solve up to kTrsvMaxR32 right-hand sides (trsm issues two dependent GEMMs per 128
+    // rows of each triangle), trsm above unless trsv_chunks32 asks for chunks of the
+    // one-kernel solve.
+    const int tmax = sizeof(R) == 8 ? w->trsv_max64 : w->trsv_max32;
     if (!w->no_trsv && nrhs <= tmax) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
     if (!w->no_trsv && sizeof(R) == 4 && w->trsv_chunks32) {
-        for (int c0 = 0; c0 < nrhs; c0 += kTrsvMaxR) {
-            hipError_t ec = trsv_solve<R>(s, w, n, std::min(kTrsvMaxR, nrhs - c0), A, lda, B + (size_t)c0 * ldb, ldb);
+        for (int c0 = 0; c0 < nrhs; c0 += kTrsvMaxR32) {
+            hipError_t ec = trsv_solve<R>(s, w, n, std::min(kTrsvMaxR32, nrhs - c0), A, lda, B + (size_t)c0 * ldb, ldb);
             if (ec != hipSuccess) return ec;
         }
         return hipSuccess;

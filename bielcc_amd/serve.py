@@ -12,7 +12,8 @@ solver or server; this is the request loop around the path:
 - resident state: meshes stay uploaded per worker; the factors of the
   collocation matrix stay on the device per (mesh, k) in an LRU cache
   (FP64 LU in the "fp64" form; the FP64 matrix plus its FP32 LU in the
-  "mixed" form, whose solves refine in FP64 to `refine_tol`).  Systems too
+  "mixed" form, whose solves refine in FP64 to `refine_tol` and to an
+  estimated forward error below 10^-`refine_ferr_exp`).  Systems too
   ill-conditioned for FP32 factors (near an interior resonance of the closed
   body, where refinement stalls or the FP32 LU meets a zero pivot) get FP64
   factors once, beside the FP32 ones, and are solved with them from then on;
@@ -214,10 +215,18 @@ class Server:
 
     meshes: name -> vertex array [n][3|4][3].  form: "fp64" (FP64 LU) or
     "mixed" (FP32 LU + FP64 refinement, FP32 far-pair field sums).
+
+    refine_ferr_exp = e > 0: a refinement stops only once its residual is below
+    refine_tol and its estimated forward error (cond(Z) from the residual's
+    contraction per correction, times the residual) is below 10^-e as well.  A
+    residual target alone does not bound the error of an ill-conditioned system:
+    refined to a 1e-10 residual, a low-frequency elongated ellipsoid (cond about
+    4e6) kept a 1.9e-5 field error.  Such systems then stall or reach the FP64
+    floor; stalled ones get FP64 factors.  0: residual only.
     """
 
     def __init__(self, meshes: dict, form: str = "mixed", workers: int = 3, hbm_budget: float = 200e9,
-                 num_param=None, refine_tol: float = 1e-10, device: int = 0):
+                 num_param=None, refine_tol: float = 1e-10, device: int = 0, refine_ferr_exp: int = 7):
         if form not in ("fp64", "mixed"):
             raise ValueError("form must be 'fp64' or 'mixed'")
         self.meshes = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in meshes.items()}
@@ -233,6 +242,8 @@ class Server:
         self.lock = threading.Lock()
         self.done_cv = threading.Condition(self.lock)
         self.ctxs = [api.Context(device) for _ in range(workers)]
+        for c in self.ctxs:
+            c.set_tuning("refine_ferr_exp", int(refine_ferr_exp))
         self.dev_meshes = [dict() for _ in range(workers)]
         self.pools = [dict() for _ in range(workers)]   # name -> DeviceArray, grow-only
         self.scratch_hw = [0] * workers   # bytes charged for the context's own grow-only scratch

@@ -191,6 +191,61 @@ def test_sync_free_solve_recovers_from_an_expired_wait(ctx):
         ctx.set_tuning("trsv_spin", 1 << 20)
 
 
+def _fp32_first_solve(ctx, A, Bm, trsv):
+    """X0 = F^-1 B of the refinement (FP32 factors, one FP32 solve: no correction
+    allowed, so the refinement stops with code 4 and leaves X0), with the factors
+    and pivots it used."""
+    n, nr = A.shape[0], Bm.shape[1]
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dB = ctx.to_device(np.ascontiguousarray(Bm.T))
+    dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.lu32_factor(dA, n, n, dF, n, piv)
+    ctx.set_tuning("trsv", trsv)
+    try:
+        with pytest.raises(B.BielccError) as e:
+            ctx.refine_solve(dA, n, n, dF, n, piv, dB, nr, n, 1e-30, 0)
+        assert e.value.code == 4
+    finally:
+        ctx.set_tuning("trsv", 1)
+    return (dB.to_numpy((nr, n), np.complex128).T, dF.to_numpy((n, n), np.complex64).T,
+            piv.to_numpy((n,), np.int32))
+
+
+@pytest.mark.parametrize("nrhs", [9, 12, 16, 17])
+def test_fp32_solve_up_to_16_rhs_in_one_kernel(ctx, nrhs):
+    """FP32 solves of 9 to 16 right-hand sides take the one-kernel solve (17: rocBLAS
+    trsm).  Its X0 against the same FP32 factors and pivots substituted in FP64, and
+    against the trsm path; ragged last blocks."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(9)
+    for n in (64, 700, 1000):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 4 * np.sqrt(n) * np.eye(n)
+        Bm = rng.normal(size=(n, nrhs)) + 1j * rng.normal(size=(n, nrhs))
+        X1, F, piv = _fp32_first_solve(ctx, A, Bm, 1)
+        X0, F0, piv0 = _fp32_first_solve(ctx, A, Bm, 0)
+        assert np.array_equal(F, F0) and np.array_equal(piv, piv0)
+        Y = Bm.copy()
+        for j in range(n):   # the factorisation's interchanges, in order
+            Y[[j, piv[j]]] = Y[[piv[j], j]]
+        F64 = F.astype(np.complex128)
+        Y = sl.solve_triangular(F64, Y, lower=True, unit_diagonal=True)
+        Xf = sl.solve_triangular(F64, Y, lower=False)
+        for X in (X1, X0):
+            assert np.linalg.norm(X - Xf, axis=0).max() / np.linalg.norm(Xf, axis=0).min() <= 1e-5, n
+        assert np.linalg.norm(X1 - X0) <= 1e-5 * np.linalg.norm(Xf)
+        # and refined to 1e-12 on both paths
+        for trsv in (1, 0):
+            ctx.set_tuning("trsv", trsv)
+            dA = ctx.to_device(np.ascontiguousarray(A.T))
+            dB = ctx.to_device(np.ascontiguousarray(Bm.T))
+            dF, dp = ctx.empty(n * n * 8), ctx.empty(4 * n)
+            ctx.lu32_factor(dA, n, n, dF, n, dp)
+            it, res = ctx.refine_solve(dA, n, n, dF, n, dp, dB, nrhs, n, 1e-12, 10)
+            X = dB.to_numpy((nrhs, n), np.complex128).T
+            assert res <= 1e-12 and np.linalg.norm(A @ X - Bm) / np.linalg.norm(Bm) <= 1e-12, (n, trsv)
+        ctx.set_tuning("trsv", 1)
+
+
 @pytest.mark.parametrize("n", [65, 300, 1000])
 def test_fused_panel_matches_two_launch_panel(ctx, n):
     """One launch per panel column (next column's pivot by the last workgroup)
