@@ -98,7 +98,7 @@ struct LuWork {
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
     int outer_nb = 0;               // columns per outer block (two-level LU); kNB: one level; 0: by order
     int trsv_max64 = 1;             // FP64 solves with at most this many right-hand sides use the one-kernel solve
-    int trsv_max32 = 16;            // FP32 solves likewise
+    int trsv_max32 = 8;             // FP32 solves likewise (up to kTrsvMaxR32)
     double refine_ferr = 0.;        // refinement also stops on an error estimate below this (0: residual only)
     bool trsv_chunks32 = false;     // FP32 solves with more than 16 right-hand sides: one-kernel solves
                                     // in chunks (else rocBLAS trsm)
@@ -1440,23 +1440,36 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
             ready = s_ready;
         }
         const int c0 = jb * kNB, ncb = min(kNB, n - c0);
-        for (int e = tid; e < ncb * nrhs; e += 256) {
-            const int c = e % ncb, q = e / ncb;
-            X[c][q] = load_relaxed<R>(B + (size_t)q * ldb + c0 + c);
+        // x_jb: every load of this thread is issued before any is waited for (a rolled
+        // loop waits for each load before its LDS store: one memory round trip per
+        // 256 values on the critical path, four at 16 right-hand sides)
+        // Value u of a thread is row r (its lane) of right-hand side cg + 4u.  Sets past
+        // nrhs are stored as zeros, so the product below needs no branch per set: a branch
+        // per term puts every LDS read behind its own wait.
+        constexpr int XPT = (kNB * NR + 255) / 256;
+        c2<R> xv[XPT];
+#pragma unroll
+        for (int u = 0; u < XPT; u++) {
+            const int q = cg + 4 * u;
+            xv[u] = mk2<R>(R(0), R(0));
+            if (r < ncb && q < nrhs) xv[u] = load_relaxed<R>(B + (size_t)q * ldb + c0 + r);
+        }
+#pragma unroll
+        for (int u = 0; u < XPT; u++) {
+            const int q = cg + 4 * u;
+            if (q < NR) X[r][q] = xv[u];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kNB / 4; k++) {
             const int c = cg + 4 * k;
-            if (c < ncb) {
+            if (c < ncb) {   // uniform in the wave
                 const c2<R> a = av[k];
 #pragma unroll
                 for (int q = 0; q < NR; q++) {
-                    if (q < nrhs) {
-                        const c2<R> x = X[c][q];
-                        acc[q].x -= a.x * x.x - a.y * x.y;
-                        acc[q].y -= a.x * x.y + a.y * x.x;
-                    }
+                    const c2<R> x = X[c][q];
+                    acc[q].x -= a.x * x.x - a.y * x.y;
+                    acc[q].y -= a.x * x.y + a.y * x.x;
                 }
             }
         }
@@ -1620,9 +1633,10 @@ static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A,
     // The one-kernel solve substitutes block by block; rocBLAS trsm multiplies by inverted
     // diagonal blocks.  FP64: the one-kernel solve only for one right-hand side (n = 10,816:
     // 4.1 vs 30 ms; from two on trsm is 1.5-4x faster).  FP32 (refinement): the one-kernel
-    // solve up to kTrsvMaxR32 right-hand sides (trsm issues two dependent GEMMs per 128
-    // rows of each triangle), trsm above unless trsv_chunks32 asks for chunks of the
-    // one-kernel solve.
+    // solve up to trsv_max32 right-hand sides (8; at most kTrsvMaxR32), trsm above unless
+    // trsv_chunks32 asks for chunks of the one-kernel solve.  Measured (refinements with
+    // two corrections, 11,616 unknowns): 1 set 15 vs 53 ms, 8 sets 30 vs 35 ms, 12 and
+    // 16 sets 42 and 46 vs 35 ms; trsm's time hardly grows with the sets.
     const int tmax = sizeof(R) == 8 ? w->trsv_max64 : w->trsv_max32;
     if (!w->no_trsv && nrhs <= tmax) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
     if (!w->no_trsv && sizeof(R) == 4 && w->trsv_chunks32) {
