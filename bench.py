@@ -419,7 +419,78 @@ def plain_fp64_run(pool, queries):
             "seconds": {k: round(v, 3) for k, v in phases.items()}}
 
 
-def serving_leg(nq=96, workers=4, refine_tol=1e-10):
+def _clock_ns():
+    """Host clocks a profiler's kernel timestamps may be on (which one is found by
+    trace_summary)."""
+    return {"mono": time.monotonic_ns(), "boot": time.clock_gettime_ns(time.CLOCK_BOOTTIME)}
+
+
+def trace_summary(trace_csv, windows_path, top=16):
+    """Kernel activity of each recorded window of a run, from a rocprofv3 kernel trace
+    (--kernel-trace --output-format csv): wall time, GPU busy time (union of the kernel
+    intervals), the sum of kernel durations over it (kernels in flight on average), the
+    kernels with the largest total time, and the whole-panel LU kernel's average duration
+    by its number of workgroups."""
+    import csv
+    import collections
+    import re
+    win = json.load(open(windows_path))
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"],
+                         int(r["Grid_Size_X"]) // max(1, int(r["Workgroup_Size_X"]))))
+    rows.sort()
+    lo, hi = rows[0][0], rows[-1][1]
+    first = next(iter(win.values()))[0]
+    key = min(first, key=lambda k: 0 if lo - 60e9 < first[k] < hi + 60e9 else 1)
+
+    def short(n):
+        n = re.sub(r"\(.*", "", n).replace("void ", "").replace("bielcc_impl::", "")
+        return (n[:40] if n.startswith("Cijk_") else n)[:60]
+
+    out = [f"{len(rows)} kernels in the trace"]
+    for name, (a, b) in win.items():
+        t0, t1 = a[key], b[key]
+        ks = [r for r in rows if t0 <= r[0] < t1]
+        if not ks:
+            continue
+        busy, cur_s, cur_e, tot = 0, None, None, 0
+        for st, en, _, _ in ks:
+            tot += en - st
+            if cur_e is None or st > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = st, en
+            else:
+                cur_e = max(cur_e, en)
+        busy += cur_e - cur_s
+        wall = t1 - t0
+        out.append(f"\n{name}: wall {wall / 1e9:.3f} s, {len(ks)} kernels, GPU busy {busy / 1e9:.3f} s "
+                   f"({busy / wall:.0%}), kernel time {tot / 1e9:.3f} s ({tot / busy:.2f} in flight on average)")
+        out.append("| kernel | launches | total (s) | average (us) |")
+        out.append("|---|---|---|---|")
+        agg = collections.defaultdict(lambda: [0, 0])
+        for st, en, n, _ in ks:
+            x = agg[short(n)]
+            x[0] += 1
+            x[1] += en - st
+        for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+            out.append(f"| `{n}` | {c} | {t / 1e9:.3f} | {t / c / 1e3:.1f} |")
+        pb = collections.defaultdict(lambda: [0, 0])
+        for st, en, n, g in ks:
+            if "k_panel_lds" in n:   # the template argument, not the parameter list (double*)
+                x = pb[(short(n).startswith("k_panel_lds<double"), (g + 31) // 32 * 32)]
+                x[0] += 1
+                x[1] += en - st
+        for fp64 in (True, False):
+            cells = [f"<={g}: {t / c / 1e3:.0f} us ({c})" for (d, g), (c, t) in sorted(pb.items()) if d == fp64]
+            if cells:
+                out.append(f"k_panel_lds<{'double' if fp64 else 'float'}> by workgroups: " + ", ".join(cells))
+    return "\n".join(out)
+
+
+def serving_leg(nq=96, workers=4, refine_tol=1e-10, windows_path=None):
     """The same query stream served in the FP64 form and in the mixed form
     (FP32 LU + FP64 refinement to 1e-10, FP32 far sums), HBM budget 200 GB:
     once as a burst (every query queued at once: capacity in solves/s) and once
@@ -433,13 +504,16 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10):
                        f"and a 1e-7 estimated forward error"}
     res = {}
     rate = None
+    windows = {}
     for mode in ("burst", "stream"):
         out[mode] = {}
         for form in ("fp64", "mixed"):
             qs = serving_queries(pool, nq)
             qps = 1e6 if mode == "burst" else rate
             with serve.Server(pool, form=form, workers=workers, hbm_budget=200e9, refine_tol=refine_tol) as s:
+                t0 = _clock_ns()
                 run = serve.poisson_run(s, qs, qps=qps, seed=7)
+                windows[f"{mode} {form}"] = [t0, _clock_ns()]
             st = run["stats"]
             log(f"[bench] serving {mode} {form}: {st['wall_s']:.1f} s, {st['solves_per_s']:.1f} solves/s, "
                 f"p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s, fallbacks {st['fp64_fallbacks']}")
@@ -487,7 +561,12 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10):
     # the plain FP64 GPU form on the same queries: each query on its own, in arrival
     # order, with its own assembly and FP64 LU (no factor reuse, no overlap, current
     # sets in passes of 4)
+    t0 = _clock_ns()
     out["plain_fp64"] = plain_fp64_run(pool, serving_queries(pool, nq))
+    windows["plain fp64"] = [t0, _clock_ns()]
+    if windows_path:
+        with open(windows_path, "w") as f:
+            json.dump(windows, f)
     out["served_mixed_vs_plain_fp64_solves_per_s"] = round(out["burst"]["mixed"]["solves_per_s"] /
                                                            out["plain_fp64"]["solves_per_s"], 3)
     out["served_fp64_vs_plain_fp64_solves_per_s"] = round(out["burst"]["fp64"]["solves_per_s"] /
@@ -830,6 +909,10 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the served forms")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="library tunable for every context (repeatable), e.g. lu_look_ahead=0")
+    ap.add_argument("--trace-windows", default=None, metavar="PATH",
+                    help="serving leg: write the host-clock window of each run (burst, stream, plain) to PATH")
+    ap.add_argument("--trace-summary", default=None, metavar="CSV",
+                    help="print the kernel activity per window of a rocprofv3 kernel trace (needs --trace-windows)")
     ap.add_argument("--lu-panel", type=int, default=1, choices=(0, 1),
                     help="LU panels as one whole-panel launch (1) or per-column launches (0), every context")
     args = ap.parse_args()
@@ -838,6 +921,9 @@ def main():
               "note only (row blocks of the matrix, observation points of the field).", file=sys.stderr)
         sys.exit(2)
 
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary, args.trace_windows))
+        return
     import bielcc_amd as B
     B.Context.default_tuning["lu_panel_lds"] = args.lu_panel
     for kv in args.tune:
@@ -856,7 +942,8 @@ def main():
         print(json.dumps({"lu": lu_only(args.lu_only)}), flush=True)
         return
     if args.serving_only:
-        print(json.dumps({"serving": serving_leg(nq=args.serve_queries, workers=args.serve_workers)}), flush=True)
+        print(json.dumps({"serving": serving_leg(nq=args.serve_queries, workers=args.serve_workers,
+                                                 windows_path=args.trace_windows)}), flush=True)
         return
     w = workload(args.config)
     verts, k, pts = w["verts"], w["k"], w["pts"]
