@@ -98,7 +98,7 @@ struct LuWork {
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
     int outer_nb = 0;               // columns per outer block (two-level LU); kNB: one level; 0: by order
     int trsv_max64 = 1;             // FP64 solves with at most this many right-hand sides use the one-kernel solve
-    int trsv_max32 = 8;             // FP32 solves likewise (up to kTrsvMaxR32)
+    int trsv_max32 = 16;            // FP32 solves likewise (up to kTrsvMaxR32)
     double refine_ferr = 0.;        // refinement also stops on an error estimate below this (0: residual only)
     bool trsv_chunks32 = false;     // FP32 solves with more than 16 right-hand sides: one-kernel solves
                                     // in chunks (else rocBLAS trsm)
@@ -1489,54 +1489,58 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
     __syncthreads();
     {
         // every wave: lane r holds row r of the tile; wave cg solves the right-hand sides
-        // q with q % 4 == cg (a wave-uniform choice); x_p is broadcast from lane p
-        c2<R> y[NR];
+        // q = cg + 4j, j < NJ, x_p is broadcast from lane p.  The chains of one wave sit in
+        // one block without a branch between them (a branch per set ran them one after
+        // the other), and a wave without a set skips the substitution.
+        constexpr int NJ = (NR + 3) / 4;
+        c2<R> y[NJ];
 #pragma unroll
-        for (int q = 0; q < NR; q++) {
-            y[q] = mk2<R>(R(0), R(0));
-            if ((q & 3) == cg && q < nrhs && r < nb) {
-                y[q] = B[(size_t)q * ldb + r0 + r];
-                for (int g = 0; g < 4; g++) { y[q].x += red[g][r][q].x; y[q].y += red[g][r][q].y; }
+        for (int j = 0; j < NJ; j++) {
+            const int q = cg + 4 * j;
+            y[j] = mk2<R>(R(0), R(0));
+            if (q < nrhs && r < nb) {
+                y[j] = B[(size_t)q * ldb + r0 + r];
+                for (int g = 0; g < 4; g++) { y[j].x += red[g][r][q].x; y[j].y += red[g][r][q].y; }
             }
         }
-        if (lower) {
-            for (int p = 0; p < nb - 1; p++) {
-                const c2<R> l = T[r][p];
-                const bool upd = r > p && r < nb;
+        if (cg < nrhs) {   // uniform in the wave
+            if (lower) {
+                for (int p = 0; p < nb - 1; p++) {
+                    const c2<R> l = T[r][p];
+                    const bool upd = r > p && r < nb;
 #pragma unroll
-                for (int q = 0; q < NR; q++) {
-                    if ((q & 3) == cg && q < nrhs) {
-                        const c2<R> x = mk2<R>(bcast_lane(y[q].x, p), bcast_lane(y[q].y, p));
+                    for (int j = 0; j < NJ; j++) {
+                        const c2<R> x = mk2<R>(bcast_lane(y[j].x, p), bcast_lane(y[j].y, p));
                         if (upd) {
-                            y[q].x -= l.x * x.x - l.y * x.y;
-                            y[q].y -= l.x * x.y + l.y * x.x;
+                            y[j].x -= l.x * x.x - l.y * x.y;
+                            y[j].y -= l.x * x.y + l.y * x.x;
                         }
                     }
                 }
-            }
-        } else {
-            for (int p = nb - 1; p >= 0; p--) {
-                const c2<R> d = T[p][p];
-                const R den = d.x * d.x + d.y * d.y;
-                const c2<R> di = mk2<R>(d.x / den, -d.y / den);
-                const c2<R> u = T[r][p];
+            } else {
+                for (int p = nb - 1; p >= 0; p--) {
+                    const c2<R> d = T[p][p];
+                    const R den = d.x * d.x + d.y * d.y;
+                    const c2<R> di = mk2<R>(d.x / den, -d.y / den);
+                    const c2<R> u = T[r][p];
 #pragma unroll
-                for (int q = 0; q < NR; q++) {
-                    if ((q & 3) == cg && q < nrhs) {
-                        const c2<R> b = mk2<R>(bcast_lane(y[q].x, p), bcast_lane(y[q].y, p));
+                    for (int j = 0; j < NJ; j++) {
+                        const c2<R> b = mk2<R>(bcast_lane(y[j].x, p), bcast_lane(y[j].y, p));
                         const c2<R> x = mk2<R>(b.x * di.x - b.y * di.y, b.x * di.y + b.y * di.x);
-                        if (r == p) y[q] = x;
+                        if (r == p) y[j] = x;
                         else if (r < p) {
-                            y[q].x -= u.x * x.x - u.y * x.y;
-                            y[q].y -= u.x * x.y + u.y * x.x;
+                            y[j].x -= u.x * x.x - u.y * x.y;
+                            y[j].y -= u.x * x.y + u.y * x.x;
                         }
                     }
                 }
             }
         }
 #pragma unroll
-        for (int q = 0; q < NR; q++)
-            if ((q & 3) == cg && q < nrhs && r < nb) st_agent2<R>(&B[(size_t)q * ldb + r0 + r], y[q]);
+        for (int j = 0; j < NJ; j++) {
+            const int q = cg + 4 * j;
+            if (q < nrhs && r < nb) st_agent2<R>(&B[(size_t)q * ldb + r0 + r], y[j]);
+        }
     }
     // x_ib goes out with device-scope stores that each wave waits for, so the flag needs
     // no release fence (a device fence writes back and invalidates this XCD's whole L2)
@@ -1640,11 +1644,11 @@ static hipError_t lu_solve_perm(hipStream_t s, LuWork* w, int n, const c2<R>* A,
                        tmp, B, ldb);
     // The one-kernel solve substitutes block by block; rocBLAS trsm multiplies by inverted
     // diagonal blocks.  FP64: the one-kernel solve only for one right-hand side (n = 10,816:
-    // 4.1 vs 30 ms; from two on trsm is 1.5-4x faster).  FP32 (refinement): the one-kernel
-    // solve up to trsv_max32 right-hand sides (8; at most kTrsvMaxR32), trsm above unless
-    // trsv_chunks32 asks for chunks of the one-kernel solve.  Measured (refinements with
-    // two corrections, 11,616 unknowns): 1 set 15 vs 53 ms, 8 sets 30 vs 35 ms, 12 and
-    // 16 sets 42 and 46 vs 35 ms; trsm's time hardly grows with the sets.
+    // 3.5 vs 31 ms; from two on trsm is as fast or faster: 3.8 ms for 2 to 16 sets against
+    // 3.9-5.3 ms).  FP32 (refinement): the one-kernel solve up to trsv_max32 right-hand
+    // sides (16), trsm above unless trsv_chunks32 asks for chunks of the one-kernel solve.
+    // Refinements with two corrections, 11,616 unknowns: 1, 8 and 16 sets 15, 19 and 26 ms
+    // against 53, 35 and 35 ms with trsm.
     const int tmax = sizeof(R) == 8 ? w->trsv_max64 : w->trsv_max32;
     if (!w->no_trsv && nrhs <= tmax) return trsv_solve<R>(s, w, n, nrhs, A, lda, B, ldb);
     if (!w->no_trsv && sizeof(R) == 4 && w->trsv_chunks32) {

@@ -244,11 +244,6 @@ class Server:
         self.ctxs = [api.Context(device) for _ in range(workers)]
         for c in self.ctxs:
             c.set_tuning("refine_ferr_exp", int(refine_ferr_exp))
-            # FP32 solves of up to 16 right-hand sides in one kernel: slower than trsm alone
-            # above 8, even under the load of four workers, and accurate enough that fewer
-            # entries stall under the forward-error stop (7 queries on FP64 factors instead
-            # of 14 in a 96-query burst)
-            c.set_tuning("trsv_max32", 16)
         self.dev_meshes = [dict() for _ in range(workers)]
         self.pools = [dict() for _ in range(workers)]   # name -> DeviceArray, grow-only
         self.scratch_hw = [0] * workers   # bytes charged for the context's own grow-only scratch

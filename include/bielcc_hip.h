@@ -349,7 +349,7 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * current sets per field pass (1, 2 or 4); "trsv" [1] = small solves use the
  * one-kernel-per-triangle solve (0: as larger ones): FP64 solves with at most
  * "trsv_max64" [1] right-hand sides, FP32 solves with at most "trsv_max32"
- * [8] (0..16), and, with "trsv_chunks32" [0] = 1, FP32 solves with more, in
+ * [16] (0..16), and, with "trsv_chunks32" [0] = 1, FP32 solves with more, in
  * chunks of 16;
  * "trsm_lib" [1] = the other solves use rocBLAS trsm per triangle (0: the
  * library's blocked TRSM + GEMM loop); "refine_ferr_exp" [0] = e > 0: the

@@ -213,15 +213,12 @@ def _fp32_first_solve(ctx, A, Bm, trsv):
 
 @pytest.mark.parametrize("nrhs", [8, 9, 12, 16, 17])
 def test_fp32_solve_up_to_16_rhs_in_one_kernel(ctx, nrhs):
-    """With trsv_max32 = 16, FP32 solves of 8 to 16 right-hand sides take the one-kernel
-    solve (8: the 8-wide instantiation; 17: rocBLAS trsm).  Its X0 against the same FP32 factors and pivots
-    substituted in FP64, and against the trsm path; ragged last blocks."""
-    assert ctx.get_tuning("trsv_max32") == 8
-    ctx.set_tuning("trsv_max32", 16)
-    try:
-        _fp32_solves_against_fp64_substitution(ctx, nrhs)
-    finally:
-        ctx.set_tuning("trsv_max32", 8)
+    """FP32 solves of up to 16 right-hand sides take the one-kernel solve (8: the
+    8-wide instantiation, 9-16 the 16-wide one; 17: rocBLAS trsm).  Its X0 against the
+    same FP32 factors and pivots substituted in FP64, and against the trsm path; ragged
+    last blocks."""
+    assert ctx.get_tuning("trsv_max32") == 16
+    _fp32_solves_against_fp64_substitution(ctx, nrhs)
 
 
 def _fp32_solves_against_fp64_substitution(ctx, nrhs):
