@@ -1423,26 +1423,35 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
     for (int s = 0; s < t; s++) {
         const int jb = lower ? s : nblk - 1 - s;
         if (s >= ready) {
-            if (tid == 0) {
-                // relaxed device-scope polls read the flag coherently without the
-                // cache invalidation an acquire load performs on every iteration;
-                // x_jb is then read with device-scope loads as well (load_relaxed),
-                // issued only after the flag value has been seen
-                int spins = 0;
-                while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                    if (++spins > spin_limit) { atomicExch(err, 1); break; }
-                    // once any wait has expired, nobody waits any more: the kernel drains
-                    if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                    __builtin_amdgcn_s_sleep(1);
+            if (tid < 64) {
+                if (tid == 0) {
+                    // relaxed device-scope polls read the flag coherently without the
+                    // cache invalidation an acquire load performs on every iteration;
+                    // x_jb is then read with device-scope loads as well (load_relaxed),
+                    // issued only after the flag value has been seen
+                    int spins = 0;
+                    while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                        if (++spins > spin_limit) { atomicExch(err, 1); break; }
+                        // once any wait has expired, nobody waits any more: the kernel drains
+                        if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                            break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
                 }
-                int rd = s + 1;   // extend the window without waiting
-                while (rd < t && rd < s + 32) {
-                    const int kb = lower ? rd : nblk - 1 - rd;
-                    if (__hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) break;
-                    rd++;
+                // extend the window without waiting: lanes 1-31 look at the next 31 blocks
+                // with one load each, all at once (a workgroup that starts late finds many
+                // blocks published; one load after another cost a round trip per block)
+                bool rdy = tid == 0;
+                const int sl = s + tid;
+                if (tid > 0 && tid < 32 && sl < t) {
+                    const int kb = lower ? sl : nblk - 1 - sl;
+                    rdy = __hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
                 }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                s_ready = rd;
+                const unsigned long long notr = ~__ballot(rdy);   // lane 0 always counts as ready
+                if (tid == 0) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    s_ready = s + (__ffsll(notr) - 1);   // blocks s .. first lane not ready - 1
+                }
             }
             __syncthreads();
             ready = s_ready;
