@@ -324,6 +324,58 @@ def lu_only(n, reps=3):
     return out
 
 
+def solve_only(n, reps=3):
+    """Triangular solves on the factors of a seeded random complex n x n system
+    (N(0,1) + i N(0,1) entries plus 2 sqrt(n) on the diagonal): FP64 solves (lu_solve)
+    and FP32-factor refinements to a 1e-12 residual (refine_solve), for 1 to 16
+    right-hand sides, through the one-kernel solve (library defaults: FP64 for one
+    right-hand side, FP32 for up to 16) and through rocBLAS trsm (tunable trsv = 0).
+    Fastest of reps each."""
+    import bielcc_amd as B
+    ctx = B.Context(0)
+    rng = np.random.default_rng(2)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2 * np.sqrt(n) * np.eye(n)
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    del A
+    dL, piv, dF, piv32 = ctx.empty(n * n * 16), ctx.empty(4 * n), ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.copy(dL, dA, n * n * 16)
+    ctx.lu_factor(dL, n, n, piv)
+    ctx.lu32_factor(dA, n, n, dF, n, piv32)
+    out = {"n": n, "fp64_solve_ms": {}, "fp32_refine_ms": {}, "refine_corrections": {}}
+    for nrhs in (1, 2, 8, 16):
+        Bm = rng.normal(size=(nrhs, n)) + 1j * rng.normal(size=(nrhs, n))
+        dB0, dB = ctx.to_device(np.ascontiguousarray(Bm)), ctx.empty(n * nrhs * 16)
+        for trsv in (1, 0):
+            ctx.set_tuning("trsv", trsv)
+            t64, t32 = [], []
+            for _ in range(reps):
+                ctx.copy(dB, dB0, n * nrhs * 16)
+                ctx.sync()
+                t0 = time.perf_counter()
+                ctx.lu_solve(dL, n, n, piv, dB, nrhs, n)
+                ctx.sync()
+                t64.append(time.perf_counter() - t0)
+                ctx.copy(dB, dB0, n * nrhs * 16)
+                ctx.sync()
+                t0 = time.perf_counter()
+                it, res = ctx.refine_solve(dA, n, n, dF, n, piv32, dB, nrhs, n, 1e-12, 30)
+                ctx.sync()
+                t32.append(time.perf_counter() - t0)
+            key = f"{nrhs} {'default' if trsv else 'trsm'}"
+            out["fp64_solve_ms"][key] = round(min(t64) * 1e3, 3)
+            out["fp32_refine_ms"][key] = round(min(t32) * 1e3, 3)
+            out["refine_corrections"][key] = it
+            log(f"[bench] n={n} nrhs={nrhs} {'default' if trsv else 'trsm'}: FP64 solve "
+                f"{out['fp64_solve_ms'][key]} ms, FP32 refinement {out['fp32_refine_ms'][key]} ms ({it} corrections)")
+        ctx.set_tuning("trsv", 1)
+        dB0.free()
+        dB.free()
+    for d in (dA, dL, piv, dF, piv32):
+        d.free()
+    ctx.close()
+    return out
+
+
 def serving_pool(seed=5, n=16, lo=1000, hi=6000):
     """Seeded pool of closed meshes: spheres, triaxial ellipsoids (semi-axes in
     [0.5, 1.5]), tori (R = 1, r in [0.2, 0.4]) and cubes of side 2, lo..hi cells."""
@@ -906,6 +958,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--lu-only", type=int, default=0, metavar="N",
                     help="time only the FP64 and FP32 LU of a seeded random complex N x N matrix")
+    ap.add_argument("--solve-only", type=int, default=0, metavar="N",
+                    help="time only the FP64 solves and FP32 refinements of a seeded random N x N system")
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the served forms")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="library tunable for every context (repeatable), e.g. lu_look_ahead=0")
@@ -940,6 +994,9 @@ def main():
         return
     if args.lu_only:
         print(json.dumps({"lu": lu_only(args.lu_only)}), flush=True)
+        return
+    if args.solve_only:
+        print(json.dumps({"solve": solve_only(args.solve_only)}), flush=True)
         return
     if args.serving_only:
         print(json.dumps({"serving": serving_leg(nq=args.serve_queries, workers=args.serve_workers,
