@@ -240,6 +240,7 @@ int lu_get_panel_lds(const LuWork*);
 void lu_set_panel_wait_us(LuWork*, int us);
 int lu_get_panel_wait_us(const LuWork*);
 int lu_panel_aborts(LuWork*);
+int lu_late_admits(const LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);

@@ -700,6 +700,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "trsv_ticket_faults") *value = lu_trsv_retries(ctx->lu, 1);
     else if (k == "lu_panel_wait_us") *value = lu_get_panel_wait_us(ctx->lu);
     else if (k == "lu_panel_aborts") *value = lu_panel_aborts(ctx->lu);
+    else if (k == "lu_panel_late_admits") *value = lu_late_admits(ctx->lu);
     else if (k == "plan_field_cells") *value = ctx->plan_field_cells;
     else if (k == "plan_asm_cells") *value = ctx->plan_asm_cells;
     else if (k == "plan_wg_per_cu") *value = ctx->plan_wg_per_cu;

@@ -106,6 +106,7 @@ struct LuWork {
                                     // 3-9% faster at orders 3,000-40,000, FP32 within 1%
     int trsv_expired = 0;           // sync-free solves redone on the blocked path: a wait expired
     int trsv_ticket_faults = 0;     // ... the ticket counter was out of step with the host
+    int late_admits = 0;            // LUs admitted to the whole-panel kernel after their first panel
     hipStream_t side = nullptr;     // side stream (default priority) and its two events
     hipEvent_t ev_upd = nullptr, ev_panel = nullptr;
 };
@@ -135,6 +136,7 @@ void lu_set_trsm_lib_panel(LuWork* w, bool on) { w->lu_trsm_lib = on; }
 bool lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
 int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
+int lu_late_admits(const LuWork* w) { return w->late_admits; }
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
 int lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
 void lu_set_panel_wait_us(LuWork* w, int us) { w->panel_wait_us = us; }
@@ -1095,7 +1097,10 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
         const int G = wgs_for(n - k0);
-        if (!lds && lds_allowed && G <= gmax) lds = admit.take(w->dev, reserve_for(n - k0), 2 * w->n_cu);
+        if (!lds && lds_allowed && G <= gmax && admit.take(w->dev, reserve_for(n - k0), 2 * w->n_cu)) {
+            lds = true;
+            w->late_admits++;
+        }
         if (lds && G <= gmax && (force || G <= w->n_cu || running.alone())) {
             if (rows_per_wg(n - k0) == 64)
                 hipLaunchKernelGGL((k_panel_lds<R, 64>), dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info,

@@ -384,6 +384,8 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * bielcc_get_tuning also reads
  * "n_cu" (compute units) and "lu_panel_aborts" (whole-panel launches that
  * aborted to one workgroup, for the context, since its first LU),
+ * "lu_panel_late_admits" (LUs that missed the whole-panel budget at their
+ * first panel and were admitted at a later one),
  * "trsv_expired" and "trsv_ticket_faults" (one-kernel triangular solves
  * redone on the blocked path because a wait expired, or because the ticket
  * counter was out of step).  Returns BIELCC_ERR_ARG for an unknown key or a

@@ -112,9 +112,9 @@ def test_lu_random_multirhs(ctx):
 
 @pytest.mark.parametrize("nrhs", [1, 3, 8, 9, 64])
 def test_lu_solve_sync_free_and_blocked_agree(ctx, nrhs):
-    """Up to 8 right-hand sides go through the one-kernel-per-triangle solve,
-    more through rocBLAS trsm or the blocked TRSM + GEMM loop; every path
-    against numpy, ragged last block."""
+    """FP64 solves: one right-hand side goes through the one-kernel-per-triangle
+    solve, more through rocBLAS trsm or the blocked TRSM + GEMM loop (trsv = 0 sends
+    one there too); every path against numpy, ragged last block."""
     rng = np.random.default_rng(4)
     for n in (1, 64, 200, 1000):
         A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 4 * np.sqrt(n) * np.eye(n)
@@ -442,6 +442,72 @@ def test_concurrent_per_column_lus_match_alone():
         finally:
             for c in ctxs:
                 c.close()
+
+
+def test_lus_admitted_to_the_whole_panel_kernel_at_a_later_panel():
+    """Four LUs of order 9,000 start at once with the default admission: 141 workgroups
+    each against a budget of two per compute unit (512), so at least one starts on the
+    per-column kernels and is admitted to the whole-panel kernel at a later, shorter
+    panel.  FP64 and FP32 factors and pivots equal, bit for bit, the same LUs run alone
+    (both panel paths give the same bits)."""
+    import hashlib
+    import threading
+    n = 9000
+    ctxs = [B.Context(0) for _ in range(4)]
+    try:
+        if ctxs[0].get_tuning("n_cu") != 256:
+            pytest.skip("sized for 256 compute units")
+        rng = np.random.default_rng(21)
+        dA0 = []
+        for t in range(4):
+            A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+            dA0.append(ctxs[t].to_device(np.ascontiguousarray(A.T)))
+            del A
+        dA = [c.empty(n * n * 16) for c in ctxs]
+        dF = [c.empty(n * n * 8) for c in ctxs]
+        piv = [c.empty(4 * n) for c in ctxs]
+        piv32 = [c.empty(4 * n) for c in ctxs]
+
+        def factor(t):
+            c = ctxs[t]
+            c.copy(dA[t], dA0[t], n * n * 16)
+            c.lu_factor(dA[t], n, n, piv[t])
+            c.lu32_factor(dA0[t], n, n, dF[t], n, piv32[t])
+
+        def digest(t):
+            h = hashlib.blake2b()
+            for d, shape, dt in ((dA[t], (n, n), np.complex128), (piv[t], (n,), np.int32),
+                                 (dF[t], (n, n), np.complex64), (piv32[t], (n,), np.int32)):
+                h.update(d.to_numpy(shape, dt).tobytes())
+            return h.hexdigest()
+
+        alone = []
+        for t in range(4):
+            factor(t)
+            alone.append(digest(t))
+        late0 = sum(c.get_tuning("lu_panel_late_admits") for c in ctxs)
+        start = threading.Barrier(4)
+        errs = []
+
+        def run(t):
+            try:
+                start.wait()
+                factor(t)
+            except BaseException as exc:   # noqa: BLE001 - reported below
+                errs.append(exc)
+
+        th = [threading.Thread(target=run, args=(t,)) for t in range(4)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        assert not errs, errs
+        assert sum(c.get_tuning("lu_panel_late_admits") for c in ctxs) > late0
+        for t in range(4):
+            assert digest(t) == alone[t], t
+    finally:
+        for c in ctxs:
+            c.close()
 
 
 def test_whole_panel_kernel_two_workgroups_per_cu(ctx):
