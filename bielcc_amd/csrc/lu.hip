@@ -1424,6 +1424,15 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
     // the diagonal block does not depend on any x: stage it before the waits
     for (int q = cg; q < nb; q += 4)
         if (r < nb) T[r][q] = A[(size_t)(r0 + q) * lda + r0 + r];
+    // This block's right-hand sides (q = cg + 4j), loaded before any wait: no other
+    // workgroup writes them, and loaded at the end they cost a round trip on the chain.
+    constexpr int NJ = (NR + 3) / 4;
+    c2<R> bv[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int q = cg + 4 * j;
+        bv[j] = (q < nrhs && r < nb) ? B[(size_t)q * ldb + r0 + r] : mk2<R>(R(0), R(0));
+    }
     int ready = 0;   // blocks known to be published (same value in every lane)
     for (int s = 0; s < t; s++) {
         const int jb = lower ? s : nblk - 1 - s;
@@ -1506,14 +1515,13 @@ __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const 
         // q = cg + 4j, j < NJ, x_p is broadcast from lane p.  The chains of one wave sit in
         // one block without a branch between them (a branch per set ran them one after
         // the other), and a wave without a set skips the substitution.
-        constexpr int NJ = (NR + 3) / 4;
         c2<R> y[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             const int q = cg + 4 * j;
             y[j] = mk2<R>(R(0), R(0));
             if (q < nrhs && r < nb) {
-                y[j] = B[(size_t)q * ldb + r0 + r];
+                y[j] = bv[j];
                 for (int g = 0; g < 4; g++) { y[j].x += red[g][r][q].x; y[j].y += red[g][r][q].y; }
             }
         }
