@@ -230,8 +230,8 @@ int lu_get_trsv_max32(const LuWork*);
 void lu_set_refine_ferr(LuWork*, double ferr);
 double lu_get_refine_ferr(const LuWork*);
 bool lu_get_trsv_chunks32(const LuWork*);
-void lu_set_trsm_lib_panel(LuWork*, bool on);
-bool lu_get_trsm_lib_panel(const LuWork*);
+void lu_set_trsm_lib_panel(LuWork*, int mode);
+int lu_get_trsm_lib_panel(const LuWork*);
 int lu_get_outer_nb(const LuWork*);
 // sync-free triangular solves redone on the blocked path (which 0: a wait expired, 1: ticket fault)
 int lu_trsv_retries(const LuWork*, int which);

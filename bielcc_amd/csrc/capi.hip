@@ -616,8 +616,8 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         return BIELCC_OK;
     }
     if (k == "lu_trsm_lib") {
-        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "lu_trsm_lib must be 0 or 1");
-        lu_set_trsm_lib_panel(ctx->lu, value == 1);
+        if (value < 0 || value > 2) return fail(ctx, BIELCC_ERR_ARG, "lu_trsm_lib must be 0, 1 or 2");
+        lu_set_trsm_lib_panel(ctx->lu, value);
         return BIELCC_OK;
     }
     if (k == "lu_outer_nb") {
@@ -688,7 +688,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
-    else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu) ? 1 : 0;
+    else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu);
     else if (k == "trsv_max64") *value = lu_get_trsv_max64(ctx->lu);
     else if (k == "trsv_max32") *value = lu_get_trsv_max32(ctx->lu);
     else if (k == "refine_ferr_exp") {

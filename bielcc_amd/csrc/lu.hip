@@ -102,7 +102,7 @@ struct LuWork {
     double refine_ferr = 0.;        // refinement also stops on an error estimate below this (0: residual only)
     bool trsv_chunks32 = false;     // FP32 solves with more than 16 right-hand sides: one-kernel solves
                                     // in chunks (else rocBLAS trsm)
-    bool lu_trsm_lib = true;        // the factorisation's U12 triangles on rocBLAS trsm (else k_trsm): FP64 LU
+    int lu_trsm_lib = 2;            // the factorisation's U12 triangles: rocBLAS trsm (1), k_trsm_lanes (2), k_trsm (0)
                                     // 3-9% faster at orders 3,000-40,000, FP32 within 1%
     int trsv_expired = 0;           // sync-free solves redone on the blocked path: a wait expired
     int trsv_ticket_faults = 0;     // ... the ticket counter was out of step with the host
@@ -132,8 +132,8 @@ int lu_get_trsv_max32(const LuWork* w) { return w->trsv_max32; }
 void lu_set_refine_ferr(LuWork* w, double ferr) { w->refine_ferr = ferr; }
 double lu_get_refine_ferr(const LuWork* w) { return w->refine_ferr; }
 bool lu_get_trsv_chunks32(const LuWork* w) { return w->trsv_chunks32; }
-void lu_set_trsm_lib_panel(LuWork* w, bool on) { w->lu_trsm_lib = on; }
-bool lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
+void lu_set_trsm_lib_panel(LuWork* w, int mode) { w->lu_trsm_lib = mode; }
+int lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
 int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 int lu_late_admits(const LuWork* w) { return w->late_admits; }
@@ -873,6 +873,55 @@ __global__ __launch_bounds__(256) void k_row_gather(int k0, int c1, int n, c2<R>
     if (h1) col[m1.x] = v1;
 }
 
+__device__ inline double bcast_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float bcast_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// U12 = L11^{-1} A12 in place: L11 the nb x nb unit lower triangle at (k0, k0) of A,
+// A12 its rows k0..k0+nb in columns [cbeg, cend).  One workgroup per 64 columns; wave w
+// takes 16 of them and lane r holds row r of each.  The forward substitution broadcasts
+// x_p from lane p, so it needs no barrier per step, and a wave's 16 columns are
+// independent chains that interleave.
+template <typename R>
+__global__ __launch_bounds__(256) void k_trsm_lanes(int nb, c2<R>* A, long long lda, int k0, int cbeg, int cend) {
+    __shared__ c2<R> L[kNB][kNB + 1];   // L[r][p]
+    const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
+    for (int p = w; p < nb; p += 4)
+        if (r < nb) L[r][p] = A[(size_t)(k0 + p) * lda + k0 + r];
+    constexpr int CW = 16;
+    const int c0 = cbeg + blockIdx.x * 64 + w * CW;
+    c2<R> y[CW];
+#pragma unroll
+    for (int j = 0; j < CW; j++) {
+        const int c = c0 + j;
+        y[j] = (c < cend && r < nb) ? A[(size_t)c * lda + k0 + r] : mk2<R>(R(0), R(0));
+    }
+    __syncthreads();
+    for (int p = 0; p < nb - 1; p++) {
+        const c2<R> l = L[r][p];
+        const bool upd = r > p && r < nb;
+#pragma unroll
+        for (int j = 0; j < CW; j++) {
+            const c2<R> x = mk2<R>(bcast_lane(y[j].x, p), bcast_lane(y[j].y, p));
+            if (upd) {
+                y[j].x -= l.x * x.x - l.y * x.y;
+                y[j].y -= l.x * x.y + l.y * x.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CW; j++) {
+        const int c = c0 + j;
+        if (c < cend && r < nb) A[(size_t)c * lda + k0 + r] = y[j];
+    }
+}
+
 // X = T^{-1} X for the nb x nb triangle T at (t0, t0) of A (lower unit when
 // lower != 0, else upper non-unit), X = columns [cbeg, cend) of B rows t0..t0+nb.
 // One workgroup of 256 lanes per 64 columns; T and the X tile live in LDS.
@@ -989,9 +1038,14 @@ template <typename R>
 static hipError_t panel_trsm(LuWork* w, hipStream_t s, int nb, c2<R>* A, long long lda, int k0, int cbeg, int cend,
                              const char** msg) {
     if (cend <= cbeg) return hipSuccess;
-    if (w->lu_trsm_lib)
+    if (w->lu_trsm_lib == 1)
         return lib_trsm(w, s, 1, nb, cend - cbeg, A + (size_t)k0 * lda + k0, lda, A + (size_t)cbeg * lda + k0, lda,
                         msg);
+    if (w->lu_trsm_lib == 2) {
+        hipLaunchKernelGGL(k_trsm_lanes<R>, dim3((cend - cbeg + 63) / 64), dim3(256), 0, s, nb, A, lda, k0, cbeg,
+                           cend);
+        return hipSuccess;
+    }
     hipLaunchKernelGGL(k_trsm<R>, dim3((cend - cbeg + 63) / 64), dim3(256), 0, s, nb, 1, A, lda, (long long)k0, A, lda,
                        (long long)k0, cbeg, cend);
     return hipSuccess;
@@ -1370,15 +1424,6 @@ __device__ inline c2<R> load_relaxed(const c2<R>* p) {
                   __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-__device__ inline double bcast_lane(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ inline float bcast_lane(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
 
 template <typename R, int NR>
 __global__ __launch_bounds__(256) void k_trsv(int n, int nrhs, int lower, const c2<R>* __restrict__ A,

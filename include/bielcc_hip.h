@@ -363,8 +363,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * per outer block of the two-level LU (64, 128, 256 or 512): the panels'
  * updates stay inside the block and the rest of the matrix gets one update of
  * that rank (64: one level; 0: one level below order 16,000, 512 from there);
- * "lu_trsm_lib" [1] = the factorisation's U12 triangles on rocBLAS trsm (0:
- * the library's LDS sweep); "lu_look_ahead" [2] = with a one-level LU, the
+ * "lu_trsm_lib" [2] = the factorisation's U12 triangles on the library's
+ * lane-substitution kernel, one row per lane (1: rocBLAS trsm; 0: the
+ * library's LDS sweep); "lu_look_ahead" [2] = with a one-level LU, the
  * next LU panel is factored on a side stream during the trailing update:
  * always (1), never (0), or for FP64 above order 20,000 (2); "lu_panel_lds" [1] =
  * LU panels of at most 32,768 rows (FP64) or 65,536 (FP32) run as one launch

@@ -343,6 +343,24 @@ def _factors(ctx, A, fp32=True, **tuning):
             ctx.set_tuning(k, v)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_lu_u12_triangle_modes(ctx, mode):
+    """The factorisation's U12 triangles through the library's LDS sweep (0), rocBLAS
+    trsm (1) and the lane-substitution kernel (2): each gives FP64 and FP32 factors
+    whose backward error ||P A - L U|| / ||A|| is at roundoff level; ragged last panel."""
+    rng = np.random.default_rng(22)
+    for n in (300, 1000, 2000):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        LU, piv, F, piv32 = _factors(ctx, A, lu_trsm_lib=mode)
+        for fac, p, tol in ((LU.T, piv, 1e-13), (F.T.astype(np.complex128), piv32, 1e-4)):
+            PA = A.copy()
+            for j in range(n):   # the factorisation's interchanges, in order
+                PA[[j, p[j]]] = PA[[p[j], j]]
+            L = np.tril(fac, -1) + np.eye(n)
+            err = np.linalg.norm(PA - L @ np.triu(fac)) / np.linalg.norm(A)
+            assert err <= tol, (n, mode, err)
+
+
 @pytest.mark.parametrize("n", [64, 65, 300, 1000, 2500])
 def test_whole_panel_kernel_matches_column_kernels(ctx, n):
     """The whole-panel launch (panel rows in LDS across workgroups, one grid
