@@ -81,11 +81,14 @@ class DeviceArray:
             ctx._check(ctx.lib.bielcc_memcpy(ctx.h, C.c_void_p(d.ptr), r.ctypes.data_as(C.c_void_p), r.nbytes, 0))
         return d
 
-    def to_numpy(self, shape, dtype=np.float64) -> np.ndarray:
+    def to_numpy(self, shape, dtype=np.float64, offset: int = 0) -> np.ndarray:
+        """Copy of the bytes [offset, offset + size) as a host array."""
         out = np.empty(shape, dtype=dtype)
+        if offset < 0 or offset + out.nbytes > self.nbytes:
+            raise ValueError("range outside the device buffer")
         if out.nbytes:
             self.ctx._check(self.ctx.lib.bielcc_memcpy(self.ctx.h, out.ctypes.data_as(C.c_void_p),
-                                                       C.c_void_p(self.ptr), out.nbytes, 1))
+                                                       C.c_void_p(self.ptr + int(offset)), out.nbytes, 1))
         return out
 
     def free(self):

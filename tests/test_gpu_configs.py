@@ -1,0 +1,242 @@
+"""The benchmark configurations at their stated sizes, on the GPU, against the
+oracle on seeded samples.
+
+- Sphere of radius 1, 100 x 200 theta-phi quads (20,000 cells, +-0.5% jitter,
+  seed 1), ka = 10: collocation-matrix row strips (every block of 64 rows x
+  all 40,000 columns) at 1e-12 per block, the scattered E and H at sampled
+  points of the 200 x 500 grid of radius 5 at 1e-10 per point, and the FP64
+  LU solve by its full residual (the 25.6 GB matrix streamed back in column
+  strips).
+- The same sphere with every quad split into two triangles: 40,000 minus the
+  400 zero-area pole triangles = 39,600 cells, 79,200 unknowns: row strips and
+  field samples against the oracle.
+- Precision form: FP32 LU + FP64 refinement and FP32 far-pair field sums on the
+  20,000-quad sphere and on the cube of side 2 (6 x 60 x 60 = 21,600 quads),
+  near field at r = 1.5 and the far-field pattern, against the oracle's FP64
+  field of the FP64 currents: at most 1e-6 per point, relative to
+  max(|E(x)|, 1e-3 max |E|) over the point set.
+"""
+import numpy as np
+import pytest
+
+import bielcc_amd as B
+import oracle as O
+from _util import block_metric, rowwise_rel
+
+pytestmark = pytest.mark.gpu
+K = 10.0
+MIXED_TOL = 1e-6
+
+
+def _floor_rel(a, b, scale):
+    """Per row |a - b| / max(|b|, 1e-3 scale) for [n][3] complex rows."""
+    nb = np.linalg.norm(b, axis=1)
+    return np.linalg.norm(a - b, axis=1) / np.maximum(nb, 1e-3 * scale)
+
+
+def _row_strips(m, orc, verts, npp_gpu, npp_orc, k, n2, starts, nrows=64):
+    worst = 0.0
+    for r0 in starts:
+        Zg = m.assemble(npp_gpu, k, row0=r0, nrows=nrows)
+        Zo = orc.assemble(verts, npp_orc, k, row0=r0, nrows=nrows)
+        assert Zg.shape == (nrows, n2)
+        assert np.isfinite(Zg).all()
+        worst = max(worst, float(block_metric(Zg, Zo).max()))
+    return worst
+
+
+def _residual(ctx, dZ, n2, b, rhs, strip_cols=2048):
+    """||Z b - rhs|| / ||rhs|| with the column-major device matrix read back in
+    column strips (no host copy of the whole matrix)."""
+    r = -np.asarray(rhs, dtype=np.complex128).copy()
+    for c0 in range(0, n2, strip_cols):
+        c1 = min(n2, c0 + strip_cols)
+        S = dZ.to_numpy((c1 - c0, n2), np.complex128, offset=c0 * n2 * 16)   # columns c0..c1-1
+        r += S.T @ b[c0:c1]
+    return float(np.linalg.norm(r) / np.linalg.norm(rhs))
+
+
+@pytest.fixture(scope="module")
+def sphere20k(ctx):
+    verts = B.mesh.sphere_quads(100, 200, radius=1.0, jitter=0.005, seed=1)
+    m = ctx.mesh(verts)
+    yield verts, m
+    m.close()
+
+
+def test_sphere20k_matrix_row_strips(sphere20k, orc):
+    verts, m = sphere20k
+    n2 = 2 * verts.shape[0]
+    # first strip: north pole cells (degenerate quads); middle: equator; last: south pole
+    worst = _row_strips(m, orc, verts, B.num_param(), O.num_param(), K, n2, (0, 20000, n2 - 64))
+    assert worst <= 1e-12, worst
+
+
+def test_sphere20k_field_samples(sphere20k, orc):
+    verts, m = sphere20k
+    n = verts.shape[0]
+    rng = np.random.default_rng(31)
+    j = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = B.mesh.grid_points(200, 500, 5.0)
+    E, H = m.field(B.num_param(), K, j, pts)
+    assert np.isfinite(E).all() and np.isfinite(H).all()
+    sel = np.sort(rng.choice(len(pts), 128, replace=False))
+    Eo, Ho = orc.field_eh(verts, j, K, O.num_param(), pts[sel])
+    assert rowwise_rel(E[sel], Eo).max() <= 1e-10
+    assert rowwise_rel(H[sel], Ho).max() <= 1e-10
+
+
+@pytest.fixture(scope="module")
+def sphere20k_solved(ctx, sphere20k):
+    """FP64 LU solve of the plane-wave system (k along +z, E along x)."""
+    verts, m = sphere20k
+    n = verts.shape[0]
+    n2 = 2 * n
+    npp = B.num_param()
+    dZ = ctx.empty(n2 * n2 * 16)
+    m.assemble_into(dZ, n2, npp, K)
+    dLU, dpiv = ctx.empty(n2 * n2 * 16), ctx.empty(n2 * 4)
+    ctx.copy(dLU, dZ, n2 * n2 * 16)
+    ctx.lu_factor(dLU, n2, n2, dpiv)
+    rhs = m.rhs_planewave((0.0, 0.0, K), (1.0, 0.0, 0.0))[:, 0]
+    dB = ctx.to_device(rhs)
+    ctx.lu_solve(dLU, n2, n2, dpiv, dB, 1, n2)
+    b = dB.to_numpy((n2,), np.complex128)
+    dLU.free()
+    yield dict(verts=verts, m=m, dZ=dZ, dpiv=dpiv, rhs=rhs, b=b, n2=n2)
+    dZ.free()
+    dpiv.free()
+
+
+def test_sphere20k_fp64_solve_residual(ctx, sphere20k_solved):
+    s = sphere20k_solved
+    res = _residual(ctx, s["dZ"], s["n2"], s["b"], s["rhs"])
+    assert res <= 1e-12, res
+
+
+def _mixed_solve(ctx, m, dZ, n2, kvec, e0, tol=1e-10):
+    """FP32 LU of the FP64 matrix + FP64 refinement: the refined currents."""
+    dF, dpiv = ctx.empty(n2 * n2 * 8), ctx.empty(n2 * 4)
+    ctx.lu32_factor(dZ, n2, n2, dF, n2, dpiv)
+    dB = ctx.empty(n2 * 16)
+    m.rhs_planewave_into(dB, n2, kvec, e0)
+    it, resid = ctx.refine_solve(dZ, n2, n2, dF, n2, dpiv, dB, 1, n2, tol, 30)
+    b = dB.to_numpy((n2,), np.complex128)
+    for d in (dF, dpiv, dB):
+        d.free()
+    return b, it, resid
+
+
+def test_sphere20k_precision_form(ctx, orc, sphere20k_solved):
+    s = sphere20k_solved
+    verts, m, n2 = s["verts"], s["m"], s["n2"]
+    b32, it, resid = _mixed_solve(ctx, m, s["dZ"], n2, (0.0, 0.0, K), (1.0, 0.0, 0.0))
+    assert resid <= 1e-10
+    assert np.linalg.norm(b32 - s["b"]) <= 1e-8 * np.linalg.norm(s["b"])
+    j64 = m.currents(s["b"])[0]
+    j32 = m.currents(b32)[0]
+    pts = B.mesh.grid_points(200, 500, 5.0)
+    npp = B.num_param()
+    E64, H64 = m.field_batch(npp, K, j64[None], pts)
+    E32, H32 = m.field_batch(npp, K, j32[None], pts, precision=B.PREC_MIXED)
+    rng = np.random.default_rng(32)
+    # sample: random points plus the 32 of smallest |E| (pattern nulls)
+    nE = np.linalg.norm(E64[0], axis=1)
+    sel = np.unique(np.concatenate([rng.choice(len(pts), 96, replace=False), np.argsort(nE)[:32]]))
+    Eo, Ho = orc.field_eh(verts, j64, K, O.num_param(), pts[sel])
+    assert rowwise_rel(E64[0][sel], Eo).max() <= 1e-10
+    eE = _floor_rel(E32[0][sel], Eo, nE.max()).max()
+    eH = _floor_rel(H32[0][sel], Ho, np.linalg.norm(H64[0], axis=1).max()).max()
+    assert eE <= MIXED_TOL and eH <= MIXED_TOL, (eE, eH)
+
+
+@pytest.fixture(scope="module")
+def sphere40k_tri(ctx):
+    quads = B.mesh.sphere_quads(100, 200, radius=1.0, jitter=0.005, seed=1)
+    verts = B.mesh.triangulate(quads)
+    m = ctx.mesh(verts)
+    yield verts, m
+    m.close()
+
+
+def test_sphere_triangles_variant(sphere40k_tri):
+    verts, m = sphere40k_tri
+    # 2 x 20,000 triangles, one zero-area triangle per pole quad (2 x 200) dropped
+    assert verts.shape == (39600, 3, 3)
+    assert len(B.mesh.triangulate(B.mesh.sphere_quads(100, 200, jitter=0.005, seed=1), drop_degenerate=False)) == 40000
+    assert np.isfinite(m.tables()["norm"]).all()
+
+
+def test_sphere_triangles_matrix_row_strips(sphere40k_tri, orc):
+    verts, m = sphere40k_tri
+    n2 = 2 * verts.shape[0]
+    worst = _row_strips(m, orc, verts, B.num_param(), O.num_param(), K, n2, (0, 39600, n2 - 64))
+    assert worst <= 1e-12, worst
+
+
+def test_sphere_triangles_field_samples(sphere40k_tri, orc):
+    verts, m = sphere40k_tri
+    n = verts.shape[0]
+    rng = np.random.default_rng(33)
+    j = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = B.mesh.grid_points(200, 500, 5.0)
+    sel = np.sort(rng.choice(len(pts), 96, replace=False))
+    E, H = m.field(B.num_param(), K, j, pts)
+    assert np.isfinite(E).all() and np.isfinite(H).all()
+    Eo, Ho = orc.field_eh(verts, j, K, O.num_param(), pts[sel])
+    assert rowwise_rel(E[sel], Eo).max() <= 1e-10
+    assert rowwise_rel(H[sel], Ho).max() <= 1e-10
+
+
+def _far_dirs(step_deg=4):
+    th = np.deg2rad(np.arange(0, 181, step_deg))[:, None]
+    ph = np.deg2rad(np.arange(0, 360, step_deg))[None, :]
+    d = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th) * np.ones_like(ph)], axis=-1)
+    return np.ascontiguousarray(d.reshape(-1, 3))
+
+
+def test_cube_precision_form(ctx, orc):
+    verts = B.mesh.cube_quads(60, side=2.0)
+    n = verts.shape[0]
+    n2 = 2 * n
+    assert n == 21600
+    m = ctx.mesh(verts)
+    npp = B.num_param()
+    kvec, e0 = (0.0, 0.0, K), (1.0, 0.0, 0.0)
+    dZ = ctx.empty(n2 * n2 * 16)
+    m.assemble_into(dZ, n2, npp, K)
+    b32, it, resid = _mixed_solve(ctx, m, dZ, n2, kvec, e0)
+    assert resid <= 1e-10
+    dpiv = ctx.empty(n2 * 4)
+    ctx.lu_factor(dZ, n2, n2, dpiv)
+    dB = ctx.empty(n2 * 16)
+    m.rhs_planewave_into(dB, n2, kvec, e0)
+    ctx.lu_solve(dZ, n2, n2, dpiv, dB, 1, n2)
+    b64 = dB.to_numpy((n2,), np.complex128)
+    for d in (dZ, dpiv, dB):
+        d.free()
+    assert np.linalg.norm(b32 - b64) <= 1e-8 * np.linalg.norm(b64)
+    j64, j32 = m.currents(b64)[0], m.currents(b32)[0]
+    # near field at r = 1.5: about a third of the points lie inside the cube or
+    # within a few cell diameters of a face (exact near pairs)
+    pts = B.mesh.sphere_points(10000, 1.5, seed=21)
+    E64, H64 = m.field_batch(npp, K, j64[None], pts)
+    E32, H32 = m.field_batch(npp, K, j32[None], pts, precision=B.PREC_MIXED)
+    dface = 1.0 - np.abs(pts).max(axis=1)          # > 0 inside the cube
+    rng = np.random.default_rng(34)
+    sel = np.unique(np.concatenate([rng.choice(len(pts), 64, replace=False), np.argsort(np.abs(dface))[:32],
+                                    np.argsort(np.linalg.norm(E64[0], axis=1))[:32]]))
+    Eo, Ho = orc.field_eh(verts, j64, K, O.num_param(), pts[sel])
+    assert rowwise_rel(E64[0][sel], Eo).max() <= 1e-10
+    assert rowwise_rel(H64[0][sel], Ho).max() <= 1e-10
+    eE = _floor_rel(E32[0][sel], Eo, np.linalg.norm(E64[0], axis=1).max()).max()
+    eH = _floor_rel(H32[0][sel], Ho, np.linalg.norm(H64[0], axis=1).max()).max()
+    # far-field pattern on a 4-degree grid
+    dirs = _far_dirs()
+    F64, G64 = m.far_pattern(npp, K, j64, dirs)
+    F32, G32 = m.far_pattern(npp, K, j32, dirs, precision=B.PREC_MIXED)
+    Fo, Go = orc.far_pattern(verts, j64, K, dirs[::7])
+    assert rowwise_rel(F64[::7], Fo).max() <= 1e-10
+    eF = _floor_rel(F32, F64, np.linalg.norm(F64, axis=1).max()).max()
+    m.close()
+    assert eE <= MIXED_TOL and eH <= MIXED_TOL and eF <= MIXED_TOL, (eE, eH, eF)
