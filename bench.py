@@ -289,16 +289,40 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
             "near_r1.5_and_far_pattern": nearfar}
 
 
+def _lu_probe_error(dF, n, piv, AX, X, dtype, itemsize, strip=2048):
+    """||P A X - L (U X)|| / ||P A X|| for the device factors dF (n x n column-major,
+    read back in column strips) and the pivots piv: a sampled backward error."""
+    y = np.zeros_like(X)
+    for c0 in range(0, n, strip):
+        c1 = min(n, c0 + strip)
+        S = dF.to_numpy((c1 - c0, n), dtype, offset=c0 * n * itemsize).T.astype(np.complex128)
+        y += np.triu(S, -c0) @ X[c0:c1]
+    z = y.copy()
+    for c0 in range(0, n, strip):
+        c1 = min(n, c0 + strip)
+        S = dF.to_numpy((c1 - c0, n), dtype, offset=c0 * n * itemsize).T.astype(np.complex128)
+        z += np.tril(S, -c0 - 1) @ y[c0:c1]
+    PAX = AX.copy()
+    for j in range(n):
+        p = int(piv[j])
+        if p != j:
+            PAX[[j, p]] = PAX[[p, j]]
+    return float(np.linalg.norm(PAX - z) / np.linalg.norm(PAX))
+
+
 def lu_only(n, reps=3):
     """FP64 and FP32 LU of a seeded random complex n x n matrix (entries N(0,1) + i N(0,1)),
-    each factored reps times from the same copy; the fastest of each is reported."""
+    each factored reps times from the same copy; the fastest of each is reported, with
+    the factors' sampled backward error ||P A X - L U X|| / ||P A X|| on 4 random columns X."""
     import bielcc_amd as B
     ctx = B.Context(0)
     rng = np.random.default_rng(1)
     A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    X = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
+    AX = A @ X
     dA0 = ctx.to_device(np.ascontiguousarray(A.T))
     del A
-    dA, piv, dF = ctx.empty(n * n * 16), ctx.empty(4 * n), ctx.empty(n * n * 8)
+    dA, piv, piv32, dF = ctx.empty(n * n * 16), ctx.empty(4 * n), ctx.empty(4 * n), ctx.empty(n * n * 8)
 
     def timed(fn):
         ctx.sync()
@@ -311,14 +335,17 @@ def lu_only(n, reps=3):
         ctx.copy(dA, dA0, n * n * 16)
         ctx.sync()
         t64.append(timed(lambda: ctx.lu_factor(dA, n, n, piv)))
-        t32.append(timed(lambda: ctx.lu32_factor(dA0, n, n, dF, n, piv)))
+        t32.append(timed(lambda: ctx.lu32_factor(dA0, n, n, dF, n, piv32)))
     flop = 8.0 / 3.0 * n ** 3
+    be64 = _lu_probe_error(dA, n, piv.to_numpy((n,), np.int32), AX, X, np.complex128, 16)
+    be32 = _lu_probe_error(dF, n, piv32.to_numpy((n,), np.int32), AX, X, np.complex64, 8)
     out = {"n": n, "lu64_s": round(min(t64), 4), "lu64_tflops": round(flop / min(t64) / 1e12, 2),
            "lu32_s": round(min(t32), 4), "lu32_tflops": round(flop / min(t32) / 1e12, 2),
+           "backward_err_probe_fp64": be64, "backward_err_probe_fp32": be32,
            "panel_aborts": ctx.get_tuning("lu_panel_aborts")}
-    log(f"[bench] LU n={n}: FP64 {out['lu64_s']} s ({out['lu64_tflops']} TFLOP/s), FP32 {out['lu32_s']} s "
-        f"({out['lu32_tflops']} TFLOP/s)")
-    for d in (dA0, dA, piv, dF):
+    log(f"[bench] LU n={n}: FP64 {out['lu64_s']} s ({out['lu64_tflops']} TFLOP/s, backward {be64:.1e}), "
+        f"FP32 {out['lu32_s']} s ({out['lu32_tflops']} TFLOP/s, backward {be32:.1e})")
+    for d in (dA0, dA, piv, piv32, dF):
         d.free()
     ctx.close()
     return out

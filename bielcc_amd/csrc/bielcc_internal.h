@@ -218,6 +218,8 @@ void lu_set_look_ahead(LuWork*, int mode);
 int lu_get_look_ahead(const LuWork*);
 // whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
 void lu_set_panel_lds(LuWork*, int mode);   // 1 within the workgroup budget, 0 never, 2 always
+void lu_set_panel_rows32(LuWork*, int);
+int lu_get_panel_rows32(const LuWork*);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)
@@ -241,6 +243,7 @@ void lu_set_panel_wait_us(LuWork*, int us);
 int lu_get_panel_wait_us(const LuWork*);
 int lu_panel_aborts(LuWork*);
 int lu_late_admits(const LuWork*);
+size_t lu_work_bytes(const LuWork*);
 // returns hipSuccess / error; info = 0 or first zero pivot (1-based); msg on library failure
 hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* A, long long lda, int* piv, int* info,
                      const char** msg);

@@ -1,5 +1,6 @@
 // Implementation of the C ABI declared in include/bielcc_hip.h.
 #include "ctx_internal.h"
+#include <climits>
 
 using namespace bielcc_impl;
 using namespace bielcc_capi;
@@ -631,6 +632,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_blas_atomics(ctx->lu, value == 1);
         return BIELCC_OK;
     }
+    if (k == "lu_panel_rows32") {
+        if (value != 0 && value != 64 && value != 128) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_rows32 must be 0, 64 or 128");
+        lu_set_panel_rows32(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "lu_panel_lds") {
         if (value < 0 || value > 2) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_lds must be 0, 1 or 2");
         lu_set_panel_lds(ctx->lu, value);
@@ -686,6 +692,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_fused_panel") *value = lu_get_fused_panel(ctx->lu);
     else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu);
+    else if (k == "lu_panel_rows32") *value = lu_get_panel_rows32(ctx->lu);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu);
@@ -707,6 +714,11 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "plan_ws_mb") *value = ctx->plan_ws_mb;
     else if (k == "opmat_min_rhs") *value = ctx->opmat_min_rhs;
     else if (k == "n_cu") *value = ctx->n_cu;
+    else if (k == "scratch_mb") {   // the context's grow-only device buffers, MiB rounded up
+        const size_t b = ctx->scratch_cap + ctx->lists_cap + ctx->near_cap + ctx->opmat_ws_cap +
+                         ctx->opmat_meta_cap + ctx->opmat_part_cap + lu_work_bytes(ctx->lu);
+        *value = (int)std::min<size_t>((b + (1u << 20) - 1) >> 20, (size_t)INT_MAX);
+    }
     else return fail(ctx, BIELCC_ERR_ARG, "unknown tuning key " + k);
     return BIELCC_OK;
 }

@@ -93,6 +93,8 @@ struct LuWork {
     int2* smap = nullptr;           // the current panel's interchanges as a gather (2 kNB entries + count)
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
     int panel_lds = 1;              // whole-panel kernel: 1 within the workgroup budget, 0 never, 2 always
+    int panel_rows32 = 0;           // rows per workgroup of FP32 whole panels: 0 by height (64, or 128 when 64
+                                    // would need more workgroups than fit), 64 or 128 forced (tests)
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
     int dev = 0, n_cu = 0, clk_khz = 0;
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
@@ -120,6 +122,8 @@ void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
 void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
 void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
+void lu_set_panel_rows32(LuWork* w, int rows) { w->panel_rows32 = rows; }
+int lu_get_panel_rows32(const LuWork* w) { return w->panel_rows32; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
 void lu_set_outer_nb(LuWork* w, int nb) { w->outer_nb = nb; }
 void lu_set_trsv_limits(LuWork* w, int max64, int chunks32, int max32) {
@@ -137,6 +141,9 @@ int lu_get_trsm_lib_panel(const LuWork* w) { return w->lu_trsm_lib; }
 int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 int lu_late_admits(const LuWork* w) { return w->late_admits; }
+size_t lu_work_bytes(const LuWork* w) {   // grow-only device work of the LU and the solves
+    return w ? w->perm_cap + w->ref_cap + (size_t)w->pcap * (sizeof(double) + sizeof(int)) : 0;
+}
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
 int lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
 void lu_set_panel_wait_us(LuWork* w, int us) { w->panel_wait_us = us; }
@@ -1125,7 +1132,11 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // whole-panel kernel for panels of at most kPanelMaxWG workgroups, two per compute unit;
     // 64 rows per workgroup, or 128 for FP32 panels that need more than gmax of 64
     const int gmax = std::min(kPanelMaxWG, 2 * w->n_cu);
-    auto rows_per_wg = [&](int m) { return (sizeof(R) == 4 && (m + 63) / 64 > gmax) ? 128 : 64; };
+    auto rows_per_wg = [&](int m) {
+        if (sizeof(R) != 4) return 64;
+        if (w->panel_rows32 == 64 || w->panel_rows32 == 128) return w->panel_rows32;
+        return (m + 63) / 64 > gmax ? 128 : 64;
+    };
     auto wgs_for = [&](int m) { const int r = rows_per_wg(m); return (m + r - 1) / r; };
     LuRunning running(w->dev);
     PanelAdmit admit;

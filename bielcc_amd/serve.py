@@ -90,24 +90,37 @@ class Budget:
         self.used = 0
         self.cv = threading.Condition()
 
-    def reserve(self, nbytes: int, evict: Callable[[int], int], timeout: float = 600.0) -> None:
-        """Block until nbytes fit; evict(need) frees idle cache bytes and returns
-        how many it freed.  Raises if nbytes exceeds the whole budget."""
+    def reserve(self, nbytes: int, evict: Callable[[int], tuple], timeout: float = 600.0) -> None:
+        """Block until nbytes fit.  evict(need) runs under the lock: it takes
+        idle cache entries out of the cache and returns (bytes, buffers).  The
+        buffers are freed after the lock is released (a device free waits for
+        the device, and other workers must not wait on the lock meanwhile); their
+        bytes stay counted as used until they are freed, so the device never
+        holds more than the budget.  Raises if nbytes exceeds the whole budget."""
         nbytes = int(nbytes)
         if nbytes > self.total:
             raise MemoryError(f"request of {nbytes} B exceeds the HBM budget of {self.total} B")
         deadline = time.monotonic() + timeout
-        with self.cv:
-            while self.used + nbytes > self.total:
-                freed = evict(self.used + nbytes - self.total)
-                if freed > 0:
-                    self.used -= freed
-                    continue
-                left = deadline - time.monotonic()
-                if left <= 0:
-                    raise TimeoutError("HBM budget: reservation did not fit in time")
-                self.cv.wait(timeout=min(left, 1.0))
-            self.used += nbytes
+        while True:
+            victims, vbytes = [], 0
+            with self.cv:
+                while self.used + nbytes > self.total:
+                    freed, bufs = evict(self.used + nbytes - self.total)
+                    if freed > 0:
+                        victims, vbytes = bufs, freed
+                        break
+                    left = deadline - time.monotonic()
+                    if left <= 0:
+                        raise TimeoutError("HBM budget: reservation did not fit in time")
+                    self.cv.wait(timeout=min(left, 1.0))
+                else:
+                    self.used += nbytes
+                    return
+            try:
+                for b in victims:
+                    b.free()
+            finally:
+                self.release(vbytes)
 
     def release(self, nbytes: int) -> None:
         with self.cv:
@@ -136,21 +149,22 @@ class FactorCache:
         self.entries: "collections.OrderedDict[tuple, _Entry]" = collections.OrderedDict()
         self.evictions = 0
 
-    def evict(self, need: int) -> int:
-        """Free idle entries (least recently used first) until `need` bytes are
-        freed or none is left; returns the bytes freed.  Caller holds budget.cv."""
-        freed = 0
+    def evict(self, need: int) -> tuple:
+        """Take idle entries (least recently used first) out of the cache until
+        `need` bytes are covered or none is left; returns (bytes, buffers): the
+        caller frees the buffers outside the lock.  Caller holds budget.cv."""
+        freed, bufs = 0, []
         for key in list(self.entries.keys()):
             if freed >= need:
                 break
             e = self.entries[key]
             if e.refs == 0 and e.ready:
-                for b in e.bufs.values():
-                    b.free()
+                bufs.extend(e.bufs.values())
+                e.bufs = {}
                 del self.entries[key]
                 freed += e.nbytes
                 self.evictions += 1
-        return freed
+        return freed, bufs
 
     def pin(self, key, nbytes: int) -> tuple:
         """Pin the entry of key (creating it, not yet built, if absent).
@@ -185,17 +199,19 @@ class FactorCache:
         """Builder: mark e ready, or failed (its buffers freed, the entry dropped;
         the builder returns the entry's bytes to the budget itself)."""
         cv = self.budget.cv
+        bufs = []
         with cv:
             if exc is None:
                 e.ready = True
             else:
                 e.failed = exc
-                for b in e.bufs.values():
-                    b.free()
+                bufs = list(e.bufs.values())
                 e.bufs = {}
                 if self.entries.get(e.key) is e:
                     del self.entries[e.key]
             cv.notify_all()
+        for b in bufs:          # device frees outside the lock
+            b.free()
 
     def release(self, e: _Entry) -> None:
         with self.budget.cv:
@@ -319,6 +335,14 @@ class Server:
         pool[name] = d
         return d
 
+    def _charge_scratch(self, w: int, ctx: api.Context) -> None:
+        """Charge what the context's grow-only work buffers (field partial sums,
+        near lists, operator matrices, solve work) hold beyond what was charged."""
+        actual = ctx.get_tuning("scratch_mb") << 20
+        if actual > self.scratch_hw[w]:
+            self.budget.reserve(actual - self.scratch_hw[w], self.cache.evict)
+            self.scratch_hw[w] = actual
+
     def _factor_bytes(self, n: int) -> int:
         n2 = 2 * n
         per = 16 + 8 if self.form == "mixed" else 16   # FP64 matrix (+ FP32 factors)
@@ -344,6 +368,24 @@ class Server:
             ctx.lu_factor(e.bufs["Z"], n2, n2, e.bufs["piv"])
         ctx.sync()
 
+    def _build_entry(self, w: int, m: api.Mesh, k: float, e: _Entry, fbytes: int) -> None:
+        """Builder of entry e: reserve its bytes (waiting callers of the same key
+        hold none), build it and mark it ready; on failure drop it and return
+        every byte it held (e.nbytes includes FP64 factors the build added)."""
+        try:
+            self.budget.reserve(fbytes, self.cache.evict)
+        except BaseException as exc:
+            self.cache.finish(e, exc)
+            raise
+        try:
+            self._build(w, m, k, e)
+        except BaseException as exc:
+            nb = e.nbytes
+            self.cache.finish(e, exc)
+            self.budget.release(nb)
+            raise
+        self.cache.finish(e)
+
     def _upgrade(self, w: int, e: _Entry, reason: str = "") -> None:
         """FP64 factors beside the FP32 ones (mixed form), for a system FP32
         factors cannot refine.  Done once per entry."""
@@ -356,10 +398,18 @@ class Server:
             n2 = int(round(np.sqrt(e.bufs["Z"].nbytes / 16)))
             extra = 16 * n2 * n2 + 4 * n2
             self.budget.reserve(extra, self.cache.evict)
-            lu, piv = ctx.empty(16 * n2 * n2), ctx.empty(4 * n2)
-            ctx.copy(lu, e.bufs["Z"], 16 * n2 * n2)
-            ctx.lu_factor(lu, n2, n2, piv)
-            ctx.sync()
+            lu = piv = None
+            try:
+                lu, piv = ctx.empty(16 * n2 * n2), ctx.empty(4 * n2)
+                ctx.copy(lu, e.bufs["Z"], 16 * n2 * n2)
+                ctx.lu_factor(lu, n2, n2, piv)
+                ctx.sync()
+            except BaseException:
+                for b in (lu, piv):
+                    if b is not None:
+                        b.free()
+                self.budget.release(extra)
+                raise
             with self.budget.cv:
                 e.bufs["LU64"], e.bufs["piv64"] = lu, piv
                 e.nbytes += extra
@@ -398,21 +448,8 @@ class Server:
         e, builder = self.cache.pin((q.mesh, float(q.k)), fbytes)
         try:
             if builder:
-                # the factors' bytes are reserved before anything is built; waiting
-                # callers of the same key hold no bytes
-                try:
-                    self.budget.reserve(fbytes, self.cache.evict)
-                except BaseException as exc:
-                    self.cache.finish(e, exc)
-                    raise
                 t0 = time.perf_counter()
-                try:
-                    self._build(w, m, q.k, e)
-                except BaseException as exc:
-                    self.cache.finish(e, exc)
-                    self.budget.release(fbytes)
-                    raise
-                self.cache.finish(e)
+                self._build_entry(w, m, q.k, e, fbytes)
                 ph["build"] = time.perf_counter() - t0
             else:
                 t0 = time.perf_counter()
@@ -455,6 +492,7 @@ class Server:
             m.field_batch_into(self.np, q.k, nrhs, dJ, dP, M, dE, dH, inc=inc, precision=prec)
             ctx.sync()
             ph["field"] = time.perf_counter() - t0
+            self._charge_scratch(w, ctx)
             t0 = time.perf_counter()
             E = dE.to_numpy((nrhs, M, 3), np.complex128)
             H = dH.to_numpy((nrhs, M, 3), np.complex128)

@@ -389,7 +389,10 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * first panel and were admitted at a later one),
  * "trsv_expired" and "trsv_ticket_faults" (one-kernel triangular solves
  * redone on the blocked path because a wait expired, or because the ticket
- * counter was out of step).  Returns BIELCC_ERR_ARG for an unknown key or a
+ * counter was out of step), and "scratch_mb" (device bytes the context holds
+ * in its grow-only work buffers - field partial sums, near lists, operator
+ * matrices, LU and solve work - in MiB, rounded up; a caller that budgets
+ * device memory charges it).  Returns BIELCC_ERR_ARG for an unknown key or a
  * value out of range. */
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);
 int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);

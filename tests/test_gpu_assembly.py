@@ -377,6 +377,27 @@ def test_whole_panel_kernel_matches_column_kernels(ctx, n):
                                       np.array_equal(a[1], b[1]), np.array_equal(a[3], b[3]))
 
 
+@pytest.mark.parametrize("n", [300, 1000, 2500])
+def test_fp32_whole_panel_two_rows_per_lane(ctx, n):
+    """FP32 whole panels with 128 rows per workgroup, two per lane (by default
+    used only for panels of more than 512 x 64 = 32,768 rows), forced here at
+    small orders: the same FP32 factors and pivots, bit for bit, as the
+    per-column launches, and a backward error ||P A - L U|| / ||A|| at FP32
+    roundoff."""
+    rng = np.random.default_rng(12)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = _factors(ctx, A, lu_panel_lds=2, lu_panel_rows32=128, lu_look_ahead=0)
+    b = _factors(ctx, A, lu_panel_lds=0, lu_look_ahead=0)
+    assert ctx.get_tuning("lu_panel_rows32") == 0
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[2], b[2])
+    F, p = a[2].T.astype(np.complex128), a[3]
+    PA = A.copy()
+    for j in range(n):
+        PA[[j, p[j]]] = PA[[p[j], j]]
+    err = np.linalg.norm(PA - (np.tril(F, -1) + np.eye(n)) @ np.triu(F)) / np.linalg.norm(A)
+    assert err <= 1e-4, err
+
+
 def test_whole_panel_abort_falls_back_to_one_workgroup(ctx):
     """A zero wait bound makes the whole-panel launches abort at their first
     barrier (forced from the host; every wait still ends).  The last workgroup

@@ -100,3 +100,109 @@ def test_build_failure_reaches_waiters_and_drops_the_entry():
     with pytest.raises(RuntimeError):
         c.wait_ready(e2)
     assert "k" not in c.entries
+
+
+def test_evicted_buffers_are_freed_outside_the_lock():
+    """A device free waits for the device: it must not run while the budget's
+    lock is held (every other worker would stall in pin, release and reserve).
+    The evicted bytes stay counted until the free is done."""
+    b = Budget(100)
+    c = FactorCache(b)
+    seen = []
+
+    class CheckBuf:
+        def free(self):
+            got = b.cv.acquire(blocking=False)   # the lock is not reentrant: fails if this thread holds it
+            if got:
+                b.cv.release()
+            seen.append((got, b.used))
+
+    e, _ = c.pin("a", 60)
+    b.reserve(60, c.evict)
+    e.bufs = {"Z": CheckBuf(), "piv": CheckBuf()}
+    c.finish(e)
+    c.release(e)
+    b.reserve(70, c.evict)
+    assert [g for g, _ in seen] == [True, True]
+    assert all(u == 60 for _, u in seen)        # still charged while being freed
+    assert b.used == 70 and "a" not in c.entries and c.evictions == 1
+
+
+class _FakeDev:
+    def __init__(self, log, nbytes):
+        self.log, self.nbytes = log, nbytes
+
+    def free(self):
+        self.log.append(self.nbytes)
+
+
+class _FakeCtx:
+    """The context calls the server makes, with a chosen one failing."""
+
+    def __init__(self, log, fail_lu64=False, fail_lu32=False):
+        self.log, self.fail_lu64, self.fail_lu32 = log, fail_lu64, fail_lu32
+
+    def empty(self, nbytes):
+        return _FakeDev(self.log, nbytes)
+
+    def copy(self, dst, src, nbytes):
+        pass
+
+    def lu_factor(self, A, n, lda, piv):
+        if self.fail_lu64:
+            from bielcc_amd.api import BielccError
+            raise BielccError(5, "zero pivot")
+
+    def lu32_factor(self, A, n, lda, F, ldf, piv):
+        if self.fail_lu32:
+            from bielcc_amd.api import BielccError
+            raise BielccError(5, "zero FP32 pivot")
+
+    def sync(self):
+        pass
+
+
+def _bare_server(ctx, form="mixed", total=10 ** 9):
+    import collections
+    from bielcc_amd.serve import Server
+    s = Server.__new__(Server)
+    s.form, s.budget = form, Budget(total)
+    s.cache = FactorCache(s.budget)
+    s.ctxs, s.lock, s.upgrades = [ctx], threading.Lock(), []
+    s.np = None
+    return s
+
+
+def test_failed_upgrade_returns_its_bytes_and_buffers():
+    from bielcc_amd.api import BielccError
+    log = []
+    s = _bare_server(_FakeCtx(log, fail_lu64=True))
+    e, _ = s.cache.pin(("m", 1.0), 100)
+    e.bufs = {"Z": _FakeDev([], 16 * 30 * 30)}
+    before = s.budget.used
+    with pytest.raises(BielccError):
+        s._upgrade(0, e, "stalled")
+    assert s.budget.used == before
+    assert sorted(log) == sorted([16 * 30 * 30, 4 * 30])   # the FP64 factors and pivots were freed
+    assert "LU64" not in e.bufs and e.nbytes == 100
+
+
+def test_failed_build_returns_the_entrys_bytes():
+    """A build whose FP32 LU meets a zero pivot gets FP64 factors (its entry
+    grows); if the FP64 LU then fails too, every byte goes back."""
+    from bielcc_amd.api import BielccError
+    log = []
+    s = _bare_server(_FakeCtx(log, fail_lu32=True, fail_lu64=True))
+
+    class M:
+        n = 15
+
+        def assemble_into(self, *a, **k):
+            pass
+    key = ("m", 1.0)
+    fbytes = s._factor_bytes(M.n)
+    e, builder = s.cache.pin(key, fbytes)
+    with pytest.raises(BielccError):
+        s._build_entry(0, M(), 1.0, e, fbytes)
+    assert s.budget.used == 0 and key not in s.cache.entries
+    assert len(s.upgrades) == 1                 # the FP64 factors were tried
