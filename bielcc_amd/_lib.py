@@ -75,6 +75,13 @@ _SIGS = {
     "bielcc_analytic_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                         C.POINTER(IntegralParam), C.c_double, C.c_double, C.c_double, C.c_void_p]),
     "bielcc_plane_wave_e": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _DP, _DP, C.c_void_p]),
+    "bielcc_plane_wave_h": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _DP, _DP, C.c_double, C.c_double,
+                                      C.c_double, C.c_void_p]),
+    "bielcc_field_add_incident_h": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _DP, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p]),
+    "bielcc_integral_y_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                          C.POINTER(KernelParam), _DP, C.POINTER(IntegralParam), C.c_void_p,
+                                          C.c_void_p]),
     "bielcc_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double,
                                   C.c_int, C.c_int, C.c_void_p, C.c_int64]),
     "bielcc_rhs_planewave": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP, _DP, C.c_void_p, C.c_int64]),

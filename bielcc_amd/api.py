@@ -247,6 +247,18 @@ class Context:
                                                    _p(dr), _p(ds)))
         return dr.to_numpy((n, 3), np.complex128), ds.to_numpy((n,))
 
+    def integral_y_batch(self, f, cells, kp: KernelParam, ip: IntegralParam, e0=(0.0, 0.0, 0.0)):
+        """IntegralUniversal F(y) forms over cells [n][nv][3] (nv = 4, 3) or the F(y)
+        segment form over [n][2][3]; f 0 = the constant 1, 1 = PlaneWaveFieldE
+        (kp.vec_dbl = wave vector, e0 = polarisation).  Returns (res [n][3], smoothR [n])."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64)
+        n, nv = cells.shape[0], cells.shape[1]
+        dc, dr, ds = self.to_device(cells), self.empty(max(n, 1) * 48), self.empty(max(n, 1) * 8)
+        ev, _a = _vec3(e0)
+        self._check(self.lib.bielcc_integral_y_batch(self.h, int(f), nv, n, _p(dc), C.byref(kp), ev, C.byref(ip),
+                                                     _p(dr), _p(ds)))
+        return dr.to_numpy((n, 3), np.complex128), ds.to_numpy((n,))
+
     def k0_batch(self, variant, f, cells, x, j, kp: KernelParam, ip: IntegralParam, analyticR=0.0):
         f = KERNELS[f] if isinstance(f, str) else int(f)
         cells = np.ascontiguousarray(cells, dtype=np.float64)
@@ -288,6 +300,26 @@ class Context:
         ev, _b = _vec3(e0)
         self._check(self.lib.bielcc_plane_wave_e(self.h, n, _p(dp), kv, ev, _p(dr)))
         return dr.to_numpy((n, 3), np.complex128)
+
+    def plane_wave_h(self, pts, kvec, e0, freq, mu=1.0) -> np.ndarray:
+        """PlaneWaveFieldH at pts [n][3]; mu the absolute permeability (real or complex)."""
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        dp, dr = self.to_device(pts), self.empty(max(n, 1) * 48)
+        kv, _a = _vec3(kvec)
+        ev, _b = _vec3(e0)
+        mu = complex(mu)
+        self._check(self.lib.bielcc_plane_wave_h(self.h, n, _p(dp), kv, ev, float(freq), mu.real, mu.imag, _p(dr)))
+        return dr.to_numpy((n, 3), np.complex128)
+
+    def add_incident_h(self, M: int, nrhs: int, pts: DeviceArray, inc, freq, H: DeviceArray, mu=1.0, scale=1.0):
+        """H[r][p] = scale H[r][p] + PlaneWaveFieldH(x_p) of incidence r (device H [nrhs][M][3],
+        device pts [M][3], host inc [nrhs][6] = (kvec, e0))."""
+        inc = np.ascontiguousarray(inc, dtype=np.float64).reshape(nrhs, 6)
+        mu, scale = complex(mu), complex(scale)
+        self._check(self.lib.bielcc_field_add_incident_h(self.h, int(M), int(nrhs), _p(pts),
+                                                         inc.ctypes.data_as(C.POINTER(C.c_double)), float(freq),
+                                                         mu.real, mu.imag, scale.real, scale.imag, _p(H)))
 
     # -- LU (device buffers)
     def lu_factor(self, A: DeviceArray, n: int, lda: int, piv: DeviceArray) -> None:

@@ -18,6 +18,9 @@
 //   GetFieldE_Ideal           electrodynamics/get_field.h:35-78
 //   SurfaceCurrents_byBasis   electrodynamics/currents.h:22-45
 //   PlaneWaveFieldE           electrodynamics/incident_field.h:20-29
+//   PlaneWaveFieldH, MaterialParam (its permeability)  incident_field.h:40-59, material.h:73-190
+//   IntegralUniversal (F(y) forms)         integrators/integral_universal.h:252-431
+//   IntegralUniversalSegPnt (F(y) form)    integrators/integral_universal_seg.h:120-179
 // Semantics kept: outputs are overwritten; KernelParam& is mutated (smoothR)
 // exactly as the reference's integrators do; NaN/Inf propagate.  Errors of the
 // GPU layer throw std::runtime_error (the reference throws only from IO).
@@ -37,6 +40,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <variant>
 #include <vector>
 
 namespace bielcc {
@@ -168,6 +172,65 @@ private:
     size_t n_ = 0;
 };
 
+// Staging of one single-function call: the inputs packed into one host
+// array, one upload into a per-thread grow-only device buffer, the outputs
+// after the inputs, one download.  A caller that loops over points does no
+// device allocation per call (the reference functions are called per pair).
+class Call {
+public:
+    size_t in(const double* h, size_t n) {
+        const size_t o = host_.size();
+        host_.insert(host_.end(), h, h + n);
+        n_in_ = host_.size();
+        return o;
+    }
+    size_t out(size_t n) {
+        const size_t o = host_.size();
+        host_.resize(o + n, 0.);
+        return o;
+    }
+    // uploads the inputs; returns the device base (valid until the next call on this thread)
+    double* upload() {
+        auto& c = default_context();
+        buf_ = &buffer(host_.size());
+        check(c.get(), bielcc_memcpy(c.get(), buf_->p, host_.data(), n_in_ * sizeof(double), 0), "bielcc_memcpy");
+        return static_cast<double*>(buf_->p);
+    }
+    // downloads everything after the inputs; returns the host copy
+    const double* download() {
+        auto& c = default_context();
+        const size_t n = host_.size() - n_in_;
+        if (n)
+            check(c.get(), bielcc_memcpy(c.get(), host_.data() + n_in_, static_cast<double*>(buf_->p) + n_in_,
+                                         n * sizeof(double), 1), "bielcc_memcpy");
+        return host_.data();
+    }
+
+private:
+    struct Buf {
+        void* p = nullptr;
+        size_t cap = 0;
+        ~Buf() { if (p) bielcc_free(default_context().get(), p); }
+    };
+    static Buf& buffer(size_t n_doubles) {
+        thread_local Buf b;
+        const size_t need = std::max<size_t>(n_doubles, 1) * sizeof(double);
+        if (b.cap < need) {
+            auto& c = default_context();
+            if (b.p) bielcc_free(c.get(), b.p);
+            b.p = nullptr;
+            b.cap = 0;
+            const size_t cap = std::max<size_t>(need, 4096);
+            check(c.get(), bielcc_malloc(c.get(), cap, &b.p), "bielcc_malloc");
+            b.cap = cap;
+        }
+        return b;
+    }
+    std::vector<double> host_;
+    size_t n_in_ = 0;
+    Buf* buf_ = nullptr;
+};
+
 inline double kre(double k) { return k; }
 inline double kim(double) { return 0.; }
 inline double kre(const std::complex<double>& k) { return k.real(); }
@@ -219,11 +282,12 @@ struct Integrand {
     template <typename KType, typename P>
     void operator()(const double* x, const double* y, const KernelParam<KType>& kp, P* res) const {
         auto& c = hip::default_context();
-        hip::DevBuf dx(x, 3), dy(y, 3), dr(6);
+        hip::Call call;
+        const size_t ix = call.in(x, 3), iy = call.in(y, 3), ir = call.out(6);
+        double* d = call.upload();
         bielcc_kernel_param k = hip::to_kp(kp);
-        hip::check(c.get(), bielcc_kernel_batch(c.get(), Id, 1, dx.ptr(), dy.ptr(), &k, dr.ptr()), "bielcc_kernel_batch");
-        double o[6];
-        dr.download(o, 6);
+        hip::check(c.get(), bielcc_kernel_batch(c.get(), Id, 1, d + ix, d + iy, &k, d + ir), "bielcc_kernel_batch");
+        const double* o = call.download() + ir;
         for (int g = 0; g < 3; g++) res[g] = hip::get_c<P>(o + 2 * g);
     }
 };
@@ -247,16 +311,17 @@ template <typename P, typename KType, int Id, bool R>
 void integral(int nv, const double* cellv, const double* x, Integrand<Id, R>, KernelParam<KType>& kp,
               const IntegralParam& ip, P* res) {
     auto& c = hip::default_context();
-    hip::DevBuf dc(cellv, (size_t)3 * (nv == 2 ? 2 : nv)), dx(x, 3), dr(6), ds(1);
+    hip::Call call;
+    const size_t ic = call.in(cellv, (size_t)3 * (nv == 2 ? 2 : nv)), ix = call.in(x, 3);
+    const size_t ir = call.out(6), is = call.out(1);
+    double* d = call.upload();
     bielcc_kernel_param k = hip::to_kp(kp);
     bielcc_integral_param p{ip.GetIDim(), ip.GetNStart(), ip.GetPMax(), ip.GetEpsAccur()};
-    hip::check(c.get(), bielcc_integral_batch(c.get(), Id, nv, 1, dc.ptr(), dx.ptr(), &k, &p, dr.ptr(), ds.ptr()),
+    hip::check(c.get(), bielcc_integral_batch(c.get(), Id, nv, 1, d + ic, d + ix, &k, &p, d + ir, d + is),
                "bielcc_integral_batch");
-    double o[6], R_out;
-    dr.download(o, 6);
-    ds.download(&R_out, 1);
-    for (int g = 0; g < ip.GetIDim(); g++) res[g] = hip::get_c<P>(o + 2 * g);
-    kp.smoothR = R_out;
+    const double* h = call.download();
+    for (int g = 0; g < ip.GetIDim(); g++) res[g] = hip::get_c<P>(h + ir + 2 * g);
+    kp.smoothR = h[is];
 }
 }  // namespace detail
 
@@ -292,16 +357,17 @@ void k0(int variant, int id, const P* j, const double* x, const double (&cell)[C
     double cv[12], jj[6] = {0, 0, 0, 0, 0, 0};
     hip::flat(cell, cv);
     if (j) for (int g = 0; g < 3; g++) hip::put_c(j[g], jj + 2 * g);
-    hip::DevBuf dc(cv, 3 * CP), dx(x, 3), dj(jj, 6), dr(6), ds(1);
+    hip::Call call;
+    const size_t ic = call.in(cv, 3 * CP), ix = call.in(x, 3), ij = call.in(jj, 6);
+    const size_t ir = call.out(6), is = call.out(1);
+    double* d = call.upload();
     bielcc_kernel_param k = hip::to_kp(kp);
     bielcc_integral_param p{ip.GetIDim(), ip.GetNStart(), ip.GetPMax(), ip.GetEpsAccur()};
-    hip::check(c.get(), bielcc_k0_batch(c.get(), variant, id, (int)CP, 1, dc.ptr(), dx.ptr(), dj.ptr(), &k, &p,
-                                        analyticR, dr.ptr(), ds.ptr()), "bielcc_k0_batch");
-    double o[6], R_out;
-    dr.download(o, 6);
-    ds.download(&R_out, 1);
-    for (int g = 0; g < 3; g++) res[g] = hip::get_c<P>(o + 2 * g);
-    kp.smoothR = R_out;
+    hip::check(c.get(), bielcc_k0_batch(c.get(), variant, id, (int)CP, 1, d + ic, d + ix, d + ij, &k, &p,
+                                        analyticR, d + ir, d + is), "bielcc_k0_batch");
+    const double* h = call.download();
+    for (int g = 0; g < 3; g++) res[g] = hip::get_c<P>(h + ir + 2 * g);
+    kp.smoothR = h[is];
 }
 }  // namespace detail
 
@@ -331,12 +397,15 @@ namespace detail {
 inline void analytic(int which, int nv, const double* cellv, const double* x, const IntegralParam* ip,
                      double analyticR, double kr, double ki, double* o) {
     auto& c = hip::default_context();
-    hip::DevBuf dc(cellv, (size_t)3 * nv), dx(x, 3), dr(6);
+    hip::Call call;
+    const size_t ic = call.in(cellv, (size_t)3 * nv), ix = call.in(x, 3), ir = call.out(6);
+    double* d = call.upload();
     bielcc_integral_param p{1, 8, 1, 0.};
     if (ip) p = bielcc_integral_param{ip->GetIDim(), ip->GetNStart(), ip->GetPMax(), ip->GetEpsAccur()};
-    hip::check(c.get(), bielcc_analytic_batch(c.get(), which, nv == 2 ? 4 : nv, 1, dc.ptr(), dx.ptr(), &p, analyticR,
-                                              kr, ki, dr.ptr()), "bielcc_analytic_batch");
-    dr.download(o, 6);
+    hip::check(c.get(), bielcc_analytic_batch(c.get(), which, nv == 2 ? 4 : nv, 1, d + ic, d + ix, &p, analyticR,
+                                              kr, ki, d + ir), "bielcc_analytic_batch");
+    const double* h = call.download() + ir;
+    for (int i = 0; i < 6; i++) o[i] = h[i];
 }
 }  // namespace detail
 
@@ -382,12 +451,13 @@ void op(int id, const JType* j, const double* x, const double (&cell)[CP][3], co
     double cv[12], jj[6];
     hip::flat(cell, cv);
     for (int g = 0; g < 3; g++) hip::put_c(j[g], jj + 2 * g);
-    hip::DevBuf dc(cv, 3 * CP), dx(x, 3), dj(jj, 6), dr(6);
+    hip::Call call;
+    const size_t ic = call.in(cv, 3 * CP), ix = call.in(x, 3), ij = call.in(jj, 6), ir = call.out(6);
+    double* d = call.upload();
     bielcc_num_param p = hip::to_np(np);
-    hip::check(c.get(), bielcc_op_batch(c.get(), id, (int)CP, 1, dc.ptr(), dx.ptr(), dj.ptr(), &p, hip::kre(k),
-                                        hip::kim(k), dr.ptr()), "bielcc_op_batch");
-    double o[6];
-    dr.download(o, 6);
+    hip::check(c.get(), bielcc_op_batch(c.get(), id, (int)CP, 1, d + ic, d + ix, d + ij, &p, hip::kre(k),
+                                        hip::kim(k), d + ir), "bielcc_op_batch");
+    const double* o = call.download() + ir;
     for (int g = 0; g < 3; g++) res[g] = {o[2 * g], o[2 * g + 1]};
 }
 }  // namespace detail
@@ -595,12 +665,107 @@ public:
 template <typename KType>
 void PlaneWaveFieldE(const double* x, const KernelParam<KType>& param, std::complex<double>* e_inc) {
     auto& c = hip::default_context();
-    hip::DevBuf dx(x, 3), dr(6);
-    hip::check(c.get(), bielcc_plane_wave_e(c.get(), 1, dx.ptr(), param.vec_dbl, param.e0, dr.ptr()),
+    hip::Call call;
+    const size_t ix = call.in(x, 3), ir = call.out(6);
+    double* d = call.upload();
+    hip::check(c.get(), bielcc_plane_wave_e(c.get(), 1, d + ix, param.vec_dbl, param.e0, d + ir),
                "bielcc_plane_wave_e");
-    double o[6];
-    dr.download(o, 6);
+    const double* o = call.download() + ir;
     for (int g = 0; g < 3; g++) e_inc[g] = {o[2 * g], o[2 * g + 1]};
+}
+
+// MaterialParam (electrodynamics/material.h:73-190): the permeability
+// PlaneWaveFieldH reads.  Calc(freq) sets mu = mu0 mu_r (material.cpp:105-116).
+enum class ChangeMaterialTypeDir { Isotropic, Anisotropic };
+enum class ChangeMaterialTypeFreq { NonDispersive, Dispersive, DispersiveByRefractIndex };
+class MaterialParam {
+public:
+    MaterialParam() = default;
+    MaterialParam(std::string name, ChangeMaterialTypeDir, ChangeMaterialTypeFreq, double eps_r, double mu_r,
+                  double sigma, double tand, double tanm)
+        : _name(std::move(name)), _eps_r(eps_r), _mu_r(mu_r), _sigma(sigma), _tand(tand), _tanm(tanm) {}
+    void Calc(double /*freq*/) { _mu = 4 * M_PI * 1e-7 * _mu_r; _tanm = 0.; }
+    std::variant<double, std::complex<double>> GetMu() const { return _mu; }
+    void SetMu(std::variant<double, std::complex<double>> mu) { _mu = mu; }
+
+private:
+    std::string _name;
+    double _eps_r = 0., _mu_r = 0., _sigma = 0., _tand = 0., _tanm = 0.;
+    std::variant<double, std::complex<double>> _mu = 0.;
+};
+
+// PlaneWaveFieldH (incident_field.h:40-59)
+template <typename KType>
+void PlaneWaveFieldH(const double* x, const KernelParam<KType>& param, const MaterialParam& material_param,
+                     std::complex<double>* h_inc) {
+    auto& c = hip::default_context();
+    const auto mu = material_param.GetMu();
+    const std::complex<double> m = std::holds_alternative<double>(mu) ? std::complex<double>(std::get<double>(mu), 0.)
+                                                                      : std::get<std::complex<double>>(mu);
+    hip::Call call;
+    const size_t ix = call.in(x, 3), ir = call.out(6);
+    double* d = call.upload();
+    hip::check(c.get(), bielcc_plane_wave_h(c.get(), 1, d + ix, param.vec_dbl, param.e0, param.freq, m.real(),
+                                            m.imag(), d + ir), "bielcc_plane_wave_h");
+    const double* o = call.download() + ir;
+    for (int g = 0; g < 3; g++) h_inc[g] = {o[2 * g], o[2 * g + 1]};
+}
+
+// The constant F(y) integrand (the area test of examples/integrators/main0_integrals.cpp:77-81),
+// the F(y) integrand besides PlaneWaveFieldE that the integrators below run on the GPU.
+template <typename KType>
+void FY_Unit(const double*, const KernelParam<KType>&, double* ff) { ff[0] = 1.; }
+
+// IntegralUniversal, F(y) forms (integral_universal.h:252-431), and the F(y) form of
+// IntegralUniversalSegPnt (integral_universal_seg.h:120-179).  The integrand must be
+// PlaneWaveFieldE<KType> or FY_Unit<KType>; any other host function throws.
+namespace detail {
+template <typename P, typename KType>
+void integral_y(int nv, const double* cellv, void (*f_0)(const double*, const KernelParam<KType>&, P*),
+                KernelParam<KType>& kp, const IntegralParam& ip, P* res) {
+    int f = -1;
+    if constexpr (std::is_same<P, std::complex<double>>::value) {
+        if (f_0 == &PlaneWaveFieldE<KType>) f = BIELCC_FY_PLANE_WAVE_E;
+    } else {
+        if (f_0 == &FY_Unit<KType>) f = BIELCC_FY_UNIT;
+    }
+    if (f < 0) throw std::runtime_error("IntegralUniversal: only PlaneWaveFieldE and FY_Unit run on the GPU");
+    auto& c = hip::default_context();
+    hip::Call call;
+    const size_t ic = call.in(cellv, (size_t)3 * (nv == 2 ? 2 : nv)), ir = call.out(6), is = call.out(1);
+    double* d = call.upload();
+    bielcc_kernel_param k = hip::to_kp(kp);
+    bielcc_integral_param p{ip.GetIDim(), ip.GetNStart(), ip.GetPMax(), ip.GetEpsAccur()};
+    hip::check(c.get(), bielcc_integral_y_batch(c.get(), f, nv, 1, d + ic, &k, kp.e0, &p, d + ir, d + is),
+               "bielcc_integral_y_batch");
+    const double* h = call.download();
+    for (int g = 0; g < ip.GetIDim(); g++) res[g] = hip::get_c<P>(h + ir + 2 * g);
+    kp.smoothR = h[is];
+}
+}  // namespace detail
+
+template <typename P, typename KType>
+void IntegralUniversal(const double (&cell)[4][3], void (*f_0)(const double*, const KernelParam<KType>&, P*),
+                       KernelParam<KType>& ker_param, const IntegralParam& int_param, P* res) {
+    double c[12];
+    hip::flat(cell, c);
+    detail::integral_y(4, c, f_0, ker_param, int_param, res);
+}
+
+template <typename P, typename KType>
+void IntegralUniversal(const double (&cell)[3][3], void (*f_0)(const double*, const KernelParam<KType>&, P*),
+                       KernelParam<KType>& ker_param, const IntegralParam& int_param, P* res) {
+    double c[9];
+    hip::flat(cell, c);
+    detail::integral_y(3, c, f_0, ker_param, int_param, res);
+}
+
+template <typename P, typename KType>
+void IntegralUniversalSegPnt(const double* a, const double* b,
+                             void (*f_0)(const double*, const KernelParam<KType>&, P*),
+                             KernelParam<KType>& ker_param, const IntegralParam& int_param, P* res) {
+    double s[6] = {a[0], a[1], a[2], b[0], b[1], b[2]};
+    detail::integral_y(2, s, f_0, ker_param, int_param, res);
 }
 
 // GetFieldE_Ideal (get_field.h:35-78).  The scattered sum runs on the GPU; the

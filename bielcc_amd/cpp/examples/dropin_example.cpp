@@ -51,6 +51,36 @@ int main(int argc, char** argv) {
     std::printf("smoothR_after %.17g\n", kp.smoothR);
     std::printf("integral1Divr %.17g\n", integral1Divr(rut1, x));
 
+    // F(y) forms: the area test of main0_integrals.cpp (rut1 = 5) and a plane wave over a cell
+    KernelParam<double> kp0;
+    kp0.smoothR = kp0.smoothRMin = 1e-16;
+    double area[1];
+    IntegralUniversal(rut1, FY_Unit<double>, kp0, IntegralParam(1, 8, 1, 1e-6), area);
+    std::printf("area_rut1 %.17g\n", area[0]);
+    KernelParam<double> pw;
+    pw.smoothR = pw.smoothRMin = 1e-16;
+    pw.vec_dbl[0] = 0.3; pw.vec_dbl[1] = -1.2; pw.vec_dbl[2] = 0.5;
+    pw.e0[0] = 1.; pw.e0[1] = 0.5; pw.e0[2] = 0.;
+    pw.freq = 0.2;
+    cd fy[3];
+    IntegralUniversal(rut1, PlaneWaveFieldE<double>, pw, IntegralParam(3, 8, 1, 1e-6), fy);
+    print("IntegralUniversal_PlaneWaveFieldE", fy, 3);
+    MaterialParam vac("vacuum", ChangeMaterialTypeDir::Isotropic, ChangeMaterialTypeFreq::NonDispersive, 1., 1., 0.,
+                      0., 0.);
+    vac.Calc(pw.freq);
+    cd h[3];
+    PlaneWaveFieldH(x, pw, vac, h);
+    print("PlaneWaveFieldH", h, 3);
+    // a reference-style loop of single-point calls: no device allocation per call
+    cd e_sum[3] = {0., 0., 0.};
+    for (int i = 0; i < 2000; i++) {
+        const double xi[3] = {0.001 * i, 0.5, -0.25};
+        cd e[3];
+        PlaneWaveFieldE(xi, pw, e);
+        for (int g = 0; g < 3; g++) e_sum[g] += e[g];
+    }
+    print("PlaneWaveFieldE_sum2000", e_sum, 3);
+
     // whole problem: mesh file -> currents -> GetFieldE_Ideal
     SurfMesh<CellQuad> mesh(argv[1]);
     std::printf("cells %d\n", mesh.GetNFrm());

@@ -270,10 +270,12 @@ __device__ inline bool pass_end(int p_n, int idim, double eps, cplx* res, cplx* 
     return false;
 }
 
-// IntegralUniversalPnt: quad integral_universal.h:38-120, triangle :136-220
-template <int F, int NV>
-__device__ void integral_pnt(const double (*c)[3], const double* x, KP& kp, int idim, int n_start,
-                             int p_max, double eps, cplx* res) {
+// The cell rules with the integrand as a functor ev(y, kp, ff).
+// F(x, y) forms, IntegralUniversalPnt: quad integral_universal.h:38-120, triangle :136-220;
+// F(y) forms, IntegralUniversal: quad :252-333, triangle :347-431 (the same rules).
+template <int NV, class Ev>
+__device__ void integral_rule(const double (*c)[3], Ev ev, KP& kp, int idim, int n_start, int p_max,
+                              double eps, cplx* res) {
     int n = n_start, p_n;
     cplx ff[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, prev[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
     for (int g = 0; g < idim; g++) res[g] = mk(0., 0.);
@@ -308,7 +310,7 @@ __device__ void integral_pnt(const double (*c)[3], const double* x, KP& kp, int 
                     }
                     cross3(m1, m2, rn);
                     double s = len3(rn);
-                    kernel_eval<F>(x, rc, kp, ff);
+                    ev(rc, kp, ff);
                     for (int g = 0; g < idim; g++) res[g] += ff[g] * s;
                 }
             }
@@ -328,7 +330,7 @@ __device__ void integral_pnt(const double (*c)[3], const double* x, KP& kp, int 
                         double CM = AM + qv[k];
                         rc[k] = (B[k] + AM + CM) / 3.0;
                     }
-                    kernel_eval<F>(x, rc, kp, ff);
+                    ev(rc, kp, ff);
                     for (int g = 0; g < idim; g++) res[g] += ff[g];
                     if (j < n - 1 - i) {
                         for (int k = 0; k < 3; k++) {
@@ -336,7 +338,7 @@ __device__ void integral_pnt(const double (*c)[3], const double* x, KP& kp, int 
                             double CP = B[k] + qv[k];
                             rc[k] = (B[k] + AP + CP) / 3.0;
                         }
-                        kernel_eval<F>(x, rc, kp, ff);
+                        ev(rc, kp, ff);
                         for (int g = 0; g < idim; g++) res[g] += ff[g];
                     }
                 }
@@ -350,10 +352,18 @@ __device__ void integral_pnt(const double (*c)[3], const double* x, KP& kp, int 
         for (int g = 0; g < idim; g++) res[g] = prev[g];
 }
 
-// IntegralUniversalSegPnt integral_universal_seg.h:38-98
-template <int F>
-__device__ void integral_seg(const double* a, const double* b, const double* x, KP& kp, int idim,
-                             int n_start, int p_max, double eps, cplx* res) {
+template <int F, int NV>
+__device__ void integral_pnt(const double (*c)[3], const double* x, KP& kp, int idim, int n_start,
+                             int p_max, double eps, cplx* res) {
+    integral_rule<NV>(c, [x](const double* y, const KP& k, cplx* ff) { kernel_eval<F>(x, y, k, ff); }, kp, idim,
+                      n_start, p_max, eps, res);
+}
+
+// The segment rule with the integrand as a functor ev(y, kp, ff): IntegralUniversalSegPnt,
+// F(x, y) form integral_universal_seg.h:38-98, F(y) form :120-179 (the same rule).
+template <class Ev>
+__device__ void integral_seg_rule(const double* a, const double* b, Ev ev, KP& kp, int idim, int n_start,
+                                  int p_max, double eps, cplx* res) {
     int n = n_start, p_n;
     cplx ff[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, prev[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
     for (int g = 0; g < idim; g++) res[g] = mk(0., 0.);
@@ -363,7 +373,7 @@ __device__ void integral_seg(const double* a, const double* b, const double* x, 
         double dl = len3(d);
         for (int i = 0; i < n; i++) {
             for (int j = 0; j < 3; j++) y[j] = a[j] + (i + 0.5) * d[j];
-            kernel_eval<F>(x, y, kp, ff);
+            ev(y, kp, ff);
             for (int g = 0; g < idim; g++) res[g] += ff[g] * dl;
         }
         if (pass_end(p_n, idim, eps, res, prev, kp)) break;
@@ -371,6 +381,13 @@ __device__ void integral_seg(const double* a, const double* b, const double* x, 
     }
     if (p_n == p_max)
         for (int g = 0; g < idim; g++) res[g] = prev[g];
+}
+
+template <int F>
+__device__ void integral_seg(const double* a, const double* b, const double* x, KP& kp, int idim,
+                             int n_start, int p_max, double eps, cplx* res) {
+    integral_seg_rule(a, b, [x](const double* y, const KP& k, cplx* ff) { kernel_eval<F>(x, y, k, ff); }, kp,
+                      idim, n_start, p_max, eps, res);
 }
 
 // ------------------------------------------------------------------ analytic

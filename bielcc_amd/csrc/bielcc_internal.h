@@ -194,6 +194,17 @@ hipError_t launch_analytic_batch(hipStream_t s, int which, int nv, int n, const 
                                  int n_start, int p_max, double eps, double analyticR, double k_re, double k_im,
                                  double* res);
 hipError_t launch_plane_wave(hipStream_t s, int n, const double* pts, const double* kvec_e0, double* out);
+// PlaneWaveFieldH parameters: ke = (kvec, e0), frequency, absolute permeability
+struct PlaneWaveH {
+    double ke[6];
+    double freq, mu_re, mu_im;
+};
+hipError_t launch_plane_wave_h(hipStream_t s, int n, const double* pts, const PlaneWaveH& pw, double* out);
+hipError_t launch_add_incident_h(hipStream_t s, int M, int nrhs, const double* pts, const double* inc, double freq,
+                                 double mu_re, double mu_im, double s_re, double s_im, double* H);
+hipError_t launch_integral_y_batch(hipStream_t s, int f, int nv, int n, const double* cells, const bielcc_dev::KP& kp,
+                                   const double* e0, int idim, int n_start, int p_max, double eps, double* res,
+                                   double* r_out);
 
 // ---- LU (lu.hip)
 struct LuWork;

@@ -45,6 +45,7 @@ int bielcc_ctx_destroy(bielcc_ctx ctx) {
         if (ctx->opmat_ws) (void)hipFree(ctx->opmat_ws);
         if (ctx->opmat_meta) (void)hipFree(ctx->opmat_meta);
         if (ctx->opmat_part) (void)hipFree(ctx->opmat_part);
+        if (ctx->small) (void)hipFree(ctx->small);
         lu_work_destroy(ctx->lu);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
@@ -245,6 +246,50 @@ int bielcc_plane_wave_e(bielcc_ctx ctx, int n, const double* pts_d, const double
     DeviceGuard g(ctx->device);
     double ke[6] = {kvec[0], kvec[1], kvec[2], e0[0], e0[1], e0[2]};
     HIPCHK(launch_plane_wave(ctx->stream, n, pts_d, ke, out_d));
+    return BIELCC_OK;
+}
+
+int bielcc_plane_wave_h(bielcc_ctx ctx, int n, const double* pts_d, const double* kvec, const double* e0,
+                        double freq, double mu_re, double mu_im, double* out_d) {
+    if (!ctx || !kvec || !e0 || n < 0) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if (n > 0 && (!pts_d || !out_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    PlaneWaveH pw{{kvec[0], kvec[1], kvec[2], e0[0], e0[1], e0[2]}, freq, mu_re, mu_im};
+    HIPCHK(launch_plane_wave_h(ctx->stream, n, pts_d, pw, out_d));
+    return BIELCC_OK;
+}
+
+int bielcc_field_add_incident_h(bielcc_ctx ctx, int M, int nrhs, const double* pts_d, const double* inc,
+                                double freq, double mu_re, double mu_im, double scale_re, double scale_im,
+                                double* H_d) {
+    if (!ctx || M < 0 || nrhs < 0) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if (M == 0 || nrhs == 0) return BIELCC_OK;
+    if (!pts_d || !inc || !H_d) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    const size_t b = (size_t)nrhs * 6 * sizeof(double);
+    HIPCHK(ensure(ctx->small, ctx->small_cap, b));
+    HIPCHK(hipMemcpyAsync(ctx->small, inc, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_add_incident_h(ctx->stream, M, nrhs, pts_d, static_cast<const double*>(ctx->small), freq, mu_re,
+                                 mu_im, scale_re, scale_im, H_d));
+    return BIELCC_OK;
+}
+
+int bielcc_integral_y_batch(bielcc_ctx ctx, int f, int nv, int n, const double* cells_d,
+                            const bielcc_kernel_param* kp, const double* e0, const bielcc_integral_param* ip,
+                            double* res_d, double* smoothR_d) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if ((f != BIELCC_FY_UNIT && f != BIELCC_FY_PLANE_WAVE_E) || n < 0 || !kp || !ip || nv < 2 || nv > 4)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if (f == BIELCC_FY_PLANE_WAVE_E && !e0) return fail(ctx, BIELCC_ERR_ARG, "PlaneWaveFieldE needs e0");
+    if (ip->idim < 1 || ip->idim > 3 || ip->n_start < 1 || ip->p_max < 1)
+        return fail(ctx, BIELCC_ERR_ARG, "idim must be 1..3, n_start and p_max >= 1");
+    if ((long long)ip->n_start << (ip->p_max - 1) > (nv == 2 ? (1 << 20) : 4096))
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED, "quadrature partition beyond 4096 (cell) / 2^20 (segment)");
+    if (n > 0 && (!cells_d || !res_d)) return fail(ctx, BIELCC_ERR_ARG, "NULL buffer");
+    DeviceGuard g(ctx->device);
+    const double zero[3] = {0., 0., 0.};
+    HIPCHK(launch_integral_y_batch(ctx->stream, f, nv, n, cells_d, to_kp(kp), e0 ? e0 : zero, ip->idim,
+                                   ip->n_start, ip->p_max, ip->eps, res_d, smoothR_d));
     return BIELCC_OK;
 }
 
@@ -714,10 +759,10 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "plan_ws_mb") *value = ctx->plan_ws_mb;
     else if (k == "opmat_min_rhs") *value = ctx->opmat_min_rhs;
     else if (k == "n_cu") *value = ctx->n_cu;
-    else if (k == "scratch_mb") {   // the context's grow-only device buffers, MiB rounded up
+    else if (k == "scratch_kb") {   // the context's grow-only device buffers, KiB rounded up
         const size_t b = ctx->scratch_cap + ctx->lists_cap + ctx->near_cap + ctx->opmat_ws_cap +
-                         ctx->opmat_meta_cap + ctx->opmat_part_cap + lu_work_bytes(ctx->lu);
-        *value = (int)std::min<size_t>((b + (1u << 20) - 1) >> 20, (size_t)INT_MAX);
+                         ctx->opmat_meta_cap + ctx->opmat_part_cap + ctx->small_cap + lu_work_bytes(ctx->lu);
+        *value = (int)std::min<size_t>((b + 1023) >> 10, (size_t)INT_MAX);
     }
     else return fail(ctx, BIELCC_ERR_ARG, "unknown tuning key " + k);
     return BIELCC_OK;

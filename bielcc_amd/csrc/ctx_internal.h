@@ -39,6 +39,8 @@ struct bielcc_ctx_s {
     size_t opmat_meta_cap = 0;
     void* opmat_part = nullptr;   // its split-K GEMM partial products (grow-only)
     size_t opmat_part_cap = 0;
+    void* small = nullptr;        // staging of small host arrays (incidences) (grow-only)
+    size_t small_cap = 0;
     int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     // plan (multi-mesh) tunables: cells per field tile and per assembly tile,
     // persistent workgroups per CU, operator-matrix workspace cap (MiB), and the

@@ -576,6 +576,67 @@ hipError_t launch_plane_wave(hipStream_t s, int n, const double* pts, const doub
 }
 
 
+// PlaneWaveFieldH (incident_field.h:40-59): (vec_dbl x e0) e^{i vec_dbl.x} / (2 pi freq) / mu,
+// mu the absolute permeability (MaterialParam::GetMu, real or complex).
+__device__ inline void plane_wave_h(const double* x, const double* ke, double freq, cplx mu, bool mu_cplx,
+                                    cplx* h) {
+    double vp[3];
+    cross3(ke, ke + 3, vp);   // vec_prod(param.vec_dbl, param.e0)
+    const double ph = ke[0] * x[0] + ke[1] * x[1] + ke[2] * x[2];
+    cplx deg = cexp(cplx{0., 1.} * ph);
+    deg = deg / (2. * kPi * freq);
+    if (mu_cplx) {   // std::complex division
+        const double den = mu.re * mu.re + mu.im * mu.im;
+        deg = cplx{(deg.re * mu.re + deg.im * mu.im) / den, (deg.im * mu.re - deg.re * mu.im) / den};
+    } else {
+        deg = deg / mu.re;
+    }
+    for (int g = 0; g < 3; g++) h[g] = vp[g] * deg;
+}
+
+__global__ void k_plane_wave_h(int n, const double* __restrict__ pts, PlaneWaveH pw, double* __restrict__ out) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    cplx h[3];
+    plane_wave_h(pts + 3 * (size_t)p, pw.ke, pw.freq, mk(pw.mu_re, pw.mu_im), pw.mu_im != 0., h);
+    for (int g = 0; g < 3; g++) { out[6 * (size_t)p + 2 * g] = h[g].re; out[6 * (size_t)p + 2 * g + 1] = h[g].im; }
+}
+
+hipError_t launch_plane_wave_h(hipStream_t s, int n, const double* pts, const PlaneWaveH& pw, double* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plane_wave_h, dim3((n + 255) / 256), dim3(256), 0, s, n, pts, pw, out);
+    return hipGetLastError();
+}
+
+// H[r][p] = scale H[r][p] + PlaneWaveFieldH(x_p) of incidence r, inc[r] = (kvec, e0) on the
+// device; H is [nrhs][M][3] complex.
+__global__ __launch_bounds__(256) void k_add_incident_h(int M, const double* __restrict__ pts,
+                                                        const double* __restrict__ inc, double freq, double mu_re,
+                                                        double mu_im, double s_re, double s_im,
+                                                        double* __restrict__ H) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (p >= M) return;
+    cplx h[3];
+    plane_wave_h(pts + 3 * (size_t)p, inc + 6 * r, freq, mk(mu_re, mu_im), mu_im != 0., h);
+    double* o = H + ((size_t)r * M + p) * 6;
+    const cplx sc = mk(s_re, s_im);
+    for (int g = 0; g < 3; g++) {
+        const cplx v = sc * mk(o[2 * g], o[2 * g + 1]) + h[g];
+        o[2 * g] = v.re;
+        o[2 * g + 1] = v.im;
+    }
+}
+
+hipError_t launch_add_incident_h(hipStream_t s, int M, int nrhs, const double* pts, const double* inc, double freq,
+                                 double mu_re, double mu_im, double s_re, double s_im, double* H) {
+    if (M <= 0 || nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_incident_h, dim3((M + 255) / 256, nrhs), dim3(256), 0, s, M, pts, inc, freq, mu_re,
+                       mu_im, s_re, s_im, H);
+    return hipGetLastError();
+}
+
+
 // ---------------------------------------------------------------- operator matrix
 // Field of many current sets: per segment (one mesh, a range of points) the
 // kernels write the operator matrices

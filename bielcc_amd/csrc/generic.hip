@@ -153,6 +153,62 @@ hipError_t launch_integral_batch(hipStream_t s, int f, int nv, int n, const doub
     return hipGetLastError();
 }
 
+// F(y) integrands of the IntegralUniversal forms (integral_universal.h:252-431,
+// integral_universal_seg.h:120-179):
+//   BIELCC_FY_UNIT  ff[0] = 1, ff[1..2] = 0 (the constant integrand f1 of
+//                   examples/integrators/main0_integrals.cpp:77-81: the integral is the area / length);
+//   BIELCC_FY_PLANE_WAVE_E  ff = e0 e^{i vec_dbl.y} (PlaneWaveFieldE, incident_field.h:20-29).
+struct FyEval {
+    int f;
+    double e0[3];
+    __device__ void operator()(const double* y, const KP& kp, cplx* ff) const {
+        if (f == 0) {
+            ff[0] = mk(1., 0.);
+            ff[1] = mk(0., 0.);
+            ff[2] = mk(0., 0.);
+            return;
+        }
+        // scal_prod(param.vec_dbl, x): element_geom.cpp:10-12
+        const double ph = kp.vec[0] * y[0] + kp.vec[1] * y[1] + kp.vec[2] * y[2];
+        const cplx deg = cexp(cplx{0., 1.} * ph);
+        for (int g = 0; g < 3; g++) ff[g] = deg * e0[g];
+    }
+};
+
+// nv = 4 / 3: IntegralUniversal over cells [n][nv][3]; nv = 2: the F(y) form of
+// IntegralUniversalSegPnt over segments [n][2][3].  r_out (optional): the mutated smoothR.
+template <int NV>
+__global__ void k_integral_y_batch(FyEval ev, int n, const double* __restrict__ cells, KP kp0, int idim,
+                                   int n_start, int p_max, double eps, double* __restrict__ res,
+                                   double* __restrict__ r_out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    KP kp = kp0;
+    cplx r[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    if constexpr (NV == 2) {
+        const double* ab = cells + 6 * i;
+        integral_seg_rule(ab, ab + 3, ev, kp, idim, n_start, p_max, eps, r);
+    } else {
+        double cv[NV][3];
+        load_cell<NV>(cells, i, cv);
+        integral_rule<NV>(cv, ev, kp, idim, n_start, p_max, eps, r);
+    }
+    store3(r, 3, res + 6 * i);
+    if (r_out) r_out[i] = kp.R;
+}
+
+hipError_t launch_integral_y_batch(hipStream_t s, int f, int nv, int n, const double* cells, const KP& kp,
+                                   const double* e0, int idim, int n_start, int p_max, double eps, double* res,
+                                   double* r_out) {
+    if (n <= 0) return hipSuccess;
+    FyEval ev{f, {e0[0], e0[1], e0[2]}};
+    dim3 g((n + 63) / 64), b(64);
+    if (nv == 4) hipLaunchKernelGGL(k_integral_y_batch<4>, g, b, 0, s, ev, n, cells, kp, idim, n_start, p_max, eps, res, r_out);
+    else if (nv == 3) hipLaunchKernelGGL(k_integral_y_batch<3>, g, b, 0, s, ev, n, cells, kp, idim, n_start, p_max, eps, res, r_out);
+    else hipLaunchKernelGGL(k_integral_y_batch<2>, g, b, 0, s, ev, n, cells, kp, idim, n_start, p_max, eps, res, r_out);
+    return hipGetLastError();
+}
+
 // variant 0: K0 (K0.h:36-86), 1: K0_Common (:112-166), 2: K0_GradSimplePot_H_HS (:197-249)
 template <int NV>
 __global__ void k_k0_batch(int variant, int f, int n, const double* __restrict__ cells,

@@ -65,8 +65,8 @@ class Query:
 @dataclasses.dataclass
 class Result:
     qid: int
-    E: np.ndarray          # [nrhs][M][3] complex, incident field included
-    H: np.ndarray          # [nrhs][M][3] complex, scattered only
+    E: np.ndarray          # [nrhs][M][3] complex, total field (incident included)
+    H: np.ndarray          # [nrhs][M][3] complex, total field (incident included)
     arrival: float
     start: float
     end: float
@@ -121,6 +121,21 @@ class Budget:
                     b.free()
             finally:
                 self.release(vbytes)
+
+    def charge(self, nbytes: int, evict: Callable[[int], tuple]) -> None:
+        """Count bytes that are already allocated (no waiting: the caller may hold
+        pins that others wait for); idle entries are evicted to get back under
+        the budget as far as possible."""
+        with self.cv:
+            self.used += int(nbytes)
+            over = self.used - self.total
+            freed, victims = evict(over) if over > 0 else (0, [])
+        try:
+            for b in victims:
+                b.free()
+        finally:
+            if freed:
+                self.release(freed)
 
     def release(self, nbytes: int) -> None:
         with self.cv:
@@ -338,9 +353,9 @@ class Server:
     def _charge_scratch(self, w: int, ctx: api.Context) -> None:
         """Charge what the context's grow-only work buffers (field partial sums,
         near lists, operator matrices, solve work) hold beyond what was charged."""
-        actual = ctx.get_tuning("scratch_mb") << 20
+        actual = ctx.get_tuning("scratch_kb") << 10
         if actual > self.scratch_hw[w]:
-            self.budget.reserve(actual - self.scratch_hw[w], self.cache.evict)
+            self.budget.charge(actual - self.scratch_hw[w], self.cache.evict)
             self.scratch_hw[w] = actual
 
     def _factor_bytes(self, n: int) -> int:
@@ -490,6 +505,10 @@ class Server:
             # exterior field cancels: FP32 far sums would lose that cancellation
             prec = api.PREC_MIXED if self.form == "mixed" and not fallback else api.PREC_FP64
             m.field_batch_into(self.np, q.k, nrhs, dJ, dP, M, dE, dH, inc=inc, precision=prec)
+            # normalised units (eps = mu = 1, omega = k): the solved currents are
+            # j' = multE j_E with multE = i/k, so E = K[j'] + E_inc is the total E and
+            # H = R[j_E] + H_inc = R[j'] / multE + PlaneWaveFieldH (freq = k / 2 pi, mu = 1)
+            ctx.add_incident_h(M, nrhs, dP, inc, q.k / (2.0 * np.pi), dH, mu=1.0, scale=-1j * q.k)
             ctx.sync()
             ph["field"] = time.perf_counter() - t0
             self._charge_scratch(w, ctx)

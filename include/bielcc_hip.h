@@ -147,6 +147,23 @@ int bielcc_integral_batch(bielcc_ctx ctx, int f, int nv, int n, const double* ce
                           const bielcc_kernel_param* kp, const bielcc_integral_param* ip, double* res_d,
                           double* smoothR_d);
 
+/* F(y) integrands of the IntegralUniversal forms. */
+enum {
+    BIELCC_FY_UNIT = 0,        /* f = 1 (examples/integrators/main0_integrals.cpp:77-81): area / length */
+    BIELCC_FY_PLANE_WAVE_E = 1 /* PlaneWaveFieldE (incident_field.h:20-29): e0 e^{i vec_dbl . y} */
+};
+
+/* IntegralUniversal, the F(y) forms (integral_universal.h:252-333 quads,
+ * :347-431 triangles; nv = 4 or 3, cells_d [n][nv][3]) or the F(y) form of
+ * IntegralUniversalSegPnt (integral_universal_seg.h:120-179; nv = 2,
+ * cells_d = segments [n][2][3]).  f: BIELCC_FY_*.  kp->vec_dbl is the plane
+ * wave's wave vector and e0 (host [3]) its polarisation (KernelParam::e0; read
+ * for BIELCC_FY_PLANE_WAVE_E only).  res_d [n][3] complex (entries past idim
+ * 0); smoothR_d[i] (optional) receives the mutated smoothR. */
+int bielcc_integral_y_batch(bielcc_ctx ctx, int f, int nv, int n, const double* cells_d,
+                            const bielcc_kernel_param* kp, const double* e0, const bielcc_integral_param* ip,
+                            double* res_d, double* smoothR_d);
+
 /* variant 0: K0 (K0.h:36-86), 1: K0_Common (:112-166), 2: K0_GradSimplePot_H_HS
  * (:197-249, uses analyticR and kp->k only).  ip->idim must be 3. */
 int bielcc_k0_batch(bielcc_ctx ctx, int variant, int f, int nv, int n, const double* cells_d, const double* x_d,
@@ -164,6 +181,23 @@ int bielcc_analytic_batch(bielcc_ctx ctx, int which, int nv, int n, const double
 /* PlaneWaveFieldE (incident_field.h:20-29): out [n][3] complex. */
 int bielcc_plane_wave_e(bielcc_ctx ctx, int n, const double* pts_d, const double* kvec, const double* e0,
                         double* out_d);
+
+/* PlaneWaveFieldH (incident_field.h:40-59): (kvec x e0) e^{i kvec . x} /
+ * (2 pi freq) / mu at n points, out [n][3] complex.  mu is the absolute
+ * permeability MaterialParam::GetMu returns: real (mu_im = 0, divided as a
+ * double, the reference's double branch) or complex.  In the normalised units
+ * of the solver (eps = mu = 1, omega = k) freq = k / (2 pi) and mu = 1. */
+int bielcc_plane_wave_h(bielcc_ctx ctx, int n, const double* pts_d, const double* kvec, const double* e0,
+                        double freq, double mu_re, double mu_im, double* out_d);
+
+/* Total magnetic field from a scattered one: H[r][p] = scale * H[r][p] +
+ * PlaneWaveFieldH(x_p) of incidence r (inc host [nrhs][6] = (kvec, e0), the
+ * layout of bielcc_field_batch), for H_d [nrhs][M][3] complex on the device.
+ * With currents solved for mult = 1 (j' = multE j_E, multE = i / (omega eps)),
+ * the Stratton-Chu H of j_E is R[j'] / multE: scale = 1 / multE. */
+int bielcc_field_add_incident_h(bielcc_ctx ctx, int M, int nrhs, const double* pts_d, const double* inc,
+                                double freq, double mu_re, double mu_im, double scale_re, double scale_im,
+                                double* H_d);
 
 /* ------------------------------------------------------------ RWG topology (host side, no context)
  * RwgSurfMesh (mesh/rwg_surf_mesh.h:300-526) of a cell list [n][nv][3]: unique
@@ -389,9 +423,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * first panel and were admitted at a later one),
  * "trsv_expired" and "trsv_ticket_faults" (one-kernel triangular solves
  * redone on the blocked path because a wait expired, or because the ticket
- * counter was out of step), and "scratch_mb" (device bytes the context holds
+ * counter was out of step), and "scratch_kb" (device bytes the context holds
  * in its grow-only work buffers - field partial sums, near lists, operator
- * matrices, LU and solve work - in MiB, rounded up; a caller that budgets
+ * matrices, LU and solve work - in KiB, rounded up; a caller that budgets
  * device memory charges it).  Returns BIELCC_ERR_ARG for an unknown key or a
  * value out of range. */
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);

@@ -318,9 +318,14 @@ static int pass_end(int p_n, int idim, double eps, cplx* res, cplx* prev, orc_ke
     return 0;
 }
 
-/* IntegralUniversalPnt, quad integral_universal.h:38-120, triangle :136-220 */
-static void integral_pnt(int f, int nv, const double (*c)[3], const double* x, orc_kernel_param* kp,
-                         const orc_integral_param* ip, cplx* res) {
+/* The integrand of a rule: ev(env, y, kp, ff). */
+typedef void (*eval_fn)(const void* env, const double* y, const orc_kernel_param* kp, cplx* ff);
+
+/* Cell rules: IntegralUniversalPnt (F(x, y) forms), quad integral_universal.h:38-120,
+ * triangle :136-220, and IntegralUniversal (F(y) forms), quad :252-333, triangle
+ * :347-431 - the same rules with the integrand's x fixed or absent. */
+static void integral_rule(eval_fn ev, const void* env, int nv, const double (*c)[3], orc_kernel_param* kp,
+                          const orc_integral_param* ip, cplx* res) {
     int n = ip->n_start, idim = ip->idim, p_n;
     cplx ff[3] = {{0, 0}, {0, 0}, {0, 0}}, prev[3] = {{0, 0}, {0, 0}, {0, 0}};
     for (int g = 0; g < idim; g++) res[g] = C(0., 0.);
@@ -351,7 +356,7 @@ static void integral_pnt(int f, int nv, const double (*c)[3], const double* x, o
                     }
                     vprod(m1, m2, rn);
                     double s = vlen(rn);
-                    kernel(f, x, rc, kp, ff);
+                    ev(env, rc, kp, ff);
                     for (int g = 0; g < idim; g++) res[g] = cadd(res[g], cmuld(ff[g], s));
                 }
             }
@@ -371,7 +376,7 @@ static void integral_pnt(int f, int nv, const double (*c)[3], const double* x, o
                         CM[k] = AM[k] + qv[k];
                         rc[k] = (B[k] + AM[k] + CM[k]) / 3.0;
                     }
-                    kernel(f, x, rc, kp, ff);
+                    ev(env, rc, kp, ff);
                     for (int g = 0; g < idim; g++) res[g] = cadd(res[g], ff[g]);
                     if (j < n - 1 - i) {
                         for (int k = 0; k < 3; k++) {
@@ -379,7 +384,7 @@ static void integral_pnt(int f, int nv, const double (*c)[3], const double* x, o
                             CP[k] = B[k] + qv[k];
                             rc[k] = (B[k] + AP[k] + CP[k]) / 3.0;
                         }
-                        kernel(f, x, rc, kp, ff);
+                        ev(env, rc, kp, ff);
                         for (int g = 0; g < idim; g++) res[g] = cadd(res[g], ff[g]);
                     }
                 }
@@ -393,9 +398,27 @@ static void integral_pnt(int f, int nv, const double (*c)[3], const double* x, o
         for (int g = 0; g < idim; g++) res[g] = prev[g];
 }
 
-/* IntegralUniversalSegPnt integral_universal_seg.h:38-98 */
-static void integral_seg(int f, const double* a, const double* b, const double* x, orc_kernel_param* kp,
+/* F(x, y) integrand with x fixed */
+typedef struct {
+    int f;
+    const double* x;
+} fxy_env;
+static void eval_fxy(const void* env, const double* y, const orc_kernel_param* kp, cplx* ff) {
+    const fxy_env* e = (const fxy_env*)env;
+    kernel(e->f, e->x, y, kp, ff);
+}
+
+/* IntegralUniversalPnt, quad integral_universal.h:38-120, triangle :136-220 */
+static void integral_pnt(int f, int nv, const double (*c)[3], const double* x, orc_kernel_param* kp,
                          const orc_integral_param* ip, cplx* res) {
+    fxy_env e = {f, x};
+    integral_rule(eval_fxy, &e, nv, c, kp, ip, res);
+}
+
+/* Segment rule: IntegralUniversalSegPnt, F(x, y) form integral_universal_seg.h:38-98,
+ * F(y) form :120-179 */
+static void integral_seg_rule(eval_fn ev, const void* env, const double* a, const double* b,
+                              orc_kernel_param* kp, const orc_integral_param* ip, cplx* res) {
     int n = ip->n_start, idim = ip->idim, p_n;
     cplx ff[3] = {{0, 0}, {0, 0}, {0, 0}}, prev[3] = {{0, 0}, {0, 0}, {0, 0}};
     for (int g = 0; g < idim; g++) res[g] = C(0., 0.);
@@ -405,7 +428,7 @@ static void integral_seg(int f, const double* a, const double* b, const double* 
         double dl = vlen(d);
         for (int i = 0; i < n; i++) {
             for (int j = 0; j < 3; j++) y[j] = a[j] + (i + 0.5) * d[j];
-            kernel(f, x, y, kp, ff);
+            ev(env, y, kp, ff);
             for (int g = 0; g < idim; g++) res[g] = cadd(res[g], cmuld(ff[g], dl));
         }
         if (pass_end(p_n, idim, ip->eps, res, prev, kp)) break;
@@ -413,6 +436,12 @@ static void integral_seg(int f, const double* a, const double* b, const double* 
     }
     if (p_n == ip->p_max)
         for (int g = 0; g < idim; g++) res[g] = prev[g];
+}
+
+static void integral_seg(int f, const double* a, const double* b, const double* x, orc_kernel_param* kp,
+                         const orc_integral_param* ip, cplx* res) {
+    fxy_env e = {f, x};
+    integral_seg_rule(eval_fxy, &e, a, b, kp, ip, res);
 }
 
 static void put3(const cplx* r, int n, double* out) {
@@ -811,6 +840,63 @@ int orc_plane_wave_e(int npts, const double* pts, const double* kvec, const doub
         plane_wave_e(pts + 3 * i, kvec, e0, e);
         put3(e, 3, out + 6 * i);
     }
+    return 0;
+}
+
+/* PlaneWaveFieldH electrodynamics/incident_field.h:40-59:
+ * (vec_dbl x e0) e^{i vec_dbl.x} / (2 pi freq) / mu, mu real (mu_im = 0) or complex */
+static void plane_wave_h(const double* x, const double* kvec, const double* e0, double freq, double mu_re,
+                         double mu_im, cplx* h) {
+    double vp[3];
+    vprod(kvec, e0, vp);
+    cplx deg = cexp_(cmuld(C(0., 1.), scal(kvec, x)));
+    deg = cdivd(deg, 2. * M_PI * freq);
+    if (mu_im != 0.) {
+        double den = mu_re * mu_re + mu_im * mu_im;
+        deg = C((deg.re * mu_re + deg.im * mu_im) / den, (deg.im * mu_re - deg.re * mu_im) / den);
+    } else {
+        deg = cdivd(deg, mu_re);
+    }
+    for (int i = 0; i < 3; i++) h[i] = cmuld(deg, vp[i]);
+}
+
+int orc_plane_wave_h(int npts, const double* pts, const double* kvec, const double* e0, double freq,
+                     double mu_re, double mu_im, double* out) {
+    for (int i = 0; i < npts; i++) {
+        cplx h[3];
+        plane_wave_h(pts + 3 * i, kvec, e0, freq, mu_re, mu_im, h);
+        put3(h, 3, out + 6 * i);
+    }
+    return 0;
+}
+
+/* F(y) integrands: 0 the constant 1 (examples/integrators/main0_integrals.cpp:77-81),
+ * 1 PlaneWaveFieldE with kp->vec_dbl as the wave vector */
+typedef struct {
+    int f;
+    const double* e0;
+} fy_env;
+static void eval_fy(const void* env, const double* y, const orc_kernel_param* kp, cplx* ff) {
+    const fy_env* e = (const fy_env*)env;
+    if (e->f == 0) {
+        ff[0] = C(1., 0.);
+        ff[1] = ff[2] = C(0., 0.);
+    } else {
+        plane_wave_e(y, kp->vec_dbl, e->e0, ff);
+    }
+}
+
+/* IntegralUniversal (F(y) forms) over a cell (nv = 4, 3) or the F(y) form of
+ * IntegralUniversalSegPnt over a segment (nv = 2, cell = a then b). */
+int orc_integral_y(int f, int nv, const double* cell, const double* e0, orc_kernel_param* kp,
+                   const orc_integral_param* ip, double* res6) {
+    if (f < 0 || f > 1 || nv < 2 || nv > 4 || ip->idim < 1 || ip->idim > 3) return -1;
+    static const double zero[3] = {0., 0., 0.};
+    fy_env e = {f, e0 ? e0 : zero};
+    cplx r[3] = {{0, 0}, {0, 0}, {0, 0}};
+    if (nv == 2) integral_seg_rule(eval_fy, &e, cell, cell + 3, kp, ip, r);
+    else integral_rule(eval_fy, &e, nv, (const double (*)[3])cell, kp, ip, r);
+    put3(r, 3, res6);
     return 0;
 }
 

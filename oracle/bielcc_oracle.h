@@ -75,6 +75,10 @@ int orc_rhs_planewave(int nv, int ncell, const double* verts, const double* kvec
                       const double* e0, double* rhs);
 int orc_currents_by_basis(int nv, int ncell, const double* verts, const double* b, double* j);
 int orc_plane_wave_e(int npts, const double* pts, const double* kvec, const double* e0, double* out);
+int orc_plane_wave_h(int npts, const double* pts, const double* kvec, const double* e0, double freq,
+                     double mu_re, double mu_im, double* out);
+int orc_integral_y(int f, int nv, const double* cell, const double* e0, orc_kernel_param* kp,
+                   const orc_integral_param* ip, double* res6);
 int orc_lu_solve(int n, double* A, long long lda, double* B, int nrhs, long long ldb);
 int orc_far_pattern(int nv, int ncell, const double* verts, const double* jE, const double* jM,
                     double multE_re, double multE_im, double multH_re, double multH_im,

@@ -226,6 +226,33 @@ class Checker:
         self._f("plane_wave_e")(C.c_int(pts.shape[0]), pp, pk, pe, out.ctypes.data_as(_DP))
         return _to_c(out).reshape(-1, 3)
 
+    def plane_wave_h(self, pts, kvec, e0, freq, mu=None, mu_r=1.0):
+        """PlaneWaveFieldH at pts [n][3] -> [n][3] complex.  The reference's form
+        takes a MaterialParam of relative permeability mu_r (mu = mu0 mu_r after
+        Calc(freq)) and returns (H, mu); the oracle takes mu itself (complex allowed)."""
+        pts, pp = _d(pts); kv, pk = _d(kvec); e, pe = _d(e0)
+        out = np.zeros(pts.shape[0] * 6)
+        if self.which == "ref":
+            mu_out = C.c_double(0.)
+            self._f("plane_wave_h")(C.c_int(pts.shape[0]), pp, pk, pe, C.c_double(freq), C.c_double(mu_r),
+                                    out.ctypes.data_as(_DP), C.byref(mu_out))
+            return _to_c(out).reshape(-1, 3), mu_out.value
+        mu = complex(mu)
+        self._f("plane_wave_h")(C.c_int(pts.shape[0]), pp, pk, pe, C.c_double(freq), C.c_double(mu.real),
+                                C.c_double(mu.imag), out.ctypes.data_as(_DP))
+        return _to_c(out).reshape(-1, 3)
+
+    def integral_y(self, f, cell, e0, kp, ip):
+        """IntegralUniversal F(y) forms over a cell [nv][3] (nv = 4, 3) or the F(y)
+        segment form over [2][3]; f 0 = the constant 1, 1 = PlaneWaveFieldE
+        (kp.vec_dbl = wave vector).  kp is mutated like the reference's."""
+        cell, pc = _d(cell); e, pe = _d(e0)
+        out = np.zeros(6)
+        args = [C.c_int(f)] + self._kt + [C.c_int(cell.shape[0]), pc, pe, C.byref(kp), C.byref(ip),
+                                          out.ctypes.data_as(_DP)]
+        assert self._f("integral_y")(*args) == 0
+        return _to_c(out)
+
     # -- oracle-only drivers of this library's own loops
     def field_eh(self, verts, jE, k, np_, pts, jM=None, multE=1.0, multH=1.0, incident=None,
                  near_mode=0):

@@ -34,6 +34,7 @@
 #include "get_field.h"
 #include "currents.h"
 #include "incident_field.h"
+#include "material.h"
 #include "num_param.h"
 #include "rwg_surf_mesh.h"
 
@@ -331,6 +332,36 @@ int mesh_tables(const std::string& path, int nv, int* ncell_out, double* verts, 
 
 }  // namespace
 
+namespace {
+// F(y) integrand of examples/integrators/main0_integrals.cpp:77-81 (the area test):
+// restated, since the example defines it in its own translation unit
+template <typename KType>
+void unit_fy(const double*, const KernelParam<KType>&, double* ff) { ff[0] = 1.; }
+
+template <typename KType>
+int integral_y(int f, int nv, const double* cellv, const double* e0, shim_kernel_param* s,
+               const shim_integral_param* ip, double* res) {
+    KernelParam<KType> kp; to_kp(s, kp);
+    for (int d = 0; d < 3; d++) kp.e0[d] = e0[d];
+    IntegralParam ipr(ip->idim, ip->n_start, ip->p_max, ip->eps);
+    if (f == 0) {
+        double r[3] = {0, 0, 0};
+        if (nv == 4) { double c[4][3]; load_cell(cellv, c); IntegralUniversal(c, unit_fy<KType>, kp, ipr, r); }
+        else if (nv == 3) { double c[3][3]; load_cell(cellv, c); IntegralUniversal(c, unit_fy<KType>, kp, ipr, r); }
+        else IntegralUniversalSegPnt(cellv, cellv + 3, unit_fy<KType>, kp, ipr, r);
+        put(r, 3, res);
+    } else {
+        cd r[3] = {};
+        if (nv == 4) { double c[4][3]; load_cell(cellv, c); IntegralUniversal(c, PlaneWaveFieldE<KType>, kp, ipr, r); }
+        else if (nv == 3) { double c[3][3]; load_cell(cellv, c); IntegralUniversal(c, PlaneWaveFieldE<KType>, kp, ipr, r); }
+        else IntegralUniversalSegPnt(cellv, cellv + 3, PlaneWaveFieldE<KType>, kp, ipr, r);
+        put(r, 3, res);
+    }
+    s->smoothR = kp.smoothR;
+    return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int ref_cellfill(int nv, const double* v, double* out) {
@@ -455,6 +486,31 @@ int ref_plane_wave_e(int npts, const double* pts, const double* kvec, const doub
         put(e, 3, out + 6 * i);
     }
     return 0;
+}
+
+// PlaneWaveFieldH through a MaterialParam of relative permeability mu_r at freq
+// (MaterialParam::Calc: mu = mu0 mu_r, electrodynamics/material.cpp:59-115)
+int ref_plane_wave_h(int npts, const double* pts, const double* kvec, const double* e0, double freq, double mu_r,
+                     double* out, double* mu_out) {
+    KernelParam<double> p;
+    p.k = 0.; p.smoothR = 0.; p.smoothRMin = 0.; p.freq = freq;
+    for (int d = 0; d < 3; d++) { p.vec_dbl[d] = kvec[d]; p.e0[d] = e0[d]; }
+    MaterialParam mat("shim", ChangeMaterialTypeDir::Isotropic, ChangeMaterialTypeFreq::NonDispersive, 1., mu_r, 0.,
+                      0., 0.);
+    mat.Calc(freq);
+    *mu_out = std::get<double>(mat.GetMu());
+    for (int i = 0; i < npts; i++) {
+        cd h[3];
+        PlaneWaveFieldH(pts + 3 * i, p, mat, h);
+        put(h, 3, out + 6 * i);
+    }
+    return 0;
+}
+
+int ref_integral_y(int f, int kt, int nv, const double* cell, const double* e0, shim_kernel_param* s,
+                   const shim_integral_param* ip, double* res) {
+    if (f < 0 || f > 1 || nv < 2 || nv > 4) return -1;
+    return kt ? integral_y<cd>(f, nv, cell, e0, s, ip, res) : integral_y<double>(f, nv, cell, e0, s, ip, res);
 }
 
 int ref_currents_by_basis(int ncell, const double* tau, const double* b, double* j) {

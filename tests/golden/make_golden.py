@@ -15,6 +15,7 @@ import sys
 
 import numpy as np
 import scipy.linalg
+from ctypes import c_int as C_int
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -375,10 +376,55 @@ def gen_aver(ref):
     print("aver: inside eps", int((res > 0).sum()), "of", len(res))
 
 
+def gen_incident_y(ref):
+    """PlaneWaveFieldH (incident_field.h:40-59) through a MaterialParam, and the F(y)
+    integrators: IntegralUniversal over quads and triangles (integral_universal.h:252-431)
+    and the F(y) segment form (integral_universal_seg.h:120-179), with the constant
+    integrand of examples/integrators/main0_integrals.cpp:77-81 and PlaneWaveFieldE."""
+    rng = np.random.default_rng(81)
+    out = {}
+    pts = rng.normal(size=(300, 3)) * 3.0
+    kv = rng.normal(size=(300, 3)) * 4.0
+    e0 = np.cross(kv, rng.normal(size=(300, 3)))
+    e0 /= np.linalg.norm(e0, axis=1, keepdims=True)
+    freq = rng.uniform(1e6, 1e10, 300)
+    mu_r = rng.uniform(0.5, 4.0, 300)
+    H, mu = [], []
+    for i in range(300):
+        h, m = ref.plane_wave_h(pts[i:i + 1], kv[i], e0[i], float(freq[i]), mu_r=float(mu_r[i]))
+        H.append(h[0]); mu.append(m)
+    out.update(h_pts=pts, h_kvec=kv, h_e0=e0, h_freq=freq, h_mu_r=mu_r, h_mu=np.array(mu), h_H=np.array(H))
+    quads, tris = case_cells()
+    KP = [(1e-16, 1e-16, 1, 8), (0.5, 1e-6, 1, 8), (0.5, 1e-3, 3, 6), (2.0, 0.1, 4, 5)]  # R, Rmin, p_max, n
+    rec = {k: [] for k in ["f", "nv", "kt", "cell", "kvec", "e0", "kp", "ip", "res", "R_out"]}
+    segs = [np.array([c[0], c[2]]) for c in quads[:2] + tris[:2]] + [np.array([[0., 0., 0.], [3., 4., 0.]])]
+    for nv, cells in ((4, quads), (3, tris), (2, segs)):
+        for cell in cells:
+            for kt in (0, 1):
+                for (R, Rmin, pm, n0) in KP:
+                    for f in (0, 1):
+                        for idim in ((1, 3) if f == 1 else (1,)):
+                            kvec = rng.normal(size=3) * 2.0
+                            e = rng.normal(size=3)
+                            kp = O.kernel_param(R, Rmin, 1.5 if kt == 0 else 1.5 - 0.3j, vec_dbl=kvec)
+                            ip = O.IntegralParam(idim, n0, pm, 1e-6)
+                            ref._kt = [C_int(kt)]
+                            r = ref.integral_y(f, cell, e, kp, ip)
+                            rec["f"].append(f); rec["nv"].append(nv); rec["kt"].append(kt)
+                            rec["cell"].append(np.pad(cell, ((0, 4 - len(cell)), (0, 0))))
+                            rec["kvec"].append(kvec); rec["e0"].append(e); rec["kp"].append([R, Rmin])
+                            rec["ip"].append([idim, n0, pm, 1e-6]); rec["res"].append(r)
+                            rec["R_out"].append(kp.smoothR)
+    ref._kt = [C_int(1)]
+    out.update({"y_" + k: np.asarray(v) for k, v in rec.items()})
+    np.savez_compressed(os.path.join(HERE, "incident_y.npz"), **out)
+    print("incident_y:", len(H), "PlaneWaveFieldH points,", len(rec["f"]), "F(y) integrals")
+
+
 if __name__ == "__main__":
     ref = O.Checker("ref")
     which = sys.argv[1:] or ["ops", "integrators", "kernels", "config1", "config1_tri", "mesh_files", "writers",
-                             "rwg", "aver"]
+                             "rwg", "aver", "incident_y"]
     if "ops" in which:
         gen_ops(ref)
     if "integrators" in which:
@@ -397,3 +443,5 @@ if __name__ == "__main__":
         gen_rwg(ref)
     if "aver" in which:
         gen_aver(ref)
+    if "incident_y" in which:
+        gen_incident_y(ref)

@@ -54,6 +54,16 @@ def test_cpp_dropin_program(orc):
     assert abs(_c(rec["IntegralUniversalPnt"])[0] - s) <= 1e-12 * abs(s)
     assert rec["smoothR_after"][0] == kp.smoothR
     assert abs(rec["integral1Divr"][0] - orc.integral1divr(RUT1, x)) <= 1e-13 * abs(orc.integral1divr(RUT1, x))
+    # F(y) forms and PlaneWaveFieldH (vacuum: mu = mu0)
+    assert abs(rec["area_rut1"][0] - 5.0) <= 1e-14
+    kv, e0 = np.array([0.3, -1.2, 0.5]), np.array([1.0, 0.5, 0.0])
+    fy = orc.integral_y(1, RUT1, e0, O.kernel_param(1e-16, 1e-16, 0.0, vec_dbl=kv), O.IntegralParam(3, 8, 1, 1e-6))
+    assert np.linalg.norm(_c(rec["IntegralUniversal_PlaneWaveFieldE"]) - fy) <= 1e-12 * np.linalg.norm(fy)
+    h = orc.plane_wave_h(x[None], kv, e0, 0.2, mu=4e-7 * np.pi)[0]
+    assert np.linalg.norm(_c(rec["PlaneWaveFieldH"]) - h) <= 1e-12 * np.linalg.norm(h)
+    xs = np.stack([0.001 * np.arange(2000), np.full(2000, 0.5), np.full(2000, -0.25)], axis=1)
+    es = orc.plane_wave_e(xs, kv, e0).sum(axis=0)
+    assert np.linalg.norm(_c(rec["PlaneWaveFieldE_sum2000"]) - es) <= 1e-12 * np.linalg.norm(es)
     # whole problem on the reference's own example mesh
     verts = mesh.read_dat(os.path.join(GOLDEN, "sphere_quad.dat"), 4)
     assert int(rec["cells"][0]) == verts.shape[0]

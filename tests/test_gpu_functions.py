@@ -140,3 +140,44 @@ def test_mesh_tables_vs_oracle(ctx, orc):
         assert t["diam"][c] == d["diam"]
     g = m.grid_step()
     assert g == max(orc.cellfill(v)["diam"] for v in verts)
+
+
+def test_integral_y_vs_reference(ctx):
+    """IntegralUniversal F(y) forms (quads, triangles) and the F(y) segment form,
+    with the constant integrand and PlaneWaveFieldE, real and complex KType: the
+    recorded results and the mutated smoothR."""
+    g = golden("incident_y.npz")
+    for i in range(len(g["y_f"])):
+        nv = int(g["y_nv"][i])
+        cell = g["y_cell"][i][:max(nv, 2)]
+        R, Rmin = g["y_kp"][i]
+        idim, n0, pm, eps = g["y_ip"][i]
+        k = 1.5 if int(g["y_kt"][i]) == 0 else 1.5 - 0.3j
+        kp = B.kernel_param(R, Rmin, k, vec_dbl=g["y_kvec"][i])
+        ip = B.IntegralParam(int(idim), int(n0), int(pm), float(eps))
+        r, Rout = ctx.integral_y_batch(int(g["y_f"][i]), cell[None], kp, ip, e0=g["y_e0"][i])
+        assert rowwise_rel(r, g["y_res"][i][None]).max() <= TOL, i
+        assert Rout[0] == g["y_R_out"][i], i
+
+
+def test_plane_wave_h_vs_reference(ctx):
+    g = golden("incident_y.npz")
+    for i in range(len(g["h_H"])):
+        H = ctx.plane_wave_h(g["h_pts"][i:i + 1], g["h_kvec"][i], g["h_e0"][i], g["h_freq"][i], mu=g["h_mu"][i])
+        assert rowwise_rel(H, g["h_H"][i][None]).max() <= TOL, i
+
+
+def test_add_incident_h(ctx, orc):
+    """H = scale H + PlaneWaveFieldH per incidence, complex permeability too."""
+    rng = np.random.default_rng(41)
+    M, nrhs = 300, 3
+    pts = rng.normal(size=(M, 3)) * 2
+    inc = np.concatenate([rng.normal(size=(nrhs, 3)) * 3, rng.normal(size=(nrhs, 3))], axis=1)
+    H0 = rng.normal(size=(nrhs, M, 3)) + 1j * rng.normal(size=(nrhs, M, 3))
+    for mu, scale in ((1.0, -2.0j), (1.3 + 0.2j, 0.5 - 1j)):
+        dP, dH = ctx.to_device(pts), ctx.to_device(H0)
+        ctx.add_incident_h(M, nrhs, dP, inc, 1.7, dH, mu=mu, scale=scale)
+        H = dH.to_numpy((nrhs, M, 3), np.complex128)
+        for r in range(nrhs):
+            expect = scale * H0[r] + orc.plane_wave_h(pts, inc[r, :3], inc[r, 3:], 1.7, mu=mu)
+            assert rowwise_rel(H[r], expect).max() <= TOL

@@ -206,3 +206,21 @@ def test_failed_build_returns_the_entrys_bytes():
         s._build_entry(0, M(), 1.0, e, fbytes)
     assert s.budget.used == 0 and key not in s.cache.entries
     assert len(s.upgrades) == 1                 # the FP64 factors were tried
+
+
+def test_charge_of_held_bytes_never_waits():
+    """Bytes a worker already holds (grow-only library work buffers) are charged
+    without waiting - the worker may hold pins others wait for - and idle entries
+    are evicted to get back under the budget."""
+    log = []
+    b = Budget(100)
+    c = FactorCache(b)
+    e = _ready_entry(c, "a", 60, log)
+    b.charge(30, c.evict)                 # fits: nothing evicted
+    assert b.used == 90 and log == []
+    t0 = time.monotonic()
+    b.charge(30, c.evict)                 # over, but "a" is pinned: counted, no wait
+    assert time.monotonic() - t0 < 0.5 and b.used == 120 and log == []
+    c.release(e)
+    b.charge(0, c.evict)                  # "a" idle now: evicted back under the budget
+    assert log == ["a"] and b.used == 60
