@@ -92,6 +92,9 @@ _SIGS = {
     "bielcc_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double, C.c_void_p,
                                C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _DP, _DP, C.c_int,
                                C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bielcc_field_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double,
+                                       C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _DP,
+                                       _DP, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bielcc_get_field_e": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, _DP, _DP,
                                      C.POINTER(NumParam), C.c_double, C.c_double, C.c_int, C.c_void_p,
                                      C.c_void_p]),

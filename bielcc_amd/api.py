@@ -469,6 +469,20 @@ class Mesh:
                                                   _p(jM), multE.real, multE.imag, multH.real, multH.imag, kv, ev,
                                                   int(near_mode), int(M), _p(pts), _p(E), _p(H)))
 
+    def field_noalloc_into(self, np_: NumParam, k, jE, pts, M, E, H=None, jM=None, multE=1.0, multH=1.0,
+                           incident=None, near_mode=0):
+        """field_into through bielcc_field_noalloc: no device allocation (work memory in the
+        library's fixed per-GPU scratch; the mesh's node tables must exist)."""
+        k, multE, multH = complex(k), complex(multE), complex(multH)
+        if incident is not None:
+            kv, _a = _vec3(incident[0])
+            ev, _b = _vec3(incident[1])
+        else:
+            kv = ev = None
+        self.ctx._check(self.ctx.lib.bielcc_field_noalloc(self.ctx.h, self.id, C.byref(np_), k.real, k.imag, _p(jE),
+                                                          _p(jM), multE.real, multE.imag, multH.real, multH.imag,
+                                                          kv, ev, int(near_mode), int(M), _p(pts), _p(E), _p(H)))
+
     def field_batch_into(self, np_: NumParam, k, nrhs: int, jE, pts, M, E, H=None, jM=None, multE=1.0,
                          multH=1.0, inc=None, near_mode=0, precision=PREC_FP64):
         """nrhs current sets (device [nrhs][N][3]) -> E, H (device [nrhs][M][3]); inc: host [nrhs][6] or None."""

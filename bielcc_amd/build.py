@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libbielcc_hip.so")
-SOURCES = ["capi.hip", "plan.hip", "tables.hip", "field.hip", "farfield.hip", "assemble.hip", "generic.hip", "lu.hip"]
+SOURCES = ["capi.hip", "noalloc.hip", "plan.hip", "tables.hip", "field.hip", "farfield.hip", "assemble.hip",
+           "generic.hip", "lu.hip"]
 HOST_SOURCES = ["rwg.cpp"]   # host-only C++ (no device code): g++
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BIELCC_ARCH", "gfx950")

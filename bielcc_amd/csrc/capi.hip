@@ -29,6 +29,14 @@ int bielcc_ctx_create(int device, bielcc_ctx* out) {
         ctx->n_cu = prop.multiProcessorCount;
     const char* gs = std::getenv("BIELCC_GENERAL_SINCOS");
     ctx->no_fast_phase = gs && gs[0] == '1';
+    if (noalloc_setup(ctx) != BIELCC_OK) {   // per-device address of bielcc_field_noalloc's scratch
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+        lu_work_destroy(ctx->lu);
+        delete ctx;
+        return BIELCC_ERR_HIP;
+    }
     *out = ctx;
     return BIELCC_OK;
 }

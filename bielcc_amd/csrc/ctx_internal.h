@@ -230,6 +230,9 @@ int opmat_gemms(bielcc_ctx ctx, const bielcc_impl::OpSeg& sg, int nrhs, const do
 // bielcc_field_batch for many current sets (electric only) through the
 // operator matrices in context-owned grow-only buffers (plan.hip).  inc_d:
 // device [nrhs][6] or null.  counts/offs/tot_d: near-list scratch for M points.
+// bielcc_field_noalloc's per-device scratch address (noalloc.hip), taken at context creation.
+int noalloc_setup(bielcc_ctx ctx);
+
 int field_opmat(bielcc_ctx ctx, DevMesh& m, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
                 const double* jE_d, double mEr, double mEi, const double* inc_d, int near_mode, int precision, int M,
                 const double* pts_d, double* E_d, double* H_d, int* counts, long long* offs, long long* tot_d);

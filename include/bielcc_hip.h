@@ -265,6 +265,26 @@ int bielcc_field(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double
                  double multH_im, const double* kvec, const double* e0, int near_mode, int M, const double* pts_d,
                  double* E_d, double* H_d);
 
+/* bielcc_field for one current set in FP64 that allocates nothing on the
+ * device.  No counterpart in the reference: it is for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as bielcc_field; its work memory (partial
+ * sums, near-pair lists and results) is one file-scope device array of
+ * 64 MiB per GPU, held until the process ends.  Limits: the mesh's quadrature
+ * nodes for np->n_start must already exist (one bielcc_field or
+ * bielcc_assemble call on the mesh builds them); inputs whose work memory
+ * exceeds 64 MiB are rejected with BIELCC_ERR_UNSUPPORTED: 108 bytes per
+ * point and split of the cell range, plus 104 bytes per near pair (200 with
+ * jM) - about 620,000 points without near pairs (beyond 32 MiB of partial
+ * sums fewer splits are used than bielcc_field uses, which changes only the
+ * summation order).  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU race.  The caller passes
+ * and owns the inputs and outputs. */
+int bielcc_field_noalloc(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                         const double* jE_d, const double* jM_d, double multE_re, double multE_im, double multH_re,
+                         double multH_im, const double* kvec, const double* e0, int near_mode, int M,
+                         const double* pts_d, double* E_d, double* H_d);
+
 /* GetFieldE_Ideal (get_field.h:35-78) with EInc_Func = PlaneWaveFieldE:
  * E = mult sum_j (near ? K_RotRot_Near_Smooth : K_RotRot_Far)[jE_j](x) + E_inc(x). */
 int bielcc_get_field_e(bielcc_ctx ctx, int mesh_id, const double* jE_d, double mult_re, double mult_im,
