@@ -224,6 +224,93 @@ def cube_precision(k=10.0, n_face=60):
     return out
 
 
+FLOPS_ASM_PAIR_TRI = 9166    # K_RotRot_HS_Colloc block, far triangle pair (30 contour nodes)
+
+
+def triangle_variant(steps=2, lu=False):
+    """The 20,000-quad sphere with every quad split into two triangles ((0,1,2) and
+    (2,3,0)); the zero-area triangle of each of the 400 pole quads is dropped
+    (CellTriangle::CellFill divides by the normal's length), leaving 39,600 cells
+    and 79,200 unknowns: the full K matrix (100.4 GB) and E and H at the
+    100,000 points of radius 5, per step.  The field uses seeded random
+    currents (its cost does not depend on them); with lu=True the FP64 and FP32
+    LU of the matrix are timed once as well."""
+    import bielcc_amd as B
+    from bielcc_amd import mesh
+    quads = mesh.sphere_quads(100, 200, radius=1.0, jitter=0.005, seed=1)
+    verts = mesh.triangulate(quads)
+    dropped = 2 * quads.shape[0] - verts.shape[0]
+    pts = mesh.grid_points(200, 500, 5.0)
+    k = 10.0
+    n, M = verts.shape[0], pts.shape[0]
+    n2 = 2 * n
+    ctx = B.Context(0)
+    m = ctx.mesh(verts)
+    npp = B.num_param()
+    rng = np.random.default_rng(7)
+    dJ = ctx.to_device(rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3)))
+    dP = ctx.to_device(pts)
+    dE, dH = ctx.empty(M * 48), ctx.empty(M * 48)
+    dZ = ctx.empty(n2 * n2 * 16)
+    log(f"[bench] triangle variant: {n} triangles ({dropped} zero-area pole triangles dropped), {n2} unknowns, "
+        f"matrix {n2 * n2 * 16 / 1e9:.1f} GB")
+    m.assemble_into(dZ, n2, npp, k)
+    m.field_into(npp, k, dJ, dP, M, dE, dH)
+    ctx.sync()
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        m.assemble_into(dZ, n2, npp, k)
+        m.field_into(npp, k, dJ, dP, M, dE, dH)
+        ctx.sync()
+        log(f"[bench] triangle step {s} done ({time.perf_counter() - t0:.2f} s)")
+    elapsed = time.perf_counter() - t0
+    fld_ms, fld_cnt = ctx.timing("field_far")
+    asm_ms, asm_cnt = ctx.timing("assemble_far")
+    ctx.timing_enable(False)
+    E = dE.to_numpy((M, 3), np.complex128)
+    assert np.all(np.isfinite(E)), "non-finite field"
+    fld_s, asm_s = fld_ms / max(fld_cnt, 1) / 1e3, asm_ms / max(asm_cnt, 1) / 1e3
+    fld_tf = M * n * FLOPS_FIELD_PAIR / fld_s / 1e12
+    asm_tf = n * n * FLOPS_ASM_PAIR_TRI / asm_s / 1e12
+    out = {"workload": f"sphere r=1, 100x200 quads split into triangles: {n} cells ({dropped} zero-area pole "
+                       f"triangles dropped), {n2} unknowns, ka=10, field at 200x500 points r=5",
+           "cells": n, "dropped_triangles": dropped, "unknowns": n2, "steps": steps,
+           "pairs_per_s": (n * n + M * n) * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+           "field_kernel": {"kernel": "k_field_far (triangles)", "launch_ms": round(fld_s * 1e3, 3),
+                            "achieved_tflops": round(fld_tf, 3), "frac": round(fld_tf / PEAK_FP64_TFLOPS, 4),
+                            "pairs_per_s": M * n / fld_s},
+           "assembly_kernel": {"kernel": "k_assemble_far (triangles)", "launch_ms": round(asm_s * 1e3, 3),
+                               "achieved_tflops": round(asm_tf, 3), "frac": round(asm_tf / PEAK_FP64_TFLOPS, 4),
+                               "pairs_per_s": n * n / asm_s, "flop_per_pair": FLOPS_ASM_PAIR_TRI}}
+    log(f"[bench] triangle variant: {out['pairs_per_s']:.3e} pairs/s; field kernel {fld_s * 1e3:.1f} ms "
+        f"({fld_tf:.1f} TFLOP/s), assembly kernel {asm_s * 1e3:.1f} ms ({asm_tf:.1f} TFLOP/s)")
+    if lu:
+        dpiv = ctx.empty(n2 * 4)
+        dF = ctx.empty(n2 * n2 * 8)
+        ctx.sync()
+        t1 = time.perf_counter()
+        ctx.lu32_factor(dZ, n2, n2, dF, n2, dpiv)
+        ctx.sync()
+        t32 = time.perf_counter() - t1
+        dF.free()
+        t1 = time.perf_counter()
+        ctx.lu_factor(dZ, n2, n2, dpiv)
+        ctx.sync()
+        t64 = time.perf_counter() - t1
+        flop = 8.0 / 3.0 * n2 ** 3
+        out["lu"] = {"lu64_s": round(t64, 3), "lu64_tflops": round(flop / t64 / 1e12, 2), "lu32_s": round(t32, 3),
+                     "lu32_tflops": round(flop / t32 / 1e12, 2)}
+        log(f"[bench] triangle variant LU: FP64 {t64:.2f} s, FP32 {t32:.2f} s")
+        dpiv.free()
+    for d in (dZ, dJ, dP, dE, dH):
+        d.free()
+    m.close()
+    ctx.close()
+    return out
+
+
 def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve64_s, step64_s):
     """One cold solve in the mixed-precision form on the workload: FP64 matrix,
     FP32 LU on the matrix cores, FP64 refinement to a 1e-10 residual, currents,
@@ -976,6 +1063,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-queries", type=int, default=96)
     ap.add_argument("--no-batched", action="store_true", help="skip the 12-mesh batched form")
+    ap.add_argument("--no-tri", action="store_true", help="skip the 39,600-triangle variant")
+    ap.add_argument("--tri-only", type=int, default=0, metavar="STEPS",
+                    help="run only the 39,600-triangle variant for STEPS steps")
+    ap.add_argument("--tri-lu", action="store_true", help="with --tri-only: also time its FP64 and FP32 LU")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
     ap.add_argument("--cube-only", action="store_true", help="run only the precision form on the cube")
     ap.add_argument("--batched-workers", type=int, default=4, help="worker streams of the batched form's solves")
@@ -1015,6 +1106,9 @@ def main():
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args.tune_cache, args.batched_workers,
                                                  solve=not args.batched_no_solve)}), flush=True)
+        return
+    if args.tri_only:
+        print(json.dumps({"triangle_variant": triangle_variant(args.tri_only, lu=args.tri_lu)}), flush=True)
         return
     if args.cube_only:
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
@@ -1117,9 +1211,10 @@ def main():
     if lu_s is not None and not args.no_mixed:
         out["precision_form"] = mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ, dE, dH, lu_s, solve_s,
                                            elapsed / args.steps)
-    if not (args.no_serve and args.no_batched and args.no_mixed):
-        for d in (dZ, dB, dpiv):
-            d.free()
+    for d in (dZ, dB, dpiv):
+        d.free()
+    if not args.no_tri:
+        out["triangle_variant"] = triangle_variant()
     if lu_s is not None and not args.no_mixed:
         out["precision_form_cube"] = cube_precision()
     if not args.no_batched:
