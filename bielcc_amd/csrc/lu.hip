@@ -1295,14 +1295,23 @@ hipError_t lu_factor(hipStream_t s, LuWork* w, int n, double* Ad, long long lda,
     return lu_factor_t<double>(s, w, n, reinterpret_cast<double2*>(Ad), lda, piv, info, msg);
 }
 
-// dst (FP32, ldd) = src (FP64, lds), n rows x ncol columns
+// dst (FP32, ldd) = src (FP64, lds), n rows x ncol columns.  Grid-stride: a
+// dispatch counts its work-items in 32 bits, so one lane per element cannot
+// cover matrices of more than 2^32 entries (79,200^2 = 6.3e9).
 __global__ void k_to_f32(int n, int ncol, const double2* __restrict__ src, long long lds, float2* __restrict__ dst,
                          long long ldd) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)n * ncol) return;
-    const int i = (int)(idx % n), c = (int)(idx / n);
-    const double2 v = src[(size_t)c * lds + i];
-    dst[(size_t)c * ldd + i] = make_float2((float)v.x, (float)v.y);
+    const long long tot = (long long)n * ncol;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        const int i = (int)(idx % n), c = (int)(idx / n);
+        const double2 v = src[(size_t)c * lds + i];
+        dst[(size_t)c * ldd + i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// blocks of a grid-stride launch over tot elements: at most 2^20 of 256 lanes
+static inline unsigned stride_blocks(long long tot) {
+    return (unsigned)std::max<long long>(1, std::min<long long>((tot + 255) / 256, 1LL << 20));
 }
 
 hipError_t blas_zgemm(hipStream_t s, LuWork* w, long long m, long long n, long long k, double2 alpha,
@@ -1349,7 +1358,7 @@ hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* Ad, long l
                        int* piv, int* info, const char** msg) {
     const long long tot = (long long)n * n;
     float2* F = reinterpret_cast<float2*>(Fd);
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, n,
+    hipLaunchKernelGGL(k_to_f32, dim3(stride_blocks(tot)), dim3(256), 0, s, n, n,
                        reinterpret_cast<const double2*>(Ad), lda, F, ldf);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
