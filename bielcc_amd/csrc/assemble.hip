@@ -313,9 +313,13 @@ hipError_t launch_assemble_multi(hipStream_t s, const AssembleArgs* items_d, con
 }
 
 template <bool CK, int W, bool P32>
-static void assemble_far_p(hipStream_t s, const AssembleArgs& a, int nsplit) {
+static void assemble_far_p(hipStream_t s, const AssembleArgs& a, int s_min) {
+    const int bx = (a.ni + kAT - 1) / kAT;
+    const void* kern = a.fast_phase ? reinterpret_cast<const void*>(&k_assemble_far<CK, W, true, P32>)
+                                    : reinterpret_cast<const void*>(&k_assemble_far<CK, W, false, P32>);
+    const int nsplit = tail_aware_splits(bx, s_min, (a.N + 63) / 64, resident_slots(kern, kAT));
     int cps = (a.N + nsplit - 1) / nsplit;
-    dim3 g((a.ni + kAT - 1) / kAT, nsplit), b(kAT);
+    dim3 g(bx, nsplit), b(kAT);
     if (a.fast_phase) hipLaunchKernelGGL((k_assemble_far<CK, W, true, P32>), g, b, 0, s, a, cps);
     else hipLaunchKernelGGL((k_assemble_far<CK, W, false, P32>), g, b, 0, s, a, cps);
 }

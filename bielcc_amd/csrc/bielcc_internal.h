@@ -75,6 +75,33 @@ struct FieldArgs {
     int prec;               // 0: FP64 node sums; 1: FP32 cell-local node sums
 };
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a);
+// Cell splits of the field tile launch for a.M points and a.N cells: at least s_min,
+// chosen against the last-round tail (tail_aware_splits) for the kernel instance of the
+// launch's first pass.
+int field_far_splits(const FieldArgs& a, int s_min);
+
+// Workgroups of a 2-D tile launch (bx point blocks x s cell splits) run in rounds of
+// `slots` = resident workgroups on the device; a last round that is only partly
+// filled leaves compute units idle.  Among s_min .. min(max_s, 3 s_min) the split
+// count that minimises the time model ceil(bx s / slots) / s (fewer on ties).  On
+// sphere20k: field 6 splits (4.6 rounds) -> 13 (9.9 rounds): 425 -> 392 ms; assembly
+// 26 splits (2.7 rounds) -> 29 (3.0 rounds): 85 -> 79 ms.
+inline int tail_aware_splits(int bx, int s_min, int max_s, int slots) {
+    s_min = s_min < 1 ? 1 : s_min;
+    if (max_s < s_min) return max_s < 1 ? 1 : max_s;
+    if (slots <= 0 || bx <= 0) return s_min;
+    int best = s_min;
+    double best_t = 1e300;
+    const int hi = max_s < 3 * s_min ? max_s : 3 * s_min;
+    for (int s = s_min; s <= hi; s++) {
+        const long long rounds = ((long long)bx * s + slots - 1) / slots;
+        const double t = (double)rounds / s;
+        if (t < best_t * (1. - 1e-9)) { best_t = t; best = s; }
+    }
+    return best;
+}
+// Resident workgroups of `kernel` (block threads, no dynamic LDS) on the current device.
+int resident_slots(const void* kernel, int block);
 // as_matrix = 0: per pair, the operators applied to jE / jM (one current set);
 // as_matrix = 1: per pair, the 3x3 operator matrices (any number of sets).
 hipError_t launch_field_near(hipStream_t s, int nv, long long npairs, const int* lq, const int* lc,

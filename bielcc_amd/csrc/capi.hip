@@ -457,9 +457,18 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     const int has_jm = jM_d != nullptr;
     const int nacc = has_jm ? 24 : 12;
     const double thresh = np->analytic_calc * m->grid_step;
-    // enough workgroups to fill the chip: split the cells when the points alone do not
+    // enough workgroups to fill the chip: split the cells when the points alone do not,
+    // into a count whose last round of workgroups is nearly full
     const int bx = (M + 255) / 256;
-    int nsplit = std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64));
+    FieldArgs a;
+    a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
+    a.nodes32 = m->nodes32; a.nodes32_yy = m->nodes32_yy;
+    a.jE = jE_d; a.jM = jM_d; a.nrhs = nrhs; a.rchunk = ctx->field_rchunk;
+    a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
+    a.thresh = thresh;
+    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.prec = precision;
+    const int nsplit = field_far_splits(a, std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64)));
     size_t part_b = align256((size_t)nsplit * nrhs * nacc * M * sizeof(double));
     size_t cnt_b = align256((size_t)M * sizeof(int));
     size_t off_b = align256((size_t)(M + 1) * sizeof(long long));
@@ -476,14 +485,7 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     if (nrhs >= ctx->opmat_min_rhs && !jM_d)
         return field_opmat(ctx, *m, np, k_re, k_im, nrhs, jE_d, mEr, mEi, inc ? inc_d : nullptr, near_mode, precision,
                            M, pts_d, E_d, H_d, counts, offs, tot_d);
-    FieldArgs a;
-    a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
-    a.nodes32 = m->nodes32; a.nodes32_yy = m->nodes32_yy;
-    a.jE = jE_d; a.jM = jM_d; a.nrhs = nrhs; a.rchunk = ctx->field_rchunk;
-    a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
-    a.thresh = thresh; a.nsplit = nsplit; a.part = part;
-    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
-    a.prec = precision;
+    a.nsplit = nsplit; a.part = part;
     {
         Timer t(ctx, "field_far");
         HIPCHK(launch_field_far(ctx->stream, a));

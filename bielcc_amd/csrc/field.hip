@@ -372,6 +372,38 @@ static void field_far_fp(hipStream_t s, const FieldArgs& a) {
     else field_far_r<CK, JM, false>(s, a);
 }
 
+int resident_slots(const void* kernel, int block) {
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess) return 0;
+    return ncu * nb;
+}
+
+template <bool CK, bool JM, bool FP, int R>
+static const void* field_far_kernel(bool p32) {
+    return p32 ? reinterpret_cast<const void*>(&k_field_far32<CK, JM, FP, R>)
+               : reinterpret_cast<const void*>(&k_field_far<CK, JM, FP, R>);
+}
+
+template <bool CK, bool JM, bool FP>
+static const void* field_far_kernel_r(const FieldArgs& a) {
+    // the instance of the first pass (field_far_r)
+    if (!JM && a.rchunk >= 4 && a.nrhs >= 4) return field_far_kernel<CK, JM, FP, 4>(a.prec);
+    if (!JM && a.rchunk >= 2 && a.nrhs >= 2) return field_far_kernel<CK, JM, FP, 2>(a.prec);
+    return field_far_kernel<CK, JM, FP, 1>(a.prec);
+}
+
+int field_far_splits(const FieldArgs& a, int s_min) {
+    const bool ck = a.k_im != 0., jm = a.jM != nullptr, fp = a.fast_phase;
+    const void* k = ck ? (jm ? (fp ? field_far_kernel_r<true, true, true>(a) : field_far_kernel_r<true, true, false>(a))
+                             : (fp ? field_far_kernel_r<true, false, true>(a) : field_far_kernel_r<true, false, false>(a)))
+                       : (jm ? (fp ? field_far_kernel_r<false, true, true>(a) : field_far_kernel_r<false, true, false>(a))
+                             : (fp ? field_far_kernel_r<false, false, true>(a) : field_far_kernel_r<false, false, false>(a)));
+    const int bx = (a.M + kFT - 1) / kFT;
+    return tail_aware_splits(bx, s_min, (a.N + 63) / 64, resident_slots(k, kFT));
+}
+
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a) {
     if (a.prec && (!a.nodes32 || !a.nodes32_yy)) return hipErrorInvalidValue;
     const bool ck = a.k_im != 0.;

@@ -78,7 +78,15 @@ int bielcc_field_noalloc(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np
     // the cell split of bielcc_field (same sums, same order), fewer splits if the
     // partial sums would not leave half of the scratch for the near pairs
     const int bx = (M + 255) / 256;
-    int nsplit = std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64));
+    FieldArgs a;
+    a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
+    a.nodes32 = nullptr; a.nodes32_yy = nullptr;
+    a.jE = jE_d; a.jM = jM_d; a.nrhs = 1; a.rchunk = ctx->field_rchunk;
+    a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
+    a.thresh = thresh;
+    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.prec = BIELCC_PREC_FP64;
+    int nsplit = field_far_splits(a, std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64)));
     const size_t cnt_b = align256((size_t)M * sizeof(int));
     const size_t off_b = align256((size_t)(M + 1) * sizeof(long long));
     const size_t inc_b = 256, tot_b = 256;
@@ -100,14 +108,7 @@ int bielcc_field_noalloc(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np
         for (int d = 0; d < 3; d++) { ke[d] = kvec[d]; ke[3 + d] = e0[d]; }
         HIPCHK(hipMemcpyAsync(inc_d, ke, sizeof ke, hipMemcpyHostToDevice, ctx->stream));
     }
-    FieldArgs a;
-    a.nv = m->nv; a.M = M; a.N = m->n; a.nn = nn; a.pts = pts_d; a.ctab = m->ctab; a.nodes = m->nodes;
-    a.nodes32 = nullptr; a.nodes32_yy = nullptr;
-    a.jE = jE_d; a.jM = jM_d; a.nrhs = 1; a.rchunk = ctx->field_rchunk;
-    a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
-    a.thresh = thresh; a.nsplit = nsplit; a.part = part;
-    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
-    a.prec = BIELCC_PREC_FP64;
+    a.nsplit = nsplit; a.part = part;
     HIPCHK(launch_field_far(ctx->stream, a));
     // near pairs: count, scan, then the lists and results in the rest of the scratch
     HIPCHK(launch_near_count(ctx->stream, M, pts_d, 3, m->n, m->ctab, near_mode, np->analytic_calc, thresh, counts));
