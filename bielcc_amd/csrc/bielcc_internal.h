@@ -125,6 +125,27 @@ struct FinalizeArgs {
 };
 hipError_t launch_field_finalize(hipStream_t s, const FinalizeArgs& f);
 
+// ---- FP32 error control of the mixed field (field.hip)
+struct FixArgs {
+    int M, N, nrhs;
+    const double* pts;       // [M][3]
+    const double* ctab;
+    const double* jE;        // [nrhs][N][3] complex
+    const double* jM;        // or null
+    double k_re;
+    double mE2, mH2;         // |multE|^2, |multH|^2
+    const double* inc;       // device [nrhs][6] (kvec, e0) or null: E minus it is the scattered field
+    const double* E;         // [nrhs][M][3] complex (FP32 sums)
+    const double* H;         // or null
+    double* mx;              // [nrhs][2] max |E_scat|, |H| (zeroed by the launch)
+    int* flags;              // [nrhs][M]
+};
+// flags, then the list of points that need FP64 sums (count zeroed by the launch)
+hipError_t launch_fix_flags(hipStream_t s, const FixArgs& f, int* count, int* list);
+hipError_t launch_fix_gather(hipStream_t s, int m, const int* list, const double* pts, double* sub);
+hipError_t launch_fix_scatter(hipStream_t s, int m, int M, int nrhs, const int* list, const double* Es,
+                              const double* Hs, double* E, double* H);
+
 // ---- operator-matrix field (field.hip): one segment = one mesh and a range of M points
 struct OpSeg {
     int M, N, nn, nv;

@@ -54,6 +54,7 @@ int bielcc_ctx_destroy(bielcc_ctx ctx) {
         if (ctx->opmat_meta) (void)hipFree(ctx->opmat_meta);
         if (ctx->opmat_part) (void)hipFree(ctx->opmat_part);
         if (ctx->small) (void)hipFree(ctx->small);
+        if (ctx->fix) (void)hipFree(ctx->fix);
         lu_work_destroy(ctx->lu);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
@@ -434,6 +435,50 @@ int bielcc_currents_by_basis(bielcc_ctx ctx, int mesh_id, int nrhs, const double
 static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
                       int nrhs, const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr,
                       double mHi, const double* inc, int near_mode, int precision, int M, const double* pts_d,
+                      double* E_d, double* H_d);
+
+// Mixed field, after its FP32 sums: the points whose modelled FP32 error is not
+// small against their scattered field (k_fix_flags, field.hip) are recomputed
+// with FP64 sums and written over.  inc: host [nrhs][6] or null; inc_d its device copy.
+static int mixed_fixup(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
+                       const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr, double mHi,
+                       const double* inc, const double* inc_d, int near_mode, int M, const double* pts_d,
+                       double* E_d, double* H_d) {
+    if (!ctx->mixed_fixup || ctx->capturing || M <= 0) return BIELCC_OK;
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    const size_t mx_b = align256((size_t)2 * nrhs * sizeof(double)), cnt_b = 256;
+    const size_t flag_b = align256((size_t)nrhs * M * sizeof(int)), list_b = align256((size_t)M * sizeof(int));
+    const size_t pts_b = align256((size_t)3 * M * sizeof(double)), f_b = align256((size_t)nrhs * M * 6 * sizeof(double));
+    HIPCHK(ensure(ctx->fix, ctx->fix_cap, mx_b + cnt_b + flag_b + list_b + pts_b + f_b * (H_d ? 2 : 1)));
+    char* b = static_cast<char*>(ctx->fix);
+    double* mx = reinterpret_cast<double*>(b);
+    int* count = reinterpret_cast<int*>(b + mx_b);
+    int* flags = reinterpret_cast<int*>(b + mx_b + cnt_b);
+    int* list = reinterpret_cast<int*>(b + mx_b + cnt_b + flag_b);
+    double* sub = reinterpret_cast<double*>(b + mx_b + cnt_b + flag_b + list_b);
+    double* Es = reinterpret_cast<double*>(b + mx_b + cnt_b + flag_b + list_b + pts_b);
+    double* Hs = H_d ? Es + f_b / sizeof(double) : nullptr;
+    FixArgs f;
+    f.M = M; f.N = m->n; f.nrhs = nrhs; f.pts = pts_d; f.ctab = m->ctab; f.jE = jE_d; f.jM = jM_d; f.k_re = k_re;
+    f.mE2 = mEr * mEr + mEi * mEi; f.mH2 = mHr * mHr + mHi * mHi; f.inc = inc ? inc_d : nullptr;
+    f.E = E_d; f.H = H_d; f.mx = mx; f.flags = flags;
+    HIPCHK(launch_fix_flags(ctx->stream, f, count, list));
+    int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (cnt <= 0) return BIELCC_OK;
+    ctx->fixup_points += cnt;
+    HIPCHK(launch_fix_gather(ctx->stream, cnt, list, pts_d, sub));
+    if (int rc = field_impl(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, jM_d, mEr, mEi, mHr, mHi, inc, near_mode,
+                            BIELCC_PREC_FP64, cnt, sub, Es, Hs))
+        return rc;
+    HIPCHK(launch_fix_scatter(ctx->stream, cnt, M, nrhs, list, Es, Hs, E_d, H_d));
+    return BIELCC_OK;
+}
+
+static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
+                      int nrhs, const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr,
+                      double mHi, const double* inc, int near_mode, int precision, int M, const double* pts_d,
                       double* E_d, double* H_d) {
     if (!ctx) return BIELCC_ERR_ARG;
     if (int rc = check_np(ctx, np)) return rc;
@@ -482,9 +527,15 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     double* inc_d = reinterpret_cast<double*>(base + part_b + cnt_b + off_b);
     long long* tot_d = reinterpret_cast<long long*>(base + part_b + cnt_b + off_b + inc_b);
     if (inc) HIPCHK(hipMemcpyAsync(inc_d, inc, (size_t)nrhs * 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    if (nrhs >= ctx->opmat_min_rhs && !jM_d)
-        return field_opmat(ctx, *m, np, k_re, k_im, nrhs, jE_d, mEr, mEi, inc ? inc_d : nullptr, near_mode, precision,
-                           M, pts_d, E_d, H_d, counts, offs, tot_d);
+    const bool fixup = precision == BIELCC_PREC_MIXED;
+    if (nrhs >= ctx->opmat_min_rhs && !jM_d) {
+        if (int rc = field_opmat(ctx, *m, np, k_re, k_im, nrhs, jE_d, mEr, mEi, inc ? inc_d : nullptr, near_mode,
+                                 precision, M, pts_d, E_d, H_d, counts, offs, tot_d))
+            return rc;
+        return fixup ? mixed_fixup(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, jM_d, mEr, mEi, mHr, mHi, inc,
+                                   inc ? inc_d : nullptr, near_mode, M, pts_d, E_d, H_d)
+                     : BIELCC_OK;
+    }
     a.nsplit = nsplit; a.part = part;
     {
         Timer t(ctx, "field_far");
@@ -513,7 +564,9 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     f.mEr = mEr; f.mEi = mEi; f.mHr = mHr; f.mHi = mHi;
     f.inc = inc ? inc_d : nullptr; f.pts = pts_d; f.E = E_d; f.H = H_d;
     HIPCHK(launch_field_finalize(ctx->stream, f));
-    return BIELCC_OK;
+    return fixup ? mixed_fixup(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, jM_d, mEr, mEi, mHr, mHi, inc,
+                               inc ? inc_d : nullptr, near_mode, M, pts_d, E_d, H_d)
+                 : BIELCC_OK;
 }
 
 int bielcc_field(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
@@ -687,6 +740,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_blas_atomics(ctx->lu, value == 1);
         return BIELCC_OK;
     }
+    if (k == "mixed_fixup") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "mixed_fixup must be 0 or 1");
+        ctx->mixed_fixup = value == 1;
+        return BIELCC_OK;
+    }
     if (k == "lu_panel_rows32") {
         if (value != 0 && value != 64 && value != 128) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_rows32 must be 0, 64 or 128");
         lu_set_panel_rows32(ctx->lu, value);
@@ -748,6 +806,8 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu);
     else if (k == "lu_panel_rows32") *value = lu_get_panel_rows32(ctx->lu);
+    else if (k == "mixed_fixup") *value = ctx->mixed_fixup ? 1 : 0;
+    else if (k == "mixed_fixup_points") *value = (int)std::min<long long>(ctx->fixup_points, INT_MAX);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu);
@@ -771,7 +831,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "n_cu") *value = ctx->n_cu;
     else if (k == "scratch_kb") {   // the context's grow-only device buffers, KiB rounded up
         const size_t b = ctx->scratch_cap + ctx->lists_cap + ctx->near_cap + ctx->opmat_ws_cap +
-                         ctx->opmat_meta_cap + ctx->opmat_part_cap + ctx->small_cap + lu_work_bytes(ctx->lu);
+                         ctx->opmat_meta_cap + ctx->opmat_part_cap + ctx->small_cap + ctx->fix_cap + lu_work_bytes(ctx->lu);
         *value = (int)std::min<size_t>((b + 1023) >> 10, (size_t)INT_MAX);
     }
     else return fail(ctx, BIELCC_ERR_ARG, "unknown tuning key " + k);
@@ -800,3 +860,12 @@ int bielcc_timing(bielcc_ctx ctx, const char* name, double* total_ms, int* count
 }
 
 }  // extern "C"
+
+namespace bielcc_capi {
+int field_mixed_fixup(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
+                      const double* jE_d, double mEr, double mEi, const double* inc, const double* inc_d, int near_mode,
+                      int M, const double* pts_d, double* E_d, double* H_d) {
+    return mixed_fixup(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, nullptr, mEr, mEi, 1., 0., inc, inc_d, near_mode, M,
+                       pts_d, E_d, H_d);
+}
+}  // namespace bielcc_capi

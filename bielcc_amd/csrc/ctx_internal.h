@@ -41,6 +41,10 @@ struct bielcc_ctx_s {
     size_t opmat_part_cap = 0;
     void* small = nullptr;        // staging of small host arrays (incidences) (grow-only)
     size_t small_cap = 0;
+    void* fix = nullptr;          // FP32 error control of the mixed field: flags, lists, FP64 results (grow-only)
+    size_t fix_cap = 0;
+    bool mixed_fixup = true;      // mixed field: points the FP32 error model does not clear are recomputed in FP64
+    long long fixup_points = 0;   // points so recomputed, cumulative
     int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     // plan (multi-mesh) tunables: cells per field tile and per assembly tile,
     // persistent workgroups per CU, operator-matrix workspace cap (MiB), and the
@@ -230,6 +234,13 @@ int opmat_gemms(bielcc_ctx ctx, const bielcc_impl::OpSeg& sg, int nrhs, const do
 // bielcc_field_batch for many current sets (electric only) through the
 // operator matrices in context-owned grow-only buffers (plan.hip).  inc_d:
 // device [nrhs][6] or null.  counts/offs/tot_d: near-list scratch for M points.
+// FP32 error control of a mixed field evaluated for electric currents only (capi.hip): points
+// the error model does not clear are recomputed with FP64 sums.  inc: host [nrhs][6] or null,
+// inc_d: its device copy.  Synchronises the stream.
+int field_mixed_fixup(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
+                      const double* jE_d, double mEr, double mEi, const double* inc, const double* inc_d, int near_mode,
+                      int M, const double* pts_d, double* E_d, double* H_d);
+
 // bielcc_field_noalloc's per-device scratch address (noalloc.hip), taken at context creation.
 int noalloc_setup(bielcc_ctx ctx);
 

@@ -669,6 +669,192 @@ hipError_t launch_add_incident_h(hipStream_t s, int M, int nrhs, const double* p
 }
 
 
+// ---------------------------------------------------------------- FP32 error control
+// The FP32 node sums of the mixed form carry a rounding error per far pair of
+// about u = 2^-24 of that pair's contribution, nearly independent from pair to
+// pair, so at a point the error of the sum over cells is about
+//   u sqrt(S),  S = sum_c |contribution_c|^2,
+// while the field itself can be far smaller than sqrt(S): near a null of the
+// pattern, or when the currents carry a large non-radiating part (near an
+// interior resonance, where |j| is 100x what the field needs).  A point is
+// recomputed in FP64 when kFixKappa u sqrt(S) exceeds kFixTau times
+// max(|E_p|, 1e-3 max_p |E_p|) (scattered field, per current set; the same for
+// H).  S is modelled from cell centres and areas: the far integrands' magnitudes
+//   |f1| + |f2| for K (the E of j_E, the H of j_M) and |ikr - 1| / r^2 for R,
+// times area_c |j_c|.  On a 7,688-cell sphere with resonant currents the FP32
+// error stayed below 1.43 u sqrt(S) of this model at every point, on an 11,880-cell
+// triangulated sphere below 8.4 u sqrt(S) (where the rest is a relative error of
+// the whole field, harmless); kFixKappa = 4 and kFixTau = 2e-7 leave a factor 5
+// to the 1e-6 target.
+namespace {
+constexpr double kFixKappa = 4.0;
+constexpr double kFixTau = 2e-7;
+constexpr double kU32 = 5.9604644775390625e-08;   // 2^-24
+constexpr int kFixCells = 256;
+
+// positive doubles order like their bit patterns
+__device__ inline void atomic_max_pos(double* a, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(a), (unsigned long long)__double_as_longlong(v));
+}
+}  // namespace
+
+// max over points of |E - E_inc| and |H| per current set (mx zeroed before)
+__global__ __launch_bounds__(256) void k_scat_max(int M, const double* __restrict__ pts, const double* __restrict__ inc,
+                                                  const double* __restrict__ E, const double* __restrict__ H,
+                                                  double* __restrict__ mx) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    double e = 0., h = 0.;
+    if (p < M) {
+        const double* o = E + ((size_t)r * M + p) * 6;
+        cplx v[3] = {mk(o[0], o[1]), mk(o[2], o[3]), mk(o[4], o[5])};
+        if (inc) {
+            const double* ke = inc + 6 * r;
+            const double* x = pts + 3 * (size_t)p;
+            const cplx deg = cexp(cplx{0., 1.} * (ke[0] * x[0] + ke[1] * x[1] + ke[2] * x[2]));
+            for (int g = 0; g < 3; g++) v[g] = v[g] - deg * ke[3 + g];
+        }
+        for (int g = 0; g < 3; g++) e += v[g].re * v[g].re + v[g].im * v[g].im;
+        if (H) {
+            const double* q = H + ((size_t)r * M + p) * 6;
+            for (int a = 0; a < 6; a++) h += q[a] * q[a];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e = fmax(e, __shfl_xor(e, off, 64));
+        h = fmax(h, __shfl_xor(h, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_pos(mx + 2 * r, sqrt(e));
+        atomic_max_pos(mx + 2 * r + 1, sqrt(h));
+    }
+}
+
+// Per (point, current set): the error model above; flag[r][p] = 1 where the point
+// needs FP64 sums.
+__global__ __launch_bounds__(256) void k_fix_flags(FixArgs f) {
+    __shared__ double sc[kFixCells][6];   // centre (3), area |jE|, area |jM|, unused
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    double x0 = 0., x1 = 0., x2 = 0.;
+    if (p < f.M) { x0 = f.pts[3 * (size_t)p]; x1 = f.pts[3 * (size_t)p + 1]; x2 = f.pts[3 * (size_t)p + 2]; }
+    const double k = f.k_re, k2 = k * k;
+    double sE = 0., sH = 0.;
+    for (int c0 = 0; c0 < f.N; c0 += kFixCells) {
+        __syncthreads();
+        for (int cc = threadIdx.x; cc < kFixCells && c0 + cc < f.N; cc += blockDim.x) {
+            const size_t c = (size_t)(c0 + cc);
+            const double* t = f.ctab + c * CT;
+            sc[cc][0] = t[CT_CENTER]; sc[cc][1] = t[CT_CENTER + 1]; sc[cc][2] = t[CT_CENTER + 2];
+            const double* je = f.jE + ((size_t)r * f.N + c) * 6;
+            double a = 0.;
+            for (int q = 0; q < 6; q++) a += je[q] * je[q];
+            sc[cc][3] = t[CT_AREA] * sqrt(a);
+            double b = 0.;
+            if (f.jM) {
+                const double* jm = f.jM + ((size_t)r * f.N + c) * 6;
+                for (int q = 0; q < 6; q++) b += jm[q] * jm[q];
+            }
+            sc[cc][4] = t[CT_AREA] * sqrt(b);
+        }
+        __syncthreads();
+        const int m = min(kFixCells, f.N - c0);
+        for (int cc = 0; cc < m; cc++) {
+            const double d0 = x0 - sc[cc][0], d1 = x1 - sc[cc][1], d2 = x2 - sc[cc][2];
+            const double R2 = d0 * d0 + d1 * d1 + d2 * d2;
+            const double ri = rsqrt(R2), ri2 = ri * ri, ri3 = ri2 * ri;
+            const double a1 = k2 * ri - ri3, a2 = k2 * ri - 3. * ri3;
+            const double fK = sqrt(a1 * a1 + k2 * ri2 * ri2) + sqrt(a2 * a2 + 9. * k2 * ri2 * ri2);
+            const double fR = sqrt(1. + k2 * R2) * ri2;
+            const double ke = sc[cc][3], km = sc[cc][4];
+            sE += f.mE2 * (ke * fK) * (ke * fK) + (km * fR) * (km * fR);
+            sH += (ke * fR) * (ke * fR) + f.mH2 * (km * fK) * (km * fK);
+        }
+    }
+    if (p >= f.M) return;
+    const double* o = f.E + ((size_t)r * f.M + p) * 6;
+    cplx v[3] = {mk(o[0], o[1]), mk(o[2], o[3]), mk(o[4], o[5])};
+    if (f.inc) {
+        const double* kv = f.inc + 6 * r;
+        const cplx deg = cexp(cplx{0., 1.} * (kv[0] * x0 + kv[1] * x1 + kv[2] * x2));
+        for (int g = 0; g < 3; g++) v[g] = v[g] - deg * kv[3 + g];
+    }
+    double e = 0.;
+    for (int g = 0; g < 3; g++) e += v[g].re * v[g].re + v[g].im * v[g].im;
+    e = sqrt(e);
+    const double estE = kFixKappa * kU32 * sqrt(sE) / (4. * kPi);
+    bool flag = estE > kFixTau * fmax(e, 1e-3 * f.mx[2 * r]);
+    if (f.H) {
+        const double* q = f.H + ((size_t)r * f.M + p) * 6;
+        double h = 0.;
+        for (int a = 0; a < 6; a++) h += q[a] * q[a];
+        const double estH = kFixKappa * kU32 * sqrt(sH) / (4. * kPi);
+        flag = flag || estH > kFixTau * fmax(sqrt(h), 1e-3 * f.mx[2 * r + 1]);
+    }
+    f.flags[(size_t)r * f.M + p] = flag ? 1 : 0;
+}
+
+// list of the points flagged for any current set (order arbitrary; each point's
+// FP64 result does not depend on the others)
+__global__ __launch_bounds__(256) void k_fix_compact(int M, int nrhs, const int* __restrict__ flags,
+                                                     int* __restrict__ count, int* __restrict__ list) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= M) return;
+    int any = 0;
+    for (int r = 0; r < nrhs; r++) any |= flags[(size_t)r * M + p];
+    if (any) list[atomicAdd(count, 1)] = p;
+}
+
+__global__ void k_fix_zero(int n, double* a, int* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.;
+    if (i == 0) *count = 0;
+}
+
+__global__ __launch_bounds__(256) void k_fix_gather(int m, const int* __restrict__ list, const double* __restrict__ pts,
+                                                    double* __restrict__ sub) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int p = list[i];
+    for (int d = 0; d < 3; d++) sub[3 * (size_t)i + d] = pts[3 * (size_t)p + d];
+}
+
+// X[r][list[i]] = Xs[r][i] for both fields ([nrhs][M][3] complex / [nrhs][m][3])
+__global__ __launch_bounds__(256) void k_fix_scatter(int m, int M, const int* __restrict__ list,
+                                                     const double* __restrict__ Es, const double* __restrict__ Hs,
+                                                     double* __restrict__ E, double* __restrict__ H) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= m) return;
+    const size_t dst = ((size_t)r * M + list[i]) * 6, src = ((size_t)r * m + i) * 6;
+    for (int a = 0; a < 6; a++) E[dst + a] = Es[src + a];
+    if (H)
+        for (int a = 0; a < 6; a++) H[dst + a] = Hs[src + a];
+}
+
+hipError_t launch_fix_flags(hipStream_t s, const FixArgs& f, int* count, int* list) {
+    if (f.M <= 0 || f.nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fix_zero, dim3((2 * f.nrhs + 255) / 256), dim3(256), 0, s, 2 * f.nrhs, f.mx, count);
+    hipLaunchKernelGGL(k_scat_max, dim3((f.M + 255) / 256, f.nrhs), dim3(256), 0, s, f.M, f.pts, f.inc, f.E, f.H, f.mx);
+    hipLaunchKernelGGL(k_fix_flags, dim3((f.M + 255) / 256, f.nrhs), dim3(256), 0, s, f);
+    hipLaunchKernelGGL(k_fix_compact, dim3((f.M + 255) / 256), dim3(256), 0, s, f.M, f.nrhs, f.flags, count, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_gather(hipStream_t s, int m, const int* list, const double* pts, double* sub) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fix_gather, dim3((m + 255) / 256), dim3(256), 0, s, m, list, pts, sub);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_scatter(hipStream_t s, int m, int M, int nrhs, const int* list, const double* Es,
+                              const double* Hs, double* E, double* H) {
+    if (m <= 0 || nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fix_scatter, dim3((m + 255) / 256, nrhs), dim3(256), 0, s, m, M, list, Es, Hs, E, H);
+    return hipGetLastError();
+}
+
+
 // ---------------------------------------------------------------- operator matrix
 // Field of many current sets: per segment (one mesh, a range of points) the
 // kernels write the operator matrices

@@ -509,6 +509,26 @@ int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan p) {
     } else if (int rc = run_impl(ctx, p)) {
         return rc;
     }
+    // mixed field: FP32 error control per item, after the launch or replay (it reads a
+    // count back, so it stays out of the captured graph)
+    if (p->kind == 1 && p->precision == BIELCC_PREC_MIXED && ctx->mixed_fixup) {
+        const bielcc_num_param np{p->np.eps, p->np.rs, p->np.rs_seg, p->np.analytic_calc, p->np.n_start,
+                                  p->np.n_start_seg, p->np.p_max, p->np.p_max_seg};
+        for (const bielcc_field_item& it : p->fitems) {
+            if (it.M <= 0) continue;
+            std::vector<double> inc;
+            if (it.inc_d) {
+                inc.resize((size_t)it.nrhs * 6);
+                HIPCHK(hipMemcpyAsync(inc.data(), it.inc_d, inc.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+            }
+            if (int rc = field_mixed_fixup(ctx, it.mesh_id, &np, it.k_re, it.k_im, it.nrhs, it.jE_d, it.multE_re,
+                                           it.multE_im, it.inc_d ? inc.data() : nullptr, it.inc_d, p->near_mode, it.M,
+                                           it.pts_d, it.E_d, it.H_d))
+                return rc;
+        }
+    }
     t.stop();
     p->runs++;
     return BIELCC_OK;
