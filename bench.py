@@ -490,15 +490,18 @@ def solve_only(n, reps=3):
     return out
 
 
-def serving_pool(seed=5, n=16, lo=1000, hi=6000):
+def serving_pool(seed=5, n=16, lo=1000, hi=30000):
     """Seeded pool of closed meshes: spheres, triaxial ellipsoids (semi-axes in
-    [0.5, 1.5]), tori (R = 1, r in [0.2, 0.4]) and cubes of side 2, lo..hi cells."""
+    [0.5, 1.5]), tori (R = 1, r in [0.2, 0.4]) and cubes of side 2, with
+    geometrically spaced target sizes lo..hi cells dealt to the kinds in a seeded
+    order (so every kind gets small and large meshes)."""
     from bielcc_amd import mesh
     rng = np.random.default_rng(seed)
+    targets = np.geomspace(lo, hi, n)[rng.permutation(n)]
     pool = {}
     for i in range(n):
         kind = ("sphere", "ellipsoid", "torus", "cube")[i % 4]
-        target = int(rng.uniform(lo, hi))
+        target = int(targets[i])
         if kind == "sphere":
             nt = max(4, int(round(np.sqrt(target / 2))))
             v = mesh.sphere_quads(nt, 2 * nt, jitter=0.005, seed=100 + i)
@@ -514,11 +517,11 @@ def serving_pool(seed=5, n=16, lo=1000, hi=6000):
     return pool
 
 
-def serving_queries(pool, nq, seed=6, m_lo=1000, m_hi=20000):
+def serving_queries(pool, nq, seed=6, m_lo=1000, m_hi=100000):
     """ka uniform in [1, 15] quantised to 8 levels, 1-16 incidences uniform on
-    the sphere with orthogonal polarisations, m_lo..m_hi points on a sphere of
-    radius uniform in [3, 5]."""
-    from bielcc_amd import mesh, serve
+    the sphere with orthogonal polarisations, m_lo..m_hi points uniform on a
+    sphere of radius uniform in [3, 5]."""
+    from bielcc_amd import serve
     rng = np.random.default_rng(seed)
     names = sorted(pool)
     levels = 1.0 + 14.0 * (np.arange(8) + 0.5) / 8
@@ -530,7 +533,10 @@ def serving_queries(pool, nq, seed=6, m_lo=1000, m_hi=20000):
         p = np.cross(d, rng.normal(size=(nrhs, 3)))
         p /= np.linalg.norm(p, axis=1, keepdims=True)
         M = int(rng.integers(m_lo, m_hi + 1))
-        pts = mesh.sphere_points(M, float(rng.uniform(3.0, 5.0)), seed=int(rng.integers(1 << 30)))
+        rad = float(rng.uniform(3.0, 5.0))
+        z, ph = 2.0 * rng.random(M) - 1.0, 2.0 * np.pi * rng.random(M)
+        sz = np.sqrt(1.0 - z * z)
+        pts = rad * np.stack([sz * np.cos(ph), sz * np.sin(ph), z], axis=1)
         qs.append(serve.Query(names[int(rng.integers(len(names)))], float(levels[int(rng.integers(8))]), d, p, pts,
                               qid=i))
     return qs
@@ -547,7 +553,9 @@ def plain_fp64_run(pool, queries):
     phases = {}
     t0 = time.perf_counter()
     lat = []
-    for q in queries:
+    for qi, q in enumerate(queries):
+        if qi % 4 == 0:
+            log(f"[bench] plain FP64: query {qi}/{len(queries)}")
         ts = time.perf_counter()
         m = meshes.get(q.mesh) or meshes.setdefault(q.mesh, ctx.mesh(pool[q.mesh]))
         n2, nr, M = 2 * m.n, q.nrhs, q.points.shape[0]
@@ -656,16 +664,18 @@ def trace_summary(trace_csv, windows_path, top=16):
     return "\n".join(out)
 
 
-def serving_leg(nq=96, workers=4, refine_tol=1e-10, windows_path=None):
+def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream=24, plain_every=2):
     """The same query stream served in the FP64 form and in the mixed form
-    (FP32 LU + FP64 refinement to 1e-10, FP32 far sums), HBM budget 200 GB:
-    once as a burst (every query queued at once: capacity in solves/s) and once
-    as a Poisson stream at 60% of the FP64 form's capacity (latency)."""
+    (FP32 LU + FP64 refinement to 1e-10, FP32 far sums with FP64 recomputation
+    of the points the FP32 error model does not clear), HBM budget 200 GB: once
+    as a burst (every query queued at once: capacity in solves/s) and once as a
+    Poisson stream (the first nq_stream queries) at 60% of the FP64 form's
+    capacity (latency).  The plain FP64 form runs every plain_every-th query."""
     from bielcc_amd import serve
     pool = serving_pool()
     cells = [v.shape[0] for v in pool.values()]
     out = {"workload": f"{len(pool)} meshes ({min(cells)}-{max(cells)} cells: spheres, ellipsoids, tori, cubes), "
-                       f"{nq} queries, ka in [1,15] on 8 levels, 1-16 incidences, 1,000-20,000 points at "
+                       f"{nq} queries, ka in [1,15] on 8 levels, 1-16 incidences, 1,000-100,000 points at "
                        f"r in [3,5]; {workers} workers; HBM budget 200 GB; mixed form refines to a {refine_tol:g} residual "
                        f"and a 1e-7 estimated forward error"}
     res = {}
@@ -675,10 +685,14 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10, windows_path=None):
         out[mode] = {}
         for form in ("fp64", "mixed"):
             qs = serving_queries(pool, nq)
+            if mode == "stream":
+                qs = qs[:nq_stream]
             qps = 1e6 if mode == "burst" else rate
             with serve.Server(pool, form=form, workers=workers, hbm_budget=200e9, refine_tol=refine_tol) as s:
                 t0 = _clock_ns()
-                run = serve.poisson_run(s, qs, qps=qps, seed=7)
+                run = serve.poisson_run(s, qs, qps=qps, seed=7,
+                                        progress=lambda i, n: log(f"[bench] serving {mode} {form}: {i}/{n} "
+                                                                  f"queries done"))
                 windows[f"{mode} {form}"] = [t0, _clock_ns()]
             st = run["stats"]
             log(f"[bench] serving {mode} {form}: {st['wall_s']:.1f} s, {st['solves_per_s']:.1f} solves/s, "
@@ -689,6 +703,7 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10, windows_path=None):
         if mode == "burst":
             rate = 0.6 * nq / out["burst"]["fp64"]["wall_s"]
             out["stream_qps"] = round(rate, 3)
+            out["stream_queries"] = min(nq, nq_stream)
     # error of the scattered field (the total field nearly vanishes at shadow points), split
     # by how the mixed server solved the query: FP32 factors + refinement, or FP64 factors
     # for systems FP32 factors cannot refine (there both servers run FP64 arithmetic and
@@ -728,7 +743,8 @@ def serving_leg(nq=96, workers=4, refine_tol=1e-10, windows_path=None):
     # order, with its own assembly and FP64 LU (no factor reuse, no overlap, current
     # sets in passes of 4)
     t0 = _clock_ns()
-    out["plain_fp64"] = plain_fp64_run(pool, serving_queries(pool, nq))
+    out["plain_fp64"] = plain_fp64_run(pool, serving_queries(pool, nq)[::plain_every])
+    out["plain_fp64"]["sample"] = f"every {plain_every}th query of the {nq}"
     windows["plain fp64"] = [t0, _clock_ns()]
     if windows_path:
         with open(windows_path, "w") as f:
@@ -1061,7 +1077,7 @@ def main():
     ap.add_argument("--copy-calib", action="store_true", help="run a 1 GiB device copy first (counter scale)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-queries", type=int, default=96)
+    ap.add_argument("--serve-queries", type=int, default=32)
     ap.add_argument("--no-batched", action="store_true", help="skip the 12-mesh batched form")
     ap.add_argument("--no-tri", action="store_true", help="skip the 39,600-triangle variant")
     ap.add_argument("--tri-only", type=int, default=0, metavar="STEPS",

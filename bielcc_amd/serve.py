@@ -88,6 +88,7 @@ class Budget:
     def __init__(self, total: int):
         self.total = int(total)
         self.used = 0
+        self.peak = 0          # highest `used` seen
         self.cv = threading.Condition()
 
     def reserve(self, nbytes: int, evict: Callable[[int], tuple], timeout: float = 600.0) -> None:
@@ -115,6 +116,7 @@ class Budget:
                     self.cv.wait(timeout=min(left, 1.0))
                 else:
                     self.used += nbytes
+                    self.peak = max(self.peak, self.used)
                     return
             try:
                 for b in victims:
@@ -128,6 +130,7 @@ class Budget:
         the budget as far as possible."""
         with self.cv:
             self.used += int(nbytes)
+            self.peak = max(self.peak, self.used)
             over = self.used - self.total
             freed, victims = evict(over) if over > 0 else (0, [])
         try:
@@ -255,6 +258,8 @@ class Server:
     4e6) kept a 1.9e-5 field error.  Such systems then stall or reach the FP64
     floor; stalled ones get FP64 factors.  0: residual only.
     """
+
+    _upgrade_gate = threading.Lock()   # one FP64 upgrade at a time (bounds the budget overshoot)
 
     def __init__(self, meshes: dict, form: str = "mixed", workers: int = 3, hbm_budget: float = 200e9,
                  num_param=None, refine_tol: float = 1e-10, device: int = 0, refine_ferr_exp: int = 7):
@@ -412,7 +417,16 @@ class Server:
             ctx = self.ctxs[w]
             n2 = int(round(np.sqrt(e.bufs["Z"].nbytes / 16)))
             extra = 16 * n2 * n2 + 4 * n2
-            self.budget.reserve(extra, self.cache.evict)
+            # the caller holds a pin on e, so it must not wait for budget bytes (pinned
+            # workers waiting on one another could deadlock): the bytes are charged at
+            # once, idle entries evicted, and one upgrade at a time bounds the overshoot
+            # to one FP64 factorisation
+            Server._upgrade_gate.acquire()
+            try:
+                self.budget.charge(extra, self.cache.evict)
+            except BaseException:
+                Server._upgrade_gate.release()
+                raise
             lu = piv = None
             try:
                 lu, piv = ctx.empty(16 * n2 * n2), ctx.empty(4 * n2)
@@ -425,6 +439,8 @@ class Server:
                         b.free()
                 self.budget.release(extra)
                 raise
+            finally:
+                Server._upgrade_gate.release()
             with self.budget.cv:
                 e.bufs["LU64"], e.bufs["piv64"] = lu, piv
                 e.nbytes += extra
@@ -458,6 +474,13 @@ class Server:
         if grow:
             self.budget.reserve(grow, self.cache.evict)
             self.scratch_hw[w] += grow
+        # the query's buffers grow (and are charged) before any cache pin is taken: a
+        # worker that waits for budget bytes must hold no pin, or workers that each
+        # pin an entry and wait could block one another until nothing is evictable
+        dB = self._buf(w, "B", n2 * nrhs * 16)
+        dJ = self._buf(w, "J", n * nrhs * 48)
+        dP = self._buf(w, "P", M * 24)
+        dE, dH = self._buf(w, "E", nrhs * M * 48), self._buf(w, "H", nrhs * M * 48)
         fbytes = self._factor_bytes(n)
         ph: dict = {}
         e, builder = self.cache.pin((q.mesh, float(q.k)), fbytes)
@@ -471,10 +494,6 @@ class Server:
                 self.cache.wait_ready(e)
                 ph["wait"] = time.perf_counter() - t0
             kvec = q.k * dirs
-            dB = self._buf(w, "B", n2 * nrhs * 16)
-            dJ = self._buf(w, "J", n * nrhs * 48)
-            dP = self._buf(w, "P", M * 24)
-            dE, dH = self._buf(w, "E", nrhs * M * 48), self._buf(w, "H", nrhs * M * 48)
             t0 = time.perf_counter()
             ctx.upload(dP, pts)
             m.rhs_planewave_into(dB, n2, kvec, pols)
@@ -528,9 +547,10 @@ def percentiles(lat) -> dict:
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
 
 
-def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
+def poisson_run(server: Server, queries, qps: float, seed: int = 0, progress=None) -> dict:
     """Submit queries at Poisson arrival times (rate qps), wait for all, and
-    return results with latency percentiles and throughput."""
+    return results with latency percentiles and throughput.  progress(done,
+    total), if given, is called as results come in."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / qps, size=len(queries))
     t0 = time.perf_counter()
@@ -542,6 +562,17 @@ def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
             time.sleep(t - now)
         q.arrival = time.perf_counter()
         server.submit(q)
+    if progress is not None:
+        ids, last, t_last = [q.qid for q in queries], -1, time.monotonic()
+        while True:
+            with server.done_cv:
+                done = sum(i in server.results or i in server.errors for i in ids)
+            if done != last or time.monotonic() - t_last > 30.0:
+                progress(done, len(ids))
+                last, t_last = done, time.monotonic()
+            if done == len(ids):
+                break
+            time.sleep(1.0)
     res = server.wait([q.qid for q in queries])
     t1 = max(r.end for r in res.values())
     solves = sum(q.nrhs for q in queries)
@@ -553,6 +584,8 @@ def poisson_run(server: Server, queries, qps: float, seed: int = 0) -> dict:
            "lu_panel_aborts": sum(max(c.get_tuning("lu_panel_aborts"), 0) for c in server.ctxs),
            "trsv_expired": sum(c.get_tuning("trsv_expired") for c in server.ctxs),
            "trsv_ticket_faults": sum(c.get_tuning("trsv_ticket_faults") for c in server.ctxs),
+           "mixed_fixup_points": sum(c.get_tuning("mixed_fixup_points") for c in server.ctxs),
+           "hbm_budget_peak_gb": round(server.budget.peak / 1e9, 2),
            "worker_seconds": {k: round(sum(r.phases.get(k, 0.0) for r in res.values()), 3)
                               for k in ("build", "wait", "solve", "field", "copy")}}
     out.update(percentiles([r.latency for r in res.values()]))
