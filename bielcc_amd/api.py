@@ -339,7 +339,8 @@ class Context:
 
     def refine_solve(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
                      B: DeviceArray, nrhs: int, ldb: int, tol=1e-10, max_iter=30):
-        """FP64 solution of A X = B by refinement on FP32 factors; returns (corrections, residual)."""
+        """FP64 solution of A X = B on FP32 factors (FGMRES preconditioned by them, or
+        classic refinement with tunable refine_mode = 0); returns (corrections, residual)."""
         it, res = C.c_int(0), C.c_double(0.)
         self._check(self.lib.bielcc_refine_solve(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv),
                                                  int(nrhs), _p(B), int(ldb), float(tol), int(max_iter),

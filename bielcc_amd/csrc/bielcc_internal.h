@@ -288,6 +288,9 @@ void lu_set_trsv_limits(LuWork*, int max64, int chunks32, int max32 = -1);
 int lu_get_trsv_max64(const LuWork*);
 int lu_get_trsv_max32(const LuWork*);
 // refinement: also stop only once cond_est x residual <= ferr (0: residual only)
+void lu_set_refine_mode(LuWork*, int mode, int restart);   // negative / 0: unchanged
+int lu_get_refine_mode(const LuWork*);
+int lu_get_refine_restart(const LuWork*);
 void lu_set_refine_ferr(LuWork*, double ferr);
 double lu_get_refine_ferr(const LuWork*);
 bool lu_get_trsv_chunks32(const LuWork*);

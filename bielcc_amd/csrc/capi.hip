@@ -709,6 +709,16 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_refine_ferr(ctx->lu, value == 0 ? 0. : std::pow(10., -value));
         return BIELCC_OK;
     }
+    if (k == "refine_mode") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "refine_mode must be 0 or 1");
+        lu_set_refine_mode(ctx->lu, value, 0);
+        return BIELCC_OK;
+    }
+    if (k == "refine_restart") {
+        if (value < 1 || value > 64) return fail(ctx, BIELCC_ERR_ARG, "refine_restart must be 1..64");
+        lu_set_refine_mode(ctx->lu, -1, value);
+        return BIELCC_OK;
+    }
     if (k == "trsv_max64") {
         if (value < 0 || value > 8) return fail(ctx, BIELCC_ERR_ARG, "trsv_max64 must be 0..8");
         lu_set_trsv_limits(ctx->lu, value, -1);
@@ -817,6 +827,8 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
         const double f = lu_get_refine_ferr(ctx->lu);
         *value = f > 0. ? (int)std::lround(-std::log10(f)) : 0;
     }
+    else if (k == "refine_mode") *value = lu_get_refine_mode(ctx->lu);
+    else if (k == "refine_restart") *value = lu_get_refine_restart(ctx->lu);
     else if (k == "trsv_chunks32") *value = lu_get_trsv_chunks32(ctx->lu) ? 1 : 0;
     else if (k == "trsv_expired") *value = lu_trsv_retries(ctx->lu, 0);
     else if (k == "trsv_ticket_faults") *value = lu_trsv_retries(ctx->lu, 1);

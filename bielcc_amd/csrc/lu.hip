@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <complex>
 #include <vector>
 
 using namespace bielcc_dev;
@@ -73,6 +74,10 @@ struct LuWork {
     size_t perm_cap = 0;
     void* ref = nullptr;        // refinement work vectors
     size_t ref_cap = 0;
+    void* kry = nullptr;        // Krylov bases of the FGMRES refinement (V, Z) and its small arrays
+    size_t kry_cap = 0;
+    int refine_mode = 1;        // 1: FGMRES on the FP32 factors, 0: classic refinement (one correction a step)
+    int refine_restart = 20;    // FGMRES basis size per cycle
     // sync-free triangular solve: per-block ready flags (epoch-stamped, never
     // reset), the ticket counter that orders the workgroups, a timeout flag
     unsigned* flags = nullptr;
@@ -133,6 +138,12 @@ void lu_set_trsv_limits(LuWork* w, int max64, int chunks32, int max32) {
 }
 int lu_get_trsv_max64(const LuWork* w) { return w->trsv_max64; }
 int lu_get_trsv_max32(const LuWork* w) { return w->trsv_max32; }
+void lu_set_refine_mode(LuWork* w, int mode, int restart) {
+    if (mode >= 0) w->refine_mode = mode;
+    if (restart > 0) w->refine_restart = restart;
+}
+int lu_get_refine_mode(const LuWork* w) { return w->refine_mode; }
+int lu_get_refine_restart(const LuWork* w) { return w->refine_restart; }
 void lu_set_refine_ferr(LuWork* w, double ferr) { w->refine_ferr = ferr; }
 double lu_get_refine_ferr(const LuWork* w) { return w->refine_ferr; }
 bool lu_get_trsv_chunks32(const LuWork* w) { return w->trsv_chunks32; }
@@ -142,7 +153,7 @@ int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 int lu_late_admits(const LuWork* w) { return w->late_admits; }
 size_t lu_work_bytes(const LuWork* w) {   // grow-only device work of the LU and the solves
-    return w ? w->perm_cap + w->ref_cap + (size_t)w->pcap * (sizeof(double) + sizeof(int)) : 0;
+    return w ? w->perm_cap + w->ref_cap + w->kry_cap + (size_t)w->pcap * (sizeof(double) + sizeof(int)) : 0;
 }
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
 int lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
@@ -183,7 +194,7 @@ void lu_work_destroy(LuWork* w) {
     if (!w) return;
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
-    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->step_count);
+    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->step_count);
     (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap);
     if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
     if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
@@ -1847,6 +1858,286 @@ __global__ void k_copy2(int n, int nrhs, const double2* __restrict__ src, long l
     dst[(size_t)r * ldd + i] = src[(size_t)r * lds + i];
 }
 
+// ---------------------------------------------------------------- FGMRES on the FP32 factors
+// The Krylov blocks hold nrhs columns of n entries each (column c at c * n); the
+// right-hand sides are independent systems that share every launch.
+
+// h[i nrhs + c] = V_i[:, c]^H W[:, c], one workgroup per (i, c)
+__global__ __launch_bounds__(256) void k_kry_dots(int n, int nrhs, const double2* __restrict__ V,
+                                                  const double2* __restrict__ W, double2* __restrict__ h) {
+    __shared__ double sr[256], si[256];
+    const int c = blockIdx.x % nrhs, i = blockIdx.x / nrhs, t = threadIdx.x;
+    const double2* v = V + ((size_t)i * nrhs + c) * n;
+    const double2* x = W + (size_t)c * n;
+    double ar = 0., ai = 0.;
+    for (int r = t; r < n; r += 256) {
+        const double2 a = v[r], b = x[r];
+        ar += a.x * b.x + a.y * b.y;
+        ai += a.x * b.y - a.y * b.x;
+    }
+    sr[t] = ar;
+    si[t] = ai;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            sr[t] += sr[t + off];
+            si[t] += si[t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) h[(size_t)i * nrhs + c] = make_double2(sr[0], si[0]);
+}
+
+// W[:, c] -= sum_{i < nb} h[i nrhs + c] V_i[:, c]
+__global__ void k_kry_sub(int n, int nrhs, int nb, const double2* __restrict__ V, const double2* __restrict__ h,
+                          double2* __restrict__ W) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t blk = (size_t)n * nrhs;
+    if (idx >= (long long)blk) return;
+    const int c = (int)(idx / n);
+    double2 acc = W[idx];
+    for (int i = 0; i < nb; i++) {
+        const double2 a = h[(size_t)i * nrhs + c], v = V[(size_t)i * blk + idx];
+        acc.x -= a.x * v.x - a.y * v.y;
+        acc.y -= a.x * v.y + a.y * v.x;
+    }
+    W[idx] = acc;
+}
+
+// W[:, c] /= nrm[c] (columns of norm 0 stay 0)
+__global__ void k_kry_scale(int n, int nrhs, const double* __restrict__ nrm, double2* __restrict__ W) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const double d = nrm[idx / n];
+    const double q = d > 0. ? 1. / d : 0.;
+    double2 v = W[idx];
+    v.x *= q;
+    v.y *= q;
+    W[idx] = v;
+}
+
+// X[:, c] += sum_{i < nb} y[i nrhs + c] Z_i[:, c]
+__global__ void k_kry_update(int n, int nrhs, int nb, const double2* __restrict__ Z, const double2* __restrict__ y,
+                             double2* __restrict__ X, long long ldx) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t blk = (size_t)n * nrhs;
+    if (idx >= (long long)blk) return;
+    const int c = (int)(idx / n), r = (int)(idx % n);
+    double2 acc = X[(size_t)c * ldx + r];
+    for (int i = 0; i < nb; i++) {
+        const double2 a = y[(size_t)i * nrhs + c], z = Z[(size_t)i * blk + idx];
+        acc.x += a.x * z.x - a.y * z.y;
+        acc.y += a.x * z.y + a.y * z.x;
+    }
+    X[(size_t)c * ldx + r] = acc;
+}
+
+// C = A B (FP64 complex, n x nrhs result)
+static hipError_t gemm_mul(LuWork* w, hipStream_t s, long long m, long long n, long long k, const double2* Aa,
+                           long long lda, const double2* Bb, long long ldb, double2* Cc, long long ldc,
+                           const char** msg) {
+    rocblas_set_stream(w->blas, s);
+    rocblas_double_complex alpha = {1., 0.}, beta = {0., 0.};
+    rocblas_status st = rocblas_zgemm_64(w->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
+                                         reinterpret_cast<const rocblas_double_complex*>(Aa), lda,
+                                         reinterpret_cast<const rocblas_double_complex*>(Bb), ldb, &beta,
+                                         reinterpret_cast<rocblas_double_complex*>(Cc), ldc);
+    if (st != rocblas_status_success) {
+        *msg = "rocblas_zgemm_64 failed";
+        return hipErrorUnknown;
+    }
+    return hipGetLastError();
+}
+
+namespace {
+using cplx = std::complex<double>;
+inline cplx to_c(double2 v) { return {v.x, v.y}; }
+}  // namespace
+
+// Restarted flexible GMRES for A X = B0, right-preconditioned by the FP32 factors
+// (Arioli and Duff, "Using FGMRES to obtain backward stability in mixed precision",
+// ETNA 33, 2009).  The preconditioned directions Z_j = F^{-1} V_j are kept, so the
+// Arnoldi relation A Z = V H holds in FP64 however inexact the FP32 solves are:
+// one cycle cuts the residual by many orders where a classic correction gains only
+// a factor cond(A) u32, and systems with cond(A) u32 near or above 1 (interior
+// resonances) still converge.  Each cycle starts from the true FP64 residual.
+// Inner steps: Z_j (FP32 solve), W = A Z_j (ZGEMM), classical Gram-Schmidt twice,
+// the Hessenberg column to the host for the Givens rotations and the residual
+// estimate.  Stops (status 0) once the true residual is <= tol and, when an error
+// estimate is asked for (ferr > 0), the residual has also reached the FP64 floor:
+// below kResidFloor, or well above what the last cycle's FP64 recurrence predicted
+// (rounding in A X then dominates); fails (status 1) on a residual that stops
+// shrinking above tol or after max_iter inner steps.
+static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long long lda, const float2* F,
+                         long long ldf, int nrhs, double2* X, long long ldb, const double2* B0, double2* Rd,
+                         float2* Rf, double* nrm, float2* tmp, const std::vector<double>& bn, double tol,
+                         int max_iter, int* iters, double* resid, int* status, const char** msg) {
+    hipError_t e;
+    const size_t blk = (size_t)n * nrhs;
+    const unsigned gv = (unsigned)((blk + 255) / 256);
+    int m = std::max(1, w->refine_restart);
+    while (m > 2 && (2 * (size_t)m + 1) * blk * sizeof(double2) > ((size_t)2 << 30)) m--;
+    const size_t vb = (size_t)(m + 1) * blk * sizeof(double2), zb = (size_t)m * blk * sizeof(double2);
+    const size_t hb = 2 * (size_t)(m + 1) * nrhs * sizeof(double2), yb = (size_t)m * nrhs * sizeof(double2);
+    const size_t need = vb + zb + hb + yb + 1024;
+    if (w->kry_cap < need) {
+        (void)hipFree(w->kry);
+        w->kry = nullptr;
+        w->kry_cap = 0;
+        if ((e = hipMalloc(&w->kry, need)) != hipSuccess) return e;
+        w->kry_cap = need;
+    }
+    char* base = reinterpret_cast<char*>(w->kry);
+    double2* V = reinterpret_cast<double2*>(base);
+    double2* Z = reinterpret_cast<double2*>(base + vb);
+    double2* hd = reinterpret_cast<double2*>(base + vb + zb);
+    double2* yd = reinterpret_cast<double2*>(base + vb + zb + hb);
+    double2* hd1 = hd + (size_t)(m + 1) * nrhs;
+
+    std::vector<double> rn(nrhs), hn(nrhs);
+    std::vector<double2> h0((size_t)(m + 1) * nrhs), h1((size_t)(m + 1) * nrhs);
+    std::vector<cplx> H((size_t)nrhs * (m + 1) * m), g((size_t)nrhs * (m + 1)), sn((size_t)nrhs * m);
+    std::vector<double> cs((size_t)nrhs * m);
+    std::vector<int> kc(nrhs);
+    std::vector<char> live(nrhs);
+    std::vector<double2> yh((size_t)m * nrhs);
+    const double target = w->refine_ferr > 0. ? 1e-15 : 0.5 * tol;   // per cycle, relative to ||b||
+    double rel_prev = HUGE_VAL, est_prev = -1.;
+    for (;;) {
+        // Rd = B0 - A X
+        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
+        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, (long long)n, msg)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, (long long)n, nrm);
+        if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if ((e = trsv_check(s, w, msg)) != hipSuccess) return e;
+        double rel = 0.;
+        for (int c = 0; c < nrhs; c++) rel = std::max(rel, bn[c] > 0. ? rn[c] / bn[c] : rn[c]);
+        *resid = rel;
+        if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }
+        if (rel <= tol) {
+            if (w->refine_ferr <= 0. || rel <= kResidFloor) return hipSuccess;
+            if (est_prev >= 0. && rel > 8. * est_prev) return hipSuccess;   // at the FP64 floor
+        }
+        if (*iters >= max_iter) { *status = 1; return hipSuccess; }
+        if (rel_prev < HUGE_VAL && !(rel < 0.5 * rel_prev)) {   // no progress: the floor, or a breakdown
+            *status = rel <= tol ? 0 : 1;
+            return hipSuccess;
+        }
+        rel_prev = rel;
+        // V_0 = R / ||R||
+        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, (long long)n, V, (long long)n);
+        hipLaunchKernelGGL(k_kry_scale, dim3(gv), dim3(256), 0, s, n, nrhs, nrm, V);
+        std::fill(g.begin(), g.end(), cplx(0.));
+        int active = 0;
+        for (int c = 0; c < nrhs; c++) {
+            g[(size_t)c * (m + 1)] = rn[c];
+            kc[c] = 0;   // Arnoldi steps taken by column c in this cycle
+            live[c] = rn[c] > 0. && rn[c] > target * bn[c];
+            active += live[c];
+        }
+        int j = 0;
+        double est = 0.;
+        for (; j < m && active > 0 && *iters < max_iter; j++) {
+            double2* Vj = V + (size_t)j * blk;
+            double2* Vn = V + (size_t)(j + 1) * blk;
+            double2* Zj = Z + (size_t)j * blk;
+            // Z_j = F^{-1} V_j (FP32 solve), V_{j+1} = A Z_j
+            hipLaunchKernelGGL(k_to_f32, dim3(stride_blocks(blk)), dim3(256), 0, s, n, nrhs, Vj, (long long)n, Rf,
+                               (long long)n);
+            if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, Zj, (long long)n);
+            if ((e = gemm_mul(w, s, n, nrhs, n, A, lda, Zj, (long long)n, Vn, (long long)n, msg)) != hipSuccess)
+                return e;
+            // classical Gram-Schmidt, twice
+            hipLaunchKernelGGL(k_kry_dots, dim3((j + 1) * nrhs), dim3(256), 0, s, n, nrhs, V, Vn, hd);
+            hipLaunchKernelGGL(k_kry_sub, dim3(gv), dim3(256), 0, s, n, nrhs, j + 1, V, hd, Vn);
+            hipLaunchKernelGGL(k_kry_dots, dim3((j + 1) * nrhs), dim3(256), 0, s, n, nrhs, V, Vn, hd1);
+            hipLaunchKernelGGL(k_kry_sub, dim3(gv), dim3(256), 0, s, n, nrhs, j + 1, V, hd1, Vn);
+            hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Vn, (long long)n, nrm);
+            hipLaunchKernelGGL(k_kry_scale, dim3(gv), dim3(256), 0, s, n, nrhs, nrm, Vn);
+            const size_t hcount = (size_t)(j + 1) * nrhs * sizeof(double2);
+            if ((e = hipMemcpyAsync(h0.data(), hd, hcount, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(h1.data(), hd1, hcount, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(hn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+                return e;
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            if ((e = trsv_check(s, w, msg)) != hipSuccess) return e;
+            ++*iters;
+            for (int c = 0; c < nrhs; c++) {
+                if (!live[c]) continue;
+                cplx* Hc = &H[((size_t)c * m + j) * (m + 1)];   // column j of column c's Hessenberg matrix
+                for (int i = 0; i <= j; i++) {
+                    const double2 a = h0[(size_t)i * nrhs + c], b = h1[(size_t)i * nrhs + c];
+                    Hc[i] = cplx(a.x + b.x, a.y + b.y);
+                }
+                const double hj1 = hn[c];
+                for (int i = 0; i < j; i++) {   // the earlier rotations
+                    const double ci = cs[(size_t)c * m + i];
+                    const cplx si = sn[(size_t)c * m + i];
+                    const cplx x = Hc[i], y = Hc[i + 1];
+                    Hc[i] = ci * x + si * y;
+                    Hc[i + 1] = -std::conj(si) * x + ci * y;
+                }
+                // the rotation that zeroes h_{j+1,j} (real, >= 0) against h_{j,j}
+                const cplx a = Hc[j];
+                const double aa = std::abs(a), t = std::hypot(aa, hj1);
+                double cj = 1.;
+                cplx sj = 0.;
+                if (aa == 0. && t > 0.) {
+                    cj = 0.;
+                    sj = 1.;
+                    Hc[j] = hj1;
+                } else if (t > 0.) {
+                    cj = aa / t;
+                    sj = (a / aa) * (hj1 / t);
+                    Hc[j] = (a / aa) * t;
+                }
+                cs[(size_t)c * m + j] = cj;
+                sn[(size_t)c * m + j] = sj;
+                cplx* gc = &g[(size_t)c * (m + 1)];
+                gc[j + 1] = -std::conj(sj) * gc[j];
+                gc[j] = cj * gc[j];
+                kc[c] = j + 1;
+                // converged, or the Krylov space holds the solution (h_{j+1,j} = 0): the column stops
+                if (std::abs(gc[j + 1]) <= target * bn[c] || !(hj1 > 0.)) {
+                    live[c] = false;
+                    active--;
+                }
+            }
+        }
+        est = 0.;
+        for (int c = 0; c < nrhs; c++) {
+            const double r = kc[c] > 0 ? std::abs(g[(size_t)c * (m + 1) + kc[c]]) : rn[c];
+            est = std::max(est, bn[c] > 0. ? r / bn[c] : r);
+        }
+        // y = H^{-1} g per column (upper triangular, kc[c] steps), X += Z y
+        std::fill(yh.begin(), yh.end(), make_double2(0., 0.));
+        for (int c = 0; c < nrhs; c++) {
+            const int k = kc[c];
+            if (k <= 0) continue;
+            std::vector<cplx> y(k);
+            for (int i = k - 1; i >= 0; i--) {
+                cplx acc = g[(size_t)c * (m + 1) + i];
+                for (int l = i + 1; l < k; l++) acc -= H[((size_t)c * m + l) * (m + 1) + i] * y[l];
+                const cplx d = H[((size_t)c * m + i) * (m + 1) + i];
+                y[i] = d != cplx(0.) ? acc / d : cplx(0.);
+            }
+            for (int i = 0; i < k; i++) yh[(size_t)i * nrhs + c] = make_double2(y[i].real(), y[i].imag());
+        }
+        if (j > 0) {
+            if ((e = hipMemcpyAsync(yd, yh.data(), (size_t)j * nrhs * sizeof(double2), hipMemcpyHostToDevice, s)) !=
+                hipSuccess)
+                return e;
+            hipLaunchKernelGGL(k_kry_update, dim3(gv), dim3(256), 0, s, n, nrhs, j, Z, yd, X, ldb);
+            // the host array must outlive the copy
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        }
+        est_prev = est;
+    }
+}
+
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const float* Fd,
                         long long ldf, const int* piv, int nrhs, double* Bd, long long ldb, double tol, int max_iter,
                         int* iters, double* resid, int* status, const char** msg) {
@@ -1890,6 +2181,9 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long 
         hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rf, (long long)n);
         if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
+        if (w->refine_mode == 1)
+            return fgmres(s, w, n, A, lda, F, ldf, nrhs, X, ldb, B0, Rd, Rf, nrm, tmp, bn, tol, max_iter, iters, resid,
+                          status, msg);
         double prev = HUGE_VAL, rho_min = HUGE_VAL;
         for (int it = 0;; it++) {
             // Rd = B0 - A X (FP64)

@@ -12,11 +12,11 @@ solver or server; this is the request loop around the path:
 - resident state: meshes stay uploaded per worker; the factors of the
   collocation matrix stay on the device per (mesh, k) in an LRU cache
   (FP64 LU in the "fp64" form; the FP64 matrix plus its FP32 LU in the
-  "mixed" form, whose solves refine in FP64 to `refine_tol` and to an
-  estimated forward error below 10^-`refine_ferr_exp`).  Systems too
-  ill-conditioned for FP32 factors (near an interior resonance of the closed
-  body, where refinement stalls or the FP32 LU meets a zero pivot) get FP64
-  factors once, beside the FP32 ones, and are solved with them from then on;
+  "mixed" form, whose solves run FGMRES in FP64 preconditioned by the FP32
+  factors, past `refine_tol` down to the FP64 rounding floor).  Systems too
+  ill-conditioned for FP32 factors (where even FGMRES stalls above
+  `refine_tol`, or the FP32 LU meets a zero pivot) get FP64 factors once,
+  beside the FP32 ones, and are solved with them from then on;
   their fields are summed in FP64 too (the currents then carry a large
   non-radiating part whose exterior field cancels);
 - admission by HBM budget: factors, every query's buffers (right-hand
@@ -250,13 +250,13 @@ class Server:
     meshes: name -> vertex array [n][3|4][3].  form: "fp64" (FP64 LU) or
     "mixed" (FP32 LU + FP64 refinement, FP32 far-pair field sums).
 
-    refine_ferr_exp = e > 0: a refinement stops only once its residual is below
-    refine_tol and its estimated forward error (cond(Z) from the residual's
-    contraction per correction, times the residual) is below 10^-e as well.  A
-    residual target alone does not bound the error of an ill-conditioned system:
-    refined to a 1e-10 residual, a low-frequency elongated ellipsoid (cond about
-    4e6) kept a 1.9e-5 field error.  Such systems then stall or reach the FP64
-    floor; stalled ones get FP64 factors.  0: residual only.
+    refine_ferr_exp = e > 0: a solve does not stop at refine_tol alone but goes on
+    to the FP64 rounding floor (FGMRES; classic refinement, refine_mode 0:
+    until its forward-error estimate, cond(Z) from the residual's contraction
+    times the residual, is below 10^-e).  A residual target alone does not bound
+    the error of an ill-conditioned system: refined to a 1e-10 residual, a
+    low-frequency elongated ellipsoid (cond about 4e6) kept a 1.9e-5 field
+    error.  0: residual only.
     """
 
     _upgrade_gate = threading.Lock()   # one FP64 upgrade at a time (bounds the budget overshoot)

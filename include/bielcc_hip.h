@@ -318,13 +318,17 @@ int bielcc_assemble_ex(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_
  * A_d rounded to FP32; A_d is not modified.  piv_d[n], info as bielcc_lu_factor. */
 int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf, int* piv_d,
                        int* info);
-/* Solves A X = B in FP64 by iterative refinement on the FP32 factors:
- * X0 = F^{-1} B, then X += F^{-1} (B - A X) with the residual in FP64, until
- * max_r ||B_r - A X_r||_2 / ||B_r||_2 <= tol or max_iter corrections.  B_d is
- * overwritten by X; *iters = corrections applied, *resid = final relative
- * residual.  Returns BIELCC_ERR_UNSUPPORTED if a correction shrinks the
- * residual by less than 10% while it is above tol (matrix too ill-conditioned
- * for FP32 factors); X is then the last iterate. */
+/* Solves A X = B in FP64 on the FP32 factors, from X0 = F^{-1} B, until
+ * max_r ||B_r - A X_r||_2 / ||B_r||_2 <= tol or max_iter corrections.  By
+ * default (tunable "refine_mode" = 1) the corrections are the steps of a
+ * restarted flexible GMRES in FP64 (basis of "refine_restart" [20] vectors per
+ * cycle), right-preconditioned by the FP32 factors; each cycle starts from the
+ * FP64 residual.  With "refine_mode" = 0 they are classic refinement steps,
+ * X += F^{-1} (B - A X).  B_d is overwritten by X; *iters = corrections applied
+ * (FP32 solves after X0), *resid = final relative residual.  Returns
+ * BIELCC_ERR_UNSUPPORTED when the residual stops shrinking above tol (classic
+ * steps: by less than 10% a step; FGMRES: by less than half a cycle), or when
+ * max_iter corrections leave it above tol; X is then the last iterate. */
 int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
                         const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
                         int* iters, double* resid);
@@ -407,9 +411,12 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * chunks of 16;
  * "trsm_lib" [1] = the other solves use rocBLAS trsm per triangle (0: the
  * library's blocked TRSM + GEMM loop); "refine_ferr_exp" [0] = e > 0: the
- * refinement also requires an estimated forward error (cond estimated from
- * the residual's contraction, times the residual) below 10^-e before it
- * stops; "trsv_spin"
+ * refinement does not stop at tol alone: classic steps also require an
+ * estimated forward error (cond estimated from the residual's contraction,
+ * times the residual) below 10^-e, FGMRES requires the residual to have
+ * reached the FP64 floor (below 1e-14, or well above what the last cycle's
+ * recurrence predicted); "refine_mode" [1] and "refine_restart" [20]: see
+ * bielcc_refine_solve; "trsv_spin"
  * [1048576] = polls per wait of the one-kernel solve before it gives up (a
  * solve whose wait gave up is redone on the blocked path; tests lower it);
  * "lu_fused_panel" [2] = one launch per LU panel column (1), two (0), or

@@ -589,11 +589,13 @@ def test_two_level_lu_matches_one_level(ctx, n):
         assert backward(out[2], out[3]) <= n * 1e-6
 
 
-def test_refine_forward_error_stop(ctx):
-    """refine_ferr_exp = e: the refinement goes on past the residual target until the
-    estimated forward error (cond estimated from the residual's contraction, times
-    the residual) is below 10^-e; the solution then matches the FP64 solve at least
-    as closely, with at least as many corrections."""
+@pytest.mark.parametrize("mode", [1, 0])
+def test_refine_forward_error_stop(ctx, mode):
+    """refine_ferr_exp = e: the refinement goes on past the residual target (classic
+    steps, mode 0: until the estimated forward error, cond estimated from the
+    residual's contraction times the residual, is below 10^-e; FGMRES, mode 1: to
+    the FP64 floor); the solution then matches the FP64 solve at least as closely,
+    with at least as many corrections."""
     n = 600
     rng = np.random.default_rng(18)
     A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 2 * np.sqrt(n) * np.eye(n)
@@ -603,6 +605,7 @@ def test_refine_forward_error_stop(ctx):
     dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
     ctx.lu32_factor(dA, n, n, dF, n, piv)
     out = {}
+    ctx.set_tuning("refine_mode", mode)
     try:
         for e in (0, 11):
             ctx.set_tuning("refine_ferr_exp", e)
@@ -612,6 +615,80 @@ def test_refine_forward_error_stop(ctx):
             out[e] = (it, res, np.linalg.norm(X - Xr) / np.linalg.norm(Xr))
     finally:
         ctx.set_tuning("refine_ferr_exp", 0)
+        ctx.set_tuning("refine_mode", 1)
     assert ctx.get_tuning("refine_ferr_exp") == 0
     assert out[11][0] >= out[0][0] and out[11][1] <= out[0][1]
     assert out[11][2] <= max(out[0][2], 1e-13) and out[11][2] <= 1e-11
+
+
+def _graded(n, smin, seed):
+    """Random complex matrix with its two smallest singular values set to 3 smin and
+    smin times the largest (cond(A) = 1 / smin)."""
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    U, s, Vh = np.linalg.svd(A)
+    s[-1], s[-2] = s[0] * smin, s[0] * smin * 3
+    return (U * s) @ Vh, rng
+
+
+def test_fgmres_converges_where_classic_refinement_fails(ctx):
+    """cond(A) = 1e7: cond u32 is near 1, so classic corrections on the FP32 factors
+    do not shrink the residual (code 4) or take more steps, while the default
+    FGMRES refinement (basis of 20) reaches tol 1e-9, below the residual of the
+    FP64 direct solve of the same system, and matches that solve to cond u64; a
+    zero right-hand side stays zero."""
+    n = 400
+    A, rng = _graded(n, 1e-7, 3)
+    b = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    b[:, 1] = 0
+    Xr = np.linalg.solve(A, b)
+    r64 = (np.linalg.norm(A @ Xr - b, axis=0)[[0, 2]] / np.linalg.norm(b, axis=0)[[0, 2]]).max()
+    assert r64 > 1e-9   # the direct FP64 solve itself stays above tol
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.lu32_factor(dA, n, n, dF, n, piv)
+    assert ctx.get_tuning("refine_mode") == 1 and ctx.get_tuning("refine_restart") == 20
+    dB = ctx.to_device(np.ascontiguousarray(b.T))
+    it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 3, n, 1e-9, 60)
+    X = dB.to_numpy((3, n), np.complex128).T
+    rel = np.linalg.norm(A @ X - b, axis=0)[[0, 2]] / np.linalg.norm(b, axis=0)[[0, 2]]
+    assert res <= 1e-9 and rel.max() <= 2e-9 and 1 <= it <= 60
+    assert not np.any(X[:, 1])
+    assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-7
+    ctx.set_tuning("refine_mode", 0)
+    try:
+        dB = ctx.to_device(np.ascontiguousarray(b.T))
+        try:
+            it0, _ = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 3, n, 1e-9, 60)
+            assert it0 > it
+        except B.BielccError as err:
+            assert err.code == 4
+    finally:
+        ctx.set_tuning("refine_mode", 1)
+
+
+def test_fgmres_restart_sizes(ctx):
+    """Bases of 1, 2 and 20 vectors per cycle (1: each cycle is one classic step
+    preconditioned and minimised along one direction) converge on a well-conditioned
+    system to 1e-12 with the same solution; the tunables reject sizes outside 1..64
+    and modes other than 0 and 1."""
+    n = 500
+    rng = np.random.default_rng(21)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + np.sqrt(n) * np.eye(n)
+    b = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
+    Xr = np.linalg.solve(A, b)
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.lu32_factor(dA, n, n, dF, n, piv)
+    try:
+        for m in (1, 2, 20):
+            ctx.set_tuning("refine_restart", m)
+            dB = ctx.to_device(np.ascontiguousarray(b.T))
+            it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 4, n, 1e-12, 30)
+            X = dB.to_numpy((4, n), np.complex128).T
+            assert res <= 1e-12 and np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e-11, m
+        for key, bad in (("refine_restart", 0), ("refine_restart", 65), ("refine_mode", 2)):
+            with pytest.raises(B.BielccError):
+                ctx.set_tuning(key, bad)
+    finally:
+        ctx.set_tuning("refine_restart", 20)
