@@ -107,6 +107,8 @@ _SIGS = {
                                      C.c_void_p, _IP]),
     "bielcc_refine_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                       C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, _IP, _DP]),
+    "bielcc_residual": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                   C.c_int64, C.c_int, C.c_int, C.c_void_p, C.c_int64]),
     "bielcc_far_pattern": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NumParam), C.c_double, C.c_double, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),

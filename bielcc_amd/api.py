@@ -347,6 +347,12 @@ class Context:
                                                  C.byref(it), C.byref(res)))
         return it.value, res.value
 
+    def residual(self, A: DeviceArray, n: int, lda: int, X: DeviceArray, ldx: int, B: DeviceArray, ldb: int,
+                 nrhs: int, R: DeviceArray, ldr: int, doubled: bool = False) -> None:
+        """R = B - A X on the device, sums in FP64 or (doubled) in twice its precision."""
+        self._check(self.lib.bielcc_residual(self.h, int(n), _p(A), int(lda), _p(X), int(ldx), _p(B), int(ldb),
+                                             int(nrhs), int(bool(doubled)), _p(R), int(ldr)))
+
     def set_tuning(self, key: str, value: int):
         self._check(self.lib.bielcc_set_tuning(self.h, key.encode(), int(value)))
 

@@ -291,6 +291,12 @@ int lu_get_trsv_max32(const LuWork*);
 void lu_set_refine_mode(LuWork*, int mode, int restart);   // negative / 0: unchanged
 int lu_get_refine_mode(const LuWork*);
 int lu_get_refine_restart(const LuWork*);
+void lu_set_resid_dd(LuWork*, int mode);
+int lu_get_resid_dd(const LuWork*);
+// R = B - A X (n x nrhs, complex FP64), sums in FP64 (ZGEMM) or doubled precision (Dot2)
+hipError_t residual(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const double* X, long long ldx,
+                    const double* B, long long ldb, int nrhs, bool doubled, double* R, long long ldr,
+                    const char** msg);
 void lu_set_refine_ferr(LuWork*, double ferr);
 double lu_get_refine_ferr(const LuWork*);
 bool lu_get_trsv_chunks32(const LuWork*);

@@ -676,6 +676,21 @@ int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, c
     return BIELCC_OK;
 }
 
+int bielcc_residual(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const double* X_d, int64_t ldx,
+                    const double* B_d, int64_t ldb, int nrhs, int doubled, double* R_d, int64_t ldr) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || nrhs <= 0 || !A_d || !X_d || !B_d || !R_d || lda < n || ldx < n || ldb < n || ldr < n)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if (doubled != 0 && doubled != 1) return fail(ctx, BIELCC_ERR_ARG, "doubled must be 0 or 1");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    Timer t(ctx, "residual");
+    hipError_t e = residual(ctx->stream, ctx->lu, n, A_d, lda, X_d, ldx, B_d, ldb, nrhs, doubled == 1, R_d, ldr, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    return BIELCC_OK;
+}
+
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
     if (!ctx || !key) return BIELCC_ERR_ARG;
     const std::string k(key);
@@ -712,6 +727,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
     if (k == "refine_mode") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "refine_mode must be 0 or 1");
         lu_set_refine_mode(ctx->lu, value, 0);
+        return BIELCC_OK;
+    }
+    if (k == "refine_resid_dd") {
+        if (value < 0 || value > 2) return fail(ctx, BIELCC_ERR_ARG, "refine_resid_dd must be 0, 1 or 2");
+        lu_set_resid_dd(ctx->lu, value);
         return BIELCC_OK;
     }
     if (k == "refine_restart") {
@@ -829,6 +849,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     }
     else if (k == "refine_mode") *value = lu_get_refine_mode(ctx->lu);
     else if (k == "refine_restart") *value = lu_get_refine_restart(ctx->lu);
+    else if (k == "refine_resid_dd") *value = lu_get_resid_dd(ctx->lu);
     else if (k == "trsv_chunks32") *value = lu_get_trsv_chunks32(ctx->lu) ? 1 : 0;
     else if (k == "trsv_expired") *value = lu_trsv_retries(ctx->lu, 0);
     else if (k == "trsv_ticket_faults") *value = lu_trsv_retries(ctx->lu, 1);

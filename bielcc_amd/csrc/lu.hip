@@ -78,6 +78,10 @@ struct LuWork {
     size_t kry_cap = 0;
     int refine_mode = 1;        // 1: FGMRES on the FP32 factors, 0: classic refinement (one correction a step)
     int refine_restart = 20;    // FGMRES basis size per cycle
+    int resid_dd = 1;           // FGMRES residuals in doubled precision: 0 never, 1 once the FP64 residual has
+                                // reached its rounding floor (forward-error requests only), 2 always
+    void* dd = nullptr;         // partial sums of the doubled-precision residual
+    size_t dd_cap = 0;
     // sync-free triangular solve: per-block ready flags (epoch-stamped, never
     // reset), the ticket counter that orders the workgroups, a timeout flag
     unsigned* flags = nullptr;
@@ -143,6 +147,8 @@ void lu_set_refine_mode(LuWork* w, int mode, int restart) {
     if (restart > 0) w->refine_restart = restart;
 }
 int lu_get_refine_mode(const LuWork* w) { return w->refine_mode; }
+void lu_set_resid_dd(LuWork* w, int mode) { w->resid_dd = mode; }
+int lu_get_resid_dd(const LuWork* w) { return w->resid_dd; }
 int lu_get_refine_restart(const LuWork* w) { return w->refine_restart; }
 void lu_set_refine_ferr(LuWork* w, double ferr) { w->refine_ferr = ferr; }
 double lu_get_refine_ferr(const LuWork* w) { return w->refine_ferr; }
@@ -153,7 +159,8 @@ int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 int lu_late_admits(const LuWork* w) { return w->late_admits; }
 size_t lu_work_bytes(const LuWork* w) {   // grow-only device work of the LU and the solves
-    return w ? w->perm_cap + w->ref_cap + w->kry_cap + (size_t)w->pcap * (sizeof(double) + sizeof(int)) : 0;
+    return w ? w->perm_cap + w->ref_cap + w->kry_cap + w->dd_cap + (size_t)w->pcap * (sizeof(double) + sizeof(int))
+             : 0;
 }
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
 int lu_get_panel_lds(const LuWork* w) { return w->panel_lds; }
@@ -194,7 +201,8 @@ void lu_work_destroy(LuWork* w) {
     if (!w) return;
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
-    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->step_count);
+    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->dd);
+    (void)hipFree(w->step_count);
     (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap);
     if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
     if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
@@ -1932,6 +1940,120 @@ __global__ void k_kry_update(int n, int nrhs, int nb, const double2* __restrict_
     X[(size_t)c * ldx + r] = acc;
 }
 
+// ---------------------------------------------------------------- doubled-precision residual
+// R = B - A X with every sum carried as an unevaluated pair s + c (Ogita, Rump and
+// Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput. 26, 2005: Dot2),
+// as accurate as sums in twice the FP64 precision; the result is rounded to FP64.
+// The rounding of b - A x in FP64 bounds how far a refinement can take x (about
+// sqrt(n) u64 |A| |x|); with this residual the limit is the rounding of x itself.
+// (no contraction in these: a fused multiply-add would change the error terms)
+__device__ inline void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+// (s, c) += x y
+__device__ inline void dot2_add(double& s, double& c, double x, double y) {
+#pragma clang fp contract(off)
+    const double p = x * y;
+    const double ep = fma(x, y, -p);
+    double t, es;
+    two_sum(s, p, t, es);
+    s = t;
+    c += es + ep;
+}
+
+constexpr int kDdNR = 8;     // right-hand sides per pass
+constexpr int kDdCols = 64;  // columns of X staged in LDS at a time
+
+// part[(ks nrhs + c) n + r] = (s_re, c_re, s_im, c_im) of sum_{j in chunk ks} A[r, j] X[j, c]
+// for the right-hand sides c0 .. c0 + kDdNR - 1 (those below nrhs)
+__global__ __launch_bounds__(256) void k_resid_dd(int n, int nrhs, int c0, int chunk, const double2* __restrict__ A,
+                                                  long long lda, const double2* __restrict__ X, long long ldx,
+                                                  double4* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double2 xs[kDdCols][kDdNR];
+    const int r = blockIdx.x * 256 + threadIdx.x, ks = blockIdx.y;
+    const int j0 = ks * chunk, j1 = min(n, j0 + chunk);
+    const int nr = min(kDdNR, nrhs - c0);
+    double sr[kDdNR], cr[kDdNR], si[kDdNR], ci[kDdNR];
+#pragma unroll
+    for (int q = 0; q < kDdNR; q++) sr[q] = cr[q] = si[q] = ci[q] = 0.;
+    for (int jt = j0; jt < j1; jt += kDdCols) {
+        const int nj = min(kDdCols, j1 - jt);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kDdCols * kDdNR; e += 256) {
+            const int jj = e / kDdNR, q = e % kDdNR;
+            xs[jj][q] = (jj < nj && q < nr) ? X[(size_t)(c0 + q) * ldx + jt + jj] : make_double2(0., 0.);
+        }
+        __syncthreads();
+        if (r < n)
+            for (int jj = 0; jj < nj; jj++) {
+                const double2 a = A[(size_t)(jt + jj) * lda + r];
+#pragma unroll
+                for (int q = 0; q < kDdNR; q++) {
+                    const double2 x = xs[jj][q];
+                    dot2_add(sr[q], cr[q], a.x, x.x);
+                    dot2_add(sr[q], cr[q], -a.y, x.y);
+                    dot2_add(si[q], ci[q], a.x, x.y);
+                    dot2_add(si[q], ci[q], a.y, x.x);
+                }
+            }
+    }
+    if (r < n)
+        for (int q = 0; q < nr; q++)
+            part[((size_t)ks * nrhs + c0 + q) * n + r] = make_double4(sr[q], cr[q], si[q], ci[q]);
+}
+
+// R[:, c] = B[:, c] - (the nks partial sums), rounded once
+__global__ void k_resid_dd_fin(int n, int nrhs, int nks, const double4* __restrict__ part,
+                               const double2* __restrict__ B, long long ldb, double2* __restrict__ R, long long ldr) {
+#pragma clang fp contract(off)
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int r = (int)(idx % n), c = (int)(idx / n);
+    const double2 b = B[(size_t)c * ldb + r];
+    double s0 = b.x, c0 = 0., s1 = b.y, c1 = 0.;
+    for (int k = 0; k < nks; k++) {
+        const double4 p = part[((size_t)k * nrhs + c) * n + r];
+        double t, e;
+        two_sum(s0, -p.x, t, e);
+        s0 = t;
+        c0 += e - p.y;
+        two_sum(s1, -p.z, t, e);
+        s1 = t;
+        c1 += e - p.w;
+    }
+    R[(size_t)c * ldr + r] = make_double2(s0 + c0, s1 + c1);
+}
+
+static hipError_t residual_dd(hipStream_t s, LuWork* w, int n, const double2* A, long long lda, const double2* X,
+                              long long ldx, const double2* B, long long ldb, int nrhs, double2* R, long long ldr) {
+    const int rb = (n + 255) / 256;
+    // column chunks: enough workgroups to fill the device, partial sums within 256 MiB
+    int nks = std::max(1, std::min(16, (8 * std::max(1, w->n_cu)) / rb));
+    while (nks > 1 && (size_t)nks * nrhs * n * sizeof(double4) > ((size_t)256 << 20)) nks--;
+    const int chunk = ((n + nks - 1) / nks + kDdCols - 1) / kDdCols * kDdCols;
+    nks = (n + chunk - 1) / chunk;
+    const size_t need = (size_t)nks * nrhs * n * sizeof(double4);
+    hipError_t e;
+    if (w->dd_cap < need) {
+        (void)hipFree(w->dd);
+        w->dd = nullptr;
+        w->dd_cap = 0;
+        if ((e = hipMalloc(&w->dd, need)) != hipSuccess) return e;
+        w->dd_cap = need;
+    }
+    double4* part = reinterpret_cast<double4*>(w->dd);
+    for (int c0 = 0; c0 < nrhs; c0 += kDdNR)
+        hipLaunchKernelGGL(k_resid_dd, dim3(rb, nks), dim3(256), 0, s, n, nrhs, c0, chunk, A, lda, X, ldx, part);
+    const size_t nv = (size_t)n * nrhs;
+    hipLaunchKernelGGL(k_resid_dd_fin, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, n, nrhs, nks, part, B, ldb,
+                       R, ldr);
+    return hipGetLastError();
+}
+
 // C = A B (FP64 complex, n x nrhs result)
 static hipError_t gemm_mul(LuWork* w, hipStream_t s, long long m, long long n, long long k, const double2* Aa,
                            long long lda, const double2* Bb, long long ldb, double2* Cc, long long ldc,
@@ -2003,10 +2125,17 @@ static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long
     std::vector<double2> yh((size_t)m * nrhs);
     const double target = w->refine_ferr > 0. ? 1e-15 : 0.5 * tol;   // per cycle, relative to ||b||
     double rel_prev = HUGE_VAL, est_prev = -1.;
+    bool dd = w->resid_dd == 2;
+    int dd_cycles = 0;
     for (;;) {
         // Rd = B0 - A X
-        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
-        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, (long long)n, msg)) != hipSuccess) return e;
+        if (dd) {
+            if ((e = residual_dd(s, w, n, A, lda, X, ldb, B0, (long long)n, nrhs, Rd, (long long)n)) != hipSuccess)
+                return e;
+        } else {
+            hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rd, (long long)n);
+            if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, (long long)n, msg)) != hipSuccess) return e;
+        }
         hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, (long long)n, nrm);
         if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
             return e;
@@ -2017,8 +2146,22 @@ static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long
         *resid = rel;
         if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }
         if (rel <= tol) {
-            if (w->refine_ferr <= 0. || rel <= kResidFloor) return hipSuccess;
-            if (est_prev >= 0. && rel > 8. * est_prev) return hipSuccess;   // at the FP64 floor
+            if (w->refine_ferr <= 0.) return hipSuccess;
+            // the floor: the true residual well above what the last cycle's FP64 recurrence
+            // predicted (rounding in B - A X, or in X itself once the residual is doubled)
+            const bool floor = est_prev >= 0. && rel > 8. * est_prev;
+            if (!dd) {
+                if (rel <= kResidFloor) return hipSuccess;
+                if (floor) {
+                    if (w->resid_dd != 1) return hipSuccess;
+                    dd = true;   // go on with doubled residuals: B - A X in FP64 hid the rest
+                    rel_prev = HUGE_VAL;
+                    est_prev = -1.;
+                    continue;
+                }
+            } else if (rel <= 0.1 * kResidFloor || floor || dd_cycles >= 3 || *iters >= max_iter) {
+                return hipSuccess;
+            }
         }
         if (*iters >= max_iter) { *status = 1; return hipSuccess; }
         if (rel_prev < HUGE_VAL && !(rel < 0.5 * rel_prev)) {   // no progress: the floor, or a breakdown
@@ -2026,6 +2169,7 @@ static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long
             return hipSuccess;
         }
         rel_prev = rel;
+        dd_cycles += dd;
         // V_0 = R / ||R||
         hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, (long long)n, V, (long long)n);
         hipLaunchKernelGGL(k_kry_scale, dim3(gv), dim3(256), 0, s, n, nrhs, nrm, V);
@@ -2136,6 +2280,21 @@ static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long
         }
         est_prev = est;
     }
+}
+
+hipError_t residual(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const double* Xd, long long ldx,
+                    const double* Bd, long long ldb, int nrhs, bool doubled, double* Rd, long long ldr,
+                    const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    const double2* A = reinterpret_cast<const double2*>(Ad);
+    const double2* X = reinterpret_cast<const double2*>(Xd);
+    const double2* B = reinterpret_cast<const double2*>(Bd);
+    double2* R = reinterpret_cast<double2*>(Rd);
+    if (doubled) return residual_dd(s, w, n, A, lda, X, ldx, B, ldb, nrhs, R, ldr);
+    const size_t nv = (size_t)n * nrhs;
+    hipLaunchKernelGGL(k_copy2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, n, nrhs, B, ldb, R, ldr);
+    return gemm_update(w, s, n, nrhs, n, A, lda, X, ldx, R, ldr, msg);
 }
 
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const float* Fd,

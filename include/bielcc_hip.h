@@ -333,6 +333,18 @@ int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, c
                         const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
                         int* iters, double* resid);
 
+/* R = B - A X for n x n A and n x nrhs X, B, R (complex FP64, column-major).
+ * doubled = 0: sums in FP64 (ZGEMM); 1: every sum carried as a pair s + c
+ * (Ogita-Rump-Oishi Dot2), as accurate as sums in twice the FP64 precision,
+ * rounded once to FP64 -- the FP64 residual of a solution near the rounding
+ * floor is mostly rounding error (about sqrt(n) u64 |A||X|), this one is not.
+ * With "refine_ferr_exp" set, the FGMRES refinement switches to doubled
+ * residuals once its FP64 residual has reached that floor ("refine_resid_dd"
+ * [1]; 0: never, 2: from the first residual), so its solution converges to the
+ * rounding of X itself.  No counterpart in the reference (which has no solver). */
+int bielcc_residual(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const double* X_d, int64_t ldx,
+                    const double* B_d, int64_t ldb, int nrhs, int doubled, double* R_d, int64_t ldr);
+
 /* Far-field pattern F(d) = lim_{R->inf} R e^{-ikR} (E, H)(R d) of the field of
  * bielcc_field (no reference counterpart: the limit of K_RotRot_Far and
  * R_Rot_Colloc, kernel_lib.h:453-492 / :295-313, over the same cell nodes):

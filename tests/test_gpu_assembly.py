@@ -667,6 +667,103 @@ def test_fgmres_converges_where_classic_refinement_fails(ctx):
         ctx.set_tuning("refine_mode", 1)
 
 
+def _two_prod(a, b):
+    """Exact products a b = p + e of float64 arrays (Dekker's splitting, no FMA)."""
+    def split(x):
+        c = 134217729.0 * x
+        hi = c - (c - x)
+        return hi, x - hi
+    p = a * b
+    ah, al = split(a)
+    bh, bl = split(b)
+    return p, ((ah * bh - p) + ah * bl + al * bh) + al * bl
+
+
+def _exact_residual(A, X, Bm):
+    """B - A X rounded once to FP64: every product split exactly, each sum by math.fsum."""
+    import math
+    n, nr = X.shape
+    R = np.empty((n, nr), complex)
+    for c in range(nr):
+        terms_re, terms_im = [], []
+        for (a, x, sgn, im) in ((A.real, X[:, c].real, -1, 0), (A.imag, X[:, c].imag, 1, 0),
+                                (A.real, X[:, c].imag, -1, 1), (A.imag, X[:, c].real, -1, 1)):
+            p, e = _two_prod(a, x[None, :])
+            (terms_im if im else terms_re).append((sgn * p, sgn * e))
+        for r in range(n):
+            re = [Bm[r, c].real] + [t[r, j] for pe in terms_re for t in pe for j in range(n)]
+            ii = [Bm[r, c].imag] + [t[r, j] for pe in terms_im for t in pe for j in range(n)]
+            R[r, c] = complex(math.fsum(re), math.fsum(ii))
+    return R
+
+
+def test_residual_doubled_is_exactly_rounded(ctx):
+    """bielcc_residual with doubled sums against the exactly rounded B - A X (products
+    split exactly, sums by math.fsum) where B = fl(A X) + a few ulps, so that the true
+    residual is tiny next to |A||X|: doubled, within a few ulps of the exact residual
+    (several column chunks, a second group of right-hand sides); FP64 sums, wrong by
+    far more than the residual itself."""
+    rng = np.random.default_rng(31)
+    n, nr = 200, 9
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    X = rng.normal(size=(n, nr)) + 1j * rng.normal(size=(n, nr))
+    Bm = A @ X
+    Bm = Bm + 4e-16 * np.abs(Bm) * (rng.normal(size=Bm.shape) + 1j * rng.normal(size=Bm.shape))
+    Rx = _exact_residual(A, X, Bm)
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dX = ctx.to_device(np.ascontiguousarray(X.T))
+    dB = ctx.to_device(np.ascontiguousarray(Bm.T))
+    out = {}
+    for doubled in (True, False):
+        dR = ctx.empty(n * nr * 16)
+        ctx.residual(dA, n, n, dX, n, dB, n, nr, dR, n, doubled=doubled)
+        out[doubled] = dR.to_numpy((nr, n), np.complex128).T
+    scale = np.abs(Rx).max()
+    assert np.abs(out[True] - Rx).max() <= 1e-12 * scale
+    assert np.abs(out[False] - Rx).max() >= 1e-2 * scale
+    with pytest.raises(B.BielccError):
+        ctx.residual(dA, n, n - 1, dX, n, dB, n, nr, ctx.empty(16 * n * nr), n)
+
+
+def test_fgmres_doubled_residual_accuracy(ctx):
+    """cond(A) = 1e6 with a forward-error request: FGMRES with FP64 residuals stops
+    at their rounding floor; switching to doubled residuals there (the default) takes
+    the solution on to about u64 of the exact one (reference: LAPACK FP64 LU refined
+    with 80-bit residuals), ten times closer than LAPACK's own FP64 solve."""
+    n = 400
+    A, rng = _graded(n, 1e-6, 5)
+    b = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    L = np.longdouble
+
+    def resid_ext(X):
+        Ar, Ai, Xr, Xi = A.real.astype(L), A.imag.astype(L), X.real.astype(L), X.imag.astype(L)
+        return ((b.real.astype(L) - (Ar @ Xr - Ai @ Xi)).astype(float)
+                + 1j * (b.imag.astype(L) - (Ar @ Xi + Ai @ Xr)).astype(float))
+    import scipy.linalg as sl
+    lu = sl.lu_factor(A)
+    X64 = sl.lu_solve(lu, b)
+    Xt = X64.copy()
+    for _ in range(8):
+        Xt = Xt + sl.lu_solve(lu, resid_ext(Xt))
+    err = lambda X: np.linalg.norm(X - Xt) / np.linalg.norm(Xt)
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dF, piv = ctx.empty(n * n * 8), ctx.empty(4 * n)
+    ctx.lu32_factor(dA, n, n, dF, n, piv)
+    assert ctx.get_tuning("refine_resid_dd") == 1
+    out = {}
+    ctx.set_tuning("refine_ferr_exp", 7)
+    try:
+        for mode in (0, 1):
+            ctx.set_tuning("refine_resid_dd", mode)
+            dB = ctx.to_device(np.ascontiguousarray(b.T))
+            it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 2, n, 1e-10, 60)
+            out[mode] = err(dB.to_numpy((2, n), np.complex128).T)
+    finally:
+        ctx.set_tuning("refine_ferr_exp", 0)
+        ctx.set_tuning("refine_resid_dd", 1)
+    assert out[1] <= 1e-12 and out[1] <= 0.1 * err(X64) and out[1] <= 0.1 * out[0], (out, err(X64))
+
+
 def test_fgmres_restart_sizes(ctx):
     """Bases of 1, 2 and 20 vectors per cycle (1: each cycle is one classic step
     preconditioned and minimised along one direction) converge on a well-conditioned
