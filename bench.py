@@ -664,7 +664,78 @@ def trace_summary(trace_csv, windows_path, top=16):
     return "\n".join(out)
 
 
-def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream=24, plain_every=2):
+def accuracy_probe(pool, q, refine_tol=1e-10):
+    """One query solved three ways on one context: FP64 LU (the FP64 form), FP32 LU +
+    FGMRES as served (refine_tol, forward-error request 1e-7), and an accurate
+    reference (the FP64 LU refined by three classic steps with doubled-precision
+    residuals, Dot2 sums).  Returns each solution's residual (doubled sums) and the
+    scattered-E errors at the query's points against the reference's (FP64 field
+    sums for all three, plus the served mixed field), per point with a floor of
+    1e-3 of the largest scattered field, and set-wise."""
+    import bielcc_amd as B
+    ctx = B.Context(0)
+    ctx.set_tuning("refine_ferr_exp", 7)
+    npp = B.num_param()
+    m = ctx.mesh(pool[q.mesh])
+    n2, nr, M = 2 * m.n, q.nrhs, q.points.shape[0]
+    kvec, pols = q.k * np.asarray(q.dirs), np.asarray(q.pols)
+    inc = np.concatenate([kvec, pols], axis=1)
+    dZ, dL = ctx.empty(n2 * n2 * 16), ctx.empty(n2 * n2 * 16)
+    dF, dpiv, dp32 = ctx.empty(n2 * n2 * 8), ctx.empty(4 * n2), ctx.empty(4 * n2)
+    dB0, dR = ctx.empty(n2 * nr * 16), ctx.empty(n2 * nr * 16)
+    m.assemble_into(dZ, n2, npp, q.k)
+    m.assemble_into(dL, n2, npp, q.k)
+    m.rhs_planewave_into(dB0, n2, kvec, pols)
+    b = dB0.to_numpy((nr, n2), np.complex128)
+    ctx.lu_factor(dL, n2, n2, dpiv)
+    ctx.lu32_factor(dZ, n2, n2, dF, n2, dp32)
+    sols = {}
+    d64 = ctx.to_device(b)
+    ctx.lu_solve(dL, n2, n2, dpiv, d64, nr, n2)
+    sols["fp64_lu"] = d64.to_numpy((nr, n2), np.complex128)
+    dm = ctx.to_device(b)
+    it, res = ctx.refine_solve(dZ, n2, n2, dF, n2, dp32, dm, nr, n2, refine_tol, 60)
+    sols["mixed_fgmres"] = dm.to_numpy((nr, n2), np.complex128)
+    xt = sols["fp64_lu"].copy()
+    for _ in range(3):
+        dX = ctx.to_device(xt)
+        ctx.residual(dZ, n2, n2, dX, n2, dB0, n2, nr, dR, n2, doubled=True)
+        ctx.lu_solve(dL, n2, n2, dpiv, dR, nr, n2)
+        xt = xt + dR.to_numpy((nr, n2), np.complex128)
+    sols["reference"] = xt
+    out = {"mesh": q.mesh, "k": q.k, "nrhs": nr, "points": M, "unknowns": n2, "mixed_refine_corrections": it}
+    dP = ctx.to_device(np.asarray(q.points))
+    dJ, dE, dH = ctx.empty(m.n * nr * 48), ctx.empty(nr * M * 48), ctx.empty(nr * M * 48)
+    fields = {}
+    for name, x in sols.items():
+        dX = ctx.to_device(x)
+        ctx.residual(dZ, n2, n2, dX, n2, dB0, n2, nr, dR, n2, doubled=True)
+        r = dR.to_numpy((nr, n2), np.complex128)
+        out[f"resid_{name}"] = float((np.linalg.norm(r, axis=1) / np.linalg.norm(b, axis=1)).max())
+        out[f"x_err_{name}"] = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
+        m.currents_into(dX, n2, nr, dJ)
+        for prec in ((B.PREC_FP64, B.PREC_MIXED) if name == "mixed_fgmres" else (B.PREC_FP64,)):
+            m.field_batch_into(npp, q.k, nr, dJ, dP, M, dE, dH, inc=None, precision=prec)
+            fields[(name, prec)] = dE.to_numpy((nr, M, 3), np.complex128)
+    ref = fields[("reference", B.PREC_FP64)]
+    for (name, prec), E in fields.items():
+        if name == "reference":
+            continue
+        pt, st = 0.0, 0.0
+        for r in range(nr):
+            nes = np.linalg.norm(ref[r], axis=1)
+            e = np.linalg.norm(E[r] - ref[r], axis=1) / np.maximum(nes, 1e-3 * nes.max())
+            pt, st = max(pt, float(e.max())), max(st, float(np.linalg.norm(E[r] - ref[r]) / np.linalg.norm(ref[r])))
+        tag = name + ("" if prec == B.PREC_FP64 else "_mixed_field")
+        out[f"E_point_max_rel_{tag}"], out[f"E_set_max_rel_{tag}"] = pt, st
+    m.close()
+    ctx.close()
+    log(f"[bench] accuracy probe {q.mesh} k={q.k}: " + ", ".join(f"{k} {v:.2e}" for k, v in out.items()
+                                                                if k.startswith(("E_", "resid_", "x_err"))))
+    return out
+
+
+def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream=24, plain_every=1):
     """The same query stream served in the FP64 form and in the mixed form
     (FP32 LU + FP64 refinement to 1e-10, FP32 far sums with FP64 recomputation
     of the points the FP32 error model does not clear), HBM budget 200 GB: once
@@ -710,7 +781,7 @@ def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream
     # differ by the system's own conditioning times rounding)
     qs = {q.qid: q for q in serving_queries(pool, nq)}
     errs = {"refined": [0.0, 0.0, 0], "fp64_factors": [0.0, 0.0, 0]}
-    worst = None
+    worst = worst_pt = None
     fp64_factor_queries = []
     for i, r64 in res["fp64"].items():
         r32, q = res["mixed"][i], qs[i]
@@ -730,6 +801,8 @@ def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream
             if not r32.fp64_fallback and (worst is None or g > worst["set_rel_err"]):
                 worst = {"query": i, "mesh": q.mesh, "k": q.k, "refine_resid": r32.refine_resid,
                          "refine_iters": r32.refine_iters, "set_rel_err": g}
+            if not r32.fp64_fallback and (worst_pt is None or float(e.max()) > worst_pt[1]):
+                worst_pt = (i, float(e.max()))
     out["err_scattered_E_refined"] = {"point_max_rel": errs["refined"][0], "set_max_rel": errs["refined"][1],
                                       "queries": errs["refined"][2]}
     out["err_scattered_E_fp64_factors"] = {"point_max_rel": errs["fp64_factors"][0],
@@ -744,11 +817,17 @@ def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream
     # sets in passes of 4)
     t0 = _clock_ns()
     out["plain_fp64"] = plain_fp64_run(pool, serving_queries(pool, nq)[::plain_every])
-    out["plain_fp64"]["sample"] = f"every {plain_every}th query of the {nq}"
+    out["plain_fp64"]["sample"] = (f"all {nq} queries" if plain_every == 1 else
+                                   f"every {plain_every}th query of the {nq}")
     windows["plain fp64"] = [t0, _clock_ns()]
     if windows_path:
         with open(windows_path, "w") as f:
             json.dump(windows, f)
+    if worst_pt is not None:
+        # which of the two differs from the exact solution: the refined query with the largest
+        # point error against the FP64 form, solved again both ways beside an accurate reference
+        out["accuracy_probe"] = accuracy_probe(pool, qs[worst_pt[0]], refine_tol)
+        out["accuracy_probe"]["query"] = worst_pt[0]
     out["served_mixed_vs_plain_fp64_solves_per_s"] = round(out["burst"]["mixed"]["solves_per_s"] /
                                                            out["plain_fp64"]["solves_per_s"], 3)
     out["served_fp64_vs_plain_fp64_solves_per_s"] = round(out["burst"]["fp64"]["solves_per_s"] /

@@ -250,6 +250,8 @@ class Server:
     meshes: name -> vertex array [n][3|4][3].  form: "fp64" (FP64 LU) or
     "mixed" (FP32 LU + FP64 refinement, FP32 far-pair field sums).
 
+    refine_max_iter: corrections (FP32 solves after the first) a refinement may take.
+
     refine_ferr_exp = e > 0: a solve does not stop at refine_tol alone but goes on
     to the FP64 rounding floor (FGMRES; classic refinement, refine_mode 0:
     until its forward-error estimate, cond(Z) from the residual's contraction
@@ -262,13 +264,15 @@ class Server:
     _upgrade_gate = threading.Lock()   # one FP64 upgrade at a time (bounds the budget overshoot)
 
     def __init__(self, meshes: dict, form: str = "mixed", workers: int = 3, hbm_budget: float = 200e9,
-                 num_param=None, refine_tol: float = 1e-10, device: int = 0, refine_ferr_exp: int = 7):
+                 num_param=None, refine_tol: float = 1e-10, device: int = 0, refine_ferr_exp: int = 7,
+                 refine_max_iter: int = 60):
         if form not in ("fp64", "mixed"):
             raise ValueError("form must be 'fp64' or 'mixed'")
         self.meshes = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in meshes.items()}
         self.form = form
         self.np = num_param if num_param is not None else api.num_param()
         self.refine_tol = refine_tol
+        self.refine_max_iter = int(refine_max_iter)
         self.budget = Budget(int(hbm_budget))
         self.cache = FactorCache(self.budget)
         self.q: "queue.Queue[Optional[Query]]" = queue.Queue()
@@ -504,7 +508,7 @@ class Server:
             if self.form == "mixed" and not fallback:
                 try:
                     iters, resid = ctx.refine_solve(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], dB, nrhs,
-                                                    n2, self.refine_tol, 30)
+                                                    n2, self.refine_tol, self.refine_max_iter)
                 except api.BielccError as err:
                     if err.code != 4:      # refinement stalled: FP64 factors for this entry
                         raise

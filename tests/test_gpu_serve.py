@@ -112,3 +112,57 @@ def test_mixed_form_falls_back_to_fp64_factors_when_refinement_stalls():
         assert out[q.qid].fp64_fallback
         for r in range(q.nrhs):
             assert rowwise_rel(out[q.qid].E[r][:, :], r64[q.qid].E[r]).max() <= 1e-6
+
+
+def _c5_pool():
+    m = B.mesh
+    return {"torus": m.torus_quads(192, 64, R=1.0, r=0.3),   # 12,288 quads
+            "cube": m.cube_quads(32, side=2.0)}                # 6,144 quads
+
+
+def _c5_queries():
+    rng = np.random.default_rng(75)
+    spec = [("torus", 3.625, 3, 100000, 4.0), ("cube", 5.375, 2, 50000, 3.0), ("torus", 7.125, 2, 20000, 5.0),
+            ("cube", 5.375, 1, 100000, 3.5)]
+    out = []
+    for i, (name, k, nrhs, M, rad) in enumerate(spec):
+        d = rng.normal(size=(nrhs, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        p = np.cross(d, rng.normal(size=(nrhs, 3)))
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        out.append(serve.Query(name, k, d, p, B.mesh.sphere_points(M, rad, seed=500 + i), qid=i))
+    return out
+
+
+@pytest.mark.parametrize("form,budget", [("fp64", 15e9), ("mixed", 22e9)])
+def test_serving_sized_queries_alone_and_in_the_mix(form, budget):
+    """Queries of the serving workload's sizes (a 12,288-quad torus and a 6,144-quad
+    cube, up to 100,000 points, one repeated (mesh, k)): served one at a time with
+    room for everything, then all at once by three workers under a budget that holds
+    one torus factorisation (the second torus query evicts the first), the results
+    agree to 1e-12; the mixed form is within 1e-6 of the FP64 form set-wise."""
+    qs = _c5_queries()
+    pool = _c5_pool()
+    alone = {}
+    with serve.Server(pool, form=form, workers=1, hbm_budget=100e9) as s:
+        for q in qs:
+            s.submit(_copy(q))
+            alone[q.qid] = s.wait([q.qid])[q.qid]
+    with serve.Server(pool, form=form, workers=3, hbm_budget=budget) as s:
+        run = serve.poisson_run(s, [_copy(q) for q in qs], qps=1e6, seed=3)
+    st = run["stats"]
+    print(form, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items() if k != "fp64_upgrades"})
+    assert st["evictions"] >= 1
+    for q in qs:
+        a, b = alone[q.qid], run["results"][q.qid]
+        assert a.E.shape == (q.nrhs, q.points.shape[0], 3)
+        assert np.abs(a.E - b.E).max() <= 1e-12 * np.abs(a.E).max(), q.qid
+        assert np.abs(a.H - b.H).max() <= 1e-12 * np.abs(a.H).max(), q.qid
+    if form == "mixed":
+        with serve.Server(pool, form="fp64", workers=1, hbm_budget=100e9) as s:
+            for q in qs[:2]:
+                s.submit(_copy(q))
+                r64 = s.wait([q.qid])[q.qid]
+                for r in range(q.nrhs):
+                    es = r64.E[r] - q.pols[r][None, :] * np.exp(1j * q.k * (q.points @ q.dirs[r]))[:, None]
+                    assert np.linalg.norm(alone[q.qid].E[r] - r64.E[r]) <= 1e-6 * np.linalg.norm(es), q.qid
