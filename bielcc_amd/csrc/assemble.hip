@@ -85,7 +85,9 @@ __device__ inline void surf_sums32(const float4* nd, const float* ny, int nn, co
 
 // One tile of far pairs: collocation points bx*256 .. +255 (one per lane) x
 // cells [c_begin, c_end), staged through the LDS arrays handed in.
-template <bool CK, int WHICH, bool FP, bool P32>
+// PH: node phase evaluation, 0 general sincos, 1 quadrant reduction (phase.h),
+// 2 centre-phase reduction with the short series (phase.h)
+template <bool CK, int WHICH, int PH, bool P32>
 __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int c_begin, int c_end, double4* ssurf,
                                               double4* sseg, double (*scell)[kARec]) {
     const int tid = threadIdx.x;
@@ -139,11 +141,14 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
             const int c = cs + cc;
             const double* rec = scell[cc];
             const double dcen = center_dist(x[0], x[1], x[2], rec);
+            constexpr bool FP = PH != 0;   // FP32 node sums: quadrant reduction in either mode
             PhaseRef pr{0., 0., 0., 0};
-            if (FP) pr = phase_ref(kr_ * dcen);
+            PhaseC pc{0., 0., 1., 0.};
+            if (PH == 1) pr = phase_ref(kr_ * dcen);
             cplx z[2][2];
             if (WHICH == 0) {
                 if (near_rule(dcen, rec[3], 0, a.ac, 0.)) continue;   // near pair: exact kernel writes it
+                if (PH == 2) pc = phase_centre(kr_, dcen);
                 // S = int e^{ikr}/r  (F_SimplePot_H, smoothR = DBL_EPSILON: factor 1/r)
                 double Sr = 0., Si = 0.;
                 const double4* nd = ssurf + cc * nn;
@@ -159,7 +164,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                     const double ri = rsqrt(r2);
                     const double r = r2 * ri;
                     double s, co;
-                    if (FP) sincos_near(kr_, r, pr, s, co); else sincos(kr_ * r, &s, &co);
+                    phase_eval<PH>(kr_, r, pr, pc, s, co);
                     double w = y.w * ri;
                     if (CK) w *= exp(-ki_ * r);
                     Sr += w * co;
@@ -177,7 +182,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                         const double ri = rsqrt(r2);
                         const double r = r2 * ri;
                         double s, co;
-                        if (FP) sincos_near(kr_, r, pr, s, co); else sincos(kr_ * r, &s, &co);
+                        phase_eval<PH>(kr_, r, pr, pc, s, co);
                         double w = y.w * ri * ri * ri;
                         if (CK) w *= exp(-ki_ * r);
                         const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
@@ -195,10 +200,14 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                         k0b[b][2] += mk(gr2, gi2) * jn;
                     }
                 }
-                if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
+                if (PH == 1) {   // restore the common factor i^n of the cell's e^{ikr}
                     if (!P32) rot_q(Sr, Si, pr.q);
                     for (int b = 0; b < 2; b++)
                         for (int g = 0; g < 3; g++) rot_q(k0b[b][g].re, k0b[b][g].im, pr.q);
+                } else if (PH == 2) {   // ... or e^{i k dcen}
+                    if (!P32) rot_c(Sr, Si, pc);
+                    for (int b = 0; b < 2; b++)
+                        for (int g = 0; g < 3; g++) rot_c(k0b[b][g].re, k0b[b][g].im, pc);
                 }
                 const cplx Sk2 = (mk(Sr, Si) * k) * k;
                 for (int b = 0; b < 2; b++) {
@@ -218,6 +227,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                     // In mixed mode LDS holds the FP32 table, so they read the FP64
                     // nodes from global memory (few pairs, L2-resident).
                     const bool use32 = P32 && !near_rule(dcen, rec[3], 0, a.ac, 0.);
+                    if (PH == 2 && !use32) pc = phase_centre(kr_, dcen);
                     const double4* nd = P32 ? reinterpret_cast<const double4*>(a.nodes) + (size_t)c * nn
                                             : ssurf + cc * nn;
                     if (use32) {
@@ -232,7 +242,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                         const double ri = rsqrt(r2);
                         const double r = r2 * ri;
                         double s, co;
-                        if (FP) sincos_near(kr_, r, pr, s, co); else sincos(kr_ * r, &s, &co);
+                        phase_eval<PH>(kr_, r, pr, pc, s, co);
                         double w = y.w * ri * ri * ri;
                         if (CK) w *= exp(-ki_ * r);
                         const double mr = CK ? -(1. + ki_ * r) : -1., mi = kr_ * r;
@@ -241,8 +251,10 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                         Gr[1] += gr * d1; Gi[1] += gi * d1;
                         Gr[2] += gr * d2; Gi[2] += gi * d2;
                     }
-                    if (FP && !use32)
+                    if (PH == 1 && !use32)
                         for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
+                    if (PH == 2 && !use32)
+                        for (int g = 0; g < 3; g++) rot_c(Gr[g], Gi[g], pc);
                     const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
                     for (int b = 0; b < 2; b++) {
                         const double* tb = rec + 4 + 3 * b;
@@ -263,13 +275,13 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
     }
 }
 
-template <bool CK, int WHICH, bool FP, bool P32>
+template <bool CK, int WHICH, int PH, bool P32>
 __global__ __launch_bounds__(kAT) void k_assemble_far(AssembleArgs a, int cells_per_split) {
     __shared__ double4 ssurf[kASurf];
     __shared__ double4 sseg[kASeg];
     __shared__ double scell[kACells][kARec];
     const int c_begin = blockIdx.y * cells_per_split;
-    assemble_tile<CK, WHICH, FP, P32>(a, blockIdx.x, c_begin, min(a.N, c_begin + cells_per_split), ssurf, sseg,
+    assemble_tile<CK, WHICH, PH, P32>(a, blockIdx.x, c_begin, min(a.N, c_begin + cells_per_split), ssurf, sseg,
                                       scell);
 }
 
@@ -285,8 +297,8 @@ __global__ __launch_bounds__(kAT) void k_assemble_multi(const AssembleArgs* __re
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 tl = tiles[t];
         const AssembleArgs a = items[tl.x];
-        if (a.fast_phase) assemble_tile<CK, WHICH, true, P32>(a, tl.y, tl.z, tl.w, ssurf, sseg, scell);
-        else assemble_tile<CK, WHICH, false, P32>(a, tl.y, tl.z, tl.w, ssurf, sseg, scell);
+        if (a.fast_phase) assemble_tile<CK, WHICH, 1, P32>(a, tl.y, tl.z, tl.w, ssurf, sseg, scell);
+        else assemble_tile<CK, WHICH, 0, P32>(a, tl.y, tl.z, tl.w, ssurf, sseg, scell);
     }
 }
 
@@ -312,16 +324,21 @@ hipError_t launch_assemble_multi(hipStream_t s, const AssembleArgs* items_d, con
     return hipGetLastError();
 }
 
-template <bool CK, int W, bool P32>
-static void assemble_far_p(hipStream_t s, const AssembleArgs& a, int s_min) {
+template <bool CK, int W, int PH, bool P32>
+static void assemble_far_ph(hipStream_t s, const AssembleArgs& a, int s_min) {
     const int bx = (a.ni + kAT - 1) / kAT;
-    const void* kern = a.fast_phase ? reinterpret_cast<const void*>(&k_assemble_far<CK, W, true, P32>)
-                                    : reinterpret_cast<const void*>(&k_assemble_far<CK, W, false, P32>);
+    const void* kern = reinterpret_cast<const void*>(&k_assemble_far<CK, W, PH, P32>);
     const int nsplit = tail_aware_splits(bx, s_min, (a.N + 63) / 64, resident_slots(kern, kAT));
     int cps = (a.N + nsplit - 1) / nsplit;
     dim3 g(bx, nsplit), b(kAT);
-    if (a.fast_phase) hipLaunchKernelGGL((k_assemble_far<CK, W, true, P32>), g, b, 0, s, a, cps);
-    else hipLaunchKernelGGL((k_assemble_far<CK, W, false, P32>), g, b, 0, s, a, cps);
+    hipLaunchKernelGGL((k_assemble_far<CK, W, PH, P32>), g, b, 0, s, a, cps);
+}
+
+template <bool CK, int W, bool P32>
+static void assemble_far_p(hipStream_t s, const AssembleArgs& a, int s_min) {
+    if (a.fast_phase == 2) assemble_far_ph<CK, W, 2, P32>(s, a, s_min);
+    else if (a.fast_phase) assemble_far_ph<CK, W, 1, P32>(s, a, s_min);
+    else assemble_far_ph<CK, W, 0, P32>(s, a, s_min);
 }
 
 template <bool CK, int W>
@@ -332,6 +349,11 @@ static void assemble_far_t(hipStream_t s, const AssembleArgs& a, int nsplit) {
 
 bool fast_phase_ok(double k_re, double grid_step) {
     return 0.7853981633974483 + fabs(k_re) * grid_step <= kPhaseRange;
+}
+
+int phase_mode(double k_re, double grid_step, double max_rad) {
+    if (fabs(k_re) * max_rad <= kShortRange) return 2;
+    return fast_phase_ok(k_re, grid_step) ? 1 : 0;
 }
 
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a) {

@@ -27,6 +27,7 @@ enum : int {
 struct DevMesh {
     int n = 0, nv = 0;
     double grid_step = 0.;   // max cell diameter (SurfMesh::GetGridStep, mesh/surf_mesh.h:322-328)
+    double max_rad = 0.;     // max distance from a cell's centre to its vertices (bounds node offsets)
     double* verts = nullptr; // [n][nv][3]
     double* ctab = nullptr;  // [n][CT]
     double* nodes = nullptr; // [n][nn][4]: sub-cell centroid (3) + weight
@@ -71,7 +72,7 @@ struct FieldArgs {
     double ac, thresh;
     int nsplit;
     double* part;           // [nsplit][nrhs][NACC][M]
-    int fast_phase;         // node phases within kPhaseRange of the cell-centre quadrant (phase.h)
+    int fast_phase;         // node phase mode: 0 general sincos, 1 quadrant reduction, 2 centre-phase (phase.h)
     int prec;               // 0: FP64 node sums; 1: FP32 cell-local node sums
 };
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a);
@@ -210,6 +211,9 @@ struct AssembleArgs {
 
 // pi/4 + |k| * (max cell diameter) bounds every reduced node phase (phase.h)
 bool fast_phase_ok(double k_re, double grid_step);
+// node phase mode of the tile kernels: 2 (centre-phase reduction, short series) when
+// |k| max_rad <= 0.5, else 1 (quadrant reduction) when fast_phase_ok, else 0
+int phase_mode(double k_re, double grid_step, double max_rad);
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a);
 // Persistent far-pair kernel over several meshes: items_d[tile.x] with point
 // block tile.y (256 collocation points) and cells [tile.z, tile.w).  All

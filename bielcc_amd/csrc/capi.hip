@@ -125,9 +125,20 @@ int bielcc_mesh_create(bielcc_ctx ctx, int n_cells, int nv, const double* verts,
     std::vector<double> tab((size_t)n_cells * CT);
     HIPCHK(hipMemcpyAsync(tab.data(), m.ctab, tab.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    double gs = 0.;
-    for (int c = 0; c < n_cells; c++) gs = std::max(gs, tab[(size_t)c * CT + CT_DIAM]);
+    double gs = 0., mr = 0.;
+    for (int c = 0; c < n_cells; c++) {
+        gs = std::max(gs, tab[(size_t)c * CT + CT_DIAM]);
+        for (int v = 0; v < nv; v++) {   // quadrature nodes lie in the cell: within max_rad of its centre
+            double d2 = 0.;
+            for (int d = 0; d < 3; d++) {
+                const double x = verts[((size_t)c * nv + v) * 3 + d] - tab[(size_t)c * CT + CT_CENTER + d];
+                d2 += x * x;
+            }
+            mr = std::max(mr, std::sqrt(d2));
+        }
+    }
     m.grid_step = gs;
+    m.max_rad = mr;
     int id = ctx->next_id++;
     ctx->meshes[id] = m;
     *mesh_id = id;
@@ -335,7 +346,7 @@ static int assemble_impl(bielcc_ctx ctx, int mesh_id, int which, const bielcc_nu
     a.nv = m->nv; a.N = m->n; a.nn = nn; a.ns = np->n_start_seg; a.which = which;
     a.i0 = i0; a.ni = ni; a.ctab = m->ctab; a.nodes = m->nodes; a.segs = m->segs;
     a.k_re = k_re; a.k_im = k_im; a.ac = np->analytic_calc; a.Z = Z_d; a.ld = ld;
-    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.fast_phase = ctx->no_fast_phase ? 0 : phase_mode(k_re, m->grid_step, m->max_rad);
     a.prec = precision;
     a.nodes32 = m->nodes32;
     a.nodes32_yy = m->nodes32_yy;
@@ -511,7 +522,7 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     a.jE = jE_d; a.jM = jM_d; a.nrhs = nrhs; a.rchunk = ctx->field_rchunk;
     a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
     a.thresh = thresh;
-    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.fast_phase = ctx->no_fast_phase ? 0 : phase_mode(k_re, m->grid_step, m->max_rad);
     a.prec = precision;
     const int nsplit = field_far_splits(a, std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64)));
     size_t part_b = align256((size_t)nsplit * nrhs * nacc * M * sizeof(double));

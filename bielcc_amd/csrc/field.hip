@@ -117,12 +117,16 @@ __device__ inline void write_partials(const FieldArgs& a, int r0, int pt, const 
 // Node sums of one far (point, cell) pair in FP64: A = sum w e f1,
 // T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh, G = sum w (ikr-1) e d / r^3
 // (e = e^{ikr}; unscaled, without 1/4pi).  nd: the cell's nn nodes.
-template <bool CK, bool FP>
+// PH: node phase evaluation, 0 general sincos, 1 quadrant reduction, 2 centre-phase
+// reduction with the short series (phase.h)
+template <bool CK, int PH>
 __device__ __forceinline__ void pair_sums64(double x0, double x1, double x2, const double4* nd, int nn, double dcen,
                                             double kr_, double ki_, double k2r, double k2i, cplx& A, cplx (&T)[6],
                                             cplx (&G)[3]) {
     PhaseRef pr{0., 0., 0., 0};
-    if (FP) pr = phase_ref(kr_ * dcen);
+    PhaseC pc{0., 0., 1., 0.};
+    if (PH == 1) pr = phase_ref(kr_ * dcen);
+    if (PH == 2) pc = phase_centre(kr_, dcen);
     // A = sum w e f1 ; T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh ; G = sum w (ikr-1) e d / r^3
     double Ar = 0., Ai = 0.;
     double Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
@@ -135,8 +139,7 @@ __device__ __forceinline__ void pair_sums64(double x0, double x1, double x2, con
         const double r = r2 * ri;
         const double ri2 = ri * ri, ri3 = ri2 * ri;
         double s, c;
-        if (FP) sincos_near(kr_, r, pr, s, c);     // e^{i(kr - n pi/2)}
-        else sincos(kr_ * r, &s, &c);
+        phase_eval<PH>(kr_, r, pr, pc, s, c);     // e^{ikr}, up to the pair's common factor
         double w = y.w;
         if (CK) w *= exp(-ki_ * r);
         const double wc = w * c, ws = w * s;
@@ -167,10 +170,14 @@ __device__ __forceinline__ void pair_sums64(double x0, double x1, double x2, con
         Gr[1] += gr * d1; Gi[1] += gi * d1;
         Gr[2] += gr * d2; Gi[2] += gi * d2;
     }
-    if (FP) {   // restore the common factor i^n of the cell's e^{ikr}
+    if (PH == 1) {   // restore the common factor i^n of the cell's e^{ikr}
         rot_q(Ar, Ai, pr.q);
         for (int t = 0; t < 6; t++) rot_q(Tr[t], Ti[t], pr.q);
         for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
+    } else if (PH == 2) {   // ... or e^{i k dcen}
+        rot_c(Ar, Ai, pc);
+        for (int t = 0; t < 6; t++) rot_c(Tr[t], Ti[t], pc);
+        for (int g = 0; g < 3; g++) rot_c(Gr[g], Gi[g], pc);
     }
     A = mk(Ar, Ai);
     for (int t = 0; t < 6; t++) T[t] = mk(Tr[t], Ti[t]);
@@ -260,7 +267,7 @@ __device__ __forceinline__ void pair_sums32(double x0, double x1, double x2, con
     for (int g = 0; g < 3; g++) G[g] = mk(GrD[g], GiD[g]);
 }
 
-template <bool CK, bool JM, bool FP, int R>
+template <bool CK, bool JM, int PH, int R>
 __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_split, int r0) {
     __shared__ double4 snode[kFNodes];
     __shared__ double scell[kFCells][kCellRec];
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(kFT) void k_field_far(FieldArgs a, int cells_per_sp
             const double dcen = center_dist(x0, x1, x2, rec);
             if (near_rule(dcen, rec[3], a.mode, a.ac, a.thresh)) continue;
             cplx A, T[6], G[3];
-            pair_sums64<CK, FP>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
+            pair_sums64<CK, PH>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
             far_contract<JM, R>(rec, A, T, G, dcen < kMachineZero, E, H, KH, RE);
         }
     }
@@ -345,31 +352,34 @@ __global__ __launch_bounds__(kFT) void k_field_far32(FieldArgs a, int cells_per_
     write_partials<JM, R>(a, r0, pt, E, H, KH, RE);
 }
 
-template <bool CK, bool JM, bool FP, int R>
+// FP32 node sums take the quadrant reduction in phase mode 2 as well (phase.h: mode 2
+// implies the quadrant range)
+template <bool CK, bool JM, int PH, int R>
 static void field_far_t(hipStream_t s, const FieldArgs& a, int r0) {
     int cps = (a.N + a.nsplit - 1) / a.nsplit;
     dim3 g((a.M + kFT - 1) / kFT, a.nsplit), b(kFT);
-    if (a.prec) hipLaunchKernelGGL((k_field_far32<CK, JM, FP, R>), g, b, 0, s, a, cps, r0);
-    else hipLaunchKernelGGL((k_field_far<CK, JM, FP, R>), g, b, 0, s, a, cps, r0);
+    if (a.prec) hipLaunchKernelGGL((k_field_far32<CK, JM, PH != 0, R>), g, b, 0, s, a, cps, r0);
+    else hipLaunchKernelGGL((k_field_far<CK, JM, PH, R>), g, b, 0, s, a, cps, r0);
 }
 
 // current sets in passes of up to rchunk (4, 2 or 1; 1 with a magnetic current)
-template <bool CK, bool JM, bool FP>
+template <bool CK, bool JM, int PH>
 static void field_far_r(hipStream_t s, const FieldArgs& a) {
     int r0 = 0;
     if (!JM) {
         if (a.rchunk >= 4)
-            for (; r0 + 4 <= a.nrhs; r0 += 4) field_far_t<CK, JM, FP, 4>(s, a, r0);
+            for (; r0 + 4 <= a.nrhs; r0 += 4) field_far_t<CK, JM, PH, 4>(s, a, r0);
         if (a.rchunk >= 2)
-            for (; r0 + 2 <= a.nrhs; r0 += 2) field_far_t<CK, JM, FP, 2>(s, a, r0);
+            for (; r0 + 2 <= a.nrhs; r0 += 2) field_far_t<CK, JM, PH, 2>(s, a, r0);
     }
-    for (; r0 < a.nrhs; r0++) field_far_t<CK, JM, FP, 1>(s, a, r0);
+    for (; r0 < a.nrhs; r0++) field_far_t<CK, JM, PH, 1>(s, a, r0);
 }
 
 template <bool CK, bool JM>
 static void field_far_fp(hipStream_t s, const FieldArgs& a) {
-    if (a.fast_phase) field_far_r<CK, JM, true>(s, a);
-    else field_far_r<CK, JM, false>(s, a);
+    if (a.fast_phase == 2) field_far_r<CK, JM, 2>(s, a);
+    else if (a.fast_phase) field_far_r<CK, JM, 1>(s, a);
+    else field_far_r<CK, JM, 0>(s, a);
 }
 
 int resident_slots(const void* kernel, int block) {
@@ -380,26 +390,31 @@ int resident_slots(const void* kernel, int block) {
     return ncu * nb;
 }
 
-template <bool CK, bool JM, bool FP, int R>
+template <bool CK, bool JM, int PH, int R>
 static const void* field_far_kernel(bool p32) {
-    return p32 ? reinterpret_cast<const void*>(&k_field_far32<CK, JM, FP, R>)
-               : reinterpret_cast<const void*>(&k_field_far<CK, JM, FP, R>);
+    return p32 ? reinterpret_cast<const void*>(&k_field_far32<CK, JM, PH != 0, R>)
+               : reinterpret_cast<const void*>(&k_field_far<CK, JM, PH, R>);
 }
 
-template <bool CK, bool JM, bool FP>
+template <bool CK, bool JM, int PH>
 static const void* field_far_kernel_r(const FieldArgs& a) {
     // the instance of the first pass (field_far_r)
-    if (!JM && a.rchunk >= 4 && a.nrhs >= 4) return field_far_kernel<CK, JM, FP, 4>(a.prec);
-    if (!JM && a.rchunk >= 2 && a.nrhs >= 2) return field_far_kernel<CK, JM, FP, 2>(a.prec);
-    return field_far_kernel<CK, JM, FP, 1>(a.prec);
+    if (!JM && a.rchunk >= 4 && a.nrhs >= 4) return field_far_kernel<CK, JM, PH, 4>(a.prec);
+    if (!JM && a.rchunk >= 2 && a.nrhs >= 2) return field_far_kernel<CK, JM, PH, 2>(a.prec);
+    return field_far_kernel<CK, JM, PH, 1>(a.prec);
+}
+
+template <bool CK, bool JM>
+static const void* field_far_kernel_ph(const FieldArgs& a) {
+    if (a.fast_phase == 2) return field_far_kernel_r<CK, JM, 2>(a);
+    if (a.fast_phase) return field_far_kernel_r<CK, JM, 1>(a);
+    return field_far_kernel_r<CK, JM, 0>(a);
 }
 
 int field_far_splits(const FieldArgs& a, int s_min) {
-    const bool ck = a.k_im != 0., jm = a.jM != nullptr, fp = a.fast_phase;
-    const void* k = ck ? (jm ? (fp ? field_far_kernel_r<true, true, true>(a) : field_far_kernel_r<true, true, false>(a))
-                             : (fp ? field_far_kernel_r<true, false, true>(a) : field_far_kernel_r<true, false, false>(a)))
-                       : (jm ? (fp ? field_far_kernel_r<false, true, true>(a) : field_far_kernel_r<false, true, false>(a))
-                             : (fp ? field_far_kernel_r<false, false, true>(a) : field_far_kernel_r<false, false, false>(a)));
+    const bool ck = a.k_im != 0., jm = a.jM != nullptr;
+    const void* k = ck ? (jm ? field_far_kernel_ph<true, true>(a) : field_far_kernel_ph<true, false>(a))
+                       : (jm ? field_far_kernel_ph<false, true>(a) : field_far_kernel_ph<false, false>(a));
     const int bx = (a.M + kFT - 1) / kFT;
     return tail_aware_splits(bx, s_min, (a.N + 63) / 64, resident_slots(k, kFT));
 }
@@ -924,9 +939,9 @@ __global__ __launch_bounds__(kFT) void k_field_opmat(const OpSeg* __restrict__ s
                         pair_sums32<CK, false>(x0, x1, x2, rec, snode32 + cc * nn, syy + cc * nn, nn, dcen, kr_, ki_, A, T, G);
                 } else {
                     if (sg.fast_phase)
-                        pair_sums64<CK, true>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
+                        pair_sums64<CK, 1>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
                     else
-                        pair_sums64<CK, false>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
+                        pair_sums64<CK, 0>(x0, x1, x2, snode + cc * nn, nn, dcen, kr_, ki_, k2r, k2i, A, T, G);
                 }
                 const int c = cs + cc;
                 // K = (A I + T)/4pi, T = (xx, yy, zz, xy, xz, yz)

@@ -84,7 +84,7 @@ int bielcc_field_noalloc(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np
     a.jE = jE_d; a.jM = jM_d; a.nrhs = 1; a.rchunk = ctx->field_rchunk;
     a.k_re = k_re; a.k_im = k_im; a.mode = near_mode; a.ac = np->analytic_calc;
     a.thresh = thresh;
-    a.fast_phase = fast_phase_ok(k_re, m->grid_step) && !ctx->no_fast_phase;
+    a.fast_phase = ctx->no_fast_phase ? 0 : phase_mode(k_re, m->grid_step, m->max_rad);
     a.prec = BIELCC_PREC_FP64;
     int nsplit = field_far_splits(a, std::max(1, std::min((ctx->field_wg + bx - 1) / bx, (m->n + 63) / 64)));
     const size_t cnt_b = align256((size_t)M * sizeof(int));

@@ -29,6 +29,14 @@ struct PhaseRef {
     int q;
 };
 
+// (re, im) * i^q
+__device__ inline void rot_q(double& re, double& im, int q) {
+    double a = re, b = im;
+    if (q == 1) { re = -b; im = a; }
+    else if (q == 2) { re = -a; im = -b; }
+    else if (q == 3) { re = b; im = -a; }
+}
+
 __device__ inline PhaseRef phase_ref(double phi_c) {
     const double n = rint(phi_c * kTwoOverPi);
     return PhaseRef{n * kPio2_1, n * kPio2_2, n * kPio2_3, static_cast<int>(n) & 3};
@@ -67,6 +75,73 @@ __device__ inline void sincos_near(double kr, double r, const PhaseRef& p, doubl
     c = fma(pc, t2, 1.0);
 }
 
+// Centre-phase reference (phase mode 2).  Per (point, cell) pair the exact
+// product k dcen is split as hi + lo (lo = fma(k, dcen, -hi)) and e^{i k dcen} is
+// evaluated once (quadrant reduction + the series above); per node the phase is
+// reduced against it, t = k r - k dcen = fma(k, r, -hi) - lo, so |t| <= |k| |r - dcen|
+// <= |k| max |y - c| (every node y lies within that distance of the centre c).
+// The host picks this mode when |k| max|y - c| <= kShortRange = 0.5, where the
+// Taylor series of sin through t^15 and of cos through t^14 truncate below 2e-20
+// and 8e-19 (sin: 7 terms instead of 11, cos: 7 instead of 11); the accumulated
+// sums, linear in e^{ikr}, are multiplied by e^{i k dcen} once per pair (rot_c).
+constexpr double kShortRange = 0.5;
+
+struct PhaseC {
+    double hi, lo;   // k dcen = hi + lo exactly
+    double c, s;     // e^{i k dcen}
+};
+
+__device__ inline PhaseC phase_centre(double kr, double dcen) {
+    PhaseC p;
+    p.hi = kr * dcen;
+    p.lo = fma(kr, dcen, -p.hi);
+    const PhaseRef q = phase_ref(p.hi);
+    double s, c;
+    sincos_near(kr, dcen, q, s, c);   // sin, cos of k dcen - n pi/2 from the exact product
+    rot_q(c, s, q.q);
+    p.c = c;
+    p.s = s;
+    return p;
+}
+
+// sin and cos of (kr * r - k dcen) on |t| <= kShortRange
+__device__ inline void sincos_short(double kr, double r, const PhaseC& p, double& s, double& c) {
+    const double t = fma(kr, r, -p.hi) - p.lo;
+    const double t2 = t * t;
+    double ps = -7.6471637318198164e-13;        // -1/15!
+    ps = fma(ps, t2, 1.6059043836821613e-10);    // 1/13!
+    ps = fma(ps, t2, -2.505210838544172e-08);    // -1/11!
+    ps = fma(ps, t2, 2.7557319223985893e-06);    // 1/9!
+    ps = fma(ps, t2, -0.00019841269841269841);   // -1/7!
+    ps = fma(ps, t2, 0.0083333333333333332);     // 1/5!
+    ps = fma(ps, t2, -0.16666666666666666);      // -1/3!
+    ps = ps * t2;
+    s = fma(t, ps, t);
+    double pc = -1.1470745597729725e-11;         // -1/14!
+    pc = fma(pc, t2, 2.08767569878681e-09);      // 1/12!
+    pc = fma(pc, t2, -2.7557319223985888e-07);   // -1/10!
+    pc = fma(pc, t2, 2.4801587301587302e-05);    // 1/8!
+    pc = fma(pc, t2, -0.0013888888888888889);    // -1/6!
+    pc = fma(pc, t2, 0.041666666666666664);      // 1/4!
+    pc = fma(pc, t2, -0.5);
+    c = fma(pc, t2, 1.0);
+}
+
+// (re, im) * e^{i k dcen}
+__device__ inline void rot_c(double& re, double& im, const PhaseC& p) {
+    const double a = re, b = im;
+    re = a * p.c - b * p.s;
+    im = a * p.s + b * p.c;
+}
+
+// e^{ikr} at one node by phase mode PH (0: general sincos, 1: pr, 2: pc)
+template <int PH>
+__device__ inline void phase_eval(double kr, double r, const PhaseRef& pr, const PhaseC& pc, double& s, double& c) {
+    if (PH == 1) sincos_near(kr, r, pr, s, c);
+    else if (PH == 2) sincos_short(kr, r, pc, s, c);
+    else sincos(kr * r, &s, &c);
+}
+
 // sin t and cos t on |t| <= kPhaseRange in FP32.  One more quadrant step in
 // FP32 (m = rint(2t/pi) in {-1, 0, 1}; t - m pi/2 with a two-piece pi/2, the
 // first subtraction exact by Sterbenz) leaves |t'| <= pi/4, where Taylor
@@ -97,12 +172,5 @@ __device__ inline void sincos_near32(float t, float& s, float& c) {
     c = rot ? -m * s1 : c1;
 }
 
-// (re, im) * i^q
-__device__ inline void rot_q(double& re, double& im, int q) {
-    double a = re, b = im;
-    if (q == 1) { re = -b; im = a; }
-    else if (q == 2) { re = -a; im = -b; }
-    else if (q == 3) { re = b; im = -a; }
-}
 
 }  // namespace bielcc_impl
