@@ -41,7 +41,7 @@ FLOPS_ASM_PAIR = 10256        # K_RotRot_HS_Colloc block, far quad pair
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector (dense) peak
 # HBM bytes moved per k_field_far launch on sphere20k: rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
 # (scale factors from a 1 GiB copy / fill in the same run); the kernel is VALU-bound, the bytes are L2 re-reads
-FIELD_TRAFFIC_BYTES = {"sphere20k": 1.002e9}
+FIELD_TRAFFIC_BYTES = {"sphere20k": 1.0565e9}
 
 
 def log(*a):
