@@ -112,6 +112,96 @@ def cpu_baseline(verts, pts, k, budget_s=20.0):
             "field_pairs_per_s": m_f * n / tf, "assembly_pairs_per_s": m_a * n / ta}
 
 
+def c1_full():
+    """The 512-cell sphere (ka = 2, 1,000 points at r = 3) end to end on the host
+    cores through the compiled reference's own routines -- every matrix entry from
+    K_RotRot_HS_Colloc (collocation point x cell x basis vector), the right-hand side
+    from PlaneWaveFieldE, SurfaceCurrents_byBasis, GetFieldE_Ideal for E and
+    R_Rot_Colloc for H; the solve by LAPACK (numpy), which the reference lacks --
+    timed phase by phase, beside the same chain on the GPU (the library's assembly,
+    LU, currents and field drivers), with the GPU's currents and total E against
+    the host chain's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import bielcc_amd as B
+    kind = "reference"
+    try:
+        chk = O.Checker("ref")
+    except (FileNotFoundError, OSError):
+        chk, kind = O.Checker("oracle"), "port"
+    w = workload("sphere512")
+    verts, k, pts = w["verts"], w["k"], w["pts"]
+    n, M = verts.shape[0], pts.shape[0]
+    n2 = 2 * n
+    kvec, e0 = np.array([0.0, 0.0, k]), np.array([1.0, 0.0, 0.0])
+    npp = O.num_param()
+    cpu = {}
+    t0 = time.perf_counter()
+    tabs = [chk.cellfill(v) for v in verts]
+    rkt = np.array([t["rkt"] for t in tabs])
+    tau = np.array([[t["tau1"], t["tau2"]] for t in tabs])
+    Z = np.empty((n2, n2), complex)
+    for b in range(2):
+        Ob = chk.op_pairs(2, verts, tau[:, b, :], rkt, npp, k)    # [point i][cell j][3]
+        for a in range(2):
+            Z[a::2, b::2] = np.einsum("ijg,ig->ij", Ob, tau[:, a, :])
+    einc = chk.plane_wave_e(rkt, kvec, e0)
+    rhs = -np.einsum("iag,ig->ia", tau, einc).reshape(n2)
+    cpu["assemble"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = np.linalg.solve(Z, rhs)
+    cpu["solve"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    j = chk.ref_currents_by_basis(tau, b) if kind == "reference" else chk.currents_by_basis(verts, b)
+    cpu["currents"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E = chk.get_field_e(verts, j, 1.0, kvec, e0, k, npp, pts)
+    cpu["field_e"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    chk.op_pairs(6, verts, j, pts, npp, k).sum(axis=1)
+    cpu["field_h"] = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+
+    ctx = B.Context(0)
+    gnp = B.num_param()
+    gpu = {}
+    m = ctx.mesh(verts)
+    dZ, dpiv, dB, dJ = ctx.empty(n2 * n2 * 16), ctx.empty(n2 * 4), ctx.empty(n2 * 16), ctx.empty(n * 48)
+    dP, dE, dH = ctx.to_device(pts), ctx.empty(M * 48), ctx.empty(M * 48)
+    for rep in range(2):    # the first pass builds tables and grows work buffers
+        t0 = time.perf_counter()
+        m.assemble_into(dZ, n2, gnp, k)
+        m.rhs_planewave_into(dB, n2, kvec, e0)
+        ctx.sync()
+        gpu["assemble"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.lu_factor(dZ, n2, n2, dpiv)
+        ctx.lu_solve(dZ, n2, n2, dpiv, dB, 1, n2)
+        ctx.sync()
+        gpu["solve"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        m.currents_into(dB, n2, 1, dJ)
+        ctx.sync()
+        gpu["currents"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        m.field_into(gnp, k, dJ, dP, M, dE, dH)
+        ctx.sync()
+        gpu["field_e_h"] = time.perf_counter() - t0
+    jg = dJ.to_numpy((n, 3), np.complex128)
+    Eg = m.get_field_e(jg, 1.0, kvec, e0, gnp, k, pts)
+    m.close()
+    ctx.close()
+    out = {"workload": w["desc"] + ", plane wave k = 2 z, E along x", "cpu_kind": kind, "cpu_cores": threads,
+           "cpu_seconds": {k_: round(v, 4) for k_, v in cpu.items()}, "cpu_total_s": round(sum(cpu.values()), 4),
+           "gpu_seconds": {k_: round(v, 5) for k_, v in gpu.items()}, "gpu_total_s": round(sum(gpu.values()), 5),
+           "currents_rel_err": float(np.linalg.norm(jg - j) / np.linalg.norm(j)),
+           "E_rel_err": float(np.linalg.norm(Eg - E) / np.linalg.norm(E))}
+    out["speedup"] = round(out["cpu_total_s"] / out["gpu_total_s"], 1)
+    log(f"[bench] C1 in full: CPU ({kind}, {threads} threads) {out['cpu_total_s']:.2f} s, GPU "
+        f"{out['gpu_total_s'] * 1e3:.1f} ms; currents {out['currents_rel_err']:.1e}, E {out['E_rel_err']:.1e}")
+    return out
+
+
 def far_directions(n_theta=181, n_phi=360):
     """Unit directions on the 1-degree theta x phi grid (theta 0..180, phi 0..359)."""
     th = np.deg2rad(np.arange(n_theta))[:, None]
@@ -1169,6 +1259,8 @@ def main():
                     help="batched form: plan kernels only, on synthetic currents (counter runs)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, tune afresh)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--c1-only", action="store_true",
+                    help="run only the 512-cell case in full on the host cores and on the GPU")
     ap.add_argument("--lu-only", type=int, default=0, metavar="N",
                     help="time only the FP64 and FP32 LU of a seeded random complex N x N matrix")
     ap.add_argument("--solve-only", type=int, default=0, metavar="N",
@@ -1207,6 +1299,9 @@ def main():
         return
     if args.cube_only:
         print(json.dumps({"precision_form_cube": cube_precision()}), flush=True)
+        return
+    if args.c1_only:
+        print(json.dumps({"c1_full": c1_full()}), flush=True)
         return
     if args.lu_only:
         print(json.dumps({"lu": lu_only(args.lu_only)}), flush=True)
@@ -1320,6 +1415,7 @@ def main():
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(verts, pts, k, args.cpu_budget)
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} pairs/s")
+        out["cpu_baseline"]["c1_full"] = c1_full()
     print(json.dumps(out), flush=True)
 
 

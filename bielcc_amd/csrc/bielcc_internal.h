@@ -37,6 +37,13 @@ struct DevMesh {
     float4* nodes32 = nullptr;   // [n][nn]: node - centre (3), weight
     float* nodes32_yy = nullptr; // [n][nn]: |node - centre|^2
     int nodes32_n = 0;
+    // near pairs of the last assembly (collocation points i0 .. i0 + ni - 1 against every
+    // cell, rule analytic_calc): reused by the next assembly of the same rows and rule
+    int* near_lq = nullptr;      // [near_tot] point index relative to i0
+    int* near_lc = nullptr;      // [near_tot] cell
+    long long near_tot = -1;     // -1: no list kept
+    int near_i0 = -1, near_ni = -1;
+    double near_ac = 0.;
 };
 
 // ---- tables (tables.hip)

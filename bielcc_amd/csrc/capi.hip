@@ -357,23 +357,43 @@ static int assemble_impl(bielcc_ctx ctx, int mesh_id, int which, const bielcc_nu
         t.stop();
     }
     if (which == 0) {
-        // near pairs (analytic branch of IntegralSurf_SimplePot_H_HS): exact per-pair kernel
-        size_t need = align256((size_t)ni * sizeof(int)) + align256((size_t)(ni + 1) * sizeof(long long)) + 256;
-        HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, need));
-        char* base = reinterpret_cast<char*>(ctx->scratch);
-        int* counts = reinterpret_cast<int*>(base);
-        long long* offs = reinterpret_cast<long long*>(base + align256((size_t)ni * sizeof(int)));
-        long long* tot_d = reinterpret_cast<long long*>(base + need - 256);
-        long long tot_h = 0;
-        int *lq = nullptr, *lc = nullptr;
-        const double* q = m->ctab + (size_t)i0 * CT + CT_RKT;
-        if (int rc = build_near_list(ctx, ni, q, CT, *m, 0, np->analytic_calc, 0., counts, offs, tot_d, &tot_h, &lq, &lc))
-            return rc;
-        if (tot_h > 0) {
+        // near pairs (analytic branch of IntegralSurf_SimplePot_H_HS): exact per-pair kernel.
+        // The list depends on the mesh, the rows and the rule only: the mesh keeps the last one.
+        if (!(m->near_tot >= 0 && m->near_i0 == i0 && m->near_ni == ni && m->near_ac == np->analytic_calc)) {
+            size_t need = align256((size_t)ni * sizeof(int)) + align256((size_t)(ni + 1) * sizeof(long long)) + 256;
+            HIPCHK(ensure(ctx->scratch, ctx->scratch_cap, need));
+            char* base = reinterpret_cast<char*>(ctx->scratch);
+            int* counts = reinterpret_cast<int*>(base);
+            long long* offs = reinterpret_cast<long long*>(base + align256((size_t)ni * sizeof(int)));
+            long long* tot_d = reinterpret_cast<long long*>(base + need - 256);
+            long long tot_h = 0;
+            int *lq = nullptr, *lc = nullptr;
+            const double* q = m->ctab + (size_t)i0 * CT + CT_RKT;
+            if (int rc = build_near_list(ctx, ni, q, CT, *m, 0, np->analytic_calc, 0., counts, offs, tot_d, &tot_h,
+                                         &lq, &lc))
+                return rc;
+            (void)hipFree(m->near_lq);
+            (void)hipFree(m->near_lc);
+            m->near_lq = m->near_lc = nullptr;
+            m->near_tot = -1;
+            if (tot_h > 0) {
+                HIPCHK(hipMalloc(&m->near_lq, (size_t)tot_h * sizeof(int)));
+                HIPCHK(hipMalloc(&m->near_lc, (size_t)tot_h * sizeof(int)));
+                HIPCHK(hipMemcpyAsync(m->near_lq, lq, (size_t)tot_h * sizeof(int), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+                HIPCHK(hipMemcpyAsync(m->near_lc, lc, (size_t)tot_h * sizeof(int), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            }
+            m->near_tot = tot_h;
+            m->near_i0 = i0;
+            m->near_ni = ni;
+            m->near_ac = np->analytic_calc;
+        }
+        if (m->near_tot > 0) {
             // the list holds point indices local to i0; the kernel adds i0
             Timer t(ctx, "near");
-            HIPCHK(launch_assemble_generic(ctx->stream, m->nv, 0, tot_h, lq, lc, m->n, i0, ni, m->verts, m->ctab,
-                                           npp, k_re, k_im, Z_d, ld));
+            HIPCHK(launch_assemble_generic(ctx->stream, m->nv, 0, m->near_tot, m->near_lq, m->near_lc, m->n, i0, ni,
+                                           m->verts, m->ctab, npp, k_re, k_im, Z_d, ld));
             t.stop();
         }
     }
@@ -874,8 +894,10 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "opmat_min_rhs") *value = ctx->opmat_min_rhs;
     else if (k == "n_cu") *value = ctx->n_cu;
     else if (k == "scratch_kb") {   // the context's grow-only device buffers, KiB rounded up
-        const size_t b = ctx->scratch_cap + ctx->lists_cap + ctx->near_cap + ctx->opmat_ws_cap +
-                         ctx->opmat_meta_cap + ctx->opmat_part_cap + ctx->small_cap + ctx->fix_cap + lu_work_bytes(ctx->lu);
+        size_t b = ctx->scratch_cap + ctx->lists_cap + ctx->near_cap + ctx->opmat_ws_cap + ctx->opmat_meta_cap +
+                   ctx->opmat_part_cap + ctx->small_cap + ctx->fix_cap + lu_work_bytes(ctx->lu);
+        for (const auto& kv : ctx->meshes)   // the meshes' kept near-pair lists
+            if (kv.second.near_tot > 0) b += (size_t)kv.second.near_tot * 2 * sizeof(int);
         *value = (int)std::min<size_t>((b + 1023) >> 10, (size_t)INT_MAX);
     }
     else return fail(ctx, BIELCC_ERR_ARG, "unknown tuning key " + k);

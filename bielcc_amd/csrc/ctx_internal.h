@@ -158,6 +158,7 @@ inline int ensure_tables(bielcc_ctx ctx, DevMesh& m, int n_start, int n_seg) {
 inline void free_mesh(DevMesh& m) {
     (void)hipFree(m.verts); (void)hipFree(m.ctab); (void)hipFree(m.nodes); (void)hipFree(m.segs);
     (void)hipFree(m.nodes32); (void)hipFree(m.nodes32_yy);
+    (void)hipFree(m.near_lq); (void)hipFree(m.near_lc);
 }
 
 // cell-local FP32 surface nodes for the mixed-precision kernels

@@ -230,7 +230,10 @@ int bielcc_rwg_write_vtk(bielcc_rwg r, const double* cell_normals, const char* p
  *   Z[2i+a, 2j+b] = tau_i^a . Op(tau_j^b, rkt_i, cell_j)
  * which 0: Op = K_RotRot_HS_Colloc, 1: Op = R_Rot_Colloc.  Rows row0 ..
  * row0+nrows-1 (both even) of the 2N x 2N matrix go to Z_d rows 0..nrows-1,
- * column-major complex with leading dimension ld (in complex elements). */
+ * column-major complex with leading dimension ld (in complex elements).  The
+ * list of near pairs (written by the exact per-pair kernel) is built on the
+ * first call for a row range and kept with the mesh for the next call with the
+ * same rows and analytic_calc. */
 int bielcc_assemble(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_param* np, double k_re, double k_im,
                     int row0, int nrows, double* Z_d, int64_t ld);
 
@@ -464,8 +467,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * redone on the blocked path because a wait expired, or because the ticket
  * counter was out of step), and "scratch_kb" (device bytes the context holds
  * in its grow-only work buffers - field partial sums, near lists, operator
- * matrices, LU and solve work - in KiB, rounded up; a caller that budgets
- * device memory charges it).  Returns BIELCC_ERR_ARG for an unknown key or a
+ * matrices, LU and solve work, and the near-pair list each mesh keeps from its
+ * last assembly for the next one of the same rows - in KiB, rounded up; a
+ * caller that budgets device memory charges it).  Returns BIELCC_ERR_ARG for an unknown key or a
  * value out of range. */
 int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value);
 int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value);

@@ -54,6 +54,28 @@ def test_r_matrix(c1, orc):
     assert np.all(ZR[2 * np.arange(n), 2 * np.arange(n)] == 0)
 
 
+def test_kept_near_list_follows_rows_and_rule(ctx, orc):
+    """The mesh keeps the near-pair list of its last assembly: the same rows again give
+    the same matrix bit for bit; other rows, then all rows again, and a wider near
+    rule (analytic_calc 4, more exact pairs) each match the oracle as on a fresh list."""
+    g = golden("config1.npz")
+    verts = g["verts"][:240]
+    m = ctx.mesh(verts)
+    k = 2.5
+    npp = B.num_param()
+    Z1 = m.assemble(npp, k)
+    assert np.array_equal(m.assemble(npp, k), Z1)
+    Zp = m.assemble(npp, k, row0=100, nrows=80)
+    assert np.array_equal(Zp, Z1[100:180])
+    assert np.array_equal(m.assemble(npp, k), Z1)
+    assert block_metric(Z1, orc.assemble(verts, O.num_param(), k)).max() <= 1e-12
+    npw = B.num_param(analytic_calc=4.0)
+    Zw = m.assemble(npw, k)
+    assert block_metric(Zw, orc.assemble(verts, O.num_param(analytic_calc=4.0), k)).max() <= 1e-12
+    assert not np.array_equal(Zw, Z1)
+    m.close()
+
+
 def test_partial_rows_and_complex_k(ctx, orc):
     g = golden("config1.npz")
     verts = g["verts"][:200]
