@@ -1247,6 +1247,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-queries", type=int, default=32)
+    ap.add_argument("--serve-stream", type=int, default=None, metavar="Q",
+                    help="queries of the Poisson stream legs (default 12 in the full run, 24 with --serving-only)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 12-mesh batched form")
     ap.add_argument("--no-tri", action="store_true", help="skip the 39,600-triangle variant")
     ap.add_argument("--tri-only", type=int, default=0, metavar="STEPS",
@@ -1311,7 +1313,8 @@ def main():
         return
     if args.serving_only:
         print(json.dumps({"serving": serving_leg(nq=args.serve_queries, workers=args.serve_workers,
-                                                 windows_path=args.trace_windows)}), flush=True)
+                                                 windows_path=args.trace_windows,
+                                                 nq_stream=args.serve_stream or 24)}), flush=True)
         return
     w = workload(args.config)
     verts, k, pts = w["verts"], w["k"], w["pts"]
@@ -1410,7 +1413,7 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_leg(args.tune_cache, args.batched_workers)
     if not args.no_serve:
-        out["serving"] = serving_leg(nq=args.serve_queries)
+        out["serving"] = serving_leg(nq=args.serve_queries, nq_stream=args.serve_stream or 12)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(verts, pts, k, args.cpu_budget)
