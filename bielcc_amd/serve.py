@@ -88,23 +88,28 @@ class Budget:
     def __init__(self, total: int):
         self.total = int(total)
         self.used = 0
+        self.fixed = 0         # part of `used` held until the server closes (worker pools, library scratch)
         self.peak = 0          # highest `used` seen
         self.cv = threading.Condition()
 
-    def reserve(self, nbytes: int, evict: Callable[[int], tuple], timeout: float = 600.0) -> None:
+    def reserve(self, nbytes: int, evict: Callable[[int], tuple], timeout: float = 600.0,
+                fixed: bool = False) -> None:
         """Block until nbytes fit.  evict(need) runs under the lock: it takes
         idle cache entries out of the cache and returns (bytes, buffers).  The
         buffers are freed after the lock is released (a device free waits for
         the device, and other workers must not wait on the lock meanwhile); their
         bytes stay counted as used until they are freed, so the device never
-        holds more than the budget.  Raises if nbytes exceeds the whole budget."""
+        holds more than the budget.  fixed: the bytes stay held until the server
+        closes.  Raises at once if nbytes cannot fit beside the fixed bytes (no
+        eviction or finished query would ever make room)."""
         nbytes = int(nbytes)
-        if nbytes > self.total:
-            raise MemoryError(f"request of {nbytes} B exceeds the HBM budget of {self.total} B")
         deadline = time.monotonic() + timeout
         while True:
             victims, vbytes = [], 0
             with self.cv:
+                if self.fixed + nbytes > self.total:
+                    raise MemoryError(f"request of {nbytes} B does not fit the HBM budget of {self.total} B "
+                                      f"beside {self.fixed} B of worker buffers held for good")
                 while self.used + nbytes > self.total:
                     freed, bufs = evict(self.used + nbytes - self.total)
                     if freed > 0:
@@ -114,8 +119,13 @@ class Budget:
                     if left <= 0:
                         raise TimeoutError("HBM budget: reservation did not fit in time")
                     self.cv.wait(timeout=min(left, 1.0))
+                    if self.fixed + nbytes > self.total:   # fixed bytes grew meanwhile
+                        raise MemoryError(f"request of {nbytes} B does not fit the HBM budget of "
+                                          f"{self.total} B beside {self.fixed} B of worker buffers held for good")
                 else:
                     self.used += nbytes
+                    if fixed:
+                        self.fixed += nbytes
                     self.peak = max(self.peak, self.used)
                     return
             try:
@@ -124,12 +134,14 @@ class Budget:
             finally:
                 self.release(vbytes)
 
-    def charge(self, nbytes: int, evict: Callable[[int], tuple]) -> None:
+    def charge(self, nbytes: int, evict: Callable[[int], tuple], fixed: bool = False) -> None:
         """Count bytes that are already allocated (no waiting: the caller may hold
         pins that others wait for); idle entries are evicted to get back under
-        the budget as far as possible."""
+        the budget as far as possible.  fixed: held until the server closes."""
         with self.cv:
             self.used += int(nbytes)
+            if fixed:
+                self.fixed += int(nbytes)
             self.peak = max(self.peak, self.used)
             over = self.used - self.total
             freed, victims = evict(over) if over > 0 else (0, [])
@@ -140,9 +152,11 @@ class Budget:
             if freed:
                 self.release(freed)
 
-    def release(self, nbytes: int) -> None:
+    def release(self, nbytes: int, fixed: bool = False) -> None:
         with self.cv:
             self.used -= int(nbytes)
+            if fixed:
+                self.fixed -= int(nbytes)
             self.cv.notify_all()
 
 
@@ -351,10 +365,10 @@ class Server:
         if d is not None and d.nbytes >= nbytes:
             return d
         grow = int(nbytes * 1.25) + 4096
-        self.budget.reserve(grow, self.cache.evict)
+        self.budget.reserve(grow, self.cache.evict, fixed=True)
         if d is not None:
             d.free()
-            self.budget.release(d.nbytes)
+            self.budget.release(d.nbytes, fixed=True)
         d = self.ctxs[w].empty(grow)
         pool[name] = d
         return d
@@ -364,7 +378,7 @@ class Server:
         near lists, operator matrices, solve work) hold beyond what was charged."""
         actual = ctx.get_tuning("scratch_kb") << 10
         if actual > self.scratch_hw[w]:
-            self.budget.charge(actual - self.scratch_hw[w], self.cache.evict)
+            self.budget.charge(actual - self.scratch_hw[w], self.cache.evict, fixed=True)
             self.scratch_hw[w] = actual
 
     def _factor_bytes(self, n: int) -> int:
@@ -476,7 +490,7 @@ class Server:
         # the field driver's scratch is context-owned and grow-only: charge its high-water mark
         grow = max(0, _field_scratch_bytes(M, n, nrhs) - self.scratch_hw[w])
         if grow:
-            self.budget.reserve(grow, self.cache.evict)
+            self.budget.reserve(grow, self.cache.evict, fixed=True)
             self.scratch_hw[w] += grow
         # the query's buffers grow (and are charged) before any cache pin is taken: a
         # worker that waits for budget bytes must hold no pin, or workers that each

@@ -42,25 +42,34 @@ def _copy(q):
     return serve.Query(q.mesh, q.k, q.dirs, q.pols, q.points, qid=q.qid)
 
 
-def _alone(form, queries):
+def _alone(form, queries, info=None):
     out = {}
     with serve.Server(_pool(), form=form, workers=1, hbm_budget=4e9) as s:
         for q in queries:
             s.submit(_copy(q))
             out[q.qid] = s.wait([q.qid])[q.qid]
+        if info is not None:   # bytes one worker holds for good, and the largest factor set
+            info["fixed"] = s.budget.fixed
+            info["factors"] = max(s._factor_bytes(len(v)) for v in _pool().values())
     return out
 
 
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
 def test_same_result_alone_and_in_the_mix(form):
     qs = _queries()
-    alone = _alone(form, qs)
-    # budget: room for about two factor sets plus workspaces, so the mix evicts
-    with serve.Server(_pool(), form=form, workers=3, hbm_budget=5e6) as s:
+    info = {}
+    alone = _alone(form, qs, info)
+    # budget: the three workers' own buffers (each at most what one worker held serving
+    # every query alone) plus room for one and a half of the largest factor sets, so the
+    # mix evicts
+    budget = 3 * info["fixed"] + 1.5 * info["factors"]
+    print(form, "budget", int(budget), info, flush=True)
+    with serve.Server(_pool(), form=form, workers=3, hbm_budget=budget) as s:
         run = serve.poisson_run(s, [_copy(q) for q in qs], qps=200.0, seed=3)
     st = run["stats"]
     print(form, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
     assert st["p50_s"] <= st["p99_s"] and st["solves"] == sum(q.nrhs for q in qs)
+    assert st["evictions"] >= 1
     for q in qs:
         a, b = alone[q.qid], run["results"][q.qid]
         assert np.abs(a.E - b.E).max() <= 1e-12 * np.abs(a.E).max(), q.qid

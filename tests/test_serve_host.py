@@ -224,3 +224,36 @@ def test_charge_of_held_bytes_never_waits():
     c.release(e)
     b.charge(0, c.evict)                  # "a" idle now: evicted back under the budget
     assert log == ["a"] and b.used == 60
+
+
+def test_reservation_beside_fixed_bytes_fails_at_once():
+    """Bytes held until the server closes (worker pools, library scratch) can never
+    be freed by an eviction: a reservation that cannot fit beside them raises at
+    once instead of waiting out its timeout, also when they grow while it waits."""
+    b = Budget(100)
+    c = FactorCache(b)
+    b.reserve(30, c.evict, fixed=True)
+    b.charge(40, c.evict, fixed=True)
+    assert b.fixed == 70 and b.used == 70
+    t0 = time.monotonic()
+    with pytest.raises(MemoryError):
+        b.reserve(31, c.evict)
+    assert time.monotonic() - t0 < 0.5
+    b.reserve(20, c.evict)                # fits beside the fixed bytes
+    err = []
+
+    def waiter():
+        try:
+            b.reserve(25, c.evict, timeout=30.0)
+        except MemoryError as x:
+            err.append(x)
+
+    t = threading.Thread(target=waiter)
+    t.start()
+    time.sleep(0.2)
+    b.charge(10, c.evict, fixed=True)     # fixed 80: 25 more can never fit
+    t.join(timeout=5)
+    assert not t.is_alive() and err
+    b.release(10, fixed=True)
+    b.release(20)
+    assert b.fixed == 70 and b.used == 70
