@@ -289,6 +289,8 @@ int lu_get_look_ahead(const LuWork*);
 // whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
 void lu_set_panel_lds(LuWork*, int mode);   // 1 within the workgroup budget, 0 never, 2 always
 void lu_set_panel_rows32(LuWork*, int);
+void lu_set_tournament(LuWork*, int mode);   // panels by tournament pivoting: 1 always, 0 never
+int lu_get_tournament(const LuWork*);
 int lu_get_panel_rows32(const LuWork*);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);

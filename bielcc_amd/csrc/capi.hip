@@ -816,6 +816,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_panel_lds(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "lu_tournament") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "lu_tournament must be 0 or 1");
+        lu_set_tournament(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "lu_panel_wait_us") {
         if (value < 0) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_wait_us must be >= 0");
         lu_set_panel_wait_us(ctx->lu, value);
@@ -867,6 +872,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu);
     else if (k == "lu_panel_rows32") *value = lu_get_panel_rows32(ctx->lu);
+    else if (k == "lu_tournament") *value = lu_get_tournament(ctx->lu);
     else if (k == "mixed_fixup") *value = ctx->mixed_fixup ? 1 : 0;
     else if (k == "mixed_fixup_points") *value = (int)std::min<long long>(ctx->fixup_points, INT_MAX);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;

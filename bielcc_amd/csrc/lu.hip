@@ -102,6 +102,9 @@ struct LuWork {
     int2* smap = nullptr;           // the current panel's interchanges as a gather (2 kNB entries + count)
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
     int panel_lds = 1;              // whole-panel kernel: 1 within the workgroup budget, 0 never, 2 always
+    int tournament = 0;             // panels by tournament pivoting (k_ts_tree): 1 always, 0 never
+    void* tbuf = nullptr;           // its candidate lists, node counters, pivot rows, U11 and multipliers
+    int tcap = 0;                   // blocks of kTsRows rows the buffer serves
     int panel_rows32 = 0;           // rows per workgroup of FP32 whole panels: 0 by height (64, or 128 when 64
                                     // would need more workgroups than fit), 64 or 128 forced (tests)
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
@@ -131,6 +134,8 @@ void lu_set_spin_limit(LuWork* w, int polls) { w->spin_limit = polls; }
 void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
 void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
+void lu_set_tournament(LuWork* w, int mode) { w->tournament = mode; }
+int lu_get_tournament(const LuWork* w) { return w->tournament; }
 void lu_set_panel_rows32(LuWork* w, int rows) { w->panel_rows32 = rows; }
 int lu_get_panel_rows32(const LuWork* w) { return w->panel_rows32; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
@@ -203,7 +208,7 @@ void lu_work_destroy(LuWork* w) {
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
     (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->dd);
     (void)hipFree(w->step_count);
-    (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap);
+    (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap); (void)hipFree(w->tbuf);
     if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
     if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
     if (w->side) (void)hipStreamDestroy(w->side);
@@ -828,19 +833,17 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
 // the row whose old contents end at d: map[q] = (d, source), the panel's own rows
 // first (q < nb, d = k0 + q), then the outside pivot rows in order of appearance;
 // *count = |R| <= 2 nb.  A pivot row outside [k0 + t, n) is not used: it raises
-// *bad, counts as no interchange, and the LU reports the error.  One wave.
-__global__ __launch_bounds__(64) void k_swap_map(int n, int k0, int nb, const int* __restrict__ piv,
-                                                 int2* __restrict__ map, int* __restrict__ count,
-                                                 int* __restrict__ bad) {
+// *bad, counts as no interchange, and the LU reports the error.  swap_map_wave does it
+// from lane t's pivot row p (t < nb) in a kernel of one wave; k_swap_map reads piv.
+__device__ void swap_map_wave(int n, int k0, int nb, int p, int2* __restrict__ map, int* __restrict__ count,
+                              int* __restrict__ bad) {
     __shared__ int sp[kNB];           // the panel's pivot rows
     __shared__ int orow[kNB];         // outside rows, by first appearance
     __shared__ int slot[kNB];         // slot of step t's partner row
     __shared__ int content[2 * kNB];  // original row now held by each slot
     const int t = threadIdx.x;
-    int p = 0;
     bool first = false;
     if (t < nb) {
-        p = piv[k0 + t];
         if (p < k0 + t || p >= n) {
             atomicExch(bad, 1);
             p = k0 + t;
@@ -875,6 +878,13 @@ __global__ __launch_bounds__(64) void k_swap_map(int n, int k0, int nb, const in
     __syncthreads();
     for (int q = t; q < nb + nout; q += 64) map[q] = make_int2(q < nb ? k0 + q : orow[q - nb], content[q]);
     if (t == 0) *count = nb + nout;
+}
+
+__global__ __launch_bounds__(64) void k_swap_map(int n, int k0, int nb, const int* __restrict__ piv,
+                                                 int2* __restrict__ map, int* __restrict__ count,
+                                                 int* __restrict__ bad) {
+    const int t = threadIdx.x;
+    swap_map_wave(n, k0, nb, t < nb ? piv[k0 + t] : 0, map, count, bad);
 }
 
 // Apply a panel's interchanges (k_swap_map) to the columns outside the panel,
@@ -1016,6 +1026,356 @@ __global__ __launch_bounds__(256) void k_trsm(int nb, int lower, const c2<R>* __
     }
     for (int cc = cg; cc < ncol; cc += 4)
         if (r < nb) B[(size_t)(c0 + cc) * ldb + brow + r] = X[r][cc];
+}
+
+// ---------------------------------------------------------------------------
+// Tournament pivoting: a panel with no grid-wide step.  The panel's m rows are
+// split into blocks of kTsRows; each workgroup factors its block with partial
+// pivoting (the rule above: largest |re|+|im|) and keeps the kNB rows it took,
+// in order, as its candidates.  Candidates meet in a binary tree: the lists of
+// two sibling nodes (left first) are stacked, their rows re-read in their
+// original state from A (nothing writes the panel while the tree runs), and
+// factored with partial pivoting again, which keeps kNB of them.  The root's
+// rows are the panel's pivot rows in the order its factorisation took them;
+// its eliminated pivot rows are U11 and its multipliers of those rows L11 (an
+// LU of the chosen rows in that order needs no interchange).
+// No workgroup waits for another.  A node's second arrival carries on with the
+// parent, the first leaves (a device-scope counter per node, reset by the
+// second; the lists go through device-scope stores and loads), so a launch
+// needs no workgroups resident together and runs beside other kernels.
+// Then k_ts_map turns the pivot rows into interchanges (LAPACK order) and the
+// gather map, k_ts_gather applies them to the panel's columns and writes
+// L11 \ U11 into its top rows, and k_ts_l21 forms L21 = A21 U11^{-1} below.
+// Growth is bounded differently from partial pivoting (no |l| <= 1 below the
+// top block); the LU is checked by backward error.
+constexpr int kTsRows = 128;     // rows of a leaf block, and 2 kNB candidates of a tree node
+constexpr int kTsLevels = 17;    // levels of counters: 2^17 blocks of 128 rows
+
+// Candidate key: |re|+|im|, floored at 2^-1020 (a normal number), with its low
+// 7 mantissa bits replaced by 127 - slot.  The largest key is the largest value,
+// the lowest slot on ties; compared values move by less than 2^-45 relative (the
+// pivot itself is read back unchanged).
+__device__ __forceinline__ double ts_key(double a, int slot) {
+    const long long b = __double_as_longlong(fmax(a, 0x1p-1020));
+    return __longlong_as_double((b & ~127ll) | (long long)(127 - slot));
+}
+__device__ __forceinline__ int ts_slot(double key) { return 127 - (int)(__double_as_longlong(key) & 127ll); }
+
+// dst = arr[i] (arr: 16 entries) for a wave-uniform i, as a switch on scalar registers.
+// It is a macro so that the switch lands in the kernel body, where the array is a local
+// that becomes registers: in a function taking the array by reference the optimiser
+// merges the cases into one indexed load, which puts the array in scratch memory.
+#define TS_PICK16(dst, arr, i)                                                                    \
+    switch (i) {                                                                                  \
+        case 0: dst = arr[0]; break;   case 1: dst = arr[1]; break;   case 2: dst = arr[2]; break;   \
+        case 3: dst = arr[3]; break;   case 4: dst = arr[4]; break;   case 5: dst = arr[5]; break;   \
+        case 6: dst = arr[6]; break;   case 7: dst = arr[7]; break;   case 8: dst = arr[8]; break;   \
+        case 9: dst = arr[9]; break;   case 10: dst = arr[10]; break; case 11: dst = arr[11]; break; \
+        case 12: dst = arr[12]; break; case 13: dst = arr[13]; break; case 14: dst = arr[14]; break; \
+        default: dst = arr[15]; break;                                                            \
+    }
+template <typename R>
+__device__ __forceinline__ void csub_mul(c2<R>& v, c2<R> l, c2<R> u) {   // v -= l u
+    v.x = fma(-l.x, u.x, fma(l.y, u.y, v.x));
+    v.y = fma(-l.x, u.y, fma(-l.y, u.x, v.y));
+}
+
+// Node rows in registers.  Lane (rw = lane % 32, cg = lane / 32) of wave w holds row
+// slot 32 w + rw; after T shifts v[t] holds column cg + 2 (t + T), so the next pivot
+// column is always v[0] of group j % 2 (no register index that varies at run time,
+// which would put the array in scratch memory).  Entries past column 63 are dead.
+constexpr int kTsRowPad = 80;   // LDS row: columns up to cg + 2 (7 + 31) of a live chunk
+
+// This wave's best remaining row for column j: its key into *key_out, its values from
+// the column of v[0] on into row.
+template <typename R>
+__device__ __forceinline__ void ts_publish(const c2<R> (&v)[32], int T, int j, bool live, int slot, c2<R>* row,
+                                           double* key_out) {
+    const int lane = threadIdx.x & 63, rw = lane & 31, cg = lane >> 5, q = j & 1;
+    double key = (live && cg == q) ? ts_key(cabs1<R>(v[0]), slot) : -1.0;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) key = fmax(key, __shfl_xor(key, off, 64));
+    key = __shfl(key, 32 * q, 64);
+    if (key >= 0. && rw == (ts_slot(key) & 31)) {
+        c2<R>* r = row + cg + 2 * T;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (8 * k + T < 32)
+#pragma unroll
+                for (int t = 8 * k; t < 8 * k + 8; t++) r[2 * t] = v[t];
+    }
+    if (lane == 0) *key_out = key;
+}
+
+// One launch per panel, a workgroup per block of kTsRows rows.  Node factorisation:
+// wave w holds slots 32 w .. 32 w + 31; lane (rw = lane % 32, cg = lane / 32) holds
+// row slot 32 w + rw in columns cg + 2 t (t < 32), in registers.  Per column j:
+// each wave publishes its best remaining row (key and the row's values from column j
+// on) into LDS, one workgroup barrier, every lane picks the same winner, takes its
+// multiplier from the lane of its row that holds column j, and updates its columns.
+// clist [blocks][kNB]: published lists (node (level, a) at a << level); ctr
+// [kTsLevels][half]: arrivals per parent node.  Root outputs: sel / sslot [kNB]
+// (chosen rows and their root slots), u11 [kNB][kNB] (row j from column j on),
+// lbuf [kTsRows][kNB] (the root's multipliers by slot and column), info.
+template <typename R>
+__global__ __launch_bounds__(256) void k_ts_tree(int n, int k0, int nb, const c2<R>* __restrict__ A, long long lda,
+                                                 int* __restrict__ clist, unsigned* __restrict__ ctr, int half,
+                                                 int* __restrict__ sel, int* __restrict__ sslot,
+                                                 c2<R>* __restrict__ u11, c2<R>* __restrict__ lbuf,
+                                                 int* __restrict__ info) {
+    __shared__ c2<R> s_row[2][4][kTsRowPad];   // each wave's candidate row, by column parity
+    __shared__ double s_key[2][4];
+    __shared__ int s_list[kTsRows];      // original row of each slot (-1: none)
+    __shared__ int s_sel[kNB];           // the node's chosen rows, in order
+    __shared__ int s_go;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int rw = lane & 31, cg = lane >> 5, slot = w * 32 + rw;
+    const int m = n - k0;
+    int a = blockIdx.x, cnt = gridDim.x, level = 0;
+    for (int s = tid; s < kTsRows; s += 256) {
+        const int r = a * kTsRows + s;
+        s_list[s] = r < m ? k0 + r : -1;
+    }
+    __syncthreads();
+    for (;;) {
+        const bool root = cnt == 1;
+        const int rid = s_list[slot];
+        bool live = rid >= 0;   // a row of this node not yet taken
+        c2<R> v[32];
+#pragma unroll
+        for (int t = 0; t < 32; t++) {
+            const int c = cg + 2 * t;
+            v[t] = (live && c < nb) ? A[(size_t)(k0 + c) * lda + rid] : mk2<R>(R(0), R(0));
+        }
+        int T = 0;   // shifts done
+        ts_publish<R>(v, T, 0, live, slot, s_row[0][w], &s_key[0][w]);
+        __syncthreads();
+        int nsel = nb;
+        for (int j = 0; j < nb; j++) {
+            const int b = j & 1;
+            double kb = s_key[b][0];
+            int wv = 0;
+#pragma unroll
+            for (int u = 1; u < 4; u++) {
+                const double k2 = s_key[b][u];
+                if (k2 > kb) { kb = k2; wv = u; }
+            }
+            if (kb < 0.) { nsel = j; break; }   // no row left (same verdict in every wave)
+            const int ps = ts_slot(kb);
+            const c2<R> pv = s_row[b][wv][j];
+            const R den = norm2_fx<R>(pv);
+            const bool sc = den > R(0);
+            const c2<R> pinv = sc ? mk2<R>(pv.x / den, -pv.y / den) : mk2<R>(R(0), R(0));
+            if (tid == 0) {
+                s_sel[j] = s_list[ps];
+                if (root) {
+                    sel[j] = s_list[ps];
+                    sslot[j] = ps;
+                    if (!sc && *info == 0) *info = k0 + j + 1;
+                }
+            }
+            if (root && tid >= j && tid < nb) u11[(size_t)j * kNB + tid] = s_row[b][wv][tid];
+            if (slot == ps) live = false;
+            // multiplier of this lane's row: column j is v[0] of the lanes of group j % 2
+            const int src = rw + 32 * (j & 1);
+            c2<R> l = mk2<R>(__shfl(v[0].x, src, 64), __shfl(v[0].y, src, 64));
+            l = cmul_fx<R>(l, pinv);
+            if (root && cg == 0 && rid >= 0) lbuf[(size_t)slot * kNB + j] = l;
+            // rank-1 update (entries of finished columns, dead entries and taken rows change
+            // too, and are never read again)
+            const c2<R>* ur = s_row[b][wv] + cg + 2 * T;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (8 * k + T < 32)
+#pragma unroll
+                    for (int t = 8 * k; t < 8 * k + 8; t++) csub_mul<R>(v[t], l, ur[2 * t]);
+            if (j & 1) {   // columns 2 T and 2 T + 1 done: shift by one
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (8 * k + T < 31)
+#pragma unroll
+                        for (int t = 8 * k; t < 8 * k + 8 && t < 31; t++) v[t] = v[t + 1];
+                T++;
+            }
+            if (j + 1 < nb) ts_publish<R>(v, T, j + 1, live, slot, s_row[b ^ 1][w], &s_key[b ^ 1][w]);
+            __syncthreads();
+        }
+        if (root) {
+            if (tid < nb && tid >= nsel) sel[tid] = -1;   // cannot happen (the root holds >= nb rows): flagged later
+            return;
+        }
+        // climb: a node without a sibling passes its list up unchanged
+        while ((a ^ 1) >= cnt) {
+            a >>= 1;
+            cnt = (cnt + 1) >> 1;
+            level++;
+        }
+        int* mine = clist + ((size_t)a << level) * kNB;
+        if (tid < kNB) st_agent<int>(&mine[tid], tid < nsel ? s_sel[tid] : -1);
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+        __syncthreads();
+        if (tid == 0) {
+            unsigned* c = &ctr[(size_t)level * half + (a >> 1)];
+            const unsigned old = atomicAdd(c, 1u);
+            s_go = old == 1u;
+            if (old == 1u) st_agent<unsigned>(c, 0u);   // second arrival: the counter is free again
+        }
+        __syncthreads();
+        if (!s_go) return;
+        const int sib = a ^ 1;
+        const int* other = clist + ((size_t)sib << level) * kNB;
+        if (tid < kNB) {
+            const int o = __hip_atomic_load(&other[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int me = tid < nsel ? s_sel[tid] : -1;
+            s_list[tid] = (a & 1) ? o : me;            // left child first
+            s_list[kNB + tid] = (a & 1) ? me : o;
+        }
+        __syncthreads();
+        a >>= 1;
+        cnt = (cnt + 1) >> 1;
+        level++;
+    }
+}
+
+// The chosen rows as LAPACK interchanges, then the gather map of swap_map_wave.
+// Row sel[t] sits at step t where a chain of displacements put it: a row moves only
+// when it is displaced from top position k0 + i at step i, to pos_i; so starting at its
+// own row, p = pos_{p - k0} while p lies in [k0, k0 + t) (each step moves it further
+// down, so the chain ends).  Rows out of range or repeated raise *bad (no interchange).
+__global__ __launch_bounds__(64) void k_ts_map(int n, int k0, int nb, const int* __restrict__ sel,
+                                               int* __restrict__ piv, int2* __restrict__ map, int* __restrict__ count,
+                                               int* __restrict__ bad) {
+    const int t = threadIdx.x;
+    const int s = t < nb ? sel[t] : -1;
+    bool ok = t >= nb || (s >= k0 && s < n);
+    for (int u = 0; u < nb; u++) {
+        const int o = __shfl(s, u, 64);
+        if (t < nb && u != t && o == s) ok = false;
+    }
+    int pos = k0 + t;
+    if (__all(ok)) {
+        for (int tt = 0; tt < nb; tt++) {
+            int p = __builtin_amdgcn_readlane(s, tt);
+            while (p >= k0 && p < k0 + tt) p = __builtin_amdgcn_readlane(pos, p - k0);
+            if (t == tt) pos = p;
+        }
+    } else if (t == 0) {
+        atomicExch(bad, 1);
+    }
+    if (t < nb) piv[k0 + t] = pos;
+    swap_map_wave(n, k0, nb, pos, map, count, bad);
+}
+
+// The panel's interchanges on its own columns, with L11 \ U11 in the top rows.  One
+// wave per panel column: the rows displaced below the panel are read before the top
+// rows are written.
+template <typename R>
+__global__ __launch_bounds__(256) void k_ts_gather(int k0, int nb, c2<R>* __restrict__ A, long long lda,
+                                                   const int2* __restrict__ map, const int* __restrict__ count,
+                                                   const int* __restrict__ sslot, const c2<R>* __restrict__ u11,
+                                                   const c2<R>* __restrict__ lbuf) {
+    const int jc = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (jc >= nb) return;   // whole waves
+    const int lane = threadIdx.x & 63, cnt = *count;
+    c2<R>* col = A + (size_t)(k0 + jc) * lda;
+    const bool out = nb + lane < cnt;
+    int2 mo = make_int2(0, 0);
+    c2<R> vo = mk2<R>(R(0), R(0));
+    if (out) {
+        mo = map[nb + lane];
+        vo = col[mo.y];
+    }
+    if (out) col[mo.x] = vo;
+    if (lane < nb)
+        col[k0 + lane] = lane > jc ? lbuf[(size_t)sslot[lane] * kNB + jc] : u11[(size_t)lane * kNB + jc];
+}
+
+// L21 = A21 U11^{-1} for the rows below the panel's top block, in place: forward
+// substitution over the panel's columns.  A workgroup per 64 rows; lane (rq = lane %
+// 16, cg = lane / 16) of wave w holds row 16 w + rq in columns cg + 4 t (t < 16); U11
+// and its reciprocal diagonal in LDS.  Per column j the lanes of group j % 4 finish
+// x_j and store it, and pass it to the other three lanes of the row (no barrier).
+template <typename R>
+__global__ __launch_bounds__(256) void k_ts_l21(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
+                                                const c2<R>* __restrict__ u11) {
+    __shared__ c2<R> Us[kNB][kNB];
+    __shared__ c2<R> dinv[kNB];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int rq = lane & 15, cg = lane >> 4;
+    for (int e = tid; e < nb * kNB; e += 256) {
+        const int j = e / kNB, c = e % kNB;
+        Us[j][c] = (c >= j && c < nb) ? u11[(size_t)j * kNB + c] : mk2<R>(R(0), R(0));
+    }
+    __syncthreads();
+    if (tid < nb) {
+        const c2<R> d = Us[tid][tid];
+        const R den = norm2_fx<R>(d);
+        dinv[tid] = den > R(0) ? mk2<R>(d.x / den, -d.y / den) : mk2<R>(R(0), R(0));
+    }
+    __syncthreads();
+    const long long d = (long long)k0 + nb + (long long)blockIdx.x * 64 + w * 16 + rq;
+    const bool valid = d < n;
+    c2<R> v[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int c = cg + 4 * t;
+        v[t] = (valid && c < nb) ? A[(size_t)(k0 + c) * lda + d] : mk2<R>(R(0), R(0));
+    }
+    for (int j = 0; j < nb; j++) {
+        const int q = j & 3;
+        c2<R> xp;
+        TS_PICK16(xp, v, j >> 2);
+        const c2<R> xj = cmul_fx<R>(xp, dinv[j]);
+        if (cg == q && valid) A[(size_t)(k0 + j) * lda + d] = xj;
+        const int src = rq + 16 * q;
+        const c2<R> x = mk2<R>(__shfl(xj.x, src, 64), __shfl(xj.y, src, 64));
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++)
+            if (16 * ch + 15 > j)
+#pragma unroll
+                for (int t = 4 * ch; t < 4 * ch + 4; t++) csub_mul<R>(v[t], x, Us[j][cg + 4 * t]);
+    }
+}
+
+// tournament buffers for panels of up to n rows: clist, ctr, sel, sslot, u11, lbuf
+struct TsBufs {
+    int* clist;
+    unsigned* ctr;
+    int half;
+    int* sel;
+    int* sslot;
+    void* u11;
+    void* lbuf;
+};
+static hipError_t ts_bufs(LuWork* w, int n, TsBufs* tb) {
+    hipError_t e;
+    const int blocks = std::max(2, (n + kTsRows - 1) / kTsRows);
+    const int half = (blocks + 1) / 2;
+    const size_t cl = (size_t)blocks * kNB * sizeof(int), ct = (size_t)kTsLevels * half * sizeof(unsigned);
+    const size_t ss = 2 * kNB * sizeof(int), ub = (size_t)kNB * kNB * 16, lb = (size_t)kTsRows * kNB * 16;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    if (w->tcap < blocks) {
+        (void)hipFree(w->tbuf);
+        w->tbuf = nullptr;
+        w->tcap = 0;
+        if ((e = hipMalloc(&w->tbuf, al(cl) + al(ct) + al(ss) + al(ub) + lb)) != hipSuccess) return e;
+        // node counters start at zero and each node's second arrival resets its own; the
+        // null-stream memset is waited for before any kernel can use them
+        if ((e = hipMemset(static_cast<char*>(w->tbuf) + al(cl), 0, ct)) != hipSuccess) return e;
+        if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+        w->tcap = blocks;
+    }
+    const int hcap = (w->tcap + 1) / 2;
+    const size_t ctc = (size_t)kTsLevels * hcap * sizeof(unsigned), clc = (size_t)w->tcap * kNB * sizeof(int);
+    char* b = static_cast<char*>(w->tbuf);
+    tb->clist = reinterpret_cast<int*>(b);
+    tb->ctr = reinterpret_cast<unsigned*>(b + al(clc));
+    tb->half = hcap;
+    tb->sel = reinterpret_cast<int*>(b + al(clc) + al(ctc));
+    tb->sslot = tb->sel + kNB;
+    tb->u11 = b + al(clc) + al(ctc) + al(ss);
+    tb->lbuf = b + al(clc) + al(ctc) + al(ss) + al(ub);
+    return hipSuccess;
 }
 
 // C -= A B (complex, column-major) on the matrix cores
@@ -1166,7 +1526,10 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // The amount covers every panel of height at most m: FP32 panels switch from 128 to 64
     // rows per workgroup below 64 gmax rows, where their workgroup count goes up.
     auto reserve_for = [&](int m) { return std::min((m + 63) / 64, gmax); };
-    const bool lds_allowed = w->panel_lds != 0 && w->n_cu > 0;
+    const bool tourn = w->tournament == 1;
+    TsBufs tb{};
+    if (tourn && (e = ts_bufs(w, n, &tb)) != hipSuccess) return e;
+    const bool lds_allowed = !tourn && w->panel_lds != 0 && w->n_cu > 0;
     bool lds = lds_allowed && (force || admit.take(w->dev, reserve_for(n), 2 * w->n_cu));
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
     char* pb = static_cast<char*>(w->pbuf);
@@ -1180,6 +1543,21 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     auto factor_panel = [&](hipStream_t st, int k0) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
+        if (tourn) {   // tournament pivoting: tree, interchanges, the panel's columns, L21
+            const int GT = (n - k0 + kTsRows - 1) / kTsRows;
+            c2<R>* u11 = static_cast<c2<R>*>(tb.u11);
+            c2<R>* lbuf = static_cast<c2<R>*>(tb.lbuf);
+            hipLaunchKernelGGL(k_ts_tree<R>, dim3(GT), dim3(256), 0, st, n, k0, nb, (const c2<R>*)A, lda, tb.clist,
+                               tb.ctr, tb.half, tb.sel, tb.sslot, u11, lbuf, w->info);
+            hipLaunchKernelGGL(k_ts_map, dim3(1), dim3(64), 0, st, n, k0, nb, tb.sel, piv, w->smap, smap_count,
+                               smap_count + 1);
+            hipLaunchKernelGGL(k_ts_gather<R>, dim3((nb + 3) / 4), dim3(256), 0, st, k0, nb, A, lda, w->smap,
+                               smap_count, tb.sslot, (const c2<R>*)u11, (const c2<R>*)lbuf);
+            if (n - k0 > nb)
+                hipLaunchKernelGGL(k_ts_l21<R>, dim3((n - k0 - nb + 63) / 64), dim3(256), 0, st, n, k0, nb, A, lda,
+                                   (const c2<R>*)u11);
+            return;
+        }
         const int G = wgs_for(n - k0);
         if (!lds && lds_allowed && G <= gmax && admit.take(w->dev, reserve_for(n - k0), 2 * w->n_cu)) {
             lds = true;
