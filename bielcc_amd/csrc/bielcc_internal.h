@@ -291,6 +291,8 @@ void lu_set_panel_lds(LuWork*, int mode);   // 1 within the workgroup budget, 0 
 void lu_set_panel_rows32(LuWork*, int);
 void lu_set_tournament(LuWork*, int mode);   // panels by tournament pivoting: 1 always, 0 never
 int lu_get_tournament(const LuWork*);
+void lu_set_gemm16(LuWork*, int mode);       // FP32 LU rank-512 updates on FP16 matrix cores: 1 split, 2 FP16, 0 off
+int lu_get_gemm16(const LuWork*);
 int lu_get_panel_rows32(const LuWork*);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);

@@ -47,6 +47,9 @@ constexpr int kLookAheadMinN = 20000;
 constexpr double kResidFloor = 1e-14;
 // From this order on the LU is blocked in two levels (outer blocks of 512 columns).
 constexpr int kTwoLevelMinN = 16000;
+// ... and an FP32 LU whose rank-512 updates run on the FP16 matrix cores (split operands)
+// from this one: order 12,000 0.130 s one-level, 0.118 s two-level; order 6,000 0.043 / 0.045 s
+constexpr int kTwoLevelMinN32 = 10000;
 // candidate buffers of the whole-panel kernel: cval, cidx [2][512], crow [2][512][64], rowj [2][64] (FP64 sizes)
 constexpr size_t kPanelBufBytes = 2 * 512 * (8 + 4) + 2 * 512 * 64 * 16 + 2 * 64 * 16;
 
@@ -103,6 +106,11 @@ struct LuWork {
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
     int panel_lds = 1;              // whole-panel kernel: 1 within the workgroup budget, 0 never, 2 always
     int tournament = 0;             // panels by tournament pivoting (k_ts_tree): 1 always, 0 never
+    int gemm16 = 1;                 // FP32 LU's rank-512 updates on the FP16 matrix cores: 1 split (FP32-class),
+                                    // 2 FP16 only, 0 complex FP32 GEMM.  Split, order 40,000: 1.81 -> 1.20 s,
+                                    // backward error 1.8e-4 -> 1.4e-4; FP16 only: 0.81 s, backward error 0.47
+    void* g16 = nullptr;            // their FP16 operands, maxima and GEMM scalars
+    size_t g16_cap = 0;
     void* tbuf = nullptr;           // its candidate lists, node counters, pivot rows, U11 and multipliers
     int tcap = 0;                   // blocks of kTsRows rows the buffer serves
     int panel_rows32 = 0;           // rows per workgroup of FP32 whole panels: 0 by height (64, or 128 when 64
@@ -135,6 +143,8 @@ void lu_set_fused_panel(LuWork* w, int mode) { w->fused_panel = mode; }
 void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
 void lu_set_tournament(LuWork* w, int mode) { w->tournament = mode; }
+void lu_set_gemm16(LuWork* w, int mode) { w->gemm16 = mode; }
+int lu_get_gemm16(const LuWork* w) { return w->gemm16; }
 int lu_get_tournament(const LuWork* w) { return w->tournament; }
 void lu_set_panel_rows32(LuWork* w, int rows) { w->panel_rows32 = rows; }
 int lu_get_panel_rows32(const LuWork* w) { return w->panel_rows32; }
@@ -164,7 +174,8 @@ int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 int lu_late_admits(const LuWork* w) { return w->late_admits; }
 size_t lu_work_bytes(const LuWork* w) {   // grow-only device work of the LU and the solves
-    return w ? w->perm_cap + w->ref_cap + w->kry_cap + w->dd_cap + (size_t)w->pcap * (sizeof(double) + sizeof(int))
+    return w ? w->perm_cap + w->ref_cap + w->kry_cap + w->dd_cap + w->g16_cap +
+                   (size_t)w->pcap * (sizeof(double) + sizeof(int))
              : 0;
 }
 bool lu_get_blas_atomics(const LuWork* w) { return w->blas_atomics; }
@@ -209,6 +220,7 @@ void lu_work_destroy(LuWork* w) {
     (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->dd);
     (void)hipFree(w->step_count);
     (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap); (void)hipFree(w->tbuf);
+    (void)hipFree(w->g16);
     if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
     if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
     if (w->side) (void)hipStreamDestroy(w->side);
@@ -1061,19 +1073,6 @@ __device__ __forceinline__ double ts_key(double a, int slot) {
 }
 __device__ __forceinline__ int ts_slot(double key) { return 127 - (int)(__double_as_longlong(key) & 127ll); }
 
-// dst = arr[i] (arr: 16 entries) for a wave-uniform i, as a switch on scalar registers.
-// It is a macro so that the switch lands in the kernel body, where the array is a local
-// that becomes registers: in a function taking the array by reference the optimiser
-// merges the cases into one indexed load, which puts the array in scratch memory.
-#define TS_PICK16(dst, arr, i)                                                                    \
-    switch (i) {                                                                                  \
-        case 0: dst = arr[0]; break;   case 1: dst = arr[1]; break;   case 2: dst = arr[2]; break;   \
-        case 3: dst = arr[3]; break;   case 4: dst = arr[4]; break;   case 5: dst = arr[5]; break;   \
-        case 6: dst = arr[6]; break;   case 7: dst = arr[7]; break;   case 8: dst = arr[8]; break;   \
-        case 9: dst = arr[9]; break;   case 10: dst = arr[10]; break; case 11: dst = arr[11]; break; \
-        case 12: dst = arr[12]; break; case 13: dst = arr[13]; break; case 14: dst = arr[14]; break; \
-        default: dst = arr[15]; break;                                                            \
-    }
 template <typename R>
 __device__ __forceinline__ void csub_mul(c2<R>& v, c2<R> l, c2<R> u) {   // v -= l u
     v.x = fma(-l.x, u.x, fma(l.y, u.y, v.x));
@@ -1086,25 +1085,49 @@ __device__ __forceinline__ void csub_mul(c2<R>& v, c2<R> l, c2<R> u) {   // v -=
 // which would put the array in scratch memory).  Entries past column 63 are dead.
 constexpr int kTsRowPad = 80;   // LDS row: columns up to cg + 2 (7 + 31) of a live chunk
 
-// This wave's best remaining row for column j: its key into *key_out, its values from
-// the column of v[0] on into row.
+// Max of a double over each 32-lane half of the wave, left in every lane of the half:
+// quad swaps and row rotations by DPP, then one swizzle across the two rows of 16.
+template <int CTRL>
+__device__ __forceinline__ double ts_dpp(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double ts_max32(double k) {
+    k = fmax(k, ts_dpp<0xB1>(k));    // quad_perm 1,0,3,2
+    k = fmax(k, ts_dpp<0x4E>(k));    // quad_perm 2,3,0,1
+    k = fmax(k, ts_dpp<0x124>(k));   // row_ror 4
+    k = fmax(k, ts_dpp<0x128>(k));   // row_ror 8
+    const long long b = __double_as_longlong(k);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x401F);   // lane ^ 16 within 32
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+    return fmax(k, __longlong_as_double(((long long)hi << 32) | (unsigned int)lo));
+}
+
+// This wave's best remaining row for column j (lanes of group j % 2 hold it in v[0]):
+// its key, the reciprocal of its pivot entry and its values from column j on.
 template <typename R>
 __device__ __forceinline__ void ts_publish(const c2<R> (&v)[32], int T, int j, bool live, int slot, c2<R>* row,
-                                           double* key_out) {
+                                           double* key_out, c2<R>* pinv_out) {
     const int lane = threadIdx.x & 63, rw = lane & 31, cg = lane >> 5, q = j & 1;
-    double key = (live && cg == q) ? ts_key(cabs1<R>(v[0]), slot) : -1.0;
+    const double key = ts_max32((live && cg == q) ? ts_key(cabs1<R>(v[0]), slot) : -1.0);
+    const double kq = bcast_lane(key, 32 * q);   // wave-uniform
+    if (kq >= 0.) {
+        const int ws = ts_slot(kq) & 31;
+        if (rw == ws) {
+            c2<R>* r = row + cg + 2 * T;
 #pragma unroll
-    for (int off = 1; off < 32; off <<= 1) key = fmax(key, __shfl_xor(key, off, 64));
-    key = __shfl(key, 32 * q, 64);
-    if (key >= 0. && rw == (ts_slot(key) & 31)) {
-        c2<R>* r = row + cg + 2 * T;
+            for (int k = 0; k < 4; k++)
+                if (8 * k + T < 32)
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (8 * k + T < 32)
-#pragma unroll
-                for (int t = 8 * k; t < 8 * k + 8; t++) r[2 * t] = v[t];
+                    for (int t = 8 * k; t < 8 * k + 8; t++) r[2 * t] = v[t];
+        }
+        const c2<R> pv = mk2<R>(bcast_lane(v[0].x, ws + 32 * q), bcast_lane(v[0].y, ws + 32 * q));
+        const R den = norm2_fx<R>(pv);
+        if (lane == 0) *pinv_out = den > R(0) ? mk2<R>(pv.x / den, -pv.y / den) : mk2<R>(R(0), R(0));
     }
-    if (lane == 0) *key_out = key;
+    if (lane == 0) *key_out = kq;
 }
 
 // One launch per panel, a workgroup per block of kTsRows rows.  Node factorisation:
@@ -1125,6 +1148,7 @@ __global__ __launch_bounds__(256) void k_ts_tree(int n, int k0, int nb, const c2
                                                  int* __restrict__ info) {
     __shared__ c2<R> s_row[2][4][kTsRowPad];   // each wave's candidate row, by column parity
     __shared__ double s_key[2][4];
+    __shared__ c2<R> s_pinv[2][4];       // reciprocal of each wave's candidate pivot entry
     __shared__ int s_list[kTsRows];      // original row of each slot (-1: none)
     __shared__ int s_sel[kNB];           // the node's chosen rows, in order
     __shared__ int s_go;
@@ -1148,7 +1172,9 @@ __global__ __launch_bounds__(256) void k_ts_tree(int n, int k0, int nb, const c2
             v[t] = (live && c < nb) ? A[(size_t)(k0 + c) * lda + rid] : mk2<R>(R(0), R(0));
         }
         int T = 0;   // shifts done
-        ts_publish<R>(v, T, 0, live, slot, s_row[0][w], &s_key[0][w]);
+        ts_publish<R>(v, T, 0, live, slot, s_row[0][w], &s_key[0][w], &s_pinv[0][w]);
+        // this lane's entry of the next pivot column (held by its row's lane of that group)
+        c2<R> xn = mk2<R>(__shfl(v[0].x, rw, 64), __shfl(v[0].y, rw, 64));
         __syncthreads();
         int nsel = nb;
         for (int j = 0; j < nb; j++) {
@@ -1162,24 +1188,18 @@ __global__ __launch_bounds__(256) void k_ts_tree(int n, int k0, int nb, const c2
             }
             if (kb < 0.) { nsel = j; break; }   // no row left (same verdict in every wave)
             const int ps = ts_slot(kb);
-            const c2<R> pv = s_row[b][wv][j];
-            const R den = norm2_fx<R>(pv);
-            const bool sc = den > R(0);
-            const c2<R> pinv = sc ? mk2<R>(pv.x / den, -pv.y / den) : mk2<R>(R(0), R(0));
+            const c2<R> pinv = s_pinv[b][wv];
             if (tid == 0) {
                 s_sel[j] = s_list[ps];
                 if (root) {
                     sel[j] = s_list[ps];
                     sslot[j] = ps;
-                    if (!sc && *info == 0) *info = k0 + j + 1;
+                    if (norm2_fx<R>(s_row[b][wv][j]) == R(0) && *info == 0) *info = k0 + j + 1;
                 }
             }
             if (root && tid >= j && tid < nb) u11[(size_t)j * kNB + tid] = s_row[b][wv][tid];
             if (slot == ps) live = false;
-            // multiplier of this lane's row: column j is v[0] of the lanes of group j % 2
-            const int src = rw + 32 * (j & 1);
-            c2<R> l = mk2<R>(__shfl(v[0].x, src, 64), __shfl(v[0].y, src, 64));
-            l = cmul_fx<R>(l, pinv);
+            const c2<R> l = cmul_fx<R>(xn, pinv);   // multiplier of this lane's row
             if (root && cg == 0 && rid >= 0) lbuf[(size_t)slot * kNB + j] = l;
             // rank-1 update (entries of finished columns, dead entries and taken rows change
             // too, and are never read again)
@@ -1197,7 +1217,11 @@ __global__ __launch_bounds__(256) void k_ts_tree(int n, int k0, int nb, const c2
                         for (int t = 8 * k; t < 8 * k + 8 && t < 31; t++) v[t] = v[t + 1];
                 T++;
             }
-            if (j + 1 < nb) ts_publish<R>(v, T, j + 1, live, slot, s_row[b ^ 1][w], &s_key[b ^ 1][w]);
+            if (j + 1 < nb) {
+                ts_publish<R>(v, T, j + 1, live, slot, s_row[b ^ 1][w], &s_key[b ^ 1][w], &s_pinv[b ^ 1][w]);
+                const int src = rw + 32 * ((j + 1) & 1);
+                xn = mk2<R>(__shfl(v[0].x, src, 64), __shfl(v[0].y, src, 64));
+            }
             __syncthreads();
         }
         if (root) {
@@ -1237,33 +1261,78 @@ __global__ __launch_bounds__(256) void k_ts_tree(int n, int k0, int nb, const c2
     }
 }
 
-// The chosen rows as LAPACK interchanges, then the gather map of swap_map_wave.
-// Row sel[t] sits at step t where a chain of displacements put it: a row moves only
-// when it is displaced from top position k0 + i at step i, to pos_i; so starting at its
-// own row, p = pos_{p - k0} while p lies in [k0, k0 + t) (each step moves it further
-// down, so the chain ends).  Rows out of range or repeated raise *bad (no interchange).
+// The chosen rows as LAPACK interchanges and the gather map (the format of
+// swap_map_wave), in one wave.  Step t swaps position k0 + t with the position p of
+// row sel[t].  The rows that sit in the panel's top positions are always original top
+// rows; cur[i] tracks which one sits at k0 + i, loc / lslot where each chosen row sits
+// (and, for a position below the panel, which outside slot it is).  The outside
+// positions are the original rows of the chosen rows below the panel; each step moves
+// the row at k0 + t to p, and the last row moved to an outside position is its final
+// content.  Every step is a few lane reads and writes of registers.  Rows out of range
+// or repeated raise *bad (identity interchanges).
 __global__ __launch_bounds__(64) void k_ts_map(int n, int k0, int nb, const int* __restrict__ sel,
                                                int* __restrict__ piv, int2* __restrict__ map, int* __restrict__ count,
                                                int* __restrict__ bad) {
+    __shared__ int srow[kNB];
     const int t = threadIdx.x;
     const int s = t < nb ? sel[t] : -1;
+    srow[t] = s;
+    __syncthreads();
     bool ok = t >= nb || (s >= k0 && s < n);
-    for (int u = 0; u < nb; u++) {
-        const int o = __shfl(s, u, 64);
-        if (t < nb && u != t && o == s) ok = false;
-    }
-    int pos = k0 + t;
-    if (__all(ok)) {
-        for (int tt = 0; tt < nb; tt++) {
-            int p = __builtin_amdgcn_readlane(s, tt);
-            while (p >= k0 && p < k0 + tt) p = __builtin_amdgcn_readlane(pos, p - k0);
-            if (t == tt) pos = p;
+    for (int u = 0; u < nb; u++)
+        if (t < nb && u != t && srow[u] == s) ok = false;
+    if (!__all(ok)) {
+        if (t == 0) atomicExch(bad, 1);
+        if (t < nb) {
+            piv[k0 + t] = k0 + t;
+            map[t] = make_int2(k0 + t, k0 + t);
         }
-    } else if (t == 0) {
-        atomicExch(bad, 1);
+        if (t == 0) *count = nb;
+        return;
     }
-    if (t < nb) piv[k0 + t] = pos;
-    swap_map_wave(n, k0, nb, pos, map, count, bad);
+    const bool out = t < nb && s >= k0 + nb;
+    const unsigned long long ob = __ballot(out);
+    const int nout = __popcll(ob);
+    const int oidx = __popcll(ob & ((1ull << t) - 1ull));
+    // lane t as chosen index t: loc / lslot (where row sel[t] sits); as top position k0 + t:
+    // cur (the original top row there) and topsel (the chosen index of row k0 + t, or -1);
+    // as outside slot t: fin (the row that ends there) and orow (its position)
+    srow[t] = -1;
+    __syncthreads();
+    if (t < nb && !out) srow[s - k0] = t;
+    __syncthreads();
+    const int topsel = srow[t];
+    __syncthreads();
+    srow[t] = 0;
+    __syncthreads();
+    if (out) srow[oidx] = s;
+    __syncthreads();
+    const int orow = srow[t];
+    int cur = k0 + t, loc = s, lslot = out ? oidx : -1, fin = 0, mypiv = k0 + t;
+    for (int tt = 0; tt < nb; tt++) {
+        const int p = __builtin_amdgcn_readlane(loc, tt);
+        const int ls = __builtin_amdgcn_readlane(lslot, tt);
+        if (t == tt) mypiv = p;
+        if (p != k0 + tt) {   // the row at k0 + tt moves to p
+            const int x = __builtin_amdgcn_readlane(cur, tt);
+            if (ls < 0) {
+                if (t == p - k0) cur = x;
+            } else if (t == ls) {
+                fin = x;
+            }
+            const int t2 = __builtin_amdgcn_readlane(topsel, x - k0);
+            if (t2 >= 0 && t == t2) {
+                loc = p;
+                lslot = ls;
+            }
+        }
+    }
+    if (t < nb) {
+        piv[k0 + t] = mypiv;
+        map[t] = make_int2(k0 + t, s);
+    }
+    if (t < nout) map[nb + t] = make_int2(orow, fin);
+    if (t == 0) *count = nb + nout;
 }
 
 // The panel's interchanges on its own columns, with L11 \ U11 in the top rows.  One
@@ -1292,25 +1361,27 @@ __global__ __launch_bounds__(256) void k_ts_gather(int k0, int nb, c2<R>* __rest
 
 // L21 = A21 U11^{-1} for the rows below the panel's top block, in place: forward
 // substitution over the panel's columns.  A workgroup per 64 rows; lane (rq = lane %
-// 16, cg = lane / 16) of wave w holds row 16 w + rq in columns cg + 4 t (t < 16); U11
-// and its reciprocal diagonal in LDS.  Per column j the lanes of group j % 4 finish
-// x_j and store it, and pass it to the other three lanes of the row (no barrier).
+// 16, cg = lane / 16) of wave w holds row 16 w + rq; after T shifts v[t] holds column
+// cg + 4 (t + T), so column j is v[0] of the lanes of group j % 4, which finish x_j,
+// store it and pass it to the other lanes of the row (no barrier).  U11 (padded with
+// zeros) and its reciprocal diagonal in LDS.
+constexpr int kTsUPad = 80;   // U11 row in LDS: columns up to cg + 4 (3 + 15) of a live chunk
 template <typename R>
 __global__ __launch_bounds__(256) void k_ts_l21(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
                                                 const c2<R>* __restrict__ u11) {
-    __shared__ c2<R> Us[kNB][kNB];
+    __shared__ c2<R> Us[kNB][kTsUPad];
     __shared__ c2<R> dinv[kNB];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int rq = lane & 15, cg = lane >> 4;
-    for (int e = tid; e < nb * kNB; e += 256) {
-        const int j = e / kNB, c = e % kNB;
-        Us[j][c] = (c >= j && c < nb) ? u11[(size_t)j * kNB + c] : mk2<R>(R(0), R(0));
+    for (int e = tid; e < kNB * kTsUPad; e += 256) {
+        const int j = e / kTsUPad, c = e % kTsUPad;
+        Us[j][c] = (j < nb && c >= j && c < nb) ? u11[(size_t)j * kNB + c] : mk2<R>(R(0), R(0));
     }
     __syncthreads();
     if (tid < nb) {
-        const c2<R> d = Us[tid][tid];
-        const R den = norm2_fx<R>(d);
-        dinv[tid] = den > R(0) ? mk2<R>(d.x / den, -d.y / den) : mk2<R>(R(0), R(0));
+        const c2<R> dg = Us[tid][tid];
+        const R den = norm2_fx<R>(dg);
+        dinv[tid] = den > R(0) ? mk2<R>(dg.x / den, -dg.y / den) : mk2<R>(R(0), R(0));
     }
     __syncthreads();
     const long long d = (long long)k0 + nb + (long long)blockIdx.x * 64 + w * 16 + rq;
@@ -1321,19 +1392,27 @@ __global__ __launch_bounds__(256) void k_ts_l21(int n, int k0, int nb, c2<R>* __
         const int c = cg + 4 * t;
         v[t] = (valid && c < nb) ? A[(size_t)(k0 + c) * lda + d] : mk2<R>(R(0), R(0));
     }
+    int T = 0;   // shifts done
     for (int j = 0; j < nb; j++) {
         const int q = j & 3;
-        c2<R> xp;
-        TS_PICK16(xp, v, j >> 2);
-        const c2<R> xj = cmul_fx<R>(xp, dinv[j]);
+        const c2<R> xj = cmul_fx<R>(v[0], dinv[j]);
         if (cg == q && valid) A[(size_t)(k0 + j) * lda + d] = xj;
         const int src = rq + 16 * q;
         const c2<R> x = mk2<R>(__shfl(xj.x, src, 64), __shfl(xj.y, src, 64));
+        const c2<R>* ur = &Us[j][cg + 4 * T];
 #pragma unroll
-        for (int ch = 0; ch < 4; ch++)
-            if (16 * ch + 15 > j)
+        for (int k = 0; k < 4; k++)
+            if (4 * k + T < 16)
 #pragma unroll
-                for (int t = 4 * ch; t < 4 * ch + 4; t++) csub_mul<R>(v[t], x, Us[j][cg + 4 * t]);
+                for (int t = 4 * k; t < 4 * k + 4; t++) csub_mul<R>(v[t], x, ur[4 * t]);
+        if (q == 3) {   // columns 4 T .. 4 T + 3 done: shift by one
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (4 * k + T < 15)
+#pragma unroll
+                    for (int t = 4 * k; t < 4 * k + 4 && t < 15; t++) v[t] = v[t + 1];
+            T++;
+        }
     }
 }
 
@@ -1408,6 +1487,143 @@ static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n
                                          reinterpret_cast<rocblas_float_complex*>(Cc), ldc);
     if (st != rocblas_status_success) {
         *msg = "rocblas_cgemm_64 failed";
+        return hipErrorUnknown;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP32 trailing update on the FP16 matrix cores, to FP32-class accuracy.
+// C (m x n complex FP32) -= A (m x k) B (k x n) as one real product: C viewed as a
+// real 2m x n matrix (re, im rows interleaved), B as a real 2k x n matrix, and A as
+// the real 2m x 2k matrix whose column pair j holds A_j and i A_j
+// (rows re, im: [Ar, -Ai; Ai, Ar]).  Each real operand X is scaled by a power of two
+// s_X (from max |re|, |im|, found on the device) and split into FP16 parts,
+// X / s_X = X_hi + 2^-11 X_lo with X_hi = fp16(X / s_X), X_lo = fp16(2^11 (X / s_X - X_hi)).
+// Then C -= s_A s_B [A_hi B_hi + 2^-11 (A_hi B_lo + A_lo B_hi)] (A_lo B_lo, 2^-22 of the
+// product, is dropped): two FP16 GEMMs with FP32 accumulation, K = 2k and K = 4k over
+// [A_hi | A_lo] and [B_lo ; B_hi].  Errors are of FP32 size relative to max|A| max|B|
+// (normwise, as a scaled FP32 product).  mode 2 keeps only A_hi B_hi (FP16 accuracy).
+// Measured (rocBLAS gemm_ex, MI355X): 815 TFLOP/s at K = 1024 against 152 for the
+// complex FP32 GEMM of the same product.
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+__global__ void k_amax2(long long m, long long k, const float2* __restrict__ A, long long lda, long long k2,
+                        long long n, const float2* __restrict__ Bm, long long ldb, unsigned* __restrict__ out) {
+    // out[0] = max over A (m x k), out[1] = max over B (k2 x n) of |re|, |im| (non-negative float bits order as unsigned)
+    float a = 0.f, b = 0.f;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m * k; e += stride) {
+        const float2 v = A[(e / m) * lda + e % m];
+        a = fmaxf(a, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < k2 * n; e += stride) {
+        const float2 v = Bm[(e / k2) * ldb + e % k2];
+        b = fmaxf(b, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmaxf(a, __shfl_xor(a, off, 64));
+        b = fmaxf(b, __shfl_xor(b, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], __float_as_uint(a));   // NaN bits sort above every number: a NaN shows
+        atomicMax(&out[1], __float_as_uint(b));
+    }
+}
+
+__device__ __forceinline__ float g16_scale(unsigned amax_bits) {   // power of two >= max (1 for 0 or non-finite)
+    const float a = __uint_as_float(amax_bits);
+    if (!(a > 0.f) || !(a < 3.0e38f)) return 1.f;
+    int e;
+    frexpf(a, &e);   // a = f 2^e, f in [0.5, 1)
+    return ldexpf(1.f, e);
+}
+
+// A (m x k complex) -> H [2m][4k] column-major FP16: columns 0..2k-1 A_hi, 2k..4k-1 A_lo
+// (column pair j: A_j, i A_j); also the GEMMs' alphas and beta into par (device pointer mode).
+__global__ void k_split_a(long long m, long long k, const float2* __restrict__ A, long long lda,
+                          const unsigned* __restrict__ amax, _Float16* __restrict__ H, float* __restrict__ par) {
+    const float sa = g16_scale(amax[0]), sb = g16_scale(amax[1]), ia = 1.f / sa;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        par[0] = -sa * sb;
+        par[1] = -sa * sb * 0x1p-11f;
+        par[2] = 1.f;
+    }
+    const long long ldh = 2 * m, stride = (long long)gridDim.x * blockDim.x;
+    _Float16* L = H + ldh * 2 * k;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m * k; e += stride) {
+        const long long i = e % m, j = e / m;
+        const float2 v = A[j * lda + i];
+        const float xr = v.x * ia, xi = v.y * ia;
+        const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+        const _Float16 lr = (_Float16)((xr - (float)hr) * 2048.f), li = (_Float16)((xi - (float)hi) * 2048.f);
+        _Float16* c0 = H + (2 * j) * ldh + 2 * i;
+        _Float16* c1 = c0 + ldh;
+        c0[0] = hr; c0[1] = hi; c1[0] = -hi; c1[1] = hr;
+        _Float16* d0 = L + (2 * j) * ldh + 2 * i;
+        _Float16* d1 = d0 + ldh;
+        d0[0] = lr; d0[1] = li; d1[0] = -li; d1[1] = lr;
+    }
+}
+
+// B (k x n complex) -> G [n][4k] column-major FP16 (ld 4k): rows 0..2k-1 B_lo, 2k..4k-1 B_hi
+__global__ void k_split_b(long long k, long long n, const float2* __restrict__ Bm, long long ldb,
+                          const unsigned* __restrict__ amax, _Float16* __restrict__ G) {
+    const float ib = 1.f / g16_scale(amax[1]);
+    const long long ldg = 4 * k, stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < k * n; e += stride) {
+        const long long j = e % k, c = e / k;
+        const float2 v = Bm[c * ldb + j];
+        const float xr = v.x * ib, xi = v.y * ib;
+        const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+        _Float16* g = G + c * ldg + 2 * j;
+        g[0] = (_Float16)((xr - (float)hr) * 2048.f);
+        g[1] = (_Float16)((xi - (float)hi) * 2048.f);
+        g[2 * k] = hr;
+        g[2 * k + 1] = hi;
+    }
+}
+
+static hipError_t gemm_update_g16(LuWork* w, hipStream_t s, long long m, long long n, long long k, const float2* Aa,
+                                  long long lda, const float2* Bb, long long ldb, float2* Cc, long long ldc,
+                                  const char** msg) {
+    if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+    hipError_t e;
+    const size_t ha = (size_t)2 * m * 4 * k * sizeof(_Float16), hb = (size_t)4 * k * n * sizeof(_Float16);
+    const size_t need = 256 + ((ha + 255) & ~(size_t)255) + hb;
+    if (w->g16_cap < need) {
+        (void)hipFree(w->g16);
+        w->g16 = nullptr;
+        w->g16_cap = 0;
+        if ((e = hipMalloc(&w->g16, need)) != hipSuccess) return e;
+        w->g16_cap = need;
+    }
+    char* base = static_cast<char*>(w->g16);
+    unsigned* amax = reinterpret_cast<unsigned*>(base);
+    float* par = reinterpret_cast<float*>(base + 16);
+    _Float16* H = reinterpret_cast<_Float16*>(base + 256);
+    _Float16* G = reinterpret_cast<_Float16*>(base + 256 + ((ha + 255) & ~(size_t)255));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, amax, 2, 0u);   // a kernel, not a memset: capturable
+    hipLaunchKernelGGL(k_amax2, dim3(1024), dim3(256), 0, s, m, k, Aa, lda, k, n, Bb, ldb, amax);
+    hipLaunchKernelGGL(k_split_a, dim3(2048), dim3(256), 0, s, m, k, Aa, lda, (const unsigned*)amax, H, par);
+    hipLaunchKernelGGL(k_split_b, dim3(2048), dim3(256), 0, s, k, n, Bb, ldb, (const unsigned*)amax, G);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    rocblas_set_stream(w->blas, s);
+    rocblas_set_pointer_mode(w->blas, rocblas_pointer_mode_device);
+    const rocblas_datatype h16 = rocblas_datatype_f16_r, f32 = rocblas_datatype_f32_r;
+    // hi x hi, K = 2k: H columns 0..2k-1, G rows 2k..4k-1
+    rocblas_status st = rocblas_gemm_ex(w->blas, rocblas_operation_none, rocblas_operation_none, 2 * m, n, 2 * k, par,
+                                        H, h16, 2 * m, G + 2 * k, h16, 4 * k, par + 2, Cc, f32, 2 * ldc, Cc, f32,
+                                        2 * ldc, f32, rocblas_gemm_algo_standard, 0, 0);
+    if (st == rocblas_status_success && w->gemm16 == 1)   // cross terms, K = 4k: [A_hi | A_lo] [B_lo ; B_hi]
+        st = rocblas_gemm_ex(w->blas, rocblas_operation_none, rocblas_operation_none, 2 * m, n, 4 * k, par + 1, H, h16,
+                             2 * m, G, h16, 4 * k, par + 2, Cc, f32, 2 * ldc, Cc, f32, 2 * ldc, f32,
+                             rocblas_gemm_algo_standard, 0, 0);
+    rocblas_set_pointer_mode(w->blas, rocblas_pointer_mode_host);
+    if (st != rocblas_status_success) {
+        *msg = "rocblas_gemm_ex (FP16 split) failed";
         return hipErrorUnknown;
     }
     return hipGetLastError();
@@ -1607,7 +1823,8 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // measured (MI355X, random complex matrices): at order 40,000 outer blocks of 64 / 128 /
     // 256 / 512 columns take 3.92 / 3.44 / 3.20 / 3.08 s (FP64) and 2.15 / 1.98 / 1.89 / 1.83 s
     // (FP32); at 12,000 all are within 3%, at 6,000 one level is 10% faster
-    const int outer = w->outer_nb > 0 ? w->outer_nb : (n >= kTwoLevelMinN ? 512 : kNB);
+    const int two_level_from = (sizeof(R) == 4 && w->gemm16 != 0) ? kTwoLevelMinN32 : kTwoLevelMinN;
+    const int outer = w->outer_nb > 0 ? w->outer_nb : (n >= two_level_from ? 512 : kNB);
     if (outer > kNB) {
         // Two-level blocking.  An outer block of outer_nb columns is factored as 64-column
         // panels whose updates stay inside the block (thin GEMMs); the rest of the matrix
@@ -1643,8 +1860,17 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                     if (e != hipSuccess) return e;
                 }
             }
-            e = gemm_update(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda, A + (size_t)KE * lda + K0,
-                            lda, A + (size_t)KE * lda + KE, lda, msg);
+            if constexpr (sizeof(R) == 4) {
+                if (w->gemm16 != 0)   // FP32 factors: the rank-outer update on the FP16 matrix cores
+                    e = gemm_update_g16(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda,
+                                        A + (size_t)KE * lda + K0, lda, A + (size_t)KE * lda + KE, lda, msg);
+                else
+                    e = gemm_update(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda,
+                                    A + (size_t)KE * lda + K0, lda, A + (size_t)KE * lda + KE, lda, msg);
+            } else {
+                e = gemm_update(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda,
+                                A + (size_t)KE * lda + K0, lda, A + (size_t)KE * lda + KE, lda, msg);
+            }
             if (e != hipSuccess) return e;
         }
         return lu_finish(s, w, info, msg);
