@@ -12,8 +12,6 @@ import bielcc_amd as B
 import _tournament as T
 
 pytestmark = pytest.mark.gpu
-
-
 def _rand(n, seed):
     rng = np.random.default_rng(seed)
     return rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
