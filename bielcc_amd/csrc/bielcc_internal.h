@@ -293,6 +293,8 @@ void lu_set_tournament(LuWork*, int mode);   // panels by tournament pivoting: 1
 int lu_get_tournament(const LuWork*);
 void lu_set_gemm16(LuWork*, int mode);       // FP32 LU rank-512 updates on FP16 matrix cores: 1 split, 2 FP16, 0 off
 int lu_get_gemm16(const LuWork*);
+void lu_set_look_ahead2(LuWork*, int mode);  // two-level LU: next outer block on the side stream (1) or not (0)
+int lu_get_look_ahead2(const LuWork*);
 int lu_get_panel_rows32(const LuWork*);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);

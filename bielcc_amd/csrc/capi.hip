@@ -826,6 +826,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_gemm16(ctx->lu, value);
         return BIELCC_OK;
     }
+    if (k == "lu_look_ahead2") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "lu_look_ahead2 must be 0 or 1");
+        lu_set_look_ahead2(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "lu_panel_wait_us") {
         if (value < 0) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_wait_us must be >= 0");
         lu_set_panel_wait_us(ctx->lu, value);
@@ -879,6 +884,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_panel_rows32") *value = lu_get_panel_rows32(ctx->lu);
     else if (k == "lu_tournament") *value = lu_get_tournament(ctx->lu);
     else if (k == "lu32_gemm16") *value = lu_get_gemm16(ctx->lu);
+    else if (k == "lu_look_ahead2") *value = lu_get_look_ahead2(ctx->lu);
     else if (k == "mixed_fixup") *value = ctx->mixed_fixup ? 1 : 0;
     else if (k == "mixed_fixup_points") *value = (int)std::min<long long>(ctx->fixup_points, INT_MAX);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;

@@ -106,6 +106,10 @@ struct LuWork {
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
     int panel_lds = 1;              // whole-panel kernel: 1 within the workgroup budget, 0 never, 2 always
     int tournament = 0;             // panels by tournament pivoting (k_ts_tree): 1 always, 0 never
+    int look_ahead2 = 0;            // two-level LU: the next outer block factored on the side stream during
+                                    // the bulk of the rank-512 update (tournament panels): 1 on, 0 off
+    int2* bmaps = nullptr;          // that block's panels' interchange maps (8 x (2 kNB entries + count, bad))
+    rocblas_handle blas_side = nullptr;   // rocBLAS handle of the side stream
     int gemm16 = 1;                 // FP32 LU's rank-512 updates on the FP16 matrix cores: 1 split (FP32-class),
                                     // 2 FP16 only, 0 complex FP32 GEMM.  Split, order 40,000: 1.81 -> 1.20 s,
                                     // backward error 1.8e-4 -> 1.4e-4; FP16 only: 0.81 s, backward error 0.47
@@ -144,6 +148,8 @@ void lu_set_look_ahead(LuWork* w, int mode) { w->look_ahead = mode; }
 void lu_set_panel_lds(LuWork* w, int mode) { w->panel_lds = mode; }
 void lu_set_tournament(LuWork* w, int mode) { w->tournament = mode; }
 void lu_set_gemm16(LuWork* w, int mode) { w->gemm16 = mode; }
+void lu_set_look_ahead2(LuWork* w, int mode) { w->look_ahead2 = mode; }
+int lu_get_look_ahead2(const LuWork* w) { return w->look_ahead2; }
 int lu_get_gemm16(const LuWork* w) { return w->gemm16; }
 int lu_get_tournament(const LuWork* w) { return w->tournament; }
 void lu_set_panel_rows32(LuWork* w, int rows) { w->panel_rows32 = rows; }
@@ -219,7 +225,8 @@ void lu_work_destroy(LuWork* w) {
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
     (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->dd);
     (void)hipFree(w->step_count);
-    (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap); (void)hipFree(w->tbuf);
+    (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap); (void)hipFree(w->tbuf); (void)hipFree(w->bmaps);
+    if (w->blas_side) rocblas_destroy_handle(w->blas_side);
     (void)hipFree(w->g16);
     if (w->ev_upd) (void)hipEventDestroy(w->ev_upd);
     if (w->ev_panel) (void)hipEventDestroy(w->ev_panel);
@@ -1462,9 +1469,10 @@ static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n
                               long long lda, const double2* Bb, long long ldb, double2* Cc, long long ldc,
                               const char** msg) {
     if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
-    rocblas_set_stream(w->blas, s);
+    rocblas_handle h = (s == w->side && w->blas_side) ? w->blas_side : w->blas;
+    rocblas_set_stream(h, s);
     rocblas_double_complex alpha = {-1., 0.}, beta = {1., 0.};
-    rocblas_status st = rocblas_zgemm_64(w->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
+    rocblas_status st = rocblas_zgemm_64(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
                                          reinterpret_cast<const rocblas_double_complex*>(Aa), lda,
                                          reinterpret_cast<const rocblas_double_complex*>(Bb), ldb, &beta,
                                          reinterpret_cast<rocblas_double_complex*>(Cc), ldc);
@@ -1479,9 +1487,10 @@ static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n
                               long long lda, const float2* Bb, long long ldb, float2* Cc, long long ldc,
                               const char** msg) {
     if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
-    rocblas_set_stream(w->blas, s);
+    rocblas_handle h = (s == w->side && w->blas_side) ? w->blas_side : w->blas;
+    rocblas_set_stream(h, s);
     rocblas_float_complex alpha = {-1.f, 0.f}, beta = {1.f, 0.f};
-    rocblas_status st = rocblas_cgemm_64(w->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
+    rocblas_status st = rocblas_cgemm_64(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
                                          reinterpret_cast<const rocblas_float_complex*>(Aa), lda,
                                          reinterpret_cast<const rocblas_float_complex*>(Bb), ldb, &beta,
                                          reinterpret_cast<rocblas_float_complex*>(Cc), ldc);
@@ -1743,8 +1752,19 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // rows per workgroup below 64 gmax rows, where their workgroup count goes up.
     auto reserve_for = [&](int m) { return std::min((m + 63) / 64, gmax); };
     const bool tourn = w->tournament == 1;
+    const int two_level_from = (sizeof(R) == 4 && w->gemm16 != 0) ? kTwoLevelMinN32 : kTwoLevelMinN;
+    const int outer = w->outer_nb > 0 ? w->outer_nb : (n >= two_level_from ? 512 : kNB);
+    const bool la2 = outer > kNB && w->look_ahead2 == 1 && w->side != nullptr && n > 2 * outer;
     TsBufs tb{};
-    if (tourn && (e = ts_bufs(w, n, &tb)) != hipSuccess) return e;
+    if ((tourn || la2) && (e = ts_bufs(w, n, &tb)) != hipSuccess) return e;
+    if (la2 && !w->bmaps && (e = hipMalloc(&w->bmaps, 8 * (2 * kNB + 1) * sizeof(int2))) != hipSuccess) return e;
+    if (la2 && !w->blas_side) {
+        if (rocblas_create_handle(&w->blas_side) != rocblas_status_success) {
+            *msg = "rocblas_create_handle failed";
+            return hipErrorUnknown;
+        }
+        rocblas_set_atomics_mode(w->blas_side, w->blas_atomics ? rocblas_atomics_allowed : rocblas_atomics_not_allowed);
+    }
     const bool lds_allowed = !tourn && w->panel_lds != 0 && w->n_cu > 0;
     bool lds = lds_allowed && (force || admit.take(w->dev, reserve_for(n), 2 * w->n_cu));
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
@@ -1756,19 +1776,21 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     int* smap_count = reinterpret_cast<int*>(w->smap + 2 * kNB);
     // panel [k0, k0 + nb) on stream st: pivot, interchange inside the panel, scale,
     // rank-1 update of the panel's remaining columns
-    auto factor_panel = [&](hipStream_t st, int k0) {
+    auto factor_panel = [&](hipStream_t st, int k0, bool ts = false, int2* map = nullptr) {
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
-        if (tourn) {   // tournament pivoting: tree, interchanges, the panel's columns, L21
+        if (!map) map = w->smap;
+        int* mcount = reinterpret_cast<int*>(map + 2 * kNB);
+        int* bad = smap_count + 1;   // one flag for every map
+        if (tourn || ts) {   // tournament pivoting: tree, interchanges, the panel's columns, L21
             const int GT = (n - k0 + kTsRows - 1) / kTsRows;
             c2<R>* u11 = static_cast<c2<R>*>(tb.u11);
             c2<R>* lbuf = static_cast<c2<R>*>(tb.lbuf);
             hipLaunchKernelGGL(k_ts_tree<R>, dim3(GT), dim3(256), 0, st, n, k0, nb, (const c2<R>*)A, lda, tb.clist,
                                tb.ctr, tb.half, tb.sel, tb.sslot, u11, lbuf, w->info);
-            hipLaunchKernelGGL(k_ts_map, dim3(1), dim3(64), 0, st, n, k0, nb, tb.sel, piv, w->smap, smap_count,
-                               smap_count + 1);
-            hipLaunchKernelGGL(k_ts_gather<R>, dim3((nb + 3) / 4), dim3(256), 0, st, k0, nb, A, lda, w->smap,
-                               smap_count, tb.sslot, (const c2<R>*)u11, (const c2<R>*)lbuf);
+            hipLaunchKernelGGL(k_ts_map, dim3(1), dim3(64), 0, st, n, k0, nb, tb.sel, piv, map, mcount, bad);
+            hipLaunchKernelGGL(k_ts_gather<R>, dim3((nb + 3) / 4), dim3(256), 0, st, k0, nb, A, lda, map, mcount,
+                               tb.sslot, (const c2<R>*)u11, (const c2<R>*)lbuf);
             if (n - k0 > nb)
                 hipLaunchKernelGGL(k_ts_l21<R>, dim3((n - k0 - nb + 63) / 64), dim3(256), 0, st, n, k0, nb, A, lda,
                                    (const c2<R>*)u11);
@@ -1786,8 +1808,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
             else
                 hipLaunchKernelGGL((k_panel_lds<R, 128>), dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv,
                                    w->info, w->psync, cval, cidx, crow, rowj, wait_ticks);
-            hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, w->smap, smap_count,
-                               smap_count + 1);
+            hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, map, mcount, bad);
             return;
         }
         int nparts = 0;
@@ -1813,7 +1834,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                                        w->pval, w->pidx);
             }
         }
-        hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, w->smap, smap_count, smap_count + 1);
+        hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, map, mcount, bad);
     };
     // Look-ahead: after panel k0's interchanges and U12, the trailing update is split.
     // The next panel's columns [c1, c2) are updated first; that panel is then factored on
@@ -1823,8 +1844,6 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // measured (MI355X, random complex matrices): at order 40,000 outer blocks of 64 / 128 /
     // 256 / 512 columns take 3.92 / 3.44 / 3.20 / 3.08 s (FP64) and 2.15 / 1.98 / 1.89 / 1.83 s
     // (FP32); at 12,000 all are within 3%, at 6,000 one level is 10% faster
-    const int two_level_from = (sizeof(R) == 4 && w->gemm16 != 0) ? kTwoLevelMinN32 : kTwoLevelMinN;
-    const int outer = w->outer_nb > 0 ? w->outer_nb : (n >= two_level_from ? 512 : kNB);
     if (outer > kNB) {
         // Two-level blocking.  An outer block of outer_nb columns is factored as 64-column
         // panels whose updates stay inside the block (thin GEMMs); the rest of the matrix
@@ -1832,21 +1851,59 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         // outer_nb, which runs near the matrix-core peak where rank 64 does not (ZGEMM,
         // 20,000 x 20,000: 50 TFLOP/s at rank 64, 70 at 256).  Each panel's interchanges
         // are applied to every other column at once, as in the one-level loop.
-        for (int K0 = 0; K0 < n; K0 += outer) {
-            const int KE = min(n, K0 + outer);
-            for (int k0 = K0; k0 < KE; k0 += kNB) {
+        //
+        // Look-ahead (lu_look_ahead2): the rank-outer update is split.  The next block's columns
+        // [KE, KE2) are updated first; that block's panels are then factored on the side stream
+        // (tournament panels: they never wait for other workgroups, so they run beside the GEMM)
+        // while the rest of the update, columns [KE2, n), runs on s.  The side stream applies
+        // each panel's interchanges inside [KE, KE2) only: the GEMM still reads the rows of
+        // L21 (columns [K0, KE)) and writes columns from KE2 on.  Once both are done, s applies
+        // the block's interchanges to the other columns, panel by panel, from the saved maps.
+        auto big_update = [&](int K0, int KE, int c0, int c1) -> hipError_t {   // A[KE:, c0:c1] -= L21 U12
+            if constexpr (sizeof(R) == 4) {
+                if (w->gemm16 != 0)   // FP32 factors: the rank-outer update on the FP16 matrix cores
+                    return gemm_update_g16(w, s, n - KE, c1 - c0, KE - K0, A + (size_t)K0 * lda + KE, lda,
+                                           A + (size_t)c0 * lda + K0, lda, A + (size_t)c0 * lda + KE, lda, msg);
+            }
+            return gemm_update(w, s, n - KE, c1 - c0, KE - K0, A + (size_t)K0 * lda + KE, lda,
+                               A + (size_t)c0 * lda + K0, lda, A + (size_t)c0 * lda + KE, lda, msg);
+        };
+        // the panels of block [B0, BE) on stream st; interchanges applied to columns [cb, ce)
+        // outside each panel (maps kept in w->bmaps when keep)
+        auto block_panels = [&](hipStream_t st, int B0, int BE, int cb, int ce, bool ts, bool keep) -> hipError_t {
+            for (int k0 = B0; k0 < BE; k0 += kNB) {
                 const int nb = min(kNB, n - k0);
                 const int c1 = k0 + nb;
-                factor_panel(s, k0);
-                if ((e = hipGetLastError()) != hipSuccess) return e;
-                if (k0 + (n - c1) > 0)
-                    hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + (n - c1) + 3) / 4), dim3(256), 0, s, k0, c1, n, A,
-                                       lda, w->smap, smap_count);
-                if (c1 < KE) {   // the block's remaining columns
-                    if ((e = panel_trsm<R>(w, s, nb, A, lda, k0, c1, KE, msg)) != hipSuccess) return e;
-                    e = gemm_update(w, s, n - c1, KE - c1, nb, A + (size_t)k0 * lda + c1, lda,
-                                    A + (size_t)c1 * lda + k0, lda, A + (size_t)c1 * lda + c1, lda, msg);
-                    if (e != hipSuccess) return e;
+                int2* map = keep ? w->bmaps + (size_t)((k0 - B0) / kNB) * (2 * kNB + 1) : w->smap;
+                factor_panel(st, k0, ts, map);
+                hipError_t e2;
+                if ((e2 = hipGetLastError()) != hipSuccess) return e2;
+                const int ncol = (k0 - cb) + (ce - c1);
+                if (ncol > 0)
+                    hipLaunchKernelGGL(k_row_gather<R>, dim3((ncol + 3) / 4), dim3(256), 0, st, k0 - cb, c1 - cb,
+                                       ce - cb, A + (size_t)cb * lda, lda, map,
+                                       reinterpret_cast<const int*>(map + 2 * kNB));
+                if (c1 < BE) {   // the block's remaining columns
+                    if ((e2 = panel_trsm<R>(w, st, nb, A, lda, k0, c1, BE, msg)) != hipSuccess) return e2;
+                    e2 = gemm_update(w, st, n - c1, BE - c1, nb, A + (size_t)k0 * lda + c1, lda,
+                                     A + (size_t)c1 * lda + k0, lda, A + (size_t)c1 * lda + c1, lda, msg);
+                    if (e2 != hipSuccess) return e2;
+                }
+            }
+            return hipSuccess;
+        };
+        bool pending = false;   // block [K0, KE) was factored on the side stream
+        for (int K0 = 0; K0 < n; K0 += outer) {
+            const int KE = min(n, K0 + outer);
+            if (!pending) {
+                if ((e = block_panels(s, K0, KE, 0, n, false, false)) != hipSuccess) return e;
+            } else {
+                if ((e = hipStreamWaitEvent(s, w->ev_panel, 0)) != hipSuccess) return e;
+                for (int k0 = K0; k0 < KE; k0 += kNB) {   // its interchanges outside [K0, KE), in panel order
+                    const int2* map = w->bmaps + (size_t)((k0 - K0) / kNB) * (2 * kNB + 1);
+                    if (K0 + (n - KE) > 0)
+                        hipLaunchKernelGGL(k_row_gather<R>, dim3((K0 + (n - KE) + 3) / 4), dim3(256), 0, s, K0, KE, n,
+                                           A, lda, map, reinterpret_cast<const int*>(map + 2 * kNB));
                 }
             }
             if (KE >= n) break;
@@ -1860,18 +1917,19 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                     if (e != hipSuccess) return e;
                 }
             }
-            if constexpr (sizeof(R) == 4) {
-                if (w->gemm16 != 0)   // FP32 factors: the rank-outer update on the FP16 matrix cores
-                    e = gemm_update_g16(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda,
-                                        A + (size_t)KE * lda + K0, lda, A + (size_t)KE * lda + KE, lda, msg);
-                else
-                    e = gemm_update(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda,
-                                    A + (size_t)KE * lda + K0, lda, A + (size_t)KE * lda + KE, lda, msg);
+            const int KE2 = min(n, KE + outer);
+            if (la2 && KE2 < n) {
+                if ((e = big_update(K0, KE, KE, KE2)) != hipSuccess) return e;
+                if ((e = hipEventRecord(w->ev_upd, s)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(w->side, w->ev_upd, 0)) != hipSuccess) return e;
+                if ((e = block_panels(w->side, KE, KE2, KE, KE2, true, true)) != hipSuccess) return e;
+                if ((e = hipEventRecord(w->ev_panel, w->side)) != hipSuccess) return e;
+                if ((e = big_update(K0, KE, KE2, n)) != hipSuccess) return e;
+                pending = true;
             } else {
-                e = gemm_update(w, s, n - KE, n - KE, KE - K0, A + (size_t)K0 * lda + KE, lda,
-                                A + (size_t)KE * lda + K0, lda, A + (size_t)KE * lda + KE, lda, msg);
+                if ((e = big_update(K0, KE, KE, n)) != hipSuccess) return e;
+                pending = false;
             }
-            if (e != hipSuccess) return e;
         }
         return lu_finish(s, w, info, msg);
     }

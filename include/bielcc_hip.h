@@ -438,7 +438,18 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * one up to order 6,144 and two above (2); "lu_outer_nb" [0] = columns
  * per outer block of the two-level LU (64, 128, 256 or 512): the panels'
  * updates stay inside the block and the rest of the matrix gets one update of
- * that rank (64: one level; 0: one level below order 16,000, 512 from there);
+ * that rank (64: one level; 0: one level below order 16,000, 512 from there;
+ * for FP32 LUs with "lu32_gemm16" on, 512 from order 10,000);
+ * "lu32_gemm16" [1] = the FP32 LU's rank-512 updates run on the FP16 matrix
+ * cores with split operands (each FP32 operand as an FP16 high part plus a
+ * 2^-11-scaled FP16 low part, power-of-two scaled; errors of FP32 size);
+ * 2: FP16 high parts only (fast, FP16-sized errors; experiments); 0: the
+ * complex FP32 GEMM; "lu_tournament" [0] = 1: LU panels by tournament
+ * pivoting (blocks of 128 rows factored in registers, candidates merged up a
+ * binary tree; no launch waits for another workgroup, pivots differ from
+ * partial pivoting; 0: partial pivoting); "lu_look_ahead2" [0] = 1: in the
+ * two-level LU the next outer block is factored on a side stream (tournament
+ * panels) while the bulk of the rank-512 update runs;
  * "lu_trsm_lib" [2] = the factorisation's U12 triangles on the library's
  * lane-substitution kernel, one row per lane (1: rocBLAS trsm; 0: the
  * library's LDS sweep); "lu_look_ahead" [2] = with a one-level LU, the

@@ -157,3 +157,22 @@ def test_zero_pivot_reported(ctx):
             assert e.value.code == 5
         finally:
             ctx.set_tuning("lu_tournament", old)
+
+
+@pytest.mark.parametrize("n", [1500, 3500])
+def test_two_level_look_ahead(ctx, n):
+    """lu_look_ahead2: the next outer block factored on the side stream (tournament
+    panels) during the bulk of the rank-outer update, its interchanges outside the
+    block applied afterwards.  With tournament panels throughout it takes the pivots
+    of the LU without look-ahead; with partial pivoting elsewhere it reaches the same
+    backward error; FP64 and FP32 (FP16 split updates)."""
+    A = _rand(n, 500 + n)
+    for outer in (128, 512):
+        ref = _factor(ctx, A, lu_outer_nb=outer, lu_look_ahead2=0)
+        la = _factor(ctx, A, lu_outer_nb=outer, lu_look_ahead2=1)
+        assert np.array_equal(la[1], ref[1]), outer
+        assert np.abs(la[0] - ref[0]).max() <= 1e-14 * n * np.abs(ref[0]).max(), outer
+        mixed = _factor(ctx, A, lu_outer_nb=outer, lu_look_ahead2=1, lu_tournament=0)
+        assert _probe_backward(A, *mixed) <= 1e-16 * n, outer
+        m32 = _factor(ctx, A, fp32=True, lu_outer_nb=outer, lu_look_ahead2=1, lu_tournament=0)
+        assert _probe_backward(A, *m32) <= 1e-7 * n, outer
