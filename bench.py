@@ -858,10 +858,6 @@ def serving_leg(nq=32, workers=4, refine_tol=1e-10, windows_path=None, nq_stream
     capacity (latency).  The plain FP64 form runs every plain_every-th query."""
     from bielcc_amd import serve
     pool = serving_pool()
-    import bielcc_amd as B
-    wctx = B.Context(0)
-    _warm_lu32(wctx)   # the library's kernels for both forms' LUs are loaded once per process, before the runs
-    wctx.close()
     cells = [v.shape[0] for v in pool.values()]
     out = {"workload": f"{len(pool)} meshes ({min(cells)}-{max(cells)} cells: spheres, ellipsoids, tori, cubes), "
                        f"{nq} queries, ka in [1,15] on 8 levels, 1-16 incidences, 1,000-100,000 points at "

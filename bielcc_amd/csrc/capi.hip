@@ -55,6 +55,7 @@ int bielcc_ctx_destroy(bielcc_ctx ctx) {
         if (ctx->opmat_part) (void)hipFree(ctx->opmat_part);
         if (ctx->small) (void)hipFree(ctx->small);
         if (ctx->fix) (void)hipFree(ctx->fix);
+        if (ctx->probe) (void)hipFree(ctx->probe);
         lu_work_destroy(ctx->lu);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
@@ -507,6 +508,55 @@ static int mixed_fixup(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, 
     return BIELCC_OK;
 }
 
+// The share of points a mixed field would send back to FP64 sums, estimated on every
+// stride-th point: their FP32 field (without the recomputation) and the FP32 error model.
+// When the currents carry large parts whose fields cancel (near a resonance), most points
+// fail the model, and the FP32 pass plus the recomputation costs more than FP64 sums
+// alone (sphere of 20,000 quads at ka = 10: 262 ms FP32 + about 340 ms recomputed, against
+// 347 ms in FP64).
+constexpr int kProbePoints = 1024;
+static int mixed_probe(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im, int nrhs,
+                       const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr, double mHi,
+                       const double* inc, int near_mode, int M, const double* pts_d, bool has_h, double* frac) {
+    DevMesh* m = find_mesh(ctx, mesh_id);
+    const int stride = std::max(1, M / kProbePoints);
+    const int Ms = (M + stride - 1) / stride;
+    const size_t list_b = align256((size_t)Ms * sizeof(int)), pts_b = align256((size_t)3 * Ms * sizeof(double));
+    const size_t f_b = align256((size_t)nrhs * Ms * 6 * sizeof(double)), flag_b = align256((size_t)nrhs * Ms * sizeof(int));
+    const size_t mx_b = align256((size_t)2 * nrhs * sizeof(double)), inc_b = align256((size_t)nrhs * 6 * sizeof(double));
+    HIPCHK(ensure(ctx->probe, ctx->probe_cap, list_b * 2 + pts_b + f_b * 2 + flag_b + mx_b + inc_b + 256));
+    char* b = static_cast<char*>(ctx->probe);
+    int* idx = reinterpret_cast<int*>(b);
+    int* list = reinterpret_cast<int*>(b + list_b);
+    double* sub = reinterpret_cast<double*>(b + 2 * list_b);
+    double* Es = reinterpret_cast<double*>(b + 2 * list_b + pts_b);
+    double* Hs = has_h ? Es + f_b / sizeof(double) : nullptr;
+    int* flags = reinterpret_cast<int*>(b + 2 * list_b + pts_b + 2 * f_b);
+    double* mx = reinterpret_cast<double*>(b + 2 * list_b + pts_b + 2 * f_b + flag_b);
+    double* inc_d = reinterpret_cast<double*>(b + 2 * list_b + pts_b + 2 * f_b + flag_b + mx_b);
+    int* count = reinterpret_cast<int*>(b + 2 * list_b + pts_b + 2 * f_b + flag_b + mx_b + inc_b);
+    std::vector<int> hidx(Ms);
+    for (int i = 0; i < Ms; i++) hidx[i] = i * stride;
+    HIPCHK(hipMemcpyAsync(idx, hidx.data(), (size_t)Ms * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    if (inc) HIPCHK(hipMemcpyAsync(inc_d, inc, (size_t)nrhs * 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_fix_gather(ctx->stream, Ms, idx, pts_d, sub));
+    ctx->in_probe = true;
+    const int rc = field_impl(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, jM_d, mEr, mEi, mHr, mHi, inc, near_mode,
+                              BIELCC_PREC_MIXED, Ms, sub, Es, Hs);
+    ctx->in_probe = false;
+    if (rc) return rc;
+    FixArgs f;
+    f.M = Ms; f.N = m->n; f.nrhs = nrhs; f.pts = sub; f.ctab = m->ctab; f.jE = jE_d; f.jM = jM_d; f.k_re = k_re;
+    f.mE2 = mEr * mEr + mEi * mEi; f.mH2 = mHr * mHr + mHi * mHi; f.inc = inc ? inc_d : nullptr;
+    f.E = Es; f.H = Hs; f.mx = mx; f.flags = flags;
+    HIPCHK(launch_fix_flags(ctx->stream, f, count, list));
+    int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *frac = (double)cnt / Ms;
+    return BIELCC_OK;
+}
+
 static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, double k_re, double k_im,
                       int nrhs, const double* jE_d, const double* jM_d, double mEr, double mEi, double mHr,
                       double mHi, const double* inc, int near_mode, int precision, int M, const double* pts_d,
@@ -527,6 +577,17 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
         return fail(ctx, BIELCC_ERR_UNSUPPORTED,
                     "field driver needs p_max = 1, n_start <= 22 and analytic_calc > 1 (use bielcc_op_batch)");
     DeviceGuard g(ctx->device);
+    if (precision == BIELCC_PREC_MIXED && ctx->mixed_fixup && ctx->mixed_probe_pct > 0 && !ctx->capturing &&
+        !ctx->in_probe && M >= 4 * kProbePoints) {
+        double frac = 0.;
+        if (int rc = mixed_probe(ctx, mesh_id, np, k_re, k_im, nrhs, jE_d, jM_d, mEr, mEi, mHr, mHi, inc, near_mode,
+                                 M, pts_d, H_d != nullptr, &frac))
+            return rc;
+        if (frac * 100. > ctx->mixed_probe_pct) {
+            precision = BIELCC_PREC_FP64;
+            ctx->probe_fp64++;
+        }
+    }
     if (int rc = ensure_tables(ctx, *m, np->n_start, 0)) return rc;
     if (precision == BIELCC_PREC_MIXED)
         if (int rc = ensure_tables32(ctx, *m, np->n_start)) return rc;
@@ -558,7 +619,7 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
     double* inc_d = reinterpret_cast<double*>(base + part_b + cnt_b + off_b);
     long long* tot_d = reinterpret_cast<long long*>(base + part_b + cnt_b + off_b + inc_b);
     if (inc) HIPCHK(hipMemcpyAsync(inc_d, inc, (size_t)nrhs * 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    const bool fixup = precision == BIELCC_PREC_MIXED;
+    const bool fixup = precision == BIELCC_PREC_MIXED && !ctx->in_probe;
     if (nrhs >= ctx->opmat_min_rhs && !jM_d) {
         if (int rc = field_opmat(ctx, *m, np, k_re, k_im, nrhs, jE_d, mEr, mEi, inc ? inc_d : nullptr, near_mode,
                                  precision, M, pts_d, E_d, H_d, counts, offs, tot_d))
@@ -801,6 +862,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         lu_set_blas_atomics(ctx->lu, value == 1);
         return BIELCC_OK;
     }
+    if (k == "mixed_probe_pct") {
+        if (value < 0 || value > 100) return fail(ctx, BIELCC_ERR_ARG, "mixed_probe_pct must be 0..100");
+        ctx->mixed_probe_pct = value;
+        return BIELCC_OK;
+    }
     if (k == "mixed_fixup") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "mixed_fixup must be 0 or 1");
         ctx->mixed_fixup = value == 1;
@@ -887,6 +953,8 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_look_ahead2") *value = lu_get_look_ahead2(ctx->lu);
     else if (k == "mixed_fixup") *value = ctx->mixed_fixup ? 1 : 0;
     else if (k == "mixed_fixup_points") *value = (int)std::min<long long>(ctx->fixup_points, INT_MAX);
+    else if (k == "mixed_probe_pct") *value = ctx->mixed_probe_pct;
+    else if (k == "mixed_probe_fp64") *value = (int)std::min<long long>(ctx->probe_fp64, INT_MAX);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu);
@@ -913,7 +981,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "n_cu") *value = ctx->n_cu;
     else if (k == "scratch_kb") {   // the context's grow-only device buffers, KiB rounded up
         size_t b = ctx->scratch_cap + ctx->lists_cap + ctx->near_cap + ctx->opmat_ws_cap + ctx->opmat_meta_cap +
-                   ctx->opmat_part_cap + ctx->small_cap + ctx->fix_cap + lu_work_bytes(ctx->lu);
+                   ctx->opmat_part_cap + ctx->small_cap + ctx->fix_cap + ctx->probe_cap + lu_work_bytes(ctx->lu);
         for (const auto& kv : ctx->meshes)   // the meshes' kept near-pair lists
             if (kv.second.near_tot > 0) b += (size_t)kv.second.near_tot * 2 * sizeof(int);
         *value = (int)std::min<size_t>((b + 1023) >> 10, (size_t)INT_MAX);

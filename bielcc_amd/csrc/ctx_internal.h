@@ -45,6 +45,12 @@ struct bielcc_ctx_s {
     size_t fix_cap = 0;
     bool mixed_fixup = true;      // mixed field: points the FP32 error model does not clear are recomputed in FP64
     long long fixup_points = 0;   // points so recomputed, cumulative
+    int mixed_probe_pct = 25;     // mixed field: if more than this % of a strided sample of the points would go
+                                  // back to FP64 sums, the whole field runs in FP64 (0: no sample)
+    bool in_probe = false;        // the field driver is computing that sample
+    void* probe = nullptr;        // the sample's points, fields, flags (grow-only)
+    size_t probe_cap = 0;
+    long long probe_fp64 = 0;     // mixed field calls run in FP64 after the sample, cumulative
     int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     // plan (multi-mesh) tunables: cells per field tile and per assembly tile,
     // persistent workgroups per CU, operator-matrix workspace cap (MiB), and the

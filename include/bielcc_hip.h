@@ -469,7 +469,16 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
- * bielcc_get_tuning also reads
+ * Mixed fields: "mixed_fixup" [1] = after the FP32 sums, the points whose
+ * modelled FP32 error is not small against their scattered field are
+ * recomputed with FP64 sums (0: never); "mixed_probe_pct" [25] = calls of at
+ * least 4,096 points first run the FP32 sums and the error model on 1,024
+ * evenly strided points, and when more than this percentage of them would be
+ * recomputed, the whole call runs with FP64 sums (the FP64 field, bit for
+ * bit; 0: no sample).
+ * bielcc_get_tuning also reads "mixed_fixup_points" (points recomputed, for
+ * the context) and "mixed_probe_fp64" (mixed calls the sample sent to FP64
+ * sums),
  * "n_cu" (compute units) and "lu_panel_aborts" (whole-panel launches that
  * aborted to one workgroup, for the context, since its first LU),
  * "lu_panel_late_admits" (LUs that missed the whole-panel budget at their

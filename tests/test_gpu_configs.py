@@ -139,7 +139,12 @@ def test_sphere20k_precision_form(ctx, orc, sphere20k_solved):
     pts = B.mesh.grid_points(200, 500, 5.0)
     npp = B.num_param()
     E64, H64 = m.field_batch(npp, K, j64[None], pts)
-    E32, H32 = m.field_batch(npp, K, j32[None], pts, precision=B.PREC_MIXED)
+    # the FP32 sums with their error control (no sample probe: it would choose FP64 sums here)
+    ctx.set_tuning("mixed_probe_pct", 0)
+    try:
+        E32, H32 = m.field_batch(npp, K, j32[None], pts, precision=B.PREC_MIXED)
+    finally:
+        ctx.set_tuning("mixed_probe_pct", 25)
     rng = np.random.default_rng(32)
     # sample: random points plus the 32 of smallest |E| (pattern nulls)
     nE = np.linalg.norm(E64[0], axis=1)
@@ -153,6 +158,13 @@ def test_sphere20k_precision_form(ctx, orc, sphere20k_solved):
     eE = _floor_rel(E32[0], E64[0], nE.max()).max()
     eH = _floor_rel(H32[0], H64[0], np.linalg.norm(H64[0], axis=1).max()).max()
     assert eE <= MIXED_TOL and eH <= MIXED_TOL, (eE, eH)
+    # With the sample probe (default): near the TM4 interior resonance the currents' fields
+    # cancel, the FP32 error model sends most points back to FP64 sums, so the sample
+    # switches the whole call to FP64 sums: the FP64 field, bit for bit.
+    before = ctx.get_tuning("mixed_probe_fp64")
+    E32p, H32p = m.field_batch(npp, K, j64[None], pts, precision=B.PREC_MIXED)
+    assert ctx.get_tuning("mixed_probe_fp64") == before + 1
+    assert np.array_equal(E32p, E64) and np.array_equal(H32p, H64)
 
 
 @pytest.fixture(scope="module")
