@@ -289,6 +289,7 @@ int lu_get_look_ahead(const LuWork*);
 // whole-panel kernel (one launch per panel, grid barrier per column, bounded waits)
 void lu_set_panel_lds(LuWork*, int mode);   // 1 within the workgroup budget, 0 never, 2 always
 void lu_set_panel_rows32(LuWork*, int);
+void lu_set_panel_rows64(LuWork*, int);
 void lu_set_tournament(LuWork*, int mode);   // panels by tournament pivoting: 1 always, 0 never
 int lu_get_tournament(const LuWork*);
 void lu_set_gemm16(LuWork*, int mode);       // FP32 LU rank-512 updates on FP16 matrix cores: 1 split, 2 FP16, 0 off
@@ -296,6 +297,7 @@ int lu_get_gemm16(const LuWork*);
 void lu_set_look_ahead2(LuWork*, int mode);  // two-level LU: next outer block on the side stream (1) or not (0)
 int lu_get_look_ahead2(const LuWork*);
 int lu_get_panel_rows32(const LuWork*);
+int lu_get_panel_rows64(const LuWork*);
 // rocBLAS atomics mode of the context's handle (off: reproducible GEMM results)
 void lu_set_blas_atomics(LuWork*, bool on);
 // columns per outer block of the two-level LU (64: one level, with the optional look-ahead)

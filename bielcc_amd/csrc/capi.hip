@@ -872,6 +872,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         ctx->mixed_fixup = value == 1;
         return BIELCC_OK;
     }
+    if (k == "lu_panel_rows64") {
+        if (value != 64 && value != 128) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_rows64 must be 64 or 128");
+        lu_set_panel_rows64(ctx->lu, value);
+        return BIELCC_OK;
+    }
     if (k == "lu_panel_rows32") {
         if (value != 0 && value != 64 && value != 128) return fail(ctx, BIELCC_ERR_ARG, "lu_panel_rows32 must be 0, 64 or 128");
         lu_set_panel_rows32(ctx->lu, value);
@@ -948,6 +953,7 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "lu_look_ahead") *value = lu_get_look_ahead(ctx->lu);
     else if (k == "lu_panel_lds") *value = lu_get_panel_lds(ctx->lu);
     else if (k == "lu_panel_rows32") *value = lu_get_panel_rows32(ctx->lu);
+    else if (k == "lu_panel_rows64") *value = lu_get_panel_rows64(ctx->lu);
     else if (k == "lu_tournament") *value = lu_get_tournament(ctx->lu);
     else if (k == "lu32_gemm16") *value = lu_get_gemm16(ctx->lu);
     else if (k == "lu_look_ahead2") *value = lu_get_look_ahead2(ctx->lu);

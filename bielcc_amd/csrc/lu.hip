@@ -119,6 +119,8 @@ struct LuWork {
     int tcap = 0;                   // blocks of kTsRows rows the buffer serves
     int panel_rows32 = 0;           // rows per workgroup of FP32 whole panels: 0 by height (64, or 128 when 64
                                     // would need more workgroups than fit), 64 or 128 forced (tests)
+    int panel_rows64 = 64;          // rows per workgroup of FP64 whole panels: 64 (two workgroups per compute
+                                    // unit) or 128 (one, with half as many barrier arrivals per column)
     int panel_wait_us = 250000;     // bounded barrier wait before a panel aborts to one workgroup
     int dev = 0, n_cu = 0, clk_khz = 0;
     bool blas_atomics = false;      // let rocBLAS use atomic (run-to-run varying) reductions
@@ -154,6 +156,8 @@ int lu_get_gemm16(const LuWork* w) { return w->gemm16; }
 int lu_get_tournament(const LuWork* w) { return w->tournament; }
 void lu_set_panel_rows32(LuWork* w, int rows) { w->panel_rows32 = rows; }
 int lu_get_panel_rows32(const LuWork* w) { return w->panel_rows32; }
+void lu_set_panel_rows64(LuWork* w, int rows) { w->panel_rows64 = rows; }
+int lu_get_panel_rows64(const LuWork* w) { return w->panel_rows64; }
 void lu_set_blas_atomics(LuWork* w, bool on) { w->blas_atomics = on; }
 void lu_set_outer_nb(LuWork* w, int nb) { w->outer_nb = nb; }
 void lu_set_trsv_limits(LuWork* w, int max64, int chunks32, int max32) {
@@ -703,15 +707,22 @@ __device__ void panel_one_wg(int n, int k0, int nb, c2<R>* __restrict__ A, long 
     }
 }
 
+__device__ void swap_map_wave(int n, int k0, int nb, int p, int2* __restrict__ map, int* __restrict__ count,
+                              int* __restrict__ bad);
+
 // sync[0] barrier word, sync[1] workgroups gone, sync[2] aborted panels (count).
 // cval/cidx [2][kPanelMaxWG], crow [2][kPanelMaxWG][kNB], rowj [2][kNB]:
 // double-buffered by column parity (a workgroup can be at most one barrier ahead).
+// The last workgroup to leave also writes the panel's interchanges as a gather
+// (map, mcount, bad: see swap_map_wave) from the pivots every workgroup chose.
 template <typename R, int RPW>
 __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
                                                    int* __restrict__ piv, int* __restrict__ info,
                                                    unsigned* __restrict__ sync, double* __restrict__ cval,
                                                    int* __restrict__ cidx, c2<R>* __restrict__ crow,
-                                                   c2<R>* __restrict__ rowj, long long wait_ticks) {
+                                                   c2<R>* __restrict__ rowj, long long wait_ticks,
+                                                   int2* __restrict__ map, int* __restrict__ mcount,
+                                                   int* __restrict__ bad) {
     constexpr int RPL = RPW / 64;   // rows per lane
     constexpr int NONE = 0x7fffffff;
     __shared__ c2<R> sa[kNB][RPW];  // [column][row]
@@ -720,6 +731,7 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
     __shared__ int s_i4[4];
     __shared__ double s_cv;
     __shared__ int s_cr, s_ok, s_last;
+    __shared__ int s_piv[kNB];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int G = gridDim.x, g = blockIdx.x;
     const int m = n - k0, r0 = g * RPW, nr = min(RPW, m - r0);
@@ -779,6 +791,7 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
         for (int q = 1; q < 4; q++) argmax_merge(best, bi, s_v4[q], s_i4[q]);
         const int p = (bi == NONE) ? j : bi;
         const int gw = (p - k0) / RPW;
+        if (tid == 0) s_piv[jj] = p;
         if (g == 0 && tid == 0) {
             piv[j] = p;
             if (!(best > 0.) && *info == 0) *info = j + 1;
@@ -834,10 +847,14 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
     __syncthreads();
     if (!s_last) return;
     // every workgroup is gone: the verdict is final
-    if (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kAbortBit) {
+    const bool aborted = __hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kAbortBit;
+    if (aborted) {
         panel_one_wg<R>(n, k0, nb, A, lda, piv, info);
         if (tid == 0) atomicAdd(&sync[2], 1u);
     }
+    __syncthreads();
+    // the pivots: this workgroup's own record, or those panel_one_wg wrote (same workgroup)
+    swap_map_wave(n, k0, nb, tid < nb ? (aborted ? piv[k0 + tid] : s_piv[tid]) : 0, map, mcount, bad);
     __syncthreads();
     if (tid == 0) {
         sync[0] = 0u;
@@ -853,7 +870,9 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
 // first (q < nb, d = k0 + q), then the outside pivot rows in order of appearance;
 // *count = |R| <= 2 nb.  A pivot row outside [k0 + t, n) is not used: it raises
 // *bad, counts as no interchange, and the LU reports the error.  swap_map_wave does it
-// from lane t's pivot row p (t < nb) in a kernel of one wave; k_swap_map reads piv.
+// from lane t's pivot row p (t < nb) in the first wave of the workgroup (the other waves
+// pass its barriers and do nothing); k_swap_map reads piv, k_panel_lds's last workgroup
+// takes the pivots it chose.
 __device__ void swap_map_wave(int n, int k0, int nb, int p, int2* __restrict__ map, int* __restrict__ count,
                               int* __restrict__ bad) {
     __shared__ int sp[kNB];           // the panel's pivot rows
@@ -1737,7 +1756,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // 64 rows per workgroup, or 128 for FP32 panels that need more than gmax of 64
     const int gmax = std::min(kPanelMaxWG, 2 * w->n_cu);
     auto rows_per_wg = [&](int m) {
-        if (sizeof(R) != 4) return 64;
+        if (sizeof(R) != 4) return w->panel_rows64;
         if (w->panel_rows32 == 64 || w->panel_rows32 == 128) return w->panel_rows32;
         return (m + 63) / 64 > gmax ? 128 : 64;
     };
@@ -1804,11 +1823,10 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         if (lds && G <= gmax && (force || G <= w->n_cu || running.alone())) {
             if (rows_per_wg(n - k0) == 64)
                 hipLaunchKernelGGL((k_panel_lds<R, 64>), dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv, w->info,
-                                   w->psync, cval, cidx, crow, rowj, wait_ticks);
+                                   w->psync, cval, cidx, crow, rowj, wait_ticks, map, mcount, bad);
             else
                 hipLaunchKernelGGL((k_panel_lds<R, 128>), dim3(G), dim3(256), 0, st, n, k0, nb, A, lda, piv,
-                                   w->info, w->psync, cval, cidx, crow, rowj, wait_ticks);
-            hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, st, n, k0, nb, piv, map, mcount, bad);
+                                   w->info, w->psync, cval, cidx, crow, rowj, wait_ticks, map, mcount, bad);
             return;
         }
         int nparts = 0;

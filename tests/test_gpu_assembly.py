@@ -420,6 +420,20 @@ def test_fp32_whole_panel_two_rows_per_lane(ctx, n):
     assert err <= 1e-4, err
 
 
+@pytest.mark.parametrize("n", [65, 1000, 2500])
+def test_fp64_whole_panel_two_rows_per_lane(ctx, n):
+    """FP64 whole panels with 128 rows per workgroup (lu_panel_rows64 = 128: one
+    workgroup per compute unit, half the barrier arrivals): the FP64 factors and
+    pivots of the per-column launches, bit for bit, and the interchange map that the
+    panel kernel's last workgroup writes gives the same matrix as the map kernel's."""
+    rng = np.random.default_rng(13)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = _factors(ctx, A, lu_panel_lds=2, lu_panel_rows64=128, lu_look_ahead=0)
+    b = _factors(ctx, A, lu_panel_lds=0, lu_look_ahead=0)
+    assert ctx.get_tuning("lu_panel_rows64") == 64
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+
+
 def test_whole_panel_abort_falls_back_to_one_workgroup(ctx):
     """A zero wait bound makes the whole-panel launches abort at their first
     barrier (forced from the host; every wait still ends).  The last workgroup
