@@ -1529,11 +1529,12 @@ static hipError_t gemm_update(LuWork* w, hipStream_t s, long long m, long long n
 // s_X (from max |re|, |im|, found on the device) and split into FP16 parts,
 // X / s_X = X_hi + 2^-11 X_lo with X_hi = fp16(X / s_X), X_lo = fp16(2^11 (X / s_X - X_hi)).
 // Then C -= s_A s_B [A_hi B_hi + 2^-11 (A_hi B_lo + A_lo B_hi)] (A_lo B_lo, 2^-22 of the
-// product, is dropped): two FP16 GEMMs with FP32 accumulation, K = 2k and K = 4k over
-// [A_hi | A_lo] and [B_lo ; B_hi].  Errors are of FP32 size relative to max|A| max|B|
-// (normwise, as a scaled FP32 product).  mode 2 keeps only A_hi B_hi (FP16 accuracy).
-// Measured (rocBLAS gemm_ex, MI355X): 815 TFLOP/s at K = 1024 against 152 for the
-// complex FP32 GEMM of the same product.
+// product, is dropped): one FP16 GEMM with FP32 accumulation (k_split_a / k_split_b
+// below lay the three products out along K).  Errors are of FP32 size relative to
+// max|A| max|B| (normwise, as a scaled FP32 product).  mode 2 keeps only A_hi B_hi
+// (FP16 accuracy).  Measured (rocBLAS gemm_ex and hipBLASLt alike, MI355X, FP32 output
+// with beta = 1): 1.04 PFLOP/s at K = 3072 and 0.74 at K = 1024, against 152 TFLOP/s for
+// the complex FP32 GEMM of the same product.
 __global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
 }
@@ -1569,48 +1570,63 @@ __device__ __forceinline__ float g16_scale(unsigned amax_bits) {   // power of t
     return ldexpf(1.f, e);
 }
 
-// A (m x k complex) -> H [2m][4k] column-major FP16: columns 0..2k-1 A_hi, 2k..4k-1 A_lo
-// (column pair j: A_j, i A_j); also the GEMMs' alphas and beta into par (device pointer mode).
+// The split FP16 update C -= (A B) of FP32 operands as ONE real GEMM with FP32 accumulation.
+// With X = s (X_hi + 2^-11 X_lo) per operand (s a power of two >= max |X|; hi and lo FP16),
+//   A B = sa sb 2^-11 (2^11 A_hi B_hi + A_hi B_lo + A_lo B_hi)  (the lo x lo term, 2^-22, dropped),
+// so C += alpha H G with alpha = -sa sb 2^-11, K = 6k, and
+//   H = [2^11 A_hi | A_hi | A_lo]  (2m x 6k: the 2^11 is an exact exponent shift, |2^11 A_hi| <= 2048),
+//   G = [B_hi ; B_lo ; B_hi]       (6k x n).
+// One pass over C instead of two, at K = 6k instead of 2k and 4k (1.04 against 0.74 PFLOP/s at 2k).
+// Complex to real: C is 2m x n (re, im interleaved down a column); column pair j of each part
+// of H is (A_j, i A_j); row pair j of each part of G is (re, im) of B's row j.
+// A (m x k complex) -> H [2m][6k] column-major FP16; also alpha and beta into par
+// (device pointer mode).
 __global__ void k_split_a(long long m, long long k, const float2* __restrict__ A, long long lda,
                           const unsigned* __restrict__ amax, _Float16* __restrict__ H, float* __restrict__ par) {
     const float sa = g16_scale(amax[0]), sb = g16_scale(amax[1]), ia = 1.f / sa;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        par[0] = -sa * sb;
-        par[1] = -sa * sb * 0x1p-11f;
+        par[0] = -sa * sb * 0x1p-11f;
+        par[1] = 0.f;
         par[2] = 1.f;
     }
     const long long ldh = 2 * m, stride = (long long)gridDim.x * blockDim.x;
-    _Float16* L = H + ldh * 2 * k;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m * k; e += stride) {
         const long long i = e % m, j = e / m;
         const float2 v = A[j * lda + i];
         const float xr = v.x * ia, xi = v.y * ia;
         const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
         const _Float16 lr = (_Float16)((xr - (float)hr) * 2048.f), li = (_Float16)((xi - (float)hi) * 2048.f);
-        _Float16* c0 = H + (2 * j) * ldh + 2 * i;
+        const _Float16 sr = (_Float16)((float)hr * 2048.f), si = (_Float16)((float)hi * 2048.f);
+        _Float16* c0 = H + (2 * j) * ldh + 2 * i;   // part 0: 2^11 hi
         _Float16* c1 = c0 + ldh;
-        c0[0] = hr; c0[1] = hi; c1[0] = -hi; c1[1] = hr;
-        _Float16* d0 = L + (2 * j) * ldh + 2 * i;
+        c0[0] = sr; c0[1] = si; c1[0] = -si; c1[1] = sr;
+        _Float16* d0 = c0 + 2 * k * ldh;             // part 1: hi
         _Float16* d1 = d0 + ldh;
-        d0[0] = lr; d0[1] = li; d1[0] = -li; d1[1] = lr;
+        d0[0] = hr; d0[1] = hi; d1[0] = -hi; d1[1] = hr;
+        _Float16* f0 = d0 + 2 * k * ldh;             // part 2: lo
+        _Float16* f1 = f0 + ldh;
+        f0[0] = lr; f0[1] = li; f1[0] = -li; f1[1] = lr;
     }
 }
 
-// B (k x n complex) -> G [n][4k] column-major FP16 (ld 4k): rows 0..2k-1 B_lo, 2k..4k-1 B_hi
+// B (k x n complex) -> G [n][6k] column-major FP16 (ld 6k): rows 0..2k-1 B_hi, 2k..4k-1 B_lo,
+// 4k..6k-1 B_hi
 __global__ void k_split_b(long long k, long long n, const float2* __restrict__ Bm, long long ldb,
                           const unsigned* __restrict__ amax, _Float16* __restrict__ G) {
     const float ib = 1.f / g16_scale(amax[1]);
-    const long long ldg = 4 * k, stride = (long long)gridDim.x * blockDim.x;
+    const long long ldg = 6 * k, stride = (long long)gridDim.x * blockDim.x;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < k * n; e += stride) {
         const long long j = e % k, c = e / k;
         const float2 v = Bm[c * ldb + j];
         const float xr = v.x * ib, xi = v.y * ib;
         const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
         _Float16* g = G + c * ldg + 2 * j;
-        g[0] = (_Float16)((xr - (float)hr) * 2048.f);
-        g[1] = (_Float16)((xi - (float)hi) * 2048.f);
-        g[2 * k] = hr;
-        g[2 * k + 1] = hi;
+        g[0] = hr;
+        g[1] = hi;
+        g[2 * k] = (_Float16)((xr - (float)hr) * 2048.f);
+        g[2 * k + 1] = (_Float16)((xi - (float)hi) * 2048.f);
+        g[4 * k] = hr;
+        g[4 * k + 1] = hi;
     }
 }
 
@@ -1619,7 +1635,7 @@ static hipError_t gemm_update_g16(LuWork* w, hipStream_t s, long long m, long lo
                                   const char** msg) {
     if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
     hipError_t e;
-    const size_t ha = (size_t)2 * m * 4 * k * sizeof(_Float16), hb = (size_t)4 * k * n * sizeof(_Float16);
+    const size_t ha = (size_t)2 * m * 6 * k * sizeof(_Float16), hb = (size_t)6 * k * n * sizeof(_Float16);
     const size_t need = 256 + ((ha + 255) & ~(size_t)255) + hb;
     if (w->g16_cap < need) {
         (void)hipFree(w->g16);
@@ -1641,14 +1657,11 @@ static hipError_t gemm_update_g16(LuWork* w, hipStream_t s, long long m, long lo
     rocblas_set_stream(w->blas, s);
     rocblas_set_pointer_mode(w->blas, rocblas_pointer_mode_device);
     const rocblas_datatype h16 = rocblas_datatype_f16_r, f32 = rocblas_datatype_f32_r;
-    // hi x hi, K = 2k: H columns 0..2k-1, G rows 2k..4k-1
-    rocblas_status st = rocblas_gemm_ex(w->blas, rocblas_operation_none, rocblas_operation_none, 2 * m, n, 2 * k, par,
-                                        H, h16, 2 * m, G + 2 * k, h16, 4 * k, par + 2, Cc, f32, 2 * ldc, Cc, f32,
-                                        2 * ldc, f32, rocblas_gemm_algo_standard, 0, 0);
-    if (st == rocblas_status_success && w->gemm16 == 1)   // cross terms, K = 4k: [A_hi | A_lo] [B_lo ; B_hi]
-        st = rocblas_gemm_ex(w->blas, rocblas_operation_none, rocblas_operation_none, 2 * m, n, 4 * k, par + 1, H, h16,
-                             2 * m, G, h16, 4 * k, par + 2, Cc, f32, 2 * ldc, Cc, f32, 2 * ldc, f32,
-                             rocblas_gemm_algo_standard, 0, 0);
+    // split operands: all three parts, K = 6k; FP16 high parts only (gemm16 = 2): the first part, K = 2k
+    const long long K = w->gemm16 == 1 ? 6 * k : 2 * k;
+    rocblas_status st = rocblas_gemm_ex(w->blas, rocblas_operation_none, rocblas_operation_none, 2 * m, n, K, par, H,
+                                        h16, 2 * m, G, h16, 6 * k, par + 2, Cc, f32, 2 * ldc, Cc, f32, 2 * ldc, f32,
+                                        rocblas_gemm_algo_standard, 0, 0);
     rocblas_set_pointer_mode(w->blas, rocblas_pointer_mode_host);
     if (st != rocblas_status_success) {
         *msg = "rocblas_gemm_ex (FP16 split) failed";
