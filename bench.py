@@ -1141,7 +1141,9 @@ def batched_leg(tune_cache=None, workers=4, solve=True):
     tune_f = T.tune(ctx, lambda: ctx.plan_field(fitems, npp, precision=B.PREC_MIXED), T.FIELD_SPACE, "plan_field",
                     sig + "|field32", repeats=2, cache_path=tune_cache, log=log)
     fplan = ctx.plan_field(fitems, npp, precision=B.PREC_MIXED)
-    t_field_eager = timed(fplan.run)
+    fixed0 = ctx.get_tuning("mixed_fixup_points")
+    t_field_eager = timed(fplan.run)   # first run: FP32 sums and their error control
+    fixed_first = ctx.get_tuning("mixed_fixup_points") - fixed0
     ctx.timing_enable(True)
     ctx.timing_reset()
     fplan.run()              # eager, with HIP events around the tile launch and the GEMMs
@@ -1153,7 +1155,8 @@ def batched_leg(tune_cache=None, workers=4, solve=True):
     fixed0 = ctx.get_tuning("mixed_fixup_points")
     t_field = timed(lambda: [fplan.run() for _ in range(reps)]) / reps
     fixed = (ctx.get_tuning("mixed_fixup_points") - fixed0) // reps   # per run, of 12 x 2,000 points
-    f_info = {k: fplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs", "workspace_bytes")}
+    f_info = {k: fplan.info(k) for k in ("tiles", "grid", "launches", "pairs", "near_pairs", "workspace_bytes",
+                                         "fp64_sums")}
     fplan.close()
     E32 = [b["E"].to_numpy((nd, M, 3), np.complex128) for b in bufs]
     H32 = [b["H"].to_numpy((nd, M, 3), np.complex128) for b in bufs]
@@ -1241,7 +1244,7 @@ def batched_leg(tune_cache=None, workers=4, solve=True):
                       "max_residual": max((r[1] for r in refine), default=0.0),
                       "fp64_fallback_meshes": fallbacks},
            "field_graph_replay_s": round(t_field, 4), "field_eager_s": round(t_field_eager, 4),
-           "field_points_recomputed_fp64": fixed,
+           "field_points_recomputed_fp64": fixed, "field_first_run_points_recomputed_fp64": fixed_first,
            "kernel_ms": {"k_assemble_multi": round(kt_asm, 3), "k_field_opmat": round(kt["plan_field_tiles"], 3),
                          "zgemm_all_meshes": round(kt["plan_field_gemm"], 3)},
            "k_field_opmat_tflops": round(f_info["pairs"] * FLOPS_FIELD_PAIR / kt["plan_field_tiles"] / 1e9, 2),

@@ -68,6 +68,12 @@ struct bielcc_plan_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int runs = 0;
+    // mixed field plans: FP64 far sums from the run after one in which the FP32 error
+    // control recomputed more than mixed_probe_pct percent of the points (the points,
+    // meshes and wavenumbers that make FP32 sums fail are the plan's own); a captured
+    // graph is captured again at that next run
+    bool fp64_sums = false;
+    bool recapture = false;
 };
 
 namespace {
@@ -155,7 +161,7 @@ int run_assemble(bielcc_ctx ctx, bielcc_plan p) {
 int run_field(bielcc_ctx ctx, bielcc_plan p) {
     for (const bielcc_field_item& it : p->fitems)
         HIPCHK(launch_init_incident(ctx->stream, it.M, it.nrhs, it.pts_d, it.inc_d, it.E_d));
-    const bool p32 = p->precision == BIELCC_PREC_MIXED;
+    const bool p32 = p->precision == BIELCC_PREC_MIXED && !p->fp64_sums;
     for (const Group& g : p->groups) {
         {
             Timer t(ctx, "plan_field_tiles");
@@ -182,6 +188,35 @@ int run_field(bielcc_ctx ctx, bielcc_plan p) {
 
 int run_impl(bielcc_ctx ctx, bielcc_plan p) {
     return p->kind == 0 ? run_assemble(ctx, p) : run_field(ctx, p);
+}
+
+// Captures one run of the plan into a graph (warm: one eager run first, so that libraries
+// finish their lazy set-up outside the capture).
+int capture_impl(bielcc_ctx ctx, bielcc_plan p, bool warm) {
+    if (warm) {
+        if (int rc = run_impl(ctx, p)) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    int rc = run_impl(ctx, p);
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    p->graph = g;
+    p->exec = x;
+    return BIELCC_OK;
 }
 
 }  // namespace
@@ -503,6 +538,10 @@ int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan p) {
     if (!ctx) return BIELCC_ERR_ARG;
     if (!p) return fail(ctx, BIELCC_ERR_ARG, "plan is NULL");
     DeviceGuard dg(ctx->device);
+    if (p->recapture) {   // the plan changed to FP64 sums after its graph was captured
+        p->recapture = false;
+        if (int rc = capture_impl(ctx, p, false)) return rc;
+    }
     Timer t(ctx, p->kind == 0 ? "plan_assemble" : "plan_field");
     if (p->exec) {
         HIPCHK(hipGraphLaunch(p->exec, ctx->stream));
@@ -511,11 +550,14 @@ int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan p) {
     }
     // mixed field: FP32 error control per item, after the launch or replay (it reads a
     // count back, so it stays out of the captured graph)
-    if (p->kind == 1 && p->precision == BIELCC_PREC_MIXED && ctx->mixed_fixup) {
+    if (p->kind == 1 && p->precision == BIELCC_PREC_MIXED && !p->fp64_sums && ctx->mixed_fixup) {
         const bielcc_num_param np{p->np.eps, p->np.rs, p->np.rs_seg, p->np.analytic_calc, p->np.n_start,
                                   p->np.n_start_seg, p->np.p_max, p->np.p_max_seg};
+        const long long fixed0 = ctx->fixup_points;
+        long long points = 0;
         for (const bielcc_field_item& it : p->fitems) {
             if (it.M <= 0) continue;
+            points += it.M;
             std::vector<double> inc;
             if (it.inc_d) {
                 inc.resize((size_t)it.nrhs * 6);
@@ -528,6 +570,17 @@ int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan p) {
                                            it.pts_d, it.E_d, it.H_d))
                 return rc;
         }
+        if (ctx->mixed_probe_pct > 0 && (ctx->fixup_points - fixed0) * 100 > (long long)ctx->mixed_probe_pct * points) {
+            p->fp64_sums = true;
+            if (p->exec) {
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                (void)hipGraphExecDestroy(p->exec);
+                (void)hipGraphDestroy(p->graph);
+                p->exec = nullptr;
+                p->graph = nullptr;
+                p->recapture = true;
+            }
+        }
     }
     t.stop();
     p->runs++;
@@ -539,29 +592,8 @@ int bielcc_plan_capture(bielcc_ctx ctx, bielcc_plan p) {
     if (!p) return fail(ctx, BIELCC_ERR_ARG, "plan is NULL");
     DeviceGuard dg(ctx->device);
     if (p->exec) return BIELCC_OK;
-    // one eager run first: libraries finish their lazy set-up outside the capture
-    if (int rc = run_impl(ctx, p)) return rc;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    ctx->capturing = true;
-    int rc = run_impl(ctx, p);
-    ctx->capturing = false;
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
-    hipGraphExec_t x = nullptr;
-    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return hip_fail(ctx, e, "hipGraphInstantiate");
-    }
-    p->graph = g;
-    p->exec = x;
-    return BIELCC_OK;
+    p->recapture = false;
+    return capture_impl(ctx, p, true);
 }
 
 int bielcc_plan_destroy(bielcc_ctx ctx, bielcc_plan p) {
@@ -584,7 +616,8 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan p, const char* key, double* val
     else if (k == "near_pairs") *value = (double)p->near_pairs;
     else if (k == "segments") *value = (double)(p->kind == 1 ? p->segs.size() : p->aitems.size());
     else if (k == "workspace_bytes") *value = (double)p->ws_bytes;
-    else if (k == "captured") *value = p->exec ? 1. : 0.;
+    else if (k == "captured") *value = (p->exec || p->recapture) ? 1. : 0.;
+    else if (k == "fp64_sums") *value = p->fp64_sums ? 1. : 0.;
     else return fail(ctx, BIELCC_ERR_ARG, "unknown plan info key " + k);
     return BIELCC_OK;
 }

@@ -368,7 +368,14 @@ int bielcc_far_pattern(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, 
  * caller's device buffers -- at creation, when it builds the tile list of its
  * persistent kernel, the near-pair lists and its workspace (plan-owned).
  * bielcc_plan_run then enqueues kernels and GEMMs only: no allocation, no
- * host synchronisation, no memset.  bielcc_plan_capture runs the plan once
+ * host synchronisation, no memset -- except in mixed field plans, whose FP32
+ * error control ("mixed_fixup") runs after the launch or replay: per item it
+ * reads a count back and sums the points that fail the error model again in
+ * FP64 (growing the context's scratch on first use).  When one run recomputes
+ * more than "mixed_probe_pct" percent of the plan's points, the plan sums in
+ * FP64 from its next run on, bit for bit as an FP64 plan, and a captured plan
+ * is captured again at that run (info key "fp64_sums").
+ * bielcc_plan_capture runs the plan once
  * and records it into a HIP graph; later bielcc_plan_run calls replay it.
  * Between runs the caller may change the CONTENTS of the item buffers
  * (currents, incidences, points), not their addresses.  Tile shapes come from
@@ -413,7 +420,7 @@ int bielcc_plan_run(bielcc_ctx ctx, bielcc_plan plan);
 int bielcc_plan_capture(bielcc_ctx ctx, bielcc_plan plan);
 int bielcc_plan_destroy(bielcc_ctx ctx, bielcc_plan plan);
 /* keys: "tiles", "launches", "grid", "pairs", "near_pairs", "segments",
- * "workspace_bytes", "captured" */
+ * "workspace_bytes", "captured", "fp64_sums" */
 int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* value);
 
 /* Launch-shape tunables (the autotuner sets these; defaults in brackets):

@@ -189,6 +189,70 @@ def test_field_plan_graph_replays(ctx):
         m.close()
 
 
+def test_mixed_field_plan_switches_to_fp64_sums(ctx):
+    """Two copies of a sphere 1e-4 apart carry currents J and -J: their fields cancel to
+    about 1e-4 of either, beyond what FP32 node sums resolve, so the FP32 error control
+    recomputes every point.  After that run the mixed plan sums in FP64: from then on its
+    results, eager and from a graph captured again, are those of the FP64 plan bit for bit.
+    With mixed_probe_pct = 0 the plan keeps FP32 sums (and the error control)."""
+    rng = np.random.default_rng(31)
+    npp = B.num_param()
+    v = B.mesh.sphere_quads(8, 16, seed=3)
+    m = ctx.mesh(np.concatenate([v, v + np.array([1e-4, 0.0, 0.0])]))
+    nr, M, k = 4, 600, 2.0
+    J = rng.normal(size=(nr, v.shape[0], 3)) + 1j * rng.normal(size=(nr, v.shape[0], 3))
+    jE = np.concatenate([J, -J], axis=1)
+    pts = _points(rng, M, 2.0, 4.0)
+
+    def items():
+        return [dict(mesh=m, k=k, nrhs=nr, M=M, jE=ctx.to_device(jE), pts=ctx.to_device(pts), inc=None,
+                     E=ctx.empty(nr * M * 48), H=ctx.empty(nr * M * 48), multE=1.0)]
+
+    def fields(it):
+        ctx.sync()
+        return it["E"].to_numpy((nr, M, 3), np.complex128), it["H"].to_numpy((nr, M, 3), np.complex128)
+
+    i64 = items()
+    p64 = ctx.plan_field(i64, npp, precision=B.PREC_FP64)
+    p64.run()
+    E64, H64 = fields(i64[0])
+    assert ctx.get_tuning("mixed_probe_pct") == 25
+    for capture in (False, True):
+        it = items()
+        plan = ctx.plan_field(it, npp, precision=B.PREC_MIXED)
+        if capture:
+            plan.capture()
+        fixed = ctx.get_tuning("mixed_fixup_points")
+        plan.run()   # FP32 sums, every point recomputed
+        E, H = fields(it[0])
+        assert ctx.get_tuning("mixed_fixup_points") - fixed == M
+        assert plan.info("fp64_sums") == 1 and plan.info("captured") == (1 if capture else 0)
+        assert rowwise_rel(E.reshape(-1, 3), E64.reshape(-1, 3)).max() <= 1e-6
+        for _ in range(2):
+            ctx.upload(it[0]["E"], np.zeros((nr, M, 3), np.complex128))
+            fixed = ctx.get_tuning("mixed_fixup_points")
+            plan.run()
+            E, H = fields(it[0])
+            assert ctx.get_tuning("mixed_fixup_points") == fixed
+            assert np.array_equal(E, E64) and np.array_equal(H, H64)
+        plan.close()
+    ctx.set_tuning("mixed_probe_pct", 0)
+    try:
+        it = items()
+        plan = ctx.plan_field(it, npp, precision=B.PREC_MIXED)
+        for _ in range(2):
+            fixed = ctx.get_tuning("mixed_fixup_points")
+            plan.run()
+            E, H = fields(it[0])
+            assert ctx.get_tuning("mixed_fixup_points") - fixed == M and plan.info("fp64_sums") == 0
+            assert rowwise_rel(E.reshape(-1, 3), E64.reshape(-1, 3)).max() <= 1e-6
+        plan.close()
+    finally:
+        ctx.set_tuning("mixed_probe_pct", 25)
+    p64.close()
+    m.close()
+
+
 def test_field_plan_without_h_and_without_incidence(ctx):
     rng = np.random.default_rng(14)
     npp = B.num_param()
