@@ -528,7 +528,9 @@ def _lu_probe_error(dF, n, piv, AX, X, dtype, itemsize, strip=2048):
 def lu_only(n, reps=3):
     """FP64 and FP32 LU of a seeded random complex n x n matrix (entries N(0,1) + i N(0,1)),
     each factored reps times from the same copy; the fastest of each is reported, with
-    the factors' sampled backward error ||P A X - L U X|| / ||P A X|| on 4 random columns X."""
+    the factors' sampled backward error ||P A X - L U X|| / ||P A X|| on 4 random columns X.
+    Also the FP64 factorisation without interchanges of the randomly butterfly-transformed
+    matrix (lu_factor_rbt), and its refined solve of A X = (A X) against X."""
     import bielcc_amd as B
     ctx = B.Context(0)
     rng = np.random.default_rng(1)
@@ -554,13 +556,25 @@ def lu_only(n, reps=3):
     flop = 8.0 / 3.0 * n ** 3
     be64 = _lu_probe_error(dA, n, piv.to_numpy((n,), np.int32), AX, X, np.complex128, 16)
     be32 = _lu_probe_error(dF, n, piv32.to_numpy((n,), np.int32), AX, X, np.complex64, 8)
+    dF.free()
+    # FP64 factors without interchanges of the butterfly-transformed matrix (no grid barrier
+    # per column), then the solve of A X = A X_true refined against A
+    dR, rbt = ctx.empty(n * n * 16), ctx.empty(32 * n)
+    trbt = [timed(lambda: ctx.lu_factor_rbt(dA0, n, n, dR, n, piv, rbt, seed=1)) for _ in range(reps)]
+    dB = ctx.to_device(np.ascontiguousarray(AX.T))
+    it_rbt, res_rbt = ctx.rbt_solve(dA0, n, n, dR, n, piv, rbt, dB, 4, n, tol=1e-13, max_iter=10)
+    err_rbt = float(np.linalg.norm(dB.to_numpy((4, n), np.complex128).T - X) / np.linalg.norm(X))
     out = {"n": n, "lu64_s": round(min(t64), 4), "lu64_tflops": round(flop / min(t64) / 1e12, 2),
            "lu32_s": round(min(t32), 4), "lu32_tflops": round(flop / min(t32) / 1e12, 2),
            "backward_err_probe_fp64": be64, "backward_err_probe_fp32": be32,
+           "lu64_rbt_s": round(min(trbt), 4), "lu64_rbt_tflops": round(flop / min(trbt) / 1e12, 2),
+           "rbt_solve_corrections": it_rbt, "rbt_solve_residual": res_rbt, "rbt_solve_error": err_rbt,
            "panel_aborts": ctx.get_tuning("lu_panel_aborts")}
     log(f"[bench] LU n={n}: FP64 {out['lu64_s']} s ({out['lu64_tflops']} TFLOP/s, backward {be64:.1e}), "
-        f"FP32 {out['lu32_s']} s ({out['lu32_tflops']} TFLOP/s, backward {be32:.1e})")
-    for d in (dA0, dA, piv, piv32, dF):
+        f"FP32 {out['lu32_s']} s ({out['lu32_tflops']} TFLOP/s, backward {be32:.1e}), "
+        f"FP64 butterfly + no interchanges {out['lu64_rbt_s']} s (solve: {it_rbt} corrections, residual "
+        f"{res_rbt:.1e}, error {err_rbt:.1e})")
+    for d in (dA0, dA, piv, piv32, dR, rbt, dB):
         d.free()
     ctx.close()
     return out

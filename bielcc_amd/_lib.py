@@ -105,6 +105,10 @@ _SIGS = {
                                      C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64]),
     "bielcc_lu32_factor": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                      C.c_void_p, _IP]),
+    "bielcc_lu_factor_rbt": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                       C.c_void_p, C.c_void_p, C.c_uint64, _IP]),
+    "bielcc_rbt_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, _IP, _DP]),
     "bielcc_refine_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                       C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, _IP, _DP]),
     "bielcc_residual": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,

@@ -347,6 +347,24 @@ class Context:
                                                  C.byref(it), C.byref(res)))
         return it.value, res.value
 
+    def lu_factor_rbt(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
+                      rbt: DeviceArray, seed: int = 1) -> None:
+        """F = U^T A V (random butterflies from seed; their diagonals into rbt, 32 n bytes)
+        factored without interchanges (piv: identity); A is kept."""
+        info = C.c_int(0)
+        self._check(self.lib.bielcc_lu_factor_rbt(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv), _p(rbt),
+                                                  int(seed), C.byref(info)))
+
+    def rbt_solve(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
+                  rbt: DeviceArray, B: DeviceArray, nrhs: int, ldb: int, tol=1e-12, max_iter=10):
+        """A X = B (X over B) on lu_factor_rbt's factors, refined against A; returns
+        (corrections, relative residual)."""
+        it, res = C.c_int(0), C.c_double(0.)
+        self._check(self.lib.bielcc_rbt_solve(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv), _p(rbt),
+                                              int(nrhs), _p(B), int(ldb), float(tol), int(max_iter), C.byref(it),
+                                              C.byref(res)))
+        return it.value, res.value
+
     def residual(self, A: DeviceArray, n: int, lda: int, X: DeviceArray, ldx: int, B: DeviceArray, ldb: int,
                  nrhs: int, R: DeviceArray, ldr: int, doubled: bool = False) -> None:
         """R = B - A X on the device, sums in FP64 or (doubled) in twice its precision."""

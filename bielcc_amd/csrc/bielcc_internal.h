@@ -340,6 +340,14 @@ hipError_t lu_solve(hipStream_t s, LuWork* w, int n, const double* A, long long 
 hipError_t lu32_factor(hipStream_t s, LuWork* w, int n, const double* A, long long lda, float* F, long long ldf,
                        int* piv, int* info, const char** msg);
 // FP64 iterative refinement on FP32 factors; status 1 = stagnated above tol
+// F = U^T A V (random butterflies from seed, diagonals into rbt: 4 (n & ~3) doubles) factored
+// without interchanges (piv: identity); A is not modified
+hipError_t lu_factor_rbt(hipStream_t s, LuWork* w, int n, const double* A, long long lda, double* F, long long ldf,
+                         int* piv, double* rbt, unsigned long long seed, int* info, const char** msg);
+// A X = B on those factors, refined against A (classic steps) to tol; status 1: not reached
+hipError_t rbt_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const double* F,
+                     long long ldf, const int* piv, const double* rbt, int nrhs, double* B, long long ldb, double tol,
+                     int max_iter, int* iters, double* resid, int* status, const char** msg);
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const float* F,
                         long long ldf, const int* piv, int nrhs, double* B, long long ldb, double tol, int max_iter,
                         int* iters, double* resid, int* status, const char** msg);

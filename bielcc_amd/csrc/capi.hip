@@ -738,6 +738,46 @@ int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, fl
     return BIELCC_OK;
 }
 
+int bielcc_lu_factor_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, double* F_d, int64_t ldf,
+                         int* piv_d, double* rbt_d, uint64_t seed, int* info) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || !A_d || !F_d || !piv_d || !rbt_d || !info || lda < n || ldf < n)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    if ((const void*)A_d == (const void*)F_d) return fail(ctx, BIELCC_ERR_ARG, "F_d must not be A_d (A is kept)");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    Timer t(ctx, "lu_factor_rbt");
+    hipError_t e = lu_factor_rbt(ctx->stream, ctx->lu, n, A_d, lda, F_d, ldf, piv_d, rbt_d, seed, info, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    if (*info != 0) return fail(ctx, BIELCC_ERR_SINGULAR, "zero pivot (no interchanges) at column " + std::to_string(*info));
+    return BIELCC_OK;
+}
+
+int bielcc_rbt_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const double* F_d, int64_t ldf,
+                     const int* piv_d, const double* rbt_d, int nrhs, double* B_d, int64_t ldb, double tol,
+                     int max_iter, int* iters, double* resid) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || nrhs <= 0 || !A_d || !F_d || !piv_d || !rbt_d || !B_d || !iters || !resid || lda < n ||
+        ldf < n || ldb < n || !(tol > 0.) || max_iter < 0)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    int status = 0;
+    Timer t(ctx, "rbt_solve");
+    hipError_t e = rbt_solve(ctx->stream, ctx->lu, n, A_d, lda, F_d, ldf, piv_d, rbt_d, nrhs, B_d, ldb, tol, max_iter,
+                             iters, resid, &status, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    if (status != 0) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "refinement stopped at relative residual %.3e > tol %.3e after %d corrections",
+                      *resid, tol, *iters);
+        return fail(ctx, BIELCC_ERR_UNSUPPORTED, buf);
+    }
+    return BIELCC_OK;
+}
+
 int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
                         const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
                         int* iters, double* resid) {

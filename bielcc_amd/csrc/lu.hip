@@ -28,6 +28,7 @@
 #include <atomic>
 #include <cmath>
 #include <complex>
+#include <random>
 #include <vector>
 
 using namespace bielcc_dev;
@@ -77,6 +78,8 @@ struct LuWork {
     size_t perm_cap = 0;
     void* ref = nullptr;        // refinement work vectors
     size_t ref_cap = 0;
+    void* rbt_buf = nullptr;    // rbt_solve work vectors
+    size_t rbt_cap = 0;
     void* kry = nullptr;        // Krylov bases of the FGMRES refinement (V, Z) and its small arrays
     size_t kry_cap = 0;
     int refine_mode = 1;        // 1: FGMRES on the FP32 factors, 0: classic refinement (one correction a step)
@@ -184,7 +187,7 @@ int lu_get_outer_nb(const LuWork* w) { return w->outer_nb; }
 int lu_trsv_retries(const LuWork* w, int which) { return which ? w->trsv_ticket_faults : w->trsv_expired; }
 int lu_late_admits(const LuWork* w) { return w->late_admits; }
 size_t lu_work_bytes(const LuWork* w) {   // grow-only device work of the LU and the solves
-    return w ? w->perm_cap + w->ref_cap + w->kry_cap + w->dd_cap + w->g16_cap +
+    return w ? w->perm_cap + w->ref_cap + w->rbt_cap + w->kry_cap + w->dd_cap + w->g16_cap +
                    (size_t)w->pcap * (sizeof(double) + sizeof(int))
              : 0;
 }
@@ -227,7 +230,7 @@ void lu_work_destroy(LuWork* w) {
     if (!w) return;
     if (w->blas) rocblas_destroy_handle(w->blas);
     (void)hipFree(w->pval); (void)hipFree(w->pidx); (void)hipFree(w->pinv); (void)hipFree(w->info);
-    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->kry); (void)hipFree(w->dd);
+    (void)hipFree(w->perm); (void)hipFree(w->ref); (void)hipFree(w->rbt_buf); (void)hipFree(w->kry); (void)hipFree(w->dd);
     (void)hipFree(w->step_count);
     (void)hipFree(w->psync); (void)hipFree(w->pbuf); (void)hipFree(w->smap); (void)hipFree(w->tbuf); (void)hipFree(w->bmaps);
     if (w->blas_side) rocblas_destroy_handle(w->blas_side);
@@ -860,6 +863,85 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
         sync[0] = 0u;
         sync[1] = 0u;
         __threadfence();
+    }
+}
+
+// LU panel without pivoting (used on butterfly-transformed matrices, lu_factor_rbt).  Every
+// workgroup factors the nb x nb diagonal block in LDS itself -- the same code on the same
+// data, so the same bits everywhere -- and then solves its own 64 rows of
+// L21 = A21 U11^{-1}; workgroup 0 also writes the diagonal block (L11 \ U11), the identity
+// pivots and info.  No workgroup waits for another: one launch, no grid barrier.  The
+// arithmetic per column follows the pivoting kernels (the column scaled by the reciprocal
+// pivot, none for a zero pivot, which info reports; the rank-1 update x -= l u).
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
+                                                     int* __restrict__ piv, int* __restrict__ info) {
+    __shared__ c2<R> d[kNB][kNB];    // diagonal block [column][row]
+    __shared__ c2<R> a[kNB][64];     // this workgroup's rows of A21 [column][row]
+    __shared__ c2<R> qv[kNB];        // reciprocal pivots
+    __shared__ int qs[kNB];          // pivot nonzero
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = blockIdx.x;
+    const int r0 = k0 + nb + 64 * g, nr = max(0, min(64, n - r0));
+    for (int e = tid; e < nb * nb; e += 256) {
+        const int c = e / nb, r = e % nb;
+        d[c][r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    }
+    for (int e = tid; e < nb * 64; e += 256) {
+        const int c = e / 64, r = e % 64;
+        if (r < nr) a[c][r] = A[(size_t)(k0 + c) * lda + r0 + r];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        const c2<R> p = d[j][j];
+        const R den = norm2_fx<R>(p);
+        const bool sc = den > R(0);
+        const c2<R> q = mk2<R>(sc ? p.x / den : R(0), sc ? -p.y / den : R(0));
+        if (tid > j && tid < nb && sc) d[j][tid] = cmul_fx<R>(d[j][tid], q);
+        if (tid == 0) {
+            qv[j] = q;
+            qs[j] = sc ? 1 : 0;
+            if (g == 0 && !sc && *info == 0) *info = k0 + j + 1;
+        }
+        __syncthreads();
+        const int m = nb - j - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int c = j + 1 + e / m, r = j + 1 + e % m;
+            const c2<R> lu = cmul_fx<R>(d[j][r], d[c][j]);
+            c2<R> x = d[c][r];
+            x.x -= lu.x;
+            x.y -= lu.y;
+            d[c][r] = x;
+        }
+        __syncthreads();
+    }
+    // L21: column j of a row is (a_j - sum_{t<j} l_t U_tj) / U_jj; wave w owns the columns c = w mod 4
+    // and scales column c right after its last update (the step j = c - 1)
+    if (w == 0 && lane < nr && qs[0]) a[0][lane] = cmul_fx<R>(a[0][lane], qv[0]);
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        if (lane < nr) {
+            const c2<R> l = a[j][lane];
+            for (int c = j + 1 + ((w - (j + 1)) & 3); c < nb; c += 4) {
+                const c2<R> lu = cmul_fx<R>(l, d[c][j]);
+                c2<R> x = a[c][lane];
+                x.x -= lu.x;
+                x.y -= lu.y;
+                if (c == j + 1 && qs[c]) x = cmul_fx<R>(x, qv[c]);
+                a[c][lane] = x;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nb * 64; e += 256) {
+        const int c = e / 64, r = e % 64;
+        if (r < nr) A[(size_t)(k0 + c) * lda + r0 + r] = a[c][r];
+    }
+    if (g == 0) {
+        for (int e = tid; e < nb * nb; e += 256) {
+            const int c = e / nb, r = e % nb;
+            A[(size_t)(k0 + c) * lda + k0 + r] = d[c][r];
+        }
+        if (tid < nb) piv[k0 + tid] = k0 + tid;
     }
 }
 
@@ -1756,9 +1838,10 @@ static hipError_t lu_finish(hipStream_t s, LuWork* w, int* info, const char** ms
     return hipSuccess;
 }
 
+// nopiv: no interchanges (k_panel_nopiv panels, identity pivots), for butterfly-transformed matrices
 template <typename R>
 static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long long lda, int* piv, int* info,
-                              const char** msg) {
+                              const char** msg, bool nopiv = false) {
     hipError_t e = lu_work_init(w, msg, n);
     if (e != hipSuccess) return e;
     R* pinv = reinterpret_cast<R*>(w->pinv);
@@ -1783,10 +1866,10 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
     // The amount covers every panel of height at most m: FP32 panels switch from 128 to 64
     // rows per workgroup below 64 gmax rows, where their workgroup count goes up.
     auto reserve_for = [&](int m) { return std::min((m + 63) / 64, gmax); };
-    const bool tourn = w->tournament == 1;
+    const bool tourn = !nopiv && w->tournament == 1;
     const int two_level_from = (sizeof(R) == 4 && w->gemm16 != 0) ? kTwoLevelMinN32 : kTwoLevelMinN;
     const int outer = w->outer_nb > 0 ? w->outer_nb : (n >= two_level_from ? 512 : kNB);
-    const bool la2 = outer > kNB && w->look_ahead2 == 1 && w->side != nullptr && n > 2 * outer;
+    const bool la2 = !nopiv && outer > kNB && w->look_ahead2 == 1 && w->side != nullptr && n > 2 * outer;
     TsBufs tb{};
     if ((tourn || la2) && (e = ts_bufs(w, n, &tb)) != hipSuccess) return e;
     if (la2 && !w->bmaps && (e = hipMalloc(&w->bmaps, 8 * (2 * kNB + 1) * sizeof(int2))) != hipSuccess) return e;
@@ -1797,7 +1880,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         }
         rocblas_set_atomics_mode(w->blas_side, w->blas_atomics ? rocblas_atomics_allowed : rocblas_atomics_not_allowed);
     }
-    const bool lds_allowed = !tourn && w->panel_lds != 0 && w->n_cu > 0;
+    const bool lds_allowed = !nopiv && !tourn && w->panel_lds != 0 && w->n_cu > 0;
     bool lds = lds_allowed && (force || admit.take(w->dev, reserve_for(n), 2 * w->n_cu));
     const long long wait_ticks = (long long)w->panel_wait_us * (w->clk_khz > 0 ? w->clk_khz : 100000) / 1000;
     char* pb = static_cast<char*>(w->pbuf);
@@ -1814,6 +1897,11 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         if (!map) map = w->smap;
         int* mcount = reinterpret_cast<int*>(map + 2 * kNB);
         int* bad = smap_count + 1;   // one flag for every map
+        if (nopiv) {
+            hipLaunchKernelGGL(k_panel_nopiv<R>, dim3(max(1, (n - c1 + 63) / 64)), dim3(256), 0, st, n, k0, nb, A,
+                               lda, piv, w->info);
+            return;
+        }
         if (tourn || ts) {   // tournament pivoting: tree, interchanges, the panel's columns, L21
             const int GT = (n - k0 + kTsRows - 1) / kTsRows;
             c2<R>* u11 = static_cast<c2<R>*>(tb.u11);
@@ -1909,7 +1997,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
                 factor_panel(st, k0, ts, map);
                 hipError_t e2;
                 if ((e2 = hipGetLastError()) != hipSuccess) return e2;
-                const int ncol = (k0 - cb) + (ce - c1);
+                const int ncol = nopiv ? 0 : (k0 - cb) + (ce - c1);
                 if (ncol > 0)
                     hipLaunchKernelGGL(k_row_gather<R>, dim3((ncol + 3) / 4), dim3(256), 0, st, k0 - cb, c1 - cb,
                                        ce - cb, A + (size_t)cb * lda, lda, map,
@@ -1971,7 +2059,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         const int nb = min(kNB, n - k0);
         const int c1 = k0 + nb;
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (k0 + (n - c1) > 0)
+        if (!nopiv && k0 + (n - c1) > 0)
             hipLaunchKernelGGL(k_row_gather<R>, dim3((k0 + (n - c1) + 3) / 4), dim3(256), 0, s, k0, c1, n, A, lda,
                                w->smap, smap_count);
         if (c1 >= n) break;
@@ -2988,6 +3076,216 @@ hipError_t residual(hipStream_t s, LuWork* w, int n, const double* Ad, long long
     const size_t nv = (size_t)n * nrhs;
     hipLaunchKernelGGL(k_copy2, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, n, nrhs, B, ldb, R, ldr);
     return gemm_update(w, s, n, nrhs, n, A, lda, X, ldx, R, ldr, msg);
+}
+
+// ---------------------------------------------------------------------------
+// Random butterfly transformation (RBT) and LU without pivoting.  Parker, "Random butterfly
+// transformations with applications in computational linear algebra", UCLA CSD-950023
+// (1995); Baboulin, Dongarra, Herrmann, Tomov, "Accelerating linear system solutions using
+// randomization techniques", ACM TOMS 39(2), 2013.
+// A butterfly of order m is B = (1/sqrt 2) [R S; R -S] with R, S random diagonal matrices of
+// order m/2, entries exp(r/10), r uniform in [-1/2, 1/2].  Depth 2 on the leading
+// n4 = 4 floor(n/4) rows: U = diag(B', B'') B (the last n - n4 rows are left as they are).
+// With U and V of that kind, F = U^T A V is factored as L U without interchanges, and
+// A x = b is solved as x = V U^{-1} L^{-1} U^T b.  The random mixing makes the pivot growth of
+// the unpivoted LU behave as for a random matrix; the solve is refined against A.
+// Diagonals, per transform, 2 n4 doubles: d[0, n4) the outer butterfly's (R then S),
+// d[n4, 2 n4) the two inner ones (R' S' R'' S'').  Each thread applies one group of four
+// rows (or columns) i, i + n4/4, i + n4/2, i + 3 n4/4.
+
+template <typename R>
+__device__ __forceinline__ c2<R> rbt_scale(double f, double2 v) {
+    return mk2<R>((R)(f * v.x), (R)(f * v.y));
+}
+__device__ __forceinline__ double2 d2_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 d2_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 d2_mul(double f, double2 a) { return make_double2(f * a.x, f * a.y); }
+template <typename R>
+__device__ __forceinline__ double2 to_d2(c2<R> v) { return make_double2((double)v.x, (double)v.y); }
+
+// The four entries a0..a3 (rows or columns i, i + q, i + h, i + h + q) under U^T (or V, from
+// the right): the inner butterflies, then the outer one.
+__device__ __forceinline__ void rbt_t4(const double* d, int n4, int i, double2& a0, double2& a1, double2& a2,
+                                       double2& a3) {
+    const int q = n4 / 4, h = n4 / 2;
+    const double s = 0.70710678118654752440, *d1 = d + n4;
+    const double2 t0 = d2_mul(d1[i] * s, d2_add(a0, a1)), t1 = d2_mul(d1[q + i] * s, d2_sub(a0, a1));
+    const double2 t2 = d2_mul(d1[h + i] * s, d2_add(a2, a3)), t3 = d2_mul(d1[h + q + i] * s, d2_sub(a2, a3));
+    a0 = d2_mul(d[i] * s, d2_add(t0, t2));
+    a1 = d2_mul(d[q + i] * s, d2_add(t1, t3));
+    a2 = d2_mul(d[h + i] * s, d2_sub(t0, t2));
+    a3 = d2_mul(d[h + q + i] * s, d2_sub(t1, t3));
+}
+
+// dst = U^T src over n rows x ncol columns (rows n4..n-1 copied); src may be dst.
+template <typename Rs, typename Rd>
+__global__ void k_rbt_rows(int n, int n4, int ncol, const c2<Rs>* src, long long lds, c2<Rd>* dst, long long ldd,
+                           const double* __restrict__ d) {
+    const int q = n4 / 4, rest = n - n4;
+    const long long groups = (long long)q * ncol, tot = groups + (long long)rest * ncol;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        if (idx < groups) {
+            const int i = (int)(idx % q), c = (int)(idx / q);
+            const c2<Rs>* sc = src + (size_t)c * lds;
+            double2 a0 = to_d2<Rs>(sc[i]), a1 = to_d2<Rs>(sc[i + q]), a2 = to_d2<Rs>(sc[i + 2 * q]),
+                    a3 = to_d2<Rs>(sc[i + 3 * q]);
+            rbt_t4(d, n4, i, a0, a1, a2, a3);
+            c2<Rd>* dc = dst + (size_t)c * ldd;
+            dc[i] = rbt_scale<Rd>(1., a0);
+            dc[i + q] = rbt_scale<Rd>(1., a1);
+            dc[i + 2 * q] = rbt_scale<Rd>(1., a2);
+            dc[i + 3 * q] = rbt_scale<Rd>(1., a3);
+        } else {
+            const long long k = idx - groups;
+            const int i = n4 + (int)(k % rest), c = (int)(k / rest);
+            dst[(size_t)c * ldd + i] = rbt_scale<Rd>(1., to_d2<Rs>(src[(size_t)c * lds + i]));
+        }
+    }
+}
+
+// F = F V in place over nrow rows (columns n4.. untouched).
+template <typename R>
+__global__ void k_rbt_cols(int nrow, int n4, c2<R>* __restrict__ F, long long ldf, const double* __restrict__ d) {
+    const int q = n4 / 4;
+    const long long tot = (long long)q * nrow, stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        const int r = (int)(idx % nrow), j = (int)(idx / nrow);
+        c2<R>* p0 = F + (size_t)j * ldf + r;
+        c2<R>* p1 = p0 + (size_t)q * ldf;
+        c2<R>* p2 = p1 + (size_t)q * ldf;
+        c2<R>* p3 = p2 + (size_t)q * ldf;
+        double2 a0 = to_d2<R>(*p0), a1 = to_d2<R>(*p1), a2 = to_d2<R>(*p2), a3 = to_d2<R>(*p3);
+        rbt_t4(d, n4, j, a0, a1, a2, a3);
+        *p0 = rbt_scale<R>(1., a0);
+        *p1 = rbt_scale<R>(1., a1);
+        *p2 = rbt_scale<R>(1., a2);
+        *p3 = rbt_scale<R>(1., a3);
+    }
+}
+
+// X = V X in place (n4 leading rows of nrhs columns): the outer butterfly, then the inner ones.
+__global__ void k_rbt_v(int n4, int nrhs, double2* __restrict__ X, long long ldx, const double* __restrict__ d) {
+    const int q = n4 / 4, h = n4 / 2;
+    const long long tot = (long long)q * nrhs, stride = (long long)gridDim.x * blockDim.x;
+    const double s = 0.70710678118654752440, *d1 = d + n4;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        const int i = (int)(idx % q), c = (int)(idx / q);
+        double2* x = X + (size_t)c * ldx;
+        const double2 y0 = x[i], y1 = x[i + q], y2 = x[i + h], y3 = x[i + h + q];
+        const double2 z0 = d2_mul(s, d2_add(d2_mul(d[i], y0), d2_mul(d[h + i], y2)));
+        const double2 z2 = d2_mul(s, d2_sub(d2_mul(d[i], y0), d2_mul(d[h + i], y2)));
+        const double2 z1 = d2_mul(s, d2_add(d2_mul(d[q + i], y1), d2_mul(d[h + q + i], y3)));
+        const double2 z3 = d2_mul(s, d2_sub(d2_mul(d[q + i], y1), d2_mul(d[h + q + i], y3)));
+        x[i] = d2_mul(s, d2_add(d2_mul(d1[i], z0), d2_mul(d1[q + i], z1)));
+        x[i + q] = d2_mul(s, d2_sub(d2_mul(d1[i], z0), d2_mul(d1[q + i], z1)));
+        x[i + h] = d2_mul(s, d2_add(d2_mul(d1[h + i], z2), d2_mul(d1[h + q + i], z3)));
+        x[i + h + q] = d2_mul(s, d2_sub(d2_mul(d1[h + i], z2), d2_mul(d1[h + q + i], z3)));
+    }
+}
+
+__global__ void k_rbt_axpy(int n, int nrhs, const double2* __restrict__ D, long long ldd, double2* __restrict__ X,
+                           long long ldx) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int i = (int)(idx % n), r = (int)(idx / n);
+    const double2 a = D[(size_t)r * ldd + i];
+    double2& x = X[(size_t)r * ldx + i];
+    x.x += a.x;
+    x.y += a.y;
+}
+
+// The butterflies' diagonals from seed (host std::mt19937_64): U's then V's, 2 n4 each.
+static std::vector<double> rbt_diagonals(int n4, unsigned long long seed) {
+    std::vector<double> dd((size_t)4 * n4);
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> u(0., 1.);
+    for (double& x : dd) x = std::exp((u(gen) - 0.5) / 10.);
+    return dd;
+}
+
+hipError_t lu_factor_rbt(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, double* Fd, long long ldf,
+                         int* piv, double* rbt, unsigned long long seed, int* info, const char** msg) {
+    hipError_t e = lu_work_init(w, msg, n);
+    if (e != hipSuccess) return e;
+    const int n4 = n & ~3;
+    const std::vector<double> dd = rbt_diagonals(n4, seed);   // alive until lu_finish has synchronised
+    if (n4 > 0 &&
+        (e = hipMemcpyAsync(rbt, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    double2* F = reinterpret_cast<double2*>(Fd);
+    const long long rows_tot = (long long)(n4 / 4) * n + (long long)(n - n4) * n;
+    hipLaunchKernelGGL((k_rbt_rows<double, double>), dim3(stride_blocks(rows_tot)), dim3(256), 0, s, n, n4, n,
+                       reinterpret_cast<const double2*>(Ad), lda, F, ldf, (const double*)rbt);
+    if (n4 > 0)
+        hipLaunchKernelGGL(k_rbt_cols<double>, dim3(stride_blocks((long long)(n4 / 4) * n)), dim3(256), 0, s, n, n4,
+                           F, ldf, (const double*)(rbt + 2 * n4));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return lu_factor_t<double>(s, w, n, F, ldf, piv, info, msg, true);
+}
+
+hipError_t rbt_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const double* Fd,
+                     long long ldf, const int* piv, const double* rbt, int nrhs, double* Bd, long long ldb, double tol,
+                     int max_iter, int* iters, double* resid, int* status, const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    *iters = 0;
+    *resid = 0.;
+    *status = 0;
+    const int n4 = n & ~3;
+    const double2* A = reinterpret_cast<const double2*>(Ad);
+    double2* X = reinterpret_cast<double2*>(Bd);
+    const size_t nv = (size_t)n * nrhs, b_b = (nv * sizeof(double2) + 255) / 256 * 256;
+    const size_t need = 2 * b_b + 2 * nrhs * sizeof(double);
+    if (w->rbt_cap < need) {
+        (void)hipFree(w->rbt_buf);
+        w->rbt_buf = nullptr;
+        w->rbt_cap = 0;
+        if ((e = hipMalloc(&w->rbt_buf, need)) != hipSuccess) return e;
+        w->rbt_cap = need;
+    }
+    char* base = reinterpret_cast<char*>(w->rbt_buf);
+    double2* B0 = reinterpret_cast<double2*>(base);
+    double2* Rd = reinterpret_cast<double2*>(base + b_b);
+    double* nrm = reinterpret_cast<double*>(base + 2 * b_b);
+    const unsigned gv = (unsigned)((nv + 255) / 256);
+    const long long ldn = n;
+    // Y = V (L U)^{-1} U^T Y, in place (Y: n x nrhs, leading dimension ldy)
+    auto apply_inverse = [&](double2* Y, long long ldy) -> hipError_t {
+        const long long tot = (long long)(n4 / 4) * nrhs;
+        if (n4 > 0)
+            hipLaunchKernelGGL((k_rbt_rows<double, double>), dim3(stride_blocks(tot)), dim3(256), 0, s, n4, n4, nrhs,
+                               (const double2*)Y, ldy, Y, ldy, rbt);
+        hipError_t e2 = lu_solve(s, w, n, Fd, ldf, piv, nrhs, reinterpret_cast<double*>(Y), ldy, msg);
+        if (e2 != hipSuccess) return e2;
+        if (n4 > 0) hipLaunchKernelGGL(k_rbt_v, dim3(stride_blocks(tot)), dim3(256), 0, s, n4, nrhs, Y, ldy, rbt + 2 * n4);
+        return hipGetLastError();
+    };
+    std::vector<double> bn(nrhs), rn(nrhs);
+    hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, X, ldb, B0, ldn);
+    hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, B0, ldn, nrm);
+    if ((e = hipMemcpyAsync(bn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = apply_inverse(X, ldb)) != hipSuccess) return e;   // X0
+    double prev = HUGE_VAL;
+    for (int it = 0;; it++) {
+        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, ldn, Rd, ldn);
+        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, ldn, msg)) != hipSuccess) return e;   // B - A X
+        hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, ldn, nrm);
+        if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        double rel = 0.;
+        for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
+        *resid = rel;
+        if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }
+        if (rel <= tol) return hipSuccess;
+        // no further gain: at most max_iter corrections, each one shrinking the residual by 10%
+        if (it >= max_iter || !(rel < 0.9 * prev)) { *status = 1; return hipSuccess; }
+        prev = rel;
+        if ((e = apply_inverse(Rd, ldn)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rbt_axpy, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, ldn, X, ldb);
+        *iters = it + 1;
+    }
 }
 
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const float* Fd,

@@ -321,6 +321,25 @@ int bielcc_assemble_ex(bielcc_ctx ctx, int mesh_id, int which, const bielcc_num_
  * A_d rounded to FP32; A_d is not modified.  piv_d[n], info as bielcc_lu_factor. */
 int bielcc_lu32_factor(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf, int* piv_d,
                        int* info);
+/* LU without interchanges of a randomly transformed matrix (no reference counterpart:
+ * the reference has no solver).  F_d (n x n complex FP64, ldf) receives the factors
+ * L U = U^T A V, where U and V are random butterflies of depth 2 on the leading
+ * 4 floor(n/4) rows / columns (Parker 1995; Baboulin, Dongarra, Herrmann, Tomov, ACM
+ * TOMS 2013), drawn from seed; their diagonals go to rbt_d (device, 32 n bytes), piv_d
+ * receives the identity.  A_d is not modified (the solve refines against it).  Each
+ * panel is one launch in which no workgroup waits for another (the pivoting LU has a
+ * device-wide barrier per column).  info as bielcc_lu_factor (zero pivot of F). */
+int bielcc_lu_factor_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, double* F_d, int64_t ldf,
+                         int* piv_d, double* rbt_d, uint64_t seed, int* info);
+/* A X = B on bielcc_lu_factor_rbt's factors: X0 = V (L U)^{-1} U^T B, then classic
+ * refinement steps X += V (L U)^{-1} U^T (B - A X) until max_r ||B_r - A X_r|| / ||B_r||
+ * <= tol or max_iter steps.  B_d is overwritten by X.  Returns BIELCC_ERR_UNSUPPORTED when
+ * the residual stops shrinking (by less than 10% a step) above tol, or max_iter steps
+ * leave it there. */
+int bielcc_rbt_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const double* F_d, int64_t ldf,
+                     const int* piv_d, const double* rbt_d, int nrhs, double* B_d, int64_t ldb, double tol,
+                     int max_iter, int* iters, double* resid);
+
 /* Solves A X = B in FP64 on the FP32 factors, from X0 = F^{-1} B, until
  * max_r ||B_r - A X_r||_2 / ||B_r||_2 <= tol or max_iter corrections.  By
  * default (tunable "refine_mode" = 1) the corrections are the steps of a
