@@ -866,83 +866,102 @@ __global__ __launch_bounds__(256) void k_panel_lds(int n, int k0, int nb, c2<R>*
     }
 }
 
-// LU panel without pivoting (used on butterfly-transformed matrices, lu_factor_rbt).  Every
-// workgroup factors the nb x nb diagonal block in LDS itself -- the same code on the same
-// data, so the same bits everywhere -- and then solves its own 64 rows of
-// L21 = A21 U11^{-1}; workgroup 0 also writes the diagonal block (L11 \ U11), the identity
-// pivots and info.  No workgroup waits for another: one launch, no grid barrier.  The
-// arithmetic per column follows the pivoting kernels (the column scaled by the reciprocal
-// pivot, none for a zero pivot, which info reports; the rank-1 update x -= l u).
+// A lane's value of a complex at lane l (l uniform across the wave): v_readlane, no LDS.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane_d(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <typename R>
+__device__ __forceinline__ c2<R> readlane_c(c2<R> v, int l) {
+    return mk2<R>(readlane_d(v.x, l), readlane_d(v.y, l));
+}
+
+// Column c of a panel below its pivot p: scaled by 1/p (block rows > c, every A21 row; none for
+// a zero pivot, which info reports) and published in lcol[c & 1] for the next step.
+template <typename R>
+__device__ __forceinline__ void nopiv_finish_column(int c, c2<R> p, c2<R>& xa, c2<R>& xb, int lane, bool ha, bool vb,
+                                                    bool first_wg, int k0, int* __restrict__ info, c2<R> (*lcol)[128]) {
+    const R den = norm2_fx<R>(p);
+    if (den > R(0)) {
+        const c2<R> q = mk2<R>(p.x / den, -p.y / den);
+        if (lane > c && ha) xa = cmul_fx<R>(xa, q);
+        if (vb) xb = cmul_fx<R>(xb, q);
+    } else if (first_wg && lane == 0 && *info == 0) {
+        *info = k0 + c + 1;
+    }
+    lcol[c & 1][lane] = xa;
+    lcol[c & 1][64 + lane] = xb;
+}
+
+// LU panel without pivoting (used on butterfly-transformed matrices, lu_factor_rbt).  Each
+// workgroup eliminates, without interchanges, the tall panel made of the nb x nb diagonal
+// block and its own 64 rows of A21: the same code on the same diagonal block in every
+// workgroup, so the same bits everywhere.  Workgroup 0 writes the diagonal block
+// (L11 \ U11), the identity pivots and info; every workgroup writes its rows of L21.  No
+// workgroup waits for another: one launch, no grid barrier.
+// In registers: lane l holds block row l (xa) and A21 row l (xb) of the columns
+// c = w + 4 t of its wave w (t < 16, a static index).  Step j: row j of those columns comes
+// from lane j (v_readlane), column j's scaled entries from LDS (double-buffered by parity);
+// the owner of column j + 1 updates it, takes its pivot from lane j + 1, scales it and
+// publishes it for the next step: one workgroup barrier per column.  The arithmetic follows
+// the pivoting kernels (the column scaled by the reciprocal pivot, none for a zero pivot,
+// which info reports; the rank-1 update x -= l u).
 template <typename R>
 __global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
                                                      int* __restrict__ piv, int* __restrict__ info) {
-    __shared__ c2<R> d[kNB][kNB];    // diagonal block [column][row]
-    __shared__ c2<R> a[kNB][64];     // this workgroup's rows of A21 [column][row]
-    __shared__ c2<R> qv[kNB];        // reciprocal pivots
-    __shared__ int qs[kNB];          // pivot nonzero
+    constexpr int T = kNB / 4;
+    __shared__ c2<R> lcol[2][128];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = blockIdx.x;
     const int r0 = k0 + nb + 64 * g, nr = max(0, min(64, n - r0));
-    for (int e = tid; e < nb * nb; e += 256) {
-        const int c = e / nb, r = e % nb;
-        d[c][r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    const bool ha = lane < nb, vb = lane < nr;
+    const c2<R> zero = mk2<R>(R(0), R(0));
+    c2<R> xa[T], xb[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) {   // (an if, not ?: on two lvalues, which would select an address)
+        const int c = w + 4 * t;
+        xa[t] = zero;
+        xb[t] = zero;
+        if (c < nb && ha) xa[t] = A[(size_t)(k0 + c) * lda + k0 + lane];
+        if (c < nb && vb) xb[t] = A[(size_t)(k0 + c) * lda + r0 + lane];
     }
-    for (int e = tid; e < nb * 64; e += 256) {
-        const int c = e / 64, r = e % 64;
-        if (r < nr) a[c][r] = A[(size_t)(k0 + c) * lda + r0 + r];
-    }
+    if (w == 0) nopiv_finish_column<R>(0, readlane_c<R>(xa[0], 0), xa[0], xb[0], lane, ha, vb, g == 0, k0, info, lcol);
     __syncthreads();
-    for (int j = 0; j < nb; j++) {
-        const c2<R> p = d[j][j];
-        const R den = norm2_fx<R>(p);
-        const bool sc = den > R(0);
-        const c2<R> q = mk2<R>(sc ? p.x / den : R(0), sc ? -p.y / den : R(0));
-        if (tid > j && tid < nb && sc) d[j][tid] = cmul_fx<R>(d[j][tid], q);
-        if (tid == 0) {
-            qv[j] = q;
-            qs[j] = sc ? 1 : 0;
-            if (g == 0 && !sc && *info == 0) *info = k0 + j + 1;
-        }
-        __syncthreads();
-        const int m = nb - j - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            const int c = j + 1 + e / m, r = j + 1 + e % m;
-            const c2<R> lu = cmul_fx<R>(d[j][r], d[c][j]);
-            c2<R> x = d[c][r];
-            x.x -= lu.x;
-            x.y -= lu.y;
-            d[c][r] = x;
-        }
-        __syncthreads();
-    }
-    // L21: column j of a row is (a_j - sum_{t<j} l_t U_tj) / U_jj; wave w owns the columns c = w mod 4
-    // and scales column c right after its last update (the step j = c - 1)
-    if (w == 0 && lane < nr && qs[0]) a[0][lane] = cmul_fx<R>(a[0][lane], qv[0]);
-    __syncthreads();
-    for (int j = 0; j < nb; j++) {
-        if (lane < nr) {
-            const c2<R> l = a[j][lane];
-            for (int c = j + 1 + ((w - (j + 1)) & 3); c < nb; c += 4) {
-                const c2<R> lu = cmul_fx<R>(l, d[c][j]);
-                c2<R> x = a[c][lane];
-                x.x -= lu.x;
-                x.y -= lu.y;
-                if (c == j + 1 && qs[c]) x = cmul_fx<R>(x, qv[c]);
-                a[c][lane] = x;
+    for (int j = 0; j + 1 < nb; j++) {
+        const c2<R> la = lcol[j & 1][lane], lb = lcol[j & 1][64 + lane];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const int c = w + 4 * t;
+            if (c > j && c < nb) {
+                const c2<R> u = readlane_c<R>(xa[t], j);   // U(j, c)
+                if (lane > j) {
+                    const c2<R> lu = cmul_fx<R>(la, u);
+                    xa[t].x -= lu.x;
+                    xa[t].y -= lu.y;
+                }
+                const c2<R> lu = cmul_fx<R>(lb, u);
+                xb[t].x -= lu.x;
+                xb[t].y -= lu.y;
+                if (c == j + 1)
+                    nopiv_finish_column<R>(c, readlane_c<R>(xa[t], j + 1), xa[t], xb[t], lane, ha, vb, g == 0, k0, info,
+                                           lcol);
             }
         }
         __syncthreads();
     }
-    for (int e = tid; e < nb * 64; e += 256) {
-        const int c = e / 64, r = e % 64;
-        if (r < nr) A[(size_t)(k0 + c) * lda + r0 + r] = a[c][r];
-    }
-    if (g == 0) {
-        for (int e = tid; e < nb * nb; e += 256) {
-            const int c = e / nb, r = e % nb;
-            A[(size_t)(k0 + c) * lda + k0 + r] = d[c][r];
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const int c = w + 4 * t;
+        if (c < nb) {
+            if (vb) A[(size_t)(k0 + c) * lda + r0 + lane] = xb[t];
+            if (g == 0 && ha) A[(size_t)(k0 + c) * lda + k0 + lane] = xa[t];
         }
-        if (tid < nb) piv[k0 + tid] = k0 + tid;
     }
+    if (g == 0 && tid < nb) piv[k0 + tid] = k0 + tid;
 }
 
 // The interchanges of one panel, as a gather.  The nb swaps (row k0 + t with row
