@@ -888,7 +888,8 @@ __device__ __forceinline__ void nopiv_finish_column(int c, c2<R> p, c2<R>& xa, c
                                                     bool first_wg, int k0, int* __restrict__ info, c2<R> (*lcol)[128]) {
     const R den = norm2_fx<R>(p);
     if (den > R(0)) {
-        const c2<R> q = mk2<R>(p.x / den, -p.y / den);
+        const R rd = R(1) / den;
+        const c2<R> q = mk2<R>(p.x * rd, -p.y * rd);
         if (lane > c && ha) xa = cmul_fx<R>(xa, q);
         if (vb) xb = cmul_fx<R>(xb, q);
     } else if (first_wg && lane == 0 && *info == 0) {
@@ -916,7 +917,8 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R
                                                      int* __restrict__ piv, int* __restrict__ info) {
     constexpr int T = kNB / 4;
     __shared__ c2<R> lcol[2][128];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = blockIdx.x;
+    // w in a scalar register: the branches on the wave's columns below are then scalar
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), g = blockIdx.x;
     const int r0 = k0 + nb + 64 * g, nr = max(0, min(64, n - r0));
     const bool ha = lane < nb, vb = lane < nr;
     const c2<R> zero = mk2<R>(R(0), R(0));
