@@ -564,16 +564,20 @@ def lu_only(n, reps=3):
     dB = ctx.to_device(np.ascontiguousarray(AX.T))
     it_rbt, res_rbt = ctx.rbt_solve(dA0, n, n, dR, n, piv, rbt, dB, 4, n, tol=1e-13, max_iter=10)
     err_rbt = float(np.linalg.norm(dB.to_numpy((4, n), np.complex128).T - X) / np.linalg.norm(X))
+    dR.free()
+    dR = ctx.empty(n * n * 8)
+    t32rbt = [timed(lambda: ctx.lu32_factor_rbt(dA0, n, n, dR, n, piv, rbt, seed=1)) for _ in range(reps)]
     out = {"n": n, "lu64_s": round(min(t64), 4), "lu64_tflops": round(flop / min(t64) / 1e12, 2),
            "lu32_s": round(min(t32), 4), "lu32_tflops": round(flop / min(t32) / 1e12, 2),
            "backward_err_probe_fp64": be64, "backward_err_probe_fp32": be32,
            "lu64_rbt_s": round(min(trbt), 4), "lu64_rbt_tflops": round(flop / min(trbt) / 1e12, 2),
+           "lu32_rbt_s": round(min(t32rbt), 4),
            "rbt_solve_corrections": it_rbt, "rbt_solve_residual": res_rbt, "rbt_solve_error": err_rbt,
            "panel_aborts": ctx.get_tuning("lu_panel_aborts")}
     log(f"[bench] LU n={n}: FP64 {out['lu64_s']} s ({out['lu64_tflops']} TFLOP/s, backward {be64:.1e}), "
         f"FP32 {out['lu32_s']} s ({out['lu32_tflops']} TFLOP/s, backward {be32:.1e}), "
         f"FP64 butterfly + no interchanges {out['lu64_rbt_s']} s (solve: {it_rbt} corrections, residual "
-        f"{res_rbt:.1e}, error {err_rbt:.1e})")
+        f"{res_rbt:.1e}, error {err_rbt:.1e}), FP32 butterfly {out['lu32_rbt_s']} s")
     for d in (dA0, dA, piv, piv32, dR, rbt, dB):
         d.free()
     ctx.close()

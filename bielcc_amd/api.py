@@ -338,14 +338,28 @@ class Context:
                                                 C.byref(info)))
 
     def refine_solve(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
-                     B: DeviceArray, nrhs: int, ldb: int, tol=1e-10, max_iter=30):
+                     B: DeviceArray, nrhs: int, ldb: int, tol=1e-10, max_iter=30, rbt: DeviceArray = None):
         """FP64 solution of A X = B on FP32 factors (FGMRES preconditioned by them, or
-        classic refinement with tunable refine_mode = 0); returns (corrections, residual)."""
+        classic refinement with tunable refine_mode = 0); returns (corrections, residual).
+        rbt: the butterflies when F came from lu32_factor_rbt."""
         it, res = C.c_int(0), C.c_double(0.)
-        self._check(self.lib.bielcc_refine_solve(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv),
-                                                 int(nrhs), _p(B), int(ldb), float(tol), int(max_iter),
-                                                 C.byref(it), C.byref(res)))
+        if rbt is None:
+            rc = self.lib.bielcc_refine_solve(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv), int(nrhs),
+                                              _p(B), int(ldb), float(tol), int(max_iter), C.byref(it), C.byref(res))
+        else:
+            rc = self.lib.bielcc_refine_solve_rbt(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv), _p(rbt),
+                                                  int(nrhs), _p(B), int(ldb), float(tol), int(max_iter), C.byref(it),
+                                                  C.byref(res))
+        self._check(rc)
         return it.value, res.value
+
+    def lu32_factor_rbt(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
+                        rbt: DeviceArray, seed: int = 1) -> None:
+        """FP32 factors (complex64) of U^T A V without interchanges (butterflies as
+        lu_factor_rbt, diagonals into rbt, 32 n bytes); A is kept."""
+        info = C.c_int(0)
+        self._check(self.lib.bielcc_lu32_factor_rbt(self.h, int(n), _p(A), int(lda), _p(F), int(ldf), _p(piv),
+                                                    _p(rbt), int(seed), C.byref(info)))
 
     def lu_factor_rbt(self, A: DeviceArray, n: int, lda: int, F: DeviceArray, ldf: int, piv: DeviceArray,
                       rbt: DeviceArray, seed: int = 1) -> None:

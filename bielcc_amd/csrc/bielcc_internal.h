@@ -348,9 +348,13 @@ hipError_t lu_factor_rbt(hipStream_t s, LuWork* w, int n, const double* A, long 
 hipError_t rbt_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const double* F,
                      long long ldf, const int* piv, const double* rbt, int nrhs, double* B, long long ldb, double tol,
                      int max_iter, int* iters, double* resid, int* status, const char** msg);
+// FP32 factors of U^T A V (butterflies as lu_factor_rbt), without interchanges
+hipError_t lu32_factor_rbt(hipStream_t s, LuWork* w, int n, const double* A, long long lda, float* F, long long ldf,
+                           int* piv, double* rbt, unsigned long long seed, int* info, const char** msg);
+// rbt: the butterflies' diagonals when F factors U^T A V (lu32_factor_rbt), else null
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* A, long long lda, const float* F,
                         long long ldf, const int* piv, int nrhs, double* B, long long ldb, double tol, int max_iter,
-                        int* iters, double* resid, int* status, const char** msg);
+                        int* iters, double* resid, int* status, const char** msg, const double* rbt = nullptr);
 // C = alpha A B + beta C, complex FP64 column-major, on the BLAS handle of w
 // (matrix cores).  Creates the handle on first use: call once before a
 // stream capture.

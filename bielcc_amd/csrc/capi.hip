@@ -778,9 +778,9 @@ int bielcc_rbt_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, cons
     return BIELCC_OK;
 }
 
-int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
-                        const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
-                        int* iters, double* resid) {
+static int refine_solve_impl(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
+                             const int* piv_d, const double* rbt_d, int nrhs, double* B_d, int64_t ldb, double tol,
+                             int max_iter, int* iters, double* resid) {
     if (!ctx) return BIELCC_ERR_ARG;
     if (n <= 0 || nrhs <= 0 || !A_d || !F_d || !piv_d || !B_d || !iters || !resid || lda < n || ldf < n ||
         ldb < n || !(tol > 0.) || max_iter < 0)
@@ -790,7 +790,7 @@ int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, c
     int status = 0;
     Timer t(ctx, "refine_solve");
     hipError_t e = refine_solve(ctx->stream, ctx->lu, n, A_d, lda, F_d, ldf, piv_d, nrhs, B_d, ldb, tol, max_iter,
-                                iters, resid, &status, &msg);
+                                iters, resid, &status, &msg, rbt_d);
     if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
     t.stop();
     if (status != 0) {
@@ -805,6 +805,34 @@ int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, c
                           *resid, tol, *iters);
         return fail(ctx, BIELCC_ERR_UNSUPPORTED, buf);
     }
+    return BIELCC_OK;
+}
+
+int bielcc_refine_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
+                        const int* piv_d, int nrhs, double* B_d, int64_t ldb, double tol, int max_iter,
+                        int* iters, double* resid) {
+    return refine_solve_impl(ctx, n, A_d, lda, F_d, ldf, piv_d, nullptr, nrhs, B_d, ldb, tol, max_iter, iters, resid);
+}
+
+int bielcc_refine_solve_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
+                            const int* piv_d, const double* rbt_d, int nrhs, double* B_d, int64_t ldb, double tol,
+                            int max_iter, int* iters, double* resid) {
+    if (ctx && !rbt_d) return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    return refine_solve_impl(ctx, n, A_d, lda, F_d, ldf, piv_d, rbt_d, nrhs, B_d, ldb, tol, max_iter, iters, resid);
+}
+
+int bielcc_lu32_factor_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf,
+                           int* piv_d, double* rbt_d, uint64_t seed, int* info) {
+    if (!ctx) return BIELCC_ERR_ARG;
+    if (n <= 0 || !A_d || !F_d || !piv_d || !rbt_d || !info || lda < n || ldf < n)
+        return fail(ctx, BIELCC_ERR_ARG, "bad argument");
+    DeviceGuard g(ctx->device);
+    const char* msg = nullptr;
+    Timer t(ctx, "lu32_factor");
+    hipError_t e = lu32_factor_rbt(ctx->stream, ctx->lu, n, A_d, lda, F_d, ldf, piv_d, rbt_d, seed, info, &msg);
+    if (e != hipSuccess) return fail(ctx, BIELCC_ERR_HIP, msg ? msg : hipGetErrorString(e));
+    t.stop();
+    if (*info != 0) return fail(ctx, BIELCC_ERR_SINGULAR, "zero FP32 pivot (no interchanges) at column " + std::to_string(*info));
     return BIELCC_OK;
 }
 

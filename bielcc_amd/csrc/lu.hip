@@ -2878,6 +2878,256 @@ using cplx = std::complex<double>;
 inline cplx to_c(double2 v) { return {v.x, v.y}; }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Random butterfly transformation (RBT) and LU without pivoting.  Parker, "Random butterfly
+// transformations with applications in computational linear algebra", UCLA CSD-950023
+// (1995); Baboulin, Dongarra, Herrmann, Tomov, "Accelerating linear system solutions using
+// randomization techniques", ACM TOMS 39(2), 2013.
+// A butterfly of order m is B = (1/sqrt 2) [R S; R -S] with R, S random diagonal matrices of
+// order m/2, entries exp(r/10), r uniform in [-1/2, 1/2].  Depth 2 on the leading
+// n4 = 4 floor(n/4) rows: U = diag(B', B'') B (the last n - n4 rows are left as they are).
+// With U and V of that kind, F = U^T A V is factored as L U without interchanges, and
+// A x = b is solved as x = V U^{-1} L^{-1} U^T b.  The random mixing makes the pivot growth of
+// the unpivoted LU behave as for a random matrix; the solve is refined against A.
+// Diagonals, per transform, 2 n4 doubles: d[0, n4) the outer butterfly's (R then S),
+// d[n4, 2 n4) the two inner ones (R' S' R'' S'').  Each thread applies one group of four
+// rows (or columns) i, i + n4/4, i + n4/2, i + 3 n4/4.
+
+template <typename R>
+__device__ __forceinline__ c2<R> rbt_scale(double f, double2 v) {
+    return mk2<R>((R)(f * v.x), (R)(f * v.y));
+}
+__device__ __forceinline__ double2 d2_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 d2_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 d2_mul(double f, double2 a) { return make_double2(f * a.x, f * a.y); }
+template <typename R>
+__device__ __forceinline__ double2 to_d2(c2<R> v) { return make_double2((double)v.x, (double)v.y); }
+
+// The four entries a0..a3 (rows or columns i, i + q, i + h, i + h + q) under U^T (or V, from
+// the right): the inner butterflies, then the outer one.
+__device__ __forceinline__ void rbt_t4(const double* d, int n4, int i, double2& a0, double2& a1, double2& a2,
+                                       double2& a3) {
+    const int q = n4 / 4, h = n4 / 2;
+    const double s = 0.70710678118654752440, *d1 = d + n4;
+    const double2 t0 = d2_mul(d1[i] * s, d2_add(a0, a1)), t1 = d2_mul(d1[q + i] * s, d2_sub(a0, a1));
+    const double2 t2 = d2_mul(d1[h + i] * s, d2_add(a2, a3)), t3 = d2_mul(d1[h + q + i] * s, d2_sub(a2, a3));
+    a0 = d2_mul(d[i] * s, d2_add(t0, t2));
+    a1 = d2_mul(d[q + i] * s, d2_add(t1, t3));
+    a2 = d2_mul(d[h + i] * s, d2_sub(t0, t2));
+    a3 = d2_mul(d[h + q + i] * s, d2_sub(t1, t3));
+}
+
+// dst = U^T src over n rows x ncol columns (rows n4..n-1 copied); src may be dst.
+template <typename Rs, typename Rd>
+__global__ void k_rbt_rows(int n, int n4, int ncol, const c2<Rs>* src, long long lds, c2<Rd>* dst, long long ldd,
+                           const double* __restrict__ d) {
+    const int q = n4 / 4, rest = n - n4;
+    const long long groups = (long long)q * ncol, tot = groups + (long long)rest * ncol;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        if (idx < groups) {
+            const int i = (int)(idx % q), c = (int)(idx / q);
+            const c2<Rs>* sc = src + (size_t)c * lds;
+            double2 a0 = to_d2<Rs>(sc[i]), a1 = to_d2<Rs>(sc[i + q]), a2 = to_d2<Rs>(sc[i + 2 * q]),
+                    a3 = to_d2<Rs>(sc[i + 3 * q]);
+            rbt_t4(d, n4, i, a0, a1, a2, a3);
+            c2<Rd>* dc = dst + (size_t)c * ldd;
+            dc[i] = rbt_scale<Rd>(1., a0);
+            dc[i + q] = rbt_scale<Rd>(1., a1);
+            dc[i + 2 * q] = rbt_scale<Rd>(1., a2);
+            dc[i + 3 * q] = rbt_scale<Rd>(1., a3);
+        } else {
+            const long long k = idx - groups;
+            const int i = n4 + (int)(k % rest), c = (int)(k / rest);
+            dst[(size_t)c * ldd + i] = rbt_scale<Rd>(1., to_d2<Rs>(src[(size_t)c * lds + i]));
+        }
+    }
+}
+
+// F = F V in place over nrow rows (columns n4.. untouched).
+template <typename R>
+__global__ void k_rbt_cols(int nrow, int n4, c2<R>* __restrict__ F, long long ldf, const double* __restrict__ d) {
+    const int q = n4 / 4;
+    const long long tot = (long long)q * nrow, stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        const int r = (int)(idx % nrow), j = (int)(idx / nrow);
+        c2<R>* p0 = F + (size_t)j * ldf + r;
+        c2<R>* p1 = p0 + (size_t)q * ldf;
+        c2<R>* p2 = p1 + (size_t)q * ldf;
+        c2<R>* p3 = p2 + (size_t)q * ldf;
+        double2 a0 = to_d2<R>(*p0), a1 = to_d2<R>(*p1), a2 = to_d2<R>(*p2), a3 = to_d2<R>(*p3);
+        rbt_t4(d, n4, j, a0, a1, a2, a3);
+        *p0 = rbt_scale<R>(1., a0);
+        *p1 = rbt_scale<R>(1., a1);
+        *p2 = rbt_scale<R>(1., a2);
+        *p3 = rbt_scale<R>(1., a3);
+    }
+}
+
+// X = V X in place (n4 leading rows of nrhs columns): the outer butterfly, then the inner ones
+// (in FP64 arithmetic for FP32 vectors too).
+template <typename R>
+__global__ void k_rbt_v(int n4, int nrhs, c2<R>* __restrict__ X, long long ldx, const double* __restrict__ d) {
+    const int q = n4 / 4, h = n4 / 2;
+    const long long tot = (long long)q * nrhs, stride = (long long)gridDim.x * blockDim.x;
+    const double s = 0.70710678118654752440, *d1 = d + n4;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+        const int i = (int)(idx % q), c = (int)(idx / q);
+        c2<R>* x = X + (size_t)c * ldx;
+        const double2 y0 = to_d2<R>(x[i]), y1 = to_d2<R>(x[i + q]), y2 = to_d2<R>(x[i + h]), y3 = to_d2<R>(x[i + h + q]);
+        const double2 z0 = d2_mul(s, d2_add(d2_mul(d[i], y0), d2_mul(d[h + i], y2)));
+        const double2 z2 = d2_mul(s, d2_sub(d2_mul(d[i], y0), d2_mul(d[h + i], y2)));
+        const double2 z1 = d2_mul(s, d2_add(d2_mul(d[q + i], y1), d2_mul(d[h + q + i], y3)));
+        const double2 z3 = d2_mul(s, d2_sub(d2_mul(d[q + i], y1), d2_mul(d[h + q + i], y3)));
+        x[i] = rbt_scale<R>(s, d2_add(d2_mul(d1[i], z0), d2_mul(d1[q + i], z1)));
+        x[i + q] = rbt_scale<R>(s, d2_sub(d2_mul(d1[i], z0), d2_mul(d1[q + i], z1)));
+        x[i + h] = rbt_scale<R>(s, d2_add(d2_mul(d1[h + i], z2), d2_mul(d1[h + q + i], z3)));
+        x[i + h + q] = rbt_scale<R>(s, d2_sub(d2_mul(d1[h + i], z2), d2_mul(d1[h + q + i], z3)));
+    }
+}
+
+__global__ void k_rbt_axpy(int n, int nrhs, const double2* __restrict__ D, long long ldd, double2* __restrict__ X,
+                           long long ldx) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * nrhs) return;
+    const int i = (int)(idx % n), r = (int)(idx / n);
+    const double2 a = D[(size_t)r * ldd + i];
+    double2& x = X[(size_t)r * ldx + i];
+    x.x += a.x;
+    x.y += a.y;
+}
+
+// The butterflies' diagonals from seed (host std::mt19937_64): U's then V's, 2 n4 each.
+static std::vector<double> rbt_diagonals(int n4, unsigned long long seed) {
+    std::vector<double> dd((size_t)4 * n4);
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> u(0., 1.);
+    for (double& x : dd) x = std::exp((u(gen) - 0.5) / 10.);
+    return dd;
+}
+
+hipError_t lu_factor_rbt(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, double* Fd, long long ldf,
+                         int* piv, double* rbt, unsigned long long seed, int* info, const char** msg) {
+    hipError_t e = lu_work_init(w, msg, n);
+    if (e != hipSuccess) return e;
+    const int n4 = n & ~3;
+    const std::vector<double> dd = rbt_diagonals(n4, seed);   // alive until lu_finish has synchronised
+    if (n4 > 0 &&
+        (e = hipMemcpyAsync(rbt, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    double2* F = reinterpret_cast<double2*>(Fd);
+    const long long rows_tot = (long long)(n4 / 4) * n + (long long)(n - n4) * n;
+    hipLaunchKernelGGL((k_rbt_rows<double, double>), dim3(stride_blocks(rows_tot)), dim3(256), 0, s, n, n4, n,
+                       reinterpret_cast<const double2*>(Ad), lda, F, ldf, (const double*)rbt);
+    if (n4 > 0)
+        hipLaunchKernelGGL(k_rbt_cols<double>, dim3(stride_blocks((long long)(n4 / 4) * n)), dim3(256), 0, s, n, n4,
+                           F, ldf, (const double*)(rbt + 2 * n4));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return lu_factor_t<double>(s, w, n, F, ldf, piv, info, msg, true);
+}
+
+hipError_t rbt_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const double* Fd,
+                     long long ldf, const int* piv, const double* rbt, int nrhs, double* Bd, long long ldb, double tol,
+                     int max_iter, int* iters, double* resid, int* status, const char** msg) {
+    hipError_t e = lu_work_init(w, msg);
+    if (e != hipSuccess) return e;
+    *iters = 0;
+    *resid = 0.;
+    *status = 0;
+    const int n4 = n & ~3;
+    const double2* A = reinterpret_cast<const double2*>(Ad);
+    double2* X = reinterpret_cast<double2*>(Bd);
+    const size_t nv = (size_t)n * nrhs, b_b = (nv * sizeof(double2) + 255) / 256 * 256;
+    const size_t need = 2 * b_b + 2 * nrhs * sizeof(double);
+    if (w->rbt_cap < need) {
+        (void)hipFree(w->rbt_buf);
+        w->rbt_buf = nullptr;
+        w->rbt_cap = 0;
+        if ((e = hipMalloc(&w->rbt_buf, need)) != hipSuccess) return e;
+        w->rbt_cap = need;
+    }
+    char* base = reinterpret_cast<char*>(w->rbt_buf);
+    double2* B0 = reinterpret_cast<double2*>(base);
+    double2* Rd = reinterpret_cast<double2*>(base + b_b);
+    double* nrm = reinterpret_cast<double*>(base + 2 * b_b);
+    const unsigned gv = (unsigned)((nv + 255) / 256);
+    const long long ldn = n;
+    // Y = V (L U)^{-1} U^T Y, in place (Y: n x nrhs, leading dimension ldy)
+    auto apply_inverse = [&](double2* Y, long long ldy) -> hipError_t {
+        const long long tot = (long long)(n4 / 4) * nrhs;
+        if (n4 > 0)
+            hipLaunchKernelGGL((k_rbt_rows<double, double>), dim3(stride_blocks(tot)), dim3(256), 0, s, n4, n4, nrhs,
+                               (const double2*)Y, ldy, Y, ldy, rbt);
+        hipError_t e2 = lu_solve(s, w, n, Fd, ldf, piv, nrhs, reinterpret_cast<double*>(Y), ldy, msg);
+        if (e2 != hipSuccess) return e2;
+        if (n4 > 0)
+            hipLaunchKernelGGL(k_rbt_v<double>, dim3(stride_blocks(tot)), dim3(256), 0, s, n4, nrhs, Y, ldy, rbt + 2 * n4);
+        return hipGetLastError();
+    };
+    std::vector<double> bn(nrhs), rn(nrhs);
+    hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, X, ldb, B0, ldn);
+    hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, B0, ldn, nrm);
+    if ((e = hipMemcpyAsync(bn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = apply_inverse(X, ldb)) != hipSuccess) return e;   // X0
+    double prev = HUGE_VAL;
+    for (int it = 0;; it++) {
+        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, ldn, Rd, ldn);
+        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, ldn, msg)) != hipSuccess) return e;   // B - A X
+        hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, ldn, nrm);
+        if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        double rel = 0.;
+        for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
+        *resid = rel;
+        if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }
+        if (rel <= tol) return hipSuccess;
+        // no further gain: at most max_iter corrections, each one shrinking the residual by 10%
+        if (it >= max_iter || !(rel < 0.9 * prev)) { *status = 1; return hipSuccess; }
+        prev = rel;
+        if ((e = apply_inverse(Rd, ldn)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rbt_axpy, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, ldn, X, ldb);
+        *iters = it + 1;
+    }
+}
+
+hipError_t lu32_factor_rbt(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, float* Fd,
+                           long long ldf, int* piv, double* rbt, unsigned long long seed, int* info, const char** msg) {
+    hipError_t e = lu_work_init(w, msg, n);
+    if (e != hipSuccess) return e;
+    const int n4 = n & ~3;
+    const std::vector<double> dd = rbt_diagonals(n4, seed);   // alive until lu_finish has synchronised
+    if (n4 > 0 &&
+        (e = hipMemcpyAsync(rbt, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    float2* F = reinterpret_cast<float2*>(Fd);
+    const long long rows_tot = (long long)(n4 / 4) * n + (long long)(n - n4) * n;
+    hipLaunchKernelGGL((k_rbt_rows<double, float>), dim3(stride_blocks(rows_tot)), dim3(256), 0, s, n, n4, n,
+                       reinterpret_cast<const double2*>(Ad), lda, F, ldf, (const double*)rbt);
+    if (n4 > 0)
+        hipLaunchKernelGGL(k_rbt_cols<float>, dim3(stride_blocks((long long)(n4 / 4) * n)), dim3(256), 0, s, n, n4,
+                           F, ldf, (const double*)(rbt + 2 * n4));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return lu_factor_t<float>(s, w, n, F, ldf, piv, info, msg, true);
+}
+
+// Rf = F^{-1} Rf in place on FP32 factors (n x nrhs, leading dimension n); with butterflies
+// (rbt), F factors U^T A V and the inverse is V F^{-1} U^T
+static hipError_t precond32(hipStream_t s, LuWork* w, int n, const float2* F, long long ldf, int nrhs, float2* Rf,
+                            float2* tmp, const double* rbt, const char** msg) {
+    const int n4 = n & ~3;
+    const long long tot = (long long)(n4 / 4) * nrhs;
+    if (rbt && n4 > 0)
+        hipLaunchKernelGGL((k_rbt_rows<float, float>), dim3(stride_blocks(tot)), dim3(256), 0, s, n4, n4, nrhs,
+                           (const float2*)Rf, (long long)n, Rf, (long long)n, rbt);
+    hipError_t e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg);
+    if (e != hipSuccess) return e;
+    if (rbt && n4 > 0)
+        hipLaunchKernelGGL(k_rbt_v<float>, dim3(stride_blocks(tot)), dim3(256), 0, s, n4, nrhs, Rf, (long long)n,
+                           rbt + 2 * n4);
+    return hipGetLastError();
+}
+
 // Restarted flexible GMRES for A X = B0, right-preconditioned by the FP32 factors
 // (Arioli and Duff, "Using FGMRES to obtain backward stability in mixed precision",
 // ETNA 33, 2009).  The preconditioned directions Z_j = F^{-1} V_j are kept, so the
@@ -2895,7 +3145,7 @@ inline cplx to_c(double2 v) { return {v.x, v.y}; }
 static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long long lda, const float2* F,
                          long long ldf, int nrhs, double2* X, long long ldb, const double2* B0, double2* Rd,
                          float2* Rf, double* nrm, float2* tmp, const std::vector<double>& bn, double tol,
-                         int max_iter, int* iters, double* resid, int* status, const char** msg) {
+                         int max_iter, int* iters, double* resid, int* status, const double* rbt, const char** msg) {
     hipError_t e;
     const size_t blk = (size_t)n * nrhs;
     const unsigned gv = (unsigned)((blk + 255) / 256);
@@ -2992,7 +3242,7 @@ static hipError_t fgmres(hipStream_t s, LuWork* w, int n, const double2* A, long
             // Z_j = F^{-1} V_j (FP32 solve), V_{j+1} = A Z_j
             hipLaunchKernelGGL(k_to_f32, dim3(stride_blocks(blk)), dim3(256), 0, s, n, nrhs, Vj, (long long)n, Rf,
                                (long long)n);
-            if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+            if ((e = precond32(s, w, n, F, ldf, nrhs, Rf, tmp, rbt, msg)) != hipSuccess) return e;
             hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, Zj, (long long)n);
             if ((e = gemm_mul(w, s, n, nrhs, n, A, lda, Zj, (long long)n, Vn, (long long)n, msg)) != hipSuccess)
                 return e;
@@ -3099,219 +3349,9 @@ hipError_t residual(hipStream_t s, LuWork* w, int n, const double* Ad, long long
     return gemm_update(w, s, n, nrhs, n, A, lda, X, ldx, R, ldr, msg);
 }
 
-// ---------------------------------------------------------------------------
-// Random butterfly transformation (RBT) and LU without pivoting.  Parker, "Random butterfly
-// transformations with applications in computational linear algebra", UCLA CSD-950023
-// (1995); Baboulin, Dongarra, Herrmann, Tomov, "Accelerating linear system solutions using
-// randomization techniques", ACM TOMS 39(2), 2013.
-// A butterfly of order m is B = (1/sqrt 2) [R S; R -S] with R, S random diagonal matrices of
-// order m/2, entries exp(r/10), r uniform in [-1/2, 1/2].  Depth 2 on the leading
-// n4 = 4 floor(n/4) rows: U = diag(B', B'') B (the last n - n4 rows are left as they are).
-// With U and V of that kind, F = U^T A V is factored as L U without interchanges, and
-// A x = b is solved as x = V U^{-1} L^{-1} U^T b.  The random mixing makes the pivot growth of
-// the unpivoted LU behave as for a random matrix; the solve is refined against A.
-// Diagonals, per transform, 2 n4 doubles: d[0, n4) the outer butterfly's (R then S),
-// d[n4, 2 n4) the two inner ones (R' S' R'' S'').  Each thread applies one group of four
-// rows (or columns) i, i + n4/4, i + n4/2, i + 3 n4/4.
-
-template <typename R>
-__device__ __forceinline__ c2<R> rbt_scale(double f, double2 v) {
-    return mk2<R>((R)(f * v.x), (R)(f * v.y));
-}
-__device__ __forceinline__ double2 d2_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 d2_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 d2_mul(double f, double2 a) { return make_double2(f * a.x, f * a.y); }
-template <typename R>
-__device__ __forceinline__ double2 to_d2(c2<R> v) { return make_double2((double)v.x, (double)v.y); }
-
-// The four entries a0..a3 (rows or columns i, i + q, i + h, i + h + q) under U^T (or V, from
-// the right): the inner butterflies, then the outer one.
-__device__ __forceinline__ void rbt_t4(const double* d, int n4, int i, double2& a0, double2& a1, double2& a2,
-                                       double2& a3) {
-    const int q = n4 / 4, h = n4 / 2;
-    const double s = 0.70710678118654752440, *d1 = d + n4;
-    const double2 t0 = d2_mul(d1[i] * s, d2_add(a0, a1)), t1 = d2_mul(d1[q + i] * s, d2_sub(a0, a1));
-    const double2 t2 = d2_mul(d1[h + i] * s, d2_add(a2, a3)), t3 = d2_mul(d1[h + q + i] * s, d2_sub(a2, a3));
-    a0 = d2_mul(d[i] * s, d2_add(t0, t2));
-    a1 = d2_mul(d[q + i] * s, d2_add(t1, t3));
-    a2 = d2_mul(d[h + i] * s, d2_sub(t0, t2));
-    a3 = d2_mul(d[h + q + i] * s, d2_sub(t1, t3));
-}
-
-// dst = U^T src over n rows x ncol columns (rows n4..n-1 copied); src may be dst.
-template <typename Rs, typename Rd>
-__global__ void k_rbt_rows(int n, int n4, int ncol, const c2<Rs>* src, long long lds, c2<Rd>* dst, long long ldd,
-                           const double* __restrict__ d) {
-    const int q = n4 / 4, rest = n - n4;
-    const long long groups = (long long)q * ncol, tot = groups + (long long)rest * ncol;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
-        if (idx < groups) {
-            const int i = (int)(idx % q), c = (int)(idx / q);
-            const c2<Rs>* sc = src + (size_t)c * lds;
-            double2 a0 = to_d2<Rs>(sc[i]), a1 = to_d2<Rs>(sc[i + q]), a2 = to_d2<Rs>(sc[i + 2 * q]),
-                    a3 = to_d2<Rs>(sc[i + 3 * q]);
-            rbt_t4(d, n4, i, a0, a1, a2, a3);
-            c2<Rd>* dc = dst + (size_t)c * ldd;
-            dc[i] = rbt_scale<Rd>(1., a0);
-            dc[i + q] = rbt_scale<Rd>(1., a1);
-            dc[i + 2 * q] = rbt_scale<Rd>(1., a2);
-            dc[i + 3 * q] = rbt_scale<Rd>(1., a3);
-        } else {
-            const long long k = idx - groups;
-            const int i = n4 + (int)(k % rest), c = (int)(k / rest);
-            dst[(size_t)c * ldd + i] = rbt_scale<Rd>(1., to_d2<Rs>(src[(size_t)c * lds + i]));
-        }
-    }
-}
-
-// F = F V in place over nrow rows (columns n4.. untouched).
-template <typename R>
-__global__ void k_rbt_cols(int nrow, int n4, c2<R>* __restrict__ F, long long ldf, const double* __restrict__ d) {
-    const int q = n4 / 4;
-    const long long tot = (long long)q * nrow, stride = (long long)gridDim.x * blockDim.x;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
-        const int r = (int)(idx % nrow), j = (int)(idx / nrow);
-        c2<R>* p0 = F + (size_t)j * ldf + r;
-        c2<R>* p1 = p0 + (size_t)q * ldf;
-        c2<R>* p2 = p1 + (size_t)q * ldf;
-        c2<R>* p3 = p2 + (size_t)q * ldf;
-        double2 a0 = to_d2<R>(*p0), a1 = to_d2<R>(*p1), a2 = to_d2<R>(*p2), a3 = to_d2<R>(*p3);
-        rbt_t4(d, n4, j, a0, a1, a2, a3);
-        *p0 = rbt_scale<R>(1., a0);
-        *p1 = rbt_scale<R>(1., a1);
-        *p2 = rbt_scale<R>(1., a2);
-        *p3 = rbt_scale<R>(1., a3);
-    }
-}
-
-// X = V X in place (n4 leading rows of nrhs columns): the outer butterfly, then the inner ones.
-__global__ void k_rbt_v(int n4, int nrhs, double2* __restrict__ X, long long ldx, const double* __restrict__ d) {
-    const int q = n4 / 4, h = n4 / 2;
-    const long long tot = (long long)q * nrhs, stride = (long long)gridDim.x * blockDim.x;
-    const double s = 0.70710678118654752440, *d1 = d + n4;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
-        const int i = (int)(idx % q), c = (int)(idx / q);
-        double2* x = X + (size_t)c * ldx;
-        const double2 y0 = x[i], y1 = x[i + q], y2 = x[i + h], y3 = x[i + h + q];
-        const double2 z0 = d2_mul(s, d2_add(d2_mul(d[i], y0), d2_mul(d[h + i], y2)));
-        const double2 z2 = d2_mul(s, d2_sub(d2_mul(d[i], y0), d2_mul(d[h + i], y2)));
-        const double2 z1 = d2_mul(s, d2_add(d2_mul(d[q + i], y1), d2_mul(d[h + q + i], y3)));
-        const double2 z3 = d2_mul(s, d2_sub(d2_mul(d[q + i], y1), d2_mul(d[h + q + i], y3)));
-        x[i] = d2_mul(s, d2_add(d2_mul(d1[i], z0), d2_mul(d1[q + i], z1)));
-        x[i + q] = d2_mul(s, d2_sub(d2_mul(d1[i], z0), d2_mul(d1[q + i], z1)));
-        x[i + h] = d2_mul(s, d2_add(d2_mul(d1[h + i], z2), d2_mul(d1[h + q + i], z3)));
-        x[i + h + q] = d2_mul(s, d2_sub(d2_mul(d1[h + i], z2), d2_mul(d1[h + q + i], z3)));
-    }
-}
-
-__global__ void k_rbt_axpy(int n, int nrhs, const double2* __restrict__ D, long long ldd, double2* __restrict__ X,
-                           long long ldx) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)n * nrhs) return;
-    const int i = (int)(idx % n), r = (int)(idx / n);
-    const double2 a = D[(size_t)r * ldd + i];
-    double2& x = X[(size_t)r * ldx + i];
-    x.x += a.x;
-    x.y += a.y;
-}
-
-// The butterflies' diagonals from seed (host std::mt19937_64): U's then V's, 2 n4 each.
-static std::vector<double> rbt_diagonals(int n4, unsigned long long seed) {
-    std::vector<double> dd((size_t)4 * n4);
-    std::mt19937_64 gen(seed);
-    std::uniform_real_distribution<double> u(0., 1.);
-    for (double& x : dd) x = std::exp((u(gen) - 0.5) / 10.);
-    return dd;
-}
-
-hipError_t lu_factor_rbt(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, double* Fd, long long ldf,
-                         int* piv, double* rbt, unsigned long long seed, int* info, const char** msg) {
-    hipError_t e = lu_work_init(w, msg, n);
-    if (e != hipSuccess) return e;
-    const int n4 = n & ~3;
-    const std::vector<double> dd = rbt_diagonals(n4, seed);   // alive until lu_finish has synchronised
-    if (n4 > 0 &&
-        (e = hipMemcpyAsync(rbt, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess)
-        return e;
-    double2* F = reinterpret_cast<double2*>(Fd);
-    const long long rows_tot = (long long)(n4 / 4) * n + (long long)(n - n4) * n;
-    hipLaunchKernelGGL((k_rbt_rows<double, double>), dim3(stride_blocks(rows_tot)), dim3(256), 0, s, n, n4, n,
-                       reinterpret_cast<const double2*>(Ad), lda, F, ldf, (const double*)rbt);
-    if (n4 > 0)
-        hipLaunchKernelGGL(k_rbt_cols<double>, dim3(stride_blocks((long long)(n4 / 4) * n)), dim3(256), 0, s, n, n4,
-                           F, ldf, (const double*)(rbt + 2 * n4));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return lu_factor_t<double>(s, w, n, F, ldf, piv, info, msg, true);
-}
-
-hipError_t rbt_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const double* Fd,
-                     long long ldf, const int* piv, const double* rbt, int nrhs, double* Bd, long long ldb, double tol,
-                     int max_iter, int* iters, double* resid, int* status, const char** msg) {
-    hipError_t e = lu_work_init(w, msg);
-    if (e != hipSuccess) return e;
-    *iters = 0;
-    *resid = 0.;
-    *status = 0;
-    const int n4 = n & ~3;
-    const double2* A = reinterpret_cast<const double2*>(Ad);
-    double2* X = reinterpret_cast<double2*>(Bd);
-    const size_t nv = (size_t)n * nrhs, b_b = (nv * sizeof(double2) + 255) / 256 * 256;
-    const size_t need = 2 * b_b + 2 * nrhs * sizeof(double);
-    if (w->rbt_cap < need) {
-        (void)hipFree(w->rbt_buf);
-        w->rbt_buf = nullptr;
-        w->rbt_cap = 0;
-        if ((e = hipMalloc(&w->rbt_buf, need)) != hipSuccess) return e;
-        w->rbt_cap = need;
-    }
-    char* base = reinterpret_cast<char*>(w->rbt_buf);
-    double2* B0 = reinterpret_cast<double2*>(base);
-    double2* Rd = reinterpret_cast<double2*>(base + b_b);
-    double* nrm = reinterpret_cast<double*>(base + 2 * b_b);
-    const unsigned gv = (unsigned)((nv + 255) / 256);
-    const long long ldn = n;
-    // Y = V (L U)^{-1} U^T Y, in place (Y: n x nrhs, leading dimension ldy)
-    auto apply_inverse = [&](double2* Y, long long ldy) -> hipError_t {
-        const long long tot = (long long)(n4 / 4) * nrhs;
-        if (n4 > 0)
-            hipLaunchKernelGGL((k_rbt_rows<double, double>), dim3(stride_blocks(tot)), dim3(256), 0, s, n4, n4, nrhs,
-                               (const double2*)Y, ldy, Y, ldy, rbt);
-        hipError_t e2 = lu_solve(s, w, n, Fd, ldf, piv, nrhs, reinterpret_cast<double*>(Y), ldy, msg);
-        if (e2 != hipSuccess) return e2;
-        if (n4 > 0) hipLaunchKernelGGL(k_rbt_v, dim3(stride_blocks(tot)), dim3(256), 0, s, n4, nrhs, Y, ldy, rbt + 2 * n4);
-        return hipGetLastError();
-    };
-    std::vector<double> bn(nrhs), rn(nrhs);
-    hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, X, ldb, B0, ldn);
-    hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, B0, ldn, nrm);
-    if ((e = hipMemcpyAsync(bn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-    if ((e = apply_inverse(X, ldb)) != hipSuccess) return e;   // X0
-    double prev = HUGE_VAL;
-    for (int it = 0;; it++) {
-        hipLaunchKernelGGL(k_copy2, dim3(gv), dim3(256), 0, s, n, nrhs, B0, ldn, Rd, ldn);
-        if ((e = gemm_update(w, s, n, nrhs, n, A, lda, X, ldb, Rd, ldn, msg)) != hipSuccess) return e;   // B - A X
-        hipLaunchKernelGGL(k_colnorm, dim3(nrhs), dim3(256), 0, s, n, Rd, ldn, nrm);
-        if ((e = hipMemcpyAsync(rn.data(), nrm, nrhs * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
-            return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-        double rel = 0.;
-        for (int r = 0; r < nrhs; r++) rel = std::max(rel, bn[r] > 0. ? rn[r] / bn[r] : rn[r]);
-        *resid = rel;
-        if (!std::isfinite(rel)) { *status = 1; return hipSuccess; }
-        if (rel <= tol) return hipSuccess;
-        // no further gain: at most max_iter corrections, each one shrinking the residual by 10%
-        if (it >= max_iter || !(rel < 0.9 * prev)) { *status = 1; return hipSuccess; }
-        prev = rel;
-        if ((e = apply_inverse(Rd, ldn)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rbt_axpy, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, ldn, X, ldb);
-        *iters = it + 1;
-    }
-}
-
 hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long long lda, const float* Fd,
                         long long ldf, const int* piv, int nrhs, double* Bd, long long ldb, double tol, int max_iter,
-                        int* iters, double* resid, int* status, const char** msg) {
+                        int* iters, double* resid, int* status, const char** msg, const double* rbt) {
     hipError_t e = lu_work_init(w, msg);
     if (e != hipSuccess) return e;
     *iters = 0;
@@ -3350,11 +3390,11 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long 
     auto attempt = [&]() -> hipError_t {
         // X0 = F^{-1} B
         hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, B0, (long long)n, Rf, (long long)n);
-        if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+        if ((e = precond32(s, w, n, F, ldf, nrhs, Rf, tmp, rbt, msg)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_from_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
         if (w->refine_mode == 1)
             return fgmres(s, w, n, A, lda, F, ldf, nrhs, X, ldb, B0, Rd, Rf, nrm, tmp, bn, tol, max_iter, iters, resid,
-                          status, msg);
+                          status, rbt, msg);
         double prev = HUGE_VAL, rho_min = HUGE_VAL;
         for (int it = 0;; it++) {
             // Rd = B0 - A X (FP64)
@@ -3391,7 +3431,7 @@ hipError_t refine_solve(hipStream_t s, LuWork* w, int n, const double* Ad, long 
             prev = rel;
             // X += F^{-1} Rd
             hipLaunchKernelGGL(k_to_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rd, (long long)n, Rf, (long long)n);
-            if ((e = lu_solve_perm<float>(s, w, n, F, ldf, nrhs, Rf, (long long)n, tmp, msg)) != hipSuccess) return e;
+            if ((e = precond32(s, w, n, F, ldf, nrhs, Rf, tmp, rbt, msg)) != hipSuccess) return e;
             hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(256), 0, s, n, nrhs, Rf, X, ldb);
             *iters = it + 1;
         }

@@ -339,6 +339,16 @@ int bielcc_lu_factor_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, 
 int bielcc_rbt_solve(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const double* F_d, int64_t ldf,
                      const int* piv_d, const double* rbt_d, int nrhs, double* B_d, int64_t ldb, double tol,
                      int max_iter, int* iters, double* resid);
+/* FP32 factors of U^T A V (butterflies as bielcc_lu_factor_rbt), without interchanges:
+ * F_d complex float, piv_d the identity, rbt_d the diagonals; A_d is not modified.  The
+ * FP32 counterpart of bielcc_lu32_factor, for bielcc_refine_solve_rbt. */
+int bielcc_lu32_factor_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, float* F_d, int64_t ldf,
+                           int* piv_d, double* rbt_d, uint64_t seed, int* info);
+/* bielcc_refine_solve on bielcc_lu32_factor_rbt's factors: every FP32 solve applies
+ * V F^{-1} U^T.  Arguments and results as bielcc_refine_solve. */
+int bielcc_refine_solve_rbt(bielcc_ctx ctx, int n, const double* A_d, int64_t lda, const float* F_d, int64_t ldf,
+                            const int* piv_d, const double* rbt_d, int nrhs, double* B_d, int64_t ldb, double tol,
+                            int max_iter, int* iters, double* resid);
 
 /* Solves A X = B in FP64 on the FP32 factors, from X0 = F^{-1} B, until
  * max_r ||B_r - A X_r||_2 / ||B_r||_2 <= tol or max_iter corrections.  By

@@ -116,3 +116,57 @@ def test_zero_matrix_reports_the_zero_pivot(ctx):
     with pytest.raises(B.BielccError) as e:
         _factor(ctx, np.zeros((n, n), np.complex128))
     assert "column 1" in str(e.value)
+
+
+def _factor32(ctx, A, seed=7):
+    n = A.shape[0]
+    dA = ctx.to_device(np.ascontiguousarray(A.T))
+    dF, piv, rbt = ctx.empty(n * n * 8), ctx.empty(4 * n), ctx.empty(32 * n)
+    ctx.lu32_factor_rbt(dA, n, n, dF, n, piv, rbt, seed=seed)
+    return dA, dF, piv, rbt
+
+
+@pytest.mark.parametrize("n", [1000, 1003])
+def test_fp32_factors_of_the_transformed_matrix(ctx, n):
+    """FP32 factors of U^T A V: backward error of FP32 size, as the pivoting FP32 LU's."""
+    rng = np.random.default_rng(300 + n)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    dA, dF, piv, rbt = _factor32(ctx, A)
+    F = dF.to_numpy((n, n), np.complex64).T.astype(np.complex128)
+    d = rbt.to_numpy((4 * (n & ~3),), np.float64)
+    n4 = n & ~3
+    T = _dense(d[:2 * n4], n).T @ A @ _dense(d[2 * n4:], n)
+    err = np.linalg.norm(T - (np.tril(F, -1) + np.eye(n)) @ np.triu(F)) / np.linalg.norm(T)
+    assert err <= 1e-7 * n, err
+    assert np.array_equal(piv.to_numpy((n,), np.int32), np.arange(n))
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("shape", ["random", "sphere"])
+def test_fp32_refinement_on_transformed_factors(ctx, mode, shape):
+    """FGMRES (1) and classic steps (0) on the FP32 butterfly factors reach a residual at
+    most LAPACK's, with the solution within the conditioning of LAPACK's."""
+    rng = np.random.default_rng(17)
+    if shape == "random":
+        n = 1500
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        m = None
+    else:
+        m = ctx.mesh(B.mesh.sphere_quads(16, 32, seed=1))
+        A = m.assemble(B.num_param(), 2.0)
+        n = A.shape[0]
+    b = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    r_lapack, xt = _lapack_residual(A, b)
+    dA, dF, piv, rbt = _factor32(ctx, A)
+    dB = ctx.to_device(np.ascontiguousarray(b.T))
+    old = ctx.get_tuning("refine_mode")
+    ctx.set_tuning("refine_mode", mode)
+    try:
+        it, res = ctx.refine_solve(dA, n, n, dF, n, piv, dB, 2, n, tol=max(r_lapack, 1e-13), max_iter=40, rbt=rbt)
+    finally:
+        ctx.set_tuning("refine_mode", old)
+    X = dB.to_numpy((2, n), np.complex128).T
+    assert res <= max(r_lapack, 1e-13), (it, res, r_lapack)
+    assert np.linalg.norm(X - xt) / np.linalg.norm(xt) <= 1e-15 * np.linalg.cond(A)
+    if m is not None:
+        m.close()
