@@ -416,10 +416,12 @@ def _warm_lu32(ctx, n=1536):
     ctx.set_tuning("lu_outer_nb", 512)
     try:
         dA, dF, dp = ctx.to_device(np.ascontiguousarray(A.T)), ctx.empty(n * n * 8), ctx.empty(4 * n)
+        rbt = ctx.empty(32 * n)
         ctx.lu32_factor(dA, n, n, dF, n, dp)
+        ctx.lu32_factor_rbt(dA, n, n, dF, n, dp, rbt)
         ctx.lu_factor(dA, n, n, dp)
         ctx.sync()
-        for d in (dA, dF, dp):
+        for d in (dA, dF, dp, rbt):
             d.free()
     finally:
         ctx.set_tuning("lu_outer_nb", old)
@@ -457,6 +459,7 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
     dB = ctx.empty(n2 * 16)
     dJ = ctx.empty(n * 48)
     dE, dH = ctx.empty(M * 48), ctx.empty(M * 48)
+    rbt = ctx.empty(32 * n2)
     t_asm = timed(lambda: m.assemble_into(dZ, n2, npp, k))
     t_lu = timed(lambda: ctx.lu32_factor(dZ, n2, n2, dF, n2, dpiv))
     m.rhs_planewave_into(dB, n2, kvec, e0)
@@ -484,13 +487,26 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
     nearfar = near_far_errors(ctx, m, npp, k, dJ64, dJ, mesh.sphere_points(10000, 1.5, seed=21), far_directions())
     rowrel = lambda a, b: float((np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)).max())
     glob = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    for d in (dZ, dF, dpiv, dB, dJ, dE, dH):
+    # for reference: the FP32 factors without interchanges of the butterfly-transformed matrix
+    t_lu_rbt = timed(lambda: ctx.lu32_factor_rbt(dZ, n2, n2, dF, n2, dpiv, rbt))
+    m.rhs_planewave_into(dB, n2, kvec, e0)
+    res_rbt = {}
+    try:
+        t_ref_rbt = timed(lambda: res_rbt.update(zip(("iters", "resid"), ctx.refine_solve(
+            dZ, n2, n2, dF, n2, dpiv, dB, 1, n2, 1e-10, 60, rbt=rbt))))
+    except B.BielccError as err:
+        if err.code != 4:
+            raise
+        t_ref_rbt, res_rbt = float("nan"), {"iters": -1, "resid": float("nan")}
+    for d in (dZ, dF, dpiv, dB, dJ, dE, dH, rbt):
         d.free()
     cold64 = step64_s + lu64_s + solve64_s   # FP64 assembly + field (one timed step) + LU + solve
     cold32 = t_asm + t_lu + t_ref + t_fld
-    return {"form": "FP64 matrix, FP32 LU (rocBLAS cgemm on MFMA) + FP64 refinement to 1e-10, FP32 far-pair "
-                    "field sums (near pairs, contours, cross-cell sums FP64)",
+    return {"form": "FP64 matrix; FP32 LU (rank-512 updates as FP16 GEMMs of split operands) + FP64 FGMRES "
+                    "refinement to 1e-10; mixed far-pair field sums (near pairs, contours, cross-cell sums FP64)",
             "assembly_s": round(t_asm, 4), "lu32_factor_s": round(t_lu, 3),
+            "butterfly_lu32_s": round(t_lu_rbt, 3), "butterfly_refine_s": round(t_ref_rbt, 4),
+            "butterfly_refine_corrections": res_rbt["iters"],
             "lu32_tflops": round(8.0 / 3.0 * n2 ** 3 / t_lu / 1e12, 2),
             "refine_s": round(t_ref, 4), "refine_corrections": res["iters"], "refine_residual": res["resid"],
             "field_s": round(t_fld, 4), "field_pairs_per_s": M * n / t_fld, "field_far_sums": field_sums,

@@ -104,7 +104,7 @@ struct LuWork {
     int look_ahead = 2;             // next panel on the side stream during the trailing update: 1 always, 0 never,
                                     // 2 for FP64 above kLookAheadMinN
     // whole-panel kernel (k_panel_lds): barrier words, published candidates, settings
-    unsigned* psync = nullptr;      // [3]: barrier word, workgroups gone, aborted panels
+    unsigned* psync = nullptr;      // [4]: barrier word, workgroups gone, aborted panels; k_panel_nopiv's counter
     int2* smap = nullptr;           // the current panel's interchanges as a gather (2 kNB entries + count)
     void* pbuf = nullptr;           // candidates: cval, cidx, crow, rowj
     int panel_lds = 1;              // whole-panel kernel: 1 within the workgroup budget, 0 never, 2 always
@@ -902,9 +902,12 @@ __device__ __forceinline__ void nopiv_finish_column(int c, c2<R> p, c2<R>& xa, c
 // LU panel without pivoting (used on butterfly-transformed matrices, lu_factor_rbt).  Each
 // workgroup eliminates, without interchanges, the tall panel made of the nb x nb diagonal
 // block and its own 64 rows of A21: the same code on the same diagonal block in every
-// workgroup, so the same bits everywhere.  Workgroup 0 writes the diagonal block
-// (L11 \ U11), the identity pivots and info; every workgroup writes its rows of L21.  No
-// workgroup waits for another: one launch, no grid barrier.
+// workgroup, so the same bits everywhere.  Every workgroup writes its rows of L21; workgroup
+// 0 the identity pivots and info; the diagonal block (L11 \ U11) is written by the last
+// workgroup to finish (counter done, back to 0 after): every workgroup reads the block from A
+// when it starts, and a workgroup scheduled late (more workgroups than fit, or other kernels
+// on the device) must still read it unfactored.  No workgroup waits for another: one launch,
+// no grid barrier.
 // In registers: lane l holds block row l (xa) and A21 row l (xb) of the columns
 // c = w + 4 t of its wave w (t < 16, a static index).  Step j: row j of those columns comes
 // from lane j (v_readlane), column j's scaled entries from LDS (double-buffered by parity);
@@ -914,7 +917,8 @@ __device__ __forceinline__ void nopiv_finish_column(int c, c2<R> p, c2<R>& xa, c
 // which info reports; the rank-1 update x -= l u).
 template <typename R>
 __global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
-                                                     int* __restrict__ piv, int* __restrict__ info) {
+                                                     int* __restrict__ piv, int* __restrict__ info,
+                                                     unsigned* __restrict__ done) {
     constexpr int T = kNB / 4;
     __shared__ c2<R> lcol[2][128];
     // w in a scalar register: the branches on the wave's columns below are then scalar
@@ -958,12 +962,21 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R
 #pragma unroll
     for (int t = 0; t < T; t++) {
         const int c = w + 4 * t;
-        if (c < nb) {
-            if (vb) A[(size_t)(k0 + c) * lda + r0 + lane] = xb[t];
-            if (g == 0 && ha) A[(size_t)(k0 + c) * lda + k0 + lane] = xa[t];
-        }
+        if (c < nb && vb) A[(size_t)(k0 + c) * lda + r0 + lane] = xb[t];
     }
     if (g == 0 && tid < nb) piv[k0 + tid] = k0 + tid;
+    __shared__ int s_last;
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1u;
+    __syncthreads();
+    if (s_last) {   // every other workgroup has finished, its read of the block included
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const int c = w + 4 * t;
+            if (c < nb && ha) A[(size_t)(k0 + c) * lda + k0 + lane] = xa[t];
+        }
+        if (tid == 0) atomicExch(done, 0u);
+    }
 }
 
 // The interchanges of one panel, as a gather.  The nb swaps (row k0 + t with row
@@ -1920,7 +1933,7 @@ static hipError_t lu_factor_t(hipStream_t s, LuWork* w, int n, c2<R>* A, long lo
         int* bad = smap_count + 1;   // one flag for every map
         if (nopiv) {
             hipLaunchKernelGGL(k_panel_nopiv<R>, dim3(max(1, (n - c1 + 63) / 64)), dim3(256), 0, st, n, k0, nb, A,
-                               lda, piv, w->info);
+                               lda, piv, w->info, w->psync + 3);
             return;
         }
         if (tourn || ts) {   // tournament pivoting: tree, interchanges, the panel's columns, L21

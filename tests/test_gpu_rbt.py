@@ -170,3 +170,45 @@ def test_fp32_refinement_on_transformed_factors(ctx, mode, shape):
     assert np.linalg.norm(X - xt) / np.linalg.norm(xt) <= 1e-15 * np.linalg.cond(A)
     if m is not None:
         m.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_concurrent_factorisations_match_alone(prec):
+    """Four contexts factor at once on four host threads (orders 6,000-9,000: up to 141
+    workgroups per panel launch each, more than the device holds together), so workgroups
+    of a panel start late while others have finished: the factors equal those of the same
+    factorisations run alone, bit for bit.  (Every workgroup reads the diagonal block
+    unfactored; the last one to finish writes it.)"""
+    import threading
+    ns = [9000, 8000, 7000, 6000]
+    rng = np.random.default_rng(41)
+    mats = [rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) for n in ns]
+    ctxs = [B.Context(0) for _ in ns]
+    item = 8 if prec == "fp32" else 16
+
+    def run(i, out):
+        c, n = ctxs[i], ns[i]
+        dA = c.to_device(np.ascontiguousarray(mats[i].T))
+        dF, piv, rbt = c.empty(n * n * item), c.empty(4 * n), c.empty(32 * n)
+        if prec == "fp32":
+            c.lu32_factor_rbt(dA, n, n, dF, n, piv, rbt, seed=5)
+        else:
+            c.lu_factor_rbt(dA, n, n, dF, n, piv, rbt, seed=5)
+        c.sync()
+        out[i] = dF.to_numpy((n * n * item // 8,), np.float64)
+        for d in (dA, dF, piv, rbt):
+            d.free()
+
+    alone = {}
+    for i in range(len(ns)):
+        run(i, alone)
+    together = {}
+    th = [threading.Thread(target=run, args=(i, together)) for i in range(len(ns))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for c in ctxs:
+        c.close()
+    for i in range(len(ns)):
+        assert np.array_equal(alone[i], together[i]), ns[i]
