@@ -264,12 +264,6 @@ class Server:
     meshes: name -> vertex array [n][3|4][3].  form: "fp64" (FP64 LU) or
     "mixed" (FP32 LU + FP64 refinement, FP32 far-pair field sums).
 
-    butterfly (mixed form): the FP32 factors are those of U^T Z V, random butterflies U, V,
-    without interchanges (lu32_factor_rbt: no device-wide barrier per panel column), and
-    the refinement applies V F^{-1} U^T.  Off by default: they factor 2-3x faster, but
-    without pivoting the FP32 factors of ill-conditioned systems precondition worse (more
-    corrections, and more entries that need FP64 factors, each charged beyond the budget).
-
     refine_max_iter: corrections (FP32 solves after the first) a refinement may take.
 
     refine_ferr_exp = e > 0: a solve does not stop at refine_tol alone but goes on
@@ -285,12 +279,11 @@ class Server:
 
     def __init__(self, meshes: dict, form: str = "mixed", workers: int = 3, hbm_budget: float = 200e9,
                  num_param=None, refine_tol: float = 1e-10, device: int = 0, refine_ferr_exp: int = 7,
-                 refine_max_iter: int = 60, butterfly: bool = False):
+                 refine_max_iter: int = 60):
         if form not in ("fp64", "mixed"):
             raise ValueError("form must be 'fp64' or 'mixed'")
         self.meshes = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in meshes.items()}
         self.form = form
-        self.butterfly = bool(butterfly) and form == "mixed"
         self.np = num_param if num_param is not None else api.num_param()
         self.refine_tol = refine_tol
         self.refine_max_iter = int(refine_max_iter)
@@ -391,7 +384,7 @@ class Server:
     def _factor_bytes(self, n: int) -> int:
         n2 = 2 * n
         per = 16 + 8 if self.form == "mixed" else 16   # FP64 matrix (+ FP32 factors)
-        return per * n2 * n2 + 4 * n2 + (32 * n2 if self.butterfly else 0)
+        return per * n2 * n2 + 4 * n2
 
     def _build(self, w: int, m: api.Mesh, k: float, e: _Entry) -> None:
         ctx = self.ctxs[w]
@@ -403,11 +396,7 @@ class Server:
         if self.form == "mixed":
             e.bufs["F"] = ctx.empty(n2 * n2 * 8)
             try:
-                if self.butterfly:
-                    e.bufs["rbt"] = ctx.empty(32 * n2)
-                    ctx.lu32_factor_rbt(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], e.bufs["rbt"])
-                else:
-                    ctx.lu32_factor(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"])
+                ctx.lu32_factor(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"])
             except api.BielccError as err:
                 if err.code != 5:          # only a zero FP32 pivot is handled here
                     raise
@@ -533,7 +522,7 @@ class Server:
             if self.form == "mixed" and not fallback:
                 try:
                     iters, resid = ctx.refine_solve(e.bufs["Z"], n2, n2, e.bufs["F"], n2, e.bufs["piv"], dB, nrhs,
-                                                    n2, self.refine_tol, self.refine_max_iter, rbt=e.bufs.get("rbt"))
+                                                    n2, self.refine_tol, self.refine_max_iter)
                 except api.BielccError as err:
                     if err.code != 4:      # refinement stalled: FP64 factors for this entry
                         raise
