@@ -28,6 +28,7 @@ struct DevMesh {
     int n = 0, nv = 0;
     double grid_step = 0.;   // max cell diameter (SurfMesh::GetGridStep, mesh/surf_mesh.h:322-328)
     double max_rad = 0.;     // max distance from a cell's centre to its vertices (bounds node offsets)
+    double bc[3] = {0., 0., 0.}, br = 0.;   // a sphere around every cell centre (centre, radius)
     double* verts = nullptr; // [n][nv][3]
     double* ctab = nullptr;  // [n][CT]
     double* nodes = nullptr; // [n][nn][4]: sub-cell centroid (3) + weight
@@ -56,11 +57,23 @@ hipError_t launch_local32(hipStream_t s, int n, int cnt, const double* nodes, co
 // ---- near lists (tables.hip)
 // mode 0: near iff dist(q, center_c) < ac * diam_c
 // mode 1: near iff !(dist(q, center_c) > thresh)
+// far: the sphere around every cell centre (far[0..2] centre, far[3] radius) plus the largest
+// near distance (far[4]: ac * max diam, or thresh); a workgroup whose points all lie beyond it
+// skips the scan (no pair of theirs can be near).  null: no shortcut.
 hipError_t launch_near_count(hipStream_t s, int nq, const double* q, int q_stride, int n, const double* ctab,
-                             int mode, double ac, double thresh, int* counts);
+                             int mode, double ac, double thresh, int* counts, const double* far = nullptr);
 hipError_t launch_scan(hipStream_t s, int n, const int* in, long long* out_excl, long long* total);
 hipError_t launch_near_fill(hipStream_t s, int nq, const double* q, int q_stride, int n, const double* ctab,
-                            int mode, double ac, double thresh, const long long* offs, int* list_q, int* list_c);
+                            int mode, double ac, double thresh, const long long* offs, int* list_q, int* list_c,
+                            const double* far = nullptr);
+
+// The far shortcut of a mesh for a near rule: {centre, radius, reach} (see launch_near_count).
+struct NearFar {
+    double v[5];
+};
+inline NearFar near_far(const DevMesh& m, int mode, double ac, double thresh) {
+    return NearFar{{m.bc[0], m.bc[1], m.bc[2], m.br, mode == 0 ? ac * m.grid_step : thresh}};
+}
 
 // ---- field (field.hip)
 struct FieldArgs {

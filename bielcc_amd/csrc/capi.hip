@@ -140,6 +140,24 @@ int bielcc_mesh_create(bielcc_ctx ctx, int n_cells, int nv, const double* verts,
     }
     m.grid_step = gs;
     m.max_rad = mr;
+    // a sphere around every cell centre: the centre of their bounding box, the largest distance
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int c = 0; c < n_cells; c++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], tab[(size_t)c * CT + CT_CENTER + d]);
+            hi[d] = std::max(hi[d], tab[(size_t)c * CT + CT_CENTER + d]);
+        }
+    for (int d = 0; d < 3; d++) m.bc[d] = 0.5 * (lo[d] + hi[d]);
+    double br2 = 0.;
+    for (int c = 0; c < n_cells; c++) {
+        double d2 = 0.;
+        for (int d = 0; d < 3; d++) {
+            const double x = tab[(size_t)c * CT + CT_CENTER + d] - m.bc[d];
+            d2 += x * x;
+        }
+        br2 = std::max(br2, d2);
+    }
+    m.br = std::sqrt(br2);
     int id = ctx->next_id++;
     ctx->meshes[id] = m;
     *mesh_id = id;

@@ -208,7 +208,8 @@ struct Timer {
 inline int build_near_list(bielcc_ctx ctx, int nq, const double* q, int q_stride, const DevMesh& m, int mode, double ac,
                     double thresh, int* counts, long long* offs, long long* total_d, long long* total_h,
                     int** lq, int** lc) {
-    HIPCHK(launch_near_count(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, counts));
+    const NearFar nf = near_far(m, mode, ac, thresh);
+    HIPCHK(launch_near_count(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, counts, nf.v));
     HIPCHK(launch_scan(ctx->stream, nq, counts, offs, total_d));
     HIPCHK(hipMemcpyAsync(total_h, total_d, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -218,7 +219,7 @@ inline int build_near_list(bielcc_ctx ctx, int nq, const double* q, int q_stride
         HIPCHK(ensure(ctx->lists, ctx->lists_cap, align256((size_t)T * sizeof(int)) * 2));
         *lq = reinterpret_cast<int*>(ctx->lists);
         *lc = reinterpret_cast<int*>(reinterpret_cast<char*>(ctx->lists) + align256((size_t)T * sizeof(int)));
-        HIPCHK(launch_near_fill(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, offs, *lq, *lc));
+        HIPCHK(launch_near_fill(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, offs, *lq, *lc, nf.v));
     }
     return BIELCC_OK;
 }

@@ -111,7 +111,9 @@ int bielcc_field_noalloc(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np
     a.nsplit = nsplit; a.part = part;
     HIPCHK(launch_field_far(ctx->stream, a));
     // near pairs: count, scan, then the lists and results in the rest of the scratch
-    HIPCHK(launch_near_count(ctx->stream, M, pts_d, 3, m->n, m->ctab, near_mode, np->analytic_calc, thresh, counts));
+    const NearFar nf = near_far(*m, near_mode, np->analytic_calc, thresh);
+    HIPCHK(launch_near_count(ctx->stream, M, pts_d, 3, m->n, m->ctab, near_mode, np->analytic_calc, thresh, counts,
+                             nf.v));
     HIPCHK(launch_scan(ctx->stream, M, counts, offs, tot_d));
     long long T = 0;
     HIPCHK(hipMemcpyAsync(&T, tot_d, sizeof T, hipMemcpyDeviceToHost, ctx->stream));
@@ -129,7 +131,7 @@ int bielcc_field_noalloc(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np
         lc = reinterpret_cast<int*>(base + fixed + l_b);
         near_out = reinterpret_cast<double*>(base + fixed + 2 * l_b);
         HIPCHK(launch_near_fill(ctx->stream, M, pts_d, 3, m->n, m->ctab, near_mode, np->analytic_calc, thresh, offs,
-                                lq, lc));
+                                lq, lc, nf.v));
         HIPCHK(launch_field_near(ctx->stream, m->nv, T, lq, lc, pts_d, m->verts, jE_d, jM_d, to_np(np), k_re, k_im,
                                  near_mode, 0, near_out));
     }

@@ -110,7 +110,8 @@ int plan_near(bielcc_ctx ctx, bielcc_plan p, int nq, const double* q, int q_stri
     long long* offs = reinterpret_cast<long long*>(reinterpret_cast<char*>(tmp) + cnt_b);
     long long* tot_d = reinterpret_cast<long long*>(reinterpret_cast<char*>(tmp) + cnt_b + off_b);
     long long tot = 0;
-    hipError_t e = launch_near_count(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, counts);
+    const NearFar nf = near_far(m, mode, ac, thresh);
+    hipError_t e = launch_near_count(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, counts, nf.v);
     if (e == hipSuccess) e = launch_scan(ctx->stream, nq, counts, offs, tot_d);
     if (e == hipSuccess) e = hipMemcpyAsync(&tot, tot_d, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -118,7 +119,8 @@ int plan_near(bielcc_ctx ctx, bielcc_plan p, int nq, const double* q, int q_stri
         e = palloc(p, (size_t)tot * sizeof(int), &out->lq);
         if (e == hipSuccess) e = palloc(p, (size_t)tot * sizeof(int), &out->lc);
         if (e == hipSuccess)
-            e = launch_near_fill(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, offs, out->lq, out->lc);
+            e = launch_near_fill(ctx->stream, nq, q, q_stride, m.n, m.ctab, mode, ac, thresh, offs, out->lq, out->lc,
+                                 nf.v);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         out->n = tot;
     }

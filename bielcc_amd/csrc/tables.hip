@@ -207,14 +207,32 @@ hipError_t launch_local32(hipStream_t s, int n, int cnt, const double* nodes, co
 // ---------------------------------------------------------------- near lists
 constexpr int kNearTile = 256;
 
+// The point is farther from every cell centre than the largest near distance: beyond the
+// sphere around the centres (fr[0..2], radius fr[3]) by more than the reach fr[4], with a
+// relative margin for rounding.  NaN coordinates are never far (the scan decides).
+struct FarArgs {
+    double c0, c1, c2, r, reach;
+    int on;
+};
+__device__ inline bool point_far(const double* x, const FarArgs& f) {
+    if (!f.on) return false;
+    const double d0 = x[0] - f.c0, d1 = x[1] - f.c1, d2 = x[2] - f.c2;
+    const double lim = (f.r + f.reach) * (1. + 1e-12);
+    return d0 * d0 + d1 * d1 + d2 * d2 > lim * lim;
+}
+
 // One thread per query point; cell centres and diameters staged through LDS.
 __global__ __launch_bounds__(256) void k_near_count(int nq, const double* __restrict__ q, int q_stride, int n,
                                                     const double* __restrict__ ctab, int mode, double ac,
-                                                    double thresh, int* __restrict__ counts) {
+                                                    double thresh, int* __restrict__ counts, FarArgs fa) {
     __shared__ double sc[kNearTile][4];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double x[3] = {0., 0., 0.};
     if (i < nq) for (int d = 0; d < 3; d++) x[d] = q[(size_t)i * q_stride + d];
+    if (__syncthreads_and(i >= nq || point_far(x, fa))) {   // no near pair in this workgroup
+        if (i < nq) counts[i] = 0;
+        return;
+    }
     int cnt = 0;
     for (int c0 = 0; c0 < n; c0 += kNearTile) {
         int c = c0 + threadIdx.x;
@@ -237,11 +255,12 @@ __global__ __launch_bounds__(256) void k_near_count(int nq, const double* __rest
 __global__ __launch_bounds__(256) void k_near_fill(int nq, const double* __restrict__ q, int q_stride, int n,
                                                    const double* __restrict__ ctab, int mode, double ac,
                                                    double thresh, const long long* __restrict__ offs,
-                                                   int* __restrict__ lq, int* __restrict__ lc) {
+                                                   int* __restrict__ lq, int* __restrict__ lc, FarArgs fa) {
     __shared__ double sc[kNearTile][4];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double x[3] = {0., 0., 0.};
     if (i < nq) for (int d = 0; d < 3; d++) x[d] = q[(size_t)i * q_stride + d];
+    if (__syncthreads_and(i >= nq || point_far(x, fa))) return;   // no near pair in this workgroup
     long long pos = (i < nq) ? offs[i] : 0;
     for (int c0 = 0; c0 < n; c0 += kNearTile) {
         int c = c0 + threadIdx.x;
@@ -264,19 +283,26 @@ __global__ __launch_bounds__(256) void k_near_fill(int nq, const double* __restr
     }
 }
 
+static FarArgs far_args(const double* far) {
+    FarArgs f{0., 0., 0., 0., 0., 0};
+    if (far) f = FarArgs{far[0], far[1], far[2], far[3], far[4], 1};
+    return f;
+}
+
 hipError_t launch_near_count(hipStream_t s, int nq, const double* q, int q_stride, int n, const double* ctab,
-                             int mode, double ac, double thresh, int* counts) {
+                             int mode, double ac, double thresh, int* counts, const double* far) {
     if (nq <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_near_count, dim3((nq + 255) / 256), dim3(256), 0, s, nq, q, q_stride, n, ctab, mode,
-                       ac, thresh, counts);
+                       ac, thresh, counts, far_args(far));
     return hipGetLastError();
 }
 
 hipError_t launch_near_fill(hipStream_t s, int nq, const double* q, int q_stride, int n, const double* ctab,
-                            int mode, double ac, double thresh, const long long* offs, int* lq, int* lc) {
+                            int mode, double ac, double thresh, const long long* offs, int* lq, int* lc,
+                            const double* far) {
     if (nq <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_near_fill, dim3((nq + 255) / 256), dim3(256), 0, s, nq, q, q_stride, n, ctab, mode, ac,
-                       thresh, offs, lq, lc);
+                       thresh, offs, lq, lc, far_args(far));
     return hipGetLastError();
 }
 

@@ -48,6 +48,32 @@ def test_near_field_vs_oracle(ctx, orc, near_mode):
     m.close()
 
 
+@pytest.mark.parametrize("near_mode", [0, 1])
+def test_near_list_shortcut_at_its_boundary(ctx, orc, near_mode):
+    """The near-pair scan is skipped by workgroups of 256 points that all lie beyond the
+    sphere around the cell centres plus the largest near distance (3 grid steps, both
+    rules).  One workgroup of points just inside that boundary (scanned, near pairs among
+    them) and one just beyond it (skipped): the field at every point equals the oracle's."""
+    rng = np.random.default_rng(23)
+    verts = B.mesh.sphere_quads(10, 20, jitter=0.005, seed=5)
+    n = verts.shape[0]
+    m = ctx.mesh(verts)
+    c = m.tables()["center"]
+    bc = 0.5 * (c.min(axis=0) + c.max(axis=0))
+    lim = np.linalg.norm(c - bc, axis=1).max() + 3.0 * m.grid_step()
+    d = rng.normal(size=(512, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.concatenate([np.sort(rng.uniform(lim - 0.25, lim - 1e-3, 256)), np.sort(rng.uniform(lim + 1e-3, lim + 0.25, 256))])
+    pts = bc + r[:, None] * d
+    jE = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    k = 3.0
+    E, H = m.field(B.num_param(), k, jE, pts, near_mode=near_mode)
+    Eo, Ho = orc.field_eh(verts, jE, k, O.num_param(), pts, near_mode=near_mode)
+    assert rowwise_rel(E, Eo).max() <= TOL
+    assert rowwise_rel(H, Ho).max() <= TOL
+    m.close()
+
+
 def test_triangles_complex_k_vs_oracle(ctx, orc):
     rng = np.random.default_rng(22)
     verts = B.mesh.triangulate(B.mesh.sphere_quads(8, 16, jitter=0.005, seed=6))
