@@ -1489,6 +1489,8 @@ def main():
         out["triangle_variant"] = triangle_variant()
     if lu_s is not None and not args.no_mixed:
         out["precision_form_cube"] = cube_precision()
+    if not args.no_lu:   # a mid-size factorisation: partial pivoting against the butterfly form
+        out["lu_order_6000"] = lu_only(6000)
     if not args.no_batched:
         out["batched"] = batched_leg(args.tune_cache, args.batched_workers)
     if not args.no_serve:
