@@ -881,6 +881,14 @@ __device__ __forceinline__ c2<R> readlane_c(c2<R> v, int l) {
     return mk2<R>(readlane_d(v.x, l), readlane_d(v.y, l));
 }
 
+// x -= l u (complex) as four fused multiply-adds.  (The panel without interchanges has no
+// other kernel to agree with bit for bit; every workgroup runs this same code.)
+template <typename R>
+__device__ __forceinline__ void nopiv_update(c2<R>& x, c2<R> l, c2<R> u) {
+    x.x = fma(l.y, u.y, fma(-l.x, u.x, x.x));
+    x.y = fma(-l.y, u.x, fma(-l.x, u.y, x.y));
+}
+
 // Column c of a panel below its pivot p: scaled by 1/p (block rows > c, every A21 row; none for
 // a zero pivot, which info reports) and published in lcol[c & 1] for the next step.
 template <typename R>
@@ -912,9 +920,9 @@ __device__ __forceinline__ void nopiv_finish_column(int c, c2<R> p, c2<R>& xa, c
 // c = w + 4 t of its wave w (t < 16, a static index).  Step j: row j of those columns comes
 // from lane j (v_readlane), column j's scaled entries from LDS (double-buffered by parity);
 // the owner of column j + 1 updates it, takes its pivot from lane j + 1, scales it and
-// publishes it for the next step: one workgroup barrier per column.  The arithmetic follows
-// the pivoting kernels (the column scaled by the reciprocal pivot, none for a zero pivot,
-// which info reports; the rank-1 update x -= l u).
+// publishes it for the next step: one workgroup barrier per column.  The column is scaled by
+// the reciprocal pivot (none for a zero pivot, which info reports); the rank-1 update
+// x -= l u is four fused multiply-adds per entry.
 template <typename R>
 __global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R>* __restrict__ A, long long lda,
                                                      int* __restrict__ piv, int* __restrict__ info,
@@ -944,14 +952,8 @@ __global__ __launch_bounds__(256) void k_panel_nopiv(int n, int k0, int nb, c2<R
             const int c = w + 4 * t;
             if (c > j && c < nb) {
                 const c2<R> u = readlane_c<R>(xa[t], j);   // U(j, c)
-                if (lane > j) {
-                    const c2<R> lu = cmul_fx<R>(la, u);
-                    xa[t].x -= lu.x;
-                    xa[t].y -= lu.y;
-                }
-                const c2<R> lu = cmul_fx<R>(lb, u);
-                xb[t].x -= lu.x;
-                xb[t].y -= lu.y;
+                if (lane > j) nopiv_update<R>(xa[t], la, u);
+                nopiv_update<R>(xb[t], lb, u);
                 if (c == j + 1)
                     nopiv_finish_column<R>(c, readlane_c<R>(xa[t], j + 1), xa[t], xb[t], lane, ha, vb, g == 0, k0, info,
                                            lcol);
