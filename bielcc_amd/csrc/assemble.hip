@@ -148,7 +148,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
             cplx z[2][2];
             if (WHICH == 0) {
                 if (near_rule(dcen, rec[3], 0, a.ac, 0.)) continue;   // near pair: exact kernel writes it
-                if (PH == 2) pc = phase_centre(kr_, dcen);
+                if (PH >= 2) pc = phase_centre(kr_, dcen);
                 // S = int e^{ikr}/r  (F_SimplePot_H, smoothR = DBL_EPSILON: factor 1/r)
                 double Sr = 0., Si = 0.;
                 const double4* nd = ssurf + cc * nn;
@@ -204,7 +204,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                     if (!P32) rot_q(Sr, Si, pr.q);
                     for (int b = 0; b < 2; b++)
                         for (int g = 0; g < 3; g++) rot_q(k0b[b][g].re, k0b[b][g].im, pr.q);
-                } else if (PH == 2) {   // ... or e^{i k dcen}
+                } else if (PH >= 2) {   // ... or e^{i k dcen}
                     if (!P32) rot_c(Sr, Si, pc);
                     for (int b = 0; b < 2; b++)
                         for (int g = 0; g < 3; g++) rot_c(k0b[b][g].re, k0b[b][g].im, pc);
@@ -227,7 +227,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                     // In mixed mode LDS holds the FP32 table, so they read the FP64
                     // nodes from global memory (few pairs, L2-resident).
                     const bool use32 = P32 && !near_rule(dcen, rec[3], 0, a.ac, 0.);
-                    if (PH == 2 && !use32) pc = phase_centre(kr_, dcen);
+                    if (PH >= 2 && !use32) pc = phase_centre(kr_, dcen);
                     const double4* nd = P32 ? reinterpret_cast<const double4*>(a.nodes) + (size_t)c * nn
                                             : ssurf + cc * nn;
                     if (use32) {
@@ -253,7 +253,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                     }
                     if (PH == 1 && !use32)
                         for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
-                    if (PH == 2 && !use32)
+                    if (PH >= 2 && !use32)
                         for (int g = 0; g < 3; g++) rot_c(Gr[g], Gi[g], pc);
                     const cplx G[3] = {mk(Gr[0], Gi[0]), mk(Gr[1], Gi[1]), mk(Gr[2], Gi[2])};
                     for (int b = 0; b < 2; b++) {
@@ -336,7 +336,8 @@ static void assemble_far_ph(hipStream_t s, const AssembleArgs& a, int s_min) {
 
 template <bool CK, int W, bool P32>
 static void assemble_far_p(hipStream_t s, const AssembleArgs& a, int s_min) {
-    if (a.fast_phase == 2) assemble_far_ph<CK, W, 2, P32>(s, a, s_min);
+    if (a.fast_phase == 3) assemble_far_ph<CK, W, 3, P32>(s, a, s_min);
+    else if (a.fast_phase == 2) assemble_far_ph<CK, W, 2, P32>(s, a, s_min);
     else if (a.fast_phase) assemble_far_ph<CK, W, 1, P32>(s, a, s_min);
     else assemble_far_ph<CK, W, 0, P32>(s, a, s_min);
 }
@@ -352,6 +353,7 @@ bool fast_phase_ok(double k_re, double grid_step) {
 }
 
 int phase_mode(double k_re, double grid_step, double max_rad) {
+    if (fabs(k_re) * max_rad <= kVShortRange) return 3;
     if (fabs(k_re) * max_rad <= kShortRange) return 2;
     return fast_phase_ok(k_re, grid_step) ? 1 : 0;
 }

@@ -92,7 +92,7 @@ struct FieldArgs {
     double ac, thresh;
     int nsplit;
     double* part;           // [nsplit][nrhs][NACC][M]
-    int fast_phase;         // node phase mode: 0 general sincos, 1 quadrant reduction, 2 centre-phase (phase.h)
+    int fast_phase;         // node phase mode: 0 general sincos, 1 quadrant reduction, 2 and 3 centre-phase (phase.h)
     int prec;               // 0: FP64 node sums; 1: FP32 cell-local node sums
 };
 hipError_t launch_field_far(hipStream_t s, const FieldArgs& a);
@@ -232,7 +232,8 @@ struct AssembleArgs {
 // pi/4 + |k| * (max cell diameter) bounds every reduced node phase (phase.h)
 bool fast_phase_ok(double k_re, double grid_step);
 // node phase mode of the tile kernels: 2 (centre-phase reduction, short series) when
-// |k| max_rad <= 0.5, else 1 (quadrant reduction) when fast_phase_ok, else 0
+// 3 when |k| max_rad <= 0.25, 2 when <= 0.5 (centre phase), else 1 (quadrant reduction) when
+// fast_phase_ok, else 0
 int phase_mode(double k_re, double grid_step, double max_rad);
 hipError_t launch_assemble_far(hipStream_t s, const AssembleArgs& a);
 // Persistent far-pair kernel over several meshes: items_d[tile.x] with point

@@ -126,7 +126,7 @@ __device__ __forceinline__ void pair_sums64(double x0, double x1, double x2, con
     PhaseRef pr{0., 0., 0., 0};
     PhaseC pc{0., 0., 1., 0.};
     if (PH == 1) pr = phase_ref(kr_ * dcen);
-    if (PH == 2) pc = phase_centre(kr_, dcen);
+    if (PH >= 2) pc = phase_centre(kr_, dcen);
     // A = sum w e f1 ; T (xx,yy,zz,xy,xz,yz) = sum w e f2 dh dh ; G = sum w (ikr-1) e d / r^3
     double Ar = 0., Ai = 0.;
     double Tr[6] = {0, 0, 0, 0, 0, 0}, Ti[6] = {0, 0, 0, 0, 0, 0};
@@ -174,7 +174,7 @@ __device__ __forceinline__ void pair_sums64(double x0, double x1, double x2, con
         rot_q(Ar, Ai, pr.q);
         for (int t = 0; t < 6; t++) rot_q(Tr[t], Ti[t], pr.q);
         for (int g = 0; g < 3; g++) rot_q(Gr[g], Gi[g], pr.q);
-    } else if (PH == 2) {   // ... or e^{i k dcen}
+    } else if (PH >= 2) {   // ... or e^{i k dcen}
         rot_c(Ar, Ai, pc);
         for (int t = 0; t < 6; t++) rot_c(Tr[t], Ti[t], pc);
         for (int g = 0; g < 3; g++) rot_c(Gr[g], Gi[g], pc);
@@ -377,7 +377,8 @@ static void field_far_r(hipStream_t s, const FieldArgs& a) {
 
 template <bool CK, bool JM>
 static void field_far_fp(hipStream_t s, const FieldArgs& a) {
-    if (a.fast_phase == 2) field_far_r<CK, JM, 2>(s, a);
+    if (a.fast_phase == 3) field_far_r<CK, JM, 3>(s, a);
+    else if (a.fast_phase == 2) field_far_r<CK, JM, 2>(s, a);
     else if (a.fast_phase) field_far_r<CK, JM, 1>(s, a);
     else field_far_r<CK, JM, 0>(s, a);
 }
@@ -406,6 +407,7 @@ static const void* field_far_kernel_r(const FieldArgs& a) {
 
 template <bool CK, bool JM>
 static const void* field_far_kernel_ph(const FieldArgs& a) {
+    if (a.fast_phase == 3) return field_far_kernel_r<CK, JM, 3>(a);
     if (a.fast_phase == 2) return field_far_kernel_r<CK, JM, 2>(a);
     if (a.fast_phase) return field_far_kernel_r<CK, JM, 1>(a);
     return field_far_kernel_r<CK, JM, 0>(a);

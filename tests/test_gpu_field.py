@@ -116,6 +116,32 @@ def test_reduced_phase_and_general_sincos_paths(ctx, orc, monkeypatch):
     cg.close()
 
 
+@pytest.mark.parametrize("t_max", [0.2, 0.4, 0.9])
+def test_phase_modes_vs_oracle(ctx, orc, t_max):
+    """k chosen so that |k| max|y - c| (the largest node phase offset from a cell's centre)
+    is 0.2, 0.4 or 0.9: the centre-phase series on |t| <= 0.25 (phase mode 3) and on
+    |t| <= 0.5 (mode 2), and the quadrant reduction (mode 1).  Field and matrix against the
+    oracle."""
+    verts = B.mesh.sphere_quads(16, 32, jitter=0.005, seed=1)
+    n = verts.shape[0]
+    m = ctx.mesh(verts)
+    cen = m.tables()["center"]
+    max_rad = np.linalg.norm(verts - cen[:, None, :], axis=2).max()
+    k = t_max / max_rad
+    rng = np.random.default_rng(25)
+    jE = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    pts = B.mesh.sphere_points(40, 3.0, seed=5)
+    E, H = m.field(B.num_param(), k, jE, pts)
+    Eo, Ho = orc.field_eh(verts, jE, k, O.num_param(), pts)
+    assert rowwise_rel(E, Eo).max() <= TOL
+    assert rowwise_rel(H, Ho).max() <= TOL
+    m.close()
+    m2 = ctx.mesh(verts[:60])
+    from _util import block_metric
+    assert block_metric(m2.assemble(B.num_param(), k), orc.assemble(verts[:60], O.num_param(), k)).max() <= 1e-12
+    m2.close()
+
+
 def test_field_linearity_in_currents(ctx):
     """E and H are linear in (jE, jM): a size-independent property."""
     rng = np.random.default_rng(23)
