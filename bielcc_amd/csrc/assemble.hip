@@ -143,7 +143,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
             const double dcen = center_dist(x[0], x[1], x[2], rec);
             constexpr bool FP = PH != 0;   // FP32 node sums: quadrant reduction in either mode
             PhaseRef pr{0., 0., 0., 0};
-            PhaseC pc{0., 0., 1., 0.};
+            PhaseC pc{0., 1., 0.};
             if (PH == 1) pr = phase_ref(kr_ * dcen);
             cplx z[2][2];
             if (WHICH == 0) {
@@ -161,7 +161,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                     const double4 y = nd[q];
                     const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
                     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                    const double ri = rsqrt(r2);
+                    const double ri = rsqrt_pos(r2);
                     const double r = r2 * ri;
                     double s, co;
                     phase_eval<PH>(kr_, r, pr, pc, s, co);
@@ -179,7 +179,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                         const double4 y = sg[e * ns + q];
                         const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
                         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                        const double ri = rsqrt(r2);
+                        const double ri = rsqrt_pos(r2);
                         const double r = r2 * ri;
                         double s, co;
                         phase_eval<PH>(kr_, r, pr, pc, s, co);
@@ -239,7 +239,7 @@ __device__ __forceinline__ void assemble_tile(const AssembleArgs& a, int bx, int
                         const double4 y = nd[q];
                         const double d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
                         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                        const double ri = rsqrt(r2);
+                        const double ri = rsqrt_pos(r2);
                         const double r = r2 * ri;
                         double s, co;
                         phase_eval<PH>(kr_, r, pr, pc, s, co);

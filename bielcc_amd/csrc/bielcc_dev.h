@@ -676,6 +676,17 @@ __device__ inline bool near_rule(double dist, double diam, int mode, double ac, 
     return !(dist > thresh);
 }
 
+// 1/sqrt(x) for finite x > 0 (far pairs: x is at least the square of a node
+// distance well away from zero).  v_rsq_f64 is correct to about 2^-23 relative;
+// one third-order step y + y e (1/2 + 3/8 e), e = 1 - x y^2, brings that below
+// rounding.  The library rsqrt does the same step and adds a class test and two
+// selects for 0, inf and denormals, which these arguments never are.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 }  // namespace bielcc_dev
 
 #pragma clang fp contract(fast)
