@@ -56,6 +56,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def fp64_calls(ctx):
+    """Mixed field calls that ran with FP64 far sums: chosen by the sample or by the phase rule."""
+    return ctx.get_tuning("mixed_probe_fp64") + ctx.get_tuning("mixed_auto_fp64")
+
+
 def workload(name):
     from bielcc_amd import mesh
     if name == "sphere20k":
@@ -236,7 +241,7 @@ def near_far_errors(ctx, m, npp, k, dJ64, dJ32, near_pts, dirs):
     dE, dH = ctx.empty(Mn * 48), ctx.empty(Mn * 48)
     res = {}
     for tag, dJ, prec in (("fp64", dJ64, B.PREC_FP64), ("mixed", dJ32, B.PREC_MIXED)):
-        probed, fixed = ctx.get_tuning("mixed_probe_fp64"), ctx.get_tuning("mixed_fixup_points")
+        probed, fixed = fp64_calls(ctx), ctx.get_tuning("mixed_fixup_points")
         ctx.sync()
         t0 = time.perf_counter()
         m.field_batch_into(npp, k, 1, dJ, dP, Mn, dE, dH, precision=prec)
@@ -245,7 +250,7 @@ def near_far_errors(ctx, m, npp, k, dJ64, dJ32, near_pts, dirs):
         res[tag] = (dE.to_numpy((Mn, 3), np.complex128), dH.to_numpy((Mn, 3), np.complex128))
         out[f"near_field_{tag}_s"] = round(t1 - t0, 4)
     # mixed call: far sums chosen by the sample, and points recomputed with FP64 sums
-    out["near_field_mixed_far_sums"] = "FP64" if ctx.get_tuning("mixed_probe_fp64") > probed else "FP32"
+    out["near_field_mixed_far_sums"] = "FP64" if fp64_calls(ctx) > probed else "FP32"
     out["near_field_mixed_points_recomputed"] = ctx.get_tuning("mixed_fixup_points") - fixed
     eE, eH = _errs(res["mixed"][0], res["fp64"][0]), _errs(res["mixed"][1], res["fp64"][1])
     out.update({"near_points": Mn, "err_near_E_point_max": eE[0], "err_near_E_set": eE[1],
@@ -475,9 +480,9 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
     t_ref = timed(lambda: res.update(zip(("iters", "resid"), ctx.refine_solve(dZ, n2, n2, dF, n2, dpiv, dB, 1, n2,
                                                                                1e-10, 30))))
     m.currents_into(dB, n2, 1, dJ)
-    probed = ctx.get_tuning("mixed_probe_fp64")
+    probed = fp64_calls(ctx)
     t_fld = timed(lambda: m.field_batch_into(npp, k, 1, dJ, dP, M, dE, dH, precision=B.PREC_MIXED))
-    field_sums = "FP64" if ctx.get_tuning("mixed_probe_fp64") > probed else "FP32"
+    field_sums = "FP64" if fp64_calls(ctx) > probed else "FP32"
     log(f"[bench] mixed: assembly {t_asm:.3f} s, LU32 {t_lu:.2f} s, refine {t_ref:.3f} s "
         f"({res['iters']} corrections, residual {res['resid']:.1e}), field {t_fld:.3f} s ({field_sums} far sums)")
     j64, j32 = dJ64.to_numpy((n, 3), np.complex128), dJ.to_numpy((n, 3), np.complex128)
@@ -486,9 +491,11 @@ def mixed_form(ctx, m, npp, k, kvec, e0, dP, M, dJ64, dE64, dH64, lu64_s, solve6
     # the FP32 far sums alone (the sample that chooses the sums switched off): mixed field of
     # the FP64 currents, FP32 sums plus the recomputation of the points they miss
     ctx.set_tuning("mixed_probe_pct", 0)
+    ctx.set_tuning("mixed_far32", 1)
     fixed0 = ctx.get_tuning("mixed_fixup_points")
     t_fld32 = timed(lambda: m.field_batch_into(npp, k, 1, dJ64, dP, M, dE, dH, precision=B.PREC_MIXED))
     fixed = ctx.get_tuning("mixed_fixup_points") - fixed0
+    ctx.set_tuning("mixed_far32", 0)
     ctx.set_tuning("mixed_probe_pct", 25)
     Eq, Hq = dE.to_numpy((M, 3), np.complex128), dH.to_numpy((M, 3), np.complex128)
     from bielcc_amd import mesh

@@ -595,6 +595,14 @@ static int field_impl(bielcc_ctx ctx, int mesh_id, const bielcc_num_param* np, d
         return fail(ctx, BIELCC_ERR_UNSUPPORTED,
                     "field driver needs p_max = 1, n_start <= 22 and analytic_calc > 1 (use bielcc_op_batch)");
     DeviceGuard g(ctx->device);
+    // FP32 node sums issue at the FP64 rate (no packed math), and where the node phase is the
+    // centre-phase series the real-k FP64 loop takes fewer instructions than the FP32 loop with
+    // its cancellation-free r - dcen, so the mixed field sums in FP64 there.
+    if (precision == BIELCC_PREC_MIXED && !ctx->mixed_far32 && k_im == 0. && !ctx->no_fast_phase &&
+        phase_mode(k_re, m->grid_step, m->max_rad) >= 2) {
+        precision = BIELCC_PREC_FP64;
+        ctx->auto_fp64++;
+    }
     if (precision == BIELCC_PREC_MIXED && ctx->mixed_fixup && ctx->mixed_probe_pct > 0 && !ctx->capturing &&
         !ctx->in_probe && M >= 4 * kProbePoints) {
         double frac = 0.;
@@ -953,6 +961,11 @@ int bielcc_set_tuning(bielcc_ctx ctx, const char* key, int value) {
         ctx->mixed_probe_pct = value;
         return BIELCC_OK;
     }
+    if (k == "mixed_far32") {
+        if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "mixed_far32 must be 0 or 1");
+        ctx->mixed_far32 = value == 1;
+        return BIELCC_OK;
+    }
     if (k == "mixed_fixup") {
         if (value != 0 && value != 1) return fail(ctx, BIELCC_ERR_ARG, "mixed_fixup must be 0 or 1");
         ctx->mixed_fixup = value == 1;
@@ -1047,6 +1060,8 @@ int bielcc_get_tuning(bielcc_ctx ctx, const char* key, int* value) {
     else if (k == "mixed_fixup_points") *value = (int)std::min<long long>(ctx->fixup_points, INT_MAX);
     else if (k == "mixed_probe_pct") *value = ctx->mixed_probe_pct;
     else if (k == "mixed_probe_fp64") *value = (int)std::min<long long>(ctx->probe_fp64, INT_MAX);
+    else if (k == "mixed_far32") *value = ctx->mixed_far32 ? 1 : 0;
+    else if (k == "mixed_auto_fp64") *value = (int)std::min<long long>(ctx->auto_fp64, INT_MAX);
     else if (k == "blas_atomics") *value = lu_get_blas_atomics(ctx->lu) ? 1 : 0;
     else if (k == "lu_outer_nb") *value = lu_get_outer_nb(ctx->lu);
     else if (k == "lu_trsm_lib") *value = lu_get_trsm_lib_panel(ctx->lu);

@@ -51,6 +51,10 @@ struct bielcc_ctx_s {
     void* probe = nullptr;        // the sample's points, fields, flags (grow-only)
     size_t probe_cap = 0;
     long long probe_fp64 = 0;     // mixed field calls run in FP64 after the sample, cumulative
+    bool mixed_far32 = false;     // mixed field: FP32 far sums always (false: FP64 far sums where the node
+                                  // phase uses the centre-phase series and k is real, where the FP64 node
+                                  // loop is the shorter of the two)
+    long long auto_fp64 = 0;      // mixed field calls so run with FP64 far sums, cumulative
     int field_wg = 2048, assemble_wg = 2048, field_rchunk = 4;   // launch-shape tunables
     // plan (multi-mesh) tunables: cells per field tile and per assembly tile,
     // persistent workgroups per CU, operator-matrix workspace cap (MiB), and the

@@ -505,7 +505,13 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * Plans: "plan_field_cells" [64] and "plan_asm_cells" [64] =
  * cells per tile, "plan_wg_per_cu" [4] = persistent workgroups per compute
  * unit, "plan_ws_mb" [24576] = operator-matrix workspace cap in MiB.
- * Mixed fields: "mixed_fixup" [1] = after the FP32 sums, the points whose
+ * Mixed fields: "mixed_far32" [0] = the field driver's far sums run in FP32
+ * for every mixed call (1); by default a mixed call with a real k, whose node
+ * phases use the centre-phase series (|k| times the largest node distance
+ * from a cell centre at most 0.5), runs with FP64 sums: there the FP64 node
+ * loop takes fewer VALU instructions than the FP32 one, which issues at the
+ * same rate, and the result is the FP64 field bit for bit (plans and the
+ * far-field pattern keep their FP32 sums); "mixed_fixup" [1] = after the FP32 sums, the points whose
  * modelled FP32 error is not small against their scattered field are
  * recomputed with FP64 sums (0: never); "mixed_probe_pct" [25] = calls of at
  * least 4,096 points first run the FP32 sums and the error model on 1,024
@@ -513,8 +519,9 @@ int bielcc_plan_info(bielcc_ctx ctx, bielcc_plan plan, const char* key, double* 
  * recomputed, the whole call runs with FP64 sums (the FP64 field, bit for
  * bit; 0: no sample).
  * bielcc_get_tuning also reads "mixed_fixup_points" (points recomputed, for
- * the context) and "mixed_probe_fp64" (mixed calls the sample sent to FP64
- * sums),
+ * the context), "mixed_probe_fp64" (mixed calls the sample sent to FP64
+ * sums) and "mixed_auto_fp64" (mixed calls run with FP64 sums by the rule of
+ * "mixed_far32"),
  * "n_cu" (compute units) and "lu_panel_aborts" (whole-panel launches that
  * aborted to one workgroup, for the context, since its first LU),
  * "lu_panel_late_admits" (LUs that missed the whole-panel budget at their

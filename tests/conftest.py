@@ -23,5 +23,6 @@ def orc():
 def ctx():
     import bielcc_amd
     c = bielcc_amd.Context(0)
+    c.set_tuning("mixed_far32", 1)   # mixed-precision tests exercise the FP32 far sums
     yield c
     c.close()

@@ -158,3 +158,40 @@ def test_field_linearity_in_currents(ctx):
     assert rowwise_rel(E3, 2.0 * E1 - 0.5j * E2).max() <= 1e-12
     assert rowwise_rel(H3, 2.0 * H1 - 0.5j * H2).max() <= 1e-12
     m.close()
+
+
+@pytest.mark.parametrize("t_max,auto64", [(0.2, True), (0.9, False)])
+def test_mixed_far_sums_rule(ctx, t_max, auto64):
+    """mixed_far32 = 0 (the library default): a mixed field call with real k whose node phases use
+    the centre-phase series (|k| max|y - c| <= 0.5) runs with FP64 far sums, the FP64 field bit for
+    bit; beyond that (quadrant reduction) it keeps the FP32 sums.  mixed_far32 = 1 keeps the FP32
+    sums everywhere.  The FP32 error control is off here (mixed_fixup = 0), so FP32 sums show as
+    a difference from the FP64 field (set-wise within 1e-5 for these random currents)."""
+    verts = B.mesh.sphere_quads(16, 32, jitter=0.005, seed=1)
+    n = verts.shape[0]
+    m = ctx.mesh(verts)
+    cen = m.tables()["center"]
+    k = t_max / np.linalg.norm(verts - cen[:, None, :], axis=2).max()
+    rng = np.random.default_rng(26)
+    jE = rng.normal(size=(2, n, 3)) + 1j * rng.normal(size=(2, n, 3))
+    pts = B.mesh.sphere_points(300, 3.0, seed=6)
+    npp = B.num_param()
+    E64, H64 = m.field_batch(npp, k, jE, pts)
+    old, oldfix = ctx.get_tuning("mixed_far32"), ctx.get_tuning("mixed_fixup")
+    try:
+        ctx.set_tuning("mixed_fixup", 0)
+        ctx.set_tuning("mixed_far32", 0)
+        a0 = ctx.get_tuning("mixed_auto_fp64")
+        Ea, Ha = m.field_batch(npp, k, jE, pts, precision=B.PREC_MIXED)
+        assert ctx.get_tuning("mixed_auto_fp64") == a0 + (1 if auto64 else 0)
+        ctx.set_tuning("mixed_far32", 1)
+        E32, H32 = m.field_batch(npp, k, jE, pts, precision=B.PREC_MIXED)
+        assert ctx.get_tuning("mixed_auto_fp64") == a0 + (1 if auto64 else 0)
+    finally:
+        ctx.set_tuning("mixed_far32", old)
+        ctx.set_tuning("mixed_fixup", oldfix)
+    assert np.array_equal(Ea, E64) == auto64 and np.array_equal(Ha, H64) == auto64
+    assert not np.array_equal(E32, E64) and not np.array_equal(H32, H64)
+    for X, Y in ((Ea, E64), (Ha, H64), (E32, E64), (H32, H64)):
+        assert np.linalg.norm(X - Y) <= 1e-5 * np.linalg.norm(Y)
+    m.close()
