@@ -53,7 +53,7 @@ __device__ inline void surf_sums32(const float4* nd, const float* ny, int nn, co
         const float ri = rsqrtf(r2);
         const float r = r2 * ri;
         const float del = fmaf(m0, y.x, fmaf(m1, y.y, fmaf(m2, y.z, ny[q])));
-        const float dr = __fdividef(del, r + dcf);
+        const float dr = del * __builtin_amdgcn_rcpf(r + dcf);   // v_rcp_f32 (1 ulp): 2 instructions, not a full division
         const float t = fmaf(kf, dr, t0);
         float s, c;
         if (FP) sincos_near32(t, s, c);

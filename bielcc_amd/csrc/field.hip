@@ -259,7 +259,7 @@ __device__ __forceinline__ void pair_sums32(double x0, double x1, double x2, con
             const float r = r2 * ri;
             // r - dcen without cancellation: (|y_c|^2 - 2 x_c.y_c) / (r + dcen)
             const float del = fmaf(m0, y.x, fmaf(m1, y.y, fmaf(m2, y.z, ny[q])));
-            const float dr = __fdividef(del, r + dcf);
+            const float dr = del * __builtin_amdgcn_rcpf(r + dcf);   // v_rcp_f32 (1 ulp): 2 instructions, not a full division
             const float t = fmaf(kf, dr, t0);
             float s, c;
             if (FP) sincos_near32(t, s, c);
